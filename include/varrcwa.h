@@ -1,0 +1,210 @@
+/*
+ * varrcwa.h -- C ABI of the MI355X (gfx950) scattering-matrix library.
+ *
+ * Conventions (every entry point):
+ *  - complex128 values are two doubles (re, im), interleaved; matrices are
+ *    column-major with an explicit leading dimension; strided batches give the
+ *    distance in complex elements between consecutive matrices.
+ *  - pointers are device pointers unless the parameter name ends in _host.
+ *  - all work is enqueued on the context's stream; calls return without waiting
+ *    unless documented (results read back to _host buffers synchronize).
+ *  - status: 0 ok, -1 bad argument, -2 HIP runtime failure, -3 other failure;
+ *    vrc_last_error() gives the text. Per-matrix numerical failures are reported
+ *    LAPACK-style through info arrays (0 ok, >0 failing index).
+ *  - one context per host thread; calls on one context are not re-entrant.
+ *
+ * Harmonic ordering follows the reference: index m*ny + n (x order outer), field
+ * rows [x harmonics; y harmonics] (src/core/LayerSampler.cpp:9-32).
+ */
+#ifndef VARRCWA_H
+#define VARRCWA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vrc_ctx vrc_ctx;
+typedef struct vrc_sampler vrc_sampler;
+typedef struct vrc_integrator vrc_integrator;
+
+/* ------------------------------------------------------------------ context */
+/* No reference counterpart: the reference creates cuBLAS/cuSOLVER/MAGMA handles per
+ * integrator (src/gpu/RedhefferIntegratorGPU.cpp:99-146). stream is an existing
+ * hipStream_t or NULL for the device's default (null) stream. */
+int vrc_ctx_create(int device, void* stream, vrc_ctx** out);
+int vrc_ctx_destroy(vrc_ctx* ctx);
+const char* vrc_last_error(const vrc_ctx* ctx);
+int vrc_sync(vrc_ctx* ctx);
+/* Bytes currently held by the context's device memory pool. */
+long long vrc_pool_bytes(const vrc_ctx* ctx);
+int vrc_pool_trim(vrc_ctx* ctx);
+const char* vrc_version(void);
+
+/* ------------------------------------------------- dense complex128 kernels */
+/* op: 0 = N, 1 = T, 2 = C (conjugate transpose).
+ * C[b] = alpha*op(A[b])*op(B[b]) + beta*C[b], b < batch.
+ * Replaces cublasZgemm3m (src/gpu/types.cuh:33) at every site listed for P/Q
+ * assembly, PQ, QW and the star products (src/gpu/CudaLayerSampler.cu:88-195,
+ * src/gpu/RedhefferIntegratorGPU.cpp:195-201,801-912); computed as a 4M product. */
+int vrc_zgemm(vrc_ctx* ctx, int opA, int opB, int m, int n, int k, const double* alpha_host, const void* A,
+              long lda, long strideA, const void* B, long ldb, long strideB, const double* beta_host, void* C,
+              long ldc, long strideC, int batch);
+
+/* LU with partial pivoting (0-based ipiv). Replaces cusolverDnZgetrf
+ * (src/gpu/types.cuh:31) and Eigen PartialPivLU::compute
+ * (src/core/RCWAIntegrator.cpp:30-34, src/rcwa/RCWAScatterMatrix.cpp:50-78). */
+int vrc_zgetrf(vrc_ctx* ctx, int n, void* A, long lda, long strideA, int* ipiv, long strideP, int* info,
+               int batch);
+
+/* X = A^{-1} B from a vrc_zgetrf factorization (X may equal B). Replaces
+ * cusolverDnZgetrs (src/gpu/types.cuh:32) and PartialPivLU::solve. */
+int vrc_zgetrs(vrc_ctx* ctx, int n, int nrhs, const void* LU, long lda, long strideA, const int* ipiv,
+               long strideP, const void* B, long ldb, long strideB, void* X, long ldx, long strideX, int batch);
+
+/* Right eigenvectors of general matrices (A destroyed): eigenvalues w (n per
+ * matrix), VR unit 2-norm with the largest component real. Replaces
+ * MKLEigenSolver::compute -> LAPACKE_zgeev(ROW_MAJOR,'N','V')
+ * (src/rcwa/MKLEigenSolver.hpp:24-74) and magma_zgeev
+ * (src/gpu/RedhefferIntegratorGPU.cpp:216-229). */
+int vrc_zgeev(vrc_ctx* ctx, int n, void* A, long lda, long strideA, void* w, long strideW, void* VR, long ldv,
+              long strideV, int* info, int batch);
+
+
+/* ------------------------------------------------------- layer geometry */
+/* One Layer(x, y, eps) slab (src/rcwa/Layer.cpp:18-34): ncoord_x coordinates x,
+ * ncoord_y coordinates y (host arrays), eps = (ncoord_y-1) x (ncoord_x-1) complex,
+ * column-major (row i = y strip [y_i, y_i+1], column j = x cell [x_j, x_j+1]). */
+typedef struct vrc_layer {
+  int ncoord_x;
+  const double* x;
+  int ncoord_y;
+  const double* y;
+  const double* eps;
+} vrc_layer;
+
+/* (a) Fourier convolution matrices fe11 (inverse rule along x), fe22 (inverse
+ * rule along y), fe33 (Laurent) of nlayers layers, n x n each (n = nx*ny), stride
+ * in complex elements between layers. Replaces FourierSolver2D::solve with
+ * DiscontinuousX / DiscontinuousY / ContinuousXY (src/rcwa/Layer.cpp:63-80,
+ * src/rcwa/FourierSolver2D.cpp:12-351) and CudaFourierSolver2D
+ * (src/gpu/CudaFourierSolver2D.cu:98-394). */
+int vrc_assemble_toeplitz(vrc_ctx* ctx, int nx, int ny, int nlayers, const vrc_layer* layers, void* fe11,
+                          void* fe22, void* fe33, long stride);
+
+/* (a)+(b) P and Q (N x N, N = 2*nx*ny) of nlayers layers at one wavelength.
+ * region: the layer whose coordinates define the cell and, with enable_pml, the
+ * PML stretch (LayerSampler::postInitialization, src/core/LayerSampler.cpp:145-153).
+ * Replaces Layer::waveEqnCoeff (src/rcwa/Layer.cpp:36-171) and
+ * CudaLayerSampler::sample (src/gpu/CudaLayerSampler.cu:10-203). */
+int vrc_assemble_pq(vrc_ctx* ctx, int nx, int ny, int nlayers, const vrc_layer* layers, double lambda,
+                    int enable_pml, const vrc_layer* region, void* P, void* Q, long stride);
+
+/* (c) PQ = P Q; eigen-decomposition; gamma = sqrt(lambda) with the reference branch
+ * rule; V = Q W diag(1/gamma). Replaces WaveEquationSolver::solve
+ * (src/rcwa/WaveEquationSolver.cpp:24-37) and evaluateEigenvalueAndSetReference
+ * (src/gpu/RedhefferIntegratorGPU.cpp:188-284). gam: N per matrix. */
+int vrc_eig_pq(vrc_ctx* ctx, int N, int batch, const void* P, const void* Q, long stride, void* W, void* V,
+               void* gam, int* info);
+
+/* (d) Slab S-matrix of each slice in the reference basis (W0, V0):
+ * dtmm::RCWAScatterMatrix::compute (src/rcwa/RCWAScatterMatrix.cpp:26-85) after its
+ * eigen step. lambda_host/dz_host: per slice. ref_stride 0 = one shared basis.
+ * Outputs Tuu (= Tdd) and Rud (= Rdu) (src/rcwa/RCWAScatterMatrix.h:44-47). */
+int vrc_layer_smatrix(vrc_ctx* ctx, int N, int batch, const double* lambda_host, const double* dz_host,
+                      const void* W, const void* V, const void* gam, const void* W0, const void* V0,
+                      long ref_stride, void* Tuu, void* Rud);
+
+/* (d) Redheffer star product S <- S * s in place, batched; S blocks (Tdd, Tuu, Rud,
+ * Rdu) each N x N with batch stride N*N; slice blocks with batch stride
+ * slice_stride (src/core/RCWAIntegrator.cpp:57-67; redhefferStarRightProduct,
+ * src/gpu/RedhefferIntegratorGPU.cpp:777-932). */
+int vrc_star_product(vrc_ctx* ctx, int N, int batch, void* Tdd, void* Tuu, void* Rud, void* Rdu,
+                     const void* tdd, const void* tuu, const void* rud, const void* rdu, long slice_stride);
+
+/* (d) End projection of the left side from basis (W1, V1) to (W0, V0)
+ * (src/core/RCWAIntegrator.cpp:73-86; reprojectLeftEnd,
+ * src/gpu/RedhefferIntegratorGPU.cpp:500-627). */
+int vrc_project_left(vrc_ctx* ctx, int N, int batch, void* Tdd, void* Tuu, void* Rud, void* Rdu,
+                     const void* W1, const void* V1, const void* W0, const void* V0, long ref_stride);
+
+/* (d) varRCWA segment [z0, z1] around the reference (Pr, Qr, W, V, gam): Simpson
+ * perturbation with samples at z0, (z0+z1)/2, z1 and the max-modulus error estimate
+ * (RedhefferIntegrator::evaluatePerturbationAndDifference,
+ * src/core/RedhefferIntegrator.cpp:317-399). Synchronizes (err_host). */
+int vrc_redheffer_segment(vrc_ctx* ctx, int N, double lambda, double z0, double z1, const void* Pr,
+                          const void* Qr, const void* W, const void* V, const void* gam, const void* P0,
+                          const void* Pm, const void* P1, const void* Q0, const void* Qm, const void* Q1,
+                          void* tdd, void* tuu, void* rud, void* rdu, double* err_host);
+
+/* ------------------------------------------------------------- samplers */
+/* VarLayerSampler (src/core/VarLayerSampler.cpp:9-69): base layer whose x/y
+ * coordinates move with z through rules (addXRule/addYRule). */
+int vrc_var_sampler_create(vrc_ctx* ctx, double lambda, int nx, int ny, const vrc_layer* base, int enable_pml,
+                           vrc_sampler** out);
+/* axis 0 = x (addXRule), 1 = y (addYRule). rule(z, user) returns the coordinate. */
+int vrc_sampler_add_rule(vrc_sampler* s, int axis, int layout, double (*rule)(double z, void* user), void* user);
+/* Piecewise-constant stack (no reference counterpart; extension): layer k occupies
+ * (z_edges[k], z_edges[k+1]); bottom at z <= z_edges[0], top at z >= z_edges[nlayers]. */
+int vrc_stack_sampler_create(vrc_ctx* ctx, double lambda, int nx, int ny, int nlayers, const vrc_layer* layers,
+                             const double* z_edges, const vrc_layer* bottom, const vrc_layer* top, int enable_pml,
+                             vrc_sampler** out);
+int vrc_sampler_destroy(vrc_sampler* s);
+/* LayerSampler::nDim (src/core/LayerSampler.h:44) */
+int vrc_sampler_ndim(const vrc_sampler* s);
+/* LayerSampler::sample (src/core/LayerSampler.h:49-51): P, Q into host N x N buffers. */
+int vrc_sampler_sample(vrc_sampler* s, double z, double* P_host, double* Q_host);
+/* LayerSampler::sampleOnGPU (src/core/LayerSampler.h:53-55): device N x N buffers. */
+int vrc_sampler_sample_gpu(vrc_sampler* s, double z, void* P, void* Q);
+/* LayerSampler::Kx()/Ky() (src/core/LayerSampler.h:46-47): host n x n buffers. */
+int vrc_sampler_k(vrc_sampler* s, double* Kx_host, double* Ky_host);
+
+/* ----------------------------------------------------------- integrators */
+/* kind: 0 RCWAIntegrator, 1 RedhefferIntegrator, 2 DifferentialIntegrator
+ * (src/core/RCWAIntegrator.h:25, RedhefferIntegrator.h:34, DifferentialIntegrator.h:25).
+ * The sampler must outlive the integrator. */
+int vrc_integrator_create(vrc_ctx* ctx, int kind, double z0, double z1, vrc_sampler* s, vrc_integrator** out);
+int vrc_integrator_destroy(vrc_integrator* it);
+/* RCWAIntegrator::compute(N) (src/core/RCWAIntegrator.cpp:20-87): N midpoint slices,
+ * solved one after another. */
+int vrc_rcwa_compute(vrc_integrator* it, int N);
+/* Same result; slices sampled and solved in batches before the ordered cascade. */
+int vrc_rcwa_compute_batched(vrc_integrator* it, int N);
+/* Extension: explicit slice edges (midpoint sampling, per-slice width). */
+int vrc_rcwa_compute_edges(vrc_integrator* it, int nedges, const double* z_edges);
+/* RedhefferIntegrator::compute(p, maxN, pe) (src/core/RedhefferIntegrator.cpp:174-224);
+ * p and maxN are accepted and unused, as in the reference. */
+int vrc_redheffer_compute(vrc_integrator* it, int p, int maxN, double pe);
+/* DifferentialIntegrator::compute(N) (src/core/DifferentialIntegrator.cpp:21-134). */
+int vrc_differential_compute(vrc_integrator* it, int N);
+/* Getters Tdd/Tuu/Rud/Rdu/W0/V0/gam0/W1/V1/gam1 (src/core/RCWAIntegrator.h:28-38):
+ * which = 0..9 in that order; host buffer of N*N (or N for gam) complex values. */
+int vrc_integrator_get(vrc_integrator* it, int which, double* host_out);
+/* N() and nEigVal() (src/core/RedhefferIntegrator.h:38-39). */
+int vrc_integrator_counts(vrc_integrator* it, int* N, int* nEigVal);
+/* Accepted varRCWA segments in cascade order as (z0, z1) pairs; returns the count
+ * (writes at most cap pairs). No reference counterpart (subdivision replay). */
+int vrc_integrator_tree(vrc_integrator* it, int cap, double* host_out);
+
+/* ------------------------------------------------------ batched stacks */
+/* No reference counterpart: many independent layer stacks (wavelength, geometry)
+ * in one call. Every instance: periodic cell taken from its bottom layer, nlayers
+ * slabs of given thickness sampled once each (RCWAIntegrator semantics per slab),
+ * half-space media bottom/top as end bases. All instances of a call share nx, ny
+ * and nlayers. S_out: per instance [Tdd|Tuu|Rud|Rdu] (4 N x N, device); the
+ * optional W0/V0/gam0/W1/V1/gam1 outputs may be NULL. chunk <= 0 picks a size. */
+typedef struct vrc_stack_instance {
+  double lambda;
+  int nlayers;
+  const vrc_layer* layers;
+  const double* thickness;
+  const vrc_layer* bottom;
+  const vrc_layer* top;
+} vrc_stack_instance;
+int vrc_batch_stack_rcwa(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
+                         void* S_out, void* W0, void* V0, void* gam0, void* W1, void* V1, void* gam1);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VARRCWA_H */

@@ -1,0 +1,733 @@
+"""CPU oracle for the varRCWA scattering-matrix hot path (TEST INFRASTRUCTURE ONLY).
+
+This module restates, in numpy, the reference's CPU algorithm for the path
+permittivity -> Fourier/Toeplitz convolution matrices -> P/Q -> eig(PQ) ->
+layer S-matrix -> Redheffer star-product cascade -> end projections.
+It exists only to check the HIP library.  Only ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker
+or the timed CPU baseline; the product path (``varrcwa_amd``) never calls it.
+
+Every function cites the reference file:line it follows (paths relative to the
+reference repository root).  Linear algebra maps onto the same LAPACK kernels the
+reference reaches through MKL:
+
+* ``LAPACKE_zgeev(ROW_MAJOR,'N','V')`` (src/rcwa/MKLEigenSolver.hpp:45-55)
+  -> ``numpy.linalg.eig`` (LAPACK ``zgeev`` jobvl='N', jobvr='V').
+* Eigen ``PartialPivLU`` (MKL ``zgetrf``/``zgetrs`` under ``EIGEN_USE_MKL_ALL``)
+  -> ``numpy.linalg.solve`` / ``inv`` (LAPACK ``zgesv``/``zgetri``).
+* Eigen ``BDCSVD(...).solve`` (src/rcwa/Layer.cpp:84-86) -> explicit SVD solve
+  with Eigen's rank threshold (``fe33_solve='svd'``, default) or LU (``'lu'``).
+
+Parity pins (what fixes each function independently of this file) live in
+``tests/test_oracle_pins.py``; functions without such a check are listed as
+"parity unpinned" there.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+# src/rcwa/defns.h:29 -- the reference's pi literal (double precision).
+PI = 3.1415926535897932
+
+CONTINUOUS_XY = 0   # src/rcwa/FourierSolver2D.h:7-12
+DISCONTINUOUS_X = 1
+DISCONTINUOUS_Y = 2
+
+
+# ---------------------------------------------------------------------------
+# 1D Fourier coefficients of a piecewise-constant function
+# ---------------------------------------------------------------------------
+
+def sinc(x: float) -> float:
+    """src/rcwa/MathFunction.h:6-9 (normalised sinc, exact 1 at 0)."""
+    return 1.0 if x == 0.0 else math.sin(PI * x) / (PI * x)
+
+
+def fourier1d(x: np.ndarray, d: float, f: np.ndarray, M: int) -> np.ndarray:
+    """FourierSolver1D::solve (src/rcwa/FourierSolver1D.cpp:11-41).
+
+    Ff[i] = sum_j f[j-1]/d * exp(-i m pi (x_j + x_{j-1})/d) * sinc(m dx/d) * dx,
+    m = i - (M-1)/2, cells taken in the given (possibly unsorted) coordinate order.
+    """
+    x = np.asarray(x, dtype=np.float64)
+    f = np.asarray(f, dtype=np.complex128)
+    assert f.size == x.size - 1
+    out = np.zeros(M, dtype=np.complex128)
+    for i in range(M):
+        m = i - (M - 1) // 2
+        acc = 0j
+        for j in range(1, x.size):
+            x1, x0 = x[j], x[j - 1]
+            if m == 0:
+                acc += f[j - 1] / d * (x1 - x0)
+            else:
+                acc += (f[j - 1] / d * np.exp(-1j * (m * PI * (x1 + x0) / d))
+                        * sinc(m * (x1 - x0) / d) * (x1 - x0))
+        out[i] = acc
+    return out
+
+
+def fourier1d_vec(x: np.ndarray, d: float, f: np.ndarray, M: int) -> np.ndarray:
+    """Vectorised form of :func:`fourier1d` (same formula, numpy broadcasting).
+
+    ``f`` may be 2-D (rows, cells): every row is transformed with the same
+    coordinates.  Used for pixel grids where the scalar loop is too slow.
+    """
+    x = np.asarray(x, dtype=np.float64)
+    f = np.asarray(f, dtype=np.complex128)
+    m = (np.arange(M) - (M - 1) // 2).astype(np.float64)
+    x0 = x[:-1]
+    x1 = x[1:]
+    dx = x1 - x0
+    arg = np.outer(m, dx) / d                      # (M, cells)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        s = np.where(arg == 0.0, 1.0, np.sin(PI * arg) / (PI * arg))
+    ph = np.exp(-1j * (np.outer(m, x1 + x0) * PI / d))
+    ph[m == 0, :] = 1.0
+    basis = ph * s * dx / d                         # (M, cells)
+    return f @ basis.T if f.ndim == 2 else basis @ f
+
+
+def reorder(x: np.ndarray) -> np.ndarray:
+    """FourierSolver1D::reorder (src/rcwa/FourierSolver1D.cpp:145-154).
+
+    Indices sorted by coordinate.  std::sort is not stable; for equal coordinates
+    the oracle uses a stable order (documented divergence, affects only
+    zero-width cells whose Fourier contribution is zero).
+    """
+    return np.argsort(np.asarray(x), kind="stable")
+
+
+def refill(x: np.ndarray, f: np.ndarray, rindex: np.ndarray) -> np.ndarray:
+    """FourierSolver1D::refill (src/rcwa/FourierSolver1D.cpp:156-193), painter's order."""
+    f = np.asarray(f, dtype=np.complex128)
+    need = any(rindex[i] != i for i in range(rindex.size - 1))
+    newf = f.copy()
+    if need:
+        for i in range(f.size):
+            if x[i] >= x[i + 1]:
+                continue
+            for j in range(newf.size):
+                if rindex[i] <= j and rindex[i + 1] >= j + 1:
+                    newf[j] = f[i]
+    return newf
+
+
+def _rindex(x: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    index = reorder(x)
+    rindex = np.empty_like(index)
+    rindex[index] = np.arange(index.size)
+    return index, rindex
+
+
+# ---------------------------------------------------------------------------
+# Toeplitz helpers
+# ---------------------------------------------------------------------------
+
+def toeplitz_dense(vec: np.ndarray) -> np.ndarray:
+    """ToeplitzMatrixXcs(vec).toDense() (src/rcwa/ToeplitzMatrixXcs.cpp:8-31).
+
+    T(r, c) = vec[n - 1 + r - c] with n = (len(vec)+1)/2.
+    """
+    vec = np.asarray(vec)
+    n = (vec.size + 1) // 2
+    r = np.arange(n)
+    return vec[n - 1 + r[:, None] - r[None, :]]
+
+
+def block_toeplitz_dense(mat: np.ndarray) -> np.ndarray:
+    """BlockToeplitzMatrixXcs(mat).toDense() (src/rcwa/BlockToeplitzMatrixXcs.cpp:5-52).
+
+    Row/col index = m*ny + n; entry = mat[nx-1 + (m-j), ny-1 + (n-l)].
+    """
+    nx = (mat.shape[0] + 1) // 2
+    ny = (mat.shape[1] + 1) // 2
+    m = np.repeat(np.arange(nx), ny)
+    n = np.tile(np.arange(ny), nx)
+    return mat[nx - 1 + m[:, None] - m[None, :], ny - 1 + n[:, None] - n[None, :]]
+
+
+# ---------------------------------------------------------------------------
+# 2D Fourier solver (three convolution rules)
+# ---------------------------------------------------------------------------
+
+def fourier2d(coordX: np.ndarray, coordY: np.ndarray, f: np.ndarray,
+              nx: int, ny: int, opt: int, fast: bool = True) -> np.ndarray:
+    """FourierSolver2D::solve without derivatives (src/rcwa/FourierSolver2D.cpp:12-351).
+
+    ``f`` has shape (nCy-1, nCx-1): row i is the y-strip [y_i, y_{i+1}].
+    ``fast`` switches the 1D transforms to the vectorised formula (identical
+    arithmetic up to summation order).
+    """
+    coordX = np.asarray(coordX, dtype=np.float64)
+    coordY = np.asarray(coordY, dtype=np.float64)
+    f = np.asarray(f, dtype=np.complex128)
+    dx = coordX[-1] - coordX[0]
+    dy = coordY[-1] - coordY[0]
+    nCy, nCx = coordY.size, coordX.size
+    f1 = fourier1d_vec if fast else fourier1d
+    n = nx * ny
+    Ff = np.zeros((n, n), dtype=np.complex128)
+    index, rindex = _rindex(coordX)
+    ii = np.repeat(np.arange(nx), ny)
+    jj = np.tile(np.arange(ny), nx)
+
+    if opt == DISCONTINUOUS_X:
+        # src/rcwa/FourierSolver2D.cpp:43-144
+        for i in range(nCy - 1):
+            newf = refill(coordX, f[i, :], rindex)
+            fInv = f1(coordX, dx, 1.0 / newf, 2 * nx - 1)
+            tFF = np.linalg.inv(toeplitz_dense(fInv))
+            fY1 = f1(np.array([coordY[i], coordY[i + 1]]), dy, np.array([1.0 + 0j]), 2 * ny - 1)
+            # Ff[(a,b),(c,d)] += fY1[b - d + ny - 1] * tFF[a, c]
+            Ff += fY1[jj[:, None] - jj[None, :] + ny - 1] * tFF[ii[:, None], ii[None, :]]
+    elif opt == DISCONTINUOUS_Y:
+        # src/rcwa/FourierSolver2D.cpp:145-249
+        newf = np.stack([refill(coordX, f[i, :], rindex) for i in range(f.shape[0])])
+        for i in range(nCx - 1):
+            fInv = f1(coordY, dy, 1.0 / newf[:, i], 2 * ny - 1)
+            tFF = np.linalg.inv(toeplitz_dense(fInv))
+            xs = np.array([coordX[index[i]], coordX[index[i + 1]]])
+            fX1 = f1(xs, dx, np.array([1.0 + 0j]), 2 * nx - 1)
+            Ff += fX1[ii[:, None] - ii[None, :] + nx - 1] * tFF[jj[:, None], jj[None, :]]
+    elif opt == CONTINUOUS_XY:
+        # src/rcwa/FourierSolver2D.cpp:250-349
+        four = np.zeros((2 * nx - 1, 2 * ny - 1), dtype=np.complex128)
+        for i in range(nCy - 1):
+            newf = refill(coordX, f[i, :], rindex)
+            fF = f1(coordX, dx, newf, 2 * nx - 1)
+            fY1 = f1(np.array([coordY[i], coordY[i + 1]]), dy, np.array([1.0 + 0j]), 2 * ny - 1)
+            four += np.outer(fF, fY1)
+        Ff = block_toeplitz_dense(four)
+    else:
+        raise ValueError("bad Fourier option")
+    return Ff
+
+
+# ---------------------------------------------------------------------------
+# PML stretch and K matrices
+# ---------------------------------------------------------------------------
+
+def _expipi(x: float) -> complex:
+    return complex(math.cos(PI * x), math.sin(PI * x))
+
+
+def pml_solve(b0: float, b1: float, L: float, gamma: complex, n: int, coeff: float) -> complex:
+    """PMLSolver1D::solve (src/rcwa/PMLSolver1D.cpp:15-30)."""
+    bm = (b1 - b0) / L
+    bp = (b1 + b0) / L
+    r = 4.0 * sinc(n * bm - 0.5) * np.exp(1j * PI * (-(n * bp + coeff * 0.5)))
+    r += 8.0 * np.exp(1j * PI * (-(n * bp))) * sinc(n * bm)
+    r += -2.0 * gamma * np.exp(1j * PI * (-(n * bp))) * sinc(n * bm)
+    r += 4.0 * sinc(n * bm + 0.5) * np.exp(1j * PI * (-(n * bp - coeff * 0.5)))
+    r += gamma * sinc(n * bm - 1.0) * np.exp(1j * PI * (-(n * bp + coeff * 1.0)))
+    r += gamma * sinc(n * bm + 1.0) * np.exp(1j * PI * (-(n * bp - coeff * 1.0)))
+    return r * (bm / 16.0)
+
+
+def alpha_beta(nx: int, ny: int, Lx: float, Ly: float) -> Tuple[np.ndarray, np.ndarray]:
+    """LayerSampler::evaluateAlphaBeta (src/core/LayerSampler.cpp:9-32)."""
+    Nx, Ny = (nx - 1) // 2, (ny - 1) // 2
+    Kx, Ky = 2 * PI / Lx, 2 * PI / Ly
+    m = np.repeat(np.arange(nx) - Nx, ny).astype(np.float64)
+    n = np.tile(np.arange(ny) - Ny, nx).astype(np.float64)
+    return Kx * m + 0j, Ky * n + 0j
+
+
+def k_matrices(nx: int, ny: int, region: Dict[str, float], enable_pml: bool
+               ) -> Tuple[np.ndarray, np.ndarray]:
+    """LayerSampler::evaluateKMatrices + evaluatePML (src/core/LayerSampler.cpp:34-143)."""
+    Lx, Ly = region["Lx"], region["Ly"]
+    alpha, beta = alpha_beta(nx, ny, Lx, Ly)
+    if not enable_pml:
+        return np.diag(alpha), np.diag(beta)
+    b0, b1, u0, u1 = region["b0"], region["b1"], region["u0"], region["u1"]
+    l0, l1, r0, r1 = region["l0"], region["l1"], region["r0"], region["r1"]
+    gamma = 1.0 / complex(1.0, -1.0)
+    nPx, nPy = 2 * nx - 1, 2 * ny - 1
+    maxPx, maxPy = (nPx - 1) // 2, (nPy - 1) // 2
+    fyy = np.zeros(nPy, dtype=np.complex128)
+    for i in range(nPy):
+        n = i - maxPy
+        fyy[i] = ((u0 - b1) / Ly * sinc(n * (u0 - b1) / Ly)
+                  / np.exp(1j * PI * (n * (b1 + u0) / Ly)))
+        fyy[i] += pml_solve(b0, b1, Ly, gamma, n, 1.0)
+        fyy[i] += pml_solve(u0, u1, Ly, gamma, n, -1.0)
+    fxx = np.zeros(nPx, dtype=np.complex128)
+    for i in range(nPx):
+        m = i - maxPx
+        fxx[i] = ((r0 - l1) * sinc(m * (r0 - l1) / Lx) / Lx
+                  / np.exp(1j * (PI * m * (l1 + r0) / Lx)))
+        fxx[i] += pml_solve(l0, l1, Lx, gamma, m, 1.0)
+        fxx[i] += pml_solve(r0, r1, Lx, gamma, m, -1.0)
+    fyx = np.zeros(nPx, dtype=np.complex128)
+    for i in range(nPx):
+        m = i - maxPx
+        if m == 0:
+            fyx[i] = r1 - l0
+        else:
+            fyx[i] = ((np.exp(-1j * (2 * PI / Lx * m * r1)) - np.exp(-1j * (2 * PI / Lx * m * l0)))
+                      * (1j * (Lx * Lx / (m * 2 * PI))))
+    fxy = np.zeros(nPy, dtype=np.complex128)
+    for i in range(nPy):
+        n = i - maxPy
+        if n == 0:
+            fxy[i] = u1 - b0
+        else:
+            fxy[i] = ((np.exp(-1j * (2 * PI / Ly * n * u1)) - np.exp(-1j * (2 * PI / Ly * n * b0)))
+                      * (1j * (Ly * Ly / (2 * n * PI))))
+    fy = 1.0 / Lx * np.outer(fyx, fyy)
+    fx = 1.0 / Ly * np.outer(fxx, fxy)
+    Fx = block_toeplitz_dense(fx)
+    Fy = block_toeplitz_dense(fy)
+    return Fx * alpha[None, :], Fy * beta[None, :]
+
+
+# ---------------------------------------------------------------------------
+# Layer operator (P, Q)
+# ---------------------------------------------------------------------------
+
+def svd_solve(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """Eigen BDCSVD(A, ThinU|ThinV).solve(B) (src/rcwa/Layer.cpp:84-86).
+
+    x = V diag(1/s) U^H b over the singular values above Eigen's default
+    threshold (diagSize * eps relative to the largest).
+    """
+    U, s, Vh = np.linalg.svd(A)
+    thr = max(A.shape) * np.finfo(np.float64).eps * (s[0] if s.size else 0.0)
+    r = int(np.sum(s > thr))
+    tmp = (U[:, :r].conj().T @ B) / s[:r, None]
+    return Vh[:r, :].conj().T @ tmp
+
+
+def wave_eqn_coeff(coordX, coordY, eps, Kx, Ky, lam, nx, ny, fe33_solve: str = "svd",
+                   return_fe: bool = False):
+    """Layer::waveEqnCoeff without derivatives (src/rcwa/Layer.cpp:36-100,134-138)."""
+    fe11 = fourier2d(coordX, coordY, eps, nx, ny, DISCONTINUOUS_X)
+    fe22 = fourier2d(coordX, coordY, eps, nx, ny, DISCONTINUOUS_Y)
+    fe33 = fourier2d(coordX, coordY, eps, nx, ny, CONTINUOUS_XY)
+    n = nx * ny
+    if fe33_solve == "svd":
+        F1h = svd_solve(fe33, Kx)
+        F2h = svd_solve(fe33, Ky)
+    else:
+        F1h = np.linalg.solve(fe33, Kx)
+        F2h = np.linalg.solve(fe33, Ky)
+    k0 = 2 * PI / lam
+    k02 = k0 * k0
+    I = np.eye(n)
+    P = np.empty((2 * n, 2 * n), dtype=np.complex128)
+    P[:n, :n] = Kx @ F2h
+    P[:n, n:] = -Kx @ F1h + k02 * I
+    P[n:, :n] = Ky @ F2h - k02 * I
+    P[n:, n:] = -Ky @ F1h
+    Q = np.empty((2 * n, 2 * n), dtype=np.complex128)
+    Q[:n, :n] = -Kx @ Ky
+    Q[:n, n:] = Kx @ Kx - k02 * fe22
+    Q[n:, :n] = -Ky @ Ky + k02 * fe11
+    Q[n:, n:] = Ky @ Kx
+    if return_fe:
+        return P, Q, (fe11, fe22, fe33)
+    return P, Q
+
+
+# ---------------------------------------------------------------------------
+# Samplers (the plugin boundary)
+# ---------------------------------------------------------------------------
+
+class LayerSampler:
+    """LayerSampler (src/core/LayerSampler.h:5-77 / .cpp:9-153)."""
+
+    def __init__(self, lam: float, nx: int, ny: int, enable_pml: bool = True):
+        self.lam, self.nx, self.ny, self.enable_pml = lam, nx, ny, enable_pml
+        self.region: Dict[str, float] = {}
+        self.Kx = self.Ky = None
+        self.fe33_solve = "svd"
+
+    def k0(self) -> float:
+        return 2 * PI / self.lam
+
+    def nDim(self) -> int:
+        return 2 * self.nx * self.ny
+
+    def _post_init(self, x: np.ndarray, y: np.ndarray) -> None:
+        # VarLayerSampler::assignSimulationRegion (src/core/VarLayerSampler.cpp:23-39)
+        self.region = dict(Lx=x[-1] - x[0], Ly=y[-1] - y[0], b0=y[0], b1=y[1], u0=y[-2],
+                           u1=y[-1], l0=x[0], l1=x[1], r0=x[-2], r1=x[-1])
+        self.Kx, self.Ky = k_matrices(self.nx, self.ny, self.region, self.enable_pml)
+
+    def layer_at(self, z: float):
+        raise NotImplementedError
+
+    def sample(self, z: float) -> Tuple[np.ndarray, np.ndarray]:
+        x, y, eps = self.layer_at(z)
+        return wave_eqn_coeff(x, y, eps, self.Kx, self.Ky, self.lam, self.nx, self.ny,
+                              self.fe33_solve)
+
+
+class VarLayerSampler(LayerSampler):
+    """VarLayerSampler (src/core/VarLayerSampler.cpp:9-69): coordinates move with z."""
+
+    def __init__(self, lam, nx, ny, x, y, eps, enable_pml=True):
+        super().__init__(lam, nx, ny, enable_pml)
+        self.x = np.asarray(x, dtype=np.float64).copy()
+        self.y = np.asarray(y, dtype=np.float64).copy()
+        self.eps = np.asarray(eps, dtype=np.complex128).copy()
+        self.xrules: Dict[int, Callable[[float], float]] = {}
+        self.yrules: Dict[int, Callable[[float], float]] = {}
+        self._post_init(self.x, self.y)
+
+    def addXRule(self, layout: int, rule: Callable[[float], float]) -> None:
+        self.xrules[layout] = rule
+
+    def addYRule(self, layout: int, rule: Callable[[float], float]) -> None:
+        self.yrules[layout] = rule
+
+    def layer_at(self, z):
+        x = self.x.copy()
+        y = self.y.copy()
+        for i, f in self.xrules.items():
+            x[i] = f(z)
+        for i, f in self.yrules.items():
+            y[i] = f(z)
+        return x, y, self.eps
+
+
+class StackLayerSampler(LayerSampler):
+    """Piecewise-constant-in-z stack of Layer(x, y, eps) slabs (extension).
+
+    Layer k occupies the open interval (z_edges[k], z_edges[k+1]).  At or below
+    z_edges[0] the sampler returns ``bottom``; at or above z_edges[-1] it returns
+    ``top`` (half-space media).  With equal slab thicknesses,
+    RCWAIntegrator(z_edges[0], z_edges[-1], stack).compute(len(layers)) samples
+    each slab exactly once at its midpoint (src/core/RCWAIntegrator.cpp:41-45).
+    All layers must share the outer cell [x0, x1] x [y0, y1].
+    """
+
+    def __init__(self, lam, nx, ny, layers, z_edges, bottom, top, enable_pml=False):
+        super().__init__(lam, nx, ny, enable_pml)
+        self.layers = layers
+        self.z_edges = np.asarray(z_edges, dtype=np.float64)
+        self.bottom, self.top = bottom, top
+        x, y, _ = bottom
+        self._post_init(np.asarray(x, np.float64), np.asarray(y, np.float64))
+
+    def layer_at(self, z):
+        if z <= self.z_edges[0]:
+            return self.bottom
+        if z >= self.z_edges[-1]:
+            return self.top
+        k = int(np.searchsorted(self.z_edges, z, side="right") - 1)
+        return self.layers[k]
+
+
+# ---------------------------------------------------------------------------
+# Eigen-system and slab S-matrix
+# ---------------------------------------------------------------------------
+
+def gamma_sqrt(g: np.ndarray) -> np.ndarray:
+    """GammaSqrtOp (src/rcwa/WaveEquationSolver.cpp:8-22)."""
+    gv = np.sqrt(np.asarray(g, dtype=np.complex128))
+    gv = np.where(gv.imag < 0, -gv, gv)
+    flip = (np.abs(gv.imag) < np.abs(gv.real)) & (gv.real < 0)
+    return np.where(flip, -gv, gv)
+
+
+def wave_equation_solve(P: np.ndarray, Q: np.ndarray):
+    """WaveEquationSolver::solve (src/rcwa/WaveEquationSolver.cpp:24-37).
+
+    PQ = P Q; zgeev(jobvl='N', jobvr='V'); gamma = GammaSqrt(lambda); V = Q W diag(1/gamma).
+    Returns (W, V, gamma, lambda).
+    """
+    PQ = P @ Q
+    lam, W = np.linalg.eig(PQ)
+    gam = gamma_sqrt(lam)
+    V = (Q @ W) * (1.0 / gam)[None, :]
+    return W, V, gam, lam
+
+
+def rcwa_scatter_matrix(lam: float, wz: float, P, Q, W0, V0):
+    """dtmm::RCWAScatterMatrix::compute (src/rcwa/RCWAScatterMatrix.cpp:26-85).
+
+    Returns (Tuu, Rud); the reference aliases Tdd := Tuu and Rdu := Rud
+    (src/rcwa/RCWAScatterMatrix.h:44-47).
+    """
+    W, V, gam, _ = wave_equation_solve(P, Q)
+    return rcwa_scatter_matrix_from_eig(lam, wz, W, V, gam, W0, V0)
+
+
+def rcwa_scatter_matrix_from_eig(lam: float, wz: float, W, V, gam, W0, V0):
+    """The slab S-matrix from a given eigen-system: the arithmetic after the eigen
+    step of RCWAScatterMatrix::compute, identical to RCWAScatterMatrix::compute_v2
+    (src/rcwa/RCWAScatterMatrix.cpp:88-147). Returns (Tuu, Rud)."""
+    invk0 = lam / (2.0 * PI)
+    ss = invk0 * wz
+    X = np.exp(1j * gam * ss)
+    F0 = np.linalg.solve(W, W0)
+    G0 = np.linalg.solve(V, V0)
+    A = F0 + G0
+    B = F0 - G0
+    lu_rhs = np.linalg.solve(A, np.concatenate([B, X[:, None] * A, X[:, None] * B], axis=1))
+    n = A.shape[0]
+    invAB, invAXA, invAXB = lu_rhs[:, :n], lu_rhs[:, n:2 * n], lu_rhs[:, 2 * n:]
+    I = np.eye(n)
+    M = I - invAXB @ invAXB
+    R1 = invAXB @ invAXA - invAB
+    T1 = invAXA - invAXB @ invAB
+    sol = np.linalg.solve(M, np.concatenate([R1, T1], axis=1))
+    Rud = sol[:, :n]
+    Tuu = sol[:, n:]
+    return Tuu, Rud
+
+
+def star_product(S, tdd, tuu, rud, rdu):
+    """Redheffer star product as written in src/core/RCWAIntegrator.cpp:57-67.
+
+    S = (Tdd, Tuu, Rud, Rdu) accumulated so far; returns the updated tuple.
+    """
+    Tdd, Tuu, Rud, Rdu = S
+    n = Tdd.shape[0]
+    I = np.eye(n)
+    invC1 = I - rdu @ Rud
+    C1 = np.linalg.inv(invC1)
+    C2 = Rud @ C1 @ rdu + I
+    Rdu_n = Rdu + Tdd @ rdu @ C2 @ Tuu
+    Tdd_n = Tdd @ C1 @ tdd
+    Rud_n = rud + tuu @ Rud @ C1 @ tdd
+    Tuu_n = tuu @ C2 @ Tuu
+    return Tdd_n, Tuu_n, Rud_n, Rdu_n
+
+
+def left_end_projection(S, W1, V1, W0, V0):
+    """End projection of src/core/RCWAIntegrator.cpp:73-86 (basis W1,V1 -> W0,V0)."""
+    Tdd, Tuu, Rud, Rdu = S
+    invW1_W0 = np.linalg.solve(W1, W0)
+    invV1_V0 = np.linalg.solve(V1, V0)
+    A = 0.5 * (invW1_W0 + invV1_V0)
+    B = 0.5 * (invW1_W0 - invV1_V0)
+    K = A - Rdu @ B
+    sol = np.linalg.solve(K, np.concatenate([Rdu @ A - B, Tdd], axis=1))
+    n = A.shape[0]
+    Rdu_n = sol[:, :n]
+    Tdd_n = sol[:, n:]
+    Rud_n = Tuu @ B @ Tdd_n + Rud
+    Tuu_n = Tuu @ (B @ Rdu_n + A)
+    return Tdd_n, Tuu_n, Rud_n, Rdu_n
+
+
+@dataclass
+class SResult:
+    """Outputs of an integrator (the getters of src/core/RCWAIntegrator.h:28-38)."""
+    Tdd: np.ndarray
+    Tuu: np.ndarray
+    Rud: np.ndarray
+    Rdu: np.ndarray
+    W0: np.ndarray
+    V0: np.ndarray
+    gam0: np.ndarray
+    W1: np.ndarray
+    V1: np.ndarray
+    gam1: np.ndarray
+    N: int = 0
+    nEigVal: int = 0
+    tree: List[Tuple[float, float]] = field(default_factory=list)
+
+
+def rcwa_integrate(sampler: LayerSampler, z0: float, z1: float, N: int,
+                   z_edges: Optional[Sequence[float]] = None) -> SResult:
+    """RCWAIntegrator::compute(N) (src/core/RCWAIntegrator.cpp:20-87).
+
+    ``z_edges`` (extension) replaces the uniform division by explicit slice
+    edges; each slice is still sampled at its midpoint and uses its own width.
+    """
+    lam = sampler.lam
+    P1, Q1 = sampler.sample(z1)
+    W1, V1, gam1, _ = wave_equation_solve(P1, Q1)
+    nD = W1.shape[0]
+    S = (np.eye(nD, dtype=np.complex128), np.eye(nD, dtype=np.complex128),
+         np.zeros((nD, nD), np.complex128), np.zeros((nD, nD), np.complex128))
+    if z_edges is None:
+        dz = (z1 - z0) / float(N)
+        slices = [(z0 + (i + 0.5) * dz, dz) for i in range(N)]
+    else:
+        ze = list(z_edges)
+        slices = [(0.5 * (ze[i] + ze[i + 1]), ze[i + 1] - ze[i]) for i in range(len(ze) - 1)]
+    for zm, dz in slices:
+        P, Q = sampler.sample(zm)
+        tuu, rud = rcwa_scatter_matrix(lam, dz, P, Q, W1, V1)
+        S = star_product(S, tuu, tuu, rud, rud)
+    P0, Q0 = sampler.sample(z0)
+    W0, V0, gam0, _ = wave_equation_solve(P0, Q0)
+    S = left_end_projection(S, W1, V1, W0, V0)
+    return SResult(*S, W0, V0, gam0, W1, V1, gam1, N=len(slices))
+
+
+# ---------------------------------------------------------------------------
+# varRCWA (RedhefferIntegrator, trisection branch)
+# ---------------------------------------------------------------------------
+
+def perturbation_and_difference(k0, Pr, Qr, W, V, gam, z0, z1, P0, Pm, P1, Q0, Qm, Q1):
+    """RedhefferIntegrator::evaluatePerturbationAndDifference
+    (src/core/RedhefferIntegrator.cpp:317-399).  Returns (tdd, tuu, rud, rdu, diffError)."""
+    def dAB(Ps, Qs):
+        a = np.linalg.solve(W, (Ps - Pr) @ V)
+        b = np.linalg.solve(V, (Qs - Qr) @ W)
+        return a + b, a - b
+    dA0, dB0 = dAB(P0, Q0)
+    dA1, dB1 = dAB(Pm, Qm)
+    dA2, dB2 = dAB(P1, Q1)
+    ph = np.exp(1j * (0.5 * (z1 - z0) / k0) * gam)
+    pf = np.exp(1j * ((z1 - z0) / k0) * gam)
+    c = 1j * ((z1 - z0) / (6 * 2 * k0))
+    Dh = ph[:, None] * ph[None, :]
+    err = 0.0
+    tuu = c * (pf[:, None] * dA0 + 4.0 * Dh * dA1 + dA2 * pf[None, :])
+    err = max(err, np.abs(tuu).max())
+    tuu = tuu + np.diag(pf)
+    rud = -c * (pf[:, None] * dB0 * pf[None, :] + 4.0 * Dh * dB1 + dB2)
+    err = max(err, np.abs(rud).max())
+    rdu = -c * (dB0 + 4.0 * Dh * dB1 + pf[:, None] * dB2 * pf[None, :])
+    err = max(err, np.abs(rdu).max())
+    tdd = c * (dA0 * pf[None, :] + 4.0 * Dh * dA1 + pf[:, None] * dA2)
+    err = max(err, np.abs(tdd).max())
+    tdd = tdd + np.diag(pf)
+    return tdd, tuu, rud, rdu, err
+
+
+def redheffer_integrate(sampler: LayerSampler, z0: float, z1: float, pe: float = 1e-2,
+                        max_depth: int = 60) -> SResult:
+    """RedhefferIntegrator::compute, active trisection branch
+    (src/core/RedhefferIntegrator.cpp:174-224 and recursive_solve :226-314).
+
+    ``tree`` records the accepted segments (z0, z1) in cascade order so that a
+    device implementation can be checked on the same subdivision.
+    """
+    lam = sampler.lam
+    k0 = 2.0 * PI / lam
+    P0_, Q0_ = sampler.sample(z0)
+    W0, V0, gam0, _ = wave_equation_solve(P0_, Q0_)
+    P1_, Q1_ = sampler.sample(z1)
+    W1, V1, gam1, _ = wave_equation_solve(P1_, Q1_)
+    zm = 0.5 * (z0 + z1)
+    Pm, Qm = sampler.sample(zm)
+    Wm, Vm, gamm, _ = wave_equation_solve(Pm, Qm)
+    nD = W0.shape[0]
+    st = dict(S=(np.eye(nD, dtype=np.complex128), np.eye(nD, dtype=np.complex128),
+                 np.zeros((nD, nD), np.complex128), np.zeros((nD, nD), np.complex128)),
+              leftW=W0, leftV=V0, N=0, tree=[])
+
+    def rec(Pr, Qr, W, V, gam, za, zb, Pa, Pmid, Pb, Qa, Qmid, Qb, depth):
+        tdd, tuu, rud, rdu, err = perturbation_and_difference(
+            k0, Pr, Qr, W, V, gam, za, zb, Pa, Pmid, Pb, Qa, Qmid, Qb)
+        if err <= pe or depth >= max_depth:
+            st["N"] += 1
+            st["tree"].append((za, zb))
+            invW_W0 = np.linalg.solve(W, st["leftW"])
+            invV_V0 = np.linalg.solve(V, st["leftV"])
+            A = 0.5 * (invW_W0 + invV_V0)
+            B = 0.5 * (invW_W0 - invV_V0)
+            K = A - rdu @ B
+            rdu_n = np.linalg.solve(K, rdu @ A - B)
+            tdd_n = np.linalg.solve(K, tdd)
+            rud_n = tuu @ B @ tdd_n + rud
+            tuu_n = tuu @ A + tuu @ B @ rdu_n
+            st["leftW"], st["leftV"] = W, V
+            st["S"] = star_product(st["S"], tdd_n, tuu_n, rud_n, rdu_n)
+            return
+        zm0 = 2.0 / 3.0 * za + 1.0 / 3.0 * zb
+        zm1 = 1.0 / 3.0 * za + 2.0 / 3.0 * zb
+        Pt, Qt = sampler.sample(0.5 * (za + zm0))
+        Pk, Qk = sampler.sample(zm0)
+        Wt, Vt, gamt, _ = wave_equation_solve(Pt, Qt)
+        rec(Pt, Qt, Wt, Vt, gamt, za, zm0, Pa, Pt, Pk, Qa, Qt, Qk, depth + 1)
+        Pt2, Qt2 = sampler.sample(zm1)
+        rec(Pr, Qr, W, V, gam, zm0, zm1, Pk, Pmid, Pt2, Qk, Qmid, Qt2, depth + 1)
+        Pk2, Qk2 = sampler.sample(0.5 * (zm1 + zb))
+        Wt2, Vt2, gamt2, _ = wave_equation_solve(Pk2, Qk2)
+        rec(Pk2, Qk2, Wt2, Vt2, gamt2, zm1, zb, Pt2, Pk2, Pb, Qt2, Qk2, Qb, depth + 1)
+
+    rec(Pm, Qm, Wm, Vm, gamm, z0, z1, P0_, Pm, P1_, Q0_, Qm, Q1_, 0)
+    Tdd, Tuu, Rud, Rdu = st["S"]
+    # right projection (src/core/RedhefferIntegrator.cpp:211-223)
+    invW0_W = np.linalg.solve(st["leftW"], W1)
+    invV0_V = np.linalg.solve(st["leftV"], V1)
+    A = 0.5 * (invW0_W + invV0_V)
+    B = 0.5 * (invW0_W - invV0_V)
+    K = A - Rud @ B
+    Tuu = np.linalg.solve(K, Tuu)
+    Rud = np.linalg.solve(K, Rud @ A - B)
+    Rdu = Rdu + Tdd @ B @ Tuu
+    Tdd = Tdd @ (B @ Rud + A)
+    return SResult(Tdd, Tuu, Rud, Rdu, W0, V0, gam0, W1, V1, gam1, N=st["N"], nEigVal=2,
+                   tree=st["tree"])
+
+
+# ---------------------------------------------------------------------------
+# Differential method
+# ---------------------------------------------------------------------------
+
+def differential_integrate(sampler: LayerSampler, z0: float, z1: float, N: int) -> SResult:
+    """DifferentialIntegrator::compute (src/core/DifferentialIntegrator.cpp:21-134)."""
+    lam = sampler.lam
+    k0 = 2.0 * PI / lam
+    P1_, Q1_ = sampler.sample(z1)
+    W1, V1, gam1, _ = wave_equation_solve(P1_, Q1_)
+    nD = W1.shape[0]
+    S = (np.eye(nD, dtype=np.complex128), np.eye(nD, dtype=np.complex128),
+         np.zeros((nD, nD), np.complex128), np.zeros((nD, nD), np.complex128))
+    dz = (z1 - z0) / float(N)
+    I = np.eye(nD)
+    P0, Q0 = sampler.sample(z0)
+    for i in range(N):
+        zl = z0 + i * dz
+        zr = z0 + (i + 1) * dz
+        zm = 0.5 * (zl + zr)
+        P1, Q1 = sampler.sample(zm)
+        P2, Q2 = sampler.sample(zr)
+        a = dz * dz / (6.0 * k0 * k0)
+        b = dz * dz / (4.0 * k0 * k0)
+        c = dz * dz / (2.0 * k0 * k0)
+        L11 = I - a * (P1 @ Q0 + P1 @ Q1 + P2 @ Q1 - b * (P2 @ Q1 @ P1 @ Q0))
+        L12 = (1j * dz / (6.0 * k0)) * (P0 + 4.0 * P1 + P2 - c * (P1 @ Q1 @ P0 + P2 @ Q1 @ P1))
+        L21 = (1j * dz / (6.0 * k0)) * (Q0 + 4.0 * Q1 + Q2 - c * (Q1 @ P1 @ Q0 + Q2 @ P1 @ Q1))
+        L22 = I - a * (Q1 @ P0 + Q1 @ P1 + Q2 @ P1 - b * (Q2 @ P1 @ Q1 @ P0))
+        iW11 = np.linalg.solve(W1, L11)
+        iW12 = np.linalg.solve(W1, L12)
+        iV21 = np.linalg.solve(V1, L21)
+        iV22 = np.linalg.solve(V1, L22)
+        A11 = 0.5 * (iW11 + iV21)
+        A12 = 0.5 * (iW12 + iV22)
+        A21 = 0.5 * (iW11 - iV21)
+        A22 = 0.5 * (iW12 - iV22)
+        T11 = A11 @ W1 + A12 @ V1
+        T12 = A11 @ W1 - A12 @ V1
+        T21 = A21 @ W1 + A22 @ V1
+        T22 = A21 @ W1 - A22 @ V1
+        tdd = np.linalg.inv(T22)
+        rud = T12 @ tdd
+        rdu = -np.linalg.solve(T22, T21)
+        tuu = T11 + T12 @ rdu
+        S = star_product(S, tdd, tuu, rud, rdu)
+        P0, Q0 = P2, Q2
+    P0_, Q0_ = sampler.sample(z0)
+    W0, V0, gam0, _ = wave_equation_solve(P0_, Q0_)
+    S = left_end_projection(S, W1, V1, W0, V0)
+    return SResult(*S, W0, V0, gam0, W1, V1, gam1, N=N)
+
+
+# ---------------------------------------------------------------------------
+# Basis-invariant views used by the parity tests
+# ---------------------------------------------------------------------------
+
+def invariant_transforms(r: SResult) -> Dict[str, np.ndarray]:
+    """Harmonic-space maps that do not depend on the eigensolver's choice of
+    eigenvector order, phase or rotation inside degenerate eigenspaces:
+    W1 Tuu W0^-1, W0 Tdd W1^-1, W0 Rdu W0^-1, W1 Rud W1^-1."""
+    iW0 = np.linalg.inv(r.W0)
+    iW1 = np.linalg.inv(r.W1)
+    return dict(Tuu=r.W1 @ r.Tuu @ iW0, Tdd=r.W0 @ r.Tdd @ iW1,
+                Rdu=r.W0 @ r.Rdu @ iW0, Rud=r.W1 @ r.Rud @ iW1)

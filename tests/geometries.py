@@ -1,0 +1,99 @@
+"""Scenario geometries of the reference drivers, restated as data (shared by tests)."""
+import math
+
+import numpy as np
+
+PI = 3.1415926535897932
+
+
+def metasurface(nh=2):
+    """src/tests/test_metasurface_cpu.cpp:129-164 (periodic, pillar edge moves with z)."""
+    x = np.array([0.0, 0.1, 0.5, 0.65])
+    y = x.copy()
+    cl, co = 1.0, 3.48 * 3.48
+    eps = np.array([[cl, cl, cl], [cl, co, cl], [cl, cl, cl]], dtype=complex)
+
+    def rule(z):
+        return 0.5 if z <= 1.0 else 0.2 * math.cos(0.5 * PI * (z - 1.0)) + 0.3
+
+    return dict(lam=1.55, nx=2 * nh + 1, ny=2 * nh + 1, x=x, y=y, eps=eps, pml=False, rules={2: rule},
+                z0=0.0, z1=2.0)
+
+
+def var12(nhx=2, nhy=2):
+    """src/tests/test_cpu.cpp:128-176 (PML waveguide taper, var12 rule)."""
+    x = np.array([0.0, 0.5, 1.5, 2.6, 3.7, 3.9, 5.0, 5.5])
+    y = np.array([0.0, 0.5, 1.5, 1.58, 1.8, 2.8, 3.3])
+    cl, co = 1.445 ** 2, 3.48 ** 2
+    eps = np.full((6, 7), cl, dtype=complex)
+    eps[3, 2] = co
+
+    def rule(z):
+        if z < 0.0:
+            return 2.6
+        if z > 10.0:
+            return 3.7
+        return (1.0 - 0.1 * z) * 2.6 + 0.1 * z * 3.7
+
+    return dict(lam=1.55, nx=2 * nhx + 1, ny=2 * nhy + 1, x=x, y=y, eps=eps, pml=True, rules={3: rule},
+                z0=-0.1, z1=10.1)
+
+
+def oracle_sampler(g):
+    from oracle import rcwa_oracle as O
+    s = O.VarLayerSampler(g["lam"], g["nx"], g["ny"], g["x"], g["y"], g["eps"], g["pml"])
+    for k, f in g["rules"].items():
+        s.addXRule(k, f)
+    return s
+
+
+def gpu_sampler(g, ctx=None):
+    from varrcwa_amd.api import VarLayerSampler
+    s = VarLayerSampler(g["lam"], g["nx"], g["ny"], g["x"], g["y"], g["eps"], g["pml"], ctx=ctx)
+    for k, f in g["rules"].items():
+        s.addXRule(k, f)
+    return s
+
+
+def random_rect_layer(rng, cell=0.6, eps_choices=(4.0, 12.25), nmax=4, side=(0.05, 0.3)):
+    """A union of 1..nmax axis-aligned rectangles (periodically wrapped) in a square
+    cell, on an eps=1 background, as an exact Layer(x, y, eps) grid."""
+    nrect = int(rng.integers(1, nmax + 1))
+    rects = []
+    for _ in range(nrect):
+        w, h = rng.uniform(*side), rng.uniform(*side)
+        x0, y0 = rng.uniform(0, cell), rng.uniform(0, cell)
+        e = float(eps_choices[int(rng.integers(0, len(eps_choices)))])
+        rects.append((x0, y0, w, h, e))
+    return rects_to_layer(rects, cell)
+
+
+def rects_to_layer(rects, cell):
+    """Rasterise rectangles (x0, y0, w, h, eps) painted in order onto an exact grid."""
+    xs, ys = {0.0, cell}, {0.0, cell}
+    pieces = []
+    for (x0, y0, w, h, e) in rects:
+        for dxs in (0.0, -cell):
+            for dys in (0.0, -cell):
+                a, b = x0 + dxs, x0 + w + dxs
+                c, d = y0 + dys, y0 + h + dys
+                a, b = max(a, 0.0), min(b, cell)
+                c, d = max(c, 0.0), min(d, cell)
+                if b > a and d > c:
+                    pieces.append((a, b, c, d, e))
+                    xs.update([a, b])
+                    ys.update([c, d])
+    x = np.array(sorted(xs))
+    y = np.array(sorted(ys))
+    eps = np.ones((y.size - 1, x.size - 1), dtype=complex)
+    xm = 0.5 * (x[1:] + x[:-1])
+    ym = 0.5 * (y[1:] + y[:-1])
+    for (a, b, c, d, e) in pieces:
+        mx = (xm > a) & (xm < b)
+        my = (ym > c) & (ym < d)
+        eps[np.ix_(my, mx)] = e
+    return x, y, eps
+
+
+def homogeneous(cell, eps):
+    return np.array([0.0, cell]), np.array([0.0, cell]), np.array([[eps]], dtype=complex)

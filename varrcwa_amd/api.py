@@ -1,0 +1,343 @@
+"""Python binding of the C ABI with the reference's class and method names.
+
+Mirrors ``VarLayerSampler`` (src/core/VarLayerSampler.h:8-37), ``RCWAIntegrator``
+(src/core/RCWAIntegrator.h:8-40), ``RedhefferIntegrator``
+(src/core/RedhefferIntegrator.h:10-90) and ``DifferentialIntegrator``
+(src/core/DifferentialIntegrator.h:8-40).  Matrices returned to Python are numpy
+complex128 arrays; the computation itself runs in ``libvarrcwa.so`` on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+from ._lib import (RULE_FN, Context, VrcError, VrcLayer, VrcStackInstance, c_double_p, cplx, from_dev_colmajor,
+                   load, ptr, to_dev_colmajor)
+
+__all__ = ["Context", "Layer", "VarLayerSampler", "StackLayerSampler", "RCWAIntegrator", "RedhefferIntegrator",
+           "DifferentialIntegrator", "batch_stack_rcwa", "assemble_toeplitz", "assemble_pq", "eig_pq",
+           "layer_smatrix", "star_product", "project_left", "redheffer_segment", "VrcError"]
+
+_DEFAULT_CTX: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+class Layer:
+    """Layer(x, y, eps) (src/rcwa/Layer.h:25-27); eps has shape (len(y)-1, len(x)-1)."""
+
+    def __init__(self, x, y, eps):
+        self.x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+        self.y = np.ascontiguousarray(np.asarray(y, dtype=np.float64))
+        e = np.asarray(eps, dtype=np.complex128)
+        if e.shape != (self.y.size - 1, self.x.size - 1):
+            raise ValueError(f"eps shape {e.shape} does not match coordinates "
+                             f"({self.y.size - 1}, {self.x.size - 1})")
+        self.eps = np.asfortranarray(e)
+        self._eps_flat = np.ascontiguousarray(self.eps.reshape(-1, order="F")).view(np.float64)
+        self.struct = VrcLayer(self.x.size, self.x.ctypes.data_as(c_double_p), self.y.size,
+                               self.y.ctypes.data_as(c_double_p), self._eps_flat.ctypes.data_as(c_double_p))
+
+
+def _as_layer(l) -> Layer:
+    return l if isinstance(l, Layer) else Layer(*l)
+
+
+def _layer_array(layers: Sequence[Layer]):
+    arr = (VrcLayer * max(1, len(layers)))()
+    for i, l in enumerate(layers):
+        arr[i] = l.struct
+    return arr
+
+
+def _host_mat(buf: np.ndarray, n: int) -> np.ndarray:
+    """Column-major host buffer of n*n complex -> numpy matrix."""
+    return buf.reshape(n, n).T.copy()
+
+
+class _Sampler:
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        self._keep: List = []
+
+    def nDim(self) -> int:
+        return int(self.ctx.lib.vrc_sampler_ndim(self.h))
+
+    def sample(self, z: float):
+        """LayerSampler::sample -> (P, Q) as numpy arrays."""
+        N = self.nDim()
+        P = np.empty(N * N, np.complex128)
+        Q = np.empty(N * N, np.complex128)
+        self.ctx.check(self.ctx.lib.vrc_sampler_sample(self.h, float(z), P.ctypes.data_as(c_double_p),
+                                                       Q.ctypes.data_as(c_double_p)), "sample")
+        return _host_mat(P, N), _host_mat(Q, N)
+
+    def sampleOnGPU(self, z: float, P, Q):
+        """LayerSampler::sampleOnGPU into device tensors (column-major N x N)."""
+        self.ctx.check(self.ctx.lib.vrc_sampler_sample_gpu(self.h, float(z), ptr(P), ptr(Q)), "sampleOnGPU")
+
+    def _K(self):
+        n = self.nDim() // 2
+        Kx = np.empty(n * n, np.complex128)
+        Ky = np.empty(n * n, np.complex128)
+        self.ctx.check(self.ctx.lib.vrc_sampler_k(self.h, Kx.ctypes.data_as(c_double_p),
+                                                  Ky.ctypes.data_as(c_double_p)), "K")
+        return _host_mat(Kx, n), _host_mat(Ky, n)
+
+    def Kx(self):
+        return self._K()[0]
+
+    def Ky(self):
+        return self._K()[1]
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.vrc_sampler_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class VarLayerSampler(_Sampler):
+    """VarLayerSampler(lambda, nx, ny, x, y, eps, enablePML=True)."""
+
+    def __init__(self, lam, nx, ny, x, y, eps, enablePML=True, ctx: Optional[Context] = None):
+        super().__init__(ctx or default_context())
+        self.lam, self.nx, self.ny = float(lam), int(nx), int(ny)
+        self.base = Layer(x, y, eps)
+        self.ctx.check(self.ctx.lib.vrc_var_sampler_create(self.ctx.h, self.lam, self.nx, self.ny,
+                                                           C.byref(self.base.struct), int(bool(enablePML)),
+                                                           C.byref(self.h)), "VarLayerSampler")
+
+    def _rule(self, axis: int, layout: int, rule: Callable[[float], float]):
+        cb = RULE_FN(lambda z, user: float(rule(z)))
+        self._keep.append(cb)
+        self.ctx.check(self.ctx.lib.vrc_sampler_add_rule(self.h, axis, int(layout), cb, None), "addRule")
+
+    def addXRule(self, layout: int, rule: Callable[[float], float]):
+        self._rule(0, layout, rule)
+
+    def addYRule(self, layout: int, rule: Callable[[float], float]):
+        self._rule(1, layout, rule)
+
+
+class StackLayerSampler(_Sampler):
+    """Piecewise-constant stack of Layer slabs between half-space media."""
+
+    def __init__(self, lam, nx, ny, layers, z_edges, bottom, top, enablePML=False, ctx: Optional[Context] = None):
+        super().__init__(ctx or default_context())
+        self.layers = [_as_layer(l) for l in layers]
+        self.bottom, self.top = _as_layer(bottom), _as_layer(top)
+        self.z_edges = np.ascontiguousarray(np.asarray(z_edges, np.float64))
+        arr = _layer_array(self.layers)
+        self._keep.append(arr)
+        self.ctx.check(self.ctx.lib.vrc_stack_sampler_create(
+            self.ctx.h, float(lam), int(nx), int(ny), len(self.layers), arr,
+            self.z_edges.ctypes.data_as(c_double_p), C.byref(self.bottom.struct), C.byref(self.top.struct),
+            int(bool(enablePML)), C.byref(self.h)), "StackLayerSampler")
+
+
+_GETTERS = ["Tdd", "Tuu", "Rud", "Rdu", "W0", "V0", "gam0", "W1", "V1", "gam1"]
+
+
+class _Integrator:
+    KIND = 0
+
+    def __init__(self, z0: float, z1: float, sampler: _Sampler):
+        self.ctx = sampler.ctx
+        self.sampler = sampler
+        self.h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.vrc_integrator_create(self.ctx.h, self.KIND, float(z0), float(z1), sampler.h,
+                                                          C.byref(self.h)), "integrator")
+
+    def _get(self, which: int):
+        N = self.sampler.nDim()
+        n = N if which in (6, 9) else N * N
+        buf = np.empty(n, np.complex128)
+        self.ctx.check(self.ctx.lib.vrc_integrator_get(self.h, which, buf.ctypes.data_as(c_double_p)), "get")
+        return buf if which in (6, 9) else _host_mat(buf, N)
+
+    def counts(self):
+        a, b = C.c_int(), C.c_int()
+        self.ctx.lib.vrc_integrator_counts(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def N(self):
+        return self.counts()[0]
+
+    def nEigVal(self):
+        return self.counts()[1]
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.vrc_integrator_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+for _i, _name in enumerate(_GETTERS):
+    setattr(_Integrator, _name, (lambda i: lambda self: self._get(i))(_i))
+
+
+class RCWAIntegrator(_Integrator):
+    """RCWAIntegrator(z0, z1, sampler).compute(N)."""
+    KIND = 0
+
+    def compute(self, N: int):
+        self.ctx.check(self.ctx.lib.vrc_rcwa_compute(self.h, int(N)), "RCWAIntegrator.compute")
+
+    def compute_batched(self, N: int):
+        self.ctx.check(self.ctx.lib.vrc_rcwa_compute_batched(self.h, int(N)), "RCWAIntegrator.compute_batched")
+
+    def compute_edges(self, z_edges):
+        ze = np.ascontiguousarray(np.asarray(z_edges, np.float64))
+        self.ctx.check(self.ctx.lib.vrc_rcwa_compute_edges(self.h, ze.size, ze.ctypes.data_as(c_double_p)),
+                       "RCWAIntegrator.compute_edges")
+
+
+class RedhefferIntegrator(_Integrator):
+    """RedhefferIntegrator(z0, z1, sampler).compute(p, maxN=10, pe=1e-2)."""
+    KIND = 1
+
+    def compute(self, p: int, maxN: int = 10, pe: float = 1e-2):
+        self.ctx.check(self.ctx.lib.vrc_redheffer_compute(self.h, int(p), int(maxN), float(pe)),
+                       "RedhefferIntegrator.compute")
+
+    def tree(self):
+        n = self.ctx.lib.vrc_integrator_tree(self.h, 0, None)
+        buf = np.empty(2 * max(n, 1), np.float64)
+        self.ctx.lib.vrc_integrator_tree(self.h, n, buf.ctypes.data_as(c_double_p))
+        return [(buf[2 * i], buf[2 * i + 1]) for i in range(n)]
+
+
+class DifferentialIntegrator(_Integrator):
+    """DifferentialIntegrator(z0, z1, sampler).compute(N)."""
+    KIND = 2
+
+    def compute(self, N: int):
+        self.ctx.check(self.ctx.lib.vrc_differential_compute(self.h, int(N)), "DifferentialIntegrator.compute")
+
+
+# ------------------------------------------------------------------ batched stacks
+def batch_stack_rcwa(ctx: Context, nx: int, ny: int, instances, chunk: int = 0, want_bases: bool = False):
+    """instances: list of dicts with keys lam, layers (Layer list), thickness, bottom, top.
+
+    Returns a dict of device tensors: S (ninst, 4, N, N) as column-major storage
+    (use ``from_dev_colmajor``), and W0, V0, gam0, W1, V1, gam1 when requested.
+    """
+    torch = ctx.torch
+    n = nx * ny
+    N = 2 * n
+    keep = []
+    arr = (VrcStackInstance * len(instances))()
+    for i, inst in enumerate(instances):
+        ls = [_as_layer(l) for l in inst["layers"]]
+        la = _layer_array(ls)
+        th = np.ascontiguousarray(np.asarray(inst["thickness"], np.float64))
+        bot, top = _as_layer(inst["bottom"]), _as_layer(inst["top"])
+        keep += [ls, la, th, bot, top]
+        arr[i] = VrcStackInstance(float(inst["lam"]), len(ls), la, th.ctypes.data_as(c_double_p),
+                                  C.pointer(bot.struct), C.pointer(top.struct))
+    dev = f"cuda:{ctx.device}"
+    S = torch.empty((len(instances), 4, N, N), dtype=torch.complex128, device=dev)
+    out = {"S": S}
+    ptrs = [None] * 6
+    if want_bases:
+        for k, name in enumerate(["W0", "V0", "gam0", "W1", "V1", "gam1"]):
+            shape = (len(instances), N) if name.startswith("gam") else (len(instances), N, N)
+            out[name] = torch.empty(shape, dtype=torch.complex128, device=dev)
+            ptrs[k] = ptr(out[name])
+    ctx.check(ctx.lib.vrc_batch_stack_rcwa(ctx.h, nx, ny, len(instances), arr, int(chunk), ptr(S), *ptrs),
+              "batch_stack_rcwa")
+    return out
+
+
+# ------------------------------------------------------------------ stage entry points
+def assemble_toeplitz(ctx: Context, nx: int, ny: int, layers):
+    torch = ctx.torch
+    ls = [_as_layer(l) for l in layers]
+    n = nx * ny
+    fe = torch.empty((3, len(ls), n, n), dtype=torch.complex128, device=f"cuda:{ctx.device}")
+    arr = _layer_array(ls)
+    ctx.check(ctx.lib.vrc_assemble_toeplitz(ctx.h, nx, ny, len(ls), arr, ptr(fe[0]), ptr(fe[1]), ptr(fe[2]),
+                                            n * n), "assemble_toeplitz")
+    return fe
+
+
+def assemble_pq(ctx: Context, nx: int, ny: int, layers, lam: float, enable_pml: bool, region):
+    torch = ctx.torch
+    ls = [_as_layer(l) for l in layers]
+    reg = _as_layer(region)
+    N = 2 * nx * ny
+    P = torch.empty((len(ls), N, N), dtype=torch.complex128, device=f"cuda:{ctx.device}")
+    Q = torch.empty_like(P)
+    arr = _layer_array(ls)
+    ctx.check(ctx.lib.vrc_assemble_pq(ctx.h, nx, ny, len(ls), arr, float(lam), int(bool(enable_pml)),
+                                      C.byref(reg.struct), ptr(P), ptr(Q), N * N), "assemble_pq")
+    return P, Q
+
+
+def eig_pq(ctx: Context, P, Q):
+    torch = ctx.torch
+    B, N, _ = P.shape
+    W = torch.empty_like(P)
+    V = torch.empty_like(P)
+    g = torch.empty((B, N), dtype=torch.complex128, device=P.device)
+    info = torch.zeros(B, dtype=torch.int32, device=P.device)
+    ctx.check(ctx.lib.vrc_eig_pq(ctx.h, N, B, ptr(P), ptr(Q), N * N, ptr(W), ptr(V), ptr(g), ptr(info)), "eig_pq")
+    return W, V, g, info
+
+
+def layer_smatrix(ctx: Context, lam, dz, W, V, gam, W0, V0, shared_ref: bool = True):
+    torch = ctx.torch
+    B, N, _ = W.shape
+    lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float64), (B,)))
+    dz = np.ascontiguousarray(np.broadcast_to(np.asarray(dz, np.float64), (B,)))
+    Tuu = torch.empty_like(W)
+    Rud = torch.empty_like(W)
+    ctx.check(ctx.lib.vrc_layer_smatrix(ctx.h, N, B, lam.ctypes.data_as(c_double_p), dz.ctypes.data_as(c_double_p),
+                                        ptr(W), ptr(V), ptr(gam), ptr(W0), ptr(V0), 0 if shared_ref else N * N,
+                                        ptr(Tuu), ptr(Rud)), "layer_smatrix")
+    return Tuu, Rud
+
+
+def star_product(ctx: Context, S, s):
+    """S, s: tensors (4, B, N, N) [dd, uu, ud, du] (S updated in place)."""
+    _, B, N, _ = S.shape
+    ctx.check(ctx.lib.vrc_star_product(ctx.h, N, B, ptr(S[0]), ptr(S[1]), ptr(S[2]), ptr(S[3]), ptr(s[0]), ptr(s[1]),
+                                       ptr(s[2]), ptr(s[3]), N * N), "star_product")
+
+
+def project_left(ctx: Context, S, W1, V1, W0, V0):
+    _, B, N, _ = S.shape
+    ctx.check(ctx.lib.vrc_project_left(ctx.h, N, B, ptr(S[0]), ptr(S[1]), ptr(S[2]), ptr(S[3]), ptr(W1), ptr(V1),
+                                       ptr(W0), ptr(V0), N * N), "project_left")
+
+
+def redheffer_segment(ctx: Context, lam, z0, z1, Pr, Qr, W, V, gam, P3, Q3):
+    torch = ctx.torch
+    N = W.shape[-1]
+    outs = [torch.empty((N, N), dtype=torch.complex128, device=W.device) for _ in range(4)]
+    err = C.c_double()
+    ctx.check(ctx.lib.vrc_redheffer_segment(ctx.h, N, float(lam), float(z0), float(z1), ptr(Pr), ptr(Qr), ptr(W),
+                                            ptr(V), ptr(gam), ptr(P3[0]), ptr(P3[1]), ptr(P3[2]), ptr(Q3[0]),
+                                            ptr(Q3[1]), ptr(Q3[2]), *[ptr(o) for o in outs], C.byref(err)),
+              "redheffer_segment")
+    return outs, err.value

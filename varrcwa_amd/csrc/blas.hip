@@ -1,0 +1,309 @@
+// Complex128 dense kernels for gfx950: FP64-MFMA ZGEMM (strided and descriptor
+// batched) and the elementwise matrix kernels used between the factorizations.
+//
+// ZGEMM tile: 64x64 output per workgroup (4 wavefronts, 32x32 each = 2x2 blocks of
+// v_mfma_f64_16x16x4f64), K staged through LDS in chunks of 16 complex values,
+// real and imaginary planes kept separate so that one complex multiply-add is four
+// real MFMAs (standard 4M product, no 3M/Gauss trick).
+#include "vrc_common.hpp"
+#include "vrc_internal.hpp"
+
+namespace vrc {
+
+constexpr int GT_M = 64, GT_N = 64, GT_K = 16, GT_THREADS = 256;
+
+template <int OPA, int OPB>
+__device__ __forceinline__ void zgemm_tile(int m, int n, int k, cd alpha, const cd* __restrict__ A, long lda,
+                                           const cd* __restrict__ B, long ldb, cd beta, cd* __restrict__ C,
+                                           long ldc, int tm, int tn) {
+  __shared__ double As_re[GT_K][GT_M];
+  __shared__ double As_im[GT_K][GT_M];
+  __shared__ double Bs_re[GT_K][GT_N];
+  __shared__ double Bs_im[GT_K][GT_N];
+
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int row0 = tm * GT_M, col0 = tn * GT_N;
+
+  d4 accR[2][2], accI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      accI[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  for (int kk = 0; kk < k; kk += GT_K) {
+    // ---- stage op(A) tile (GT_M x GT_K) into LDS as [k][m]
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int r, c;
+      cd v = mk(0.0, 0.0);
+      if (OPA == OP_N) {
+        r = t & 63;
+        c = (t >> 6) + 4 * i;
+        int gr = row0 + r, gc = kk + c;
+        if (gr < m && gc < k) v = A[gr + (long)gc * lda];
+      } else {
+        c = t & 15;
+        r = (t >> 4) + 16 * i;
+        int gr = row0 + r, gc = kk + c;
+        if (gr < m && gc < k) v = A[gc + (long)gr * lda];
+        if (OPA == OP_C) v.y = -v.y;
+      }
+      As_re[c][r] = v.x;
+      As_im[c][r] = v.y;
+    }
+    // ---- stage op(B) tile (GT_K x GT_N) into LDS as [k][n]
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int c, j;
+      cd v = mk(0.0, 0.0);
+      if (OPB == OP_N) {
+        c = t & 15;
+        j = (t >> 4) + 16 * i;
+        int gc = kk + c, gj = col0 + j;
+        if (gc < k && gj < n) v = B[gc + (long)gj * ldb];
+      } else {
+        j = t & 63;
+        c = (t >> 6) + 4 * i;
+        int gc = kk + c, gj = col0 + j;
+        if (gc < k && gj < n) v = B[gj + (long)gc * ldb];
+        if (OPB == OP_C) v.y = -v.y;
+      }
+      Bs_re[c][j] = v.x;
+      Bs_im[c][j] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < GT_K / 4; ++s) {
+      const int kr = s * 4 + (lane >> 4);
+      double ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        ar[b] = As_re[kr][wr * 32 + b * 16 + (lane & 15)];
+        ai[b] = As_im[kr][wr * 32 + b * 16 + (lane & 15)];
+        nai[b] = -ai[b];
+        br[b] = Bs_re[kr][wc * 32 + b * 16 + (lane & 15)];
+        bi[b] = Bs_im[kr][wc * 32 + b * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], accR[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+
+  // ---- epilogue: D layout of v_mfma_f64_16x16x4f64: register r of a lane holds row lane/16 + 4*r,
+  // column lane%16 (established on gfx950 against a reference product)
+  const bool beta0 = (beta.x == 0.0 && beta.y == 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int gc = col0 + wc * 32 + j * 16 + (lane & 15);
+      if (gc >= n) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = row0 + wr * 32 + i * 16 + (lane >> 4) + 4 * r;
+        if (gr >= m) continue;
+        cd acc = mk(accR[i][j][r], accI[i][j][r]);
+        cd out = cmul(alpha, acc);
+        cd* cp = C + gr + (long)gc * ldc;
+        if (!beta0) out = cadd(out, cmul(beta, *cp));
+        *cp = out;
+      }
+    }
+}
+
+// Scale-only path for k == 0 (C = beta*C), shared by both batch modes.
+__device__ __forceinline__ void scale_tile(int m, int n, cd beta, cd* C, long ldc, int tm, int tn) {
+  const bool beta0 = (beta.x == 0.0 && beta.y == 0.0);
+  for (int e = threadIdx.x; e < GT_M * GT_N; e += GT_THREADS) {
+    int r = e % GT_M, c = e / GT_M;
+    int gr = tm * GT_M + r, gc = tn * GT_N + c;
+    if (gr < m && gc < n) {
+      cd* p = C + gr + (long)gc * ldc;
+      *p = beta0 ? mk(0.0, 0.0) : cmul(beta, *p);
+    }
+  }
+}
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(GT_THREADS) void zgemm_strided_kernel(int m, int n, int k, cd alpha, const cd* A,
+                                                                   long lda, long sA, const cd* B, long ldb,
+                                                                   long sB, cd beta, cd* C, long ldc, long sC) {
+  const int tm = blockIdx.x, tn = blockIdx.y;
+  const long b = blockIdx.z;
+  if (k <= 0) {
+    scale_tile(m, n, beta, C + b * sC, ldc, tm, tn);
+    return;
+  }
+  zgemm_tile<OPA, OPB>(m, n, k, alpha, A + b * sA, lda, B + b * sB, ldb, beta, C + b * sC, ldc, tm, tn);
+}
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(GT_THREADS) void zgemm_desc_kernel(const GemmDesc* __restrict__ desc) {
+  const GemmDesc d = desc[blockIdx.z];
+  const int tm = blockIdx.x, tn = blockIdx.y;
+  if (d.m <= 0 || d.n <= 0) return;
+  if (tm * GT_M >= d.m || tn * GT_N >= d.n) return;
+  if (d.k <= 0) {
+    scale_tile(d.m, d.n, d.beta, d.C, d.ldc, tm, tn);
+    return;
+  }
+  zgemm_tile<OPA, OPB>(d.m, d.n, d.k, d.alpha, d.A, d.lda, d.B, d.ldb, d.beta, d.C, d.ldc, tm, tn);
+}
+
+void zgemm_strided(hipStream_t s, int opA, int opB, int m, int n, int k, cd alpha, const cd* A, long lda,
+                   long sA, const cd* B, long ldb, long sB, cd beta, cd* C, long ldc, long sC, int batch) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  dim3 grid(cdiv_up(m, GT_M), cdiv_up(n, GT_N), batch);
+  switch (opA * 3 + opB) {
+    case 0: zgemm_strided_kernel<OP_N, OP_N><<<grid, GT_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC); break;
+    case 1: zgemm_strided_kernel<OP_N, OP_T><<<grid, GT_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC); break;
+    case 2: zgemm_strided_kernel<OP_N, OP_C><<<grid, GT_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC); break;
+    case 3: zgemm_strided_kernel<OP_T, OP_N><<<grid, GT_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC); break;
+    case 4: zgemm_strided_kernel<OP_T, OP_T><<<grid, GT_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC); break;
+    case 5: zgemm_strided_kernel<OP_T, OP_C><<<grid, GT_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC); break;
+    case 6: zgemm_strided_kernel<OP_C, OP_N><<<grid, GT_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC); break;
+    case 7: zgemm_strided_kernel<OP_C, OP_T><<<grid, GT_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC); break;
+    default: zgemm_strided_kernel<OP_C, OP_C><<<grid, GT_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC); break;
+  }
+  launch_check("zgemm_strided");
+}
+
+void zgemm_desc(hipStream_t s, int opA, int opB, const GemmDesc* d_desc, int count, int max_m, int max_n) {
+  if (count <= 0 || max_m <= 0 || max_n <= 0) return;
+  dim3 grid(cdiv_up(max_m, GT_M), cdiv_up(max_n, GT_N), count);
+  switch (opA * 3 + opB) {
+    case 0: zgemm_desc_kernel<OP_N, OP_N><<<grid, GT_THREADS, 0, s>>>(d_desc); break;
+    case 1: zgemm_desc_kernel<OP_N, OP_T><<<grid, GT_THREADS, 0, s>>>(d_desc); break;
+    case 2: zgemm_desc_kernel<OP_N, OP_C><<<grid, GT_THREADS, 0, s>>>(d_desc); break;
+    case 3: zgemm_desc_kernel<OP_T, OP_N><<<grid, GT_THREADS, 0, s>>>(d_desc); break;
+    case 4: zgemm_desc_kernel<OP_T, OP_T><<<grid, GT_THREADS, 0, s>>>(d_desc); break;
+    case 5: zgemm_desc_kernel<OP_T, OP_C><<<grid, GT_THREADS, 0, s>>>(d_desc); break;
+    case 6: zgemm_desc_kernel<OP_C, OP_N><<<grid, GT_THREADS, 0, s>>>(d_desc); break;
+    case 7: zgemm_desc_kernel<OP_C, OP_T><<<grid, GT_THREADS, 0, s>>>(d_desc); break;
+    default: zgemm_desc_kernel<OP_C, OP_C><<<grid, GT_THREADS, 0, s>>>(d_desc); break;
+  }
+  launch_check("zgemm_desc");
+}
+
+// ============================================================ elementwise kernels
+// All take a strided batch of m x n column-major matrices.
+
+__global__ void geam_kernel(int m, int n, cd a, const cd* A, long lda, long sA, cd b, const cd* B, long ldb,
+                            long sB, cd* C, long ldc, long sC) {
+  const long bz = blockIdx.z;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= m) return;
+  cd v = cmul(a, A[bz * sA + r + (long)c * lda]);
+  if (B) v = cadd(v, cmul(b, B[bz * sB + r + (long)c * ldb]));
+  C[bz * sC + r + (long)c * ldc] = v;
+}
+
+void zgeam(hipStream_t s, int m, int n, cd a, const cd* A, long lda, long sA, cd b, const cd* B, long ldb, long sB,
+           cd* C, long ldc, long sC, int batch) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  dim3 grid(cdiv_up(m, 256), n, batch);
+  geam_kernel<<<grid, 256, 0, s>>>(m, n, a, A, lda, sA, b, B, ldb, sB, C, ldc, sC);
+  launch_check("zgeam");
+}
+
+__global__ void set_ident_kernel(int m, int n, cd diag, cd off, cd* C, long ldc, long sC) {
+  const long bz = blockIdx.z;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= m) return;
+  C[bz * sC + r + (long)c * ldc] = (r == c) ? diag : off;
+}
+
+void zset(hipStream_t s, int m, int n, cd diag, cd off, cd* C, long ldc, long sC, int batch) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  dim3 grid(cdiv_up(m, 256), n, batch);
+  set_ident_kernel<<<grid, 256, 0, s>>>(m, n, diag, off, C, ldc, sC);
+  launch_check("zset");
+}
+
+__global__ void add_diag_kernel(int n, cd v, const cd* vec, long svec, cd* C, long ldc, long sC) {
+  const long bz = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cd add = vec ? cmul(v, vec[bz * svec + i]) : v;
+  cd* p = C + bz * sC + i + (long)i * ldc;
+  *p = cadd(*p, add);
+}
+
+void zadd_diag(hipStream_t s, int n, cd v, const cd* vec, long svec, cd* C, long ldc, long sC, int batch) {
+  if (n <= 0 || batch <= 0) return;
+  dim3 grid(cdiv_up(n, 256), batch);
+  add_diag_kernel<<<grid, 256, 0, s>>>(n, v, vec, svec, C, ldc, sC);
+  launch_check("zadd_diag");
+}
+
+// C = diag(left) * A * diag(right) (either vector may be null); in place allowed.
+__global__ void dscale_kernel(int m, int n, const cd* left, long sl, const cd* right, long sr, const cd* A, long lda,
+                              long sA, cd* C, long ldc, long sC, int inv_right) {
+  const long bz = blockIdx.z;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= m) return;
+  cd v = A[bz * sA + r + (long)c * lda];
+  if (left) v = cmul(left[bz * sl + r], v);
+  if (right) {
+    cd f = right[bz * sr + c];
+    v = inv_right ? cdiv(v, f) : cmul(v, f);
+  }
+  C[bz * sC + r + (long)c * ldc] = v;
+}
+
+void zdscale(hipStream_t s, int m, int n, const cd* left, long sl, const cd* right, long sr, int inv_right,
+             const cd* A, long lda, long sA, cd* C, long ldc, long sC, int batch) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  dim3 grid(cdiv_up(m, 256), n, batch);
+  dscale_kernel<<<grid, 256, 0, s>>>(m, n, left, sl, right, sr, A, lda, sA, C, ldc, sC, inv_right);
+  launch_check("zdscale");
+}
+
+// Max |a_ij| (true modulus) per batch entry -> out[b] (double). One block per entry.
+__global__ void maxabs_kernel(int m, int n, const cd* A, long lda, long sA, double* out) {
+  const long bz = blockIdx.x;
+  __shared__ double red[256];
+  double mx = 0.0;
+  const long total = (long)m * n;
+  for (long e = threadIdx.x; e < total; e += blockDim.x) {
+    int r = (int)(e % m), c = (int)(e / m);
+    double v = cabsv(A[bz * sA + r + (long)c * lda]);
+    mx = (v > mx || v != v) ? v : mx;
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      double o = red[threadIdx.x + w];
+      double c = red[threadIdx.x];
+      red[threadIdx.x] = (o > c || o != o) ? o : c;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[bz] = red[0];
+}
+
+void zmaxabs(hipStream_t s, int m, int n, const cd* A, long lda, long sA, double* out, int batch) {
+  if (batch <= 0) return;
+  maxabs_kernel<<<batch, 256, 0, s>>>(m, n, A, lda, sA, out);
+  launch_check("zmaxabs");
+}
+
+}  // namespace vrc

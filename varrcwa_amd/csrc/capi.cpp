@@ -1,0 +1,481 @@
+// extern "C" entry points (see include/varrcwa.h for the contract of each).
+#include "../../include/varrcwa.h"
+
+#include <cstring>
+#include <memory>
+
+#include "vrc_context.hpp"
+
+using namespace vrc;
+
+namespace {
+inline cd c2(const double* p) { return p ? mk(p[0], p[1]) : mk(0.0, 0.0); }
+inline void need(bool ok, const char* what) {
+  if (!ok) throw ArgFailure(what);
+}
+}  // namespace
+
+extern "C" {
+
+const char* vrc_version(void) { return "varrcwa_amd 0.1 (gfx950)"; }
+
+int vrc_ctx_create(int device, void* stream, vrc_ctx** out) {
+  if (!out) return -1;
+  *out = nullptr;
+  vrc_ctx* c = new vrc_ctx();
+  c->device = device;
+  // NULL selects the device's default (null) stream, which orders the library's work
+  // with every other default-stream user in the process (e.g. torch's default stream).
+  int rc = guarded(c, [&] { c->stream = (hipStream_t)stream; });
+  if (rc != 0) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int vrc_ctx_destroy(vrc_ctx* c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+const char* vrc_last_error(const vrc_ctx* c) { return c ? c->last_error.c_str() : "null context"; }
+
+int vrc_sync(vrc_ctx* c) {
+  return guarded(c, [&] { VRC_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+long long vrc_pool_bytes(const vrc_ctx* c) { return c ? (long long)c->ws.reserved() : 0; }
+
+int vrc_pool_trim(vrc_ctx* c) {
+  return guarded(c, [&] {
+    VRC_HIP(hipStreamSynchronize(c->stream));
+    c->ws.trim();
+  });
+}
+
+int vrc_zgemm(vrc_ctx* c, int opA, int opB, int m, int n, int k, const double* alpha, const void* A, long lda,
+              long sA, const void* B, long ldb, long sB, const double* beta, void* C, long ldc, long sC,
+              int batch) {
+  return guarded(c, [&] {
+    need(opA >= 0 && opA <= 2 && opB >= 0 && opB <= 2, "op must be 0 (N), 1 (T) or 2 (C)");
+    need(m >= 0 && n >= 0 && k >= 0 && batch >= 0, "negative size");
+    need(C != nullptr || m == 0 || n == 0, "null C");
+    zgemm_strided(c->stream, opA, opB, m, n, k, c2(alpha), (const cd*)A, lda, sA, (const cd*)B, ldb, sB, c2(beta),
+                  (cd*)C, ldc, sC, batch);
+  });
+}
+
+int vrc_zgetrf(vrc_ctx* c, int n, void* A, long lda, long sA, int* ipiv, long sP, int* info, int batch) {
+  return guarded(c, [&] {
+    need(n >= 0 && batch >= 0 && lda >= n, "bad size");
+    need(A && ipiv, "null pointer");
+    if (info) VRC_HIP(hipMemsetAsync(info, 0, sizeof(int) * batch, c->stream));
+    zgetrf(c->stream, n, (cd*)A, lda, sA, ipiv, sP, info, batch);
+  });
+}
+
+int vrc_zgetrs(vrc_ctx* c, int n, int nrhs, const void* LU, long lda, long sA, const int* ipiv, long sP,
+               const void* B, long ldb, long sB, void* X, long ldx, long sX, int batch) {
+  return guarded(c, [&] {
+    need(n >= 0 && nrhs >= 0 && batch >= 0 && lda >= n && ldb >= n && ldx >= n, "bad size");
+    zgetrs(c->stream, n, nrhs, (const cd*)LU, lda, sA, ipiv, sP, (const cd*)B, ldb, sB, (cd*)X, ldx, sX, batch,
+           c->ws);
+  });
+}
+
+int vrc_zgeev(vrc_ctx* c, int n, void* A, long lda, long sA, void* w, long sw, void* VR, long ldv, long sV,
+              int* info, int batch) {
+  return guarded(c, [&] {
+    need(n >= 0 && batch >= 0 && lda >= n && ldv >= n, "bad size");
+    need(A && w && VR && info, "null pointer");
+    zgeev_batched(c->stream, n, (cd*)A, lda, sA, (cd*)w, sw, (cd*)VR, ldv, sV, info, batch, c->ws);
+  });
+}
+
+}  // extern "C"
+
+// ============================================================ path-level entry points
+#include "vrc_ops.hpp"
+
+struct vrc_sampler {
+  vrc_ctx* ctx;
+  std::unique_ptr<vrc::Sampler> s;
+};
+struct vrc_integrator {
+  vrc_ctx* ctx;
+  int kind;
+  std::unique_ptr<vrc::Integrator> it;
+};
+
+namespace {
+HostLayer to_host(const vrc_layer* l) {
+  if (!l || !l->x || !l->y || !l->eps) throw ArgFailure("null layer field");
+  if (l->ncoord_x < 2 || l->ncoord_y < 2) throw ArgFailure("layer needs >= 2 coordinates per axis");
+  HostLayer h;
+  h.x.assign(l->x, l->x + l->ncoord_x);
+  h.y.assign(l->y, l->y + l->ncoord_y);
+  const size_t ne = (size_t)(l->ncoord_x - 1) * (l->ncoord_y - 1);
+  h.eps.resize(ne);
+  for (size_t i = 0; i < ne; ++i) h.eps[i] = mk(l->eps[2 * i], l->eps[2 * i + 1]);
+  return h;
+}
+std::vector<HostLayer> to_host(const vrc_layer* l, int n) {
+  if (n < 0 || (n > 0 && !l)) throw ArgFailure("bad layer array");
+  std::vector<HostLayer> v;
+  v.reserve(n);
+  for (int i = 0; i < n; ++i) v.push_back(to_host(l + i));
+  return v;
+}
+void check_harmonics(int nx, int ny) {
+  need(nx >= 1 && ny >= 1 && (nx & 1) && (ny & 1), "nx, ny must be odd and positive");
+  need(nx <= 63 && ny <= 63, "nx, ny above 63 not supported");
+}
+}  // namespace
+
+extern "C" {
+
+int vrc_assemble_toeplitz(vrc_ctx* c, int nx, int ny, int nlayers, const vrc_layer* layers, void* fe11, void* fe22,
+                          void* fe33, long stride) {
+  return guarded(c, [&] {
+    check_harmonics(nx, ny);
+    need(nlayers >= 0 && fe11 && fe22 && fe33, "bad arguments");
+    const long n = (long)nx * ny;
+    need(stride >= n * n || nlayers <= 1, "stride smaller than n*n");
+    std::vector<HostLayer> hl = to_host(layers, nlayers);
+    AssemblyBatch a;
+    a.nx = nx;
+    a.ny = ny;
+    a.nlayers = nlayers;
+    for (auto& l : hl) a.layers.push_back(&l);
+    if (nlayers == 0) return;
+    assemble_fe(*c, a, (cd*)fe11, (cd*)fe22, (cd*)fe33, stride);
+  });
+}
+
+int vrc_assemble_pq(vrc_ctx* c, int nx, int ny, int nlayers, const vrc_layer* layers, double lambda, int enable_pml,
+                    const vrc_layer* region, void* P, void* Q, long stride) {
+  return guarded(c, [&] {
+    check_harmonics(nx, ny);
+    need(nlayers >= 0 && P && Q, "bad arguments");
+    const long N = 2L * nx * ny;
+    need(stride == N * N, "P/Q stride must be N*N");
+    std::vector<HostLayer> hl = to_host(layers, nlayers);
+    VarSampler s(c, lambda, nx, ny, to_host(region), enable_pml != 0);
+    s.sample_layers(hl, (cd*)P, (cd*)Q);
+  });
+}
+
+int vrc_eig_pq(vrc_ctx* c, int N, int batch, const void* P, const void* Q, long stride, void* W, void* V, void* gam,
+               int* info) {
+  return guarded(c, [&] {
+    need(N > 0 && batch >= 0 && P && Q && W && V && gam && info, "bad arguments");
+    need(stride == (long)N * N, "stride must be N*N");
+    fill_int(c->stream, info, 0, batch);
+    wave_eqn(*c, N, (const cd*)P, (const cd*)Q, stride, (cd*)W, (cd*)V, (cd*)gam, info, batch);
+  });
+}
+
+int vrc_layer_smatrix(vrc_ctx* c, int N, int batch, const double* lam, const double* dz, const void* W, const void* V,
+                      const void* gam, const void* W0, const void* V0, long ref_stride, void* Tuu, void* Rud) {
+  return guarded(c, [&] {
+    need(N > 0 && batch >= 0 && lam && dz && W && V && gam && W0 && V0 && Tuu && Rud, "bad arguments");
+    std::vector<double> s(batch);
+    for (int b = 0; b < batch; ++b) {
+      need(lam[b] > 0, "wavelength must be positive");
+      s[b] = dz[b] * lam[b] / (2.0 * 3.1415926535897932);
+    }
+    DevBuf ds = c->ws.get(sizeof(double) * (batch > 0 ? batch : 1));
+    VRC_HIP(hipMemcpyAsync(ds.ptr, s.data(), sizeof(double) * batch, hipMemcpyHostToDevice, c->stream));
+    VRC_HIP(hipStreamSynchronize(c->stream));
+    layer_smatrix(*c, N, batch, ds.as<double>(), (const cd*)W, (const cd*)V, (const cd*)gam, (const cd*)W0,
+                  (const cd*)V0, ref_stride, (cd*)Tuu, (cd*)Rud);
+  });
+}
+
+int vrc_star_product(vrc_ctx* c, int N, int batch, void* Tdd, void* Tuu, void* Rud, void* Rdu, const void* tdd,
+                     const void* tuu, const void* rud, const void* rdu, long ss) {
+  return guarded(c, [&] {
+    need(N > 0 && batch >= 0 && Tdd && Tuu && Rud && Rdu && tdd && tuu && rud && rdu, "bad arguments");
+    SMat S;
+    S.Tdd = (cd*)Tdd;
+    S.Tuu = (cd*)Tuu;
+    S.Rud = (cd*)Rud;
+    S.Rdu = (cd*)Rdu;
+    star_product(*c, N, batch, S, (const cd*)tdd, (const cd*)tuu, (const cd*)rud, (const cd*)rdu, ss);
+  });
+}
+
+int vrc_project_left(vrc_ctx* c, int N, int batch, void* Tdd, void* Tuu, void* Rud, void* Rdu, const void* W1,
+                     const void* V1, const void* W0, const void* V0, long ref_stride) {
+  return guarded(c, [&] {
+    need(N > 0 && batch >= 0 && Tdd && Tuu && Rud && Rdu && W1 && V1 && W0 && V0, "bad arguments");
+    const long NN = (long)N * N;
+    SMat S;
+    S.Tdd = (cd*)Tdd;
+    S.Tuu = (cd*)Tuu;
+    S.Rud = (cd*)Rud;
+    S.Rdu = (cd*)Rdu;
+    DevBuf LW = c->ws.get(sizeof(cd) * NN * batch), LV = c->ws.get(sizeof(cd) * NN * batch);
+    DevBuf pW = c->ws.get(sizeof(int) * (size_t)N * batch), pV = c->ws.get(sizeof(int) * (size_t)N * batch);
+    DevBuf inf = c->ws.get(sizeof(int) * (size_t)batch);
+    fill_int(c->stream, inf.as<int>(), 0, batch);
+    zgeam(c->stream, N, N, mk(1, 0), (const cd*)W1, N, ref_stride, mk(0, 0), nullptr, 0, 0, LW.as<cd>(), N, NN,
+          batch);
+    zgeam(c->stream, N, N, mk(1, 0), (const cd*)V1, N, ref_stride, mk(0, 0), nullptr, 0, 0, LV.as<cd>(), N, NN,
+          batch);
+    zgetrf(c->stream, N, LW.as<cd>(), N, NN, pW.as<int>(), N, inf.as<int>(), batch);
+    zgetrf(c->stream, N, LV.as<cd>(), N, NN, pV.as<int>(), N, inf.as<int>(), batch);
+    LUPair lu;
+    lu.W = LW.as<cd>();
+    lu.V = LV.as<cd>();
+    lu.pW = pW.as<int>();
+    lu.pV = pV.as<int>();
+    lu.sM = NN;
+    project_left(*c, N, batch, S, lu, (const cd*)W0, (const cd*)V0, ref_stride);
+  });
+}
+
+int vrc_redheffer_segment(vrc_ctx* c, int N, double lambda, double z0, double z1, const void* Pr, const void* Qr,
+                          const void* W, const void* V, const void* gam, const void* P0, const void* Pm,
+                          const void* P1, const void* Q0, const void* Qm, const void* Q1, void* tdd, void* tuu,
+                          void* rud, void* rdu, double* err) {
+  return guarded(c, [&] {
+    need(N > 0 && lambda > 0 && Pr && Qr && W && V && gam && P0 && Pm && P1 && Q0 && Qm && Q1, "bad arguments");
+    need(tdd && tuu && rud && rdu && err, "bad arguments");
+    const long NN = (long)N * N;
+    DevBuf LW = c->ws.get(sizeof(cd) * NN), LV = c->ws.get(sizeof(cd) * NN);
+    DevBuf pW = c->ws.get(sizeof(int) * N), pV = c->ws.get(sizeof(int) * N), inf = c->ws.get(sizeof(int));
+    fill_int(c->stream, inf.as<int>(), 0, 1);
+    zgeam(c->stream, N, N, mk(1, 0), (const cd*)W, N, 0, mk(0, 0), nullptr, 0, 0, LW.as<cd>(), N, 0, 1);
+    zgeam(c->stream, N, N, mk(1, 0), (const cd*)V, N, 0, mk(0, 0), nullptr, 0, 0, LV.as<cd>(), N, 0, 1);
+    zgetrf(c->stream, N, LW.as<cd>(), N, 0, pW.as<int>(), N, inf.as<int>(), 1);
+    zgetrf(c->stream, N, LV.as<cd>(), N, 0, pV.as<int>(), N, inf.as<int>(), 1);
+    RefSys R;
+    R.W = (const cd*)W;
+    R.V = (const cd*)V;
+    R.gam = (const cd*)gam;
+    R.LU.W = LW.as<cd>();
+    R.LU.V = LV.as<cd>();
+    R.LU.pW = pW.as<int>();
+    R.LU.pV = pV.as<int>();
+    R.LU.sM = NN;
+    const cd* P3[3] = {(const cd*)P0, (const cd*)Pm, (const cd*)P1};
+    const cd* Q3[3] = {(const cd*)Q0, (const cd*)Qm, (const cd*)Q1};
+    *err = redheffer_perturbation(*c, N, 2.0 * 3.1415926535897932 / lambda, z0, z1, (const cd*)Pr, (const cd*)Qr, R,
+                                  P3, Q3, (cd*)tdd, (cd*)tuu, (cd*)rud, (cd*)rdu);
+  });
+}
+
+int vrc_var_sampler_create(vrc_ctx* c, double lambda, int nx, int ny, const vrc_layer* base, int enable_pml,
+                           vrc_sampler** out) {
+  return guarded(c, [&] {
+    need(out != nullptr, "null out");
+    *out = nullptr;
+    check_harmonics(nx, ny);
+    auto s = std::make_unique<vrc_sampler>();
+    s->ctx = c;
+    s->s = std::make_unique<VarSampler>(c, lambda, nx, ny, to_host(base), enable_pml != 0);
+    *out = s.release();
+  });
+}
+
+int vrc_sampler_add_rule(vrc_sampler* s, int axis, int layout, double (*rule)(double, void*), void* user) {
+  if (!s) return -1;
+  return guarded(s->ctx, [&] {
+    auto* v = dynamic_cast<VarSampler*>(s->s.get());
+    need(v != nullptr, "rules apply to a VarLayerSampler only");
+    need(axis == 0 || axis == 1, "axis must be 0 (x) or 1 (y)");
+    v->add_rule(axis, layout, rule, user);
+  });
+}
+
+int vrc_stack_sampler_create(vrc_ctx* c, double lambda, int nx, int ny, int nlayers, const vrc_layer* layers,
+                             const double* z_edges, const vrc_layer* bottom, const vrc_layer* top, int enable_pml,
+                             vrc_sampler** out) {
+  return guarded(c, [&] {
+    need(out != nullptr && z_edges != nullptr && nlayers >= 0, "bad arguments");
+    *out = nullptr;
+    check_harmonics(nx, ny);
+    auto s = std::make_unique<vrc_sampler>();
+    s->ctx = c;
+    s->s = std::make_unique<StackSampler>(c, lambda, nx, ny, to_host(layers, nlayers),
+                                          std::vector<double>(z_edges, z_edges + nlayers + 1), to_host(bottom),
+                                          to_host(top), enable_pml != 0);
+    *out = s.release();
+  });
+}
+
+int vrc_sampler_destroy(vrc_sampler* s) {
+  if (!s) return 0;
+  vrc_ctx* c = s->ctx;
+  if (c) (void)hipStreamSynchronize(c->stream);
+  delete s;
+  return 0;
+}
+
+int vrc_sampler_ndim(const vrc_sampler* s) { return s ? s->s->nDim() : -1; }
+
+int vrc_sampler_sample(vrc_sampler* s, double z, double* Ph, double* Qh) {
+  if (!s) return -1;
+  return guarded(s->ctx, [&] {
+    need(Ph && Qh, "null output");
+    const long NN = (long)s->s->nDim() * s->s->nDim();
+    DevBuf P = s->ctx->ws.get(sizeof(cd) * NN), Q = s->ctx->ws.get(sizeof(cd) * NN);
+    s->s->sample_gpu({z}, P.as<cd>(), Q.as<cd>());
+    VRC_HIP(hipMemcpyAsync(Ph, P.ptr, sizeof(cd) * NN, hipMemcpyDeviceToHost, s->ctx->stream));
+    VRC_HIP(hipMemcpyAsync(Qh, Q.ptr, sizeof(cd) * NN, hipMemcpyDeviceToHost, s->ctx->stream));
+    VRC_HIP(hipStreamSynchronize(s->ctx->stream));
+  });
+}
+
+int vrc_sampler_sample_gpu(vrc_sampler* s, double z, void* P, void* Q) {
+  if (!s) return -1;
+  return guarded(s->ctx, [&] {
+    need(P && Q, "null output");
+    s->s->sample_gpu({z}, (cd*)P, (cd*)Q);
+  });
+}
+
+int vrc_sampler_k(vrc_sampler* s, double* Kx, double* Ky) {
+  if (!s) return -1;
+  return guarded(s->ctx, [&] {
+    need(Kx && Ky, "null output");
+    std::memcpy(Kx, s->s->Kx().data(), sizeof(cd) * s->s->Kx().size());
+    std::memcpy(Ky, s->s->Ky().data(), sizeof(cd) * s->s->Ky().size());
+  });
+}
+
+int vrc_integrator_create(vrc_ctx* c, int kind, double z0, double z1, vrc_sampler* s, vrc_integrator** out) {
+  return guarded(c, [&] {
+    need(out && s && s->ctx == c, "bad arguments");
+    need(kind >= 0 && kind <= 2, "kind must be 0, 1 or 2");
+    *out = nullptr;
+    auto it = std::make_unique<vrc_integrator>();
+    it->ctx = c;
+    it->kind = kind;
+    it->it = std::make_unique<Integrator>(c, z0, z1, s->s.get());
+    *out = it.release();
+  });
+}
+
+int vrc_integrator_destroy(vrc_integrator* it) {
+  if (!it) return 0;
+  if (it->ctx) (void)hipStreamSynchronize(it->ctx->stream);
+  delete it;
+  return 0;
+}
+
+static std::vector<std::pair<double, double>> uniform_slices(double z0, double z1, int N) {
+  std::vector<std::pair<double, double>> v;
+  const double dz = (z1 - z0) / (double)N;
+  for (int i = 0; i < N; ++i) v.push_back({z0 + (i + 0.5) * dz, dz});
+  return v;
+}
+
+int vrc_rcwa_compute(vrc_integrator* it, int N) {
+  if (!it) return -1;
+  return guarded(it->ctx, [&] {
+    need(it->kind == 0 && N >= 1, "RCWA integrator with N >= 1 expected");
+    it->it->rcwa(uniform_slices(it->it->it_z0(), it->it->it_z1(), N), false);
+  });
+}
+
+int vrc_rcwa_compute_batched(vrc_integrator* it, int N) {
+  if (!it) return -1;
+  return guarded(it->ctx, [&] {
+    need(it->kind == 0 && N >= 1, "RCWA integrator with N >= 1 expected");
+    it->it->rcwa(uniform_slices(it->it->it_z0(), it->it->it_z1(), N), true);
+  });
+}
+
+int vrc_rcwa_compute_edges(vrc_integrator* it, int ne, const double* ze) {
+  if (!it) return -1;
+  return guarded(it->ctx, [&] {
+    need(it->kind == 0 && ne >= 2 && ze, "RCWA integrator with >= 2 edges expected");
+    std::vector<std::pair<double, double>> v;
+    for (int i = 0; i + 1 < ne; ++i) {
+      need(ze[i + 1] > ze[i], "edges must increase");
+      v.push_back({0.5 * (ze[i] + ze[i + 1]), ze[i + 1] - ze[i]});
+    }
+    it->it->rcwa(v, true);
+  });
+}
+
+int vrc_redheffer_compute(vrc_integrator* it, int p, int maxN, double pe) {
+  if (!it) return -1;
+  (void)p;
+  (void)maxN;
+  return guarded(it->ctx, [&] {
+    need(it->kind == 1 && pe > 0, "Redheffer integrator with pe > 0 expected");
+    it->it->redheffer(pe, 40);
+  });
+}
+
+int vrc_differential_compute(vrc_integrator* it, int N) {
+  if (!it) return -1;
+  return guarded(it->ctx, [&] {
+    need(it->kind == 2 && N >= 1, "differential integrator with N >= 1 expected");
+    it->it->differential(N);
+  });
+}
+
+int vrc_integrator_get(vrc_integrator* it, int which, double* host) {
+  if (!it) return -1;
+  return guarded(it->ctx, [&] {
+    need(host != nullptr, "null output");
+    it->it->get(which, (cd*)host);
+  });
+}
+
+int vrc_integrator_counts(vrc_integrator* it, int* N, int* nEig) {
+  if (!it) return -1;
+  if (N) *N = it->it->N_;
+  if (nEig) *nEig = it->it->nEig_;
+  return 0;
+}
+
+int vrc_integrator_tree(vrc_integrator* it, int cap, double* out) {
+  if (!it) return -1;
+  const auto& t = it->it->tree_;
+  for (int i = 0; i < (int)t.size() && i < cap && out; ++i) {
+    out[2 * i] = t[i].first;
+    out[2 * i + 1] = t[i].second;
+  }
+  return (int)t.size();
+}
+
+int vrc_batch_stack_rcwa(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
+                         void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1) {
+  return guarded(c, [&] {
+    check_harmonics(nx, ny);
+    need(ninst >= 0 && (ninst == 0 || inst) && S_out, "bad arguments");
+    std::vector<std::vector<HostLayer>> keep(ninst);
+    std::vector<HostLayer> bottoms, tops;
+    bottoms.reserve(ninst);
+    tops.reserve(ninst);
+    std::vector<StackInstance> si(ninst);
+    for (int i = 0; i < ninst; ++i) {
+      const vrc_stack_instance& I = inst[i];
+      need(I.nlayers >= 1 && I.thickness, "instance needs >= 1 layer and thicknesses");
+      keep[i] = to_host(I.layers, I.nlayers);
+      bottoms.push_back(to_host(I.bottom));
+      tops.push_back(to_host(I.top));
+    }
+    for (int i = 0; i < ninst; ++i) {
+      si[i].lambda = inst[i].lambda;
+      for (auto& l : keep[i]) si[i].layers.push_back(&l);
+      si[i].thickness.assign(inst[i].thickness, inst[i].thickness + inst[i].nlayers);
+      si[i].bottom = &bottoms[i];
+      si[i].top = &tops[i];
+    }
+    batch_stack_rcwa(*c, nx, ny, si, chunk, (cd*)S_out, (cd*)W0, (cd*)V0, (cd*)g0, (cd*)W1, (cd*)V1, (cd*)g1);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,894 @@
+// Batched complex128 non-Hermitian eigensolver for gfx950 (replaces the reference's
+// LAPACKE_zgeev / magma_zgeev call, src/rcwa/MKLEigenSolver.hpp:45-55,
+// src/gpu/RedhefferIntegratorGPU.cpp:216-229).
+//
+// Pipeline for a strided batch of n x n matrices:
+//  1. Hessenberg reduction H = Q^H A Q: blocked Householder, panel of HB columns
+//     reduced by one workgroup per matrix (the trailing-matrix product A*v is the
+//     only memory-bound part), trailing update as three MFMA ZGEMMs.
+//  2. Q formed explicitly from the stored reflectors (backward, blocked, ZGEMM).
+//  3. Schur form T = Z^H H Z by a multishift QR sweep with tightly packed 3x3 bulges:
+//     one workgroup per matrix; the bulge chain is chased inside a <=64x64 diagonal
+//     window held in LDS, the window transform U is accumulated in LDS and applied
+//     to the off-window rows/columns of H and to Z with in-kernel MFMA products.
+//     Shifts are eigenvalues of the trailing block; blocks of size <= QR_NMIN are
+//     finished by single-shift QR inside LDS.
+//  4. Eigenvectors of T by a tiled back-substitution (64x64 tiles; one workgroup per
+//     block column), then VR = Z * X (ZGEMM) and LAPACK zgeev normalisation.
+#include "vrc_common.hpp"
+#include "vrc_internal.hpp"
+
+namespace vrc {
+
+constexpr double ULP = 2.220446049250313e-16;      // 2^-52
+constexpr double SAFMIN = 2.2250738585072014e-308;
+constexpr int HB = 32;         // Hessenberg panel width
+constexpr int QW = 64;         // maximal QR window
+constexpr int NBUL = 10;       // bulges per sweep (2 shifts each)
+constexpr int QR_NMIN = 48;    // active blocks this small are finished in LDS
+constexpr int EXC_SWEEPS = 6;  // sweeps without deflation before exceptional shifts
+
+__device__ inline double wsum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Householder reflector G = I - tau v v^H, v = (1, x*scale), with G^H (alpha; x) = (beta; 0).
+__device__ inline void house(cd alpha, double xnorm, double& beta, cd& tau, cd& scale) {
+  if (xnorm == 0.0 && alpha.y == 0.0) {
+    beta = alpha.x;
+    tau = mk(0.0, 0.0);
+    scale = mk(0.0, 0.0);
+    return;
+  }
+  double nrm = hypot(hypot(alpha.x, alpha.y), xnorm);
+  beta = alpha.x >= 0.0 ? -nrm : nrm;
+  tau = mk((beta - alpha.x) / beta, -alpha.y / beta);
+  scale = crecip(mk(alpha.x - beta, alpha.y));
+}
+
+// ===================================================================== Hessenberg
+// out[i] = sum_{r>=r0} conj(Vf[r, p+i]) * x[r] for i < j, x in LDS.
+__device__ void panel_multi_dot(int N, int r0, int p, int j, const cd* Vf, const cd* x, cd (*red)[HB], cd* out) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int i = 0; i < j; ++i) {
+    const cd* vc = Vf + (long)(p + i) * N;
+    double sr = 0.0, si = 0.0;
+    for (int r = r0 + t; r < N; r += 256) {
+      cd a = vc[r], xb = x[r];
+      sr += a.x * xb.x + a.y * xb.y;
+      si += a.x * xb.y - a.y * xb.x;
+    }
+    sr = wsum(sr);
+    si = wsum(si);
+    if (lane == 0) red[wave][i] = mk(sr, si);
+  }
+  __syncthreads();
+  if (t < j) out[t] = cadd(cadd(red[0][t], red[1][t]), cadd(red[2][t], red[3][t]));
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void gehrd_panel_kernel(int N, int p, int nb, cd* A, long lda, long sA, cd* Vf,
+                                                          long sV, cd* Y, long sY, cd* Tp, long sT, cd* tau,
+                                                          long sTau) {
+  const long b = blockIdx.x;
+  A += b * sA;
+  Vf += b * sV;
+  Y += b * sY;
+  Tp += b * sT;
+  tau += b * sTau;
+  extern __shared__ cd dyn[];
+  cd* bvec = dyn;
+  cd* vvec = dyn + N;
+  __shared__ cd Tl[HB][HB + 1];
+  __shared__ cd red[4][HB];
+  __shared__ cd vec1[HB], vec2[HB];
+  __shared__ double rn[4];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+
+  for (int j = 0; j < nb; ++j) {
+    const int c = p + j;
+    if (t < j) vec1[t] = cconj(Vf[c + (long)(p + t) * N]);
+    __syncthreads();
+    // column c of A0*Q_j
+    for (int r = t; r < N; r += 256) {
+      cd acc = A[r + (long)c * lda];
+      for (int i = 0; i < j; ++i) acc = csub(acc, cmul(Y[r + (long)i * N], vec1[i]));
+      bvec[r] = acc;
+    }
+    __syncthreads();
+    if (j > 0) {
+      // left transform by Q_j^H on rows >= p+1
+      panel_multi_dot(N, p + 1, p, j, Vf, bvec, red, vec2);
+      if (t < j) {
+        cd s = mk(0.0, 0.0);
+        for (int l = 0; l <= t; ++l) s = cadd(s, cmul(cconj(Tl[l][t]), vec2[l]));
+        vec1[t] = s;
+      }
+      __syncthreads();
+      for (int r = p + 1 + t; r < N; r += 256) {
+        cd acc = bvec[r];
+        for (int i = 0; i < j; ++i) acc = csub(acc, cmul(Vf[r + (long)(p + i) * N], vec1[i]));
+        bvec[r] = acc;
+      }
+      __syncthreads();
+    }
+    // final entries of column c and the reflector from rows c+1..N-1
+    double part = 0.0;
+    for (int r = t; r < N; r += 256) {
+      if (r <= c) A[r + (long)c * lda] = bvec[r];
+      if (r >= c + 2) part += cnorm2(bvec[r]);
+    }
+    part = wsum(part);
+    if (lane == 0) rn[wave] = part;
+    __syncthreads();
+    const double xnorm = sqrt(rn[0] + rn[1] + rn[2] + rn[3]);
+    double beta;
+    cd tv, scale;
+    house(bvec[c + 1], xnorm, beta, tv, scale);
+    const bool ident = (tv.x == 0.0 && tv.y == 0.0);
+    for (int r = t; r < N; r += 256) {
+      cd v;
+      if (r <= c) v = mk(0.0, 0.0);
+      else if (r == c + 1) v = mk(1.0, 0.0);
+      else v = ident ? mk(0.0, 0.0) : cmul(bvec[r], scale);
+      vvec[r] = v;
+      Vf[r + (long)c * N] = v;
+      if (r == c + 1) A[r + (long)c * lda] = ident ? bvec[r] : mk(beta, 0.0);
+      else if (r > c + 1) A[r + (long)c * lda] = mk(0.0, 0.0);
+    }
+    if (t == 0) tau[c] = tv;
+    __syncthreads();
+    // z = V_j^H v  (rows >= c+1)
+    if (j > 0) panel_multi_dot(N, c + 1, p, j, Vf, vvec, red, vec2);
+    // y_j = tau*(A0[:, c+1:] v - Y_j z)
+    for (int r = t; r < N; r += 256) {
+      double gr = 0.0, gi = 0.0;
+      const cd* ar = A + r;
+      for (int col = c + 1; col < N; ++col) {
+        cd a = ar[(long)col * lda], v = vvec[col];
+        gr += a.x * v.x - a.y * v.y;
+        gi += a.x * v.y + a.y * v.x;
+      }
+      cd acc = mk(gr, gi);
+      for (int i = 0; i < j; ++i) acc = csub(acc, cmul(Y[r + (long)i * N], vec2[i]));
+      Y[r + (long)j * N] = cmul(tv, acc);
+    }
+    if (t < j) {
+      cd s = mk(0.0, 0.0);
+      for (int l = t; l < j; ++l) s = cadd(s, cmul(Tl[t][l], vec2[l]));
+      Tl[t][j] = cneg(cmul(tv, s));
+    }
+    if (t == 0) Tl[j][j] = tv;
+    __syncthreads();
+  }
+  for (int e = t; e < nb * nb; e += 256) {
+    int i = e % nb, jj = e / nb;
+    Tp[i + (long)jj * HB] = (i <= jj) ? Tl[i][jj] : mk(0.0, 0.0);
+  }
+}
+
+static int n_reflectors(int n) { return n >= 3 ? n - 2 : 0; }
+
+// A <- H (Hessenberg, zeros below the subdiagonal). Vf (n x n) receives the
+// reflector vectors by column, Tall (npanel x HB x HB) the panel T factors.
+static void gehrd(hipStream_t s, int n, cd* A, long lda, long sA, cd* Vf, cd* Tall, int batch, Workspace& ws) {
+  const int nref = n_reflectors(n);
+  if (nref == 0) return;
+  const int npanel = cdiv_up(nref, HB);
+  DevBuf Yb = ws.get(sizeof(cd) * (size_t)n * HB * batch);
+  DevBuf taub = ws.get(sizeof(cd) * (size_t)n * batch);
+  DevBuf Wb = ws.get(sizeof(cd) * (size_t)HB * n * batch);
+  DevBuf W2b = ws.get(sizeof(cd) * (size_t)HB * n * batch);
+  cd* Y = Yb.as<cd>();
+  cd* W = Wb.as<cd>();
+  cd* W2 = W2b.as<cd>();
+  const long sV = (long)n * n, sY = (long)n * HB, sT = (long)npanel * HB * HB, sW = (long)HB * n;
+  const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0), zero = mk(0.0, 0.0);
+  for (int p = 0; p < nref; p += HB) {
+    const int nb = nref - p < HB ? nref - p : HB;
+    cd* Tp = Tall + (long)(p / HB) * HB * HB;
+    gehrd_panel_kernel<<<batch, 256, 2 * sizeof(cd) * n, s>>>(n, p, nb, A, lda, sA, Vf, sV, Y, sY, Tp, sT,
+                                                                taub.as<cd>(), n);
+    launch_check("gehrd_panel");
+    const int c0 = p + nb, ncols = n - c0;
+    if (ncols <= 0) continue;
+    zgemm_strided(s, OP_N, OP_C, n, ncols, nb, mone, Y, n, sY, Vf + c0 + (long)p * n, n, sV, one,
+                  A + (long)c0 * lda, lda, sA, batch);
+    zgemm_strided(s, OP_C, OP_N, nb, ncols, n - p - 1, one, Vf + (p + 1) + (long)p * n, n, sV,
+                  A + (p + 1) + (long)c0 * lda, lda, sA, zero, W, HB, sW, batch);
+    zgemm_strided(s, OP_C, OP_N, nb, ncols, nb, one, Tp, HB, sT, W, HB, sW, zero, W2, HB, sW, batch);
+    zgemm_strided(s, OP_N, OP_N, n - p - 1, ncols, nb, mone, Vf + (p + 1) + (long)p * n, n, sV, W2, HB, sW, one,
+                  A + (p + 1) + (long)c0 * lda, lda, sA, batch);
+  }
+}
+
+// Q = H_0 H_1 ... H_{nref-1} into Z (n x n).
+static void orghr(hipStream_t s, int n, const cd* Vf, const cd* Tall, cd* Z, long ldz, long sZ, int batch,
+                  Workspace& ws) {
+  zset(s, n, n, mk(1.0, 0.0), mk(0.0, 0.0), Z, ldz, sZ, batch);
+  const int nref = n_reflectors(n);
+  if (nref == 0) return;
+  const int npanel = cdiv_up(nref, HB);
+  DevBuf Wb = ws.get(sizeof(cd) * (size_t)HB * n * batch);
+  DevBuf W2b = ws.get(sizeof(cd) * (size_t)HB * n * batch);
+  const long sV = (long)n * n, sT = (long)npanel * HB * HB, sW = (long)HB * n;
+  const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0), zero = mk(0.0, 0.0);
+  for (int pi = npanel - 1; pi >= 0; --pi) {
+    const int p = pi * HB;
+    const int nb = nref - p < HB ? nref - p : HB;
+    const int m = n - p - 1;
+    const cd* V = Vf + (p + 1) + (long)p * n;
+    cd* Zs = Z + (p + 1) + (long)(p + 1) * ldz;
+    zgemm_strided(s, OP_C, OP_N, nb, m, m, one, V, n, sV, Zs, ldz, sZ, zero, Wb.as<cd>(), HB, sW, batch);
+    zgemm_strided(s, OP_N, OP_N, nb, m, nb, one, Tall + (long)pi * HB * HB, HB, sT, Wb.as<cd>(), HB, sW, zero,
+                  W2b.as<cd>(), HB, sW, batch);
+    zgemm_strided(s, OP_N, OP_N, m, m, nb, mone, V, n, sV, W2b.as<cd>(), HB, sW, one, Zs, ldz, sZ, batch);
+  }
+}
+
+// ===================================================================== LDS matrix products
+typedef cd LdsMat[QW][QW + 1];
+
+// acc = op(A) * B over K = 64 for the 64x64 output tile; CT selects op(A)(i,k) = conj(A[k][i]).
+template <bool CT>
+__device__ __forceinline__ void mm64(const LdsMat& As, const LdsMat& Bs, d4 (&accR)[2][2], d4 (&accI)[2][2]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      accI[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+  for (int s = 0; s < QW / 4; ++s) {
+    const int kr = 4 * s + (lane >> 4);
+    double ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int i = wr * 32 + q * 16 + (lane & 15);
+      cd a = CT ? cconj(As[kr][i]) : As[i][kr];
+      ar[q] = a.x;
+      ai[q] = a.y;
+      nai[q] = -a.y;
+      const int j = wc * 32 + q * 16 + (lane & 15);
+      cd bb = Bs[kr][j];
+      br[q] = bb.x;
+      bi[q] = bb.y;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
+        accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], accR[i][j], 0, 0, 0);
+        accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
+        accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+      }
+  }
+}
+
+// Calls f(i, j, value) for every output element owned by this thread.
+template <class F>
+__device__ __forceinline__ void mm64_for_each(const d4 (&accR)[2][2], const d4 (&accI)[2][2], F f) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ii = wr * 32 + i * 16 + (lane >> 4) + 4 * r;
+        const int jj = wc * 32 + j * 16 + (lane & 15);
+        f(ii, jj, mk(accR[i][j][r], accI[i][j][r]));
+      }
+}
+
+// M[r0:r0+w, c0:c1] <- U^H * M[...]  (U: w x w in LDS, zero padded), chunk buffer C.
+__device__ void apply_left_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, int r0, int c0, int c1) {
+  const int t = threadIdx.x;
+  for (int cc = c0; cc < c1; cc += QW) {
+    const int cw = c1 - cc < QW ? c1 - cc : QW;
+    for (int e = t; e < QW * QW; e += 256) {
+      int k = e & 63, j = e >> 6;
+      C[k][j] = (k < w && j < cw) ? M[(r0 + k) + (long)(cc + j) * ld] : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    d4 aR[2][2], aI[2][2];
+    mm64<true>(U, C, aR, aI);
+    mm64_for_each(aR, aI, [&](int i, int j, cd v) {
+      if (i < w && j < cw) M[(r0 + i) + (long)(cc + j) * ld] = v;
+    });
+    __syncthreads();
+  }
+}
+
+// M[r0:r1, c0:c0+w] <- M[...] * U.
+__device__ void apply_right_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, int r0, int r1, int c0) {
+  const int t = threadIdx.x;
+  for (int rr = r0; rr < r1; rr += QW) {
+    const int rh = r1 - rr < QW ? r1 - rr : QW;
+    for (int e = t; e < QW * QW; e += 256) {
+      int i = e & 63, k = e >> 6;
+      C[i][k] = (i < rh && k < w) ? M[(rr + i) + (long)(c0 + k) * ld] : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    d4 aR[2][2], aI[2][2];
+    mm64<false>(C, U, aR, aI);
+    mm64_for_each(aR, aI, [&](int i, int j, cd v) {
+      if (i < rh && j < w) M[(rr + i) + (long)(c0 + j) * ld] = v;
+    });
+    __syncthreads();
+  }
+}
+
+// ===================================================================== QR iteration
+struct QrLds {
+  LdsMat Hw;
+  LdsMat U;
+  cd shifts[2 * NBUL];
+  cd rv1[NBUL], rv2[NBUL], rtau[NBUL];
+  double rbeta[NBUL];
+  int rsize[NBUL], rrow[NBUL], rcol[NBUL];
+  double redv[256];
+  int redi[256];
+  int ctl[4];
+};
+
+// Negligible-subdiagonal test (Ahues-Tisseur form) on an LDS block, row k of [l..i].
+__device__ inline bool negl_lds(const LdsMat& Hw, int k, int l, int i, double smlnum) {
+  const cd hs = Hw[k][k - 1];
+  const double ahs = cabs1(hs);
+  if (ahs <= smlnum) return true;
+  double tst = cabs1(Hw[k - 1][k - 1]) + cabs1(Hw[k][k]);
+  if (tst == 0.0) {
+    if (k - 2 >= l) tst += cabs1(Hw[k - 1][k - 2]);
+    if (k + 1 <= i) tst += cabs1(Hw[k + 1][k]);
+  }
+  if (ahs <= ULP * tst) {
+    const double a1 = ahs, a2 = cabs1(Hw[k - 1][k]);
+    const double ab = fmax(a1, a2), ba = fmin(a1, a2);
+    const double b1 = cabs1(Hw[k][k]), b2 = cabs1(csub(Hw[k - 1][k - 1], Hw[k][k]));
+    const double aa = fmax(b1, b2), bb = fmin(b1, b2);
+    const double s = aa + ab;
+    if (ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)))) return true;
+  }
+  return false;
+}
+
+// Eigenvalue of the 2x2 block [[a, b], [c, d]] closest to d.
+__device__ inline cd wilkinson(cd a, cd b, cd c, cd d) {
+  cd u = cmul(csqrt_principal(b), csqrt_principal(c));
+  double su = cabs1(u);
+  if (su == 0.0) return d;
+  cd x = cscale(csub(a, d), 0.5);
+  double sx = cabs1(x);
+  double sc = fmax(su, sx);
+  cd xs = cscale(x, 1.0 / sc), us = cscale(u, 1.0 / sc);
+  cd y = cscale(csqrt_principal(cadd(cmul(xs, xs), cmul(us, us))), sc);
+  if (sx > 0.0) {
+    cd xn = cscale(x, 1.0 / sx);
+    if (xn.x * y.x + xn.y * y.y < 0.0) y = cneg(y);
+  }
+  return csub(d, cmul(u, cdiv(u, cadd(x, y))));
+}
+
+// Complete Schur decomposition of the s x s Hessenberg block in L.Hw (s <= QW) by
+// single-shift QR; accumulates the transform into L.U when wantU. Returns 0 on
+// success, else (index of the unconverged eigenvalue + 1). All threads call it.
+__device__ int small_schur(QrLds& L, int s, bool wantU) {
+  const int t = threadIdx.x;
+  const double smlnum = SAFMIN * ((double)s / ULP);
+  const int itmax = 30 * (s > 10 ? s : 10);
+  int i = s - 1;
+  while (i >= 0) {
+    int l = 0;
+    bool conv = false;
+    for (int its = 0; its <= itmax; ++its) {
+      if (t == 0) {
+        int k;
+        for (k = i; k > 0; --k)
+          if (negl_lds(L.Hw, k, 0, i, smlnum)) break;
+        if (k > 0) L.Hw[k][k - 1] = mk(0.0, 0.0);
+        L.ctl[0] = k;
+      }
+      __syncthreads();
+      l = L.ctl[0];
+      __syncthreads();
+      if (l >= i) {
+        conv = true;
+        break;
+      }
+      cd sh;
+      if (its == 10) {
+        sh = cadd(L.Hw[l][l], mk(0.75 * cabs1(L.Hw[l + 1][l]), 0.0));
+      } else if (its == 20) {
+        sh = cadd(L.Hw[i][i], mk(0.75 * cabs1(L.Hw[i][i - 1]), 0.0));
+      } else {
+        sh = wilkinson(L.Hw[i - 1][i - 1], L.Hw[i - 1][i], L.Hw[i][i - 1], L.Hw[i][i]);
+      }
+      for (int k = l; k < i; ++k) {
+        cd x, y;
+        if (k == l) {
+          x = csub(L.Hw[l][l], sh);
+          y = L.Hw[l + 1][l];
+        } else {
+          x = L.Hw[k][k - 1];
+          y = L.Hw[k + 1][k - 1];
+        }
+        double beta;
+        cd tau, scale;
+        house(x, cabsv(y), beta, tau, scale);
+        const cd v1 = cmul(y, scale);
+        const bool ident = (tau.x == 0.0 && tau.y == 0.0);
+        __syncthreads();
+        if (!ident) {
+          if (t < s) {
+            const int j = t;
+            if (k > l && j == k - 1) {
+              L.Hw[k][j] = mk(beta, 0.0);
+              L.Hw[k + 1][j] = mk(0.0, 0.0);
+            } else {
+              cd h1 = L.Hw[k][j], h2 = L.Hw[k + 1][j];
+              cd sm = cmul(cconj(tau), cadd(h1, cmul(cconj(v1), h2)));
+              L.Hw[k][j] = csub(h1, sm);
+              L.Hw[k + 1][j] = csub(h2, cmul(sm, v1));
+            }
+          }
+          __syncthreads();
+          if (t < s) {
+            cd c1 = L.Hw[t][k], c2 = L.Hw[t][k + 1];
+            cd sm = cmul(tau, cadd(c1, cmul(c2, v1)));
+            L.Hw[t][k] = csub(c1, sm);
+            L.Hw[t][k + 1] = csub(c2, cmul(sm, cconj(v1)));
+          } else if (wantU && t >= 64 && t < 64 + s) {
+            const int r = t - 64;
+            cd c1 = L.U[r][k], c2 = L.U[r][k + 1];
+            cd sm = cmul(tau, cadd(c1, cmul(c2, v1)));
+            L.U[r][k] = csub(c1, sm);
+            L.U[r][k + 1] = csub(c2, cmul(sm, cconj(v1)));
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (!conv) return i + 1;
+    i = l - 1;
+  }
+  return 0;
+}
+
+__device__ void lds_load_block(QrLds& L, const cd* H, long ldh, int r0, int w) {
+  for (int e = threadIdx.x; e < QW * QW; e += 256) {
+    int i = e & 63, j = e >> 6;
+    L.Hw[i][j] = (i < w && j < w) ? H[(r0 + i) + (long)(r0 + j) * ldh] : mk(0.0, 0.0);
+    L.U[i][j] = (i == j && i < w) ? mk(1.0, 0.0) : mk(0.0, 0.0);
+  }
+  __syncthreads();
+}
+
+__device__ void lds_store_block(QrLds& L, cd* H, long ldh, int r0, int w) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < w * w; e += 256) {
+    int i = e % w, j = e / w;
+    H[(r0 + i) + (long)(r0 + j) * ldh] = L.Hw[i][j];
+  }
+  __syncthreads();
+}
+
+// Apply the window transform U (on indices ws..ws+w-1) to everything outside the window.
+__device__ void apply_window(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int ws, int w) {
+  const int we = ws + w - 1;
+  if (we + 1 < N) apply_left_u(L.U, L.Hw, w, H, ldh, ws, we + 1, N);
+  if (ws > 0) apply_right_u(L.U, L.Hw, w, H, ldh, 0, ws, ws);
+  apply_right_u(L.U, L.Hw, w, Z, ldz, 0, N, ws);
+}
+
+// Largest k in [1, hi] whose subdiagonal is negligible (sets it to zero); 0 if none.
+__device__ int find_lo(QrLds& L, cd* H, long ldh, int N, int hi) {
+  const int t = threadIdx.x;
+  const double smlnum = SAFMIN * ((double)N / ULP);
+  int best = 0;
+  for (int k = hi - t; k >= 1; k -= 256) {
+    const cd hs = H[k + (long)(k - 1) * ldh];
+    const double ahs = cabs1(hs);
+    bool ng = ahs <= smlnum;
+    if (!ng) {
+      const cd hkk = H[k + (long)k * ldh], hk1 = H[(k - 1) + (long)(k - 1) * ldh];
+      double tst = cabs1(hk1) + cabs1(hkk);
+      if (tst == 0.0) {
+        if (k >= 2) tst += cabs1(H[(k - 1) + (long)(k - 2) * ldh]);
+        if (k + 1 <= hi) tst += cabs1(H[(k + 1) + (long)k * ldh]);
+      }
+      if (ahs <= ULP * tst) {
+        const double a2 = cabs1(H[(k - 1) + (long)k * ldh]);
+        const double ab = fmax(ahs, a2), ba = fmin(ahs, a2);
+        const double b1 = cabs1(hkk), b2 = cabs1(csub(hk1, hkk));
+        const double aa = fmax(b1, b2), bb = fmin(b1, b2);
+        const double s = aa + ab;
+        ng = ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)));
+      }
+    }
+    if (ng) {
+      best = k;
+      break;
+    }
+  }
+  L.redi[t] = best;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w && L.redi[t + w] > L.redi[t]) L.redi[t] = L.redi[t + w];
+    __syncthreads();
+  }
+  const int lo = L.redi[0];
+  __syncthreads();
+  if (t == 0 && lo > 0) H[lo + (long)(lo - 1) * ldh] = mk(0.0, 0.0);
+  __syncthreads();
+  return lo;
+}
+
+// One multishift sweep over the active block [lo, hi] with nbul bulges (shifts in L.shifts).
+__device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int lo, int hi, int nbul) {
+  const int t = threadIdx.x;
+  const int adv = QW - 3 * nbul - 1;
+  const int T_end = (hi - 2) - (lo - 1) + 3 * (nbul - 1) + 1;
+  for (int t0 = 0; t0 < T_end; t0 += adv) {
+    const int t1 = t0 + adv < T_end ? t0 + adv : T_end;
+    int pmin = lo - 1 + t0 - 3 * (nbul - 1);
+    if (pmin < lo - 1) pmin = lo - 1;
+    int pmax = lo - 1 + t1 - 1;
+    if (pmax > hi - 2) pmax = hi - 2;
+    const int ws = pmin > lo ? pmin : lo;
+    const int we = pmax + 4 < hi ? pmax + 4 : hi;
+    const int w = we - ws + 1;
+    lds_load_block(L, H, ldh, ws, w);
+    for (int tt = t0; tt < t1; ++tt) {
+      // reflectors of all bulges at this step, from the current window state
+      if (t < nbul) {
+        const int bb = t;
+        const int p = lo - 1 + tt - 3 * bb;
+        int size = 0;
+        if (p >= lo - 1 && p <= hi - 2) {
+          cd x, y, z = mk(0.0, 0.0);
+          size = (p + 3 <= hi) ? 3 : 2;
+          if (p == lo - 1) {
+            const int a = lo - ws;
+            const cd s1 = L.shifts[2 * bb], s2 = L.shifts[2 * bb + 1];
+            const cd h11 = L.Hw[a][a], h21 = L.Hw[a + 1][a], h12 = L.Hw[a][a + 1];
+            const cd h22 = L.Hw[a + 1][a + 1];
+            const cd h32 = (size == 3) ? L.Hw[a + 2][a + 1] : mk(0.0, 0.0);
+            const double sc = cabs1(csub(h11, s2)) + cabs1(h21);
+            if (sc == 0.0) {
+              x = y = mk(0.0, 0.0);
+            } else {
+              const cd h21s = cscale(h21, 1.0 / sc);
+              x = cadd(cmul(h21s, h12), cmul(csub(h11, s1), cscale(csub(h11, s2), 1.0 / sc)));
+              y = cmul(h21s, csub(cadd(h11, h22), cadd(s1, s2)));
+              z = cmul(h21s, h32);
+            }
+            L.rcol[bb] = -1;
+          } else {
+            const int a = p - ws;
+            x = L.Hw[a + 1][a];
+            y = L.Hw[a + 2][a];
+            if (size == 3) z = L.Hw[a + 3][a];
+            L.rcol[bb] = a;
+          }
+          double beta;
+          cd tau, scale;
+          house(x, sqrt(cnorm2(y) + cnorm2(z)), beta, tau, scale);
+          L.rv1[bb] = cmul(y, scale);
+          L.rv2[bb] = cmul(z, scale);
+          L.rtau[bb] = tau;
+          L.rbeta[bb] = beta;
+          L.rrow[bb] = p + 1 - ws;
+          if (tau.x == 0.0 && tau.y == 0.0) size = 0;
+        }
+        L.rsize[bb] = size;
+      }
+      __syncthreads();
+      if (t < w) {
+        const int j = t;
+        for (int bb = 0; bb < nbul; ++bb) {
+          const int sz = L.rsize[bb];
+          if (sz == 0) continue;
+          const int r1 = L.rrow[bb];
+          if (j == L.rcol[bb]) {
+            L.Hw[r1][j] = mk(L.rbeta[bb], 0.0);
+            L.Hw[r1 + 1][j] = mk(0.0, 0.0);
+            if (sz == 3) L.Hw[r1 + 2][j] = mk(0.0, 0.0);
+            continue;
+          }
+          const cd v1 = L.rv1[bb], v2 = L.rv2[bb], ct = cconj(L.rtau[bb]);
+          cd h1 = L.Hw[r1][j], h2 = L.Hw[r1 + 1][j];
+          cd h3 = sz == 3 ? L.Hw[r1 + 2][j] : mk(0.0, 0.0);
+          cd sm = cadd(h1, cmul(cconj(v1), h2));
+          if (sz == 3) sm = cadd(sm, cmul(cconj(v2), h3));
+          sm = cmul(ct, sm);
+          L.Hw[r1][j] = csub(h1, sm);
+          L.Hw[r1 + 1][j] = csub(h2, cmul(sm, v1));
+          if (sz == 3) L.Hw[r1 + 2][j] = csub(h3, cmul(sm, v2));
+        }
+      }
+      __syncthreads();
+      {
+        const bool onH = t < w;
+        const bool onU = (t >= 64 && t < 64 + w);
+        if (onH || onU) {
+          const int i = onH ? t : t - 64;
+          LdsMat& M = onH ? L.Hw : L.U;
+          for (int bb = 0; bb < nbul; ++bb) {
+            const int sz = L.rsize[bb];
+            if (sz == 0) continue;
+            const int r1 = L.rrow[bb];
+            const cd v1 = L.rv1[bb], v2 = L.rv2[bb], tu = L.rtau[bb];
+            cd c1 = M[i][r1], c2 = M[i][r1 + 1];
+            cd c3 = sz == 3 ? M[i][r1 + 2] : mk(0.0, 0.0);
+            cd sm = cadd(c1, cmul(c2, v1));
+            if (sz == 3) sm = cadd(sm, cmul(c3, v2));
+            sm = cmul(tu, sm);
+            M[i][r1] = csub(c1, sm);
+            M[i][r1 + 1] = csub(c2, cmul(sm, cconj(v1)));
+            if (sz == 3) M[i][r1 + 2] = csub(c3, cmul(sm, cconj(v2)));
+          }
+        }
+      }
+      __syncthreads();
+    }
+    lds_store_block(L, H, ldh, ws, w);
+    apply_window(L, N, H, ldh, Z, ldz, ws, w);
+  }
+}
+
+__global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long sH, cd* Z, long ldz, long sZ,
+                                                  cd* wout, long sw, int* info) {
+  __shared__ QrLds L;
+  const long b = blockIdx.x;
+  H += b * sH;
+  Z += b * sZ;
+  wout += b * sw;
+  const int t = threadIdx.x;
+  int ihi = N - 1;
+  int sweeps = 0, last_progress = 0, prev_lo = -1, prev_hi = N;
+  const int itmax = 30 * (N > 10 ? N : 10);
+  int fail = 0;
+  while (ihi >= 0) {
+    const int lo = find_lo(L, H, ldh, N, ihi);
+    const int m = ihi - lo + 1;
+    if (lo != prev_lo || ihi != prev_hi) last_progress = sweeps;
+    prev_lo = lo;
+    prev_hi = ihi;
+    if (m <= QR_NMIN) {
+      lds_load_block(L, H, ldh, lo, m);
+      const int r = small_schur(L, m, true);
+      lds_store_block(L, H, ldh, lo, m);
+      if (r != 0) {
+        fail = lo + r;
+        break;
+      }
+      apply_window(L, N, H, ldh, Z, ldz, lo, m);
+      ihi = lo - 1;
+      continue;
+    }
+    if (sweeps >= itmax) {
+      fail = ihi + 1;
+      break;
+    }
+    int nbul = (m - 2) / 6;
+    if (nbul > NBUL) nbul = NBUL;
+    if (nbul < 1) nbul = 1;
+    const int ns = 2 * nbul;
+    const bool exceptional = (sweeps - last_progress) > 0 && ((sweeps - last_progress) % EXC_SWEEPS) == 0;
+    if (!exceptional) {
+      const int r0 = ihi - ns + 1;
+      for (int e = t; e < QW * QW; e += 256) {
+        int i = e & 63, j = e >> 6;
+        L.Hw[i][j] = (i < ns && j < ns) ? H[(r0 + i) + (long)(r0 + j) * ldh] : mk(0.0, 0.0);
+      }
+      __syncthreads();
+      if (t == 0) {
+        // the block is a trailing principal submatrix: clear its (irrelevant) left neighbour
+      }
+      small_schur(L, ns, false);
+      if (t < ns) L.shifts[t] = L.Hw[t][t];
+      __syncthreads();
+    } else {
+      if (t < ns) {
+        const int k = ihi - t;
+        cd hk = H[k + (long)k * ldh];
+        double sub = cabs1(H[k + (long)(k - 1) * ldh]);
+        L.shifts[t] = cadd(hk, mk(0.75 * sub, 0.0));
+      }
+      __syncthreads();
+    }
+    qr_sweep(L, N, H, ldh, Z, ldz, lo, ihi, nbul);
+    ++sweeps;
+  }
+  __syncthreads();
+  for (int i = t; i < N; i += 256) wout[i] = H[i + (long)i * ldh];
+  if (t == 0) info[b] = fail;
+}
+
+// ===================================================================== eigenvectors
+// X (n x n, pre-zeroed) <- upper-triangular eigenvector matrix of upper-triangular T:
+// (T - T_kk I) x_k = 0, x_k(k) = 1, tiny pivots replaced by smin (LAPACK ztrevc rule).
+__global__ __launch_bounds__(256) void trevc_kernel(int N, const cd* T, long ldt, long sT, cd* X, long ldx,
+                                                    long sX) {
+  __shared__ LdsMat Ts;
+  __shared__ LdsMat Xs;
+  __shared__ cd lam[QW];
+  __shared__ double smin[QW];
+  const long b = blockIdx.y;
+  T += b * sT;
+  X += b * sX;
+  const int K = blockIdx.x;
+  const int k0 = K * QW;
+  const int bk = N - k0 < QW ? N - k0 : QW;
+  const int t = threadIdx.x;
+  for (int e = t; e < QW * QW; e += 256) {
+    int i = e & 63, j = e >> 6;
+    Ts[i][j] = (i < bk && j < bk) ? T[(k0 + i) + (long)(k0 + j) * ldt] : mk(0.0, 0.0);
+  }
+  __syncthreads();
+  if (t < bk) {
+    lam[t] = Ts[t][t];
+    smin[t] = fmax(ULP * cabs1(lam[t]), SAFMIN * ((double)N / ULP));
+  }
+  __syncthreads();
+  // diagonal tile: thread c solves its own column
+  if (t < QW) {
+    const int c = t;
+    for (int i = 0; i < QW; ++i) Xs[i][c] = mk(0.0, 0.0);
+    if (c < bk) {
+      Xs[c][c] = mk(1.0, 0.0);
+      for (int i = c - 1; i >= 0; --i) {
+        cd s = mk(0.0, 0.0);
+        for (int j = i + 1; j <= c; ++j) s = cadd(s, cmul(Ts[i][j], Xs[j][c]));
+        cd d = csub(Ts[i][i], lam[c]);
+        if (cabs1(d) < smin[c]) d = mk(smin[c], 0.0);
+        Xs[i][c] = cneg(cdiv(s, d));
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < bk * bk; e += 256) {
+    int i = e % bk, j = e / bk;
+    X[(k0 + i) + (long)(k0 + j) * ldx] = Xs[i][j];
+  }
+  // right-hand sides for the rows above: X[0:k0, K] = -T[0:k0, K] * X_KK
+  for (int rr = 0; rr < k0; rr += QW) {
+    __syncthreads();
+    for (int e = t; e < QW * QW; e += 256) {
+      int i = e & 63, k = e >> 6;
+      Ts[i][k] = (k < bk) ? T[(rr + i) + (long)(k0 + k) * ldt] : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    d4 aR[2][2], aI[2][2];
+    mm64<false>(Ts, Xs, aR, aI);
+    mm64_for_each(aR, aI, [&](int i, int j, cd v) {
+      if (j < bk) X[(rr + i) + (long)(k0 + j) * ldx] = cneg(v);
+    });
+  }
+  // block back-substitution upward
+  for (int I = K - 1; I >= 0; --I) {
+    const int i0 = I * QW;
+    __syncthreads();
+    for (int e = t; e < QW * QW; e += 256) {
+      int i = e & 63, j = e >> 6;
+      Ts[i][j] = T[(i0 + i) + (long)(i0 + j) * ldt];
+      Xs[i][j] = (j < bk) ? X[(i0 + i) + (long)(k0 + j) * ldx] : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    if (t < bk) {
+      const int c = t;
+      for (int i = QW - 1; i >= 0; --i) {
+        cd s = Xs[i][c];
+        for (int j = i + 1; j < QW; ++j) s = csub(s, cmul(Ts[i][j], Xs[j][c]));
+        cd d = csub(Ts[i][i], lam[c]);
+        if (cabs1(d) < smin[c]) d = mk(smin[c], 0.0);
+        Xs[i][c] = cdiv(s, d);
+      }
+    }
+    __syncthreads();
+    for (int e = t; e < QW * bk; e += 256) {
+      int i = e & 63, j = e >> 6;
+      X[(i0 + i) + (long)(k0 + j) * ldx] = Xs[i][j];
+    }
+    for (int rr = 0; rr < i0; rr += QW) {
+      __syncthreads();
+      for (int e = t; e < QW * QW; e += 256) {
+        int i = e & 63, k = e >> 6;
+        Ts[i][k] = T[(rr + i) + (long)(i0 + k) * ldt];
+      }
+      __syncthreads();
+      d4 aR[2][2], aI[2][2];
+      mm64<false>(Ts, Xs, aR, aI);
+      mm64_for_each(aR, aI, [&](int i, int j, cd v) {
+        if (j < bk) {
+          cd* p = X + (rr + i) + (long)(k0 + j) * ldx;
+          *p = csub(*p, v);
+        }
+      });
+    }
+  }
+}
+
+// Unit 2-norm, then rotate so that the largest-modulus component (first on ties) is real.
+__global__ __launch_bounds__(256) void normalize_cols_kernel(int N, cd* V, long ldv, long sV) {
+  const long b = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = blockIdx.x * 4 + wave;
+  if (col >= N) return;
+  cd* v = V + b * sV + (long)col * ldv;
+  double ss = 0.0;
+  for (int r = lane; r < N; r += 64) ss += cnorm2(v[r]);
+  ss = wsum(ss);
+  const double scl = 1.0 / sqrt(ss);
+  double best = -1.0;
+  int bi = 0x7fffffff;
+  for (int r = lane; r < N; r += 64) {
+    cd x = cscale(v[r], scl);
+    double m = cnorm2(x);
+    if (m > best) { best = m; bi = r; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double ob = __shfl_xor(best, o, 64);
+    int oi = __shfl_xor(bi, o, 64);
+    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+  }
+  const cd vk = cscale(v[bi], scl);
+  const cd rot = cscale(cconj(vk), 1.0 / sqrt(best));
+  for (int r = lane; r < N; r += 64) {
+    cd x = cmul(cscale(v[r], scl), rot);
+    if (r == bi) x.y = 0.0;
+    v[r] = x;
+  }
+}
+
+void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long sw, cd* VR, long ldv, long sV,
+                   int* info, int batch, Workspace& ws) {
+  if (n <= 0 || batch <= 0) return;
+  const long nn = (long)n * n;
+  const int nref = n_reflectors(n);
+  const int npanel = nref > 0 ? cdiv_up(nref, HB) : 1;
+  DevBuf Vfb = ws.get(sizeof(cd) * (size_t)nn * batch);
+  DevBuf Tb = ws.get(sizeof(cd) * (size_t)npanel * HB * HB * batch);
+  DevBuf Zb = ws.get(sizeof(cd) * (size_t)nn * batch);
+  cd* Vf = Vfb.as<cd>();
+  cd* Z = Zb.as<cd>();
+  gehrd(s, n, A, lda, sA, Vf, Tb.as<cd>(), batch, ws);
+  orghr(s, n, Vf, Tb.as<cd>(), Z, n, nn, batch, ws);
+  hqr_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info);
+  launch_check("hqr");
+  cd* X = Vf;  // reflectors no longer needed
+  zset(s, n, n, mk(0.0, 0.0), mk(0.0, 0.0), X, n, nn, batch);
+  dim3 g(cdiv_up(n, QW), batch);
+  trevc_kernel<<<g, 256, 0, s>>>(n, A, lda, sA, X, n, nn);
+  launch_check("trevc");
+  zgemm_strided(s, OP_N, OP_N, n, n, n, mk(1.0, 0.0), Z, n, nn, X, n, nn, mk(0.0, 0.0), VR, ldv, sV, batch);
+  dim3 gn(cdiv_up(n, 4), batch);
+  normalize_cols_kernel<<<gn, 256, 0, s>>>(n, VR, ldv, sV);
+  launch_check("normalize_cols");
+}
+
+__global__ void gamma_sqrt_kernel(int n, const cd* lam, long sl, cd* gam, long sg) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cd g = csqrt_principal(lam[b * sl + i]);
+  if (g.y < 0.0) g = cneg(g);
+  if (fabs(g.y) < fabs(g.x) && g.x < 0.0) g = cneg(g);
+  gam[b * sg + i] = g;
+}
+
+void gamma_sqrt(hipStream_t s, int n, const cd* lam, long sl, cd* gam, long sg, int batch) {
+  if (n <= 0 || batch <= 0) return;
+  dim3 g(cdiv_up(n, 256), batch);
+  gamma_sqrt_kernel<<<g, 256, 0, s>>>(n, lam, sl, gam, sg);
+  launch_check("gamma_sqrt");
+}
+
+}  // namespace vrc

@@ -1,0 +1,223 @@
+// Part (a): Fourier convolution matrices of a piecewise-constant permittivity layer
+// (replaces FourierSolver2D::solve, src/rcwa/FourierSolver2D.cpp:12-351, and the CUDA
+// kernels of src/gpu/CudaFourierSolver2D.cu).
+//
+// For a batch of layers (same nx, ny) the three rules are produced as:
+//   fe33 (Laurent, ContinuousXY):  F2[p][q] = sum_i FX_i[p] FY1_i[q]; fe33 = BlockToeplitz(F2)
+//   fe11 (inverse rule X):         G11[(a,c)][q] = sum_i inv(Toep(FXinv_i))[a][c] FY1_i[q]
+//   fe22 (inverse rule Y):         G22[(b,d)][m] = sum_i' inv(Toep(FYinv_i'))[b][d] FX1_i'[m]
+// where every 1D coefficient is the exact integral of a piecewise constant:
+//   E(x0,x1,m,d) = f/d * exp(-i pi m (x0+x1)/d) * sinc(m (x1-x0)/d) * (x1-x0)
+// (src/rcwa/FourierSolver1D.cpp:11-41). The final expansion writes the three n x n
+// matrices once (the HBM-bound step); everything before it is O(cells * harmonics).
+#include "vrc_common.hpp"
+#include "vrc_internal.hpp"
+#include "vrc_geom.hpp"
+
+namespace vrc {
+
+constexpr double PI_REF = 3.1415926535897932;  // src/rcwa/defns.h:29
+
+// One term of FourierSolver1D::solve, evaluated in the reference's operation order.
+__device__ inline cd fterm(cd f, double x0, double x1, int m, double d) {
+  const double dx = x1 - x0;
+  if (m == 0) return cscale(cscale(f, 1.0 / d), dx);
+  cd t = mk(f.x / d, f.y / d);
+  const double ang = ((double)m * PI_REF) * (x1 + x0) / d;
+  double sn, cs;
+  sincos(ang, &sn, &cs);
+  t = cmul(t, mk(cs, -sn));
+  const double sx = (double)m * dx / d;
+  const double sc = (sx == 0.0) ? 1.0 : sin(PI_REF * sx) / (PI_REF * sx);
+  return cscale(cscale(t, sc), dx);
+}
+
+// Output slots per layer l of the 1D coefficient kernel:
+//  FX [MY][M1], FXi[MY][M1], FY1[MY][M2], FYi[MX][M2], FX1[MX][M1]
+__global__ void fourier_coef_kernel(FourierDev g) {
+  const int l = blockIdx.y;
+  const int ncx = g.ncx[l], ncy = g.ncy[l];
+  const double Lx = g.Lx[l], Ly = g.Ly[l];
+  const int M1 = 2 * g.nx - 1, M2 = 2 * g.ny - 1;
+  const int hx = (M1 - 1) / 2, hy = (M2 - 1) / 2;
+  const long MX = g.MX, MY = g.MY;
+  const cd* val = g.val + (long)l * MY * MX;
+  const double* xu = g.xu + (long)l * (MX + 1);
+  const double* xs = g.xs + (long)l * (MX + 1);
+  const double* ys = g.ys + (long)l * (MY + 1);
+  const long nFX = MY * M1, nFY1 = MY * M2, nFYi = MX * M2, nFX1 = MX * M1;
+  const long total = 2 * nFX + nFY1 + nFYi + nFX1;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    if (e < 2 * nFX) {
+      const bool inv = e >= nFX;
+      const long ee = inv ? e - nFX : e;
+      const int i = (int)(ee / M1), p = (int)(ee % M1);
+      cd acc = mk(0.0, 0.0);
+      if (i < ncy) {
+        const int m = p - hx;
+        for (int j = 0; j < ncx; ++j) {
+          cd f = val[(long)i * MX + j];
+          if (inv) f = crecip(f);
+          acc = cadd(acc, fterm(f, xu[j], xu[j + 1], m, Lx));
+        }
+      }
+      (inv ? g.FXi : g.FX)[(long)l * nFX + ee] = acc;
+    } else if (e < 2 * nFX + nFY1) {
+      const long ee = e - 2 * nFX;
+      const int i = (int)(ee / M2), q = (int)(ee % M2);
+      cd r = mk(0.0, 0.0);
+      if (i < ncy) r = fterm(mk(1.0, 0.0), ys[i], ys[i + 1], q - hy, Ly);
+      g.FY1[(long)l * nFY1 + ee] = r;
+    } else if (e < 2 * nFX + nFY1 + nFYi) {
+      const long ee = e - 2 * nFX - nFY1;
+      const int i = (int)(ee / M2), q = (int)(ee % M2);
+      cd acc = mk(0.0, 0.0);
+      if (i < ncx) {
+        for (int s = 0; s < ncy; ++s)
+          acc = cadd(acc, fterm(crecip(val[(long)s * MX + i]), ys[s], ys[s + 1], q - hy, Ly));
+      }
+      g.FYi[(long)l * nFYi + ee] = acc;
+    } else {
+      const long ee = e - 2 * nFX - nFY1 - nFYi;
+      const int i = (int)(ee / M1), p = (int)(ee % M1);
+      cd r = mk(0.0, 0.0);
+      if (i < ncx) r = fterm(mk(1.0, 0.0), xs[i], xs[i + 1], p - hx, Lx);
+      g.FX1[(long)l * nFX1 + ee] = r;
+    }
+  }
+}
+
+// Inverse of the k x k Toeplitz matrix T(r,c) = v[k-1 + r - c] (ToeplitzMatrixXcs,
+// src/rcwa/ToeplitzMatrixXcs.cpp:8-44) by Gauss-Jordan elimination with partial
+// pivoting in LDS; one workgroup per matrix. Inactive slots (beyond the layer's
+// strip/column count) are left untouched.
+__global__ __launch_bounds__(256) void toeplitz_inv_kernel(int k, const cd* v, long sv, cd* out, long so,
+                                                           const int* count, int per_layer) {
+  __shared__ cd M[64][129];
+  __shared__ int piv;
+  const long id = blockIdx.x;
+  const int layer = (int)(id / per_layer), slot = (int)(id % per_layer);
+  if (slot >= count[layer]) return;
+  v += id * sv;
+  out += id * so;
+  const int t = threadIdx.x;
+  for (int e = t; e < k * 2 * k; e += 256) {
+    int r = e % k, c = e / k;
+    M[r][c] = (c < k) ? v[k - 1 + r - c] : ((c - k == r) ? mk(1.0, 0.0) : mk(0.0, 0.0));
+  }
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    if (t == 0) {
+      int p = j;
+      double best = cabs1(M[j][j]);
+      for (int r = j + 1; r < k; ++r) {
+        double a = cabs1(M[r][j]);
+        if (a > best) { best = a; p = r; }
+      }
+      piv = p;
+    }
+    __syncthreads();
+    const int p = piv;
+    if (p != j) {
+      for (int c = t; c < 2 * k; c += 256) {
+        cd tmp = M[j][c];
+        M[j][c] = M[p][c];
+        M[p][c] = tmp;
+      }
+    }
+    __syncthreads();
+    const cd inv = crecip(M[j][j]);
+    __syncthreads();
+    for (int c = t; c < 2 * k; c += 256) M[j][c] = cmul(M[j][c], inv);
+    __syncthreads();
+    for (int e = t; e < k * 2 * k; e += 256) {
+      int r = e % k, c = e / k;
+      if (r != j && c != j) M[r][c] = csub(M[r][c], cmul(M[r][j], M[j][c]));
+    }
+    __syncthreads();
+    for (int r = t; r < k; r += 256)
+      if (r != j) M[r][j] = mk(0.0, 0.0);
+    __syncthreads();
+  }
+  for (int e = t; e < k * k; e += 256) {
+    int r = e % k, c = e / k;
+    out[r + (long)c * k] = M[r][k + c];
+  }
+}
+
+// F2[p][q], G11[(a,c)][q], G22[(b,d)][m] (sums over strips/columns in ascending order).
+__global__ void fourier_contract_kernel(FourierDev g) {
+  const int l = blockIdx.y;
+  const int nx = g.nx, ny = g.ny, M1 = 2 * nx - 1, M2 = 2 * ny - 1;
+  const int ncx = g.ncx[l], ncy = g.ncy[l];
+  const long MX = g.MX, MY = g.MY;
+  const long nF2 = (long)M1 * M2, nG11 = (long)nx * nx * M2, nG22 = (long)ny * ny * M1;
+  const cd* FX = g.FX + (long)l * MY * M1;
+  const cd* FY1 = g.FY1 + (long)l * MY * M2;
+  const cd* FX1 = g.FX1 + (long)l * MX * M1;
+  const cd* tX = g.tFFx + (long)l * MY * nx * nx;
+  const cd* tY = g.tFFy + (long)l * MX * ny * ny;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nF2 + nG11 + nG22;
+       e += (long)gridDim.x * blockDim.x) {
+    cd acc = mk(0.0, 0.0);
+    if (e < nF2) {
+      const int p = (int)(e % M1), q = (int)(e / M1);
+      for (int i = 0; i < ncy; ++i) acc = cadd(acc, cmul(FX[(long)i * M1 + p], FY1[(long)i * M2 + q]));
+      g.F2[(long)l * nF2 + e] = acc;
+    } else if (e < nF2 + nG11) {
+      const long ee = e - nF2;
+      const int ac = (int)(ee % (nx * nx)), q = (int)(ee / (nx * nx));
+      for (int i = 0; i < ncy; ++i) acc = cadd(acc, cmul(FY1[(long)i * M2 + q], tX[(long)i * nx * nx + ac]));
+      g.G11[(long)l * nG11 + ee] = acc;
+    } else {
+      const long ee = e - nF2 - nG11;
+      const int bd = (int)(ee % (ny * ny)), m = (int)(ee / (ny * ny));
+      for (int i = 0; i < ncx; ++i) acc = cadd(acc, cmul(FX1[(long)i * M1 + m], tY[(long)i * ny * ny + bd]));
+      g.G22[(long)l * nG22 + ee] = acc;
+    }
+  }
+}
+
+// Expansion into the three n x n column-major matrices (row index a*ny+b).
+__global__ void fourier_expand_kernel(FourierDev g, cd* fe11, cd* fe22, cd* fe33, long sfe) {
+  const int l = blockIdx.z;
+  const int nx = g.nx, ny = g.ny, n = nx * ny, M1 = 2 * nx - 1, M2 = 2 * ny - 1;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= n) return;
+  const int a = r / ny, bb = r % ny, cc = c / ny, d = c % ny;
+  const long o = (long)l * sfe + r + (long)c * n;
+  const cd* F2 = g.F2 + (long)l * M1 * M2;
+  const cd* G11 = g.G11 + (long)l * nx * nx * M2;
+  const cd* G22 = g.G22 + (long)l * ny * ny * M1;
+  fe33[o] = F2[(a - cc + nx - 1) + (long)(bb - d + ny - 1) * M1];
+  fe11[o] = G11[(a + cc * nx) + (long)(bb - d + ny - 1) * nx * nx];
+  fe22[o] = G22[(bb + d * ny) + (long)(a - cc + nx - 1) * ny * ny];
+}
+
+void fourier_assemble(hipStream_t s, const FourierDev& g, int nlayers, cd* fe11, cd* fe22, cd* fe33, long sfe) {
+  const int M1 = 2 * g.nx - 1, M2 = 2 * g.ny - 1;
+  {
+    dim3 grid(64, nlayers);
+    fourier_coef_kernel<<<grid, 256, 0, s>>>(g);
+    launch_check("fourier_coef");
+  }
+  if (g.nx > 64 || g.ny > 64) throw ArgFailure("harmonic count per axis above 64 is not supported");
+  toeplitz_inv_kernel<<<nlayers * (int)g.MY, 256, 0, s>>>(g.nx, g.FXi, M1, g.tFFx, (long)g.nx * g.nx, g.ncy,
+                                                           (int)g.MY);
+  launch_check("toeplitz_inv_x");
+  toeplitz_inv_kernel<<<nlayers * (int)g.MX, 256, 0, s>>>(g.ny, g.FYi, M2, g.tFFy, (long)g.ny * g.ny, g.ncx,
+                                                           (int)g.MX);
+  launch_check("toeplitz_inv_y");
+  {
+    dim3 grid(64, nlayers);
+    fourier_contract_kernel<<<grid, 256, 0, s>>>(g);
+    launch_check("fourier_contract");
+  }
+  const int n = g.nx * g.ny;
+  dim3 grid(cdiv_up(n, 256), n, nlayers);
+  fourier_expand_kernel<<<grid, 256, 0, s>>>(g, fe11, fe22, fe33, sfe);
+  launch_check("fourier_expand");
+}
+
+}  // namespace vrc

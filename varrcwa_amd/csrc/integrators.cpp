@@ -1,0 +1,608 @@
+// Host-side samplers and integrators with the reference's compute-S-matrix API
+// (src/core/LayerSampler.*, VarLayerSampler.*, RCWAIntegrator.*, RedhefferIntegrator.*,
+// DifferentialIntegrator.*), driving the batched device procedures.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "vrc_ops.hpp"
+
+namespace vrc {
+
+static constexpr double PI_REF = 3.1415926535897932;
+static const cd ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
+
+// Host -> device copy of a small array on the context stream (kept alive until done).
+static DevBuf upload(vrc_ctx& c, const void* host, size_t bytes) {
+  DevBuf b = c.ws.get(bytes);
+  VRC_HIP(hipMemcpyAsync(b.ptr, host, bytes, hipMemcpyHostToDevice, c.stream));
+  VRC_HIP(hipStreamSynchronize(c.stream));
+  return b;
+}
+
+// ===================================================================== samplers
+Sampler::Sampler(vrc_ctx* c, double lambda, int nx, int ny, bool pml)
+    : c_(c), lambda_(lambda), nx_(nx), ny_(ny), pml_(pml) {
+  if (nx < 1 || ny < 1) throw ArgFailure("harmonic counts must be positive");
+  if (nx % 2 == 0 || ny % 2 == 0) throw ArgFailure("harmonic counts per axis must be odd (2*order+1)");
+  if (!(lambda > 0)) throw ArgFailure("wavelength must be positive");
+}
+
+double Sampler::k0() const { return 2 * PI_REF / lambda_; }
+
+void Sampler::post_init(const std::vector<double>& x, const std::vector<double>& y) {
+  const int n = nx_ * ny_;
+  const long nn = (long)n * n;
+  const double Lx = x.back() - x.front(), Ly = y.back() - y.front();
+  if (!(Lx > 0) || !(Ly > 0)) throw ArgFailure("cell size must be positive");
+  if (pml_) {
+    if (x.size() < 4 || y.size() < 4) throw ArgFailure("PML needs at least 4 coordinates per axis");
+    pml_k_matrices(nx_, ny_, x, y, hKx_, hKy_);
+    std::vector<cd> both(2 * nn);
+    std::copy(hKx_.begin(), hKx_.end(), both.begin());
+    std::copy(hKy_.begin(), hKy_.end(), both.begin() + nn);
+    d_K_ = c_->ws.get(sizeof(cd) * 6 * nn);
+    cd* K = d_K_.as<cd>();
+    VRC_HIP(hipMemcpyAsync(K, both.data(), sizeof(cd) * 2 * nn, hipMemcpyHostToDevice, c_->stream));
+    VRC_HIP(hipStreamSynchronize(c_->stream));
+    const cd* Kx = K;
+    const cd* Ky = K + nn;
+    zgemm_strided(c_->stream, OP_N, OP_N, n, n, n, ONE, Kx, n, 0, Ky, n, 0, ZERO, K + 2 * nn, n, 0, 1);
+    zgemm_strided(c_->stream, OP_N, OP_N, n, n, n, ONE, Kx, n, 0, Kx, n, 0, ZERO, K + 3 * nn, n, 0, 1);
+    zgemm_strided(c_->stream, OP_N, OP_N, n, n, n, ONE, Ky, n, 0, Ky, n, 0, ZERO, K + 4 * nn, n, 0, 1);
+    zgemm_strided(c_->stream, OP_N, OP_N, n, n, n, ONE, Ky, n, 0, Kx, n, 0, ZERO, K + 5 * nn, n, 0, 1);
+  } else {
+    alpha_beta(nx_, ny_, Lx, Ly, alpha_, beta_);
+    hKx_.assign(nn, mk(0, 0));
+    hKy_.assign(nn, mk(0, 0));
+    for (int i = 0; i < n; ++i) {
+      hKx_[(size_t)i * (n + 1)] = alpha_[i];
+      hKy_[(size_t)i * (n + 1)] = beta_[i];
+    }
+    std::vector<cd> ab(2 * n);
+    std::copy(alpha_.begin(), alpha_.end(), ab.begin());
+    std::copy(beta_.begin(), beta_.end(), ab.begin() + n);
+    d_alpha_ = upload(*c_, ab.data(), sizeof(cd) * 2 * n);
+  }
+}
+
+void Sampler::sample_gpu(const std::vector<double>& zs, cd* P, cd* Q) {
+  std::vector<HostLayer> ls;
+  ls.reserve(zs.size());
+  for (double z : zs) ls.push_back(layer_at(z));
+  sample_layers(ls, P, Q);
+}
+
+void Sampler::sample_layers(const std::vector<HostLayer>& ls, cd* P, cd* Q) {
+  const int L = (int)ls.size();
+  if (L == 0) return;
+  AssemblyBatch a;
+  a.nx = nx_;
+  a.ny = ny_;
+  a.nlayers = L;
+  for (auto& l : ls) a.layers.push_back(&l);
+  a.periodic = !pml_;
+  const double k2 = k0() * k0();
+  a.k0sq_uniform = k2;
+  std::vector<double> k2v(L, k2);
+  DevBuf dk = upload(*c_, k2v.data(), sizeof(double) * L);
+  a.d_k0sq = dk.as<double>();
+  if (pml_) {
+    const long nn = (long)n() * n();
+    const cd* K = d_K_.as<cd>();
+    a.d_Kx = K;
+    a.d_Ky = K + nn;
+    a.d_KxKy = K + 2 * nn;
+    a.d_KxKx = K + 3 * nn;
+    a.d_KyKy = K + 4 * nn;
+    a.d_KyKx = K + 5 * nn;
+  } else {
+    a.d_alpha = d_alpha_.as<cd>();
+    a.d_beta = d_alpha_.as<cd>() + n();
+    a.sab = 0;
+  }
+  assemble_pq(*c_, a, P, Q);
+}
+
+VarSampler::VarSampler(vrc_ctx* c, double lambda, int nx, int ny, HostLayer base, bool pml)
+    : Sampler(c, lambda, nx, ny, pml), base_(std::move(base)) {
+  prepare_layer(base_);  // validates shapes
+  post_init(base_.x, base_.y);
+}
+
+void VarSampler::add_rule(int axis, int layout, RuleFn fn, void* user) {
+  const int lim = axis == 0 ? (int)base_.x.size() : (int)base_.y.size();
+  if (layout < 0 || layout >= lim) throw ArgFailure("rule layout index out of range");
+  if (!fn) throw ArgFailure("null rule");
+  (axis == 0 ? xr_ : yr_)[layout] = {fn, user};
+}
+
+HostLayer VarSampler::layer_at(double z) const {
+  HostLayer l = base_;
+  for (auto& kv : xr_) l.x[kv.first] = kv.second.first(z, kv.second.second);
+  for (auto& kv : yr_) l.y[kv.first] = kv.second.first(z, kv.second.second);
+  return l;
+}
+
+StackSampler::StackSampler(vrc_ctx* c, double lambda, int nx, int ny, std::vector<HostLayer> layers,
+                           std::vector<double> z_edges, HostLayer bottom, HostLayer top, bool pml)
+    : Sampler(c, lambda, nx, ny, pml), layers_(std::move(layers)), ze_(std::move(z_edges)),
+      bottom_(std::move(bottom)), top_(std::move(top)) {
+  if (ze_.size() != layers_.size() + 1) throw ArgFailure("need nlayers+1 z edges");
+  for (size_t i = 0; i + 1 < ze_.size(); ++i)
+    if (!(ze_[i + 1] > ze_[i])) throw ArgFailure("z edges must increase");
+  for (auto& l : layers_) prepare_layer(l);
+  prepare_layer(bottom_);
+  prepare_layer(top_);
+  post_init(bottom_.x, bottom_.y);
+}
+
+HostLayer StackSampler::layer_at(double z) const {
+  if (z <= ze_.front()) return bottom_;
+  if (z >= ze_.back()) return top_;
+  size_t k = std::upper_bound(ze_.begin(), ze_.end(), z) - ze_.begin() - 1;
+  return layers_[k];
+}
+
+// ===================================================================== integrators
+Integrator::Integrator(vrc_ctx* c, double z0, double z1, Sampler* s) : c_(c), z0_(z0), z1_(z1), s_(s) {
+  if (!s) throw ArgFailure("null sampler");
+  Nd_ = s->nDim();
+  alloc();
+}
+
+void Integrator::alloc() {
+  const long NN = (long)Nd_ * Nd_;
+  S_ = c_->ws.get(sizeof(cd) * 4 * NN);
+  W0_ = c_->ws.get(sizeof(cd) * NN);
+  V0_ = c_->ws.get(sizeof(cd) * NN);
+  W1_ = c_->ws.get(sizeof(cd) * NN);
+  V1_ = c_->ws.get(sizeof(cd) * NN);
+  g0_ = c_->ws.get(sizeof(cd) * Nd_);
+  g1_ = c_->ws.get(sizeof(cd) * Nd_);
+  info_ = c_->ws.get(sizeof(int) * 64);
+}
+
+SMat Integrator::S() const {
+  const long NN = (long)Nd_ * Nd_;
+  cd* p = S_.as<cd>();
+  SMat s;
+  s.Tdd = p;
+  s.Tuu = p + NN;
+  s.Rud = p + 2 * NN;
+  s.Rdu = p + 3 * NN;
+  return s;
+}
+
+void Integrator::get(int which, cd* host) const {
+  const long NN = (long)Nd_ * Nd_;
+  const cd* src = nullptr;
+  size_t cnt = NN;
+  switch (which) {
+    case 0: src = S().Tdd; break;
+    case 1: src = S().Tuu; break;
+    case 2: src = S().Rud; break;
+    case 3: src = S().Rdu; break;
+    case 4: src = W0_.as<cd>(); break;
+    case 5: src = V0_.as<cd>(); break;
+    case 6: src = g0_.as<cd>(); cnt = Nd_; break;
+    case 7: src = W1_.as<cd>(); break;
+    case 8: src = V1_.as<cd>(); break;
+    case 9: src = g1_.as<cd>(); cnt = Nd_; break;
+    default: throw ArgFailure("unknown output selector");
+  }
+  VRC_HIP(hipMemcpyAsync(host, src, sizeof(cd) * cnt, hipMemcpyDeviceToHost, c_->stream));
+  VRC_HIP(hipStreamSynchronize(c_->stream));
+}
+
+void Integrator::eig_at(double z, cd* P, cd* Q, cd* W, cd* V, cd* gam) {
+  s_->sample_gpu({z}, P, Q);
+  wave_eqn(*c_, Nd_, P, Q, (long)Nd_ * Nd_, W, V, gam, info_.as<int>(), 1);
+}
+
+namespace {
+struct LUStore {
+  DevBuf W, V, pW, pV, inf;
+  LUPair pair;
+  void make(vrc_ctx& c, int N, const cd* Wsrc, const cd* Vsrc, int batch, long sSrc) {
+    const long NN = (long)N * N;
+    W = c.ws.get(sizeof(cd) * NN * batch);
+    V = c.ws.get(sizeof(cd) * NN * batch);
+    pW = c.ws.get(sizeof(int) * (size_t)N * batch);
+    pV = c.ws.get(sizeof(int) * (size_t)N * batch);
+    inf = c.ws.get(sizeof(int) * (size_t)batch);
+    fill_int(c.stream, inf.as<int>(), 0, batch);
+    zgeam(c.stream, N, N, ONE, Wsrc, N, sSrc, ZERO, nullptr, 0, 0, W.as<cd>(), N, NN, batch);
+    zgeam(c.stream, N, N, ONE, Vsrc, N, sSrc, ZERO, nullptr, 0, 0, V.as<cd>(), N, NN, batch);
+    zgetrf(c.stream, N, W.as<cd>(), N, NN, pW.as<int>(), N, inf.as<int>(), batch);
+    zgetrf(c.stream, N, V.as<cd>(), N, NN, pV.as<int>(), N, inf.as<int>(), batch);
+    pair.W = W.as<cd>();
+    pair.V = V.as<cd>();
+    pair.pW = pW.as<int>();
+    pair.pV = pV.as<int>();
+    pair.sM = NN;
+  }
+};
+
+void init_identity_S(vrc_ctx& c, int N, SMat& S, int batch) {
+  const long NN = (long)N * N;
+  zset(c.stream, N, N, ONE, ZERO, S.Tdd, N, NN, batch);
+  zset(c.stream, N, N, ONE, ZERO, S.Tuu, N, NN, batch);
+  zset(c.stream, N, N, ZERO, ZERO, S.Rud, N, NN, batch);
+  zset(c.stream, N, N, ZERO, ZERO, S.Rdu, N, NN, batch);
+}
+}  // namespace
+
+// RCWAIntegrator::compute (src/core/RCWAIntegrator.cpp:20-87) over explicit slices
+// (midpoint z, width). batched=false: one slice at a time as the reference does;
+// batched=true: slices sampled/solved in groups, then cascaded in order.
+void Integrator::rcwa(const std::vector<std::pair<double, double>>& slices, bool batched) {
+  vrc_ctx& c = *c_;
+  const int N = Nd_;
+  const long NN = (long)N * N;
+  tree_.clear();
+  DevBuf P = c.ws.get(sizeof(cd) * NN), Q = c.ws.get(sizeof(cd) * NN);
+  eig_at(z1_, P.as<cd>(), Q.as<cd>(), W1_.as<cd>(), V1_.as<cd>(), g1_.as<cd>());
+  LUStore lu1;
+  lu1.make(c, N, W1_.as<cd>(), V1_.as<cd>(), 1, 0);
+  SMat S = this->S();
+  init_identity_S(c, N, S, 1);
+  const double k0 = s_->k0();
+  const int G = batched ? std::max(1, std::min((int)slices.size(), 64)) : 1;
+  DevBuf Pb = c.ws.get(sizeof(cd) * NN * G), Qb = c.ws.get(sizeof(cd) * NN * G);
+  DevBuf Wb = c.ws.get(sizeof(cd) * NN * G), Vb = c.ws.get(sizeof(cd) * NN * G), gb = c.ws.get(sizeof(cd) * N * G);
+  DevBuf tu = c.ws.get(sizeof(cd) * NN * G), ru = c.ws.get(sizeof(cd) * NN * G);
+  DevBuf infb = c.ws.get(sizeof(int) * G);
+  for (size_t i0 = 0; i0 < slices.size(); i0 += G) {
+    const int g = (int)std::min<size_t>(G, slices.size() - i0);
+    std::vector<double> zs(g), sv(g);
+    for (int j = 0; j < g; ++j) {
+      zs[j] = slices[i0 + j].first;
+      sv[j] = slices[i0 + j].second / k0;
+    }
+    s_->sample_gpu(zs, Pb.as<cd>(), Qb.as<cd>());
+    wave_eqn(c, N, Pb.as<cd>(), Qb.as<cd>(), NN, Wb.as<cd>(), Vb.as<cd>(), gb.as<cd>(), infb.as<int>(), g);
+    DevBuf ds = upload(c, sv.data(), sizeof(double) * g);
+    layer_smatrix(c, N, g, ds.as<double>(), Wb.as<cd>(), Vb.as<cd>(), gb.as<cd>(), W1_.as<cd>(), V1_.as<cd>(), 0,
+                  tu.as<cd>(), ru.as<cd>());
+    for (int j = 0; j < g; ++j) {
+      const cd* t = tu.as<cd>() + j * NN;
+      const cd* r = ru.as<cd>() + j * NN;
+      star_product(c, N, 1, S, t, t, r, r, 0);
+    }
+  }
+  eig_at(z0_, P.as<cd>(), Q.as<cd>(), W0_.as<cd>(), V0_.as<cd>(), g0_.as<cd>());
+  project_left(c, N, 1, S, lu1.pair, W0_.as<cd>(), V0_.as<cd>(), 0);
+  N_ = (int)slices.size();
+  nEig_ = (int)slices.size() + 2;
+  VRC_HIP(hipStreamSynchronize(c.stream));
+}
+
+// ===================================================================== varRCWA
+namespace {
+struct RefHolder {
+  DevBuf W, V, g;
+  LUStore lu;
+  RefSys sys() const {
+    RefSys r;
+    r.W = W.as<cd>();
+    r.V = V.as<cd>();
+    r.gam = g.as<cd>();
+    r.LU = lu.pair;
+    return r;
+  }
+};
+struct Sample {
+  DevBuf P, Q;
+};
+}  // namespace
+
+// RedhefferIntegrator::compute, trisection branch (src/core/RedhefferIntegrator.cpp:174-314).
+void Integrator::redheffer(double pe, int max_depth) {
+  vrc_ctx& c = *c_;
+  const int N = Nd_;
+  const long NN = (long)N * N;
+  const double k0 = s_->k0();
+  tree_.clear();
+  int* info = info_.as<int>();
+  auto sample = [&](double z) {
+    Sample s;
+    s.P = c.ws.get(sizeof(cd) * NN);
+    s.Q = c.ws.get(sizeof(cd) * NN);
+    s_->sample_gpu({z}, s.P.as<cd>(), s.Q.as<cd>());
+    return s;
+  };
+  auto make_ref = [&](const Sample& smp) {
+    auto R = std::make_unique<RefHolder>();
+    R->W = c.ws.get(sizeof(cd) * NN);
+    R->V = c.ws.get(sizeof(cd) * NN);
+    R->g = c.ws.get(sizeof(cd) * N);
+    wave_eqn(c, N, smp.P.as<cd>(), smp.Q.as<cd>(), NN, R->W.as<cd>(), R->V.as<cd>(), R->g.as<cd>(), info, 1);
+    R->lu.make(c, N, R->W.as<cd>(), R->V.as<cd>(), 1, 0);
+    return R;
+  };
+  Sample s0 = sample(z0_);
+  wave_eqn(c, N, s0.P.as<cd>(), s0.Q.as<cd>(), NN, W0_.as<cd>(), V0_.as<cd>(), g0_.as<cd>(), info, 1);
+  Sample s1 = sample(z1_);
+  wave_eqn(c, N, s1.P.as<cd>(), s1.Q.as<cd>(), NN, W1_.as<cd>(), V1_.as<cd>(), g1_.as<cd>(), info, 1);
+  Sample sm = sample(0.5 * (z0_ + z1_));
+  auto Rm = make_ref(sm);
+  SMat S = this->S();
+  init_identity_S(c, N, S, 1);
+  DevBuf leftW = c.ws.get(sizeof(cd) * NN), leftV = c.ws.get(sizeof(cd) * NN);
+  zgeam(c.stream, N, N, ONE, W0_.as<cd>(), N, 0, ZERO, nullptr, 0, 0, leftW.as<cd>(), N, 0, 1);
+  zgeam(c.stream, N, N, ONE, V0_.as<cd>(), N, 0, ZERO, nullptr, 0, 0, leftV.as<cd>(), N, 0, 1);
+  N_ = 0;
+  nEig_ = 2;
+
+  std::function<void(const RefSys&, const cd*, const cd*, double, double, const Sample&, const Sample&,
+                     const Sample&, int)>
+      rec = [&](const RefSys& R, const cd* Pr, const cd* Qr, double za, double zb, const Sample& a,
+                const Sample& mid, const Sample& b, int depth) {
+        DevBuf seg = c.ws.get(sizeof(cd) * 4 * NN);
+        SMat T;
+        T.Tdd = seg.as<cd>();
+        T.Tuu = T.Tdd + NN;
+        T.Rud = T.Tdd + 2 * NN;
+        T.Rdu = T.Tdd + 3 * NN;
+        const cd* P3[3] = {a.P.as<cd>(), mid.P.as<cd>(), b.P.as<cd>()};
+        const cd* Q3[3] = {a.Q.as<cd>(), mid.Q.as<cd>(), b.Q.as<cd>()};
+        const double err =
+            redheffer_perturbation(c, N, k0, za, zb, Pr, Qr, R, P3, Q3, T.Tdd, T.Tuu, T.Rud, T.Rdu);
+        if (err <= pe || depth >= max_depth) {
+          ++N_;
+          tree_.push_back({za, zb});
+          project_left(c, N, 1, T, R.LU, leftW.as<cd>(), leftV.as<cd>(), 0);
+          zgeam(c.stream, N, N, ONE, R.W, N, 0, ZERO, nullptr, 0, 0, leftW.as<cd>(), N, 0, 1);
+          zgeam(c.stream, N, N, ONE, R.V, N, 0, ZERO, nullptr, 0, 0, leftV.as<cd>(), N, 0, 1);
+          star_product(c, N, 1, S, T.Tdd, T.Tuu, T.Rud, T.Rdu, 0);
+          return;
+        }
+        const double zm0 = 2. / 3. * za + 1. / 3. * zb;
+        const double zm1 = 1. / 3. * za + 2. / 3. * zb;
+        {
+          Sample st = sample(0.5 * (za + zm0));
+          Sample sk = sample(zm0);
+          auto Rt = make_ref(st);
+          rec(Rt->sys(), st.P.as<cd>(), st.Q.as<cd>(), za, zm0, a, st, sk, depth + 1);
+          Sample st2 = sample(zm1);
+          rec(R, Pr, Qr, zm0, zm1, sk, mid, st2, depth + 1);
+          Sample sk2 = sample(0.5 * (zm1 + zb));
+          auto Rt2 = make_ref(sk2);
+          rec(Rt2->sys(), sk2.P.as<cd>(), sk2.Q.as<cd>(), zm1, zb, st2, sk2, b, depth + 1);
+        }
+      };
+  rec(Rm->sys(), sm.P.as<cd>(), sm.Q.as<cd>(), z0_, z1_, s0, sm, s1, 0);
+  // right projection (src/core/RedhefferIntegrator.cpp:211-223)
+  LUStore lul;
+  lul.make(c, N, leftW.as<cd>(), leftV.as<cd>(), 1, 0);
+  project_right(c, N, 1, S, lul.pair, W1_.as<cd>(), V1_.as<cd>(), 0);
+  VRC_HIP(hipStreamSynchronize(c.stream));
+}
+
+// ===================================================================== differential method
+// DifferentialIntegrator::compute (src/core/DifferentialIntegrator.cpp:21-134).
+void Integrator::differential(int Nsteps) {
+  vrc_ctx& c = *c_;
+  hipStream_t st = c.stream;
+  const int N = Nd_;
+  const long NN = (long)N * N;
+  const double k0 = s_->k0();
+  tree_.clear();
+  DevBuf P = c.ws.get(sizeof(cd) * NN), Q = c.ws.get(sizeof(cd) * NN);
+  eig_at(z1_, P.as<cd>(), Q.as<cd>(), W1_.as<cd>(), V1_.as<cd>(), g1_.as<cd>());
+  LUStore lu1;
+  lu1.make(c, N, W1_.as<cd>(), V1_.as<cd>(), 1, 0);
+  SMat S = this->S();
+  init_identity_S(c, N, S, 1);
+  const double dz = (z1_ - z0_) / (double)Nsteps;
+  DevBuf buf = c.ws.get(sizeof(cd) * NN * 30);
+  cd* b = buf.as<cd>();
+  cd *P0 = b, *Q0 = b + NN, *P1 = b + 2 * NN, *Q1 = b + 3 * NN, *P2 = b + 4 * NN, *Q2 = b + 5 * NN;
+  cd *X1 = b + 6 * NN, *X2 = b + 7 * NN, *X3 = b + 8 * NN, *Y1 = b + 9 * NN, *Y2 = b + 10 * NN, *Y3 = b + 11 * NN;
+  cd *L11 = b + 12 * NN, *L12 = b + 13 * NN, *L21 = b + 14 * NN, *L22 = b + 15 * NN;
+  cd *iW11 = b + 16 * NN, *iW12 = b + 17 * NN, *iV21 = b + 18 * NN, *iV22 = b + 19 * NN;
+  cd *A11 = b + 20 * NN, *A12 = b + 21 * NN, *A21 = b + 22 * NN, *A22 = b + 23 * NN;
+  cd *T11 = b + 24 * NN, *T12 = b + 25 * NN, *T21 = b + 26 * NN, *T22 = b + 27 * NN, *tdd = b + 28 * NN,
+     *rdu = b + 29 * NN;
+  DevBuf extra = c.ws.get(sizeof(cd) * NN * 2);
+  cd *tuu = extra.as<cd>(), *rud = extra.as<cd>() + NN;
+  DevBuf piv = c.ws.get(sizeof(int) * N), inf = c.ws.get(sizeof(int));
+  auto mm = [&](cd al, const cd* A, const cd* B, cd be, cd* C) {
+    zgemm_strided(st, OP_N, OP_N, N, N, N, al, A, N, 0, B, N, 0, be, C, N, 0, 1);
+  };
+  auto axpy = [&](cd al, const cd* A, cd* C) { zgeam(st, N, N, al, A, N, 0, ONE, C, N, 0, C, N, 0, 1); };
+  s_->sample_gpu({z0_}, P0, Q0);
+  const double a = dz * dz / (6. * k0 * k0), bb = dz * dz / (4. * k0 * k0), cc = dz * dz / (2. * k0 * k0);
+  const cd ci = mk(0.0, dz / (6. * k0));
+  for (int i = 0; i < Nsteps; ++i) {
+    const double zl = z0_ + i * dz, zr = z0_ + (i + 1) * dz, zm = 0.5 * (zl + zr);
+    s_->sample_gpu({zm}, P1, Q1);
+    s_->sample_gpu({zr}, P2, Q2);
+    mm(ONE, P1, Q0, ZERO, X1);
+    mm(ONE, P1, Q1, ZERO, X2);
+    mm(ONE, P2, Q1, ZERO, X3);
+    mm(ONE, Q1, P0, ZERO, Y1);
+    mm(ONE, Q1, P1, ZERO, Y2);
+    mm(ONE, Q2, P1, ZERO, Y3);
+    // L11 = I - a (X1 + X2 + X3) + a b X3 X1
+    zset(st, N, N, ONE, ZERO, L11, N, 0, 1);
+    axpy(mk(-a, 0), X1, L11);
+    axpy(mk(-a, 0), X2, L11);
+    axpy(mk(-a, 0), X3, L11);
+    mm(mk(a * bb, 0), X3, X1, ONE, L11);
+    // L22 = I - a (Y1 + Y2 + Y3) + a b Y3 Y1
+    zset(st, N, N, ONE, ZERO, L22, N, 0, 1);
+    axpy(mk(-a, 0), Y1, L22);
+    axpy(mk(-a, 0), Y2, L22);
+    axpy(mk(-a, 0), Y3, L22);
+    mm(mk(a * bb, 0), Y3, Y1, ONE, L22);
+    // L12 = ci (P0 + 4 P1 + P2) - ci cc (X2 P0 + X3 P1)
+    zgeam(st, N, N, ci, P0, N, 0, cscale(ci, 4.0), P1, N, 0, L12, N, 0, 1);
+    axpy(ci, P2, L12);
+    mm(cscale(ci, -cc), X2, P0, ONE, L12);
+    mm(cscale(ci, -cc), X3, P1, ONE, L12);
+    // L21 = ci (Q0 + 4 Q1 + Q2) - ci cc (Y2 Q0 + Y3 Q1)
+    zgeam(st, N, N, ci, Q0, N, 0, cscale(ci, 4.0), Q1, N, 0, L21, N, 0, 1);
+    axpy(ci, Q2, L21);
+    mm(cscale(ci, -cc), Y2, Q0, ONE, L21);
+    mm(cscale(ci, -cc), Y3, Q1, ONE, L21);
+    zgetrs(st, N, N, lu1.pair.W, N, 0, lu1.pair.pW, N, L11, N, 0, iW11, N, 0, 1, c.ws);
+    zgetrs(st, N, N, lu1.pair.W, N, 0, lu1.pair.pW, N, L12, N, 0, iW12, N, 0, 1, c.ws);
+    zgetrs(st, N, N, lu1.pair.V, N, 0, lu1.pair.pV, N, L21, N, 0, iV21, N, 0, 1, c.ws);
+    zgetrs(st, N, N, lu1.pair.V, N, 0, lu1.pair.pV, N, L22, N, 0, iV22, N, 0, 1, c.ws);
+    const cd h = mk(0.5, 0), mh = mk(-0.5, 0);
+    zgeam(st, N, N, h, iW11, N, 0, h, iV21, N, 0, A11, N, 0, 1);
+    zgeam(st, N, N, h, iW12, N, 0, h, iV22, N, 0, A12, N, 0, 1);
+    zgeam(st, N, N, h, iW11, N, 0, mh, iV21, N, 0, A21, N, 0, 1);
+    zgeam(st, N, N, h, iW12, N, 0, mh, iV22, N, 0, A22, N, 0, 1);
+    const cd* W1 = W1_.as<cd>();
+    const cd* V1 = V1_.as<cd>();
+    mm(ONE, A11, W1, ZERO, T11);
+    mm(ONE, A12, V1, ONE, T11);
+    mm(ONE, A11, W1, ZERO, T12);
+    mm(MONE, A12, V1, ONE, T12);
+    mm(ONE, A21, W1, ZERO, T21);
+    mm(ONE, A22, V1, ONE, T21);
+    mm(ONE, A21, W1, ZERO, T22);
+    mm(MONE, A22, V1, ONE, T22);
+    // tdd = T22^{-1}; rud = T12 tdd; rdu = -T22^{-1} T21; tuu = T11 + T12 rdu
+    fill_int(st, inf.as<int>(), 0, 1);
+    zgetrf(st, N, T22, N, 0, piv.as<int>(), N, inf.as<int>(), 1);
+    zgetri(st, N, T22, N, 0, piv.as<int>(), N, tdd, N, 0, 1, c.ws);
+    mm(ONE, T12, tdd, ZERO, rud);
+    zgetrs(st, N, N, T22, N, 0, piv.as<int>(), N, T21, N, 0, rdu, N, 0, 1, c.ws);
+    zgeam(st, N, N, MONE, rdu, N, 0, ZERO, nullptr, 0, 0, rdu, N, 0, 1);
+    zgeam(st, N, N, ONE, T11, N, 0, ZERO, nullptr, 0, 0, tuu, N, 0, 1);
+    mm(ONE, T12, rdu, ONE, tuu);
+    star_product(c, N, 1, S, tdd, tuu, rud, rdu, 0);
+    std::swap(P0, P2);
+    std::swap(Q0, Q2);
+  }
+  eig_at(z0_, P.as<cd>(), Q.as<cd>(), W0_.as<cd>(), V0_.as<cd>(), g0_.as<cd>());
+  project_left(c, N, 1, S, lu1.pair, W0_.as<cd>(), V0_.as<cd>(), 0);
+  N_ = Nsteps;
+  nEig_ = 2;
+  VRC_HIP(hipStreamSynchronize(st));
+}
+
+// ===================================================================== batched stacks
+void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int chunk, cd* S_out,
+                      cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o) {
+  hipStream_t st = c.stream;
+  const int n = nx * ny, N = 2 * n;
+  const long NN = (long)N * N;
+  const int ninst = (int)inst.size();
+  if (ninst == 0) return;
+  const int L = (int)inst[0].layers.size();
+  for (auto& I : inst) {
+    if ((int)I.layers.size() != L || (int)I.thickness.size() != L)
+      throw ArgFailure("all instances of one batch must have the same layer count");
+    if (!I.bottom || !I.top) throw ArgFailure("instance without bottom/top medium");
+    if (!(I.lambda > 0)) throw ArgFailure("wavelength must be positive");
+  }
+  if (chunk <= 0) chunk = std::max(1, std::min(ninst, 256 / std::max(1, L)));
+  for (int i0 = 0; i0 < ninst; i0 += chunk) {
+    const int B = std::min(chunk, ninst - i0);
+    // ---- jobs: [bottoms(B) | tops(B) | slices(B*L, instance-major)]
+    const int J = 2 * B + B * L;
+    std::vector<const HostLayer*> jl(J);
+    std::vector<double> k2(J), sv(B * L);
+    std::vector<cd> ab((size_t)J * 2 * n);
+    std::vector<int> owner(B * L);
+    for (int b = 0; b < B; ++b) {
+      const StackInstance& I = inst[i0 + b];
+      const double k0 = 2 * PI_REF / I.lambda;
+      std::vector<cd> al, be;
+      const double Lx = I.bottom->x.back() - I.bottom->x.front(), Ly = I.bottom->y.back() - I.bottom->y.front();
+      alpha_beta(nx, ny, Lx, Ly, al, be);
+      auto put_job = [&](int j, const HostLayer* l) {
+        jl[j] = l;
+        k2[j] = k0 * k0;
+        std::copy(al.begin(), al.end(), ab.begin() + (size_t)j * 2 * n);
+        std::copy(be.begin(), be.end(), ab.begin() + (size_t)j * 2 * n + n);
+      };
+      put_job(b, I.bottom);
+      put_job(B + b, I.top);
+      for (int l = 0; l < L; ++l) {
+        put_job(2 * B + b * L + l, I.layers[l]);
+        sv[b * L + l] = I.thickness[l] / k0;
+        owner[b * L + l] = b;
+      }
+    }
+    DevBuf dk2 = upload(c, k2.data(), sizeof(double) * J);
+    DevBuf dab = upload(c, ab.data(), sizeof(cd) * ab.size());
+    DevBuf dsv = upload(c, sv.data(), sizeof(double) * sv.size());
+    DevBuf down = upload(c, owner.data(), sizeof(int) * owner.size());
+    // ---- ends: P, Q, eig
+    DevBuf Wend = c.ws.get(sizeof(cd) * NN * 2 * B), Vend = c.ws.get(sizeof(cd) * NN * 2 * B),
+           gend = c.ws.get(sizeof(cd) * (size_t)N * 2 * B), infe = c.ws.get(sizeof(int) * 2 * B);
+    auto solve_jobs = [&](int j0, int cnt, cd* W, cd* V, cd* gam, int* inf) {
+      AssemblyBatch a;
+      a.nx = nx;
+      a.ny = ny;
+      a.nlayers = cnt;
+      a.layers.assign(jl.begin() + j0, jl.begin() + j0 + cnt);
+      a.periodic = true;
+      a.d_alpha = dab.as<cd>() + (size_t)j0 * 2 * n;
+      a.d_beta = a.d_alpha + n;
+      a.sab = 2 * n;
+      a.d_k0sq = dk2.as<double>() + j0;
+      DevBuf P = c.ws.get(sizeof(cd) * NN * cnt), Q = c.ws.get(sizeof(cd) * NN * cnt);
+      assemble_pq(c, a, P.as<cd>(), Q.as<cd>());
+      wave_eqn(c, N, P.as<cd>(), Q.as<cd>(), NN, W, V, gam, inf, cnt);
+    };
+    solve_jobs(0, 2 * B, Wend.as<cd>(), Vend.as<cd>(), gend.as<cd>(), infe.as<int>());
+    const cd* W0 = Wend.as<cd>();
+    const cd* V0 = Vend.as<cd>();
+    const cd* W1 = Wend.as<cd>() + B * NN;
+    const cd* V1 = Vend.as<cd>() + B * NN;
+    LUStore lu1;
+    lu1.make(c, N, W1, V1, B, NN);
+    // ---- slices: eig + layer S (reference basis of the owning instance)
+    const int SL = B * L;
+    DevBuf tu = c.ws.get(sizeof(cd) * NN * SL), ru = c.ws.get(sizeof(cd) * NN * SL);
+    {
+      DevBuf W = c.ws.get(sizeof(cd) * NN * SL), V = c.ws.get(sizeof(cd) * NN * SL),
+             g = c.ws.get(sizeof(cd) * (size_t)N * SL), inf = c.ws.get(sizeof(int) * SL);
+      solve_jobs(2 * B, SL, W.as<cd>(), V.as<cd>(), g.as<cd>(), inf.as<int>());
+      DevBuf Wr = c.ws.get(sizeof(cd) * NN * SL), Vr = c.ws.get(sizeof(cd) * NN * SL);
+      gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
+      gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
+      layer_smatrix(c, N, SL, dsv.as<double>(), W.as<cd>(), V.as<cd>(), g.as<cd>(), Wr.as<cd>(), Vr.as<cd>(), NN,
+                    tu.as<cd>(), ru.as<cd>());
+    }
+    // ---- cascade over layers, batched over instances
+    DevBuf Sb = c.ws.get(sizeof(cd) * 4 * NN * B);
+    SMat S;
+    S.Tdd = Sb.as<cd>();
+    S.Tuu = S.Tdd + B * NN;
+    S.Rud = S.Tdd + 2 * B * NN;
+    S.Rdu = S.Tdd + 3 * B * NN;
+    init_identity_S(c, N, S, B);
+    for (int l = 0; l < L; ++l) {
+      const cd* t = tu.as<cd>() + l * NN;
+      const cd* r = ru.as<cd>() + l * NN;
+      star_product(c, N, B, S, t, t, r, r, (long)L * NN);
+    }
+    project_left(c, N, B, S, lu1.pair, W0, V0, NN);
+    // ---- outputs
+    const long so = 4 * NN;
+    cd* So = S_out + (long)i0 * so;
+    zgeam(st, N, N, ONE, S.Tdd, N, NN, ZERO, nullptr, 0, 0, So, N, so, B);
+    zgeam(st, N, N, ONE, S.Tuu, N, NN, ZERO, nullptr, 0, 0, So + NN, N, so, B);
+    zgeam(st, N, N, ONE, S.Rud, N, NN, ZERO, nullptr, 0, 0, So + 2 * NN, N, so, B);
+    zgeam(st, N, N, ONE, S.Rdu, N, NN, ZERO, nullptr, 0, 0, So + 3 * NN, N, so, B);
+    auto out = [&](cd* dst, const cd* src, long sz) {
+      if (dst) zgeam(st, (int)sz, 1, ONE, src, sz, sz, ZERO, nullptr, 0, 0, dst + (long)i0 * sz, sz, sz, B);
+    };
+    out(W0o, W0, NN);
+    out(V0o, V0, NN);
+    out(g0o, gend.as<cd>(), N);
+    out(W1o, W1, NN);
+    out(V1o, V1, NN);
+    out(g1o, gend.as<cd>() + (long)B * N, N);
+  }
+}
+
+}  // namespace vrc

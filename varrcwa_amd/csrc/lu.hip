@@ -1,0 +1,282 @@
+// Batched LU factorization with partial pivoting and triangular solves (complex128).
+//
+// getrf: right-looking, panel width LU_NB. Panel factored by one workgroup per
+// matrix (pivot = first index of max |re|+|im|, as LAPACK izamax); row swaps and the
+// unit-lower solve for U12 in a column-parallel kernel; trailing update on MFMA ZGEMM.
+// getrs: rows gathered through the final permutation, then blocked forward/backward
+// substitution (64-row diagonal blocks solved in LDS, off-diagonal updates on ZGEMM).
+#include "vrc_common.hpp"
+#include "vrc_internal.hpp"
+
+namespace vrc {
+
+constexpr int LU_NB = 32;
+constexpr int TRS_NB = 64;
+
+__global__ __launch_bounds__(256) void getrf_panel_kernel(int n, int k, int kb, cd* A, long lda, long sA, int* ipiv,
+                                                          long sP, int* info) {
+  const long b = blockIdx.x;
+  A += b * sA;
+  ipiv += b * sP;
+  const int t = threadIdx.x;
+  __shared__ double rv[256];
+  __shared__ int ri[256];
+  __shared__ cd prow[LU_NB];
+  __shared__ cd pivval;
+  for (int j = 0; j < kb; ++j) {
+    const int c = k + j;
+    double best = -1.0;
+    int bi = 0x7fffffff;
+    for (int r = c + t; r < n; r += 256) {
+      double v = cabs1(A[r + (long)c * lda]);
+      if (v > best) { best = v; bi = r; }
+    }
+    rv[t] = best;
+    ri[t] = bi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (t < w) {
+        double ov = rv[t + w];
+        int oi = ri[t + w];
+        if (ov > rv[t] || (ov == rv[t] && oi < ri[t])) { rv[t] = ov; ri[t] = oi; }
+      }
+      __syncthreads();
+    }
+    // no comparable candidate (NaN column): keep the diagonal as pivot row
+    const int p = (ri[0] >= c && ri[0] < n) ? ri[0] : c;
+    const double pv = rv[0];
+    __syncthreads();
+    if (t == 0) ipiv[c] = p;
+    if (!(pv > 0.0)) {
+      // exactly singular column (or NaN): record, skip elimination (LAPACK getf2 semantics)
+      if (t == 0 && info && info[b] == 0) info[b] = c + 1;
+      __syncthreads();
+      continue;
+    }
+    if (p != c && t < kb) {
+      cd* a1 = A + c + (long)(k + t) * lda;
+      cd* a2 = A + p + (long)(k + t) * lda;
+      cd tmp = *a1;
+      *a1 = *a2;
+      *a2 = tmp;
+    }
+    __syncthreads();
+    if (t < kb) prow[t] = A[c + (long)(k + t) * lda];
+    if (t == 0) pivval = A[c + (long)c * lda];
+    __syncthreads();
+    const cd rinv = crecip(pivval);
+    for (int r = c + 1 + t; r < n; r += 256) {
+      cd* ar = A + r;
+      cd l = cmul(ar[(long)c * lda], rinv);
+      ar[(long)c * lda] = l;
+      for (int jj = j + 1; jj < kb; ++jj) {
+        cd* e = ar + (long)(k + jj) * lda;
+        *e = csub(*e, cmul(l, prow[jj]));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Row swaps of the panel applied to all other columns; for columns right of the
+// panel also U12 = L11^{-1} A12 (unit lower kb x kb).
+__global__ __launch_bounds__(256) void getrf_swap_trsm_kernel(int n, int k, int kb, cd* A, long lda, long sA,
+                                                              const int* ipiv, long sP) {
+  const long b = blockIdx.y;
+  A += b * sA;
+  ipiv += b * sP;
+  __shared__ cd L[LU_NB][LU_NB];
+  __shared__ int pv[LU_NB];
+  const int t = threadIdx.x;
+  for (int e = t; e < kb * kb; e += 256) {
+    int i = e % kb, j = e / kb;
+    L[i][j] = A[(k + i) + (long)(k + j) * lda];
+  }
+  if (t < kb) pv[t] = ipiv[k + t];
+  __syncthreads();
+  const int idx = blockIdx.x * 256 + t;
+  if (idx >= n - kb) return;
+  const int col = idx < k ? idx : idx + kb;
+  cd* ac = A + (long)col * lda;
+  for (int j = 0; j < kb; ++j) {
+    int p = pv[j];
+    if (p != k + j) {
+      cd tmp = ac[k + j];
+      ac[k + j] = ac[p];
+      ac[p] = tmp;
+    }
+  }
+  if (col < k + kb) return;
+  cd x[LU_NB];
+#pragma unroll
+  for (int i = 0; i < LU_NB; ++i)
+    if (i < kb) x[i] = ac[k + i];
+#pragma unroll
+  for (int i = 0; i < LU_NB; ++i) {
+    if (i < kb) {
+#pragma unroll
+      for (int j = 0; j < LU_NB; ++j)
+        if (j < i) x[i] = csub(x[i], cmul(L[i][j], x[j]));
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < LU_NB; ++i)
+    if (i < kb) ac[k + i] = x[i];
+}
+
+void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, int* info, int batch) {
+  if (n <= 0 || batch <= 0) return;
+  for (int k = 0; k < n; k += LU_NB) {
+    const int kb = n - k < LU_NB ? n - k : LU_NB;
+    getrf_panel_kernel<<<batch, 256, 0, s>>>(n, k, kb, A, lda, sA, ipiv, sP, info);
+    launch_check("getrf_panel");
+    if (n - kb > 0) {
+      dim3 g(cdiv_up(n - kb, 256), batch);
+      getrf_swap_trsm_kernel<<<g, 256, 0, s>>>(n, k, kb, A, lda, sA, ipiv, sP);
+      launch_check("getrf_swap_trsm");
+    }
+    const int rest = n - k - kb;
+    if (rest > 0) {
+      zgemm_strided(s, OP_N, OP_N, rest, rest, kb, mk(-1.0, 0.0), A + (k + kb) + (long)k * lda, lda, sA,
+                    A + k + (long)(k + kb) * lda, lda, sA, mk(1.0, 0.0), A + (k + kb) + (long)(k + kb) * lda, lda,
+                    sA, batch);
+    }
+  }
+}
+
+// perm[i] = original row that ends up in row i after applying ipiv[0..n-1] in order.
+__global__ void ipiv_to_perm_kernel(int n, const int* ipiv, long sP, int* perm, long sPerm) {
+  const long b = blockIdx.x;
+  extern __shared__ int pm[];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) pm[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int* ip = ipiv + b * sP;
+    for (int i = 0; i < n; ++i) {
+      int p = ip[i];
+      if (p != i) {
+        int tmp = pm[i];
+        pm[i] = pm[p];
+        pm[p] = tmp;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) perm[b * sPerm + i] = pm[i];
+}
+
+__global__ void gather_rows_kernel(int n, int nrhs, const int* perm, long sPerm, const cd* B, long ldb, long sB,
+                                   cd* Y, long ldy, long sY) {
+  const long b = blockIdx.z;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= n) return;
+  Y[b * sY + r + (long)c * ldy] = B[b * sB + perm[b * sPerm + r] + (long)c * ldb];
+}
+
+// Solve the bs x bs diagonal block of a triangular factor against a 64-column tile
+// of the right-hand side held in LDS. UPPER=0: unit lower; UPPER=1: non-unit upper.
+template <int UPPER>
+__global__ __launch_bounds__(256) void trsm_diag_kernel(int k0, int bs, int nrhs, const cd* T, long ldt, long sT,
+                                                        cd* Y, long ldy, long sY) {
+  const long b = blockIdx.y;
+  T += b * sT;
+  Y += b * sY;
+  __shared__ cd Ls[TRS_NB][TRS_NB + 1];
+  __shared__ cd Ys[TRS_NB][TRS_NB + 1];
+  const int t = threadIdx.x;
+  const int c0 = blockIdx.x * TRS_NB;
+  for (int e = t; e < bs * bs; e += 256) {
+    int i = e % bs, j = e / bs;
+    Ls[i][j] = T[(k0 + i) + (long)(k0 + j) * ldt];
+  }
+  for (int e = t; e < bs * TRS_NB; e += 256) {
+    int i = e % bs, j = e / bs;
+    Ys[i][j] = (c0 + j < nrhs) ? Y[(k0 + i) + (long)(c0 + j) * ldy] : mk(0.0, 0.0);
+  }
+  __syncthreads();
+  const int col = t & 63, grp = t >> 6;
+  if (!UPPER) {
+    for (int i = 0; i < bs; ++i) {
+      const cd xi = Ys[i][col];
+      for (int j = i + 1 + grp; j < bs; j += 4) Ys[j][col] = csub(Ys[j][col], cmul(Ls[j][i], xi));
+      __syncthreads();
+    }
+  } else {
+    for (int i = bs - 1; i >= 0; --i) {
+      if (grp == 0) Ys[i][col] = cdiv(Ys[i][col], Ls[i][i]);
+      __syncthreads();
+      const cd xi = Ys[i][col];
+      for (int j = grp; j < i; j += 4) Ys[j][col] = csub(Ys[j][col], cmul(Ls[j][i], xi));
+      __syncthreads();
+    }
+  }
+  for (int e = t; e < bs * TRS_NB; e += 256) {
+    int i = e % bs, j = e / bs;
+    if (c0 + j < nrhs) Y[(k0 + i) + (long)(c0 + j) * ldy] = Ys[i][j];
+  }
+}
+
+// In-place triangular solve on Y (n x nrhs): UPPER=0 -> L (unit) ; UPPER=1 -> U (non-unit).
+static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* LU, long lda, long sA, cd* Y,
+                           long ldy, long sY, int batch) {
+  const int nblk = cdiv_up(n, TRS_NB);
+  dim3 g(cdiv_up(nrhs, TRS_NB), batch);
+  if (!upper) {
+    for (int kb = 0; kb < nblk; ++kb) {
+      const int k0 = kb * TRS_NB;
+      const int bs = n - k0 < TRS_NB ? n - k0 : TRS_NB;
+      trsm_diag_kernel<0><<<g, 256, 0, s>>>(k0, bs, nrhs, LU, lda, sA, Y, ldy, sY);
+      launch_check("trsm_diag_lower");
+      const int rest = n - k0 - bs;
+      if (rest > 0)
+        zgemm_strided(s, OP_N, OP_N, rest, nrhs, bs, mk(-1.0, 0.0), LU + (k0 + bs) + (long)k0 * lda, lda, sA,
+                      Y + k0, ldy, sY, mk(1.0, 0.0), Y + (k0 + bs), ldy, sY, batch);
+    }
+  } else {
+    for (int kb = nblk - 1; kb >= 0; --kb) {
+      const int k0 = kb * TRS_NB;
+      const int bs = n - k0 < TRS_NB ? n - k0 : TRS_NB;
+      trsm_diag_kernel<1><<<g, 256, 0, s>>>(k0, bs, nrhs, LU, lda, sA, Y, ldy, sY);
+      launch_check("trsm_diag_upper");
+      if (k0 > 0)
+        zgemm_strided(s, OP_N, OP_N, k0, nrhs, bs, mk(-1.0, 0.0), LU + (long)k0 * lda, lda, sA, Y + k0, ldy, sY,
+                      mk(1.0, 0.0), Y, ldy, sY, batch);
+    }
+  }
+}
+
+void zgetrs(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, const int* ipiv, long sP, const cd* B,
+            long ldb, long sB, cd* X, long ldx, long sX, int batch, Workspace& ws) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return;
+  DevBuf permb = ws.get(sizeof(int) * (size_t)n * batch);
+  int* perm = (int*)permb.ptr;
+  ipiv_to_perm_kernel<<<batch, 256, sizeof(int) * n, s>>>(n, ipiv, sP, perm, n);
+  launch_check("ipiv_to_perm");
+  // Gather into X directly when X does not alias B, otherwise through a temporary.
+  DevBuf tmpb;
+  cd* Y = X;
+  long ldy = ldx, sY = sX;
+  if (X == B) {
+    tmpb = ws.get(sizeof(cd) * (size_t)n * nrhs * batch);
+    Y = (cd*)tmpb.ptr;
+    ldy = n;
+    sY = (long)n * nrhs;
+  }
+  dim3 g(cdiv_up(n, 256), nrhs, batch);
+  gather_rows_kernel<<<g, 256, 0, s>>>(n, nrhs, perm, n, B, ldb, sB, Y, ldy, sY);
+  launch_check("gather_rows");
+  trsm_lu_factor(s, 0, n, nrhs, LU, lda, sA, Y, ldy, sY, batch);
+  trsm_lu_factor(s, 1, n, nrhs, LU, lda, sA, Y, ldy, sY, batch);
+  if (Y != X) zgeam(s, n, nrhs, mk(1.0, 0.0), Y, ldy, sY, mk(0.0, 0.0), nullptr, 0, 0, X, ldx, sX, batch);
+}
+
+void zgetri(hipStream_t s, int n, const cd* LU, long lda, long sA, const int* ipiv, long sP, cd* Ainv, long ldi,
+            long sI, int batch, Workspace& ws) {
+  DevBuf idb = ws.get(sizeof(cd) * (size_t)n * n * batch);
+  cd* I = (cd*)idb.ptr;
+  zset(s, n, n, mk(1.0, 0.0), mk(0.0, 0.0), I, n, (long)n * n, batch);
+  zgetrs(s, n, n, LU, lda, sA, ipiv, sP, I, n, (long)n * n, Ainv, ldi, sI, batch, ws);
+}
+
+}  // namespace vrc

@@ -1,0 +1,415 @@
+// Parts (b) and (d): operator assembly, eigen-system post-processing, layer S-matrix,
+// Redheffer star product, end projections and the varRCWA perturbation, as batched
+// device procedures built on the dense kernels (ZGEMM, LU) plus fused elementwise
+// kernels defined here.
+#include "vrc_ops.hpp"
+
+namespace vrc {
+
+// ===================================================================== small kernels
+// Periodic-boundary P and Q (Kx = diag(alpha), Ky = diag(beta)), src/rcwa/Layer.cpp:95-100,134-138:
+//  P = [[a inv33 b, k0^2 I - a inv33 a], [b inv33 b - k0^2 I, -b inv33 a]]
+//  Q = [[-a b, a^2 - k0^2 fe22], [-b^2 + k0^2 fe11, b a]]
+// with (a X b)[r][c] = alpha_r X[r][c] beta_c; per-layer alpha/beta/k0^2.
+__global__ void pq_periodic_kernel(int n, const cd* inv33, const cd* fe11, const cd* fe22, long sfe,
+                                   const cd* alpha, const cd* beta, long sab, const double* k0sq, cd* P, cd* Q,
+                                   long sPQ) {
+  const long l = blockIdx.z;
+  const int N = 2 * n;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= N) return;
+  const int rb = r / n, ri = r % n, cb = c / n, ci = c % n;
+  const cd* al = alpha + l * sab;
+  const cd* be = beta + l * sab;
+  const double k2 = k0sq[l];
+  const long e = l * sfe + ri + (long)ci * n;
+  const cd inv = inv33[e];
+  const cd ar = al[ri], br = be[ri], ac = al[ci], bc = be[ci];
+  const bool dg = (ri == ci);
+  cd p, q;
+  if (rb == 0 && cb == 0) {
+    p = cmul(cmul(ar, inv), bc);
+    q = dg ? cneg(cmul(ar, br)) : mk(0.0, 0.0);
+  } else if (rb == 0) {
+    p = cneg(cmul(cmul(ar, inv), ac));
+    if (dg) p.x += k2;
+    q = cscale(fe22[e], -k2);
+    if (dg) q = cadd(q, cmul(ar, ar));
+  } else if (cb == 0) {
+    p = cmul(cmul(br, inv), bc);
+    if (dg) p.x -= k2;
+    q = cscale(fe11[e], k2);
+    if (dg) q = csub(q, cmul(br, br));
+  } else {
+    p = cneg(cmul(cmul(br, inv), ac));
+    q = dg ? cmul(br, ar) : mk(0.0, 0.0);
+  }
+  const long o = l * sPQ + r + (long)c * N;
+  P[o] = p;
+  Q[o] = q;
+}
+
+// X_i = exp(i * gamma_i * s) with s = dz/k0 per batch entry (src/rcwa/RCWAScatterMatrix.cpp:36-40).
+__global__ void phase_kernel(int n, const cd* gam, long sg, const double* s, cd* X, long sx) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cd g = gam[b * sg + i];
+  const double ss = s[b];
+  // exp(i*g*ss) = exp(-Im(g)*ss) * (cos(Re(g)*ss) + i sin(Re(g)*ss))
+  X[b * sx + i] = cexp(mk(-g.y * ss, g.x * ss));
+}
+
+// dst[b] = src[idx[b]] (n x n, ld n), used to give every slice its instance's reference basis.
+__global__ void gather_mats_kernel(long nn, const cd* src, long ss, const int* idx, cd* dst, long sd) {
+  const long b = blockIdx.y;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= nn) return;
+  dst[b * sd + e] = src[(long)idx[b] * ss + e];
+}
+
+// Simpson combination of the varRCWA segment (src/core/RedhefferIntegrator.cpp:336-397):
+// per entry (r,c): with ph, pf the half/full phases and coefficient c0 = i*h/(12 k0)
+//   tuu = c0 (pf_r dA0 + 4 ph_r dA1 ph_c + dA2 pf_c) (+ pf on the diagonal)
+//   rud = -c0 (pf_r dB0 pf_c + 4 ph_r dB1 ph_c + dB2)
+//   rdu = -c0 (dB0 + 4 ph_r dB1 ph_c + pf_r dB2 pf_c)
+//   tdd = c0 (dA0 pf_c + 4 ph_r dA1 ph_c + pf_r dA2) (+ pf on the diagonal)
+// and the per-block max |perturbation| (before the diagonal is added) into err[4*blocks].
+__global__ __launch_bounds__(256) void simpson_kernel(int n, const cd* dA0, const cd* dA1, const cd* dA2,
+                                                      const cd* dB0, const cd* dB1, const cd* dB2, const cd* ph,
+                                                      const cd* pf, cd c0, cd* tdd, cd* tuu, cd* rud, cd* rdu,
+                                                      double* err) {
+  __shared__ double red[4][256];
+  const long total = (long)n * n;
+  double m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256L) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    const cd phr = ph[r], phc = ph[c], pfr = pf[r], pfc = pf[c];
+    const cd h4 = cscale(cmul(phr, phc), 4.0);
+    cd a = cadd(cadd(cmul(pfr, dA0[e]), cmul(h4, dA1[e])), cmul(dA2[e], pfc));
+    a = cmul(c0, a);
+    m0 = fmax(m0, cabsv(a));
+    if (r == c) a = cadd(a, pfr);
+    tuu[e] = a;
+    cd b = cadd(cadd(cmul(cmul(pfr, dB0[e]), pfc), cmul(h4, dB1[e])), dB2[e]);
+    b = cneg(cmul(c0, b));
+    m1 = fmax(m1, cabsv(b));
+    rud[e] = b;
+    cd d = cadd(cadd(dB0[e], cmul(h4, dB1[e])), cmul(cmul(pfr, dB2[e]), pfc));
+    d = cneg(cmul(c0, d));
+    m2 = fmax(m2, cabsv(d));
+    rdu[e] = d;
+    cd f = cadd(cadd(cmul(dA0[e], pfc), cmul(h4, dA1[e])), cmul(pfr, dA2[e]));
+    f = cmul(c0, f);
+    m3 = fmax(m3, cabsv(f));
+    if (r == c) f = cadd(f, pfr);
+    tdd[e] = f;
+  }
+  const int t = threadIdx.x;
+  red[0][t] = fmax(fmax(m0, m1), fmax(m2, m3));
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[0][t] = fmax(red[0][t], red[0][t + w]);
+    __syncthreads();
+  }
+  if (t == 0) err[blockIdx.x] = red[0][0];
+}
+
+// ===================================================================== helpers
+static const cd ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
+
+void gather_mats(hipStream_t s, long nn, const cd* src, long ss, const int* d_idx, cd* dst, long sd, int batch) {
+  if (batch <= 0) return;
+  dim3 g(cdiv_up(nn, 256), batch);
+  gather_mats_kernel<<<g, 256, 0, s>>>(nn, src, ss, d_idx, dst, sd);
+  launch_check("gather_mats");
+}
+
+static void mm(hipStream_t s, int N, cd alpha, const cd* A, const cd* B, cd beta, cd* C, long stride, int batch) {
+  zgemm_strided(s, OP_N, OP_N, N, N, N, alpha, A, N, stride, B, N, stride, beta, C, N, stride, batch);
+}
+
+static void copy(hipStream_t s, int m, int n, const cd* A, cd* C, long stride, int batch, cd a = ONE) {
+  zgeam(s, m, n, a, A, m, stride, ZERO, nullptr, 0, 0, C, m, stride, batch);
+}
+
+// ===================================================================== (b) assembly
+void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q) {
+  hipStream_t s = c.stream;
+  const int n = a.nx * a.ny, N = 2 * n, L = a.nlayers;
+  const long nn = (long)n * n, NN = (long)N * N;
+  DevBuf fe = c.ws.get(sizeof(cd) * 3 * (size_t)nn * L);
+  cd* fe11 = fe.as<cd>();
+  cd* fe22 = fe11 + nn * L;
+  cd* fe33 = fe22 + nn * L;
+  assemble_fe(c, a, fe11, fe22, fe33, nn);
+  DevBuf piv = c.ws.get(sizeof(int) * (size_t)n * L);
+  DevBuf inf = c.ws.get(sizeof(int) * (size_t)L);
+  fill_int(s, inf.as<int>(), 0, L);
+  zgetrf(s, n, fe33, n, nn, piv.as<int>(), n, inf.as<int>(), L);
+  if (a.periodic) {
+    DevBuf inv = c.ws.get(sizeof(cd) * (size_t)nn * L);
+    zgetri(s, n, fe33, n, nn, piv.as<int>(), n, inv.as<cd>(), n, nn, L, c.ws);
+    dim3 g(cdiv_up(N, 256), N, L);
+    pq_periodic_kernel<<<g, 256, 0, s>>>(n, inv.as<cd>(), fe11, fe22, nn, a.d_alpha, a.d_beta, a.sab, a.d_k0sq, P,
+                                         Q, NN);
+    launch_check("pq_periodic");
+  } else {
+    // F = fe33^{-1} [Kx | Ky]  (n x 2n), then P blocks by ZGEMM against the dense K
+    DevBuf Fb = c.ws.get(sizeof(cd) * (size_t)nn * 2 * L);
+    cd* F = Fb.as<cd>();
+    DevBuf Kb = c.ws.get(sizeof(cd) * (size_t)nn * 2);
+    VRC_HIP(hipMemcpyAsync(Kb.as<cd>(), a.d_Kx, sizeof(cd) * nn, hipMemcpyDeviceToDevice, s));
+    VRC_HIP(hipMemcpyAsync(Kb.as<cd>() + nn, a.d_Ky, sizeof(cd) * nn, hipMemcpyDeviceToDevice, s));
+    zgetrs(s, n, 2 * n, fe33, n, nn, piv.as<int>(), n, Kb.as<cd>(), n, 0, F, n, 2 * nn, L, c.ws);
+    const cd* F1h = F;
+    const cd* F2h = F + nn;
+    const long sF = 2 * nn;
+    // P11 = Kx F2h ; P12 = -Kx F1h + k0^2 I ; P21 = Ky F2h - k0^2 I ; P22 = -Ky F1h
+    zgemm_strided(s, OP_N, OP_N, n, n, n, ONE, a.d_Kx, n, 0, F2h, n, sF, ZERO, P, N, NN, L);
+    zgemm_strided(s, OP_N, OP_N, n, n, n, MONE, a.d_Kx, n, 0, F1h, n, sF, ZERO, P + (long)n * N, N, NN, L);
+    zgemm_strided(s, OP_N, OP_N, n, n, n, ONE, a.d_Ky, n, 0, F2h, n, sF, ZERO, P + n, N, NN, L);
+    zgemm_strided(s, OP_N, OP_N, n, n, n, MONE, a.d_Ky, n, 0, F1h, n, sF, ZERO, P + n + (long)n * N, N, NN, L);
+    const double k2 = a.k0sq_uniform;
+    zadd_diag(s, n, mk(k2, 0.0), nullptr, 0, P + (long)n * N, N, NN, L);
+    zadd_diag(s, n, mk(-k2, 0.0), nullptr, 0, P + n, N, NN, L);
+    // Q11 = -KxKy ; Q12 = KxKx - k0^2 fe22 ; Q21 = -KyKy + k0^2 fe11 ; Q22 = KyKx
+    zgeam(s, n, n, MONE, a.d_KxKy, n, 0, ZERO, nullptr, 0, 0, Q, N, NN, L);
+    zgeam(s, n, n, ONE, a.d_KxKx, n, 0, mk(-k2, 0.0), fe22, n, nn, Q + (long)n * N, N, NN, L);
+    zgeam(s, n, n, MONE, a.d_KyKy, n, 0, mk(k2, 0.0), fe11, n, nn, Q + n, N, NN, L);
+    zgeam(s, n, n, ONE, a.d_KyKx, n, 0, ZERO, nullptr, 0, 0, Q + n + (long)n * N, N, NN, L);
+  }
+}
+
+// ===================================================================== (c) wave equation
+void wave_eqn(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* W, cd* V, cd* gam, int* info, int batch) {
+  hipStream_t s = c.stream;
+  const long NN = (long)N * N;
+  DevBuf PQb = c.ws.get(sizeof(cd) * (size_t)NN * batch);
+  DevBuf lamb = c.ws.get(sizeof(cd) * (size_t)N * batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, P, N, sPQ, Q, N, sPQ, ZERO, PQb.as<cd>(), N, NN, batch);
+  zgeev_batched(s, N, PQb.as<cd>(), N, NN, lamb.as<cd>(), N, W, N, NN, info, batch, c.ws);
+  gamma_sqrt(s, N, lamb.as<cd>(), N, gam, N, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, Q, N, sPQ, W, N, NN, ZERO, V, N, NN, batch);
+  zdscale(s, N, N, nullptr, 0, gam, N, 1, V, N, NN, V, N, NN, batch);
+}
+
+// ===================================================================== (d) layer S-matrix
+void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
+                   const cd* W0, const cd* V0, long sRef, cd* Tuu, cd* Rud) {
+  hipStream_t s = c.stream;
+  const long NN = (long)N * N;
+  DevBuf LUb = c.ws.get(sizeof(cd) * (size_t)NN * batch);
+  DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * batch);
+  DevBuf infb = c.ws.get(sizeof(int) * (size_t)batch);
+  DevBuf F0b = c.ws.get(sizeof(cd) * (size_t)NN * batch);
+  DevBuf G0b = c.ws.get(sizeof(cd) * (size_t)NN * batch);
+  cd* LU = LUb.as<cd>();
+  int* piv = pivb.as<int>();
+  cd* F0 = F0b.as<cd>();
+  cd* G0 = G0b.as<cd>();
+  fill_int(s, infb.as<int>(), 0, batch);
+  // F0 = W^{-1} W0 ; G0 = V^{-1} V0
+  copy(s, N, N, W, LU, NN, batch);
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrs(s, N, N, LU, N, NN, piv, N, W0, N, sRef, F0, N, NN, batch, c.ws);
+  copy(s, N, N, V, LU, NN, batch);
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrs(s, N, N, LU, N, NN, piv, N, V0, N, sRef, G0, N, NN, batch, c.ws);
+  // A = F0 + G0 (into LU, factored); RHS = [B | X A | X B] with B = F0 - G0
+  DevBuf Xb = c.ws.get(sizeof(cd) * (size_t)N * batch);
+  {
+    dim3 g(cdiv_up(N, 256), batch);
+    phase_kernel<<<g, 256, 0, s>>>(N, gam, N, d_s, Xb.as<cd>(), N);
+    launch_check("phase");
+  }
+  DevBuf R3b = c.ws.get(sizeof(cd) * (size_t)NN * 3 * batch);
+  cd* R3 = R3b.as<cd>();
+  const long s3 = 3 * NN;
+  zgeam(s, N, N, ONE, F0, N, NN, ONE, G0, N, NN, LU, N, NN, batch);              // A
+  zgeam(s, N, N, ONE, F0, N, NN, MONE, G0, N, NN, R3, N, s3, batch);             // B
+  zdscale(s, N, N, Xb.as<cd>(), N, nullptr, 0, 0, LU, N, NN, R3 + NN, N, s3, batch);      // X A
+  zdscale(s, N, N, Xb.as<cd>(), N, nullptr, 0, 0, R3, N, s3, R3 + 2 * NN, N, s3, batch);  // X B
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrs(s, N, 3 * N, LU, N, NN, piv, N, R3, N, s3, R3, N, s3, batch, c.ws);
+  const cd* invAB = R3;
+  const cd* invAXA = R3 + NN;
+  const cd* invAXB = R3 + 2 * NN;
+  // M = I - invAXB^2 ; RHS2 = [invAXB invAXA - invAB | invAXA - invAXB invAB]
+  zset(s, N, N, ONE, ZERO, LU, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, invAXB, N, s3, invAXB, N, s3, ONE, LU, N, NN, batch);
+  DevBuf R2b = c.ws.get(sizeof(cd) * (size_t)NN * 2 * batch);
+  cd* R2 = R2b.as<cd>();
+  const long s2 = 2 * NN;
+  zgeam(s, N, N, ONE, invAB, N, s3, ZERO, nullptr, 0, 0, R2, N, s2, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, invAXB, N, s3, invAXA, N, s3, MONE, R2, N, s2, batch);
+  zgeam(s, N, N, ONE, invAXA, N, s3, ZERO, nullptr, 0, 0, R2 + NN, N, s2, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, invAXB, N, s3, invAB, N, s3, ONE, R2 + NN, N, s2, batch);
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrs(s, N, 2 * N, LU, N, NN, piv, N, R2, N, s2, R2, N, s2, batch, c.ws);
+  zgeam(s, N, N, ONE, R2, N, s2, ZERO, nullptr, 0, 0, Rud, N, NN, batch);
+  zgeam(s, N, N, ONE, R2 + NN, N, s2, ZERO, nullptr, 0, 0, Tuu, N, NN, batch);
+}
+
+// ===================================================================== (d) star product
+void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
+                  const cd* rdu, long ss) {
+  hipStream_t s = c.stream;
+  const long NN = (long)N * N;
+  const size_t B = sizeof(cd) * (size_t)NN * batch;
+  DevBuf C1b = c.ws.get(B), RC1b = c.ws.get(B), C2b = c.ws.get(B), T1b = c.ws.get(B), T2b = c.ws.get(B);
+  DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * batch), infb = c.ws.get(sizeof(int) * (size_t)batch);
+  cd *C1 = C1b.as<cd>(), *RC1 = RC1b.as<cd>(), *C2 = C2b.as<cd>(), *T1 = T1b.as<cd>(), *T2 = T2b.as<cd>();
+  fill_int(s, infb.as<int>(), 0, batch);
+  // invC1 = I - rdu Rud ; C1 = invC1^{-1}  (src/core/RCWAIntegrator.cpp:57-60)
+  zset(s, N, N, ONE, ZERO, T1, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, rdu, N, ss, S.Rud, N, NN, ONE, T1, N, NN, batch);
+  zgetrf(s, N, T1, N, NN, pivb.as<int>(), N, infb.as<int>(), batch);
+  zgetri(s, N, T1, N, NN, pivb.as<int>(), N, C1, N, NN, batch, c.ws);
+  // RC1 = Rud C1 ; C2 = I + RC1 rdu
+  mm(s, N, ONE, S.Rud, C1, ZERO, RC1, NN, batch);
+  zset(s, N, N, ONE, ZERO, C2, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, RC1, N, NN, rdu, N, ss, ONE, C2, N, NN, batch);
+  // Rdu += Tdd rdu C2 Tuu
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, S.Tdd, N, NN, rdu, N, ss, ZERO, T1, N, NN, batch);
+  mm(s, N, ONE, T1, C2, ZERO, T2, NN, batch);
+  mm(s, N, ONE, T2, S.Tuu, ONE, S.Rdu, NN, batch);
+  // Tdd = Tdd C1 tdd
+  mm(s, N, ONE, S.Tdd, C1, ZERO, T1, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, T1, N, NN, tdd, N, ss, ZERO, S.Tdd, N, NN, batch);
+  // Rud = rud + tuu Rud C1 tdd
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, tuu, N, ss, RC1, N, NN, ZERO, T1, N, NN, batch);
+  zgeam(s, N, N, ONE, rud, N, ss, ZERO, nullptr, 0, 0, S.Rud, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, T1, N, NN, tdd, N, ss, ONE, S.Rud, N, NN, batch);
+  // Tuu = tuu C2 Tuu
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, tuu, N, ss, C2, N, NN, ZERO, T1, N, NN, batch);
+  mm(s, N, ONE, T1, S.Tuu, ZERO, T2, NN, batch);
+  copy(s, N, N, T2, S.Tuu, NN, batch);
+}
+
+// ===================================================================== (d) projections
+// Left end (src/core/RCWAIntegrator.cpp:73-86): S expressed with basis (W1,V1) on the left
+// is re-expressed with (W0,V0). LU1 holds the factorizations of W1 and V1.
+void project_left(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LU1, const cd* W0, const cd* V0,
+                  long sRef) {
+  hipStream_t s = c.stream;
+  const long NN = (long)N * N;
+  const size_t B = sizeof(cd) * (size_t)NN * batch;
+  DevBuf Fb = c.ws.get(B), Gb = c.ws.get(B), Kb = c.ws.get(B), R2b = c.ws.get(2 * B), Tb = c.ws.get(B);
+  DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * batch), infb = c.ws.get(sizeof(int) * (size_t)batch);
+  cd *F = Fb.as<cd>(), *G = Gb.as<cd>(), *K = Kb.as<cd>(), *R2 = R2b.as<cd>(), *T = Tb.as<cd>();
+  const long s2 = 2 * NN;
+  fill_int(s, infb.as<int>(), 0, batch);
+  zgetrs(s, N, N, LU1.W, N, LU1.sM, LU1.pW, N, W0, N, sRef, F, N, NN, batch, c.ws);
+  zgetrs(s, N, N, LU1.V, N, LU1.sM, LU1.pV, N, V0, N, sRef, G, N, NN, batch, c.ws);
+  // A = (F + G)/2 -> F ; B = (F - G)/2 -> G   (in place, element-wise)
+  zgeam(s, N, N, mk(0.5, 0), F, N, NN, mk(0.5, 0), G, N, NN, K, N, NN, batch);   // K = A
+  zgeam(s, N, N, mk(0.5, 0), F, N, NN, mk(-0.5, 0), G, N, NN, G, N, NN, batch);  // G = B
+  copy(s, N, N, K, F, NN, batch);                                                // F = A
+  // K = A - Rdu B
+  mm(s, N, MONE, S.Rdu, G, ONE, K, NN, batch);
+  // RHS = [Rdu A - B | Tdd]
+  zgeam(s, N, N, MONE, G, N, NN, ZERO, nullptr, 0, 0, R2, N, s2, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, S.Rdu, N, NN, F, N, NN, ONE, R2, N, s2, batch);
+  zgeam(s, N, N, ONE, S.Tdd, N, NN, ZERO, nullptr, 0, 0, R2 + NN, N, s2, batch);
+  zgetrf(s, N, K, N, NN, pivb.as<int>(), N, infb.as<int>(), batch);
+  zgetrs(s, N, 2 * N, K, N, NN, pivb.as<int>(), N, R2, N, s2, R2, N, s2, batch, c.ws);
+  zgeam(s, N, N, ONE, R2, N, s2, ZERO, nullptr, 0, 0, S.Rdu, N, NN, batch);
+  zgeam(s, N, N, ONE, R2 + NN, N, s2, ZERO, nullptr, 0, 0, S.Tdd, N, NN, batch);
+  // Rud = Tuu B Tdd + Rud
+  mm(s, N, ONE, S.Tuu, G, ZERO, T, NN, batch);
+  mm(s, N, ONE, T, S.Tdd, ONE, S.Rud, NN, batch);
+  // Tuu = Tuu (B Rdu + A)
+  mm(s, N, ONE, G, S.Rdu, ONE, F, NN, batch);
+  mm(s, N, ONE, S.Tuu, F, ZERO, T, NN, batch);
+  copy(s, N, N, T, S.Tuu, NN, batch);
+}
+
+// Right end (src/core/RedhefferIntegrator.cpp:211-223): S whose right basis is
+// (Wl, Vl) (factorized in LUl) re-expressed with (W1, V1).
+void project_right(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LUl, const cd* W1, const cd* V1,
+                   long sRef) {
+  hipStream_t s = c.stream;
+  const long NN = (long)N * N;
+  const size_t B = sizeof(cd) * (size_t)NN * batch;
+  DevBuf Fb = c.ws.get(B), Gb = c.ws.get(B), Kb = c.ws.get(B), R2b = c.ws.get(2 * B), Tb = c.ws.get(B);
+  DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * batch), infb = c.ws.get(sizeof(int) * (size_t)batch);
+  cd *F = Fb.as<cd>(), *G = Gb.as<cd>(), *K = Kb.as<cd>(), *R2 = R2b.as<cd>(), *T = Tb.as<cd>();
+  const long s2 = 2 * NN;
+  fill_int(s, infb.as<int>(), 0, batch);
+  zgetrs(s, N, N, LUl.W, N, LUl.sM, LUl.pW, N, W1, N, sRef, F, N, NN, batch, c.ws);
+  zgetrs(s, N, N, LUl.V, N, LUl.sM, LUl.pV, N, V1, N, sRef, G, N, NN, batch, c.ws);
+  zgeam(s, N, N, mk(0.5, 0), F, N, NN, mk(0.5, 0), G, N, NN, K, N, NN, batch);   // A
+  zgeam(s, N, N, mk(0.5, 0), F, N, NN, mk(-0.5, 0), G, N, NN, G, N, NN, batch);  // B
+  copy(s, N, N, K, F, NN, batch);                                                // F = A
+  // K = A - Rud B
+  mm(s, N, MONE, S.Rud, G, ONE, K, NN, batch);
+  // [Tuu | Rud A - B]
+  zgeam(s, N, N, ONE, S.Tuu, N, NN, ZERO, nullptr, 0, 0, R2, N, s2, batch);
+  zgeam(s, N, N, MONE, G, N, NN, ZERO, nullptr, 0, 0, R2 + NN, N, s2, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, S.Rud, N, NN, F, N, NN, ONE, R2 + NN, N, s2, batch);
+  zgetrf(s, N, K, N, NN, pivb.as<int>(), N, infb.as<int>(), batch);
+  zgetrs(s, N, 2 * N, K, N, NN, pivb.as<int>(), N, R2, N, s2, R2, N, s2, batch, c.ws);
+  zgeam(s, N, N, ONE, R2, N, s2, ZERO, nullptr, 0, 0, S.Tuu, N, NN, batch);
+  zgeam(s, N, N, ONE, R2 + NN, N, s2, ZERO, nullptr, 0, 0, S.Rud, N, NN, batch);
+  // Rdu = Rdu + Tdd B Tuu
+  mm(s, N, ONE, S.Tdd, G, ZERO, T, NN, batch);
+  mm(s, N, ONE, T, S.Tuu, ONE, S.Rdu, NN, batch);
+  // Tdd = Tdd (B Rud + A)
+  mm(s, N, ONE, G, S.Rud, ONE, F, NN, batch);
+  mm(s, N, ONE, S.Tdd, F, ZERO, T, NN, batch);
+  copy(s, N, N, T, S.Tdd, NN, batch);
+}
+
+// ===================================================================== (d) varRCWA segment
+// Perturbation of one segment [z0, z1] around the reference (Pr, Qr, W, V, gamma):
+// produces (tdd, tuu, rud, rdu) and the max-modulus error estimate (returned on host).
+double redheffer_perturbation(vrc_ctx& c, int N, double k0, double z0, double z1, const cd* Pr, const cd* Qr,
+                              const RefSys& R, const cd* P3[3], const cd* Q3[3], cd* tdd, cd* tuu, cd* rud,
+                              cd* rdu) {
+  hipStream_t s = c.stream;
+  const long NN = (long)N * N;
+  DevBuf dAb = c.ws.get(sizeof(cd) * 6 * (size_t)NN);
+  DevBuf tmpb = c.ws.get(sizeof(cd) * 2 * (size_t)NN);
+  cd* dA = dAb.as<cd>();       // dA0 dA1 dA2
+  cd* dB = dA + 3 * NN;        // dB0 dB1 dB2
+  cd* tmp = tmpb.as<cd>();
+  for (int k = 0; k < 3; ++k) {
+    // a = W^{-1} (P_k - Pr) V ; b = V^{-1} (Q_k - Qr) W
+    zgeam(s, N, N, ONE, P3[k], N, 0, MONE, Pr, N, 0, tmp, N, 0, 1);
+    mm(s, N, ONE, tmp, R.V, ZERO, tmp + NN, 0, 1);
+    zgetrs(s, N, N, R.LU.W, N, 0, R.LU.pW, N, tmp + NN, N, 0, dA + k * NN, N, 0, 1, c.ws);
+    zgeam(s, N, N, ONE, Q3[k], N, 0, MONE, Qr, N, 0, tmp, N, 0, 1);
+    mm(s, N, ONE, tmp, R.W, ZERO, tmp + NN, 0, 1);
+    zgetrs(s, N, N, R.LU.V, N, 0, R.LU.pV, N, tmp + NN, N, 0, dB + k * NN, N, 0, 1, c.ws);
+    // dA = a + b ; dB = a - b
+    zgeam(s, N, N, ONE, dA + k * NN, N, 0, ONE, dB + k * NN, N, 0, tmp, N, 0, 1);
+    zgeam(s, N, N, ONE, dA + k * NN, N, 0, MONE, dB + k * NN, N, 0, dB + k * NN, N, 0, 1);
+    copy(s, N, N, tmp, dA + k * NN, 0, 1);
+  }
+  DevBuf phb = c.ws.get(sizeof(cd) * 2 * (size_t)N);
+  DevBuf sb = c.ws.get(sizeof(double) * 2);
+  const double hs[2] = {0.5 * (z1 - z0) / k0, (z1 - z0) / k0};
+  VRC_HIP(hipMemcpyAsync(sb.as<double>(), hs, sizeof(hs), hipMemcpyHostToDevice, s));
+  {
+    dim3 g(cdiv_up(N, 256), 1);
+    phase_kernel<<<g, 256, 0, s>>>(N, R.gam, 0, sb.as<double>(), phb.as<cd>(), 0);
+    phase_kernel<<<g, 256, 0, s>>>(N, R.gam, 0, sb.as<double>() + 1, phb.as<cd>() + N, 0);
+    launch_check("phase");
+  }
+  const int nblk = 256;
+  DevBuf errb = c.ws.get(sizeof(double) * nblk);
+  const cd c0 = mk(0.0, (z1 - z0) / (6.0 * 2.0 * k0));
+  simpson_kernel<<<nblk, 256, 0, s>>>(N, dA, dA + NN, dA + 2 * NN, dB, dB + NN, dB + 2 * NN, phb.as<cd>(),
+                                     phb.as<cd>() + N, c0, tdd, tuu, rud, rdu, errb.as<double>());
+  launch_check("simpson");
+  std::vector<double> h(nblk);
+  VRC_HIP(hipMemcpyAsync(h.data(), errb.as<double>(), sizeof(double) * nblk, hipMemcpyDeviceToHost, s));
+  VRC_HIP(hipStreamSynchronize(s));
+  double e = 0.0;
+  for (double v : h) e = v > e ? v : e;
+  return e;
+}
+
+}  // namespace vrc

@@ -1,0 +1,38 @@
+// Layer geometry (host form) and the device view used by the Fourier assembly.
+#pragma once
+
+#include <vector>
+
+#include "vrc_common.hpp"
+
+namespace vrc {
+
+// A layer in the reference's Layer(x, y, eps) form (src/rcwa/Layer.cpp:18-34):
+// x: nCx coordinates, y: nCy coordinates, eps: (nCy-1) x (nCx-1) column-major
+// (row i = y strip [y_i, y_{i+1}], column j = x cell [x_j, x_{j+1}]).
+struct HostLayer {
+  std::vector<double> x, y;
+  std::vector<cd> eps;
+  int ncx() const { return (int)x.size() - 1; }
+  int ncy() const { return (int)y.size() - 1; }
+  cd at(int i, int j) const { return eps[(size_t)i + (size_t)j * ncy()]; }
+};
+
+// Device arrays for a batch of layers (capacities MX cells, MY strips per layer).
+struct FourierDev {
+  int nx, ny;
+  long MX, MY;
+  const int* ncx;
+  const int* ncy;
+  const double* Lx;
+  const double* Ly;
+  const double* xu;  // [l][MX+1] coordinates in the given order (Fourier modes X and XY)
+  const double* xs;  // [l][MX+1] sorted coordinates (mode Y intervals)
+  const double* ys;  // [l][MY+1]
+  const cd* val;     // [l][MY][MX] painter-order (refilled) permittivity, sorted cell order
+  cd *FX, *FXi, *FY1, *FYi, *FX1, *tFFx, *tFFy, *F2, *G11, *G22;
+};
+
+void fourier_assemble(hipStream_t s, const FourierDev& g, int nlayers, cd* fe11, cd* fe22, cd* fe33, long sfe);
+
+}  // namespace vrc

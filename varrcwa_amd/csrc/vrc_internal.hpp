@@ -1,0 +1,92 @@
+// Internal (C++) interfaces shared between the kernel files and the C ABI layer.
+#pragma once
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "vrc_common.hpp"
+
+namespace vrc {
+
+class Workspace;
+
+// A device buffer leased from a Workspace; returned to the pool when destroyed.
+// All work of one context runs on one stream, so a buffer returned here can be
+// reused by later launches on that stream without synchronization.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  Workspace* owner = nullptr;
+  DevBuf() = default;
+  DevBuf(void* p, size_t b, Workspace* o) : ptr(p), bytes(b), owner(o) {}
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes), owner(o.owner) { o.owner = nullptr; o.ptr = nullptr; }
+  DevBuf& operator=(DevBuf&& o) noexcept;
+  ~DevBuf();
+  template <class T>
+  T* as() const { return (T*)ptr; }
+};
+
+class Workspace {
+ public:
+  Workspace() = default;
+  ~Workspace();
+  DevBuf get(size_t bytes);
+  void put(void* p, size_t bytes);
+  size_t reserved() const { return reserved_; }
+  void trim();  // release cached blocks (caller must have synchronized the stream)
+
+ private:
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> owned_;
+  size_t reserved_ = 0;
+};
+
+inline DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
+  if (this != &o) {
+    if (owner) owner->put(ptr, bytes);
+    ptr = o.ptr; bytes = o.bytes; owner = o.owner;
+    o.owner = nullptr; o.ptr = nullptr;
+  }
+  return *this;
+}
+inline DevBuf::~DevBuf() {
+  if (owner) owner->put(ptr, bytes);
+}
+
+// ---------------------------------------------------------------- blas.hip
+void zgemm_strided(hipStream_t s, int opA, int opB, int m, int n, int k, cd alpha, const cd* A, long lda, long sA,
+                   const cd* B, long ldb, long sB, cd beta, cd* C, long ldc, long sC, int batch);
+void zgemm_desc(hipStream_t s, int opA, int opB, const GemmDesc* d_desc, int count, int max_m, int max_n);
+void zgeam(hipStream_t s, int m, int n, cd a, const cd* A, long lda, long sA, cd b, const cd* B, long ldb, long sB,
+           cd* C, long ldc, long sC, int batch);
+void zset(hipStream_t s, int m, int n, cd diag, cd off, cd* C, long ldc, long sC, int batch);
+void zadd_diag(hipStream_t s, int n, cd v, const cd* vec, long svec, cd* C, long ldc, long sC, int batch);
+void zdscale(hipStream_t s, int m, int n, const cd* left, long sl, const cd* right, long sr, int inv_right,
+             const cd* A, long lda, long sA, cd* C, long ldc, long sC, int batch);
+void zmaxabs(hipStream_t s, int m, int n, const cd* A, long lda, long sA, double* out, int batch);
+
+// ---------------------------------------------------------------- lu.hip
+void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, int* info, int batch);
+void zgetrs(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, const int* ipiv, long sP, const cd* B,
+            long ldb, long sB, cd* X, long ldx, long sX, int batch, Workspace& ws);
+void zgetri(hipStream_t s, int n, const cd* LU, long lda, long sA, const int* ipiv, long sP, cd* Ainv, long ldi,
+            long sI, int batch, Workspace& ws);
+
+// ---------------------------------------------------------------- eig.hip
+// Right eigen-decomposition of a strided batch of general n x n matrices
+// (A is destroyed). w: n eigenvalues per entry; VR: unit-2-norm eigenvectors with
+// the largest component real (LAPACK zgeev convention); info per entry
+// (0 ok, k>0: QR iteration failed with eigenvalues k..n unconverged).
+void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long sw, cd* VR, long ldv, long sV,
+                   int* info, int batch, Workspace& ws);
+
+// gamma = sqrt(lambda) with the reference's branch rule.
+void gamma_sqrt(hipStream_t s, int n, const cd* lam, long sl, cd* gam, long sg, int batch);
+
+// ---------------------------------------------------------------- fourier.hip / operators
+struct LayerGeom;  // defined in vrc_geom.hpp
+
+}  // namespace vrc

@@ -1,0 +1,178 @@
+// Batched device procedures of the S-matrix path and the host-side sampler /
+// integrator classes that keep the reference's API.
+#pragma once
+
+#include <functional>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "vrc_context.hpp"
+#include "vrc_geom.hpp"
+#include "vrc_internal.hpp"
+
+namespace vrc {
+
+void fill_int(hipStream_t s, int* p, int v, long n);
+
+// ---------------------------------------------------------------- geometry (host)
+// FourierSolver1D::reorder/refill semantics (src/rcwa/FourierSolver1D.cpp:145-193)
+// applied to one layer: painter-order values and the two interval lists.
+struct PreparedLayer {
+  std::vector<double> xu, xs, ys;
+  std::vector<cd> val;  // [strip][cell], ncy x ncx
+  double Lx, Ly;
+  int ncx, ncy;
+};
+PreparedLayer prepare_layer(const HostLayer& L);
+
+// alpha/beta of LayerSampler::evaluateAlphaBeta (src/core/LayerSampler.cpp:9-32).
+void alpha_beta(int nx, int ny, double Lx, double Ly, std::vector<cd>& alpha, std::vector<cd>& beta);
+// Dense PML-stretched K matrices (src/core/LayerSampler.cpp:34-131), n x n column-major.
+void pml_k_matrices(int nx, int ny, const std::vector<double>& x, const std::vector<double>& y,
+                    std::vector<cd>& Kx, std::vector<cd>& Ky);
+
+// ---------------------------------------------------------------- (a) + (b)
+struct AssemblyBatch {
+  int nx = 0, ny = 0, nlayers = 0;
+  std::vector<const HostLayer*> layers;
+  bool periodic = true;
+  const cd* d_alpha = nullptr;  // periodic: [l][n] (stride sab; 0 = shared)
+  const cd* d_beta = nullptr;
+  long sab = 0;
+  const double* d_k0sq = nullptr;  // [l]
+  // PML: shared dense K and products
+  const cd *d_Kx = nullptr, *d_Ky = nullptr, *d_KxKy = nullptr, *d_KxKx = nullptr, *d_KyKy = nullptr,
+           *d_KyKx = nullptr;
+  double k0sq_uniform = 0.0;
+};
+void assemble_fe(vrc_ctx& c, const AssemblyBatch& a, cd* fe11, cd* fe22, cd* fe33, long sfe);
+void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q);
+
+// ---------------------------------------------------------------- (c)
+// PQ = P Q; eig; gamma = sqrt branch; V = Q W diag(1/gamma)  (src/rcwa/WaveEquationSolver.cpp:24-37)
+void wave_eqn(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* W, cd* V, cd* gam, int* info, int batch);
+
+// ---------------------------------------------------------------- (d)
+struct SMat {
+  cd *Tdd = nullptr, *Tuu = nullptr, *Rud = nullptr, *Rdu = nullptr;
+};
+struct LUPair {
+  const cd* W = nullptr;  // LU factors of W
+  const cd* V = nullptr;  // LU factors of V
+  const int* pW = nullptr;
+  const int* pV = nullptr;
+  long sM = 0;  // batch stride of the factors (pivot stride is N)
+};
+struct RefSys {
+  const cd *W, *V, *gam;
+  LUPair LU;
+};
+
+void gather_mats(hipStream_t s, long nn, const cd* src, long ss, const int* d_idx, cd* dst, long sd, int batch);
+// Slab S-matrix (src/rcwa/RCWAScatterMatrix.cpp:26-85); d_s[b] = dz_b / k0_b.
+void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
+                   const cd* W0, const cd* V0, long sRef, cd* Tuu, cd* Rud);
+// Star product of src/core/RCWAIntegrator.cpp:57-67 (S updated in place; slice stride ss).
+void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
+                  const cd* rdu, long ss);
+void project_left(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LU1, const cd* W0, const cd* V0, long sRef);
+void project_right(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LUl, const cd* W1, const cd* V1,
+                   long sRef);
+double redheffer_perturbation(vrc_ctx& c, int N, double k0, double z0, double z1, const cd* Pr, const cd* Qr,
+                              const RefSys& R, const cd* P3[3], const cd* Q3[3], cd* tdd, cd* tuu, cd* rud,
+                              cd* rdu);
+
+// ---------------------------------------------------------------- samplers
+typedef double (*RuleFn)(double z, void* user);
+
+class Sampler {
+ public:
+  Sampler(vrc_ctx* c, double lambda, int nx, int ny, bool pml);
+  virtual ~Sampler() = default;
+  virtual HostLayer layer_at(double z) const = 0;
+  int nDim() const { return 2 * nx_ * ny_; }
+  int n() const { return nx_ * ny_; }
+  double lambda() const { return lambda_; }
+  double k0() const;
+  vrc_ctx* ctx() const { return c_; }
+  // P, Q (N x N, stride N*N) for every z (device).
+  void sample_gpu(const std::vector<double>& zs, cd* P, cd* Q);
+  // P, Q of explicit layers with this sampler's wavelength and K matrices.
+  void sample_layers(const std::vector<HostLayer>& ls, cd* P, cd* Q);
+  const std::vector<cd>& Kx() const { return hKx_; }
+  const std::vector<cd>& Ky() const { return hKy_; }
+
+ protected:
+  void post_init(const std::vector<double>& x, const std::vector<double>& y);
+  vrc_ctx* c_;
+  double lambda_;
+  int nx_, ny_;
+  bool pml_;
+  std::vector<cd> hKx_, hKy_, alpha_, beta_;
+  DevBuf d_alpha_, d_beta_, d_K_;  // d_K_: Kx, Ky, KxKy, KxKx, KyKy, KyKx (PML)
+};
+
+class VarSampler : public Sampler {
+ public:
+  VarSampler(vrc_ctx* c, double lambda, int nx, int ny, HostLayer base, bool pml);
+  void add_rule(int axis, int layout, RuleFn fn, void* user);
+  HostLayer layer_at(double z) const override;
+
+ private:
+  HostLayer base_;
+  std::map<int, std::pair<RuleFn, void*>> xr_, yr_;
+};
+
+class StackSampler : public Sampler {
+ public:
+  StackSampler(vrc_ctx* c, double lambda, int nx, int ny, std::vector<HostLayer> layers, std::vector<double> z_edges,
+               HostLayer bottom, HostLayer top, bool pml);
+  HostLayer layer_at(double z) const override;
+
+ private:
+  std::vector<HostLayer> layers_;
+  std::vector<double> ze_;
+  HostLayer bottom_, top_;
+};
+
+// ---------------------------------------------------------------- integrators
+class Integrator {
+ public:
+  Integrator(vrc_ctx* c, double z0, double z1, Sampler* s);
+  virtual ~Integrator() = default;
+  // which: 0 Tdd, 1 Tuu, 2 Rud, 3 Rdu, 4 W0, 5 V0, 6 gam0, 7 W1, 8 V1, 9 gam1
+  void get(int which, cd* host) const;
+  int N_ = 0, nEig_ = 0;
+  std::vector<std::pair<double, double>> tree_;
+
+  void rcwa(const std::vector<std::pair<double, double>>& slices, bool batched);  // (mid z, dz)
+  double it_z0() const { return z0_; }
+  double it_z1() const { return z1_; }
+  void redheffer(double pe, int max_depth);
+  void differential(int N);
+
+ protected:
+  void alloc();
+  void eig_at(double z, cd* P, cd* Q, cd* W, cd* V, cd* gam);
+  vrc_ctx* c_;
+  double z0_, z1_;
+  Sampler* s_;
+  int Nd_ = 0;
+  DevBuf S_, W0_, V0_, g0_, W1_, V1_, g1_, info_;
+  SMat S() const;
+};
+
+// ---------------------------------------------------------------- batched stacks
+struct StackInstance {
+  double lambda;
+  std::vector<const HostLayer*> layers;
+  std::vector<double> thickness;
+  const HostLayer* bottom;
+  const HostLayer* top;
+};
+// Out: S[inst] = [Tdd|Tuu|Rud|Rdu] (4 N x N); optional W0/V0/gam0/W1/V1/gam1 per instance.
+void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int chunk, cd* S_out,
+                      cd* W0, cd* V0, cd* g0, cd* W1, cd* V1, cd* g1);
+
+}  // namespace vrc
