@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""S-matrix throughput on one MI355X: slice solves/s at 441 harmonics (N = 882), complex128.
+
+One step = one batched call of the library (``vrc_batch_stack_rcwa``) over a batch of
+instances of the rectangle-stack workload (tests/golden/metric_geometry.json): each
+instance is 10 slabs (1-4 rectangles, eps in {4, 12.25}, 0.6 um cell, thickness
+0.05-0.2 um) between air half-spaces at one wavelength in 0.8-1.6 um, Fourier orders
++-10 in x and y. A slice solve = Toeplitz assembly + P/Q + eig(PQ) + slab S-matrix +
+star product into the cascade; every step also solves the two end media and the
+end projection of each instance (counted as overhead, not as slices).
+
+Also reported: the plain per-instance path (RCWAIntegrator over the same stack, one
+slice after another), live stage times from HIP events on the library stream, the
+roofline of the dominant kernel, and the CPU oracle timed on a bounded sample.
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "S-matrix solves/s @441 harmonics c128, batched+adjoint vs plain; FP64 MFMA %"
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+STAGES = ["fourier", "pq", "hessenberg", "qr", "eigvec", "eigpost", "layer_s", "star", "project"]
+
+
+def flops_slice(N):
+    """Algorithmic real flops of one slice solve (F(N) = 305 N^3)."""
+    return 305.0 * N ** 3
+
+
+def flops_qr(N):
+    """Algorithmic real flops of the Schur (QR-iteration) phase of one eigensolve:
+    the eig total (316/3) N^3 minus Hessenberg (40/3), Q formation (16/3) and
+    eigenvectors + back-transform (28/3)."""
+    return (316.0 - 40.0 - 16.0 - 28.0) / 3.0 * N ** 3
+
+
+def load_instances(nh: int, count: int):
+    from varrcwa_amd.api import Layer
+    d = json.loads((ROOT / "tests" / "golden" / "metric_geometry.json").read_text())
+    cell = d["cell"]
+    lams = d["wavelengths"]
+    geoms = d["geometries"]
+    air = Layer([0.0, cell], [0.0, cell], [[1.0]])
+    insts = []
+    for i in range(count):
+        g = geoms[i % len(geoms)]
+        lam = lams[(i // len(geoms)) % len(lams)]
+        layers = [Layer(l["x"], l["y"], np.asarray(l["eps"], dtype=complex)) for l in g["layers"]]
+        insts.append(dict(lam=lam, layers=layers, thickness=g["thickness"], bottom=air, top=air))
+    return insts
+
+
+def read_stages(ctx):
+    ms = (C.c_double * len(STAGES))()
+    cnt = (C.c_int * len(STAGES))()
+    ctx.check(ctx.lib.vrc_timing_read(ctx.h, ms, cnt, len(STAGES)), "timing_read")
+    return {s: (ms[i], cnt[i]) for i, s in enumerate(STAGES)}
+
+
+def cpu_baseline(nh: int, nslices: int):
+    """Oracle (restated reference CPU path, numpy/LAPACK) on the first slices of
+    instance 0: per slice, sample + eig + slab S-matrix + star product."""
+    from oracle import rcwa_oracle as O
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # pragma: no cover
+        threadpool_limits = None
+    d = json.loads((ROOT / "tests" / "golden" / "metric_geometry.json").read_text())
+    g = d["geometries"][0]
+    cell = d["cell"]
+    lam = d["wavelengths"][0]
+    air = (np.array([0.0, cell]), np.array([0.0, cell]), np.array([[1.0 + 0j]]))
+    layers = [(np.array(l["x"]), np.array(l["y"]), np.array(l["eps"], dtype=complex)) for l in g["layers"]]
+    ze = np.concatenate([[0.0], np.cumsum(g["thickness"])])
+    nx = ny = nh
+    s = O.StackLayerSampler(lam, nx, ny, layers, ze, air, air)
+    cores = min(16, os.cpu_count() or 1)
+    ctxm = threadpool_limits(limits=cores) if threadpool_limits else None
+    try:
+        P1, Q1 = s.sample(ze[-1])
+        W1, V1, _, _ = O.wave_equation_solve(P1, Q1)
+        nD = W1.shape[0]
+        S = (np.eye(nD, dtype=complex), np.eye(nD, dtype=complex), np.zeros((nD, nD), complex),
+             np.zeros((nD, nD), complex))
+        t0 = time.perf_counter()
+        for i in range(nslices):
+            zm = 0.5 * (ze[i] + ze[i + 1])
+            P, Q = s.sample(zm)
+            tuu, rud = O.rcwa_scatter_matrix(lam, ze[i + 1] - ze[i], P, Q, W1, V1)
+            S = O.star_product(S, tuu, tuu, rud, rud)
+        t = time.perf_counter() - t0
+    finally:
+        if ctxm is not None:
+            ctxm.__exit__(None, None, None)
+    return dict(value=nslices / t, unit="slice solves/s", cores=cores, kind="port",
+                sample=f"{nslices} slices of instance 0 (N={2 * nx * ny}, numpy/OpenBLAS LAPACK, "
+                       f"{t:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=16, help="instances per step")
+    ap.add_argument("--nh", type=int, default=21, help="harmonics per axis (2*order+1)")
+    ap.add_argument("--plain", type=int, default=1, help="instances timed through the plain path (0 = skip)")
+    ap.add_argument("--cpu-slices", type=int, default=2, help="slices in the CPU sample (0 = skip)")
+    ap.add_argument("--chunk", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(instances shard with no exchange; see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+    from varrcwa_amd import api
+    from varrcwa_amd._lib import Context
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    ctx = Context(0)
+    nh = args.nh
+    n = nh * nh
+    N = 2 * n
+    insts = load_instances(nh, args.instances)
+    L = len(insts[0]["layers"])
+    ctx.check(ctx.lib.vrc_timing_enable(ctx.h, 1), "timing")
+
+    for w in range(args.warmup):
+        api.batch_stack_rcwa(ctx, nh, nh, insts, chunk=args.chunk)
+        torch.cuda.synchronize()
+        log(f"warmup {w + 1}/{args.warmup} done")
+    read_stages(ctx)  # reset
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        out = api.batch_stack_rcwa(ctx, nh, nh, insts, chunk=args.chunk)
+        del out
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = read_stages(ctx)
+    dt = t1 - t0
+    slices = args.steps * len(insts) * L
+    value = slices / dt
+    log(f"batched: {slices} slices in {dt:.2f} s -> {value:.2f} slices/s")
+
+    # dominant stage (stage 1 includes stage 0; report exclusive times)
+    excl = {k: v[0] for k, v in st.items()}
+    excl["pq"] = max(0.0, excl["pq"] - excl["fourier"])
+    dom = max(excl, key=excl.get)
+    eigs_per_step = len(insts) * (L + 2)
+    qr_ms, qr_cnt = st["qr"]
+    roof = None
+    if qr_cnt:
+        avg_ms = qr_ms / qr_cnt
+        per_launch = flops_qr(N) * eigs_per_step * args.steps / qr_cnt
+        ach = per_launch / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "hqr_kernel (multishift QR, Schur phase)", "achieved": ach,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "avg_launch_ms": avg_ms, "flops_per_launch": per_launch}
+
+    plain = None
+    if args.plain > 0:
+        inst = insts[0]
+        ze = np.concatenate([[0.0], np.cumsum(inst["thickness"])])
+        s = api.StackLayerSampler(inst["lam"], nh, nh, inst["layers"], ze, inst["bottom"], inst["top"], ctx=ctx)
+        it = api.RCWAIntegrator(ze[0], ze[-1], s)
+        torch.cuda.synchronize()
+        tp0 = time.perf_counter()
+        for _ in range(args.plain):
+            it.compute_edges(ze)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - tp0
+        plain = args.plain * L / tp
+        read_stages(ctx)
+        log(f"plain: {args.plain * L} slices in {tp:.2f} s -> {plain:.3f} slices/s")
+        del it, s
+
+    cpu = None
+    if args.cpu_slices > 0:
+        cpu = cpu_baseline(nh, args.cpu_slices)
+        log(f"cpu: {cpu}")
+
+    frac = flops_slice(N) * value / (FP64_PEAK_TFLOPS * 1e12)
+    line = {
+        "metric": METRIC, "value": value, "unit": "slice solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+        "config": {"workload": f"rectangle stacks, {nh}x{nh} harmonics (N={N}), {L} slabs/instance, "
+                               f"{len(insts)} instances/step (8 seeded geometries x wavelengths in 0.8-1.6 um)",
+                   "instances_per_step": len(insts), "slices_per_step": len(insts) * L, "N": N,
+                   "parallelism": "single GPU"},
+        "fp64_matrix_frac": frac,
+        "path_tflops": flops_slice(N) * value / 1e12,
+        "plain_slices_per_s": plain,
+        "batched_over_plain": (value / plain) if plain else None,
+        "adjoint": "not built in this version",
+        "stage_ms_per_step": {k: v / args.steps for k, v in excl.items()},
+        "dominant_stage": dom,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -39,6 +39,14 @@ int vrc_sync(vrc_ctx* ctx);
 long long vrc_pool_bytes(const vrc_ctx* ctx);
 int vrc_pool_trim(vrc_ctx* ctx);
 const char* vrc_version(void);
+/* Stage timers (no reference counterpart; the reference's timers are commented out,
+ * src/core/VarLayerSampler.cpp:62-68): when enabled, HIP events bracket each stage on
+ * the context stream. vrc_timing_read synchronizes, returns the accumulated
+ * milliseconds and launch counts per stage (0 Fourier assembly, 1 P/Q assembly
+ * incl. stage 0, 2 Hessenberg + Q, 3 QR iteration, 4 eigenvectors, 5 PQ product and
+ * V formation, 6 slab S-matrices, 7 star products, 8 end projections) and resets. */
+int vrc_timing_enable(vrc_ctx* ctx, int on);
+int vrc_timing_read(vrc_ctx* ctx, double* ms, int* counts, int nstages);
 
 /* ------------------------------------------------- dense complex128 kernels */
 /* op: 0 = N, 1 = T, 2 = C (conjugate transpose).

@@ -20,6 +20,15 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+def spectrum_rel(a, b):
+    """Max distance between two eigenvalue multisets under the optimal pairing
+    (order-free; robust to near-degenerate pairs), relative to max |b|."""
+    from scipy.optimize import linear_sum_assignment
+    d = np.abs(a[:, None] - b[None, :])
+    r, c = linear_sum_assignment(d)
+    return d[r, c].max() / np.abs(b).max()
+
+
 def layer_set():
     rng = np.random.default_rng(3)
     g = G.metasurface(2)
@@ -86,8 +95,7 @@ def test_eig_pq_properties(ctx, which):
     assert rel(V, (Q @ W) / gam[None, :]) < 1e-12
     # eigenvalue multiset vs LAPACK
     ref = np.linalg.eigvals(PQ)
-    a, b = np.sort_complex(lam), np.sort_complex(ref)
-    assert np.abs(a - b).max() / np.abs(ref).max() < 1e-12
+    assert spectrum_rel(lam, ref) < 1e-12
 
 
 def test_layer_smatrix_and_star_product(ctx):
@@ -149,7 +157,7 @@ def test_rcwa_integrator_parity(ctx, which):
     r2 = gpu_result(it)
     invariants_close(r2, ref)
     # gamma sets agree with the oracle's (order-free)
-    assert rel(np.sort_complex(r1.gam1), np.sort_complex(ref.gam1)) < 1e-11
+    assert spectrum_rel(r1.gam1, ref.gam1) < 1e-11
 
 
 def test_redheffer_parity_metasurface(ctx):
@@ -166,13 +174,15 @@ def test_redheffer_parity_metasurface(ctx):
 
 
 def test_differential_parity_metasurface(ctx):
-    g = G.metasurface(2)
-    N = 6
+    # well-conditioned regime of the 4th-order step (evanescent growth over one step
+    # stays O(1)); at coarse steps the method itself amplifies rounding
+    g = G.metasurface(1)
+    N = 32
     ref = O.differential_integrate(G.oracle_sampler(g), g["z0"], g["z1"], N)
     s = G.gpu_sampler(g, ctx)
     it = api.DifferentialIntegrator(g["z0"], g["z1"], s)
     it.compute(N)
-    invariants_close(gpu_result(it), ref, tol=1e-8)
+    invariants_close(gpu_result(it), ref, tol=1e-9)
 
 
 def _stack_case(seed, nlayers=2):

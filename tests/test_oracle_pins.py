@@ -1,0 +1,291 @@
+"""Pins of the CPU oracle against what mathematics fixes independently of it.
+
+The reference ships no golden vectors or assertions (its drivers only print
+residuals), so each oracle function is pinned here by a closed form, a brute-force
+integral, a structural identity or a physical invariant chosen so that a plausible
+mistake (dropped term, wrong sign or index, transposed operand) fails.
+
+Pinned: fourier1d/fourier1d_vec (quadrature + DFT closed form), refill (painter's
+order), toeplitz_dense/block_toeplitz_dense (convolution identities), fourier2d
+(homogeneous, single-strip inverse rules, Laurent = separable product), k_matrices
+periodic (closed form) and PML (gamma=1 stretch reduces to the unstretched cell),
+wave_eqn_coeff (homogeneous closed form), wave_equation_solve (eigen residual,
+homogeneous eigenvalues), gamma_sqrt (branch table), rcwa_scatter_matrix
+(homogeneous slab), star_product (identity, associativity), rcwa_integrate
+(z-invariance, energy conservation), redheffer_integrate (single segment for a
+z-invariant structure, convergence to RCWA), differential_integrate (convergence).
+Parity unpinned: svd_solve rank threshold for singular fe33 (never exercised).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import rcwa_oracle as O
+from tests import geometries as G
+
+
+def quad_coeff(xs, fs, d, m, npts=20000):
+    """(1/d) * integral of the piecewise constant f(x) exp(-2 pi i m x / d) over each cell (midpoint rule)."""
+    tot = 0j
+    for j in range(len(fs)):
+        a, b = xs[j], xs[j + 1]
+        t = a + (np.arange(npts) + 0.5) * (b - a) / npts
+        tot += fs[j] * np.sum(np.exp(-2j * math.pi * m * t / d)) * (b - a) / npts
+    return tot / d
+
+
+def test_fourier1d_matches_quadrature():
+    xs = np.array([0.0, 0.13, 0.4, 0.41, 0.9])
+    fs = np.array([1.0, 3.5 - 0.2j, 7.0, 2.0 + 1j])
+    d = xs[-1] - xs[0]
+    M = 9
+    got = O.fourier1d(xs, d, fs, M)
+    vec = O.fourier1d_vec(xs, d, fs, M)
+    for i in range(M):
+        m = i - (M - 1) // 2
+        ref = quad_coeff(xs, fs, d, m)
+        assert abs(got[i] - ref) < 1e-7, (m, got[i], ref)
+    assert np.abs(got - vec).max() < 1e-14
+
+
+def test_fourier1d_pixel_grid_is_scaled_dft():
+    # uniform grid of Gp pixels on [x0, x0+d]: F_m = sinc(m/Gp) exp(-i pi m/Gp) exp(-2 pi i m x0/d) DFT(f)_m / Gp
+    rng = np.random.default_rng(0)
+    Gp, d, x0 = 16, 0.8, 0.1
+    f = rng.random(Gp) + 1j * rng.random(Gp)
+    xs = x0 + d * np.arange(Gp + 1) / Gp
+    M = 11
+    got = O.fourier1d(xs, d, f, M)
+    dft = np.fft.fft(f)
+    for i in range(M):
+        m = i - (M - 1) // 2
+        s = 1.0 if m == 0 else math.sin(math.pi * m / Gp) / (math.pi * m / Gp)
+        ref = s * np.exp(-1j * math.pi * m / Gp) * np.exp(-2j * math.pi * m * x0 / d) * dft[m % Gp] / Gp
+        assert abs(got[i] - ref) < 1e-13
+
+
+def test_refill_painters_order():
+    # coordinates cross: cell 1 (x=0.4..0.2) is inverted; sorted intervals take the
+    # material of the last non-inverted original cell covering them
+    x = np.array([0.0, 0.4, 0.2, 0.65])
+    index, rindex = O._rindex(x)
+    assert list(index) == [0, 2, 1, 3]
+    f = np.array([1.0, 2.0, 3.0], dtype=complex)
+    newf = O.refill(x, f, rindex)
+    # sorted intervals: [0,0.2] (cell 0 covers 0..0.4), [0.2,0.4] (cell 0, then cell 2 covers 0.2..0.65), [0.4,0.65] (cell 2)
+    assert list(newf) == [1.0, 3.0, 3.0]
+    # no crossing: identity
+    assert list(O.refill(np.array([0.0, 0.1, 0.5, 0.65]), f, np.arange(4))) == [1.0, 2.0, 3.0]
+
+
+def test_toeplitz_is_convolution():
+    # Laurent rule: the coefficients of f*g equal T_f applied to the coefficients of g
+    rng = np.random.default_rng(1)
+    K = 3
+    n = 2 * K + 1
+    fc = rng.standard_normal(2 * n - 1) + 1j * rng.standard_normal(2 * n - 1)  # orders -2K..2K
+    gc = rng.standard_normal(n) + 1j * rng.standard_normal(n)                    # orders -K..K
+    T = O.toeplitz_dense(fc)
+    prod = np.convolve(fc, gc)                                                   # orders -3K..3K
+    centre = (prod.size - 1) // 2
+    ref = prod[centre - K:centre + K + 1]
+    assert np.abs(T @ gc - ref).max() < 1e-12
+
+
+def test_block_toeplitz_index_map():
+    rng = np.random.default_rng(2)
+    nx, ny = 3, 2
+    mat = rng.standard_normal((2 * nx - 1, 2 * ny - 1))
+    B = O.block_toeplitz_dense(mat)
+    for r in range(nx * ny):
+        for c in range(nx * ny):
+            m, n = divmod(r, ny)
+            j, l = divmod(c, ny)
+            assert B[r, c] == mat[nx - 1 + m - j, ny - 1 + n - l]
+
+
+def test_fourier2d_homogeneous_and_inverse_rules():
+    nx, ny = 5, 3
+    x, y, eps = G.homogeneous(0.7, 4.0 + 0.1j)
+    for opt in (O.DISCONTINUOUS_X, O.DISCONTINUOUS_Y, O.CONTINUOUS_XY):
+        F = O.fourier2d(x, y, eps, nx, ny, opt)
+        assert np.abs(F - (4.0 + 0.1j) * np.eye(nx * ny)).max() < 1e-13
+    # one strip in y: inverse rule along x is inv(Toep(coeff(1/eps))) (x) I_y
+    x = np.array([0.0, 0.2, 0.5, 1.0])
+    y = np.array([0.0, 0.8])
+    e = np.array([[2.0, 9.0, 1.0]], dtype=complex)
+    F11 = O.fourier2d(x, y, e, nx, ny, O.DISCONTINUOUS_X)
+    inv = np.linalg.inv(O.toeplitz_dense(O.fourier1d(x, 1.0, 1.0 / e[0], 2 * nx - 1)))
+    assert np.abs(F11 - np.kron(inv, np.eye(ny))).max() < 1e-12
+    # one cell in x: inverse rule along y is I_x (x) inv(Toep(coeff_y(1/eps)))
+    x1 = np.array([0.0, 1.0])
+    y1 = np.array([0.0, 0.3, 0.8])
+    e1 = np.array([[2.0], [6.0]], dtype=complex)
+    F22 = O.fourier2d(x1, y1, e1, nx, ny, O.DISCONTINUOUS_Y)
+    invy = np.linalg.inv(O.toeplitz_dense(O.fourier1d(y1, 0.8, 1.0 / e1[:, 0], 2 * ny - 1)))
+    assert np.abs(F22 - np.kron(np.eye(nx), invy)).max() < 1e-12
+    # Laurent rule of a separable (rectangle) layer equals the product of 1D Laurent matrices
+    x2 = np.array([0.0, 0.3, 0.6, 1.0])
+    y2 = np.array([0.0, 0.2, 0.5, 0.8])
+    ex = np.array([1.0, 5.0, 1.0])
+    ey = np.array([1.0, 3.0, 1.0])
+    # eps = 1 + (ex-1)(ey-1)/... use indicator product: eps = 1 + 8 * chi_x * chi_y
+    chi = np.outer([0, 1, 0], [0, 1, 0]).astype(complex)
+    e2 = 1.0 + 8.0 * chi
+    F33 = O.fourier2d(x2, y2, e2, nx, ny, O.CONTINUOUS_XY)
+    cx = O.toeplitz_dense(O.fourier1d(x2, 1.0, np.array([0, 1, 0], complex), 2 * nx - 1))
+    cy = O.toeplitz_dense(O.fourier1d(y2, 0.8, np.array([0, 1, 0], complex), 2 * ny - 1))
+    ref = np.eye(nx * ny) + 8.0 * np.kron(cx, cy)
+    assert np.abs(F33 - ref).max() < 1e-12
+    assert ex.size == ey.size
+
+
+def test_k_matrices_periodic_and_pml_limits():
+    nx, ny = 3, 5
+    x = np.array([0.0, 0.5, 2.0, 2.5])
+    y = np.array([0.0, 0.4, 1.6, 2.0])
+    reg = dict(Lx=2.5, Ly=2.0, b0=0.0, b1=0.4, u0=1.6, u1=2.0, l0=0.0, l1=0.5, r0=2.0, r1=2.5)
+    Kx, Ky = O.k_matrices(nx, ny, reg, False)
+    a, b = O.alpha_beta(nx, ny, 2.5, 2.0)
+    assert np.allclose(np.diag(Kx), a) and np.allclose(np.diag(Ky), b)
+    assert a[0] == -2 * O.PI / 2.5 and b[1] == -2 * O.PI / 2.0 * 1
+    # PML with zero-width absorbing regions is the unstretched identity: K = diag(alpha)
+    reg0 = dict(Lx=2.5, Ly=2.0, b0=0.0, b1=0.0, u0=2.0, u1=2.0, l0=0.0, l1=0.0, r0=2.5, r1=2.5)
+    Kx0, Ky0 = O.k_matrices(nx, ny, reg0, True)
+    assert np.abs(Kx0 - np.diag(a)).max() < 1e-12 and np.abs(Ky0 - np.diag(b)).max() < 1e-12
+
+
+def test_homogeneous_operator_eigensystem_and_slab():
+    nx = ny = 3
+    eps = 2.25
+    lam = 1.3
+    x, y, e = G.homogeneous(0.9, eps)
+    s = O.StackLayerSampler(lam, nx, ny, [(x, y, e)], [0.0, 1.0], (x, y, e), (x, y, e))
+    P, Q = s.sample(0.5)
+    k0 = 2 * O.PI / lam
+    a, b = O.alpha_beta(nx, ny, 0.9, 0.9)
+    n = nx * ny
+    Pref = np.block([[np.diag(a * b / eps), np.diag(k0 ** 2 - a * a / eps)],
+                     [np.diag(b * b / eps - k0 ** 2), np.diag(-b * a / eps)]])
+    Qref = np.block([[np.diag(-a * b), np.diag(a * a - k0 ** 2 * eps)],
+                     [np.diag(-b * b + k0 ** 2 * eps), np.diag(b * a)]])
+    assert np.abs(P - Pref).max() < 1e-11 and np.abs(Q - Qref).max() < 1e-11
+    W, V, gam, lamv = O.wave_equation_solve(P, Q)
+    kz2 = k0 ** 2 * (k0 ** 2 * eps - a * a - b * b)
+    assert np.abs(np.sort_complex(lamv) - np.sort_complex(np.concatenate([kz2, kz2]))).max() < 1e-9
+    # slab of the same medium in its own basis: T = X, R = 0
+    dz = 0.37
+    Tuu, Rud = O.rcwa_scatter_matrix(lam, dz, P, Q, W, V)
+    X = np.exp(1j * gam * dz / k0)
+    assert np.abs(Tuu - np.diag(X)).max() < 1e-12 and np.abs(Rud).max() < 1e-12
+    assert n == 9
+
+
+def test_gamma_sqrt_branch_table():
+    vals = np.array([4.0, -4.0, 3 - 4j, -3 + 4j, -3 - 4j, 1e-3 - 1j])
+    got = O.gamma_sqrt(vals)
+    ref = np.array([2.0, 2j, 2 - 1j, 1 + 2j, -1 + 2j, None], dtype=object)
+    for g, r in zip(got[:5], ref[:5]):
+        assert abs(g - r) < 1e-15
+    assert got[5].imag >= 0 or (abs(got[5].imag) < abs(got[5].real) and got[5].real > 0)
+    assert np.allclose(got * got, vals)
+
+
+def _rand_S(rng, n, scale=0.3):
+    def r():
+        return scale * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    return (np.eye(n) + r(), np.eye(n) + r(), r(), r())
+
+
+def test_star_product_identity_and_associativity():
+    rng = np.random.default_rng(4)
+    n = 6
+    I = (np.eye(n, dtype=complex), np.eye(n, dtype=complex), np.zeros((n, n), complex), np.zeros((n, n), complex))
+    A, B, C = _rand_S(rng, n), _rand_S(rng, n), _rand_S(rng, n)
+    AI = O.star_product(A, *I)
+    assert all(np.abs(p - q).max() < 1e-13 for p, q in zip(AI, A))
+    IA = O.star_product(I, *A)
+    assert all(np.abs(p - q).max() < 1e-13 for p, q in zip(IA, A))
+    ABC1 = O.star_product(O.star_product(A, *B), *C)
+    BC = O.star_product(B, *C)
+    ABC2 = O.star_product(A, *BC)
+    assert all(np.abs(p - q).max() < 1e-12 for p, q in zip(ABC1, ABC2))
+
+
+def _invariant_sampler():
+    g = G.metasurface(1)
+    s = O.VarLayerSampler(g["lam"], g["nx"], g["ny"], g["x"], g["y"], g["eps"], False)
+    return s
+
+
+def test_rcwa_z_invariant_slicing_and_single_varrcwa_segment():
+    s = _invariant_sampler()
+    r1 = O.rcwa_integrate(s, 0.0, 0.7, 1)
+    r4 = O.rcwa_integrate(s, 0.0, 0.7, 4)
+    a, b = O.invariant_transforms(r1), O.invariant_transforms(r4)
+    for k in a:
+        assert np.abs(a[k] - b[k]).max() < 1e-10 * max(1.0, np.abs(a[k]).max()), k
+    rv = O.redheffer_integrate(s, 0.0, 0.7, 1e-3)
+    assert rv.N == 1 and len(rv.tree) == 1
+    c = O.invariant_transforms(rv)
+    for k in a:
+        assert np.abs(a[k] - c[k]).max() < 1e-10 * max(1.0, np.abs(a[k]).max()), k
+
+
+def flux_gram(W, V):
+    """Hermitian form G with flux(a) = a^H G a for the field (E, H) = (W a, V a):
+    flux = 1/2 Re sum_h (Ex conj(Hy) - Ey conj(Hx)) (evaluateEnergy, src/tests/test_cpu.cpp:337-412,
+    integrated over the cell by Parseval; the cell area is a common factor and dropped)."""
+    n = W.shape[0] // 2
+    M = V[n:].conj().T @ W[:n] - V[:n].conj().T @ W[n:]
+    return 0.25 * (M + M.conj().T)
+
+
+def energy_balance(r, k0, tol):
+    """For each propagating incident mode j from below: incident - reflected = transmitted flux."""
+    G0 = flux_gram(r.W0, r.V0)
+    G1 = flux_gram(r.W1, r.V1)
+    p0 = np.abs(r.gam0.imag) < 1e-9 * k0 ** 2
+    p1 = np.abs(r.gam1.imag) < 1e-9 * k0 ** 2
+    assert p0.sum() >= 2 and p1.sum() >= 2
+    G0p, G1p = G0[np.ix_(p0, p0)], G1[np.ix_(p1, p1)]
+    for j in np.nonzero(p0)[0]:
+        rr = r.Rdu[p0, j]
+        tt = r.Tuu[p1, j]
+        inc = G0[j, j].real
+        out = (tt.conj() @ G1p @ tt).real + (rr.conj() @ G0p @ rr).real
+        assert abs(out - inc) < tol * abs(inc), (j, out, inc)
+
+
+def test_energy_conservation_lossless_stack():
+    # lossless periodic stack between glass and air: for each propagating incident mode,
+    # transmitted + reflected propagating flux = incident flux
+    rng = np.random.default_rng(5)
+    nx = ny = 3
+    cell = 0.6
+    layers = [G.random_rect_layer(rng, cell) for _ in range(2)]
+    ze = np.array([0.0, 0.11, 0.23])
+    lam = 1.1
+    s = O.StackLayerSampler(lam, nx, ny, layers, ze, G.homogeneous(cell, 2.1), G.homogeneous(cell, 1.0))
+    r = O.rcwa_integrate(s, ze[0], ze[-1], 2, z_edges=ze)
+    energy_balance(r, 2 * O.PI / lam, 1e-10)
+    # a planted mistake (dropped reflection) breaks the balance
+    bad = O.SResult(r.Tdd, r.Tuu, r.Rud, 0 * r.Rdu, r.W0, r.V0, r.gam0, r.W1, r.V1, r.gam1)
+    with pytest.raises(AssertionError):
+        energy_balance(bad, 2 * O.PI / lam, 1e-10)
+
+
+def test_convergence_rcwa_varrcwa_differential():
+    g = G.metasurface(1)
+    s = O.VarLayerSampler(g["lam"], g["nx"], g["ny"], g["x"], g["y"], g["eps"], False)
+    s.addXRule(2, g["rules"][2])
+    ref = O.invariant_transforms(O.rcwa_integrate(s, 0.0, 2.0, 256))["Tuu"]
+    errs = [np.abs(O.invariant_transforms(O.rcwa_integrate(s, 0.0, 2.0, N))["Tuu"] - ref).max() for N in (8, 32)]
+    assert errs[1] < errs[0] / 4
+    ev = [np.abs(O.invariant_transforms(O.redheffer_integrate(s, 0.0, 2.0, pe))["Tuu"] - ref).max()
+          for pe in (1e-1, 1e-3)]
+    assert ev[1] < ev[0] and ev[1] < 1e-3
+    ed = [np.abs(O.invariant_transforms(O.differential_integrate(s, 0.0, 2.0, N))["Tuu"] - ref).max()
+          for N in (32, 128)]
+    assert ed[1] < ed[0]

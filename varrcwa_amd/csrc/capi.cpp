@@ -37,10 +37,7 @@ int vrc_ctx_create(int device, void* stream, vrc_ctx** out) {
 
 int vrc_ctx_destroy(vrc_ctx* c) {
   if (!c) return 0;
-  (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
-  delete c;
+  ctx_release(c);
   return 0;
 }
 
@@ -56,6 +53,32 @@ int vrc_pool_trim(vrc_ctx* c) {
   return guarded(c, [&] {
     VRC_HIP(hipStreamSynchronize(c->stream));
     c->ws.trim();
+  });
+}
+
+int vrc_timing_enable(vrc_ctx* c, int on) {
+  return guarded(c, [&] { c->timing = on != 0; });
+}
+
+int vrc_timing_read(vrc_ctx* c, double* ms, int* counts, int nstages) {
+  return guarded(c, [&] {
+    need(ms && nstages > 0, "bad arguments");
+    VRC_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < nstages; ++i) {
+      ms[i] = 0.0;
+      if (counts) counts[i] = 0;
+    }
+    for (auto& r : c->recs) {
+      float t = 0.0f;
+      VRC_HIP(hipEventElapsedTime(&t, r.a, r.b));
+      if (r.stage >= 0 && r.stage < nstages) {
+        ms[r.stage] += t;
+        if (counts) counts[r.stage] += 1;
+      }
+      c->event_pool.push_back(r.a);
+      c->event_pool.push_back(r.b);
+    }
+    c->recs.clear();
   });
 }
 
@@ -281,6 +304,7 @@ int vrc_var_sampler_create(vrc_ctx* c, double lambda, int nx, int ny, const vrc_
     auto s = std::make_unique<vrc_sampler>();
     s->ctx = c;
     s->s = std::make_unique<VarSampler>(c, lambda, nx, ny, to_host(base), enable_pml != 0);
+    ctx_retain(c);
     *out = s.release();
   });
 }
@@ -307,6 +331,7 @@ int vrc_stack_sampler_create(vrc_ctx* c, double lambda, int nx, int ny, int nlay
     s->s = std::make_unique<StackSampler>(c, lambda, nx, ny, to_host(layers, nlayers),
                                           std::vector<double>(z_edges, z_edges + nlayers + 1), to_host(bottom),
                                           to_host(top), enable_pml != 0);
+    ctx_retain(c);
     *out = s.release();
   });
 }
@@ -314,8 +339,10 @@ int vrc_stack_sampler_create(vrc_ctx* c, double lambda, int nx, int ny, int nlay
 int vrc_sampler_destroy(vrc_sampler* s) {
   if (!s) return 0;
   vrc_ctx* c = s->ctx;
-  if (c) (void)hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
   delete s;
+  ctx_release(c);
   return 0;
 }
 
@@ -360,14 +387,18 @@ int vrc_integrator_create(vrc_ctx* c, int kind, double z0, double z1, vrc_sample
     it->ctx = c;
     it->kind = kind;
     it->it = std::make_unique<Integrator>(c, z0, z1, s->s.get());
+    ctx_retain(c);
     *out = it.release();
   });
 }
 
 int vrc_integrator_destroy(vrc_integrator* it) {
   if (!it) return 0;
-  if (it->ctx) (void)hipStreamSynchronize(it->ctx->stream);
+  vrc_ctx* c = it->ctx;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
   delete it;
+  ctx_release(c);
   return 0;
 }
 
@@ -403,7 +434,7 @@ int vrc_rcwa_compute_edges(vrc_integrator* it, int ne, const double* ze) {
       need(ze[i + 1] > ze[i], "edges must increase");
       v.push_back({0.5 * (ze[i] + ze[i + 1]), ze[i + 1] - ze[i]});
     }
-    it->it->rcwa(v, true);
+    it->it->rcwa(v, false);
   });
 }
 

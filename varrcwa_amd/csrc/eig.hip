@@ -17,6 +17,7 @@
 //     block column), then VR = Z * X (ZGEMM) and LAPACK zgeev normalisation.
 #include "vrc_common.hpp"
 #include "vrc_internal.hpp"
+#include "vrc_context.hpp"
 
 namespace vrc {
 
@@ -859,10 +860,17 @@ void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
   DevBuf Zb = ws.get(sizeof(cd) * (size_t)nn * batch);
   cd* Vf = Vfb.as<cd>();
   cd* Z = Zb.as<cd>();
-  gehrd(s, n, A, lda, sA, Vf, Tb.as<cd>(), batch, ws);
-  orghr(s, n, Vf, Tb.as<cd>(), Z, n, nn, batch, ws);
-  hqr_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info);
-  launch_check("hqr");
+  {
+    StageScope st(ST_HESS);
+    gehrd(s, n, A, lda, sA, Vf, Tb.as<cd>(), batch, ws);
+    orghr(s, n, Vf, Tb.as<cd>(), Z, n, nn, batch, ws);
+  }
+  {
+    StageScope st(ST_QR);
+    hqr_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info);
+    launch_check("hqr");
+  }
+  StageScope st(ST_EIGVEC);
   cd* X = Vf;  // reflectors no longer needed
   zset(s, n, n, mk(0.0, 0.0), mk(0.0, 0.0), X, n, nn, batch);
   dim3 g(cdiv_up(n, QW), batch);

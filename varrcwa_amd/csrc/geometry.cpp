@@ -220,6 +220,7 @@ void assemble_fe(vrc_ctx& c, const AssemblyBatch& a, cd* fe11, cd* fe22, cd* fe3
   g.F2 = g.tFFy + nty * L;
   g.G11 = g.F2 + nF2 * L;
   g.G22 = g.G11 + nG11 * L;
+  StageScope st(ST_FOURIER);
   fourier_assemble(s, g, L, fe11, fe22, fe33, sfe);
 }
 

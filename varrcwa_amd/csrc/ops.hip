@@ -136,6 +136,7 @@ static void copy(hipStream_t s, int m, int n, const cd* A, cd* C, long stride, i
 
 // ===================================================================== (b) assembly
 void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q) {
+  StageScope st(ST_PQ);
   hipStream_t s = c.stream;
   const int n = a.nx * a.ny, N = 2 * n, L = a.nlayers;
   const long nn = (long)n * n, NN = (long)N * N;
@@ -188,8 +189,12 @@ void wave_eqn(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* W, cd* 
   const long NN = (long)N * N;
   DevBuf PQb = c.ws.get(sizeof(cd) * (size_t)NN * batch);
   DevBuf lamb = c.ws.get(sizeof(cd) * (size_t)N * batch);
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, P, N, sPQ, Q, N, sPQ, ZERO, PQb.as<cd>(), N, NN, batch);
+  {
+    StageScope st(ST_EIGPOST);
+    zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, P, N, sPQ, Q, N, sPQ, ZERO, PQb.as<cd>(), N, NN, batch);
+  }
   zgeev_batched(s, N, PQb.as<cd>(), N, NN, lamb.as<cd>(), N, W, N, NN, info, batch, c.ws);
+  StageScope st(ST_EIGPOST);
   gamma_sqrt(s, N, lamb.as<cd>(), N, gam, N, batch);
   zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, Q, N, sPQ, W, N, NN, ZERO, V, N, NN, batch);
   zdscale(s, N, N, nullptr, 0, gam, N, 1, V, N, NN, V, N, NN, batch);
@@ -198,6 +203,7 @@ void wave_eqn(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* W, cd* 
 // ===================================================================== (d) layer S-matrix
 void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
                    const cd* W0, const cd* V0, long sRef, cd* Tuu, cd* Rud) {
+  StageScope st(ST_LAYER_S);
   hipStream_t s = c.stream;
   const long NN = (long)N * N;
   DevBuf LUb = c.ws.get(sizeof(cd) * (size_t)NN * batch);
@@ -255,6 +261,7 @@ void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W,
 // ===================================================================== (d) star product
 void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
                   const cd* rdu, long ss) {
+  StageScope st(ST_STAR);
   hipStream_t s = c.stream;
   const long NN = (long)N * N;
   const size_t B = sizeof(cd) * (size_t)NN * batch;
@@ -293,6 +300,7 @@ void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd
 // is re-expressed with (W0,V0). LU1 holds the factorizations of W1 and V1.
 void project_left(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LU1, const cd* W0, const cd* V0,
                   long sRef) {
+  StageScope st(ST_PROJECT);
   hipStream_t s = c.stream;
   const long NN = (long)N * N;
   const size_t B = sizeof(cd) * (size_t)NN * batch;
@@ -330,6 +338,7 @@ void project_left(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LU1, cons
 // (Wl, Vl) (factorized in LUl) re-expressed with (W1, V1).
 void project_right(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LUl, const cd* W1, const cd* V1,
                    long sRef) {
+  StageScope st(ST_PROJECT);
   hipStream_t s = c.stream;
   const long NN = (long)N * N;
   const size_t B = sizeof(cd) * (size_t)NN * batch;

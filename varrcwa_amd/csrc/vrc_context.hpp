@@ -1,25 +1,86 @@
 // Library context: one device, one stream, one workspace pool, last error text.
 #pragma once
 
+#include <atomic>
 #include <string>
+#include <vector>
 
 #include "vrc_internal.hpp"
 
+// Reference counted: samplers and integrators hold a reference, so the pool and the
+// context outlive every object that leases device memory from them.
 struct vrc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  bool own_stream = false;
   vrc::Workspace ws;
   std::string last_error;
+  std::atomic<int> refs{1};
+  // stage timers (HIP events on the context stream), see vrc::StageScope
+  bool timing = false;
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> event_pool;
+  ~vrc_ctx() {
+    for (auto& r : recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : event_pool) (void)hipEventDestroy(e);
+  }
 };
 
 namespace vrc {
+inline void ctx_retain(vrc_ctx* c) { c->refs.fetch_add(1); }
+inline void ctx_release(vrc_ctx* c) {
+  if (c->refs.fetch_sub(1) == 1) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    delete c;
+  }
+}
+}  // namespace vrc
+
+namespace vrc {
+
+// Stage identifiers of the timers (also the index in vrc_timing_read).
+enum Stage : int {
+  ST_FOURIER = 0,    // (a) Fourier/Toeplitz assembly kernels
+  ST_PQ = 1,         // (b) fe33 factorization + P/Q formation (includes ST_FOURIER)
+  ST_HESS = 2,       // (c) Hessenberg reduction + Q formation
+  ST_QR = 3,         // (c) multishift QR kernel
+  ST_EIGVEC = 4,     // (c) triangular eigenvectors + back-transform + normalisation
+  ST_EIGPOST = 5,    // (c) PQ product, gamma, V = Q W / gamma
+  ST_LAYER_S = 6,    // (d) slab S-matrices
+  ST_STAR = 7,       // (d) star products
+  ST_PROJECT = 8,    // (d) end projections
+  ST_COUNT = 9
+};
+
+vrc_ctx*& tl_ctx();
+
+// Records a pair of events around a stage on the active context's stream when
+// timing is enabled; no-op otherwise.
+struct StageScope {
+  vrc_ctx* c = nullptr;
+  int stage;
+  hipEvent_t a = nullptr;
+  explicit StageScope(int st);
+  ~StageScope();
+};
 
 // Runs f, converting exceptions into the C-ABI status convention:
 // 0 ok, -1 bad argument, -2 HIP runtime failure, -3 other failure.
 template <class F>
 int guarded(vrc_ctx* c, F&& f) {
   if (!c) return -1;
+  struct Bind {
+    vrc_ctx* prev;
+    explicit Bind(vrc_ctx* c) : prev(tl_ctx()) { tl_ctx() = c; }
+    ~Bind() { tl_ctx() = prev; }
+  } bind(c);
   try {
     VRC_HIP(hipSetDevice(c->device));
     f();

@@ -50,3 +50,45 @@ void Workspace::trim() {
 }
 
 }  // namespace vrc
+
+// ---------------------------------------------------------------- stage timers
+#include "vrc_context.hpp"
+
+namespace vrc {
+
+vrc_ctx*& tl_ctx() {
+  static thread_local vrc_ctx* c = nullptr;
+  return c;
+}
+
+static hipEvent_t take_event(vrc_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  VRC_HIP(hipEventCreate(&e));
+  return e;
+}
+
+StageScope::StageScope(int st) : stage(st) {
+  vrc_ctx* cur = tl_ctx();
+  if (!cur || !cur->timing) return;
+  c = cur;
+  a = take_event(c);
+  VRC_HIP(hipEventRecord(a, c->stream));
+}
+
+StageScope::~StageScope() {
+  if (!c) return;
+  hipEvent_t b = nullptr;
+  try {
+    b = take_event(c);
+  } catch (...) {
+    return;
+  }
+  if (hipEventRecord(b, c->stream) == hipSuccess) c->recs.push_back({stage, a, b});
+}
+
+}  // namespace vrc
