@@ -61,7 +61,7 @@ def load():
     _sig("vrc_zgeev", i, vp, i, vp, l, l, vp, l, vp, l, l, vp, i)
     _optional = {
         "vrc_assemble_toeplitz": (i, vp, i, i, i, C.POINTER(VrcLayer), vp, vp, vp, l),
-        "vrc_assemble_pq": (i, vp, i, i, i, C.POINTER(VrcLayer), d, i, vp, vp, vp, vp, l),
+        "vrc_assemble_pq": (i, vp, i, i, i, C.POINTER(VrcLayer), d, i, vp, vp, vp, l),
         "vrc_eig_pq": (i, vp, i, i, vp, vp, l, vp, vp, vp, vp),
         "vrc_layer_smatrix": (i, vp, i, i, c_double_p, c_double_p, vp, vp, vp, vp, vp, l, vp, vp),
         "vrc_star_product": (i, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, l),
