@@ -114,7 +114,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--instances", type=int, default=16, help="instances per step")
+    ap.add_argument("--instances", type=int, default=21, help="instances per step (21 x 12 eigensolves ~ one per CU)")
     ap.add_argument("--nh", type=int, default=21, help="harmonics per axis (2*order+1)")
     ap.add_argument("--plain", type=int, default=1, help="instances timed through the plain path (0 = skip)")
     ap.add_argument("--cpu-slices", type=int, default=2, help="slices in the CPU sample (0 = skip)")
@@ -153,6 +153,8 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st = read_stages(ctx)
+    qs = (C.c_longlong * 6)()
+    ctx.check(ctx.lib.vrc_qr_stats(ctx.h, qs), "qr_stats")
     dt = t1 - t0
     slices = args.steps * len(insts) * L
     value = slices / dt
@@ -211,6 +213,11 @@ def main():
         "adjoint": "not built in this version",
         "stage_ms_per_step": {k: v / args.steps for k, v in excl.items()},
         "dominant_stage": dom,
+        "qr_stats": {"matrices": qs[0], "sweeps_per_matrix": qs[1] / max(qs[0], 1),
+                     "windows_per_matrix": qs[2] / max(qs[0], 1),
+                     "chase_ms_per_matrix": qs[3] / max(qs[0], 1) / 1e5,
+                     "apply_ms_per_matrix": qs[4] / max(qs[0], 1) / 1e5,
+                     "small_block_ms_per_matrix": qs[5] / max(qs[0], 1) / 1e5},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

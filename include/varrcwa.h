@@ -47,6 +47,10 @@ const char* vrc_version(void);
  * V formation, 6 slab S-matrices, 7 star products, 8 end projections) and resets. */
 int vrc_timing_enable(vrc_ctx* ctx, int on);
 int vrc_timing_read(vrc_ctx* ctx, double* ms, int* counts, int nstages);
+/* QR-iteration statistics gathered while timing is enabled, then reset: out[6] =
+ * {matrices, sweeps, windows, window-chase ticks, off-window-apply ticks,
+ * small-block ticks} (ticks of the device's constant-rate clock, 100 MHz). */
+int vrc_qr_stats(vrc_ctx* ctx, long long* out);
 
 /* ------------------------------------------------- dense complex128 kernels */
 /* op: 0 = N, 1 = T, 2 = C (conjugate transpose).

@@ -55,6 +55,7 @@ def load():
     _sig("vrc_pool_trim", i, vp)
     _sig("vrc_timing_enable", i, vp, i)
     _sig("vrc_timing_read", i, vp, c_double_p, c_int_p, i)
+    _sig("vrc_qr_stats", i, vp, C.POINTER(C.c_longlong))
     _sig("vrc_zgemm", i, vp, i, i, i, i, i, c_double_p, vp, l, l, vp, l, l, c_double_p, vp, l, l, i)
     _sig("vrc_zgetrf", i, vp, i, vp, l, l, vp, l, vp, i)
     _sig("vrc_zgetrs", i, vp, i, i, vp, l, l, vp, l, vp, l, l, vp, l, l, i)

@@ -82,6 +82,16 @@ int vrc_timing_read(vrc_ctx* c, double* ms, int* counts, int nstages) {
   });
 }
 
+int vrc_qr_stats(vrc_ctx* c, long long* out) {
+  return guarded(c, [&] {
+    need(out != nullptr, "null output");
+    for (int k = 0; k < 6; ++k) {
+      out[k] = c->qr_stats[k];
+      c->qr_stats[k] = 0;
+    }
+  });
+}
+
 int vrc_zgemm(vrc_ctx* c, int opA, int opB, int m, int n, int k, const double* alpha, const void* A, long lda,
               long sA, const void* B, long ldb, long sB, const double* beta, void* C, long ldc, long sC,
               int batch) {

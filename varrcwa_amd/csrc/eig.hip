@@ -24,6 +24,7 @@ namespace vrc {
 constexpr double ULP = 2.220446049250313e-16;      // 2^-52
 constexpr double SAFMIN = 2.2250738585072014e-308;
 constexpr int HB = 32;         // Hessenberg panel width
+constexpr int PT = 1024;       // threads of the Hessenberg panel workgroup (16 waves)
 constexpr int QW = 64;         // maximal QR window
 constexpr int NBUL = 10;       // bulges per sweep (2 shifts each)
 constexpr int QR_NMIN = 48;    // active blocks this small are finished in LDS
@@ -56,7 +57,7 @@ __device__ void panel_multi_dot(int N, int r0, int p, int j, const cd* Vf, const
   for (int i = 0; i < j; ++i) {
     const cd* vc = Vf + (long)(p + i) * N;
     double sr = 0.0, si = 0.0;
-    for (int r = r0 + t; r < N; r += 256) {
+    for (int r = r0 + t; r < N; r += PT) {
       cd a = vc[r], xb = x[r];
       sr += a.x * xb.x + a.y * xb.y;
       si += a.x * xb.y - a.y * xb.x;
@@ -66,11 +67,15 @@ __device__ void panel_multi_dot(int N, int r0, int p, int j, const cd* Vf, const
     if (lane == 0) red[wave][i] = mk(sr, si);
   }
   __syncthreads();
-  if (t < j) out[t] = cadd(cadd(red[0][t], red[1][t]), cadd(red[2][t], red[3][t]));
+  if (t < j) {
+    cd acc = mk(0.0, 0.0);
+    for (int w = 0; w < PT / 64; ++w) acc = cadd(acc, red[w][t]);
+    out[t] = acc;
+  }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void gehrd_panel_kernel(int N, int p, int nb, cd* A, long lda, long sA, cd* Vf,
+__global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, cd* A, long lda, long sA, cd* Vf,
                                                           long sV, cd* Y, long sY, cd* Tp, long sT, cd* tau,
                                                           long sTau) {
   const long b = blockIdx.x;
@@ -83,9 +88,9 @@ __global__ __launch_bounds__(256) void gehrd_panel_kernel(int N, int p, int nb, 
   cd* bvec = dyn;
   cd* vvec = dyn + N;
   __shared__ cd Tl[HB][HB + 1];
-  __shared__ cd red[4][HB];
+  __shared__ cd red[PT / 64][HB];
   __shared__ cd vec1[HB], vec2[HB];
-  __shared__ double rn[4];
+  __shared__ double rn[PT / 64];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
 
   for (int j = 0; j < nb; ++j) {
@@ -93,7 +98,7 @@ __global__ __launch_bounds__(256) void gehrd_panel_kernel(int N, int p, int nb, 
     if (t < j) vec1[t] = cconj(Vf[c + (long)(p + t) * N]);
     __syncthreads();
     // column c of A0*Q_j
-    for (int r = t; r < N; r += 256) {
+    for (int r = t; r < N; r += PT) {
       cd acc = A[r + (long)c * lda];
       for (int i = 0; i < j; ++i) acc = csub(acc, cmul(Y[r + (long)i * N], vec1[i]));
       bvec[r] = acc;
@@ -108,7 +113,7 @@ __global__ __launch_bounds__(256) void gehrd_panel_kernel(int N, int p, int nb, 
         vec1[t] = s;
       }
       __syncthreads();
-      for (int r = p + 1 + t; r < N; r += 256) {
+      for (int r = p + 1 + t; r < N; r += PT) {
         cd acc = bvec[r];
         for (int i = 0; i < j; ++i) acc = csub(acc, cmul(Vf[r + (long)(p + i) * N], vec1[i]));
         bvec[r] = acc;
@@ -117,19 +122,21 @@ __global__ __launch_bounds__(256) void gehrd_panel_kernel(int N, int p, int nb, 
     }
     // final entries of column c and the reflector from rows c+1..N-1
     double part = 0.0;
-    for (int r = t; r < N; r += 256) {
+    for (int r = t; r < N; r += PT) {
       if (r <= c) A[r + (long)c * lda] = bvec[r];
       if (r >= c + 2) part += cnorm2(bvec[r]);
     }
     part = wsum(part);
     if (lane == 0) rn[wave] = part;
     __syncthreads();
-    const double xnorm = sqrt(rn[0] + rn[1] + rn[2] + rn[3]);
+    double xn2 = 0.0;
+    for (int w = 0; w < PT / 64; ++w) xn2 += rn[w];
+    const double xnorm = sqrt(xn2);
     double beta;
     cd tv, scale;
     house(bvec[c + 1], xnorm, beta, tv, scale);
     const bool ident = (tv.x == 0.0 && tv.y == 0.0);
-    for (int r = t; r < N; r += 256) {
+    for (int r = t; r < N; r += PT) {
       cd v;
       if (r <= c) v = mk(0.0, 0.0);
       else if (r == c + 1) v = mk(1.0, 0.0);
@@ -143,18 +150,53 @@ __global__ __launch_bounds__(256) void gehrd_panel_kernel(int N, int p, int nb, 
     __syncthreads();
     // z = V_j^H v  (rows >= c+1)
     if (j > 0) panel_multi_dot(N, c + 1, p, j, Vf, vvec, red, vec2);
-    // y_j = tau*(A0[:, c+1:] v - Y_j z)
-    for (int r = t; r < N; r += 256) {
-      double gr = 0.0, gi = 0.0;
-      const cd* ar = A + r;
-      for (int col = c + 1; col < N; ++col) {
-        cd a = ar[(long)col * lda], v = vvec[col];
-        gr += a.x * v.x - a.y * v.y;
-        gi += a.x * v.y + a.y * v.x;
+    // y_j = tau*(A0[:, c+1:] v - Y_j z). The trailing-matrix product is the
+    // memory-bound part: every thread keeps 4 rows and 4 columns of independent
+    // loads in flight.
+    for (int rb = t; rb < N; rb += 4 * PT) {
+      double gr[4] = {0.0, 0.0, 0.0, 0.0}, gi[4] = {0.0, 0.0, 0.0, 0.0};
+      int col = c + 1;
+      for (; col + 3 < N; col += 4) {
+        cd v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = vvec[col + u];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = rb + PT * q;
+          if (r < N) {
+            const cd* ar = A + r + (long)col * lda;
+            cd a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] = ar[(long)u * lda];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              gr[q] += a[u].x * v[u].x - a[u].y * v[u].y;
+              gi[q] += a[u].x * v[u].y + a[u].y * v[u].x;
+            }
+          }
+        }
       }
-      cd acc = mk(gr, gi);
-      for (int i = 0; i < j; ++i) acc = csub(acc, cmul(Y[r + (long)i * N], vec2[i]));
-      Y[r + (long)j * N] = cmul(tv, acc);
+      for (; col < N; ++col) {
+        const cd v = vvec[col];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = rb + PT * q;
+          if (r < N) {
+            const cd a = A[r + (long)col * lda];
+            gr[q] += a.x * v.x - a.y * v.y;
+            gi[q] += a.x * v.y + a.y * v.x;
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = rb + PT * q;
+        if (r < N) {
+          cd acc = mk(gr[q], gi[q]);
+          for (int i = 0; i < j; ++i) acc = csub(acc, cmul(Y[r + (long)i * N], vec2[i]));
+          Y[r + (long)j * N] = cmul(tv, acc);
+        }
+      }
     }
     if (t < j) {
       cd s = mk(0.0, 0.0);
@@ -164,7 +206,7 @@ __global__ __launch_bounds__(256) void gehrd_panel_kernel(int N, int p, int nb, 
     if (t == 0) Tl[j][j] = tv;
     __syncthreads();
   }
-  for (int e = t; e < nb * nb; e += 256) {
+  for (int e = t; e < nb * nb; e += PT) {
     int i = e % nb, jj = e / nb;
     Tp[i + (long)jj * HB] = (i <= jj) ? Tl[i][jj] : mk(0.0, 0.0);
   }
@@ -190,7 +232,7 @@ static void gehrd(hipStream_t s, int n, cd* A, long lda, long sA, cd* Vf, cd* Ta
   for (int p = 0; p < nref; p += HB) {
     const int nb = nref - p < HB ? nref - p : HB;
     cd* Tp = Tall + (long)(p / HB) * HB * HB;
-    gehrd_panel_kernel<<<batch, 256, 2 * sizeof(cd) * n, s>>>(n, p, nb, A, lda, sA, Vf, sV, Y, sY, Tp, sT,
+    gehrd_panel_kernel<<<batch, PT, 2 * sizeof(cd) * n, s>>>(n, p, nb, A, lda, sA, Vf, sV, Y, sY, Tp, sT,
                                                                 taub.as<cd>(), n);
     launch_check("gehrd_panel");
     const int c0 = p + nb, ncols = n - c0;
@@ -289,6 +331,7 @@ __device__ __forceinline__ void mm64_for_each(const d4 (&accR)[2][2], const d4 (
 }
 
 // M[r0:r0+w, c0:c1] <- U^H * M[...]  (U: w x w in LDS, zero padded), chunk buffer C.
+// The next chunk is prefetched into registers while the current one is multiplied.
 __device__ void apply_left_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, int r0, int c0, int c1) {
   const int t = threadIdx.x;
   for (int cc = c0; cc < c1; cc += QW) {
@@ -306,8 +349,6 @@ __device__ void apply_left_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, 
     __syncthreads();
   }
 }
-
-// M[r0:r1, c0:c0+w] <- M[...] * U.
 __device__ void apply_right_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, int r0, int r1, int c0) {
   const int t = threadIdx.x;
   for (int rr = r0; rr < r1; rr += QW) {
@@ -325,7 +366,6 @@ __device__ void apply_right_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld,
     __syncthreads();
   }
 }
-
 // ===================================================================== QR iteration
 struct QrLds {
   LdsMat Hw;
@@ -532,7 +572,8 @@ __device__ int find_lo(QrLds& L, cd* H, long ldh, int N, int hi) {
 }
 
 // One multishift sweep over the active block [lo, hi] with nbul bulges (shifts in L.shifts).
-__device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int lo, int hi, int nbul) {
+__device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int lo, int hi, int nbul,
+                         long long* st) {
   const int t = threadIdx.x;
   const int adv = QW - 3 * nbul - 1;
   const int T_end = (hi - 2) - (lo - 1) + 3 * (nbul - 1) + 1;
@@ -545,6 +586,7 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
     const int ws = pmin > lo ? pmin : lo;
     const int we = pmax + 4 < hi ? pmax + 4 : hi;
     const int w = we - ws + 1;
+    const long long c0 = wall_clock64();
     lds_load_block(L, H, ldh, ws, w);
     for (int tt = t0; tt < t1; ++tt) {
       // reflectors of all bulges at this step, from the current window state
@@ -591,61 +633,65 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
         L.rsize[bb] = size;
       }
       __syncthreads();
-      if (t < w) {
-        const int j = t;
-        for (int bb = 0; bb < nbul; ++bb) {
-          const int sz = L.rsize[bb];
-          if (sz == 0) continue;
-          const int r1 = L.rrow[bb];
-          if (j == L.rcol[bb]) {
-            L.Hw[r1][j] = mk(L.rbeta[bb], 0.0);
-            L.Hw[r1 + 1][j] = mk(0.0, 0.0);
-            if (sz == 3) L.Hw[r1 + 2][j] = mk(0.0, 0.0);
-            continue;
-          }
-          const cd v1 = L.rv1[bb], v2 = L.rv2[bb], ct = cconj(L.rtau[bb]);
-          cd h1 = L.Hw[r1][j], h2 = L.Hw[r1 + 1][j];
-          cd h3 = sz == 3 ? L.Hw[r1 + 2][j] : mk(0.0, 0.0);
-          cd sm = cadd(h1, cmul(cconj(v1), h2));
-          if (sz == 3) sm = cadd(sm, cmul(cconj(v2), h3));
-          sm = cmul(ct, sm);
-          L.Hw[r1][j] = csub(h1, sm);
-          L.Hw[r1 + 1][j] = csub(h2, cmul(sm, v1));
-          if (sz == 3) L.Hw[r1 + 2][j] = csub(h3, cmul(sm, v2));
+      // row operations: items (bulge, column), spread over the workgroup
+      for (int it = t; it < nbul * w; it += 256) {
+        const int bb = it / w, j = it - bb * w;
+        const int sz = L.rsize[bb];
+        if (sz == 0) continue;
+        const int r1 = L.rrow[bb];
+        if (j == L.rcol[bb]) {
+          L.Hw[r1][j] = mk(L.rbeta[bb], 0.0);
+          L.Hw[r1 + 1][j] = mk(0.0, 0.0);
+          if (sz == 3) L.Hw[r1 + 2][j] = mk(0.0, 0.0);
+          continue;
         }
+        const cd v1 = L.rv1[bb], v2 = L.rv2[bb], ct = cconj(L.rtau[bb]);
+        cd h1 = L.Hw[r1][j], h2 = L.Hw[r1 + 1][j];
+        cd h3 = sz == 3 ? L.Hw[r1 + 2][j] : mk(0.0, 0.0);
+        cd sm = cadd(h1, cmul(cconj(v1), h2));
+        if (sz == 3) sm = cadd(sm, cmul(cconj(v2), h3));
+        sm = cmul(ct, sm);
+        L.Hw[r1][j] = csub(h1, sm);
+        L.Hw[r1 + 1][j] = csub(h2, cmul(sm, v1));
+        if (sz == 3) L.Hw[r1 + 2][j] = csub(h3, cmul(sm, v2));
       }
       __syncthreads();
-      {
-        const bool onH = t < w;
-        const bool onU = (t >= 64 && t < 64 + w);
-        if (onH || onU) {
-          const int i = onH ? t : t - 64;
-          LdsMat& M = onH ? L.Hw : L.U;
-          for (int bb = 0; bb < nbul; ++bb) {
-            const int sz = L.rsize[bb];
-            if (sz == 0) continue;
-            const int r1 = L.rrow[bb];
-            const cd v1 = L.rv1[bb], v2 = L.rv2[bb], tu = L.rtau[bb];
-            cd c1 = M[i][r1], c2 = M[i][r1 + 1];
-            cd c3 = sz == 3 ? M[i][r1 + 2] : mk(0.0, 0.0);
-            cd sm = cadd(c1, cmul(c2, v1));
-            if (sz == 3) sm = cadd(sm, cmul(c3, v2));
-            sm = cmul(tu, sm);
-            M[i][r1] = csub(c1, sm);
-            M[i][r1 + 1] = csub(c2, cmul(sm, cconj(v1)));
-            if (sz == 3) M[i][r1 + 2] = csub(c3, cmul(sm, cconj(v2)));
-          }
-        }
+      // column operations on the window and on U: items (matrix, bulge, row)
+      for (int it = t; it < 2 * nbul * w; it += 256) {
+        const int mat = it / (nbul * w);
+        const int rem = it - mat * nbul * w;
+        const int bb = rem / w, i = rem - bb * w;
+        const int sz = L.rsize[bb];
+        if (sz == 0) continue;
+        LdsMat& M = mat == 0 ? L.Hw : L.U;
+        const int r1 = L.rrow[bb];
+        const cd v1 = L.rv1[bb], v2 = L.rv2[bb], tu = L.rtau[bb];
+        cd c1 = M[i][r1], c2 = M[i][r1 + 1];
+        cd c3 = sz == 3 ? M[i][r1 + 2] : mk(0.0, 0.0);
+        cd sm = cadd(c1, cmul(c2, v1));
+        if (sz == 3) sm = cadd(sm, cmul(c3, v2));
+        sm = cmul(tu, sm);
+        M[i][r1] = csub(c1, sm);
+        M[i][r1 + 1] = csub(c2, cmul(sm, cconj(v1)));
+        if (sz == 3) M[i][r1 + 2] = csub(c3, cmul(sm, cconj(v2)));
       }
       __syncthreads();
     }
     lds_store_block(L, H, ldh, ws, w);
+    const long long c1 = wall_clock64();
     apply_window(L, N, H, ldh, Z, ldz, ws, w);
+    const long long c2 = wall_clock64();
+    st[1] += 1;
+    st[2] += c1 - c0;
+    st[3] += c2 - c1;
   }
 }
 
+// stats (nullable): per matrix {sweeps, windows, window-chase ticks, off-window-apply ticks,
+// small-block ticks} of the constant-rate device clock.
 __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long sH, cd* Z, long ldz, long sZ,
-                                                  cd* wout, long sw, int* info) {
+                                                  cd* wout, long sw, int* info, long long* stats) {
+  long long st[5] = {0, 0, 0, 0, 0};
   __shared__ QrLds L;
   const long b = blockIdx.x;
   H += b * sH;
@@ -663,6 +709,7 @@ __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long s
     prev_lo = lo;
     prev_hi = ihi;
     if (m <= QR_NMIN) {
+      const long long c0 = wall_clock64();
       lds_load_block(L, H, ldh, lo, m);
       const int r = small_schur(L, m, true);
       lds_store_block(L, H, ldh, lo, m);
@@ -671,6 +718,7 @@ __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long s
         break;
       }
       apply_window(L, N, H, ldh, Z, ldz, lo, m);
+      st[4] += wall_clock64() - c0;
       ihi = lo - 1;
       continue;
     }
@@ -705,12 +753,18 @@ __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long s
       }
       __syncthreads();
     }
-    qr_sweep(L, N, H, ldh, Z, ldz, lo, ihi, nbul);
+    qr_sweep(L, N, H, ldh, Z, ldz, lo, ihi, nbul, st);
     ++sweeps;
   }
   __syncthreads();
   for (int i = t; i < N; i += 256) wout[i] = H[i + (long)i * ldh];
-  if (t == 0) info[b] = fail;
+  if (t == 0) {
+    info[b] = fail;
+    if (stats) {
+      st[0] = sweeps;
+      for (int k = 0; k < 5; ++k) stats[b * 5 + k] = st[k];
+    }
+  }
 }
 
 // ===================================================================== eigenvectors
@@ -867,8 +921,20 @@ void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
   }
   {
     StageScope st(ST_QR);
-    hqr_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info);
+    vrc_ctx* c = tl_ctx();
+    const bool want = c && c->timing;
+    DevBuf sb;
+    if (want) sb = ws.get(sizeof(long long) * 5 * batch);
+    hqr_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, want ? sb.as<long long>() : nullptr);
     launch_check("hqr");
+    if (want) {
+      std::vector<long long> h(5 * (size_t)batch);
+      VRC_HIP(hipMemcpyAsync(h.data(), sb.ptr, sizeof(long long) * h.size(), hipMemcpyDeviceToHost, s));
+      VRC_HIP(hipStreamSynchronize(s));
+      c->qr_stats[0] += batch;
+      for (int b = 0; b < batch; ++b)
+        for (int k = 0; k < 5; ++k) c->qr_stats[1 + k] += h[(size_t)b * 5 + k];
+    }
   }
   StageScope st(ST_EIGVEC);
   cd* X = Vf;  // reflectors no longer needed

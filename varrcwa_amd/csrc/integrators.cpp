@@ -534,9 +534,9 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     DevBuf dab = upload(c, ab.data(), sizeof(cd) * ab.size());
     DevBuf dsv = upload(c, sv.data(), sizeof(double) * sv.size());
     DevBuf down = upload(c, owner.data(), sizeof(int) * owner.size());
-    // ---- ends: P, Q, eig
-    DevBuf Wend = c.ws.get(sizeof(cd) * NN * 2 * B), Vend = c.ws.get(sizeof(cd) * NN * 2 * B),
-           gend = c.ws.get(sizeof(cd) * (size_t)N * 2 * B), infe = c.ws.get(sizeof(int) * 2 * B);
+    // ---- one eigen-solve launch for end media and slices: [bottoms | tops | slices]
+    DevBuf Wall = c.ws.get(sizeof(cd) * NN * J), Vall = c.ws.get(sizeof(cd) * NN * J),
+           gall = c.ws.get(sizeof(cd) * (size_t)N * J), infall = c.ws.get(sizeof(int) * J);
     auto solve_jobs = [&](int j0, int cnt, cd* W, cd* V, cd* gam, int* inf) {
       AssemblyBatch a;
       a.nx = nx;
@@ -552,25 +552,22 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
       assemble_pq(c, a, P.as<cd>(), Q.as<cd>());
       wave_eqn(c, N, P.as<cd>(), Q.as<cd>(), NN, W, V, gam, inf, cnt);
     };
-    solve_jobs(0, 2 * B, Wend.as<cd>(), Vend.as<cd>(), gend.as<cd>(), infe.as<int>());
-    const cd* W0 = Wend.as<cd>();
-    const cd* V0 = Vend.as<cd>();
-    const cd* W1 = Wend.as<cd>() + B * NN;
-    const cd* V1 = Vend.as<cd>() + B * NN;
+    solve_jobs(0, J, Wall.as<cd>(), Vall.as<cd>(), gall.as<cd>(), infall.as<int>());
+    const cd* W0 = Wall.as<cd>();
+    const cd* V0 = Vall.as<cd>();
+    const cd* W1 = Wall.as<cd>() + B * NN;
+    const cd* V1 = Vall.as<cd>() + B * NN;
     LUStore lu1;
     lu1.make(c, N, W1, V1, B, NN);
-    // ---- slices: eig + layer S (reference basis of the owning instance)
+    // ---- slices: layer S in the reference basis of the owning instance
     const int SL = B * L;
     DevBuf tu = c.ws.get(sizeof(cd) * NN * SL), ru = c.ws.get(sizeof(cd) * NN * SL);
     {
-      DevBuf W = c.ws.get(sizeof(cd) * NN * SL), V = c.ws.get(sizeof(cd) * NN * SL),
-             g = c.ws.get(sizeof(cd) * (size_t)N * SL), inf = c.ws.get(sizeof(int) * SL);
-      solve_jobs(2 * B, SL, W.as<cd>(), V.as<cd>(), g.as<cd>(), inf.as<int>());
       DevBuf Wr = c.ws.get(sizeof(cd) * NN * SL), Vr = c.ws.get(sizeof(cd) * NN * SL);
       gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
       gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
-      layer_smatrix(c, N, SL, dsv.as<double>(), W.as<cd>(), V.as<cd>(), g.as<cd>(), Wr.as<cd>(), Vr.as<cd>(), NN,
-                    tu.as<cd>(), ru.as<cd>());
+      layer_smatrix(c, N, SL, dsv.as<double>(), Wall.as<cd>() + 2 * B * NN, Vall.as<cd>() + 2 * B * NN,
+                    gall.as<cd>() + (long)2 * B * N, Wr.as<cd>(), Vr.as<cd>(), NN, tu.as<cd>(), ru.as<cd>());
     }
     // ---- cascade over layers, batched over instances
     DevBuf Sb = c.ws.get(sizeof(cd) * 4 * NN * B);
@@ -598,10 +595,10 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     };
     out(W0o, W0, NN);
     out(V0o, V0, NN);
-    out(g0o, gend.as<cd>(), N);
+    out(g0o, gall.as<cd>(), N);
     out(W1o, W1, NN);
     out(V1o, V1, NN);
-    out(g1o, gend.as<cd>() + (long)B * N, N);
+    out(g1o, gall.as<cd>() + (long)B * N, N);
   }
 }
 

@@ -23,6 +23,9 @@ struct vrc_ctx {
   };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> event_pool;
+  // QR iteration statistics accumulated while timing: matrices, sweeps, windows,
+  // chase ticks, apply ticks, small-block ticks
+  long long qr_stats[6] = {0, 0, 0, 0, 0, 0};
   ~vrc_ctx() {
     for (auto& r : recs) {
       (void)hipEventDestroy(r.a);
