@@ -1,17 +1,18 @@
 #!/usr/bin/env python3
 """S-matrix throughput on one MI355X: slice solves/s at 441 harmonics (N = 882), complex128.
 
-One step = one batched call of the library (``vrc_batch_stack_rcwa``) over a batch of
-instances of the rectangle-stack workload (tests/golden/metric_geometry.json): each
-instance is 10 slabs (1-4 rectangles, eps in {4, 12.25}, 0.6 um cell, thickness
-0.05-0.2 um) between air half-spaces at one wavelength in 0.8-1.6 um, Fourier orders
-+-10 in x and y. A slice solve = Toeplitz assembly + P/Q + eig(PQ) + slab S-matrix +
-star product into the cascade; every step also solves the two end media and the
-end projection of each instance (counted as overhead, not as slices).
+One step = one batched library call (``vrc_batch_stack_rcwa``) over the 512-instance
+rectangle-stack sweep (tests/golden/metric_geometry.json): 8 seeded geometries x 64
+wavelengths in 0.8-1.6 um; each instance is 10 slabs (1-4 rectangles, eps in {4, 12.25},
+0.6 um cell, thickness 0.05-0.2 um) between air half-spaces, Fourier orders +-10 in x
+and y. A slice solve = Toeplitz assembly + P/Q + eig(PQ) + slab S-matrix + star product
+into the cascade; every instance also solves its two end media and the end projection
+(counted as overhead, not as slices). The library splits the batch into launches of at
+most 256 eigensolves.
 
-Also reported: the plain per-instance path (RCWAIntegrator over the same stack, one
-slice after another), live stage times from HIP events on the library stream, the
-roofline of the dominant kernel, and the CPU oracle timed on a bounded sample.
+Also reported: the plain per-instance path (RCWAIntegrator over one instance's stack,
+one slice after another), live stage times from HIP events on the library stream, the
+roofline of the dominant kernel, and the CPU oracle timed on one whole instance.
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -69,9 +70,9 @@ def read_stages(ctx):
     return {s: (ms[i], cnt[i]) for i, s in enumerate(STAGES)}
 
 
-def cpu_baseline(nh: int, nslices: int):
-    """Oracle (restated reference CPU path, numpy/LAPACK) on the first slices of
-    instance 0: per slice, sample + eig + slab S-matrix + star product."""
+def cpu_baseline(nh: int):
+    """Oracle (restated reference CPU path, numpy/LAPACK) on instance 0 of the sweep:
+    RCWAIntegrator over its 10 slabs (end media, slices, star products, projection)."""
     from oracle import rcwa_oracle as O
     try:
         from threadpoolctl import threadpool_limits
@@ -84,45 +85,36 @@ def cpu_baseline(nh: int, nslices: int):
     air = (np.array([0.0, cell]), np.array([0.0, cell]), np.array([[1.0 + 0j]]))
     layers = [(np.array(l["x"]), np.array(l["y"]), np.array(l["eps"], dtype=complex)) for l in g["layers"]]
     ze = np.concatenate([[0.0], np.cumsum(g["thickness"])])
-    nx = ny = nh
-    s = O.StackLayerSampler(lam, nx, ny, layers, ze, air, air)
+    s = O.StackLayerSampler(lam, nh, nh, layers, ze, air, air)
     cores = min(16, os.cpu_count() or 1)
     ctxm = threadpool_limits(limits=cores) if threadpool_limits else None
     try:
-        P1, Q1 = s.sample(ze[-1])
-        W1, V1, _, _ = O.wave_equation_solve(P1, Q1)
-        nD = W1.shape[0]
-        S = (np.eye(nD, dtype=complex), np.eye(nD, dtype=complex), np.zeros((nD, nD), complex),
-             np.zeros((nD, nD), complex))
         t0 = time.perf_counter()
-        for i in range(nslices):
-            zm = 0.5 * (ze[i] + ze[i + 1])
-            P, Q = s.sample(zm)
-            tuu, rud = O.rcwa_scatter_matrix(lam, ze[i + 1] - ze[i], P, Q, W1, V1)
-            S = O.star_product(S, tuu, tuu, rud, rud)
+        O.rcwa_integrate(s, ze[0], ze[-1], len(layers), z_edges=ze)
         t = time.perf_counter() - t0
     finally:
         if ctxm is not None:
             ctxm.__exit__(None, None, None)
-    return dict(value=nslices / t, unit="slice solves/s", cores=cores, kind="port",
-                sample=f"{nslices} slices of instance 0 (N={2 * nx * ny}, numpy/OpenBLAS LAPACK, "
-                       f"{t:.1f} s)")
+    nsl = len(layers)
+    return dict(value=nsl / t, unit="slice solves/s", cores=cores, kind="port",
+                sample=f"instance 0 of the sweep ({nsl} slabs + 2 end media + projection, N={2 * nh * nh}; "
+                       f"numpy/OpenBLAS LAPACK, {cores} threads, {t:.1f} s)")
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--instances", type=int, default=21, help="instances per step (21 x 12 eigensolves ~ one per CU)")
+    ap.add_argument("--instances", type=int, default=512, help="instances per step (8 geometries x 64 wavelengths)")
     ap.add_argument("--nh", type=int, default=21, help="harmonics per axis (2*order+1)")
     ap.add_argument("--plain", type=int, default=1, help="instances timed through the plain path (0 = skip)")
-    ap.add_argument("--cpu-slices", type=int, default=2, help="slices in the CPU sample (0 = skip)")
+    ap.add_argument("--cpu", type=int, default=1, help="time the CPU oracle on one instance (0 = skip)")
     ap.add_argument("--chunk", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(instances shard with no exchange; see DESIGN.md)"}))
+                                   "(instances would shard across GPUs with no exchange)"}))
         sys.exit(2)
 
     import torch
@@ -193,8 +185,8 @@ def main():
         del it, s
 
     cpu = None
-    if args.cpu_slices > 0:
-        cpu = cpu_baseline(nh, args.cpu_slices)
+    if args.cpu > 0:
+        cpu = cpu_baseline(nh)
         log(f"cpu: {cpu}")
 
     frac = flops_slice(N) * value / (FP64_PEAK_TFLOPS * 1e12)
@@ -203,13 +195,14 @@ def main():
         "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "c128", "data": "synthetic",
         "config": {"workload": f"rectangle stacks, {nh}x{nh} harmonics (N={N}), {L} slabs/instance, "
-                               f"{len(insts)} instances/step (8 seeded geometries x wavelengths in 0.8-1.6 um)",
+                               f"{len(insts)} instances/step (8 seeded geometries x 64 wavelengths in 0.8-1.6 um)",
                    "instances_per_step": len(insts), "slices_per_step": len(insts) * L, "N": N,
                    "parallelism": "single GPU"},
         "fp64_matrix_frac": frac,
         "path_tflops": flops_slice(N) * value / 1e12,
         "plain_slices_per_s": plain,
         "batched_over_plain": (value / plain) if plain else None,
+        "batched_over_cpu": (value / cpu["value"]) if cpu else None,
         "adjoint": "not built in this version",
         "stage_ms_per_step": {k: v / args.steps for k, v in excl.items()},
         "dominant_stage": dom,

@@ -203,7 +203,9 @@ int vrc_integrator_tree(vrc_integrator* it, int cap, double* host_out);
  * slabs of given thickness sampled once each (RCWAIntegrator semantics per slab),
  * half-space media bottom/top as end bases. All instances of a call share nx, ny
  * and nlayers. S_out: per instance [Tdd|Tuu|Rud|Rdu] (4 N x N, device); the
- * optional W0/V0/gam0/W1/V1/gam1 outputs may be NULL. chunk <= 0 picks a size. */
+ * optional W0/V0/gam0/W1/V1/gam1 outputs may be NULL. Instances are solved in chunks
+ * of at most `chunk` (<= 0: as many as keep one launch within 256 eigensolves), split
+ * evenly over the chunks. */
 typedef struct vrc_stack_instance {
   double lambda;
   int nlayers;

@@ -1,0 +1,295 @@
+// varrcwa.hpp -- header-only C++ API over the C ABI of libvarrcwa.so, with the
+// class names, constructors, compute signatures and getters of the reference's
+// CPU path, so that a caller of the reference can switch by changing includes:
+//
+//   reference                                  here
+//   VarLayerSampler (src/core/VarLayerSampler.h:8-39)      varrcwa::VarLayerSampler
+//   RCWAIntegrator (src/core/RCWAIntegrator.h:8-40)        varrcwa::RCWAIntegrator
+//   RedhefferIntegrator (src/core/RedhefferIntegrator.h:10-90) varrcwa::RedhefferIntegrator
+//   DifferentialIntegrator (src/core/DifferentialIntegrator.h:8-40) varrcwa::DifferentialIntegrator
+//
+// Matrices are column-major complex128 (`MatrixXcs`, `VectorXcs` below; the memory
+// layout of Eigen::MatrixXcd, so `Eigen::Map<const Eigen::MatrixXcd>(m.data(),
+// m.rows(), m.cols())` views one without a copy). Errors throw varrcwa::Error with
+// the library's message. All objects of one thread share a per-device context on the
+// device's default stream unless a Context is passed explicitly.
+#pragma once
+
+#include <complex>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "varrcwa.h"
+
+namespace varrcwa {
+
+using scalar = double;
+using scalex = std::complex<double>;
+
+class Error : public std::runtime_error {
+ public:
+  Error(const std::string& what, int code) : std::runtime_error(what), code(code) {}
+  int code;
+};
+
+// Column-major dense complex matrix (vector = one column).
+class MatrixXcs {
+ public:
+  MatrixXcs() = default;
+  MatrixXcs(long rows, long cols) : r_(rows), c_(cols), v_((size_t)(rows * cols)) {}
+  long rows() const { return r_; }
+  long cols() const { return c_; }
+  long size() const { return r_ * c_; }
+  void resize(long rows, long cols) {
+    r_ = rows;
+    c_ = cols;
+    v_.assign((size_t)(rows * cols), scalex(0.0, 0.0));
+  }
+  scalex& operator()(long i, long j) { return v_[(size_t)(i + j * r_)]; }
+  const scalex& operator()(long i, long j) const { return v_[(size_t)(i + j * r_)]; }
+  scalex& operator()(long i) { return v_[(size_t)i]; }
+  const scalex& operator()(long i) const { return v_[(size_t)i]; }
+  scalex* data() { return v_.data(); }
+  const scalex* data() const { return v_.data(); }
+
+ private:
+  long r_ = 0, c_ = 0;
+  std::vector<scalex> v_;
+};
+using VectorXcs = MatrixXcs;
+using VectorXs = std::vector<double>;
+
+namespace detail {
+inline void check(int rc, const vrc_ctx* c, const char* what) {
+  if (rc != 0) {
+    std::string msg = std::string(what) + " failed";
+    if (c) msg += std::string(": ") + vrc_last_error(c);
+    throw Error(msg, rc);
+  }
+}
+}  // namespace detail
+
+// Owns a vrc_ctx (device + stream + device memory pool).
+class Context {
+ public:
+  explicit Context(int device = 0, void* stream = nullptr) {
+    vrc_ctx* h = nullptr;
+    detail::check(vrc_ctx_create(device, stream, &h), nullptr, "vrc_ctx_create");
+    h_ = h;
+  }
+  ~Context() {
+    if (h_) vrc_ctx_destroy(h_);
+  }
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+  vrc_ctx* get() const { return h_; }
+  // Per-thread default context of device 0 on the default stream.
+  static std::shared_ptr<Context> thread_default() {
+    thread_local std::shared_ptr<Context> c = std::make_shared<Context>(0, nullptr);
+    return c;
+  }
+
+ private:
+  vrc_ctx* h_ = nullptr;
+};
+
+// One Layer(x, y, eps) slab (src/rcwa/Layer.cpp:18-34): eps is (ny-1) x (nx-1),
+// row i = y strip, column j = x cell.
+struct LayerDesc {
+  VectorXs x, y;
+  MatrixXcs eps;
+  vrc_layer view() const {
+    if (eps.rows() != (long)y.size() - 1 || eps.cols() != (long)x.size() - 1)
+      throw Error("eps must be (y.size()-1) x (x.size()-1)", -1);
+    return vrc_layer{(int)x.size(), x.data(), (int)y.size(), y.data(), reinterpret_cast<const double*>(eps.data())};
+  }
+};
+
+// LayerSampler (src/core/LayerSampler.h:5-77): common part of the geometric samplers.
+class LayerSampler {
+ public:
+  virtual ~LayerSampler() {
+    if (s_) vrc_sampler_destroy(s_);
+  }
+  LayerSampler(const LayerSampler&) = delete;
+  LayerSampler& operator=(const LayerSampler&) = delete;
+
+  scalar lambda() const { return lambda_; }
+  scalar k0() const { return 2 * 3.1415926535897932 / lambda_; }
+  int nDim() const { return vrc_sampler_ndim(s_); }
+  const MatrixXcs& Kx() const { return Kx_; }
+  const MatrixXcs& Ky() const { return Ky_; }
+
+  // LayerSampler::sample (src/core/LayerSampler.h:49-51): host P, Q (resized).
+  void sample(scalar z, MatrixXcs& P, MatrixXcs& Q) {
+    const int N = nDim();
+    P.resize(N, N);
+    Q.resize(N, N);
+    detail::check(vrc_sampler_sample(s_, z, reinterpret_cast<double*>(P.data()), reinterpret_cast<double*>(Q.data())),
+                  ctx_->get(), "sample");
+  }
+  // LayerSampler::sampleOnGPU (src/core/LayerSampler.h:53-55): device N x N buffers.
+  void sampleOnGPU(scalar z, void* P, void* Q) {
+    detail::check(vrc_sampler_sample_gpu(s_, z, P, Q), ctx_->get(), "sampleOnGPU");
+  }
+
+  vrc_sampler* handle() const { return s_; }
+  const std::shared_ptr<Context>& context() const { return ctx_; }
+
+ protected:
+  LayerSampler(scalar lambda, std::shared_ptr<Context> ctx)
+      : lambda_(lambda), ctx_(ctx ? std::move(ctx) : Context::thread_default()) {}
+  void finish(vrc_sampler* s, int n) {
+    s_ = s;
+    Kx_.resize(n, n);
+    Ky_.resize(n, n);
+    detail::check(vrc_sampler_k(s_, reinterpret_cast<double*>(Kx_.data()), reinterpret_cast<double*>(Ky_.data())),
+                  ctx_->get(), "Kx/Ky");
+  }
+  scalar lambda_;
+  std::shared_ptr<Context> ctx_;
+  vrc_sampler* s_ = nullptr;
+  MatrixXcs Kx_, Ky_;
+};
+
+// VarLayerSampler (src/core/VarLayerSampler.h:8-39).
+class VarLayerSampler : public LayerSampler {
+ public:
+  VarLayerSampler(scalar lambda, int nx, int ny, const VectorXs& x, const VectorXs& y, const MatrixXcs& eps,
+                  bool enablePML = true, std::shared_ptr<Context> ctx = nullptr)
+      : LayerSampler(lambda, std::move(ctx)) {
+    base_.x = x;
+    base_.y = y;
+    base_.eps = eps;
+    vrc_layer l = base_.view();
+    vrc_sampler* s = nullptr;
+    detail::check(vrc_var_sampler_create(ctx_->get(), lambda, nx, ny, &l, enablePML ? 1 : 0, &s), ctx_->get(),
+                  "VarLayerSampler");
+    finish(s, nx * ny);
+  }
+  // x(layout) = rule(z) at every sample (VarLayerSampler::addXRule, src/core/VarLayerSampler.cpp:42-50).
+  void addXRule(int layout, std::function<scalar(scalar)> rule) { add(0, layout, std::move(rule)); }
+  void addYRule(int layout, std::function<scalar(scalar)> rule) { add(1, layout, std::move(rule)); }
+
+ private:
+  static double trampoline(double z, void* user) { return (*static_cast<std::function<scalar(scalar)>*>(user))(z); }
+  void add(int axis, int layout, std::function<scalar(scalar)> rule) {
+    rules_.push_back(std::make_unique<std::function<scalar(scalar)>>(std::move(rule)));
+    detail::check(vrc_sampler_add_rule(s_, axis, layout, &trampoline, rules_.back().get()), ctx_->get(),
+                  axis == 0 ? "addXRule" : "addYRule");
+  }
+  LayerDesc base_;
+  std::vector<std::unique_ptr<std::function<scalar(scalar)>>> rules_;
+};
+
+// Piecewise-constant stack of layers (no reference counterpart; extension):
+// layer k occupies (z_edges[k], z_edges[k+1]); bottom below, top above.
+class StackLayerSampler : public LayerSampler {
+ public:
+  StackLayerSampler(scalar lambda, int nx, int ny, const std::vector<LayerDesc>& layers, const VectorXs& z_edges,
+                    const LayerDesc& bottom, const LayerDesc& top, bool enablePML = false,
+                    std::shared_ptr<Context> ctx = nullptr)
+      : LayerSampler(lambda, std::move(ctx)) {
+    std::vector<vrc_layer> ls;
+    for (auto& l : layers) ls.push_back(l.view());
+    vrc_layer b = bottom.view(), t = top.view();
+    vrc_sampler* s = nullptr;
+    detail::check(vrc_stack_sampler_create(ctx_->get(), lambda, nx, ny, (int)ls.size(), ls.data(), z_edges.data(), &b,
+                                           &t, enablePML ? 1 : 0, &s),
+                  ctx_->get(), "StackLayerSampler");
+    finish(s, nx * ny);
+  }
+};
+
+namespace detail {
+// Common state of the three integrators: handle, sampler reference, host copies of
+// the results (filled after each compute).
+class IntegratorBase {
+ public:
+  virtual ~IntegratorBase() {
+    if (it_) vrc_integrator_destroy(it_);
+  }
+  IntegratorBase(const IntegratorBase&) = delete;
+  IntegratorBase& operator=(const IntegratorBase&) = delete;
+
+  const MatrixXcs& Tdd() const { return m_[0]; }
+  const MatrixXcs& Tuu() const { return m_[1]; }
+  const MatrixXcs& Rud() const { return m_[2]; }
+  const MatrixXcs& Rdu() const { return m_[3]; }
+  const MatrixXcs& W0() const { return m_[4]; }
+  const MatrixXcs& V0() const { return m_[5]; }
+  const VectorXcs& gam0() const { return m_[6]; }
+  const MatrixXcs& W1() const { return m_[7]; }
+  const MatrixXcs& V1() const { return m_[8]; }
+  const VectorXcs& gam1() const { return m_[9]; }
+  vrc_integrator* handle() const { return it_; }
+
+ protected:
+  IntegratorBase(int kind, scalar z0, scalar z1, std::shared_ptr<LayerSampler> sampler)
+      : sampler_(std::move(sampler)) {
+    if (!sampler_) throw Error("null sampler", -1);
+    vrc_integrator* it = nullptr;
+    check(vrc_integrator_create(ctx(), kind, z0, z1, sampler_->handle(), &it), ctx(), "integrator");
+    it_ = it;
+  }
+  vrc_ctx* ctx() const { return sampler_->context()->get(); }
+  void fetch() {
+    const int N = sampler_->nDim();
+    for (int w = 0; w < 10; ++w) {
+      const bool vec = (w == 6 || w == 9);
+      m_[w].resize(N, vec ? 1 : N);
+      check(vrc_integrator_get(it_, w, reinterpret_cast<double*>(m_[w].data())), ctx(), "result read-back");
+    }
+    check(vrc_integrator_counts(it_, &N_, &nEig_), ctx(), "counts");
+  }
+  std::shared_ptr<LayerSampler> sampler_;
+  vrc_integrator* it_ = nullptr;
+  MatrixXcs m_[10];
+  int N_ = 0, nEig_ = 0;
+};
+}  // namespace detail
+
+// RCWAIntegrator (src/core/RCWAIntegrator.h:8-40): N midpoint slices.
+class RCWAIntegrator : public detail::IntegratorBase {
+ public:
+  RCWAIntegrator(scalar z0, scalar z1, std::shared_ptr<LayerSampler> sampler)
+      : IntegratorBase(0, z0, z1, std::move(sampler)) {}
+  void compute(int N) {
+    detail::check(vrc_rcwa_compute(it_, N), ctx(), "RCWAIntegrator::compute");
+    fetch();
+  }
+  // Same result; the slices' eigenproblems are solved in batches.
+  void computeBatched(int N) {
+    detail::check(vrc_rcwa_compute_batched(it_, N), ctx(), "RCWAIntegrator::computeBatched");
+    fetch();
+  }
+};
+
+// RedhefferIntegrator (src/core/RedhefferIntegrator.h:10-90): adaptive trisection.
+class RedhefferIntegrator : public detail::IntegratorBase {
+ public:
+  RedhefferIntegrator(scalar z0, scalar z1, std::shared_ptr<LayerSampler> sampler)
+      : IntegratorBase(1, z0, z1, std::move(sampler)) {}
+  void compute(int p, int maxN = 10, scalar pe = 1e-2) {
+    detail::check(vrc_redheffer_compute(it_, p, maxN, pe), ctx(), "RedhefferIntegrator::compute");
+    fetch();
+  }
+  int N() const { return N_; }
+  int nEigVal() const { return nEig_; }
+};
+
+// DifferentialIntegrator (src/core/DifferentialIntegrator.h:8-40).
+class DifferentialIntegrator : public detail::IntegratorBase {
+ public:
+  DifferentialIntegrator(scalar z0, scalar z1, std::shared_ptr<LayerSampler> sampler)
+      : IntegratorBase(2, z0, z1, std::move(sampler)) {}
+  void compute(int N) {
+    detail::check(vrc_differential_compute(it_, N), ctx(), "DifferentialIntegrator::compute");
+    fetch();
+  }
+};
+
+}  // namespace varrcwa

@@ -53,9 +53,25 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp_check()
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources", file=sys.stderr)
+        print(f"built {LIB} from {len(srcs)} sources (+ {CPP_CHECK.name})", file=sys.stderr)
     return LIB
+
+
+CPP_CHECK = HERE.parent / "tests" / "cpp" / "cpp_api_check"
+
+
+def build_cpp_check() -> Path:
+    """Host-only C++ program over include/varrcwa.hpp, linked against the library."""
+    src = CPP_CHECK.with_suffix(".cpp")
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-O2", "-std=c++17", "-I", str(HERE.parent / "include"), str(src), "-o", str(CPP_CHECK),
+           "-L", str(HERE), "-lvarrcwa", "-Wl,-rpath,$ORIGIN/../../varrcwa_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src.name}:\n{r.stdout}\n{r.stderr}")
+    return CPP_CHECK
 
 
 if __name__ == "__main__":

@@ -501,9 +501,16 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     if (!I.bottom || !I.top) throw ArgFailure("instance without bottom/top medium");
     if (!(I.lambda > 0)) throw ArgFailure("wavelength must be positive");
   }
-  if (chunk <= 0) chunk = std::max(1, std::min(ninst, 256 / std::max(1, L)));
-  for (int i0 = 0; i0 < ninst; i0 += chunk) {
-    const int B = std::min(chunk, ninst - i0);
+  // Default chunk: at most 256 eigensolves (L slices + 2 end media per instance) per
+  // launch, one matrix per CU of the QR kernel; the instances are then split evenly
+  // over the chunks so that no launch runs a short tail.
+  if (chunk <= 0) chunk = std::max(1, std::min(ninst, 256 / (L + 2)));
+  const int nchunks = (ninst + chunk - 1) / chunk;
+  std::vector<int> starts(nchunks + 1, 0);
+  for (int ci = 0; ci < nchunks; ++ci) starts[ci + 1] = starts[ci] + ninst / nchunks + (ci < ninst % nchunks ? 1 : 0);
+  for (int ci = 0; ci < nchunks; ++ci) {
+    const int i0 = starts[ci];
+    const int B = starts[ci + 1] - i0;
     // ---- jobs: [bottoms(B) | tops(B) | slices(B*L, instance-major)]
     const int J = 2 * B + B * L;
     std::vector<const HostLayer*> jl(J);
