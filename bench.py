@@ -32,7 +32,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "S-matrix solves/s @441 harmonics c128, batched+adjoint vs plain; FP64 MFMA %"
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
-STAGES = ["fourier", "pq", "hessenberg", "qr", "eigvec", "eigpost", "layer_s", "star", "project"]
+STAGES = ["fourier", "pq", "hessenberg", "qr", "eigvec", "eigpost", "layer_s", "star", "project", "adjoint"]
 
 
 def flops_slice(N):
@@ -101,6 +101,44 @@ def cpu_baseline(nh: int):
                        f"numpy/OpenBLAS LAPACK, {cores} threads, {t:.1f} s)")
 
 
+def adjoint_bench(ctx, nh: int, insts, log):
+    """Forward S-matrices with the adjoint tape + the reverse pass for the objective
+    sum_i zeroth-order transmission (TE+TM), timed against the plain batched forward
+    on the same instances (one launch chunk)."""
+    import torch
+    from varrcwa_amd import api
+    from varrcwa_amd.autograd import StackProblem, stack_smatrix, zeroth_order_transmission
+    plain = [dict(i, layers=[(l.x, l.y, l.eps) for l in i["layers"]]) for i in insts]
+    prob = StackProblem(ctx, nh, nh, plain)
+    eps = prob.eps_vector().real.clone().to("cuda").requires_grad_(True)
+
+    def fwd_bwd():
+        S, W0, W1 = stack_smatrix(eps, prob)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J = zeroth_order_transmission(S, W0, W1, nh, nh).sum()
+        J.backward()
+        torch.cuda.synchronize()
+        return t1, time.perf_counter()
+
+    fwd_bwd()  # warm
+    eps.grad = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t1, t2 = fwd_bwd()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    api.batch_stack_rcwa(ctx, nh, nh, plain)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    fwd_tape, bwd, fwd = t1 - t0, t2 - t1, t4 - t3
+    log(f"adjoint: {len(insts)} instances, forward {fwd * 1e3:.0f} ms, forward+tape {fwd_tape * 1e3:.0f} ms, "
+        f"reverse {bwd * 1e3:.0f} ms")
+    return {"instances": len(insts), "objective": "sum of 0th-order transmitted power (TE+TM) per instance",
+            "gradient_wrt": f"{prob.cells} eps cells", "forward_ms": fwd * 1e3, "forward_with_tape_ms": fwd_tape * 1e3,
+            "reverse_ms": bwd * 1e3, "reverse_over_forward": bwd / fwd, "gradient_over_forward": (fwd_tape + bwd) / fwd}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +149,7 @@ def main():
     ap.add_argument("--plain", type=int, default=1, help="instances timed through the plain path (0 = skip)")
     ap.add_argument("--cpu", type=int, default=1, help="time the CPU oracle on one instance (0 = skip)")
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--adjoint", type=int, default=21, help="instances in the adjoint timing (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -184,6 +223,11 @@ def main():
         log(f"plain: {args.plain * L} slices in {tp:.2f} s -> {plain:.3f} slices/s")
         del it, s
 
+    adj = None
+    if args.adjoint > 0:
+        adj = adjoint_bench(ctx, nh, insts[:args.adjoint], log)
+        read_stages(ctx)
+
     cpu = None
     if args.cpu > 0:
         cpu = cpu_baseline(nh)
@@ -203,7 +247,7 @@ def main():
         "plain_slices_per_s": plain,
         "batched_over_plain": (value / plain) if plain else None,
         "batched_over_cpu": (value / cpu["value"]) if cpu else None,
-        "adjoint": "not built in this version",
+        "adjoint": adj,
         "stage_ms_per_step": {k: v / args.steps for k, v in excl.items()},
         "dominant_stage": dom,
         "qr_stats": {"matrices": qs[0], "sweeps_per_matrix": qs[1] / max(qs[0], 1),

@@ -44,8 +44,13 @@ const char* vrc_version(void);
  * the context stream. vrc_timing_read synchronizes, returns the accumulated
  * milliseconds and launch counts per stage (0 Fourier assembly, 1 P/Q assembly
  * incl. stage 0, 2 Hessenberg + Q, 3 QR iteration, 4 eigenvectors, 5 PQ product and
- * V formation, 6 slab S-matrices, 7 star products, 8 end projections) and resets. */
+ * V formation, 6 slab S-matrices, 7 star products, 8 end projections, 9 reverse
+ * pass) and resets. */
 int vrc_timing_enable(vrc_ctx* ctx, int on);
+/* Kernel tuning switches (process-wide; no reference counterpart). Keys:
+ * "zgemm.large_tile" (default 1): 128x64-tile ZGEMM for m >= 128, n >= 64, else the
+ * 64x64 tile. Returns -1 for an unknown key. */
+int vrc_tune_set(const char* key, long long value);
 int vrc_timing_read(vrc_ctx* ctx, double* ms, int* counts, int nstages);
 /* QR-iteration statistics gathered while timing is enabled, then reset: out[6] =
  * {matrices, sweeps, windows, window-chase ticks, off-window-apply ticks,
@@ -216,6 +221,63 @@ typedef struct vrc_stack_instance {
 } vrc_stack_instance;
 int vrc_batch_stack_rcwa(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
                          void* S_out, void* W0, void* V0, void* gam0, void* W1, void* V1, void* gam1);
+
+/* ------------------------------------------------------ adjoint (gradients) */
+/* No reference counterpart: the reference differentiates nothing on this path.
+ * Convention for every *_vjp: for a real objective L and complex Z, Z_bar is the
+ * array with dL = Re sum(conj(Z_bar) * dZ) (PyTorch's convention; L = |z|^2 gives
+ * z_bar = 2z). Cotangent inputs are device arrays in the layout of the forward
+ * output, results are written in the layout of the forward input.
+ *
+ * vrc_batch_stack_forward: vrc_batch_stack_rcwa that also returns a tape (NULL tape
+ * pointer = no tape) holding the geometry and every instance's eigen-systems
+ * (bottom, top, slabs: 2 N x N + N complex each) for the reverse pass.
+ * vrc_batch_stack_vjp: eps_bar (vrc_tape_cells complex values) receives, per
+ * instance and slab in order, the cotangent of every eps cell of the slab in the
+ * layout of vrc_layer.eps, for S_bar = the cotangent of S_out ([Tdd|Tuu|Rud|Rdu]
+ * per instance). Slab coordinates must be non-decreasing (no painter's refill),
+ * periodic cells only. chunk (<= 0: automatic) bounds the instances per pass. */
+typedef struct vrc_tape vrc_tape;
+int vrc_batch_stack_forward(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
+                            void* S_out, void* W0, void* V0, void* gam0, void* W1, void* V1, void* gam1,
+                            vrc_tape** tape);
+long long vrc_tape_cells(const vrc_tape* tape);
+int vrc_batch_stack_vjp(vrc_tape* tape, const void* S_bar, void* eps_bar, int chunk);
+int vrc_tape_destroy(vrc_tape* tape);
+
+/* Stage adjoints (each the vector-Jacobian product of the forward entry point of
+ * the same name above, with the same argument layout):
+ *  - vrc_assemble_toeplitz_vjp: fe11/fe22/fe33 cotangents -> eps_bar (all layers'
+ *    cells concatenated, layer order, vrc_layer.eps layout);
+ *  - vrc_assemble_pq_vjp: periodic P/Q cotangents -> eps_bar (region gives the cell);
+ *  - vrc_eig_pq_vjp: (W_bar, V_bar, gam_bar) -> (P_bar, Q_bar); W_bar and gam_bar are
+ *    overwritten. Valid for objectives invariant under the eigenvector gauge
+ *    (W -> W D with V -> V D), as every S-matrix objective is;
+ *  - vrc_layer_smatrix_vjp: (Tuu_bar, Rud_bar) -> (W_bar, V_bar, gam_bar), reference
+ *    basis constant;
+ *  - vrc_star_product_vjp: S (Tdd..Rdu) is the cascade before the product; the S_bar
+ *    arrays hold the cotangent after the product on entry and before it on return;
+ *    slice cotangents (tdd_bar..rdu_bar) are written with stride N*N;
+ *  - vrc_project_left_vjp: S is the cascade before the projection; S_bar as above. */
+int vrc_assemble_toeplitz_vjp(vrc_ctx* ctx, int nx, int ny, int nlayers, const vrc_layer* layers,
+                              const void* fe11_bar, const void* fe22_bar, const void* fe33_bar, long stride,
+                              void* eps_bar);
+int vrc_assemble_pq_vjp(vrc_ctx* ctx, int nx, int ny, int nlayers, const vrc_layer* layers, double lambda,
+                        const vrc_layer* region, const void* P_bar, const void* Q_bar, long stride, void* eps_bar);
+int vrc_eig_pq_vjp(vrc_ctx* ctx, int N, int batch, const void* P, const void* Q, long stride, const void* W,
+                   const void* V, const void* gam, void* W_bar, const void* V_bar, void* gam_bar, void* P_bar,
+                   void* Q_bar);
+int vrc_layer_smatrix_vjp(vrc_ctx* ctx, int N, int batch, const double* lambda_host, const double* dz_host,
+                          const void* W, const void* V, const void* gam, const void* W0, const void* V0,
+                          long ref_stride, const void* Tuu_bar, const void* Rud_bar, void* W_bar, void* V_bar,
+                          void* gam_bar);
+int vrc_star_product_vjp(vrc_ctx* ctx, int N, int batch, const void* Tdd, const void* Tuu, const void* Rud,
+                         const void* Rdu, const void* tdd, const void* tuu, const void* rud, const void* rdu,
+                         long slice_stride, void* Tdd_bar, void* Tuu_bar, void* Rud_bar, void* Rdu_bar,
+                         void* tdd_bar, void* tuu_bar, void* rud_bar, void* rdu_bar);
+int vrc_project_left_vjp(vrc_ctx* ctx, int N, int batch, const void* Tdd, const void* Tuu, const void* Rud,
+                         const void* Rdu, const void* W1, const void* V1, const void* W0, const void* V0,
+                         long ref_stride, void* Tdd_bar, void* Tuu_bar, void* Rud_bar, void* Rdu_bar);
 
 #ifdef __cplusplus
 }

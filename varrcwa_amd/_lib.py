@@ -54,6 +54,7 @@ def load():
     _sig("vrc_pool_bytes", C.c_longlong, vp)
     _sig("vrc_pool_trim", i, vp)
     _sig("vrc_timing_enable", i, vp, i)
+    _sig("vrc_tune_set", i, C.c_char_p, C.c_longlong)
     _sig("vrc_timing_read", i, vp, c_double_p, c_int_p, i)
     _sig("vrc_qr_stats", i, vp, C.POINTER(C.c_longlong))
     _sig("vrc_zgemm", i, vp, i, i, i, i, i, c_double_p, vp, l, l, vp, l, l, c_double_p, vp, l, l, i)
@@ -88,6 +89,17 @@ def load():
         "vrc_integrator_counts": (i, vp, c_int_p, c_int_p),
         "vrc_integrator_tree": (i, vp, i, c_double_p),
         "vrc_batch_stack_rcwa": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp),
+        "vrc_batch_stack_forward": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp,
+                                    C.POINTER(vp)),
+        "vrc_tape_cells": (C.c_longlong, vp),
+        "vrc_batch_stack_vjp": (i, vp, vp, vp, i),
+        "vrc_tape_destroy": (i, vp),
+        "vrc_assemble_toeplitz_vjp": (i, vp, i, i, i, C.POINTER(VrcLayer), vp, vp, vp, l, vp),
+        "vrc_assemble_pq_vjp": (i, vp, i, i, i, C.POINTER(VrcLayer), d, C.POINTER(VrcLayer), vp, vp, l, vp),
+        "vrc_eig_pq_vjp": (i, vp, i, i, vp, vp, l, vp, vp, vp, vp, vp, vp, vp, vp),
+        "vrc_layer_smatrix_vjp": (i, vp, i, i, c_double_p, c_double_p, vp, vp, vp, vp, vp, l, vp, vp, vp, vp, vp),
+        "vrc_star_product_vjp": (i, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, l, vp, vp, vp, vp, vp, vp, vp, vp),
+        "vrc_project_left_vjp": (i, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, l, vp, vp, vp, vp),
     }
     for name, (res, *args) in _optional.items():
         if hasattr(_lib, name):

@@ -17,8 +17,10 @@ from ._lib import (RULE_FN, Context, VrcError, VrcLayer, VrcStackInstance, c_dou
                    load, ptr, to_dev_colmajor)
 
 __all__ = ["Context", "Layer", "VarLayerSampler", "StackLayerSampler", "RCWAIntegrator", "RedhefferIntegrator",
-           "DifferentialIntegrator", "batch_stack_rcwa", "assemble_toeplitz", "assemble_pq", "eig_pq",
-           "layer_smatrix", "star_product", "project_left", "redheffer_segment", "VrcError"]
+           "DifferentialIntegrator", "batch_stack_rcwa", "StackTape", "assemble_toeplitz", "assemble_pq", "eig_pq",
+           "layer_smatrix", "star_product", "project_left", "redheffer_segment", "assemble_toeplitz_vjp",
+           "assemble_pq_vjp", "eig_pq_vjp", "layer_smatrix_vjp", "star_product_vjp", "project_left_vjp",
+           "VrcError"]
 
 _DEFAULT_CTX: Optional[Context] = None
 
@@ -236,17 +238,9 @@ class DifferentialIntegrator(_Integrator):
 
 
 # ------------------------------------------------------------------ batched stacks
-def batch_stack_rcwa(ctx: Context, nx: int, ny: int, instances, chunk: int = 0, want_bases: bool = False):
-    """instances: list of dicts with keys lam, layers (Layer list), thickness, bottom, top.
-
-    Returns a dict of device tensors: S (ninst, 4, N, N) as column-major storage
-    (use ``from_dev_colmajor``), and W0, V0, gam0, W1, V1, gam1 when requested.
-    """
-    torch = ctx.torch
-    n = nx * ny
-    N = 2 * n
+def _stack_array(instances):
     keep = []
-    arr = (VrcStackInstance * len(instances))()
+    arr = (VrcStackInstance * max(1, len(instances)))()
     for i, inst in enumerate(instances):
         ls = [_as_layer(l) for l in inst["layers"]]
         la = _layer_array(ls)
@@ -255,6 +249,52 @@ def batch_stack_rcwa(ctx: Context, nx: int, ny: int, instances, chunk: int = 0, 
         keep += [ls, la, th, bot, top]
         arr[i] = VrcStackInstance(float(inst["lam"]), len(ls), la, th.ctypes.data_as(c_double_p),
                                   C.pointer(bot.struct), C.pointer(top.struct))
+    return arr, keep
+
+
+class StackTape:
+    """Reverse-pass state of one ``batch_stack_rcwa(..., tape=True)`` call."""
+
+    def __init__(self, ctx: Context, h: C.c_void_p):
+        self.ctx, self.h = ctx, h
+
+    def cells(self) -> int:
+        """Length of the permittivity gradient (all slab cells of all instances)."""
+        return int(self.ctx.lib.vrc_tape_cells(self.h))
+
+    def vjp(self, S_bar, chunk: int = 0):
+        """Cotangent of every slab eps cell (device complex128 vector, instance then
+        slab order, each slab's cells column-major) for the cotangent S_bar of S."""
+        torch = self.ctx.torch
+        S_bar = S_bar.contiguous()
+        out = torch.empty(self.cells(), dtype=torch.complex128, device=S_bar.device)
+        self.ctx.check(self.ctx.lib.vrc_batch_stack_vjp(self.h, ptr(S_bar), ptr(out), int(chunk)), "batch_stack_vjp")
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.vrc_tape_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_stack_rcwa(ctx: Context, nx: int, ny: int, instances, chunk: int = 0, want_bases: bool = False,
+                     tape: bool = False):
+    """instances: list of dicts with keys lam, layers (Layer list), thickness, bottom, top.
+
+    Returns a dict of device tensors: S (ninst, 4, N, N) as column-major storage
+    (use ``from_dev_colmajor``), W0, V0, gam0, W1, V1, gam1 when requested, and a
+    StackTape under "tape" when tape=True.
+    """
+    torch = ctx.torch
+    n = nx * ny
+    N = 2 * n
+    arr, keep = _stack_array(instances)
     dev = f"cuda:{ctx.device}"
     S = torch.empty((len(instances), 4, N, N), dtype=torch.complex128, device=dev)
     out = {"S": S}
@@ -264,8 +304,15 @@ def batch_stack_rcwa(ctx: Context, nx: int, ny: int, instances, chunk: int = 0, 
             shape = (len(instances), N) if name.startswith("gam") else (len(instances), N, N)
             out[name] = torch.empty(shape, dtype=torch.complex128, device=dev)
             ptrs[k] = ptr(out[name])
-    ctx.check(ctx.lib.vrc_batch_stack_rcwa(ctx.h, nx, ny, len(instances), arr, int(chunk), ptr(S), *ptrs),
-              "batch_stack_rcwa")
+    if tape:
+        h = C.c_void_p()
+        ctx.check(ctx.lib.vrc_batch_stack_forward(ctx.h, nx, ny, len(instances), arr, int(chunk), ptr(S), *ptrs,
+                                                  C.byref(h)), "batch_stack_forward")
+        out["tape"] = StackTape(ctx, h)
+    else:
+        ctx.check(ctx.lib.vrc_batch_stack_rcwa(ctx.h, nx, ny, len(instances), arr, int(chunk), ptr(S), *ptrs),
+                  "batch_stack_rcwa")
+    del keep
     return out
 
 
@@ -341,3 +388,78 @@ def redheffer_segment(ctx: Context, lam, z0, z1, Pr, Qr, W, V, gam, P3, Q3):
                                             ptr(Q3[1]), ptr(Q3[2]), *[ptr(o) for o in outs], C.byref(err)),
               "redheffer_segment")
     return outs, err.value
+
+
+# ------------------------------------------------------------------ stage adjoints
+def _cells(layers) -> int:
+    return sum(l.eps.size for l in layers)
+
+
+def assemble_toeplitz_vjp(ctx: Context, nx: int, ny: int, layers, fe_bar):
+    """fe_bar: (3, L, n, n) cotangents of (fe11, fe22, fe33) in the storage layout of
+    assemble_toeplitz -> eps cotangents (all layers' cells, column-major, concatenated)."""
+    torch = ctx.torch
+    ls = [_as_layer(l) for l in layers]
+    n = nx * ny
+    fe_bar = fe_bar.contiguous()
+    out = torch.empty(_cells(ls), dtype=torch.complex128, device=fe_bar.device)
+    ctx.check(ctx.lib.vrc_assemble_toeplitz_vjp(ctx.h, nx, ny, len(ls), _layer_array(ls), ptr(fe_bar[0]),
+                                                ptr(fe_bar[1]), ptr(fe_bar[2]), n * n, ptr(out)),
+              "assemble_toeplitz_vjp")
+    return out
+
+
+def assemble_pq_vjp(ctx: Context, nx: int, ny: int, layers, lam: float, region, P_bar, Q_bar):
+    torch = ctx.torch
+    ls = [_as_layer(l) for l in layers]
+    reg = _as_layer(region)
+    N = 2 * nx * ny
+    P_bar, Q_bar = P_bar.contiguous(), Q_bar.contiguous()
+    out = torch.empty(_cells(ls), dtype=torch.complex128, device=P_bar.device)
+    ctx.check(ctx.lib.vrc_assemble_pq_vjp(ctx.h, nx, ny, len(ls), _layer_array(ls), float(lam), C.byref(reg.struct),
+                                          ptr(P_bar), ptr(Q_bar), N * N, ptr(out)), "assemble_pq_vjp")
+    return out
+
+
+def eig_pq_vjp(ctx: Context, P, Q, W, V, gam, W_bar, V_bar, gam_bar):
+    torch = ctx.torch
+    B, N, _ = P.shape
+    Wb, gb = W_bar.clone().contiguous(), gam_bar.clone().contiguous()
+    Pb, Qb = torch.empty_like(P), torch.empty_like(Q)
+    ctx.check(ctx.lib.vrc_eig_pq_vjp(ctx.h, N, B, ptr(P), ptr(Q), N * N, ptr(W), ptr(V), ptr(gam), ptr(Wb),
+                                     ptr(V_bar.contiguous()), ptr(gb), ptr(Pb), ptr(Qb)), "eig_pq_vjp")
+    return Pb, Qb
+
+
+def layer_smatrix_vjp(ctx: Context, lam, dz, W, V, gam, W0, V0, Tuu_bar, Rud_bar, shared_ref: bool = True):
+    torch = ctx.torch
+    B, N, _ = W.shape
+    lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float64), (B,)))
+    dz = np.ascontiguousarray(np.broadcast_to(np.asarray(dz, np.float64), (B,)))
+    Wb, Vb = torch.empty_like(W), torch.empty_like(V)
+    gb = torch.empty_like(gam)
+    ctx.check(ctx.lib.vrc_layer_smatrix_vjp(ctx.h, N, B, lam.ctypes.data_as(c_double_p),
+                                            dz.ctypes.data_as(c_double_p), ptr(W), ptr(V), ptr(gam), ptr(W0),
+                                            ptr(V0), 0 if shared_ref else N * N, ptr(Tuu_bar.contiguous()),
+                                            ptr(Rud_bar.contiguous()), ptr(Wb), ptr(Vb), ptr(gb)),
+              "layer_smatrix_vjp")
+    return Wb, Vb, gb
+
+
+def star_product_vjp(ctx: Context, S0, s, S_bar):
+    """S0, s, S_bar: (4, B, N, N) [dd, uu, ud, du]. Returns (cotangent of S0, of s)."""
+    _, B, N, _ = S0.shape
+    Sb = S_bar.clone().contiguous()
+    sb = ctx.torch.empty_like(S0)
+    ctx.check(ctx.lib.vrc_star_product_vjp(ctx.h, N, B, *[ptr(S0[k]) for k in range(4)], *[ptr(s[k]) for k in range(4)],
+                                           N * N, *[ptr(Sb[k]) for k in range(4)], *[ptr(sb[k]) for k in range(4)]),
+              "star_product_vjp")
+    return Sb, sb
+
+
+def project_left_vjp(ctx: Context, S, W1, V1, W0, V0, S_bar):
+    _, B, N, _ = S.shape
+    Sb = S_bar.clone().contiguous()
+    ctx.check(ctx.lib.vrc_project_left_vjp(ctx.h, N, B, *[ptr(S[k]) for k in range(4)], ptr(W1), ptr(V1), ptr(W0),
+                                           ptr(V0), N * N, *[ptr(Sb[k]) for k in range(4)]), "project_left_vjp")
+    return Sb

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <memory>
 
+#include "vrc_adjoint.hpp"
 #include "vrc_context.hpp"
 
 using namespace vrc;
@@ -54,6 +55,15 @@ int vrc_pool_trim(vrc_ctx* c) {
     VRC_HIP(hipStreamSynchronize(c->stream));
     c->ws.trim();
   });
+}
+
+int vrc_tune_set(const char* key, long long value) {
+  if (!key) return -1;
+  if (std::strcmp(key, "zgemm.large_tile") == 0) {
+    zgemm_set_big(value != 0);
+    return 0;
+  }
+  return -1;
 }
 
 int vrc_timing_enable(vrc_ctx* c, int on) {
@@ -491,16 +501,15 @@ int vrc_integrator_tree(vrc_integrator* it, int cap, double* out) {
   return (int)t.size();
 }
 
-int vrc_batch_stack_rcwa(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
-                         void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1) {
-  return guarded(c, [&] {
-    check_harmonics(nx, ny);
-    need(ninst >= 0 && (ninst == 0 || inst) && S_out, "bad arguments");
-    std::vector<std::vector<HostLayer>> keep(ninst);
-    std::vector<HostLayer> bottoms, tops;
+namespace {
+// Host copies of a batch of stack instances (the StackInstance pointers refer into it).
+struct HostStack {
+  std::vector<std::vector<HostLayer>> keep;
+  std::vector<HostLayer> bottoms, tops;
+  std::vector<StackInstance> si;
+  HostStack(int ninst, const vrc_stack_instance* inst) : keep(ninst), si(ninst) {
     bottoms.reserve(ninst);
     tops.reserve(ninst);
-    std::vector<StackInstance> si(ninst);
     for (int i = 0; i < ninst; ++i) {
       const vrc_stack_instance& I = inst[i];
       need(I.nlayers >= 1 && I.thickness, "instance needs >= 1 layer and thicknesses");
@@ -515,7 +524,214 @@ int vrc_batch_stack_rcwa(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_
       si[i].bottom = &bottoms[i];
       si[i].top = &tops[i];
     }
-    batch_stack_rcwa(*c, nx, ny, si, chunk, (cd*)S_out, (cd*)W0, (cd*)V0, (cd*)g0, (cd*)W1, (cd*)V1, (cd*)g1);
+  }
+};
+}  // namespace
+
+struct vrc_tape {
+  vrc_ctx* ctx;
+  std::unique_ptr<StackTape> t;
+};
+
+int vrc_batch_stack_rcwa(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
+                         void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1) {
+  return vrc_batch_stack_forward(c, nx, ny, ninst, inst, chunk, S_out, W0, V0, g0, W1, V1, g1, nullptr);
+}
+
+int vrc_batch_stack_forward(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
+                            void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1,
+                            vrc_tape** tape) {
+  if (tape) *tape = nullptr;
+  return guarded(c, [&] {
+    check_harmonics(nx, ny);
+    need(ninst >= 0 && (ninst == 0 || inst) && S_out, "bad arguments");
+    HostStack hs(ninst, inst);
+    std::unique_ptr<vrc_tape> tp;
+    if (tape) {
+      tp = std::make_unique<vrc_tape>();
+      tp->ctx = c;
+      tp->t = make_stack_tape(*c, nx, ny, hs.si);
+    }
+    batch_stack_rcwa(*c, nx, ny, hs.si, chunk, (cd*)S_out, (cd*)W0, (cd*)V0, (cd*)g0, (cd*)W1, (cd*)V1, (cd*)g1,
+                     tp ? tp->t.get() : nullptr);
+    if (tape) {
+      ctx_retain(c);
+      *tape = tp.release();
+    }
+  });
+}
+
+long long vrc_tape_cells(const vrc_tape* t) { return (t && t->t) ? (long long)t->t->cells() : -1; }
+
+int vrc_batch_stack_vjp(vrc_tape* t, const void* S_bar, void* eps_bar, int chunk) {
+  if (!t) return -1;
+  return guarded(t->ctx, [&] {
+    need(S_bar && eps_bar, "bad arguments");
+    stack_tape_vjp(*t->t, (const cd*)S_bar, (cd*)eps_bar, chunk);
+  });
+}
+
+int vrc_tape_destroy(vrc_tape* t) {
+  if (!t) return 0;
+  vrc_ctx* c = t->ctx;
+  delete t;  // device buffers go back to the context's pool (stream-ordered reuse)
+  ctx_release(c);
+  return 0;
+}
+
+int vrc_assemble_toeplitz_vjp(vrc_ctx* c, int nx, int ny, int nlayers, const vrc_layer* layers, const void* fe11b,
+                              const void* fe22b, const void* fe33b, long stride, void* eps_bar) {
+  return guarded(c, [&] {
+    check_harmonics(nx, ny);
+    need(nlayers >= 0 && fe11b && fe22b && fe33b && eps_bar, "bad arguments");
+    const long n = (long)nx * ny;
+    need(stride >= n * n || nlayers <= 1, "stride smaller than n*n");
+    std::vector<HostLayer> hl = to_host(layers, nlayers);
+    std::vector<const HostLayer*> lp;
+    std::vector<long> off;
+    long o = 0;
+    for (auto& l : hl) {
+      lp.push_back(&l);
+      off.push_back(o);
+      o += (long)l.eps.size();
+    }
+    if (nlayers == 0) return;
+    assemble_fe_vjp(*c, nx, ny, lp, (const cd*)fe11b, (const cd*)fe22b, (const cd*)fe33b, stride, (cd*)eps_bar, off);
+  });
+}
+
+int vrc_assemble_pq_vjp(vrc_ctx* c, int nx, int ny, int nlayers, const vrc_layer* layers, double lambda,
+                        const vrc_layer* region, const void* Pb, const void* Qb, long stride, void* eps_bar) {
+  return guarded(c, [&] {
+    check_harmonics(nx, ny);
+    need(nlayers >= 0 && Pb && Qb && eps_bar && region, "bad arguments");
+    need(lambda > 0, "wavelength must be positive");
+    const long N = 2L * nx * ny;
+    need(stride == N * N, "P/Q stride must be N*N");
+    if (nlayers == 0) return;
+    std::vector<HostLayer> hl = to_host(layers, nlayers);
+    HostLayer reg = to_host(region);
+    const int n = nx * ny;
+    std::vector<cd> al, be;
+    alpha_beta(nx, ny, reg.x.back() - reg.x.front(), reg.y.back() - reg.y.front(), al, be);
+    std::vector<cd> ab(2 * (size_t)n);
+    std::copy(al.begin(), al.end(), ab.begin());
+    std::copy(be.begin(), be.end(), ab.begin() + n);
+    const double k0 = 2.0 * 3.1415926535897932 / lambda;
+    std::vector<double> k2(nlayers, k0 * k0);
+    DevBuf dab = c->ws.get(sizeof(cd) * ab.size()), dk = c->ws.get(sizeof(double) * nlayers);
+    VRC_HIP(hipMemcpyAsync(dab.ptr, ab.data(), sizeof(cd) * ab.size(), hipMemcpyHostToDevice, c->stream));
+    VRC_HIP(hipMemcpyAsync(dk.ptr, k2.data(), sizeof(double) * nlayers, hipMemcpyHostToDevice, c->stream));
+    AssemblyBatch a;
+    a.nx = nx;
+    a.ny = ny;
+    a.nlayers = nlayers;
+    a.periodic = true;
+    std::vector<long> off;
+    long o = 0;
+    for (auto& l : hl) {
+      a.layers.push_back(&l);
+      off.push_back(o);
+      o += (long)l.eps.size();
+    }
+    a.d_alpha = dab.as<cd>();
+    a.d_beta = a.d_alpha + n;
+    a.sab = 0;
+    a.d_k0sq = dk.as<double>();
+    assemble_pq_vjp(*c, a, (const cd*)Pb, (const cd*)Qb, stride, (cd*)eps_bar, off);
+  });
+}
+
+int vrc_eig_pq_vjp(vrc_ctx* c, int N, int batch, const void* P, const void* Q, long stride, const void* W,
+                   const void* V, const void* gam, void* W_bar, const void* V_bar, void* gam_bar, void* P_bar,
+                   void* Q_bar) {
+  return guarded(c, [&] {
+    need(N > 0 && batch >= 0 && P && Q && W && V && gam && W_bar && V_bar && gam_bar && P_bar && Q_bar,
+         "bad arguments");
+    need(stride == (long)N * N, "stride must be N*N");
+    if (batch == 0) return;
+    wave_eqn_vjp(*c, N, batch, (const cd*)P, (const cd*)Q, stride, (const cd*)W, (const cd*)V, (const cd*)gam,
+                 nullptr, (cd*)W_bar, (const cd*)V_bar, (cd*)gam_bar, (cd*)P_bar, (cd*)Q_bar);
+  });
+}
+
+int vrc_layer_smatrix_vjp(vrc_ctx* c, int N, int batch, const double* lam, const double* dz, const void* W,
+                          const void* V, const void* gam, const void* W0, const void* V0, long ref_stride,
+                          const void* Tuu_bar, const void* Rud_bar, void* W_bar, void* V_bar, void* gam_bar) {
+  return guarded(c, [&] {
+    need(N > 0 && batch >= 0 && lam && dz && W && V && gam && W0 && V0, "bad arguments");
+    need(Tuu_bar && Rud_bar && W_bar && V_bar && gam_bar, "bad arguments");
+    if (batch == 0) return;
+    std::vector<double> s(batch);
+    for (int b = 0; b < batch; ++b) {
+      need(lam[b] > 0, "wavelength must be positive");
+      s[b] = dz[b] * lam[b] / (2.0 * 3.1415926535897932);
+    }
+    DevBuf ds = c->ws.get(sizeof(double) * batch);
+    VRC_HIP(hipMemcpyAsync(ds.ptr, s.data(), sizeof(double) * batch, hipMemcpyHostToDevice, c->stream));
+    VRC_HIP(hipStreamSynchronize(c->stream));
+    layer_smatrix_vjp(*c, N, batch, ds.as<double>(), (const cd*)W, (const cd*)V, (const cd*)gam, (const cd*)W0,
+                      (const cd*)V0, ref_stride, (const cd*)Tuu_bar, (const cd*)Rud_bar, (cd*)W_bar, (cd*)V_bar,
+                      (cd*)gam_bar, nullptr);
+  });
+}
+
+int vrc_star_product_vjp(vrc_ctx* c, int N, int batch, const void* Tdd, const void* Tuu, const void* Rud,
+                         const void* Rdu, const void* tdd, const void* tuu, const void* rud, const void* rdu,
+                         long slice_stride, void* Tdd_bar, void* Tuu_bar, void* Rud_bar, void* Rdu_bar, void* tdd_bar,
+                         void* tuu_bar, void* rud_bar, void* rdu_bar) {
+  return guarded(c, [&] {
+    need(N > 0 && batch >= 0 && Tdd && Tuu && Rud && Rdu && tdd && tuu && rud && rdu, "bad arguments");
+    need(Tdd_bar && Tuu_bar && Rud_bar && Rdu_bar && tdd_bar && tuu_bar && rud_bar && rdu_bar, "bad arguments");
+    if (batch == 0) return;
+    SMat S0, Sb;
+    S0.Tdd = (cd*)Tdd;
+    S0.Tuu = (cd*)Tuu;
+    S0.Rud = (cd*)Rud;
+    S0.Rdu = (cd*)Rdu;
+    Sb.Tdd = (cd*)Tdd_bar;
+    Sb.Tuu = (cd*)Tuu_bar;
+    Sb.Rud = (cd*)Rud_bar;
+    Sb.Rdu = (cd*)Rdu_bar;
+    star_product_vjp(*c, N, batch, S0, (const cd*)tdd, (const cd*)tuu, (const cd*)rud, (const cd*)rdu, slice_stride,
+                     Sb, (cd*)tdd_bar, (cd*)tuu_bar, (cd*)rud_bar, (cd*)rdu_bar, (long)N * N);
+  });
+}
+
+int vrc_project_left_vjp(vrc_ctx* c, int N, int batch, const void* Tdd, const void* Tuu, const void* Rud,
+                         const void* Rdu, const void* W1, const void* V1, const void* W0, const void* V0,
+                         long ref_stride, void* Tdd_bar, void* Tuu_bar, void* Rud_bar, void* Rdu_bar) {
+  return guarded(c, [&] {
+    need(N > 0 && batch >= 0 && Tdd && Tuu && Rud && Rdu && W1 && V1 && W0 && V0, "bad arguments");
+    need(Tdd_bar && Tuu_bar && Rud_bar && Rdu_bar, "bad arguments");
+    if (batch == 0) return;
+    const long NN = (long)N * N;
+    SMat S, Sb;
+    S.Tdd = (cd*)Tdd;
+    S.Tuu = (cd*)Tuu;
+    S.Rud = (cd*)Rud;
+    S.Rdu = (cd*)Rdu;
+    Sb.Tdd = (cd*)Tdd_bar;
+    Sb.Tuu = (cd*)Tuu_bar;
+    Sb.Rud = (cd*)Rud_bar;
+    Sb.Rdu = (cd*)Rdu_bar;
+    DevBuf LW = c->ws.get(sizeof(cd) * NN * batch), LV = c->ws.get(sizeof(cd) * NN * batch);
+    DevBuf pW = c->ws.get(sizeof(int) * (size_t)N * batch), pV = c->ws.get(sizeof(int) * (size_t)N * batch);
+    DevBuf inf = c->ws.get(sizeof(int) * (size_t)batch);
+    fill_int(c->stream, inf.as<int>(), 0, batch);
+    zgeam(c->stream, N, N, mk(1, 0), (const cd*)W1, N, ref_stride, mk(0, 0), nullptr, 0, 0, LW.as<cd>(), N, NN,
+          batch);
+    zgeam(c->stream, N, N, mk(1, 0), (const cd*)V1, N, ref_stride, mk(0, 0), nullptr, 0, 0, LV.as<cd>(), N, NN,
+          batch);
+    zgetrf(c->stream, N, LW.as<cd>(), N, NN, pW.as<int>(), N, inf.as<int>(), batch);
+    zgetrf(c->stream, N, LV.as<cd>(), N, NN, pV.as<int>(), N, inf.as<int>(), batch);
+    LUPair lu;
+    lu.W = LW.as<cd>();
+    lu.V = LV.as<cd>();
+    lu.pW = pW.as<int>();
+    lu.pV = pV.as<int>();
+    lu.sM = NN;
+    project_left_vjp(*c, N, batch, S, lu, (const cd*)W0, (const cd*)V0, ref_stride, Sb);
   });
 }
 

@@ -903,6 +903,99 @@ __global__ __launch_bounds__(256) void normalize_cols_kernel(int N, cd* V, long 
   }
 }
 
+// ===================================================================== balancing
+// Diagonal similarity scaling by powers of two (the scaling half of LAPACK zgebal,
+// job 'S', reference LAPACK 3.5+ norms): for each index i, scale row i by 1/f and
+// column i by f when that reduces ||col i||_2 + ||row i||_2 below 0.95 of its value;
+// sweeps repeat until no index changes (at most 64 sweeps). scale[i] receives the
+// accumulated factor; eigenvectors of the original matrix are diag(scale) times those
+// of the balanced one. One workgroup per matrix.
+__global__ __launch_bounds__(256) void balance_kernel(int N, cd* A, long lda, long sA, cd* scale, long ss) {
+  __shared__ double red[4][4];
+  __shared__ int changed;
+  const long b = blockIdx.x;
+  A += b * sA;
+  scale += b * ss;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const double sfmin1 = SAFMIN / (ULP * 2.0), sfmax1 = 1.0 / sfmin1;
+  const double sfmin2 = sfmin1 * 2.0, sfmax2 = 1.0 / sfmin2;
+  for (int i = t; i < N; i += 256) scale[i] = mk(1.0, 0.0);
+  __syncthreads();
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    if (t == 0) changed = 0;
+    __syncthreads();
+    for (int i = 0; i < N; ++i) {
+      // column and row 2-norms (with the diagonal) and largest moduli
+      double c2 = 0.0, r2 = 0.0, ca = 0.0, ra = 0.0;
+      for (int k = t; k < N; k += 256) {
+        const cd cv = A[k + (long)i * lda], rv = A[i + (long)k * lda];
+        c2 += cnorm2(cv);
+        r2 += cnorm2(rv);
+        ca = fmax(ca, cnorm2(cv));
+        ra = fmax(ra, cnorm2(rv));
+      }
+      c2 = wsum(c2);
+      r2 = wsum(r2);
+      for (int o = 32; o > 0; o >>= 1) {
+        ca = fmax(ca, __shfl_xor(ca, o, 64));
+        ra = fmax(ra, __shfl_xor(ra, o, 64));
+      }
+      if (lane == 0) {
+        red[0][wave] = c2;
+        red[1][wave] = r2;
+        red[2][wave] = ca;
+        red[3][wave] = ra;
+      }
+      __syncthreads();
+      double c = sqrt(red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+      double r = sqrt(red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+      ca = sqrt(fmax(fmax(red[2][0], red[2][1]), fmax(red[2][2], red[2][3])));
+      ra = sqrt(fmax(fmax(red[3][0], red[3][1]), fmax(red[3][2], red[3][3])));
+      __syncthreads();
+      if (c == 0.0 || r == 0.0) continue;
+      double g = r / 2.0, f = 1.0;
+      const double s = c + r;
+      while (c < g && fmax(f, fmax(c, ca)) < sfmax2 && fmin(r, fmin(g, ra)) > sfmin2) {
+        f *= 2.0;
+        c *= 2.0;
+        ca *= 2.0;
+        r /= 2.0;
+        g /= 2.0;
+        ra /= 2.0;
+      }
+      g = c / 2.0;
+      while (g >= r && fmax(r, ra) < sfmax2 && fmin(fmin(f, c), fmin(g, ca)) > sfmin2) {
+        f /= 2.0;
+        c /= 2.0;
+        g /= 2.0;
+        ca /= 2.0;
+        r *= 2.0;
+        ra *= 2.0;
+      }
+      if (c + r >= 0.95 * s) continue;
+      const double sc = scale[i].x;
+      if (f < 1.0 && sc < 1.0 && f * sc <= sfmin1) continue;
+      if (f > 1.0 && sc > 1.0 && sc >= sfmax1 / f) continue;
+      const double gi = 1.0 / f;
+      for (int k = t; k < N; k += 256) {
+        A[i + (long)k * lda] = cscale(A[i + (long)k * lda], gi);
+      }
+      __syncthreads();
+      for (int k = t; k < N; k += 256) {
+        A[k + (long)i * lda] = cscale(A[k + (long)i * lda], f);
+      }
+      if (t == 0) {
+        scale[i] = mk(sc * f, 0.0);
+        changed = 1;
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+    if (!changed) break;
+    __syncthreads();
+  }
+}
+
 void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long sw, cd* VR, long ldv, long sV,
                    int* info, int batch, Workspace& ws) {
   if (n <= 0 || batch <= 0) return;
@@ -914,8 +1007,11 @@ void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
   DevBuf Zb = ws.get(sizeof(cd) * (size_t)nn * batch);
   cd* Vf = Vfb.as<cd>();
   cd* Z = Zb.as<cd>();
+  DevBuf Db = ws.get(sizeof(cd) * (size_t)n * batch);
   {
     StageScope st(ST_HESS);
+    balance_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Db.as<cd>(), n);
+    launch_check("balance");
     gehrd(s, n, A, lda, sA, Vf, Tb.as<cd>(), batch, ws);
     orghr(s, n, Vf, Tb.as<cd>(), Z, n, nn, batch, ws);
   }
@@ -943,6 +1039,7 @@ void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
   trevc_kernel<<<g, 256, 0, s>>>(n, A, lda, sA, X, n, nn);
   launch_check("trevc");
   zgemm_strided(s, OP_N, OP_N, n, n, n, mk(1.0, 0.0), Z, n, nn, X, n, nn, mk(0.0, 0.0), VR, ldv, sV, batch);
+  zdscale(s, n, n, Db.as<cd>(), n, nullptr, 0, 0, VR, ldv, sV, VR, ldv, sV, batch);  // undo the balancing
   dim3 gn(cdiv_up(n, 4), batch);
   normalize_cols_kernel<<<gn, 256, 0, s>>>(n, VR, ldv, sV);
   launch_check("normalize_cols");

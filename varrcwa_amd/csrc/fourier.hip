@@ -195,7 +195,7 @@ __global__ void fourier_expand_kernel(FourierDev g, cd* fe11, cd* fe22, cd* fe33
   fe22[o] = G22[(bb + d * ny) + (long)(a - cc + nx - 1) * ny * ny];
 }
 
-void fourier_assemble(hipStream_t s, const FourierDev& g, int nlayers, cd* fe11, cd* fe22, cd* fe33, long sfe) {
+void fourier_tables(hipStream_t s, const FourierDev& g, int nlayers) {
   const int M1 = 2 * g.nx - 1, M2 = 2 * g.ny - 1;
   {
     dim3 grid(64, nlayers);
@@ -209,6 +209,10 @@ void fourier_assemble(hipStream_t s, const FourierDev& g, int nlayers, cd* fe11,
   toeplitz_inv_kernel<<<nlayers * (int)g.MX, 256, 0, s>>>(g.ny, g.FYi, M2, g.tFFy, (long)g.ny * g.ny, g.ncx,
                                                            (int)g.MX);
   launch_check("toeplitz_inv_y");
+}
+
+void fourier_assemble(hipStream_t s, const FourierDev& g, int nlayers, cd* fe11, cd* fe22, cd* fe33, long sfe) {
+  fourier_tables(s, g, nlayers);
   {
     dim3 grid(64, nlayers);
     fourier_contract_kernel<<<grid, 256, 0, s>>>(g);

@@ -136,18 +136,20 @@ void fill_int(hipStream_t s, int* p, int v, long n) {
   launch_check("fill_int");
 }
 
-void assemble_fe(vrc_ctx& c, const AssemblyBatch& a, cd* fe11, cd* fe22, cd* fe33, long sfe) {
+FourierStage fourier_prepare(vrc_ctx& c, int nx, int ny, const std::vector<const HostLayer*>& layers) {
   hipStream_t s = c.stream;
-  const int L = a.nlayers;
+  const int L = (int)layers.size();
+  FourierStage fs;
+  fs.L = L;
   std::vector<PreparedLayer> pl;
   pl.reserve(L);
   int MX = 1, MY = 1;
   for (int l = 0; l < L; ++l) {
-    pl.push_back(prepare_layer(*a.layers[l]));
+    pl.push_back(prepare_layer(*layers[l]));
     MX = std::max(MX, pl.back().ncx);
     MY = std::max(MY, pl.back().ncy);
   }
-  const int nx = a.nx, ny = a.ny, M1 = 2 * nx - 1, M2 = 2 * ny - 1;
+  const int M1 = 2 * nx - 1, M2 = 2 * ny - 1;
   // host staging (one upload)
   std::vector<int> ncx(L), ncy(L);
   std::vector<double> Lx(L), Ly(L), xu((size_t)L * (MX + 1), 0.0), xs((size_t)L * (MX + 1), 0.0),
@@ -186,11 +188,11 @@ void assemble_fe(vrc_ctx& c, const AssemblyBatch& a, cd* fe11, cd* fe22, cd* fe3
   host.resize(oc + bytes_c);
   h = host.data();
   std::memcpy(h + oc, val.data(), bytes_c);
-  DevBuf geo = c.ws.get(host.size());
-  VRC_HIP(hipMemcpyAsync(geo.ptr, host.data(), host.size(), hipMemcpyHostToDevice, s));
+  fs.geo = c.ws.get(host.size());
+  VRC_HIP(hipMemcpyAsync(fs.geo.ptr, host.data(), host.size(), hipMemcpyHostToDevice, s));
   VRC_HIP(hipStreamSynchronize(s));  // the staging vector is released on return
-  char* d = (char*)geo.ptr;
-  FourierDev g;
+  char* d = (char*)fs.geo.ptr;
+  FourierDev& g = fs.g;
   g.nx = nx;
   g.ny = ny;
   g.MX = MX;
@@ -208,8 +210,8 @@ void assemble_fe(vrc_ctx& c, const AssemblyBatch& a, cd* fe11, cd* fe22, cd* fe3
                ntx = (size_t)MY * nx * nx, nty = (size_t)MX * ny * ny, nF2 = (size_t)M1 * M2,
                nG11 = (size_t)nx * nx * M2, nG22 = (size_t)ny * ny * M1;
   const size_t per = 2 * nFX + nFY1 + nFYi + nFX1 + ntx + nty + nF2 + nG11 + nG22;
-  DevBuf wk = c.ws.get(sizeof(cd) * per * L);
-  cd* w = wk.as<cd>();
+  fs.wk = c.ws.get(sizeof(cd) * per * L);
+  cd* w = fs.wk.as<cd>();
   g.FX = w;
   g.FXi = g.FX + nFX * L;
   g.FY1 = g.FXi + nFX * L;
@@ -220,8 +222,13 @@ void assemble_fe(vrc_ctx& c, const AssemblyBatch& a, cd* fe11, cd* fe22, cd* fe3
   g.F2 = g.tFFy + nty * L;
   g.G11 = g.F2 + nF2 * L;
   g.G22 = g.G11 + nG11 * L;
+  return fs;
+}
+
+void assemble_fe(vrc_ctx& c, const AssemblyBatch& a, cd* fe11, cd* fe22, cd* fe33, long sfe) {
+  FourierStage fs = fourier_prepare(c, a.nx, a.ny, a.layers);
   StageScope st(ST_FOURIER);
-  fourier_assemble(s, g, L, fe11, fe22, fe33, sfe);
+  fourier_assemble(c.stream, fs.g, fs.L, fe11, fe22, fe33, sfe);
 }
 
 }  // namespace vrc

@@ -5,7 +5,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "vrc_ops.hpp"
+#include "vrc_adjoint.hpp"
 
 namespace vrc {
 
@@ -488,7 +488,7 @@ void Integrator::differential(int Nsteps) {
 
 // ===================================================================== batched stacks
 void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int chunk, cd* S_out,
-                      cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o) {
+                      cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o, StackTape* tape) {
   hipStream_t st = c.stream;
   const int n = nx * ny, N = 2 * n;
   const long NN = (long)N * N;
@@ -560,6 +560,22 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
       wave_eqn(c, N, P.as<cd>(), Q.as<cd>(), NN, W, V, gam, inf, cnt);
     };
     solve_jobs(0, J, Wall.as<cd>(), Vall.as<cd>(), gall.as<cd>(), infall.as<int>());
+    if (tape) {
+      // eigen-systems in the tape's order: bottoms [inst], tops [inst], slabs [inst][L]
+      auto keep = [&](int rows, const cd* src, cd* dst, int cnt) {
+        zgeam(st, rows, 1, ONE, src, rows, rows, ZERO, nullptr, 0, 0, dst, rows, rows, cnt);
+      };
+      const int nn2 = (int)NN;
+      keep(nn2, Wall.as<cd>(), tape->Wb.as<cd>() + (long)i0 * NN, B);
+      keep(nn2, Vall.as<cd>(), tape->Vb.as<cd>() + (long)i0 * NN, B);
+      keep(N, gall.as<cd>(), tape->gb.as<cd>() + (long)i0 * N, B);
+      keep(nn2, Wall.as<cd>() + B * NN, tape->Wt.as<cd>() + (long)i0 * NN, B);
+      keep(nn2, Vall.as<cd>() + B * NN, tape->Vt.as<cd>() + (long)i0 * NN, B);
+      keep(N, gall.as<cd>() + (long)B * N, tape->gt.as<cd>() + (long)i0 * N, B);
+      keep(nn2, Wall.as<cd>() + 2 * B * NN, tape->Ws.as<cd>() + (long)i0 * L * NN, B * L);
+      keep(nn2, Vall.as<cd>() + 2 * B * NN, tape->Vs.as<cd>() + (long)i0 * L * NN, B * L);
+      keep(N, gall.as<cd>() + (long)2 * B * N, tape->gs.as<cd>() + (long)i0 * L * N, B * L);
+    }
     const cd* W0 = Wall.as<cd>();
     const cd* V0 = Vall.as<cd>();
     const cd* W1 = Wall.as<cd>() + B * NN;
@@ -607,6 +623,180 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     out(V1o, V1, NN);
     out(g1o, gall.as<cd>() + (long)B * N, N);
   }
+}
+
+}  // namespace vrc
+
+// ===================================================================== adjoint of batched stacks
+namespace vrc {
+
+long StackTape::cells() const {
+  long n = 0;
+  for (int i = 0; i < ninst; ++i)
+    for (int l = 0; l < L; ++l) n += (long)slab(i, l)->eps.size();
+  return n;
+}
+
+std::unique_ptr<StackTape> make_stack_tape(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst) {
+  auto t = std::make_unique<StackTape>();
+  t->c = &c;
+  t->nx = nx;
+  t->ny = ny;
+  t->ninst = (int)inst.size();
+  t->L = inst.empty() ? 0 : (int)inst[0].layers.size();
+  const int n = nx * ny, N = 2 * n, L = t->L, I = t->ninst;
+  const size_t NN = (size_t)N * N;
+  t->layers.reserve((size_t)I * (L + 2));
+  for (auto& s : inst) {
+    if ((int)s.layers.size() != L) throw ArgFailure("all instances of one batch must have the same layer count");
+    for (auto* l : s.layers) t->layers.push_back(*l);
+    t->lambda.push_back(s.lambda);
+    t->thickness.insert(t->thickness.end(), s.thickness.begin(), s.thickness.end());
+  }
+  for (auto& s : inst) t->layers.push_back(*s.bottom);
+  for (auto& s : inst) t->layers.push_back(*s.top);
+  t->Wb = c.ws.get(sizeof(cd) * NN * I);
+  t->Vb = c.ws.get(sizeof(cd) * NN * I);
+  t->gb = c.ws.get(sizeof(cd) * (size_t)N * I);
+  t->Wt = c.ws.get(sizeof(cd) * NN * I);
+  t->Vt = c.ws.get(sizeof(cd) * NN * I);
+  t->gt = c.ws.get(sizeof(cd) * (size_t)N * I);
+  t->Ws = c.ws.get(sizeof(cd) * NN * I * L);
+  t->Vs = c.ws.get(sizeof(cd) * NN * I * L);
+  t->gs = c.ws.get(sizeof(cd) * (size_t)N * I * L);
+  return t;
+}
+
+// Reverse pass of batch_stack_rcwa: per chunk of instances, recompute P/Q and the slab
+// S-matrices from the taped eigen-systems, rerun the cascade keeping every prefix,
+// then propagate the cotangents back through the projection, the star products (last
+// slab first), the slab S-matrices, the eigen-decompositions and the Fourier assembly.
+void stack_tape_vjp(StackTape& t, const cd* Sb_in, cd* epsb, int chunk) {
+  vrc_ctx& c = *t.c;
+  hipStream_t st = c.stream;
+  const int nx = t.nx, ny = t.ny, n = nx * ny, N = 2 * n, L = t.L, I = t.ninst;
+  const long NN = (long)N * N;
+  if (I == 0 || L == 0) return;
+  std::vector<long> off((size_t)I * L + 1, 0);
+  for (int i = 0; i < I; ++i)
+    for (int l = 0; l < L; ++l) off[(size_t)i * L + l + 1] = off[(size_t)i * L + l] + (long)t.slab(i, l)->eps.size();
+  if (chunk <= 0) chunk = std::max(1, std::min(I, 64 / L));
+  for (int i0 = 0; i0 < I; i0 += chunk) {
+    const int B = std::min(chunk, I - i0), SL = B * L;
+    // ---- P, Q of the slabs (per-instance alpha, beta, k0^2)
+    std::vector<double> k2(SL), sv(SL);
+    std::vector<cd> ab((size_t)SL * 2 * n);
+    std::vector<int> owner(SL);
+    AssemblyBatch a;
+    a.nx = nx;
+    a.ny = ny;
+    a.nlayers = SL;
+    a.periodic = true;
+    for (int b = 0; b < B; ++b) {
+      const int i = i0 + b;
+      const double k0 = 2 * PI_REF / t.lambda[i];
+      const HostLayer* bot = t.bottom(i);
+      std::vector<cd> al, be;
+      alpha_beta(nx, ny, bot->x.back() - bot->x.front(), bot->y.back() - bot->y.front(), al, be);
+      for (int l = 0; l < L; ++l) {
+        const int j = b * L + l;
+        a.layers.push_back(t.slab(i, l));
+        k2[j] = k0 * k0;
+        sv[j] = t.thickness[(size_t)i * L + l] / k0;
+        owner[j] = b;
+        std::copy(al.begin(), al.end(), ab.begin() + (size_t)j * 2 * n);
+        std::copy(be.begin(), be.end(), ab.begin() + (size_t)j * 2 * n + n);
+      }
+    }
+    DevBuf dk2 = upload(c, k2.data(), sizeof(double) * SL);
+    DevBuf dab = upload(c, ab.data(), sizeof(cd) * ab.size());
+    DevBuf dsv = upload(c, sv.data(), sizeof(double) * SL);
+    DevBuf down = upload(c, owner.data(), sizeof(int) * SL);
+    a.d_alpha = dab.as<cd>();
+    a.d_beta = a.d_alpha + n;
+    a.sab = 2 * n;
+    a.d_k0sq = dk2.as<double>();
+    DevBuf P = c.ws.get(sizeof(cd) * NN * SL), Q = c.ws.get(sizeof(cd) * NN * SL);
+    assemble_pq(c, a, P.as<cd>(), Q.as<cd>());
+    const cd* Ws = t.Ws.as<cd>() + (long)i0 * L * NN;
+    const cd* Vs = t.Vs.as<cd>() + (long)i0 * L * NN;
+    const cd* gs = t.gs.as<cd>() + (long)i0 * L * N;
+    const cd* W0 = t.Wb.as<cd>() + (long)i0 * NN;
+    const cd* V0 = t.Vb.as<cd>() + (long)i0 * NN;
+    const cd* W1 = t.Wt.as<cd>() + (long)i0 * NN;
+    const cd* V1 = t.Vt.as<cd>() + (long)i0 * NN;
+    LUStore lu1;
+    lu1.make(c, N, W1, V1, B, NN);
+    DevBuf Wr = c.ws.get(sizeof(cd) * NN * SL), Vr = c.ws.get(sizeof(cd) * NN * SL);
+    gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
+    gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
+    // ---- slab S-matrices and the cascade with every prefix kept
+    DevBuf tu = c.ws.get(sizeof(cd) * NN * SL), ru = c.ws.get(sizeof(cd) * NN * SL);
+    layer_smatrix(c, N, SL, dsv.as<double>(), Ws, Vs, gs, Wr.as<cd>(), Vr.as<cd>(), NN, tu.as<cd>(), ru.as<cd>());
+    DevBuf pre = c.ws.get(sizeof(cd) * 4 * NN * B * L);
+    DevBuf Sbuf = c.ws.get(sizeof(cd) * 4 * NN * B);
+    auto blocks = [&](cd* base) {
+      SMat m;
+      m.Tdd = base;
+      m.Tuu = base + B * NN;
+      m.Rud = base + 2 * B * NN;
+      m.Rdu = base + 3 * B * NN;
+      return m;
+    };
+    SMat S = blocks(Sbuf.as<cd>());
+    init_identity_S(c, N, S, B);
+    const long ss = (long)L * NN;
+    for (int l = 0; l < L; ++l) {
+      VRC_HIP(hipMemcpyAsync(pre.as<cd>() + (long)l * 4 * B * NN, Sbuf.ptr, sizeof(cd) * 4 * NN * B,
+                             hipMemcpyDeviceToDevice, st));
+      const cd* tl = tu.as<cd>() + l * NN;
+      const cd* rl = ru.as<cd>() + l * NN;
+      star_product(c, N, B, S, tl, tl, rl, rl, ss);
+    }
+    // ---- cotangents: projection, then the cascade in reverse
+    DevBuf Sbb = c.ws.get(sizeof(cd) * 4 * NN * B);
+    SMat Sb = blocks(Sbb.as<cd>());
+    {
+      const cd* src = Sb_in + (long)i0 * 4 * NN;
+      zgeam(st, N, N, ONE, src, N, 4 * NN, ZERO, nullptr, 0, 0, Sb.Tdd, N, NN, B);
+      zgeam(st, N, N, ONE, src + NN, N, 4 * NN, ZERO, nullptr, 0, 0, Sb.Tuu, N, NN, B);
+      zgeam(st, N, N, ONE, src + 2 * NN, N, 4 * NN, ZERO, nullptr, 0, 0, Sb.Rud, N, NN, B);
+      zgeam(st, N, N, ONE, src + 3 * NN, N, 4 * NN, ZERO, nullptr, 0, 0, Sb.Rdu, N, NN, B);
+    }
+    project_left_vjp(c, N, B, S, lu1.pair, W0, V0, NN, Sb);
+    DevBuf tb = c.ws.get(sizeof(cd) * NN * SL), rb = c.ws.get(sizeof(cd) * NN * SL);
+    {
+      DevBuf q = c.ws.get(sizeof(cd) * 4 * NN * B);
+      cd* q0 = q.as<cd>();
+      for (int l = L - 1; l >= 0; --l) {
+        SMat S0 = blocks(pre.as<cd>() + (long)l * 4 * B * NN);
+        const cd* tl = tu.as<cd>() + l * NN;
+        const cd* rl = ru.as<cd>() + l * NN;
+        star_product_vjp(c, N, B, S0, tl, tl, rl, rl, ss, Sb, q0, q0 + B * NN, q0 + 2 * B * NN, q0 + 3 * B * NN, NN);
+        // the slab's Tdd and Tuu (Rud and Rdu) are one matrix: their cotangents add
+        zgeam(st, N, N, ONE, q0, N, NN, ONE, q0 + B * NN, N, NN, tb.as<cd>() + l * NN, N, ss, B);
+        zgeam(st, N, N, ONE, q0 + 2 * B * NN, N, NN, ONE, q0 + 3 * B * NN, N, NN, rb.as<cd>() + l * NN, N, ss, B);
+      }
+    }
+    pre = DevBuf();
+    // ---- slab S -> (W, V, gamma) -> (P, Q) -> eps
+    DevBuf Wbar = c.ws.get(sizeof(cd) * NN * SL), Vbar = c.ws.get(sizeof(cd) * NN * SL),
+           gbar = c.ws.get(sizeof(cd) * (size_t)N * SL), Wi = c.ws.get(sizeof(cd) * NN * SL);
+    layer_smatrix_vjp(c, N, SL, dsv.as<double>(), Ws, Vs, gs, Wr.as<cd>(), Vr.as<cd>(), NN, tb.as<cd>(), rb.as<cd>(),
+                      Wbar.as<cd>(), Vbar.as<cd>(), gbar.as<cd>(), Wi.as<cd>());
+    tu = DevBuf();
+    ru = DevBuf();
+    DevBuf& Pbar = tb;  // slab cotangents consumed
+    DevBuf& Qbar = rb;
+    {
+      StageScope sc(ST_ADJOINT);
+      wave_eqn_vjp(c, N, SL, P.as<cd>(), Q.as<cd>(), NN, Ws, Vs, gs, Wi.as<cd>(), Wbar.as<cd>(), Vbar.as<cd>(),
+                   gbar.as<cd>(), Pbar.as<cd>(), Qbar.as<cd>());
+      std::vector<long> o(off.begin() + (size_t)i0 * L, off.begin() + (size_t)i0 * L + SL);
+      assemble_pq_vjp(c, a, Pbar.as<cd>(), Qbar.as<cd>(), NN, epsb, o);
+    }
+  }
+  VRC_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace vrc

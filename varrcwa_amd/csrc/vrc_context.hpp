@@ -59,7 +59,8 @@ enum Stage : int {
   ST_LAYER_S = 6,    // (d) slab S-matrices
   ST_STAR = 7,       // (d) star products
   ST_PROJECT = 8,    // (d) end projections
-  ST_COUNT = 9
+  ST_ADJOINT = 9,    // reverse pass: slab, star-product and projection adjoints
+  ST_COUNT = 10
 };
 
 vrc_ctx*& tl_ctx();

@@ -33,6 +33,9 @@ struct FourierDev {
   cd *FX, *FXi, *FY1, *FYi, *FX1, *tFFx, *tFFy, *F2, *G11, *G22;
 };
 
+// Coefficient tables (FX, FXi, FY1, FYi, FX1) and inverse Toeplitz blocks (tFFx, tFFy).
+void fourier_tables(hipStream_t s, const FourierDev& g, int nlayers);
+// fourier_tables + contraction + expansion into fe11, fe22, fe33 (n x n, stride sfe).
 void fourier_assemble(hipStream_t s, const FourierDev& g, int nlayers, cd* fe11, cd* fe22, cd* fe33, long sfe);
 
 }  // namespace vrc

@@ -47,6 +47,14 @@ struct AssemblyBatch {
   double k0sq_uniform = 0.0;
 };
 void assemble_fe(vrc_ctx& c, const AssemblyBatch& a, cd* fe11, cd* fe22, cd* fe33, long sfe);
+
+// Uploaded layer batch and workspace of the Fourier kernels (device view in g).
+struct FourierStage {
+  DevBuf geo, wk;
+  FourierDev g{};
+  int L = 0;
+};
+FourierStage fourier_prepare(vrc_ctx& c, int nx, int ny, const std::vector<const HostLayer*>& layers);
 void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q);
 
 // ---------------------------------------------------------------- (c)
@@ -171,8 +179,10 @@ struct StackInstance {
   const HostLayer* bottom;
   const HostLayer* top;
 };
+struct StackTape;
 // Out: S[inst] = [Tdd|Tuu|Rud|Rdu] (4 N x N); optional W0/V0/gam0/W1/V1/gam1 per instance.
+// With a tape, the eigen-systems of every instance are kept for the reverse pass.
 void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int chunk, cd* S_out,
-                      cd* W0, cd* V0, cd* g0, cd* W1, cd* V1, cd* g1);
+                      cd* W0, cd* V0, cd* g0, cd* W1, cd* V1, cd* g1, StackTape* tape = nullptr);
 
 }  // namespace vrc
