@@ -150,6 +150,7 @@ def main():
     ap.add_argument("--cpu", type=int, default=1, help="time the CPU oracle on one instance (0 = skip)")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--adjoint", type=int, default=21, help="instances in the adjoint timing (0 = skip)")
+    ap.add_argument("--tune", action="append", default=[], help="kernel switch key=value (vrc_tune_set)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -164,6 +165,10 @@ def main():
         print(*a, file=sys.stderr, flush=True)
 
     ctx = Context(0)
+    for kv in args.tune:
+        k, v = kv.split("=")
+        if ctx.lib.vrc_tune_set(k.encode(), int(v)) != 0:
+            raise SystemExit(f"unknown tuning key {k}")
     nh = args.nh
     n = nh * nh
     N = 2 * n
@@ -241,7 +246,7 @@ def main():
         "config": {"workload": f"rectangle stacks, {nh}x{nh} harmonics (N={N}), {L} slabs/instance, "
                                f"{len(insts)} instances/step (8 seeded geometries x 64 wavelengths in 0.8-1.6 um)",
                    "instances_per_step": len(insts), "slices_per_step": len(insts) * L, "N": N,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "tuning": args.tune},
         "fp64_matrix_frac": frac,
         "path_tflops": flops_slice(N) * value / 1e12,
         "plain_slices_per_s": plain,

@@ -29,6 +29,8 @@ constexpr int QW = 64;         // maximal QR window
 constexpr int NBUL = 10;       // bulges per sweep (2 shifts each)
 constexpr int QR_NMIN = 48;    // active blocks this small are finished in LDS
 constexpr int EXC_SWEEPS = 6;  // sweeps without deflation before exceptional shifts
+constexpr int AED_NW = 24;     // default deflation window (measured best of 16..64 at N = 882)
+constexpr int AED_NIBBLE = 14; // skip the sweep when the window deflated more than this percentage
 
 __device__ inline double wsum(double v) {
 #pragma unroll
@@ -376,6 +378,7 @@ struct QrLds {
   int rsize[NBUL], rrow[NBUL], rcol[NBUL];
   double redv[256];
   int redi[256];
+  cd aedv[QW];
   int ctl[4];
 };
 
@@ -417,11 +420,26 @@ __device__ inline cd wilkinson(cd a, cd b, cd c, cd d) {
   return csub(d, cmul(u, cdiv(u, cadd(x, y))));
 }
 
-// Complete Schur decomposition of the s x s Hessenberg block in L.Hw (s <= QW) by
-// single-shift QR; accumulates the transform into L.U when wantU. Returns 0 on
-// success, else (index of the unconverged eigenvalue + 1). All threads call it.
-__device__ int small_schur(QrLds& L, int s, bool wantU) {
-  const int t = threadIdx.x;
+// Orders the LDS accesses of one wavefront: every lane's earlier LDS writes are seen by
+// every lane's later reads (LDS executes a wavefront's instructions in order).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Complete Schur decomposition of the s x s Hessenberg block in Hw (s <= QW) by
+// single-shift QR, run by ONE wavefront (lane = column / row index), accumulating the
+// transform into U when wantU (U must hold the identity or an earlier transform).
+// Subdiagonal entries are set to exact zeros on deflation, so Hw ends upper
+// triangular. Returns 0 on success, else (index of the unconverged eigenvalue + 1).
+__device__ int wave_schur(LdsMat& Hw, LdsMat& U, int s, bool wantU, int lane) {
   const double smlnum = SAFMIN * ((double)s / ULP);
   const int itmax = 30 * (s > 10 ? s : 10);
   int i = s - 1;
@@ -429,77 +447,250 @@ __device__ int small_schur(QrLds& L, int s, bool wantU) {
     int l = 0;
     bool conv = false;
     for (int its = 0; its <= itmax; ++its) {
-      if (t == 0) {
-        int k;
-        for (k = i; k > 0; --k)
-          if (negl_lds(L.Hw, k, 0, i, smlnum)) break;
-        if (k > 0) L.Hw[k][k - 1] = mk(0.0, 0.0);
-        L.ctl[0] = k;
-      }
-      __syncthreads();
-      l = L.ctl[0];
-      __syncthreads();
+      const bool ng = (lane >= 1 && lane <= i) ? negl_lds(Hw, lane, 0, i, smlnum) : false;
+      const unsigned long long mask = __ballot(ng);
+      const int k = mask ? 63 - __clzll(mask) : 0;
+      wave_sync();
+      if (k > 0 && lane == 0) Hw[k][k - 1] = mk(0.0, 0.0);
+      wave_sync();
+      l = k;
       if (l >= i) {
         conv = true;
         break;
       }
       cd sh;
       if (its == 10) {
-        sh = cadd(L.Hw[l][l], mk(0.75 * cabs1(L.Hw[l + 1][l]), 0.0));
+        sh = cadd(Hw[l][l], mk(0.75 * cabs1(Hw[l + 1][l]), 0.0));
       } else if (its == 20) {
-        sh = cadd(L.Hw[i][i], mk(0.75 * cabs1(L.Hw[i][i - 1]), 0.0));
+        sh = cadd(Hw[i][i], mk(0.75 * cabs1(Hw[i][i - 1]), 0.0));
       } else {
-        sh = wilkinson(L.Hw[i - 1][i - 1], L.Hw[i - 1][i], L.Hw[i][i - 1], L.Hw[i][i]);
+        sh = wilkinson(Hw[i - 1][i - 1], Hw[i - 1][i], Hw[i][i - 1], Hw[i][i]);
       }
-      for (int k = l; k < i; ++k) {
+      for (int k2 = l; k2 < i; ++k2) {
         cd x, y;
-        if (k == l) {
-          x = csub(L.Hw[l][l], sh);
-          y = L.Hw[l + 1][l];
+        if (k2 == l) {
+          x = csub(Hw[l][l], sh);
+          y = Hw[l + 1][l];
         } else {
-          x = L.Hw[k][k - 1];
-          y = L.Hw[k + 1][k - 1];
+          x = Hw[k2][k2 - 1];
+          y = Hw[k2 + 1][k2 - 1];
         }
         double beta;
         cd tau, scale;
         house(x, cabsv(y), beta, tau, scale);
         const cd v1 = cmul(y, scale);
-        const bool ident = (tau.x == 0.0 && tau.y == 0.0);
-        __syncthreads();
-        if (!ident) {
-          if (t < s) {
-            const int j = t;
-            if (k > l && j == k - 1) {
-              L.Hw[k][j] = mk(beta, 0.0);
-              L.Hw[k + 1][j] = mk(0.0, 0.0);
-            } else {
-              cd h1 = L.Hw[k][j], h2 = L.Hw[k + 1][j];
-              cd sm = cmul(cconj(tau), cadd(h1, cmul(cconj(v1), h2)));
-              L.Hw[k][j] = csub(h1, sm);
-              L.Hw[k + 1][j] = csub(h2, cmul(sm, v1));
-            }
-          }
-          __syncthreads();
-          if (t < s) {
-            cd c1 = L.Hw[t][k], c2 = L.Hw[t][k + 1];
-            cd sm = cmul(tau, cadd(c1, cmul(c2, v1)));
-            L.Hw[t][k] = csub(c1, sm);
-            L.Hw[t][k + 1] = csub(c2, cmul(sm, cconj(v1)));
-          } else if (wantU && t >= 64 && t < 64 + s) {
-            const int r = t - 64;
-            cd c1 = L.U[r][k], c2 = L.U[r][k + 1];
-            cd sm = cmul(tau, cadd(c1, cmul(c2, v1)));
-            L.U[r][k] = csub(c1, sm);
-            L.U[r][k + 1] = csub(c2, cmul(sm, cconj(v1)));
+        wave_sync();
+        if (tau.x == 0.0 && tau.y == 0.0) continue;
+        if (lane < s) {
+          const int j = lane;
+          if (k2 > l && j == k2 - 1) {
+            Hw[k2][j] = mk(beta, 0.0);
+            Hw[k2 + 1][j] = mk(0.0, 0.0);
+          } else {
+            const cd h1 = Hw[k2][j], h2 = Hw[k2 + 1][j];
+            const cd sm = cmul(cconj(tau), cadd(h1, cmul(cconj(v1), h2)));
+            Hw[k2][j] = csub(h1, sm);
+            Hw[k2 + 1][j] = csub(h2, cmul(sm, v1));
           }
         }
-        __syncthreads();
+        wave_sync();
+        if (lane < s) {
+          const cd c1 = Hw[lane][k2], c2 = Hw[lane][k2 + 1];
+          const cd sm = cmul(tau, cadd(c1, cmul(c2, v1)));
+          Hw[lane][k2] = csub(c1, sm);
+          Hw[lane][k2 + 1] = csub(c2, cmul(sm, cconj(v1)));
+          if (wantU) {
+            const cd u1 = U[lane][k2], u2 = U[lane][k2 + 1];
+            const cd su = cmul(tau, cadd(u1, cmul(u2, v1)));
+            U[lane][k2] = csub(u1, su);
+            U[lane][k2 + 1] = csub(u2, cmul(su, cconj(v1)));
+          }
+        }
+        wave_sync();
       }
     }
     if (!conv) return i + 1;
     i = l - 1;
   }
   return 0;
+}
+
+// Complex Givens rotation (LAPACK zlartg semantics): [cs sn; -conj(sn) cs] [f; g] = [r; 0].
+__device__ inline void givens(cd f, cd g, double& cs, cd& sn) {
+  const double f1 = cabsv(f), g1 = cabsv(g);
+  if (g1 == 0.0) {
+    cs = 1.0;
+    sn = mk(0.0, 0.0);
+  } else if (f1 == 0.0) {
+    cs = 0.0;
+    sn = cscale(cconj(g), 1.0 / g1);
+  } else {
+    const double d = sqrt(f1 * f1 + g1 * g1);
+    cs = f1 / d;
+    sn = cscale(cmul(cscale(f, 1.0 / f1), cconj(g)), 1.0 / d);
+  }
+}
+
+// Swap the adjacent diagonal entries k, k+1 of the n x n upper-triangular T by a Givens
+// rotation (LAPACK ztrexc step), updating the Schur vectors U (n rows). One wavefront.
+__device__ void wave_swap(LdsMat& T, LdsMat& U, int n, int k, int lane) {
+  const cd t11 = T[k][k], t22 = T[k + 1][k + 1];
+  double cs;
+  cd sn;
+  givens(T[k][k + 1], csub(t22, t11), cs, sn);
+  wave_sync();
+  if (lane >= k + 2 && lane < n) {  // rows k, k+1 from column k+2 on
+    const cd x = T[k][lane], y = T[k + 1][lane];
+    T[k][lane] = cadd(cscale(x, cs), cmul(sn, y));
+    T[k + 1][lane] = csub(cscale(y, cs), cmul(cconj(sn), x));
+  }
+  const cd snc = cconj(sn);
+  if (lane < k) {  // columns k, k+1 above row k
+    const cd x = T[lane][k], y = T[lane][k + 1];
+    T[lane][k] = cadd(cscale(x, cs), cmul(snc, y));
+    T[lane][k + 1] = csub(cscale(y, cs), cmul(sn, x));
+  }
+  if (lane < n) {
+    const cd x = U[lane][k], y = U[lane][k + 1];
+    U[lane][k] = cadd(cscale(x, cs), cmul(snc, y));
+    U[lane][k + 1] = csub(cscale(y, cs), cmul(sn, x));
+  }
+  if (lane == 0) {
+    T[k][k] = t22;
+    T[k + 1][k + 1] = t11;
+  }
+  wave_sync();
+}
+
+// T <- G^H T on rows r0..r0+len-1 (columns c0..n-1), then T <- T G on those columns
+// (rows 0..rows_t-1) and U <- U G (rows 0..n-1), G = I - tau v v^H with v in vv[0..len).
+__device__ void wave_reflect(LdsMat& T, LdsMat& U, const cd* vv, cd tau, int r0, int len, int c0, int n, int rows_t,
+                             int lane) {
+  const cd ct = cconj(tau);
+  if (lane >= c0 && lane < n) {
+    cd d = mk(0.0, 0.0);
+    for (int q = 0; q < len; ++q) d = cadd(d, cmul(cconj(vv[q]), T[r0 + q][lane]));
+    d = cmul(ct, d);
+    for (int q = 0; q < len; ++q) T[r0 + q][lane] = csub(T[r0 + q][lane], cmul(vv[q], d));
+  }
+  wave_sync();
+  if (lane < n) {
+    if (lane < rows_t) {
+      cd d = mk(0.0, 0.0);
+      for (int q = 0; q < len; ++q) d = cadd(d, cmul(T[lane][r0 + q], vv[q]));
+      d = cmul(tau, d);
+      for (int q = 0; q < len; ++q) T[lane][r0 + q] = csub(T[lane][r0 + q], cmul(d, cconj(vv[q])));
+    }
+    cd e = mk(0.0, 0.0);
+    for (int q = 0; q < len; ++q) e = cadd(e, cmul(U[lane][r0 + q], vv[q]));
+    e = cmul(tau, e);
+    for (int q = 0; q < len; ++q) U[lane][r0 + q] = csub(U[lane][r0 + q], cmul(e, cconj(vv[q])));
+  }
+  wave_sync();
+}
+
+__device__ void lds_load_block(QrLds& L, const cd* H, long ldh, int r0, int w);
+__device__ void lds_store_block(QrLds& L, cd* H, long ldh, int r0, int w);
+__device__ void apply_window(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int ws, int w);
+
+// Aggressive early deflation (the deflation-window step of LAPACK zlaqr3) on the
+// trailing nw x nw window of the active block [lo, hi]: Schur form of the window,
+// spike deflation test with reordering of the undeflatable eigenvalues to the top,
+// re-reduction of the undeflated part to Hessenberg form, and the window transform
+// applied to the rest of H and to Z. Returns the number of deflated eigenvalues (-1 if
+// the window's QR did not converge; nothing is changed then). The undeflated
+// eigenvalues are left in L.aedv[0..*nsa) for use as shifts.
+__device__ int aed(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int lo, int hi, int nw, int* nsa) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int kwtop = hi - nw + 1;
+  const cd spike = (kwtop > lo) ? H[kwtop + (long)(kwtop - 1) * ldh] : mk(0.0, 0.0);
+  lds_load_block(L, H, ldh, kwtop, nw);
+  if (wave == 0) {
+    const int info = wave_schur(L.Hw, L.U, nw, true, lane);
+    if (lane == 0) L.ctl[1] = info;
+    if (info == 0) {
+      LdsMat& T = L.Hw;
+      LdsMat& U = L.U;
+      const double smlnum = SAFMIN * ((double)N / ULP);
+      const double as = cabs1(spike);
+      int ns = nw, ilst = 0;
+      for (int knt = 0; knt < nw; ++knt) {
+        const int j = ns - 1;
+        double foo = cabs1(T[j][j]);
+        if (foo == 0.0) foo = as;
+        const bool defl = as * cabs1(U[0][j]) <= fmax(smlnum, ULP * foo);
+        wave_sync();
+        if (defl) {
+          --ns;
+        } else {
+          for (int k = j - 1; k >= ilst; --k) wave_swap(T, U, nw, k, lane);
+          ++ilst;
+        }
+      }
+      if (lane < ns) L.aedv[lane] = T[lane][lane];
+      wave_sync();
+      cd newspike = mk(0.0, 0.0);
+      if (ns > 0 && (spike.x != 0.0 || spike.y != 0.0)) {
+        // shifts are kept in ctl-free storage: copy the undeflated eigenvalues first
+        cd ev = lane < ns ? L.aedv[lane] : mk(0.0, 0.0);
+        if (ns > 1) {
+          // reflect the spike s * conj(U[0][0..ns)) onto e1 (zlaqr3 "reflect spike back")
+          cd w = lane < ns ? cconj(U[0][lane]) : mk(0.0, 0.0);
+          const double xn = sqrt(wsum(lane >= 1 && lane < ns ? cnorm2(w) : 0.0));
+          const cd alpha = cconj(U[0][0]);
+          double beta;
+          cd tau, scale;
+          house(alpha, xn, beta, tau, scale);
+          wave_sync();
+          L.aedv[lane] = lane == 0 ? mk(1.0, 0.0) : (lane < ns ? cmul(w, scale) : mk(0.0, 0.0));
+          wave_sync();
+          wave_reflect(T, U, L.aedv, tau, 0, ns, 0, nw, ns, lane);
+          // Hessenberg reduction of T[0:ns, 0:ns] (columns 0..ns-3)
+          for (int c = 0; c + 2 < ns; ++c) {
+            const int len = ns - c - 1;
+            const cd a0 = T[c + 1][c];
+            const cd xv = (lane >= c + 2 && lane < ns) ? T[lane][c] : mk(0.0, 0.0);
+            const double xnc = sqrt(wsum(cnorm2(xv)));
+            double bc;
+            cd tc, sc;
+            house(a0, xnc, bc, tc, sc);
+            wave_sync();
+            if (lane < len) L.aedv[lane] = lane == 0 ? mk(1.0, 0.0) : cmul(T[c + 1 + lane][c], sc);
+            wave_sync();
+            if (lane == c) {
+              T[c + 1][c] = mk(bc, 0.0);
+              for (int r = c + 2; r < ns; ++r) T[r][c] = mk(0.0, 0.0);
+            }
+            wave_sync();
+            if (tc.x != 0.0 || tc.y != 0.0) wave_reflect(T, U, L.aedv, tc, c + 1, len, c + 1, nw, ns, lane);
+          }
+        }
+        newspike = cmul(spike, cconj(U[0][0]));
+        wave_sync();
+        if (lane < ns) L.aedv[lane] = ev;  // shifts for the next sweep
+        wave_sync();
+      }
+      if (lane == 0) {
+        L.ctl[2] = ns;
+        L.shifts[0] = newspike;  // staging; read back before the next use of shifts
+      }
+    }
+  }
+  __syncthreads();
+  if (L.ctl[1] != 0) {
+    __syncthreads();
+    *nsa = 0;
+    return -1;
+  }
+  const int ns = L.ctl[2];
+  const cd newspike = L.shifts[0];
+  __syncthreads();
+  lds_store_block(L, H, ldh, kwtop, nw);
+  if (t == 0 && kwtop > lo) H[kwtop + (long)(kwtop - 1) * ldh] = newspike;
+  __syncthreads();
+  apply_window(L, N, H, ldh, Z, ldz, kwtop, nw);
+  *nsa = ns;
+  return nw - ns;
 }
 
 __device__ void lds_load_block(QrLds& L, const cd* H, long ldh, int r0, int w) {
@@ -689,8 +880,9 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
 
 // stats (nullable): per matrix {sweeps, windows, window-chase ticks, off-window-apply ticks,
 // small-block ticks} of the constant-rate device clock.
+// use_aed: deflation-window size (0 = no aggressive early deflation).
 __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long sH, cd* Z, long ldz, long sZ,
-                                                  cd* wout, long sw, int* info, long long* stats) {
+                                                  cd* wout, long sw, int* info, long long* stats, int use_aed) {
   long long st[5] = {0, 0, 0, 0, 0};
   __shared__ QrLds L;
   const long b = blockIdx.x;
@@ -711,7 +903,12 @@ __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long s
     if (m <= QR_NMIN) {
       const long long c0 = wall_clock64();
       lds_load_block(L, H, ldh, lo, m);
-      const int r = small_schur(L, m, true);
+      if ((t >> 6) == 0) {
+        const int r = wave_schur(L.Hw, L.U, m, true, t & 63);
+        if (t == 0) L.ctl[1] = r;
+      }
+      __syncthreads();
+      const int r = L.ctl[1];
       lds_store_block(L, H, ldh, lo, m);
       if (r != 0) {
         fail = lo + r;
@@ -729,28 +926,57 @@ __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long s
     int nbul = (m - 2) / 6;
     if (nbul > NBUL) nbul = NBUL;
     if (nbul < 1) nbul = 1;
-    const int ns = 2 * nbul;
+    // aggressive early deflation on the trailing window
+    int nsa = 0;
+    if (use_aed) {
+      const long long c0 = wall_clock64();
+      const int nw = m - 1 < use_aed ? m - 1 : use_aed;
+      const int nd = aed(L, N, H, ldh, Z, ldz, lo, ihi, nw, &nsa);
+      st[4] += wall_clock64() - c0;
+      if (nd > 0) {
+        ihi -= nd;
+        last_progress = sweeps;
+        prev_hi = ihi;
+        // a productive window: look again before sweeping (LAPACK's nibble rule)
+        if (100 * nd > AED_NIBBLE * nw || ihi - lo + 1 <= QR_NMIN) {
+          ++sweeps;
+          continue;
+        }
+      }
+      if (nd < 0) nsa = 0;
+      const int mm = ihi - lo + 1;
+      nbul = (mm - 2) / 6;
+      if (nbul > NBUL) nbul = NBUL;
+      if (nbul < 1) nbul = 1;
+    }
+    int ns = 2 * nbul;
     const bool exceptional = (sweeps - last_progress) > 0 && ((sweeps - last_progress) % EXC_SWEEPS) == 0;
-    if (!exceptional) {
-      const int r0 = ihi - ns + 1;
-      for (int e = t; e < QW * QW; e += 256) {
-        int i = e & 63, j = e >> 6;
-        L.Hw[i][j] = (i < ns && j < ns) ? H[(r0 + i) + (long)(r0 + j) * ldh] : mk(0.0, 0.0);
-      }
-      __syncthreads();
-      if (t == 0) {
-        // the block is a trailing principal submatrix: clear its (irrelevant) left neighbour
-      }
-      small_schur(L, ns, false);
-      if (t < ns) L.shifts[t] = L.Hw[t][t];
-      __syncthreads();
-    } else {
+    if (exceptional) {
       if (t < ns) {
         const int k = ihi - t;
         cd hk = H[k + (long)k * ldh];
         double sub = cabs1(H[k + (long)(k - 1) * ldh]);
         L.shifts[t] = cadd(hk, mk(0.75 * sub, 0.0));
       }
+      __syncthreads();
+    } else if (nsa >= nbul) {
+      // the undeflated window eigenvalues nearest the bottom serve as shifts
+      if (nsa < ns) {
+        nbul = nsa / 2 > 0 ? nsa / 2 : 1;
+        ns = 2 * nbul;
+      }
+      if (t < ns) L.shifts[t] = L.aedv[nsa - ns + t];
+      __syncthreads();
+    } else {
+      const int r0 = ihi - ns + 1;
+      for (int e = t; e < QW * QW; e += 256) {
+        int i = e & 63, j = e >> 6;
+        L.Hw[i][j] = (i < ns && j < ns) ? H[(r0 + i) + (long)(r0 + j) * ldh] : mk(0.0, 0.0);
+      }
+      __syncthreads();
+      if ((t >> 6) == 0) wave_schur(L.Hw, L.U, ns, false, t & 63);
+      __syncthreads();
+      if (t < ns) L.shifts[t] = L.Hw[t][t];
       __syncthreads();
     }
     qr_sweep(L, N, H, ldh, Z, ldz, lo, ihi, nbul, st);
@@ -996,21 +1222,24 @@ __global__ __launch_bounds__(256) void balance_kernel(int N, cd* A, long lda, lo
   }
 }
 
-void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long sw, cd* VR, long ldv, long sV,
+static int g_aed_window = AED_NW;
+void zgeev_set_aed(int window) { g_aed_window = window < 0 ? 0 : (window > QW ? QW : window); }
+
+// Schur factors of a strided batch (A destroyed): A = D Z T Z^H D^{-1} with D = diag(dbal)
+// (balancing), Z unitary (ld n), T upper triangular (left in A); X (ld n, unit upper
+// triangular) holds the eigenvectors of T, so that W = D Z X are eigenvectors of A
+// (not normalized).
+void zgeev_factors(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long sw, cd* Z, cd* X, cd* dbal,
                    int* info, int batch, Workspace& ws) {
   if (n <= 0 || batch <= 0) return;
   const long nn = (long)n * n;
   const int nref = n_reflectors(n);
   const int npanel = nref > 0 ? cdiv_up(nref, HB) : 1;
-  DevBuf Vfb = ws.get(sizeof(cd) * (size_t)nn * batch);
   DevBuf Tb = ws.get(sizeof(cd) * (size_t)npanel * HB * HB * batch);
-  DevBuf Zb = ws.get(sizeof(cd) * (size_t)nn * batch);
-  cd* Vf = Vfb.as<cd>();
-  cd* Z = Zb.as<cd>();
-  DevBuf Db = ws.get(sizeof(cd) * (size_t)n * batch);
+  cd* Vf = X;  // the reflectors live in X until the eigenvectors overwrite them
   {
     StageScope st(ST_HESS);
-    balance_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Db.as<cd>(), n);
+    balance_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, dbal, n);
     launch_check("balance");
     gehrd(s, n, A, lda, sA, Vf, Tb.as<cd>(), batch, ws);
     orghr(s, n, Vf, Tb.as<cd>(), Z, n, nn, batch, ws);
@@ -1021,7 +1250,8 @@ void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
     const bool want = c && c->timing;
     DevBuf sb;
     if (want) sb = ws.get(sizeof(long long) * 5 * batch);
-    hqr_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, want ? sb.as<long long>() : nullptr);
+    hqr_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, want ? sb.as<long long>() : nullptr,
+                                     g_aed_window);
     launch_check("hqr");
     if (want) {
       std::vector<long long> h(5 * (size_t)batch);
@@ -1033,16 +1263,33 @@ void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
     }
   }
   StageScope st(ST_EIGVEC);
-  cd* X = Vf;  // reflectors no longer needed
   zset(s, n, n, mk(0.0, 0.0), mk(0.0, 0.0), X, n, nn, batch);
   dim3 g(cdiv_up(n, QW), batch);
   trevc_kernel<<<g, 256, 0, s>>>(n, A, lda, sA, X, n, nn);
   launch_check("trevc");
+}
+
+// VR = D Z X with LAPACK zgeev's normalisation (unit 2-norm, largest component real).
+void zgeev_vectors(hipStream_t s, int n, const cd* Z, const cd* X, const cd* dbal, cd* VR, long ldv, long sV,
+                   int batch) {
+  if (n <= 0 || batch <= 0) return;
+  StageScope st(ST_EIGVEC);
+  const long nn = (long)n * n;
   zgemm_strided(s, OP_N, OP_N, n, n, n, mk(1.0, 0.0), Z, n, nn, X, n, nn, mk(0.0, 0.0), VR, ldv, sV, batch);
-  zdscale(s, n, n, Db.as<cd>(), n, nullptr, 0, 0, VR, ldv, sV, VR, ldv, sV, batch);  // undo the balancing
+  zdscale(s, n, n, dbal, n, nullptr, 0, 0, VR, ldv, sV, VR, ldv, sV, batch);  // undo the balancing
   dim3 gn(cdiv_up(n, 4), batch);
   normalize_cols_kernel<<<gn, 256, 0, s>>>(n, VR, ldv, sV);
   launch_check("normalize_cols");
+}
+
+void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long sw, cd* VR, long ldv, long sV,
+                   int* info, int batch, Workspace& ws) {
+  if (n <= 0 || batch <= 0) return;
+  const long nn = (long)n * n;
+  DevBuf Zb = ws.get(sizeof(cd) * (size_t)nn * batch), Xb = ws.get(sizeof(cd) * (size_t)nn * batch);
+  DevBuf Db = ws.get(sizeof(cd) * (size_t)n * batch);
+  zgeev_factors(s, n, A, lda, sA, w, sw, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(), info, batch, ws);
+  zgeev_vectors(s, n, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(), VR, ldv, sV, batch);
 }
 
 __global__ void gamma_sqrt_kernel(int n, const cd* lam, long sl, cd* gam, long sg) {

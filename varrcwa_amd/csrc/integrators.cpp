@@ -542,25 +542,31 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     DevBuf dsv = upload(c, sv.data(), sizeof(double) * sv.size());
     DevBuf down = upload(c, owner.data(), sizeof(int) * owner.size());
     // ---- one eigen-solve launch for end media and slices: [bottoms | tops | slices]
-    DevBuf Wall = c.ws.get(sizeof(cd) * NN * J), Vall = c.ws.get(sizeof(cd) * NN * J),
+    const int SL = B * L;
+    AssemblyBatch a;
+    a.nx = nx;
+    a.ny = ny;
+    a.nlayers = J;
+    a.layers = jl;
+    a.periodic = true;
+    a.d_alpha = dab.as<cd>();
+    a.d_beta = a.d_alpha + n;
+    a.sab = 2 * n;
+    a.d_k0sq = dk2.as<double>();
+    DevBuf P = c.ws.get(sizeof(cd) * NN * J), Q = c.ws.get(sizeof(cd) * NN * J);
+    assemble_pq(c, a, P.as<cd>(), Q.as<cd>());
+    const int JW = tape ? J : 2 * B;  // jobs whose W, V are formed explicitly
+    DevBuf Wall = c.ws.get(sizeof(cd) * NN * JW), Vall = c.ws.get(sizeof(cd) * NN * JW),
            gall = c.ws.get(sizeof(cd) * (size_t)N * J), infall = c.ws.get(sizeof(int) * J);
-    auto solve_jobs = [&](int j0, int cnt, cd* W, cd* V, cd* gam, int* inf) {
-      AssemblyBatch a;
-      a.nx = nx;
-      a.ny = ny;
-      a.nlayers = cnt;
-      a.layers.assign(jl.begin() + j0, jl.begin() + j0 + cnt);
-      a.periodic = true;
-      a.d_alpha = dab.as<cd>() + (size_t)j0 * 2 * n;
-      a.d_beta = a.d_alpha + n;
-      a.sab = 2 * n;
-      a.d_k0sq = dk2.as<double>() + j0;
-      DevBuf P = c.ws.get(sizeof(cd) * NN * cnt), Q = c.ws.get(sizeof(cd) * NN * cnt);
-      assemble_pq(c, a, P.as<cd>(), Q.as<cd>());
-      wave_eqn(c, N, P.as<cd>(), Q.as<cd>(), NN, W, V, gam, inf, cnt);
-    };
-    solve_jobs(0, J, Wall.as<cd>(), Vall.as<cd>(), gall.as<cd>(), infall.as<int>());
+    fill_int(st, infall.as<int>(), 0, J);
+    DevBuf tu = c.ws.get(sizeof(cd) * NN * SL), ru = c.ws.get(sizeof(cd) * NN * SL);
+    DevBuf Wr = c.ws.get(sizeof(cd) * NN * SL), Vr = c.ws.get(sizeof(cd) * NN * SL);
+    const cd* W0 = Wall.as<cd>();
+    const cd* V0 = Vall.as<cd>();
+    const cd* W1 = Wall.as<cd>() + B * NN;
+    const cd* V1 = Vall.as<cd>() + B * NN;
     if (tape) {
+      wave_eqn(c, N, P.as<cd>(), Q.as<cd>(), NN, Wall.as<cd>(), Vall.as<cd>(), gall.as<cd>(), infall.as<int>(), J);
       // eigen-systems in the tape's order: bottoms [inst], tops [inst], slabs [inst][L]
       auto keep = [&](int rows, const cd* src, cd* dst, int cnt) {
         zgeam(st, rows, 1, ONE, src, rows, rows, ZERO, nullptr, 0, 0, dst, rows, rows, cnt);
@@ -575,23 +581,31 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
       keep(nn2, Wall.as<cd>() + 2 * B * NN, tape->Ws.as<cd>() + (long)i0 * L * NN, B * L);
       keep(nn2, Vall.as<cd>() + 2 * B * NN, tape->Vs.as<cd>() + (long)i0 * L * NN, B * L);
       keep(N, gall.as<cd>() + (long)2 * B * N, tape->gs.as<cd>() + (long)i0 * L * N, B * L);
-    }
-    const cd* W0 = Wall.as<cd>();
-    const cd* V0 = Vall.as<cd>();
-    const cd* W1 = Wall.as<cd>() + B * NN;
-    const cd* V1 = Vall.as<cd>() + B * NN;
-    LUStore lu1;
-    lu1.make(c, N, W1, V1, B, NN);
-    // ---- slices: layer S in the reference basis of the owning instance
-    const int SL = B * L;
-    DevBuf tu = c.ws.get(sizeof(cd) * NN * SL), ru = c.ws.get(sizeof(cd) * NN * SL);
-    {
-      DevBuf Wr = c.ws.get(sizeof(cd) * NN * SL), Vr = c.ws.get(sizeof(cd) * NN * SL);
       gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
       gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
       layer_smatrix(c, N, SL, dsv.as<double>(), Wall.as<cd>() + 2 * B * NN, Vall.as<cd>() + 2 * B * NN,
                     gall.as<cd>() + (long)2 * B * N, Wr.as<cd>(), Vr.as<cd>(), NN, tu.as<cd>(), ru.as<cd>());
+    } else {
+      // slabs keep their eigenvectors factored (W = D Z X); only the end media form W, V
+      DevBuf Zb = c.ws.get(sizeof(cd) * NN * J), Xb = c.ws.get(sizeof(cd) * NN * J),
+             Db = c.ws.get(sizeof(cd) * (size_t)N * J);
+      wave_eqn_factors(c, N, P.as<cd>(), Q.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(), gall.as<cd>(),
+                       infall.as<int>(), J);
+      wave_eqn_vectors(c, N, Q.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(), gall.as<cd>(), Wall.as<cd>(),
+                       Vall.as<cd>(), 2 * B);
+      gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
+      gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
+      const long o = 2L * B * NN;
+      layer_smatrix_factors(c, N, SL, dsv.as<double>(), P.as<cd>() + o, NN, Zb.as<cd>() + o, Xb.as<cd>() + o,
+                            Db.as<cd>() + (long)2 * B * N, gall.as<cd>() + (long)2 * B * N, Wr.as<cd>(), Vr.as<cd>(),
+                            NN, tu.as<cd>(), ru.as<cd>());
     }
+    Wr = DevBuf();
+    Vr = DevBuf();
+    P = DevBuf();
+    Q = DevBuf();
+    LUStore lu1;
+    lu1.make(c, N, W1, V1, B, NN);
     // ---- cascade over layers, batched over instances
     DevBuf Sb = c.ws.get(sizeof(cd) * 4 * NN * B);
     SMat S;

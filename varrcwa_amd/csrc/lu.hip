@@ -246,6 +246,12 @@ static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* 
   }
 }
 
+void ztrsm_upper(hipStream_t s, int n, int nrhs, const cd* U, long ldu, long sU, cd* Y, long ldy, long sY,
+                 int batch) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return;
+  trsm_lu_factor(s, 1, n, nrhs, U, ldu, sU, Y, ldy, sY, batch);
+}
+
 void zgetrs(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, const int* ipiv, long sP, const cd* B,
             long ldb, long sB, cd* X, long ldx, long sX, int batch, Workspace& ws) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return;

@@ -230,18 +230,19 @@ void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W,
     phase_kernel<<<g, 256, 0, s>>>(N, gam, N, d_s, Xb.as<cd>(), N);
     launch_check("phase");
   }
+  // RHS = [B | X A]; invAXB = A^{-1} X B = (A^{-1} X A)(A^{-1} B) is then one product
   DevBuf R3b = c.ws.get(sizeof(cd) * (size_t)NN * 3 * batch);
   cd* R3 = R3b.as<cd>();
   const long s3 = 3 * NN;
   zgeam(s, N, N, ONE, F0, N, NN, ONE, G0, N, NN, LU, N, NN, batch);              // A
   zgeam(s, N, N, ONE, F0, N, NN, MONE, G0, N, NN, R3, N, s3, batch);             // B
   zdscale(s, N, N, Xb.as<cd>(), N, nullptr, 0, 0, LU, N, NN, R3 + NN, N, s3, batch);      // X A
-  zdscale(s, N, N, Xb.as<cd>(), N, nullptr, 0, 0, R3, N, s3, R3 + 2 * NN, N, s3, batch);  // X B
   zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
-  zgetrs(s, N, 3 * N, LU, N, NN, piv, N, R3, N, s3, R3, N, s3, batch, c.ws);
+  zgetrs(s, N, 2 * N, LU, N, NN, piv, N, R3, N, s3, R3, N, s3, batch, c.ws);
   const cd* invAB = R3;
   const cd* invAXA = R3 + NN;
   const cd* invAXB = R3 + 2 * NN;
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, invAXA, N, s3, invAB, N, s3, ZERO, R3 + 2 * NN, N, s3, batch);
   // M = I - invAXB^2 ; RHS2 = [invAXB invAXA - invAB | invAXA - invAXB invAB]
   zset(s, N, N, ONE, ZERO, LU, N, NN, batch);
   zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, invAXB, N, s3, invAXB, N, s3, ONE, LU, N, NN, batch);
@@ -256,6 +257,114 @@ void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W,
   zgetrs(s, N, 2 * N, LU, N, NN, piv, N, R2, N, s2, R2, N, s2, batch, c.ws);
   zgeam(s, N, N, ONE, R2, N, s2, ZERO, nullptr, 0, 0, Rud, N, NN, batch);
   zgeam(s, N, N, ONE, R2 + NN, N, s2, ZERO, nullptr, 0, 0, Tuu, N, NN, batch);
+}
+
+// C = diag(1/d) A (row scaling by the reciprocal of d), batched.
+__global__ void rowscale_inv_kernel(int m, int n, const cd* d, long sd, const cd* A, long lda, long sA, cd* C,
+                                    long ldc, long sC) {
+  const long b = blockIdx.z;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = blockIdx.y;
+  if (r >= m) return;
+  C[b * sC + r + (long)col * ldc] = cdiv(A[b * sA + r + (long)col * lda], d[b * sd + r]);
+}
+
+static void rowscale_inv(hipStream_t s, int m, int n, const cd* d, long sd, const cd* A, long lda, long sA, cd* C,
+                         long ldc, long sC, int batch) {
+  dim3 g(cdiv_up(m, 256), n, batch);
+  rowscale_inv_kernel<<<g, 256, 0, s>>>(m, n, d, sd, A, lda, sA, C, ldc, sC);
+  launch_check("rowscale_inv");
+}
+
+// (c) as wave_eqn, but the eigenvectors stay factored: PQ = D Z T Z^H D^{-1},
+// W = D Z X (unnormalized; every S-matrix is invariant under the column scaling).
+void wave_eqn_factors(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* Z, cd* X, cd* dbal, cd* gam,
+                      int* info, int batch) {
+  hipStream_t s = c.stream;
+  const long NN = (long)N * N;
+  DevBuf PQb = c.ws.get(sizeof(cd) * (size_t)NN * batch);
+  DevBuf lamb = c.ws.get(sizeof(cd) * (size_t)N * batch);
+  {
+    StageScope st(ST_EIGPOST);
+    zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, P, N, sPQ, Q, N, sPQ, ZERO, PQb.as<cd>(), N, NN, batch);
+  }
+  zgeev_factors(s, N, PQb.as<cd>(), N, NN, lamb.as<cd>(), N, Z, X, dbal, info, batch, c.ws);
+  StageScope st(ST_EIGPOST);
+  gamma_sqrt(s, N, lamb.as<cd>(), N, gam, N, batch);
+}
+
+// Normalized W (zgeev convention) and V = Q W diag(1/gamma) from the factors.
+void wave_eqn_vectors(vrc_ctx& c, int N, const cd* Q, long sQ, const cd* Z, const cd* X, const cd* dbal,
+                      const cd* gam, cd* W, cd* V, int batch) {
+  hipStream_t s = c.stream;
+  const long NN = (long)N * N;
+  zgeev_vectors(s, N, Z, X, dbal, W, N, NN, batch);
+  StageScope st(ST_EIGPOST);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, Q, N, sQ, W, N, NN, ZERO, V, N, NN, batch);
+  zdscale(s, N, N, nullptr, 0, gam, N, 1, V, N, NN, V, N, NN, batch);
+}
+
+// (d) Slab S-matrix from the eigen factors, without forming W or V:
+//   W^{-1} = X^{-1} Z^H D^{-1}, and since PQ W = W Gamma^2 gives V = Q W / Gamma =
+//   P^{-1} W Gamma, V^{-1} V0 = Gamma^{-1} W^{-1} (P V0). So
+//   [F0 | Gamma G0] = X^{-1} Z^H D^{-1} [W0 | P V0]  (one triangular solve, no LU),
+// then the arithmetic of RCWAScatterMatrix::compute (src/rcwa/RCWAScatterMatrix.cpp:26-85).
+void layer_smatrix_factors(vrc_ctx& c, int N, int batch, const double* d_s, const cd* P, long sP, const cd* Z,
+                           const cd* X, const cd* dbal, const cd* gam, const cd* W0, const cd* V0, long sRef,
+                           cd* Tuu, cd* Rud) {
+  StageScope st(ST_LAYER_S);
+  hipStream_t s = c.stream;
+  const long NN = (long)N * N, s2 = 2 * NN;
+  DevBuf R2b = c.ws.get(sizeof(cd) * (size_t)NN * 2 * batch), Yb = c.ws.get(sizeof(cd) * (size_t)NN * 2 * batch);
+  cd* R2 = R2b.as<cd>();
+  cd* Y = Yb.as<cd>();
+  // R2 = D^{-1} [W0 | P V0]
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, P, N, sP, V0, N, sRef, ZERO, Y + NN, N, s2, batch);
+  zgeam(s, N, N, ONE, W0, N, sRef, ZERO, nullptr, 0, 0, Y, N, s2, batch);
+  rowscale_inv(s, N, 2 * N, dbal, N, Y, N, s2, R2, N, s2, batch);
+  // Y = Z^H R2 ; Y <- X^{-1} Y
+  zgemm_strided(s, OP_C, OP_N, N, 2 * N, N, ONE, Z, N, NN, R2, N, s2, ZERO, Y, N, s2, batch);
+  ztrsm_upper(s, N, 2 * N, X, N, NN, Y, N, s2, batch);
+  // F0 = Y[:, :N] ; G0 = Gamma^{-1} Y[:, N:]
+  cd* F0 = Y;
+  cd* G0 = R2;  // reuse
+  rowscale_inv(s, N, N, gam, N, Y + NN, N, s2, G0, N, NN, batch);
+  DevBuf LUb = c.ws.get(sizeof(cd) * (size_t)NN * batch);
+  DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * batch);
+  DevBuf infb = c.ws.get(sizeof(int) * (size_t)batch);
+  cd* LU = LUb.as<cd>();
+  int* piv = pivb.as<int>();
+  fill_int(s, infb.as<int>(), 0, batch);
+  DevBuf Xb = c.ws.get(sizeof(cd) * (size_t)N * batch);
+  {
+    dim3 g(cdiv_up(N, 256), batch);
+    phase_kernel<<<g, 256, 0, s>>>(N, gam, N, d_s, Xb.as<cd>(), N);
+    launch_check("phase");
+  }
+  DevBuf R3b = c.ws.get(sizeof(cd) * (size_t)NN * 3 * batch);
+  cd* R3 = R3b.as<cd>();
+  const long s3 = 3 * NN;
+  zgeam(s, N, N, ONE, F0, N, s2, ONE, G0, N, NN, LU, N, NN, batch);              // A
+  zgeam(s, N, N, ONE, F0, N, s2, MONE, G0, N, NN, R3, N, s3, batch);             // B
+  zdscale(s, N, N, Xb.as<cd>(), N, nullptr, 0, 0, LU, N, NN, R3 + NN, N, s3, batch);  // X A
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrs(s, N, 2 * N, LU, N, NN, piv, N, R3, N, s3, R3, N, s3, batch, c.ws);
+  const cd* invAB = R3;
+  const cd* invAXA = R3 + NN;
+  const cd* invAXB = R3 + 2 * NN;
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, invAXA, N, s3, invAB, N, s3, ZERO, R3 + 2 * NN, N, s3, batch);
+  // M = I - invAXB^2 ; RHS2 = [invAXB invAXA - invAB | invAXA - invAXB invAB]
+  zset(s, N, N, ONE, ZERO, LU, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, invAXB, N, s3, invAXB, N, s3, ONE, LU, N, NN, batch);
+  cd* RH = Y;  // 2N columns, F0/G0 consumed
+  zgeam(s, N, N, ONE, invAB, N, s3, ZERO, nullptr, 0, 0, RH, N, s2, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, invAXB, N, s3, invAXA, N, s3, MONE, RH, N, s2, batch);
+  zgeam(s, N, N, ONE, invAXA, N, s3, ZERO, nullptr, 0, 0, RH + NN, N, s2, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, invAXB, N, s3, invAB, N, s3, ONE, RH + NN, N, s2, batch);
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrs(s, N, 2 * N, LU, N, NN, piv, N, RH, N, s2, RH, N, s2, batch, c.ws);
+  zgeam(s, N, N, ONE, RH, N, s2, ZERO, nullptr, 0, 0, Rud, N, NN, batch);
+  zgeam(s, N, N, ONE, RH + NN, N, s2, ZERO, nullptr, 0, 0, Tuu, N, NN, batch);
 }
 
 // ===================================================================== (d) star product

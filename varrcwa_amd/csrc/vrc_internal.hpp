@@ -73,6 +73,9 @@ void zmaxabs(hipStream_t s, int m, int n, const cd* A, long lda, long sA, double
 void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, int* info, int batch);
 void zgetrs(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, const int* ipiv, long sP, const cd* B,
             long ldb, long sB, cd* X, long ldx, long sX, int batch, Workspace& ws);
+// Y <- U^{-1} Y for upper-triangular U (non-unit diagonal), in place.
+void ztrsm_upper(hipStream_t s, int n, int nrhs, const cd* U, long ldu, long sU, cd* Y, long ldy, long sY,
+                 int batch);
 void zgetri(hipStream_t s, int n, const cd* LU, long lda, long sA, const int* ipiv, long sP, cd* Ainv, long ldi,
             long sI, int batch, Workspace& ws);
 
@@ -84,6 +87,13 @@ void zgetri(hipStream_t s, int n, const cd* LU, long lda, long sA, const int* ip
 void zgeev_batched(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long sw, cd* VR, long ldv, long sV,
                    int* info, int batch, Workspace& ws);
 
+// Schur factors (W = diag(dbal) Z X, X unit upper triangular, not normalized) and the
+// normalized eigenvectors from them.
+void zgeev_factors(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long sw, cd* Z, cd* X, cd* dbal,
+                   int* info, int batch, Workspace& ws);
+void zgeev_vectors(hipStream_t s, int n, const cd* Z, const cd* X, const cd* dbal, cd* VR, long ldv, long sV,
+                   int batch);
+void zgeev_set_aed(int window);  // deflation-window size of the QR iteration (0 = off)
 // gamma = sqrt(lambda) with the reference's branch rule.
 void gamma_sqrt(hipStream_t s, int n, const cd* lam, long sl, cd* gam, long sg, int batch);
 

@@ -61,6 +61,12 @@ void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q);
 // PQ = P Q; eig; gamma = sqrt branch; V = Q W diag(1/gamma)  (src/rcwa/WaveEquationSolver.cpp:24-37)
 void wave_eqn(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* W, cd* V, cd* gam, int* info, int batch);
 
+// Eigen factors instead of W, V: W = diag(dbal) Z X (unnormalized), and the vectors.
+void wave_eqn_factors(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* Z, cd* X, cd* dbal, cd* gam,
+                      int* info, int batch);
+void wave_eqn_vectors(vrc_ctx& c, int N, const cd* Q, long sQ, const cd* Z, const cd* X, const cd* dbal,
+                      const cd* gam, cd* W, cd* V, int batch);
+
 // ---------------------------------------------------------------- (d)
 struct SMat {
   cd *Tdd = nullptr, *Tuu = nullptr, *Rud = nullptr, *Rdu = nullptr;
@@ -81,6 +87,10 @@ void gather_mats(hipStream_t s, long nn, const cd* src, long ss, const int* d_id
 // Slab S-matrix (src/rcwa/RCWAScatterMatrix.cpp:26-85); d_s[b] = dz_b / k0_b.
 void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
                    const cd* W0, const cd* V0, long sRef, cd* Tuu, cd* Rud);
+// Slab S-matrix from the eigen factors of the slab (P per slab, stride sP).
+void layer_smatrix_factors(vrc_ctx& c, int N, int batch, const double* d_s, const cd* P, long sP, const cd* Z,
+                           const cd* X, const cd* dbal, const cd* gam, const cd* W0, const cd* V0, long sRef,
+                           cd* Tuu, cd* Rud);
 // Star product of src/core/RCWAIntegrator.cpp:57-67 (S updated in place; slice stride ss).
 void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
                   const cd* rdu, long ss);
