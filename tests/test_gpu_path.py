@@ -217,3 +217,46 @@ def test_batch_stack_parity(ctx):
     got = O.SResult(S[0, 0], S[0, 1], S[0, 2], S[0, 3], bases["W0"][0], bases["V0"][0], bases["gam0"][0],
                     bases["W1"][0], bases["V1"][0], bases["gam1"][0])
     invariants_close(gpu_result(it), got, tol=1e-11)
+
+
+def test_batch_stack_patterned_ends(ctx):
+    """End media with several cells take the dense eigensolver path of the batch."""
+    nx = ny = 3
+    rng = np.random.default_rng(9)
+    insts = []
+    for s in range(2):
+        c = _stack_case(s)
+        c["bottom"] = G.random_rect_layer(rng, 0.6)
+        insts.append(c)
+    out = api.batch_stack_rcwa(ctx, nx, ny, insts, want_bases=True)
+    S = from_dev_colmajor(out["S"])
+    bases = {k: (from_dev_colmajor(out[k]) if not k.startswith("gam") else out[k].cpu().numpy())
+             for k in ["W0", "V0", "gam0", "W1", "V1", "gam1"]}
+    for i, inst in enumerate(insts):
+        ze = np.concatenate([[0.0], np.cumsum(inst["thickness"])])
+        so = O.StackLayerSampler(inst["lam"], nx, ny, inst["layers"], ze, inst["bottom"], inst["top"])
+        ref = O.rcwa_integrate(so, ze[0], ze[-1], len(inst["layers"]), z_edges=ze)
+        got = O.SResult(S[i, 0], S[i, 1], S[i, 2], S[i, 3], bases["W0"][i], bases["V0"][i], bases["gam0"][i],
+                        bases["W1"][i], bases["V1"][i], bases["gam1"][i])
+        invariants_close(got, ref)
+
+
+def test_homogeneous_closed_form(ctx):
+    """Single-value end media: W = I, PQ W = W Gamma^2 and V = Q W / Gamma against the
+    oracle's P and Q (the closed form of a homogeneous medium)."""
+    nx, ny, lam = 3, 5, 1.3
+    rng = np.random.default_rng(4)
+    bottom = G.homogeneous(0.7, 2.1 + 0.05j)
+    inst = dict(lam=lam, layers=[G.random_rect_layer(rng, 0.7)], thickness=[0.1], bottom=bottom,
+                top=G.homogeneous(0.7, 1.0))
+    out = api.batch_stack_rcwa(ctx, nx, ny, [inst], want_bases=True)
+    W0 = from_dev_colmajor(out["W0"])[0]
+    V0 = from_dev_colmajor(out["V0"])[0]
+    g0 = out["gam0"].cpu().numpy()[0]
+    ze = np.array([0.0, 0.1])
+    P, Q = O.StackLayerSampler(lam, nx, ny, inst["layers"], ze, bottom, inst["top"]).sample(-1.0)
+    assert np.array_equal(W0, np.eye(W0.shape[0]))
+    PQ = P @ Q
+    assert np.abs(PQ @ W0 - W0 * (g0 ** 2)[None, :]).max() <= 1e-13 * np.abs(PQ).max()
+    assert np.abs(V0 - (Q @ W0) / g0[None, :]).max() <= 1e-13 * np.abs(V0).max()
+    assert np.all(g0.imag >= 0)

@@ -1296,10 +1296,7 @@ __global__ void gamma_sqrt_kernel(int n, const cd* lam, long sl, cd* gam, long s
   const long b = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  cd g = csqrt_principal(lam[b * sl + i]);
-  if (g.y < 0.0) g = cneg(g);
-  if (fabs(g.y) < fabs(g.x) && g.x < 0.0) g = cneg(g);
-  gam[b * sg + i] = g;
+  gam[b * sg + i] = gamma_branch(lam[b * sl + i]);
 }
 
 void gamma_sqrt(hipStream_t s, int n, const cd* lam, long sl, cd* gam, long sg, int batch) {

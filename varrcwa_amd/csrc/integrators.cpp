@@ -501,10 +501,15 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     if (!I.bottom || !I.top) throw ArgFailure("instance without bottom/top medium");
     if (!(I.lambda > 0)) throw ArgFailure("wavelength must be positive");
   }
-  // Default chunk: at most 256 eigensolves (L slices + 2 end media per instance) per
-  // launch, one matrix per CU of the QR kernel; the instances are then split evenly
-  // over the chunks so that no launch runs a short tail.
-  if (chunk <= 0) chunk = std::max(1, std::min(ninst, 256 / (L + 2)));
+  // End media given by one permittivity value get their eigen-system in closed form.
+  bool homog = true;
+  for (auto& I : inst)
+    homog = homog && I.bottom->x.size() == 2 && I.bottom->y.size() == 2 && I.top->x.size() == 2 &&
+            I.top->y.size() == 2;
+  // Default chunk: at most 256 dense eigensolves (L slices, + 2 end media per instance
+  // unless closed form) per launch, one matrix per CU of the QR kernel; the instances
+  // are then split evenly over the chunks so that no launch runs a short tail.
+  if (chunk <= 0) chunk = std::max(1, std::min(ninst, 256 / (L + (homog ? 0 : 2))));
   const int nchunks = (ninst + chunk - 1) / chunk;
   std::vector<int> starts(nchunks + 1, 0);
   for (int ci = 0; ci < nchunks; ++ci) starts[ci + 1] = starts[ci] + ninst / nchunks + (ci < ninst % nchunks ? 1 : 0);
@@ -541,32 +546,43 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     DevBuf dab = upload(c, ab.data(), sizeof(cd) * ab.size());
     DevBuf dsv = upload(c, sv.data(), sizeof(double) * sv.size());
     DevBuf down = upload(c, owner.data(), sizeof(int) * owner.size());
-    // ---- one eigen-solve launch for end media and slices: [bottoms | tops | slices]
+    // ---- one eigen-solve launch for end media (unless closed form) and slices
     const int SL = B * L;
+    const int je = homog ? 2 * B : 0;  // first job solved by the dense eigensolver
+    const int JD = J - je;
     AssemblyBatch a;
     a.nx = nx;
     a.ny = ny;
-    a.nlayers = J;
-    a.layers = jl;
+    a.nlayers = JD;
+    a.layers.assign(jl.begin() + je, jl.end());
     a.periodic = true;
-    a.d_alpha = dab.as<cd>();
+    a.d_alpha = dab.as<cd>() + (size_t)je * 2 * n;
     a.d_beta = a.d_alpha + n;
     a.sab = 2 * n;
-    a.d_k0sq = dk2.as<double>();
-    DevBuf P = c.ws.get(sizeof(cd) * NN * J), Q = c.ws.get(sizeof(cd) * NN * J);
+    a.d_k0sq = dk2.as<double>() + je;
+    DevBuf P = c.ws.get(sizeof(cd) * NN * JD), Q = c.ws.get(sizeof(cd) * NN * JD);
     assemble_pq(c, a, P.as<cd>(), Q.as<cd>());
     const int JW = tape ? J : 2 * B;  // jobs whose W, V are formed explicitly
     DevBuf Wall = c.ws.get(sizeof(cd) * NN * JW), Vall = c.ws.get(sizeof(cd) * NN * JW),
            gall = c.ws.get(sizeof(cd) * (size_t)N * J), infall = c.ws.get(sizeof(int) * J);
     fill_int(st, infall.as<int>(), 0, J);
+    if (homog) {
+      std::vector<cd> ev(2 * B);
+      for (int j = 0; j < 2 * B; ++j) ev[j] = jl[j]->eps[0];
+      DevBuf dev = upload(c, ev.data(), sizeof(cd) * ev.size());
+      homogeneous_eig(st, n, 2 * B, dab.as<cd>(), dab.as<cd>() + n, 2 * n, dk2.as<double>(), dev.as<cd>(),
+                      Wall.as<cd>(), Vall.as<cd>(), gall.as<cd>());
+    }
     DevBuf tu = c.ws.get(sizeof(cd) * NN * SL), ru = c.ws.get(sizeof(cd) * NN * SL);
     DevBuf Wr = c.ws.get(sizeof(cd) * NN * SL), Vr = c.ws.get(sizeof(cd) * NN * SL);
     const cd* W0 = Wall.as<cd>();
     const cd* V0 = Vall.as<cd>();
     const cd* W1 = Wall.as<cd>() + B * NN;
     const cd* V1 = Vall.as<cd>() + B * NN;
+    const long s0 = (long)(2 * B - je);  // first slab within the dense jobs
     if (tape) {
-      wave_eqn(c, N, P.as<cd>(), Q.as<cd>(), NN, Wall.as<cd>(), Vall.as<cd>(), gall.as<cd>(), infall.as<int>(), J);
+      wave_eqn(c, N, P.as<cd>(), Q.as<cd>(), NN, Wall.as<cd>() + (long)je * NN, Vall.as<cd>() + (long)je * NN,
+               gall.as<cd>() + (long)je * N, infall.as<int>() + je, JD);
       // eigen-systems in the tape's order: bottoms [inst], tops [inst], slabs [inst][L]
       auto keep = [&](int rows, const cd* src, cd* dst, int cnt) {
         zgeam(st, rows, 1, ONE, src, rows, rows, ZERO, nullptr, 0, 0, dst, rows, rows, cnt);
@@ -587,18 +603,18 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
                     gall.as<cd>() + (long)2 * B * N, Wr.as<cd>(), Vr.as<cd>(), NN, tu.as<cd>(), ru.as<cd>());
     } else {
       // slabs keep their eigenvectors factored (W = D Z X); only the end media form W, V
-      DevBuf Zb = c.ws.get(sizeof(cd) * NN * J), Xb = c.ws.get(sizeof(cd) * NN * J),
-             Db = c.ws.get(sizeof(cd) * (size_t)N * J);
-      wave_eqn_factors(c, N, P.as<cd>(), Q.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(), gall.as<cd>(),
-                       infall.as<int>(), J);
-      wave_eqn_vectors(c, N, Q.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(), gall.as<cd>(), Wall.as<cd>(),
-                       Vall.as<cd>(), 2 * B);
+      DevBuf Zb = c.ws.get(sizeof(cd) * NN * JD), Xb = c.ws.get(sizeof(cd) * NN * JD),
+             Db = c.ws.get(sizeof(cd) * (size_t)N * JD);
+      wave_eqn_factors(c, N, P.as<cd>(), Q.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(),
+                       gall.as<cd>() + (long)je * N, infall.as<int>() + je, JD);
+      if (!homog)
+        wave_eqn_vectors(c, N, Q.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(), gall.as<cd>(), Wall.as<cd>(),
+                         Vall.as<cd>(), 2 * B);
       gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
       gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
-      const long o = 2L * B * NN;
-      layer_smatrix_factors(c, N, SL, dsv.as<double>(), P.as<cd>() + o, NN, Zb.as<cd>() + o, Xb.as<cd>() + o,
-                            Db.as<cd>() + (long)2 * B * N, gall.as<cd>() + (long)2 * B * N, Wr.as<cd>(), Vr.as<cd>(),
-                            NN, tu.as<cd>(), ru.as<cd>());
+      layer_smatrix_factors(c, N, SL, dsv.as<double>(), P.as<cd>() + s0 * NN, NN, Zb.as<cd>() + s0 * NN,
+                            Xb.as<cd>() + s0 * NN, Db.as<cd>() + s0 * N, gall.as<cd>() + (long)2 * B * N,
+                            Wr.as<cd>(), Vr.as<cd>(), NN, tu.as<cd>(), ru.as<cd>());
     }
     Wr = DevBuf();
     Vr = DevBuf();

@@ -367,6 +367,47 @@ void layer_smatrix_factors(vrc_ctx& c, int N, int batch, const double* d_s, cons
   zgeam(s, N, N, ONE, RH + NN, N, s2, ZERO, nullptr, 0, 0, Tuu, N, NN, batch);
 }
 
+// Closed-form eigen-system of a homogeneous periodic medium (one permittivity value):
+// P and Q only couple the harmonic pair (i, n+i), and on it PQ = lambda_i I_2 with
+// lambda_i = k0^2 (k0^2 eps - alpha_i^2 - beta_i^2) (exactly, from the block formulas of
+// src/rcwa/Layer.cpp:95-100,134-138 with fe = eps I). So W = I, gamma_i from lambda_i
+// and V = Q diag(1/gamma). Per batch entry b: alpha/beta at stride sab, k0sq[b], eps[b].
+__global__ void homogeneous_eig_kernel(int n, const cd* alpha, const cd* beta, long sab, const double* k0sq,
+                                       const cd* eps, cd* W, cd* V, cd* gam, long sM) {
+  const long b = blockIdx.z;
+  const int N = 2 * n;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= N) return;
+  const int ic = c % n;
+  const cd a = alpha[b * sab + ic], be = beta[b * sab + ic];
+  const double k2 = k0sq[b];
+  const cd e = eps[b];
+  const cd lam = cscale(csub(cscale(e, k2), cadd(cmul(a, a), cmul(be, be))), k2);
+  const cd g = gamma_branch(lam);
+  cd q = mk(0.0, 0.0);
+  if (r % n == ic) {
+    const int rb = r / n, cb = c / n;
+    if (rb == 0 && cb == 0) q = cneg(cmul(a, be));
+    else if (rb == 0) q = csub(cmul(a, a), cscale(e, k2));
+    else if (cb == 0) q = csub(cscale(e, k2), cmul(be, be));
+    else q = cmul(be, a);
+  }
+  const long o = b * sM + r + (long)c * N;
+  W[o] = (r == c) ? mk(1.0, 0.0) : mk(0.0, 0.0);
+  V[o] = cdiv(q, g);
+  if (r == 0) gam[b * N + c] = g;
+}
+
+void homogeneous_eig(hipStream_t s, int n, int batch, const cd* alpha, const cd* beta, long sab, const double* k0sq,
+                     const cd* eps, cd* W, cd* V, cd* gam) {
+  if (batch <= 0) return;
+  const int N = 2 * n;
+  dim3 g(cdiv_up(N, 256), N, batch);
+  homogeneous_eig_kernel<<<g, 256, 0, s>>>(n, alpha, beta, sab, k0sq, eps, W, V, gam, (long)N * N);
+  launch_check("homogeneous_eig");
+}
+
 // ===================================================================== (d) star product
 void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
                   const cd* rdu, long ss) {

@@ -99,4 +99,13 @@ struct GemmDesc {
   cd alpha, beta;
 };
 
+// gamma = sqrt(lambda) with the branch rule of WaveEquationSolver's GammaSqrtOp
+// (src/rcwa/WaveEquationSolver.cpp:8-22): Im >= 0, flipped if |Im| < |Re| and Re < 0.
+__device__ inline cd gamma_branch(cd lam) {
+  cd g = csqrt_principal(lam);
+  if (g.y < 0.0) g = cneg(g);
+  if (fabs(g.y) < fabs(g.x) && g.x < 0.0) g = cneg(g);
+  return g;
+}
+
 }  // namespace vrc

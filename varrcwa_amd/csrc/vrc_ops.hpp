@@ -67,6 +67,10 @@ void wave_eqn_factors(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd*
 void wave_eqn_vectors(vrc_ctx& c, int N, const cd* Q, long sQ, const cd* Z, const cd* X, const cd* dbal,
                       const cd* gam, cd* W, cd* V, int batch);
 
+// Closed-form W (= I), V, gamma of homogeneous periodic media (eps per batch entry).
+void homogeneous_eig(hipStream_t s, int n, int batch, const cd* alpha, const cd* beta, long sab, const double* k0sq,
+                     const cd* eps, cd* W, cd* V, cd* gam);
+
 // ---------------------------------------------------------------- (d)
 struct SMat {
   cd *Tdd = nullptr, *Tuu = nullptr, *Rud = nullptr, *Rdu = nullptr;
