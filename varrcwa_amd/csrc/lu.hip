@@ -4,7 +4,8 @@
 // matrix (pivot = first index of max |re|+|im|, as LAPACK izamax); row swaps and the
 // unit-lower solve for U12 in a column-parallel kernel; trailing update on MFMA ZGEMM.
 // getrs: rows gathered through the final permutation, then blocked forward/backward
-// substitution (64-row diagonal blocks solved in LDS, off-diagonal updates on ZGEMM).
+// substitution (64-row diagonal blocks applied through their inverses, off-diagonal
+// updates on ZGEMM).
 #include "vrc_common.hpp"
 #include "vrc_internal.hpp"
 
@@ -174,82 +175,101 @@ __global__ void gather_rows_kernel(int n, int nrhs, const int* perm, long sPerm,
   Y[b * sY + r + (long)c * ldy] = B[b * sB + perm[b * sPerm + r] + (long)c * ldb];
 }
 
-// Solve the bs x bs diagonal block of a triangular factor against a 64-column tile
-// of the right-hand side held in LDS. UPPER=0: unit lower; UPPER=1: non-unit upper.
+// Inverse of every TRS_NB x TRS_NB diagonal block of a triangular factor (UPPER=0: unit
+// lower, UPPER=1: non-unit upper) into Dinv[b][blk] (TRS_NB x TRS_NB, zero padded past
+// the block size). One workgroup per block: thread (column j, quarter q) substitutes
+// the identity column j, rows split over the four quarters.
 template <int UPPER>
-__global__ __launch_bounds__(256) void trsm_diag_kernel(int k0, int bs, int nrhs, const cd* T, long ldt, long sT,
-                                                        cd* Y, long ldy, long sY) {
+__global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const cd* T, long ldt, long sT, cd* Dinv,
+                                                           long sD) {
   const long b = blockIdx.y;
+  const int blk = blockIdx.x;
+  const int k0 = blk * TRS_NB;
+  const int bs = n - k0 < TRS_NB ? n - k0 : TRS_NB;
   T += b * sT;
-  Y += b * sY;
+  cd* D = Dinv + b * sD + (long)blk * TRS_NB * TRS_NB;
   __shared__ cd Ls[TRS_NB][TRS_NB + 1];
-  __shared__ cd Ys[TRS_NB][TRS_NB + 1];
+  __shared__ cd Xs[TRS_NB][TRS_NB + 1];
   const int t = threadIdx.x;
-  const int c0 = blockIdx.x * TRS_NB;
-  for (int e = t; e < bs * bs; e += 256) {
-    int i = e % bs, j = e / bs;
-    Ls[i][j] = T[(k0 + i) + (long)(k0 + j) * ldt];
-  }
-  for (int e = t; e < bs * TRS_NB; e += 256) {
-    int i = e % bs, j = e / bs;
-    Ys[i][j] = (c0 + j < nrhs) ? Y[(k0 + i) + (long)(c0 + j) * ldy] : mk(0.0, 0.0);
+  for (int e = t; e < TRS_NB * TRS_NB; e += 256) {
+    const int i = e % TRS_NB, j = e / TRS_NB;
+    Ls[i][j] = (i < bs && j < bs) ? T[(k0 + i) + (long)(k0 + j) * ldt] : mk(0.0, 0.0);
+    Xs[i][j] = (i == j && i < bs) ? mk(1.0, 0.0) : mk(0.0, 0.0);
   }
   __syncthreads();
   const int col = t & 63, grp = t >> 6;
   if (!UPPER) {
     for (int i = 0; i < bs; ++i) {
-      const cd xi = Ys[i][col];
-      for (int j = i + 1 + grp; j < bs; j += 4) Ys[j][col] = csub(Ys[j][col], cmul(Ls[j][i], xi));
+      const cd xi = Xs[i][col];
+      for (int j = i + 1 + grp; j < bs; j += 4) Xs[j][col] = csub(Xs[j][col], cmul(Ls[j][i], xi));
       __syncthreads();
     }
   } else {
     for (int i = bs - 1; i >= 0; --i) {
-      if (grp == 0) Ys[i][col] = cdiv(Ys[i][col], Ls[i][i]);
+      if (grp == 0 && col < bs) Xs[i][col] = cdiv(Xs[i][col], Ls[i][i]);
       __syncthreads();
-      const cd xi = Ys[i][col];
-      for (int j = grp; j < i; j += 4) Ys[j][col] = csub(Ys[j][col], cmul(Ls[j][i], xi));
+      const cd xi = Xs[i][col];
+      for (int j = grp; j < i; j += 4) Xs[j][col] = csub(Xs[j][col], cmul(Ls[j][i], xi));
       __syncthreads();
     }
   }
-  for (int e = t; e < bs * TRS_NB; e += 256) {
-    int i = e % bs, j = e / bs;
-    if (c0 + j < nrhs) Y[(k0 + i) + (long)(c0 + j) * ldy] = Ys[i][j];
+  for (int e = t; e < TRS_NB * TRS_NB; e += 256) {
+    const int i = e % TRS_NB, j = e / TRS_NB;
+    D[i + (long)j * TRS_NB] = Xs[i][j];
   }
 }
 
 // In-place triangular solve on Y (n x nrhs): UPPER=0 -> L (unit) ; UPPER=1 -> U (non-unit).
+// Blocked by TRS_NB rows; each diagonal block is applied as a product with its inverse
+// (computed once per call), the off-diagonal blocks as ZGEMM updates.
 static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* LU, long lda, long sA, cd* Y,
-                           long ldy, long sY, int batch) {
+                           long ldy, long sY, int batch, Workspace& ws) {
   const int nblk = cdiv_up(n, TRS_NB);
-  dim3 g(cdiv_up(nrhs, TRS_NB), batch);
+  const long sD = (long)nblk * TRS_NB * TRS_NB;
+  DevBuf Db = ws.get(sizeof(cd) * (size_t)sD * batch);
+  DevBuf Tb = ws.get(sizeof(cd) * (size_t)TRS_NB * nrhs * batch);
+  cd* Dinv = Db.as<cd>();
+  cd* Tmp = Tb.as<cd>();
+  const long sTmp = (long)TRS_NB * nrhs;
+  dim3 g(nblk, batch);
+  if (!upper)
+    trtri_blocks_kernel<0><<<g, 256, 0, s>>>(n, LU, lda, sA, Dinv, sD);
+  else
+    trtri_blocks_kernel<1><<<g, 256, 0, s>>>(n, LU, lda, sA, Dinv, sD);
+  launch_check("trtri_blocks");
+  const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0), zero = mk(0.0, 0.0);
+  auto diag = [&](int kb, int k0, int bs) {
+    // Y[k0:k0+bs] <- inv(T_kk) Y[k0:k0+bs]  (through Tmp)
+    zgemm_strided(s, OP_N, OP_N, bs, nrhs, bs, one, Dinv + (long)kb * TRS_NB * TRS_NB, TRS_NB, sD, Y + k0, ldy, sY,
+                  zero, Tmp, TRS_NB, sTmp, batch);
+    zgeam(s, bs, nrhs, one, Tmp, TRS_NB, sTmp, zero, nullptr, 0, 0, Y + k0, ldy, sY, batch);
+  };
   if (!upper) {
     for (int kb = 0; kb < nblk; ++kb) {
       const int k0 = kb * TRS_NB;
       const int bs = n - k0 < TRS_NB ? n - k0 : TRS_NB;
-      trsm_diag_kernel<0><<<g, 256, 0, s>>>(k0, bs, nrhs, LU, lda, sA, Y, ldy, sY);
-      launch_check("trsm_diag_lower");
+      diag(kb, k0, bs);
       const int rest = n - k0 - bs;
       if (rest > 0)
-        zgemm_strided(s, OP_N, OP_N, rest, nrhs, bs, mk(-1.0, 0.0), LU + (k0 + bs) + (long)k0 * lda, lda, sA,
-                      Y + k0, ldy, sY, mk(1.0, 0.0), Y + (k0 + bs), ldy, sY, batch);
+        zgemm_strided(s, OP_N, OP_N, rest, nrhs, bs, mone, LU + (k0 + bs) + (long)k0 * lda, lda, sA, Y + k0, ldy,
+                      sY, one, Y + (k0 + bs), ldy, sY, batch);
     }
   } else {
     for (int kb = nblk - 1; kb >= 0; --kb) {
       const int k0 = kb * TRS_NB;
       const int bs = n - k0 < TRS_NB ? n - k0 : TRS_NB;
-      trsm_diag_kernel<1><<<g, 256, 0, s>>>(k0, bs, nrhs, LU, lda, sA, Y, ldy, sY);
-      launch_check("trsm_diag_upper");
+      diag(kb, k0, bs);
       if (k0 > 0)
-        zgemm_strided(s, OP_N, OP_N, k0, nrhs, bs, mk(-1.0, 0.0), LU + (long)k0 * lda, lda, sA, Y + k0, ldy, sY,
-                      mk(1.0, 0.0), Y, ldy, sY, batch);
+        zgemm_strided(s, OP_N, OP_N, k0, nrhs, bs, mone, LU + (long)k0 * lda, lda, sA, Y + k0, ldy, sY, one, Y, ldy,
+                      sY, batch);
     }
   }
 }
 
 void ztrsm_upper(hipStream_t s, int n, int nrhs, const cd* U, long ldu, long sU, cd* Y, long ldy, long sY,
-                 int batch) {
+                 int batch, Workspace& ws) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return;
-  trsm_lu_factor(s, 1, n, nrhs, U, ldu, sU, Y, ldy, sY, batch);
+  trsm_lu_factor(s, 1, n, nrhs, U, ldu, sU, Y, ldy, sY, batch, ws);
 }
 
 void zgetrs(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, const int* ipiv, long sP, const cd* B,
@@ -272,8 +292,8 @@ void zgetrs(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, con
   dim3 g(cdiv_up(n, 256), nrhs, batch);
   gather_rows_kernel<<<g, 256, 0, s>>>(n, nrhs, perm, n, B, ldb, sB, Y, ldy, sY);
   launch_check("gather_rows");
-  trsm_lu_factor(s, 0, n, nrhs, LU, lda, sA, Y, ldy, sY, batch);
-  trsm_lu_factor(s, 1, n, nrhs, LU, lda, sA, Y, ldy, sY, batch);
+  trsm_lu_factor(s, 0, n, nrhs, LU, lda, sA, Y, ldy, sY, batch, ws);
+  trsm_lu_factor(s, 1, n, nrhs, LU, lda, sA, Y, ldy, sY, batch, ws);
   if (Y != X) zgeam(s, n, nrhs, mk(1.0, 0.0), Y, ldy, sY, mk(0.0, 0.0), nullptr, 0, 0, X, ldx, sX, batch);
 }
 

@@ -324,7 +324,7 @@ void layer_smatrix_factors(vrc_ctx& c, int N, int batch, const double* d_s, cons
   rowscale_inv(s, N, 2 * N, dbal, N, Y, N, s2, R2, N, s2, batch);
   // Y = Z^H R2 ; Y <- X^{-1} Y
   zgemm_strided(s, OP_C, OP_N, N, 2 * N, N, ONE, Z, N, NN, R2, N, s2, ZERO, Y, N, s2, batch);
-  ztrsm_upper(s, N, 2 * N, X, N, NN, Y, N, s2, batch);
+  ztrsm_upper(s, N, 2 * N, X, N, NN, Y, N, s2, batch, c.ws);
   // F0 = Y[:, :N] ; G0 = Gamma^{-1} Y[:, N:]
   cd* F0 = Y;
   cd* G0 = R2;  // reuse

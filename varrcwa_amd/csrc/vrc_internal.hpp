@@ -75,7 +75,7 @@ void zgetrs(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, con
             long ldb, long sB, cd* X, long ldx, long sX, int batch, Workspace& ws);
 // Y <- U^{-1} Y for upper-triangular U (non-unit diagonal), in place.
 void ztrsm_upper(hipStream_t s, int n, int nrhs, const cd* U, long ldu, long sU, cd* Y, long ldy, long sY,
-                 int batch);
+                 int batch, Workspace& ws);
 void zgetri(hipStream_t s, int n, const cd* LU, long lda, long sA, const int* ipiv, long sP, cd* Ainv, long ldi,
             long sI, int batch, Workspace& ws);
 
