@@ -310,12 +310,14 @@ __global__ __launch_bounds__(GT_THREADS) void zgemm_desc_kernel(const GemmDesc* 
 }
 
 static bool g_big_gemm = true;  // large-tile kernel for m >= 128, n >= 64 (switchable for comparison)
+static int g_big_min_k = 65;    // ... and k at least this (short-k updates run better on the 64x64 tile)
 void zgemm_set_big(bool on) { g_big_gemm = on; }
+void zgemm_set_big_min_k(int k) { g_big_min_k = k; }
 
 void zgemm_strided(hipStream_t s, int opA, int opB, int m, int n, int k, cd alpha, const cd* A, long lda,
                    long sA, const cd* B, long ldb, long sB, cd beta, cd* C, long ldc, long sC, int batch) {
   if (m <= 0 || n <= 0 || batch <= 0) return;
-  if (g_big_gemm && k > 0 && m >= GB_M && n >= GB_N) {
+  if (g_big_gemm && k > 0 && k >= g_big_min_k && m >= GB_M && n >= GB_N) {
     const int gm = cdiv_up(m, GB_M), gn = cdiv_up(n, GB_N);
     const long total = (long)gm * gn * batch;
     dim3 grid((unsigned)total);

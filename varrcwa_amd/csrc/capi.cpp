@@ -63,6 +63,10 @@ int vrc_tune_set(const char* key, long long value) {
     zgemm_set_big(value != 0);
     return 0;
   }
+  if (std::strcmp(key, "zgemm.large_tile_min_k") == 0) {
+    zgemm_set_big_min_k((int)value);
+    return 0;
+  }
   if (std::strcmp(key, "zgeev.aed_window") == 0) {
     zgeev_set_aed((int)value);
     return 0;

@@ -333,16 +333,29 @@ __device__ __forceinline__ void mm64_for_each(const d4 (&accR)[2][2], const d4 (
 }
 
 // M[r0:r0+w, c0:c1] <- U^H * M[...]  (U: w x w in LDS, zero padded), chunk buffer C.
-// The next chunk is prefetched into registers while the current one is multiplied.
+// The next 64-column chunk is fetched into registers while the current one is multiplied.
 __device__ void apply_left_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, int r0, int c0, int c1) {
   const int t = threadIdx.x;
+  const int k = t & 63, j0 = t >> 6;  // element (k, j0 + 4q) of a chunk, q < 16
+  cd pre[16];
+  if (c0 < c1) {
+    const int cw = c1 - c0 < QW ? c1 - c0 : QW;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      pre[q] = (k < w && j0 + 4 * q < cw) ? M[(r0 + k) + (long)(c0 + j0 + 4 * q) * ld] : mk(0.0, 0.0);
+  }
   for (int cc = c0; cc < c1; cc += QW) {
     const int cw = c1 - cc < QW ? c1 - cc : QW;
-    for (int e = t; e < QW * QW; e += 256) {
-      int k = e & 63, j = e >> 6;
-      C[k][j] = (k < w && j < cw) ? M[(r0 + k) + (long)(cc + j) * ld] : mk(0.0, 0.0);
-    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) C[k][j0 + 4 * q] = pre[q];
     __syncthreads();
+    const int cn = cc + QW;
+    if (cn < c1) {
+      const int cwn = c1 - cn < QW ? c1 - cn : QW;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        pre[q] = (k < w && j0 + 4 * q < cwn) ? M[(r0 + k) + (long)(cn + j0 + 4 * q) * ld] : mk(0.0, 0.0);
+    }
     d4 aR[2][2], aI[2][2];
     mm64<true>(U, C, aR, aI);
     mm64_for_each(aR, aI, [&](int i, int j, cd v) {
@@ -351,19 +364,33 @@ __device__ void apply_left_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, 
     __syncthreads();
   }
 }
+// M[r0:r1, c0:c0+w] <- M[...] * U, 64-row chunks, next chunk prefetched into registers.
 __device__ void apply_right_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, int r0, int r1, int c0) {
   const int t = threadIdx.x;
+  const int i = t & 63, k0 = t >> 6;  // element (i, k0 + 4q) of a chunk, q < 16
+  cd pre[16];
+  if (r0 < r1) {
+    const int rh = r1 - r0 < QW ? r1 - r0 : QW;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      pre[q] = (i < rh && k0 + 4 * q < w) ? M[(r0 + i) + (long)(c0 + k0 + 4 * q) * ld] : mk(0.0, 0.0);
+  }
   for (int rr = r0; rr < r1; rr += QW) {
     const int rh = r1 - rr < QW ? r1 - rr : QW;
-    for (int e = t; e < QW * QW; e += 256) {
-      int i = e & 63, k = e >> 6;
-      C[i][k] = (i < rh && k < w) ? M[(rr + i) + (long)(c0 + k) * ld] : mk(0.0, 0.0);
-    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) C[i][k0 + 4 * q] = pre[q];
     __syncthreads();
+    const int rn = rr + QW;
+    if (rn < r1) {
+      const int rhn = r1 - rn < QW ? r1 - rn : QW;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        pre[q] = (i < rhn && k0 + 4 * q < w) ? M[(rn + i) + (long)(c0 + k0 + 4 * q) * ld] : mk(0.0, 0.0);
+    }
     d4 aR[2][2], aI[2][2];
     mm64<false>(C, U, aR, aI);
-    mm64_for_each(aR, aI, [&](int i, int j, cd v) {
-      if (i < rh && j < w) M[(rr + i) + (long)(c0 + j) * ld] = v;
+    mm64_for_each(aR, aI, [&](int ii, int j, cd v) {
+      if (ii < rh && j < w) M[(rr + ii) + (long)(c0 + j) * ld] = v;
     });
     __syncthreads();
   }
