@@ -16,10 +16,10 @@ template <int OPA, int OPB>
 __device__ __forceinline__ void zgemm_tile(int m, int n, int k, cd alpha, const cd* __restrict__ A, long lda,
                                            const cd* __restrict__ B, long ldb, cd beta, cd* __restrict__ C,
                                            long ldc, int tm, int tn) {
-  __shared__ double As_re[GT_K][GT_M];
-  __shared__ double As_im[GT_K][GT_M];
-  __shared__ double Bs_re[GT_K][GT_N];
-  __shared__ double Bs_im[GT_K][GT_N];
+  __shared__ double As_re[GT_K][GT_M + 1];
+  __shared__ double As_im[GT_K][GT_M + 1];
+  __shared__ double Bs_re[GT_K][GT_N + 1];
+  __shared__ double Bs_im[GT_K][GT_N + 1];
 
   const int t = threadIdx.x;
   const int lane = t & 63;
@@ -36,48 +36,58 @@ __device__ __forceinline__ void zgemm_tile(int m, int n, int k, cd alpha, const 
       accI[i][j] = d4{0.0, 0.0, 0.0, 0.0};
     }
 
-  for (int kk = 0; kk < k; kk += GT_K) {
-    // ---- stage op(A) tile (GT_M x GT_K) into LDS as [k][m]
+  // op(A) tile (GT_M x GT_K) and op(B) tile (GT_K x GT_N), 4 elements each per thread;
+  // the next K chunk is fetched into registers while the current one is multiplied
+  cd pa[4], pb[4];
+  auto fetch = [&](int kk) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      int r, c;
       cd v = mk(0.0, 0.0);
       if (OPA == OP_N) {
-        r = t & 63;
-        c = (t >> 6) + 4 * i;
-        int gr = row0 + r, gc = kk + c;
+        const int gr = row0 + (t & 63), gc = kk + (t >> 6) + 4 * i;
         if (gr < m && gc < k) v = A[gr + (long)gc * lda];
       } else {
-        c = t & 15;
-        r = (t >> 4) + 16 * i;
-        int gr = row0 + r, gc = kk + c;
+        const int gr = row0 + (t >> 4) + 16 * i, gc = kk + (t & 15);
         if (gr < m && gc < k) v = A[gc + (long)gr * lda];
         if (OPA == OP_C) v.y = -v.y;
       }
-      As_re[c][r] = v.x;
-      As_im[c][r] = v.y;
+      pa[i] = v;
     }
-    // ---- stage op(B) tile (GT_K x GT_N) into LDS as [k][n]
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      int c, j;
       cd v = mk(0.0, 0.0);
       if (OPB == OP_N) {
-        c = t & 15;
-        j = (t >> 4) + 16 * i;
-        int gc = kk + c, gj = col0 + j;
+        const int gc = kk + (t & 15), gj = col0 + (t >> 4) + 16 * i;
         if (gc < k && gj < n) v = B[gc + (long)gj * ldb];
       } else {
-        j = t & 63;
-        c = (t >> 6) + 4 * i;
-        int gc = kk + c, gj = col0 + j;
+        const int gc = kk + (t >> 6) + 4 * i, gj = col0 + (t & 63);
         if (gc < k && gj < n) v = B[gj + (long)gc * ldb];
         if (OPB == OP_C) v.y = -v.y;
       }
-      Bs_re[c][j] = v.x;
-      Bs_im[c][j] = v.y;
+      pb[i] = v;
+    }
+  };
+  fetch(0);
+  for (int kk = 0; kk < k; kk += GT_K) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (OPA == OP_N) {
+        As_re[(t >> 6) + 4 * i][t & 63] = pa[i].x;
+        As_im[(t >> 6) + 4 * i][t & 63] = pa[i].y;
+      } else {
+        As_re[t & 15][(t >> 4) + 16 * i] = pa[i].x;
+        As_im[t & 15][(t >> 4) + 16 * i] = pa[i].y;
+      }
+      if (OPB == OP_N) {
+        Bs_re[t & 15][(t >> 4) + 16 * i] = pb[i].x;
+        Bs_im[t & 15][(t >> 4) + 16 * i] = pb[i].y;
+      } else {
+        Bs_re[(t >> 6) + 4 * i][t & 63] = pb[i].x;
+        Bs_im[(t >> 6) + 4 * i][t & 63] = pb[i].y;
+      }
     }
     __syncthreads();
+    if (kk + GT_K < k) fetch(kk + GT_K);
 #pragma unroll
     for (int s = 0; s < GT_K / 4; ++s) {
       const int kr = s * 4 + (lane >> 4);
