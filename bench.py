@@ -124,10 +124,12 @@ def adjoint_bench(ctx, nh: int, insts, log):
     fwd_bwd()  # warm
     eps.grad = None
     torch.cuda.synchronize()
+    read_stages(ctx)  # reset
     t0 = time.perf_counter()
     t1, t2 = fwd_bwd()
     torch.cuda.synchronize()
     t3 = time.perf_counter()
+    st = read_stages(ctx)
     api.batch_stack_rcwa(ctx, nh, nh, plain)
     torch.cuda.synchronize()
     t4 = time.perf_counter()
@@ -136,7 +138,8 @@ def adjoint_bench(ctx, nh: int, insts, log):
         f"reverse {bwd * 1e3:.0f} ms")
     return {"instances": len(insts), "objective": "sum of 0th-order transmitted power (TE+TM) per instance",
             "gradient_wrt": f"{prob.cells} eps cells", "forward_ms": fwd * 1e3, "forward_with_tape_ms": fwd_tape * 1e3,
-            "reverse_ms": bwd * 1e3, "reverse_over_forward": bwd / fwd, "gradient_over_forward": (fwd_tape + bwd) / fwd}
+            "reverse_ms": bwd * 1e3, "reverse_over_forward": bwd / fwd, "gradient_over_forward": (fwd_tape + bwd) / fwd,
+            "stage_ms_forward_and_reverse": {k: v[0] for k, v in st.items()}}
 
 
 def main():

@@ -468,155 +468,190 @@ void wave_eqn_vjp(vrc_ctx& c, int N, int batch, const cd* P, const cd* Q, long s
 }
 
 // ===================================================================== (d) slab S
-void layer_smatrix_vjp(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
-                       const cd* W0, const cd* V0, long sRef, const cd* tb, const cd* rb, cd* Wb, cd* Vb, cd* gb,
-                       cd* Wi_out) {
+// Forward of RCWAScatterMatrix::compute with explicit inverses, keeping what the
+// reverse pass needs.
+void layer_smatrix_fwd_saved(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
+                             const cd* W0, const cd* V0, long sRef, SlabSaved& sv) {
+  StageScope st(ST_ADJOINT);
+  hipStream_t s = c.stream;
+  Ops o(s, N, batch);
+  const long NN = o.NN;
+  const size_t M = sizeof(cd) * (size_t)NN * batch;
+  for (DevBuf* b : {&sv.Wi, &sv.Vi, &sv.F0, &sv.G0, &sv.A, &sv.B, &sv.Ai, &sv.H, &sv.Gm, &sv.F, &sv.Mi, &sv.Rud,
+                    &sv.Tuu})
+    *b = c.ws.get(M);
+  sv.x = c.ws.get(sizeof(cd) * (size_t)N * batch);
+  Buf XA(c, (size_t)NN * batch), XB(c, (size_t)NN * batch), t1(c, (size_t)NN * batch), t2(c, (size_t)NN * batch);
+  cd *Wi = sv.Wi.as<cd>(), *Vi = sv.Vi.as<cd>(), *F0 = sv.F0.as<cd>(), *G0 = sv.G0.as<cd>(), *A = sv.A.as<cd>(),
+     *B = sv.B.as<cd>(), *Ai = sv.Ai.as<cd>(), *H = sv.H.as<cd>(), *Gm = sv.Gm.as<cd>(), *F = sv.F.as<cd>(),
+     *Mi = sv.Mi.as<cd>(), *x = sv.x.as<cd>();
+  invert(c, o, W, NN, Wi);
+  invert(c, o, V, NN, Vi);
+  o.mm(OP_N, OP_N, ONE, Wi, NN, W0, sRef, ZERO, F0, NN);
+  o.mm(OP_N, OP_N, ONE, Vi, NN, V0, sRef, ZERO, G0, NN);
+  o.add(ONE, F0, NN, ONE, G0, NN, A, NN);
+  o.add(ONE, F0, NN, MONE, G0, NN, B, NN);
+  invert(c, o, A, NN, Ai);
+  {
+    dim3 g(cdiv_up(N, 256), batch);
+    phase_fwd_kernel<<<g, 256, 0, s>>>(N, gam, d_s, x, N);
+    launch_check("phase_fwd");
+  }
+  zdscale(s, N, N, x, N, nullptr, 0, 0, A, N, NN, XA.p, N, NN, batch);
+  zdscale(s, N, N, x, N, nullptr, 0, 0, B, N, NN, XB.p, N, NN, batch);
+  o.mm(OP_N, OP_N, ONE, Ai, NN, B, NN, ZERO, H, NN);
+  o.mm(OP_N, OP_N, ONE, Ai, NN, XA.p, NN, ZERO, Gm, NN);
+  o.mm(OP_N, OP_N, ONE, Ai, NN, XB.p, NN, ZERO, F, NN);
+  o.eye(t1.p, NN);
+  o.mm(OP_N, OP_N, MONE, F, NN, F, NN, ONE, t1.p, NN);  // M = I - F F
+  invert(c, o, t1.p, NN, Mi);
+  o.copy(H, NN, t1.p, NN, MONE);
+  o.mm(OP_N, OP_N, ONE, F, NN, Gm, NN, ONE, t1.p, NN);  // R1 = F Gm - H
+  o.copy(Gm, NN, t2.p, NN);
+  o.mm(OP_N, OP_N, MONE, F, NN, H, NN, ONE, t2.p, NN);  // T1 = Gm - F H
+  o.mm(OP_N, OP_N, ONE, Mi, NN, t1.p, NN, ZERO, sv.Rud.as<cd>(), NN);
+  o.mm(OP_N, OP_N, ONE, Mi, NN, t2.p, NN, ZERO, sv.Tuu.as<cd>(), NN);
+}
+
+// Reverse of layer_smatrix_fwd_saved (consumes sv: several buffers are reused).
+void layer_smatrix_bwd_saved(vrc_ctx& c, int N, int batch, const double* d_s, SlabSaved& sv, const cd* tb,
+                             const cd* rb, cd* Wb, cd* Vb, cd* gb) {
   StageScope st(ST_ADJOINT);
   hipStream_t s = c.stream;
   Ops o(s, N, batch);
   const long NN = o.NN;
   const size_t M = (size_t)NN * batch;
-  Buf Wi(c, Wi_out ? 1 : M), Vi(c, M), F0(c, M), G0(c, M), A(c, M), B(c, M), Ai(c, M), XA(c, M), XB(c, M);
-  Buf H(c, M), Gm(c, M), F(c, M), Mi(c, M), R(c, M), Tt(c, M), t1(c, M), t2(c, M), t3(c, M);
-  Buf x(c, (size_t)N * batch), xb(c, (size_t)N * batch), cx(c, (size_t)N * batch);
-  cd* wi = Wi_out ? Wi_out : Wi.p;
-  // ---- forward recomputation (RCWAScatterMatrix::compute arithmetic with explicit inverses)
-  invert(c, o, W, NN, wi);
-  invert(c, o, V, NN, Vi.p);
-  o.mm(OP_N, OP_N, ONE, wi, NN, W0, sRef, ZERO, F0.p, NN);
-  o.mm(OP_N, OP_N, ONE, Vi.p, NN, V0, sRef, ZERO, G0.p, NN);
-  o.add(ONE, F0.p, NN, ONE, G0.p, NN, A.p, NN);
-  o.add(ONE, F0.p, NN, MONE, G0.p, NN, B.p, NN);
-  invert(c, o, A.p, NN, Ai.p);
-  {
-    dim3 g(cdiv_up(N, 256), batch);
-    phase_fwd_kernel<<<g, 256, 0, s>>>(N, gam, d_s, x.p, N);
-    launch_check("phase_fwd");
-  }
-  zdscale(s, N, N, x.p, N, nullptr, 0, 0, A.p, N, NN, XA.p, N, NN, batch);
-  zdscale(s, N, N, x.p, N, nullptr, 0, 0, B.p, N, NN, XB.p, N, NN, batch);
-  o.mm(OP_N, OP_N, ONE, Ai.p, NN, B.p, NN, ZERO, H.p, NN);
-  o.mm(OP_N, OP_N, ONE, Ai.p, NN, XA.p, NN, ZERO, Gm.p, NN);
-  o.mm(OP_N, OP_N, ONE, Ai.p, NN, XB.p, NN, ZERO, F.p, NN);
-  o.eye(t1.p, NN);
-  o.mm(OP_N, OP_N, MONE, F.p, NN, F.p, NN, ONE, t1.p, NN);  // M = I - F F
-  invert(c, o, t1.p, NN, Mi.p);
-  o.copy(H.p, NN, t1.p, NN, MONE);
-  o.mm(OP_N, OP_N, ONE, F.p, NN, Gm.p, NN, ONE, t1.p, NN);  // R1 = F Gm - H
-  o.copy(Gm.p, NN, t2.p, NN);
-  o.mm(OP_N, OP_N, MONE, F.p, NN, H.p, NN, ONE, t2.p, NN);  // T1 = Gm - F H
-  o.mm(OP_N, OP_N, ONE, Mi.p, NN, t1.p, NN, ZERO, R.p, NN);   // Rud
-  o.mm(OP_N, OP_N, ONE, Mi.p, NN, t2.p, NN, ZERO, Tt.p, NN);  // Tuu
-  // ---- backward
+  Buf t1(c, M), t2(c, M), t3(c, M), XA(c, M), XB(c, M);
+  Buf xb(c, (size_t)N * batch), cx(c, (size_t)N * batch);
+  cd *Wi = sv.Wi.as<cd>(), *Vi = sv.Vi.as<cd>(), *F0 = sv.F0.as<cd>(), *G0 = sv.G0.as<cd>(), *A = sv.A.as<cd>(),
+     *B = sv.B.as<cd>(), *Ai = sv.Ai.as<cd>(), *H = sv.H.as<cd>(), *Gm = sv.Gm.as<cd>(), *F = sv.F.as<cd>(),
+     *Mi = sv.Mi.as<cd>(), *x = sv.x.as<cd>(), *R = sv.Rud.as<cd>(), *Tt = sv.Tuu.as<cd>();
   cd* R1b = t1.p;
   cd* T1b = t2.p;
-  o.mm(OP_C, OP_N, ONE, Mi.p, NN, rb, NN, ZERO, R1b, NN);
-  o.mm(OP_C, OP_N, ONE, Mi.p, NN, tb, NN, ZERO, T1b, NN);
-  cd* Mb = Mi.p;  // M^{-1} no longer needed
-  o.mm(OP_N, OP_C, MONE, R1b, NN, R.p, NN, ZERO, Mb, NN);
-  o.mm(OP_N, OP_C, MONE, T1b, NN, Tt.p, NN, ONE, Mb, NN);
-  cd* Fb = R.p;    // Rud no longer needed
-  cd* Gmb = Tt.p;  // Tuu no longer needed
+  o.mm(OP_C, OP_N, ONE, Mi, NN, rb, NN, ZERO, R1b, NN);
+  o.mm(OP_C, OP_N, ONE, Mi, NN, tb, NN, ZERO, T1b, NN);
+  cd* Mb = Mi;  // M^{-1} no longer needed
+  o.mm(OP_N, OP_C, MONE, R1b, NN, R, NN, ZERO, Mb, NN);
+  o.mm(OP_N, OP_C, MONE, T1b, NN, Tt, NN, ONE, Mb, NN);
+  cd* Fb = R;    // Rud no longer needed
+  cd* Gmb = Tt;  // Tuu no longer needed
   cd* Hb = t3.p;
   // Fb = -Mb F^H - F^H Mb + R1b Gm^H - T1b H^H
-  o.mm(OP_N, OP_C, MONE, Mb, NN, F.p, NN, ZERO, Fb, NN);
-  o.mm(OP_C, OP_N, MONE, F.p, NN, Mb, NN, ONE, Fb, NN);
-  o.mm(OP_N, OP_C, ONE, R1b, NN, Gm.p, NN, ONE, Fb, NN);
-  o.mm(OP_N, OP_C, MONE, T1b, NN, H.p, NN, ONE, Fb, NN);
+  o.mm(OP_N, OP_C, MONE, Mb, NN, F, NN, ZERO, Fb, NN);
+  o.mm(OP_C, OP_N, MONE, F, NN, Mb, NN, ONE, Fb, NN);
+  o.mm(OP_N, OP_C, ONE, R1b, NN, Gm, NN, ONE, Fb, NN);
+  o.mm(OP_N, OP_C, MONE, T1b, NN, H, NN, ONE, Fb, NN);
   // Gmb = F^H R1b + T1b ; Hb = -R1b - F^H T1b
   o.copy(T1b, NN, Gmb, NN);
-  o.mm(OP_C, OP_N, ONE, F.p, NN, R1b, NN, ONE, Gmb, NN);
+  o.mm(OP_C, OP_N, ONE, F, NN, R1b, NN, ONE, Gmb, NN);
   o.copy(R1b, NN, Hb, NN, MONE);
-  o.mm(OP_C, OP_N, MONE, F.p, NN, T1b, NN, ONE, Hb, NN);
+  o.mm(OP_C, OP_N, MONE, F, NN, T1b, NN, ONE, Hb, NN);
   // [B | XA | XB]b = A^{-H} [Hb | Gmb | Fb]; Ab = -(Bb H^H + XAb Gm^H + XBb F^H)
   cd* Bb = t1.p;   // R1b consumed
   cd* XAb = t2.p;  // T1b consumed
   cd* XBb = Mb;    // Mb consumed
-  o.mm(OP_C, OP_N, ONE, Ai.p, NN, Hb, NN, ZERO, Bb, NN);
-  o.mm(OP_C, OP_N, ONE, Ai.p, NN, Gmb, NN, ZERO, XAb, NN);
-  o.mm(OP_C, OP_N, ONE, Ai.p, NN, Fb, NN, ZERO, XBb, NN);
+  o.mm(OP_C, OP_N, ONE, Ai, NN, Hb, NN, ZERO, Bb, NN);
+  o.mm(OP_C, OP_N, ONE, Ai, NN, Gmb, NN, ZERO, XAb, NN);
+  o.mm(OP_C, OP_N, ONE, Ai, NN, Fb, NN, ZERO, XBb, NN);
   cd* Ab = Hb;
-  o.mm(OP_N, OP_C, MONE, Bb, NN, H.p, NN, ZERO, Ab, NN);
-  o.mm(OP_N, OP_C, MONE, XAb, NN, Gm.p, NN, ONE, Ab, NN);
-  o.mm(OP_N, OP_C, MONE, XBb, NN, F.p, NN, ONE, Ab, NN);
+  o.mm(OP_N, OP_C, MONE, Bb, NN, H, NN, ZERO, Ab, NN);
+  o.mm(OP_N, OP_C, MONE, XAb, NN, Gm, NN, ONE, Ab, NN);
+  o.mm(OP_N, OP_C, MONE, XBb, NN, F, NN, ONE, Ab, NN);
   // phases: xb_i = sum_k conj(A_ik) XAb_ik + conj(B_ik) XBb_ik ; Ab += conj(X) XAb ; Bb += conj(X) XBb
   {
     dim3 g(cdiv_up(N, 256), batch);
-    rowdot_kernel<<<g, 256, 0, s>>>(N, A.p, XAb, B.p, XBb, NN, xb.p, N);
+    rowdot_kernel<<<g, 256, 0, s>>>(N, A, XAb, B, XBb, NN, xb.p, N);
     launch_check("rowdot");
-    phase_bwd_kernel<<<g, 256, 0, s>>>(N, x.p, xb.p, d_s, gb, N);
+    phase_bwd_kernel<<<g, 256, 0, s>>>(N, x, xb.p, d_s, gb, N);
     launch_check("phase_bwd");
   }
-  o.vec(x.p, N, cx.p, N, VEC_CONJ);
+  o.vec(x, N, cx.p, N, VEC_CONJ);
   zdscale(s, N, N, cx.p, N, nullptr, 0, 0, XAb, N, NN, XAb, N, NN, batch);
   zdscale(s, N, N, cx.p, N, nullptr, 0, 0, XBb, N, NN, XBb, N, NN, batch);
   o.add(ONE, Ab, NN, ONE, XAb, NN, Ab, NN);
   o.add(ONE, Bb, NN, ONE, XBb, NN, Bb, NN);
   // F0b = Ab + Bb ; G0b = Ab - Bb ; Wb = -W^{-H} F0b F0^H ; Vb = -V^{-H} G0b G0^H
-  cd* F0b = Gm.p;
-  cd* G0b = H.p;
+  cd* F0b = Gm;
+  cd* G0b = H;
   o.add(ONE, Ab, NN, ONE, Bb, NN, F0b, NN);
   o.add(ONE, Ab, NN, MONE, Bb, NN, G0b, NN);
-  o.mm(OP_N, OP_C, ONE, F0b, NN, F0.p, NN, ZERO, XA.p, NN);
-  o.mm(OP_C, OP_N, MONE, wi, NN, XA.p, NN, ZERO, Wb, NN);
-  o.mm(OP_N, OP_C, ONE, G0b, NN, G0.p, NN, ZERO, XB.p, NN);
-  o.mm(OP_C, OP_N, MONE, Vi.p, NN, XB.p, NN, ZERO, Vb, NN);
+  o.mm(OP_N, OP_C, ONE, F0b, NN, F0, NN, ZERO, XA.p, NN);
+  o.mm(OP_C, OP_N, MONE, Wi, NN, XA.p, NN, ZERO, Wb, NN);
+  o.mm(OP_N, OP_C, ONE, G0b, NN, G0, NN, ZERO, XB.p, NN);
+  o.mm(OP_C, OP_N, MONE, Vi, NN, XB.p, NN, ZERO, Vb, NN);
+}
+
+void layer_smatrix_vjp(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
+                       const cd* W0, const cd* V0, long sRef, const cd* tb, const cd* rb, cd* Wb, cd* Vb, cd* gb,
+                       cd* Wi_out) {
+  SlabSaved sv;
+  layer_smatrix_fwd_saved(c, N, batch, d_s, W, V, gam, W0, V0, sRef, sv);
+  if (Wi_out)
+    zgeam(c.stream, N, N, ONE, sv.Wi.as<cd>(), N, (long)N * N, ZERO, nullptr, 0, 0, Wi_out, N, (long)N * N, batch);
+  layer_smatrix_bwd_saved(c, N, batch, d_s, sv, tb, rb, Wb, Vb, gb);
 }
 
 // ===================================================================== (d) star product
 void star_product_vjp(vrc_ctx& c, int N, int batch, const SMat& S0, const cd* tdd, const cd* tuu, const cd* rud,
-                      const cd* rdu, long ss, SMat& Sb, cd* tddb, cd* tuub, cd* rudb, cd* rdub, long sb) {
+                      const cd* rdu, long ss, SMat& Sb, cd* tddb, cd* tuub, cd* rudb, cd* rdub, long sb,
+                      const cd* saved) {
   StageScope st(ST_ADJOINT);
   hipStream_t s = c.stream;
   Ops o(s, N, batch);
   const long NN = o.NN;
   const size_t M = (size_t)NN * batch;
-  Buf C1(c, M), RC1(c, M), C2(c, M), U1(c, M), U2(c, M), Wk(c, M), C1b(c, M), C2b(c, M), RC1b(c, M);
+  Buf C1o(c, saved ? 1 : M), RC1o(c, saved ? 1 : M), C2o(c, saved ? 1 : M);
+  Buf U1(c, M), U2(c, M), Wk(c, M), C1b(c, M), C2b(c, M), RC1b(c, M);
   Buf Tddn(c, M), Tuun(c, M), Rudn(c, M);
-  // recompute C1 = (I - rdu Rud)^{-1}, RC1 = Rud C1, C2 = I + RC1 rdu
-  o.eye(Wk.p, NN);
-  o.mm(OP_N, OP_N, MONE, rdu, ss, S0.Rud, NN, ONE, Wk.p, NN);
-  invert(c, o, Wk.p, NN, C1.p);
-  o.mm(OP_N, OP_N, ONE, S0.Rud, NN, C1.p, NN, ZERO, RC1.p, NN);
-  o.eye(C2.p, NN);
-  o.mm(OP_N, OP_N, ONE, RC1.p, NN, rdu, ss, ONE, C2.p, NN);
+  const cd* C1 = saved ? saved : C1o.p;
+  const cd* RC1 = saved ? saved + M : RC1o.p;
+  const cd* C2 = saved ? saved + 2 * M : C2o.p;
+  if (!saved) {
+    // recompute C1 = (I - rdu Rud)^{-1}, RC1 = Rud C1, C2 = I + RC1 rdu
+    o.eye(Wk.p, NN);
+    o.mm(OP_N, OP_N, MONE, rdu, ss, S0.Rud, NN, ONE, Wk.p, NN);
+    invert(c, o, Wk.p, NN, C1o.p);
+    o.mm(OP_N, OP_N, ONE, S0.Rud, NN, C1o.p, NN, ZERO, RC1o.p, NN);
+    o.eye(C2o.p, NN);
+    o.mm(OP_N, OP_N, ONE, RC1o.p, NN, rdu, ss, ONE, C2o.p, NN);
+  }
   // Rdu' = Rdu + (Tdd rdu) C2 Tuu
   o.mm(OP_N, OP_N, ONE, S0.Tdd, NN, rdu, ss, ZERO, U1.p, NN);  // U1 = Tdd rdu
-  o.mm(OP_N, OP_N, ONE, U1.p, NN, C2.p, NN, ZERO, U2.p, NN);    // U2 = U1 C2
+  o.mm(OP_N, OP_N, ONE, U1.p, NN, C2, NN, ZERO, U2.p, NN);    // U2 = U1 C2
   o.mm(OP_C, OP_N, ONE, U2.p, NN, Sb.Rdu, NN, ZERO, Tuun.p, NN);      // Tuub = U2^H Rdu'b
   o.mm(OP_N, OP_C, ONE, Sb.Rdu, NN, S0.Tuu, NN, ZERO, Wk.p, NN);      // U2b = Rdu'b Tuu^H
   o.mm(OP_C, OP_N, ONE, U1.p, NN, Wk.p, NN, ZERO, C2b.p, NN);         // C2b = U1^H U2b
-  o.mm(OP_N, OP_C, ONE, Wk.p, NN, C2.p, NN, ZERO, U2.p, NN);          // U1b = U2b C2^H (into U2)
+  o.mm(OP_N, OP_C, ONE, Wk.p, NN, C2, NN, ZERO, U2.p, NN);          // U1b = U2b C2^H (into U2)
   o.mm(OP_N, OP_C, ONE, U2.p, NN, rdu, ss, ZERO, Tddn.p, NN);         // Tddb = U1b rdu^H
   o.mm(OP_C, OP_N, ONE, S0.Tdd, NN, U2.p, NN, ZERO, rdub, sb);        // rdub = Tdd^H U1b
   // Tdd' = (Tdd C1) tdd
-  o.mm(OP_N, OP_N, ONE, S0.Tdd, NN, C1.p, NN, ZERO, U1.p, NN);        // U3 = Tdd C1
+  o.mm(OP_N, OP_N, ONE, S0.Tdd, NN, C1, NN, ZERO, U1.p, NN);        // U3 = Tdd C1
   o.mm(OP_C, OP_N, ONE, U1.p, NN, Sb.Tdd, NN, ZERO, tddb, sb);        // tddb = U3^H Tdd'b
   o.mm(OP_N, OP_C, ONE, Sb.Tdd, NN, tdd, ss, ZERO, Wk.p, NN);         // U3b = Tdd'b tdd^H
-  o.mm(OP_N, OP_C, ONE, Wk.p, NN, C1.p, NN, ONE, Tddn.p, NN);         // Tddb += U3b C1^H
+  o.mm(OP_N, OP_C, ONE, Wk.p, NN, C1, NN, ONE, Tddn.p, NN);         // Tddb += U3b C1^H
   o.mm(OP_C, OP_N, ONE, S0.Tdd, NN, Wk.p, NN, ZERO, C1b.p, NN);       // C1b = Tdd^H U3b
   // Rud' = rud + (tuu RC1) tdd
   o.copy(Sb.Rud, NN, rudb, sb);                                        // rudb = Rud'b
-  o.mm(OP_N, OP_N, ONE, tuu, ss, RC1.p, NN, ZERO, U1.p, NN);          // U5 = tuu RC1
+  o.mm(OP_N, OP_N, ONE, tuu, ss, RC1, NN, ZERO, U1.p, NN);          // U5 = tuu RC1
   o.mm(OP_C, OP_N, ONE, U1.p, NN, Sb.Rud, NN, ONE, tddb, sb);         // tddb += U5^H Rud'b
   o.mm(OP_N, OP_C, ONE, Sb.Rud, NN, tdd, ss, ZERO, Wk.p, NN);         // U5b = Rud'b tdd^H
   o.mm(OP_C, OP_N, ONE, tuu, ss, Wk.p, NN, ZERO, RC1b.p, NN);         // RC1b = tuu^H U5b
-  o.mm(OP_N, OP_C, ONE, Wk.p, NN, RC1.p, NN, ZERO, tuub, sb);         // tuub = U5b RC1^H
+  o.mm(OP_N, OP_C, ONE, Wk.p, NN, RC1, NN, ZERO, tuub, sb);         // tuub = U5b RC1^H
   // Tuu' = (tuu C2) Tuu
-  o.mm(OP_N, OP_N, ONE, tuu, ss, C2.p, NN, ZERO, U1.p, NN);           // U6 = tuu C2
+  o.mm(OP_N, OP_N, ONE, tuu, ss, C2, NN, ZERO, U1.p, NN);           // U6 = tuu C2
   o.mm(OP_C, OP_N, ONE, U1.p, NN, Sb.Tuu, NN, ONE, Tuun.p, NN);       // Tuub += U6^H Tuu'b
   o.mm(OP_N, OP_C, ONE, Sb.Tuu, NN, S0.Tuu, NN, ZERO, Wk.p, NN);      // U6b = Tuu'b Tuu^H
-  o.mm(OP_N, OP_C, ONE, Wk.p, NN, C2.p, NN, ONE, tuub, sb);           // tuub += U6b C2^H
+  o.mm(OP_N, OP_C, ONE, Wk.p, NN, C2, NN, ONE, tuub, sb);           // tuub += U6b C2^H
   o.mm(OP_C, OP_N, ONE, tuu, ss, Wk.p, NN, ONE, C2b.p, NN);           // C2b += tuu^H U6b
   // C2 = I + RC1 rdu
   o.mm(OP_N, OP_C, ONE, C2b.p, NN, rdu, ss, ONE, RC1b.p, NN);         // RC1b += C2b rdu^H
-  o.mm(OP_C, OP_N, ONE, RC1.p, NN, C2b.p, NN, ONE, rdub, sb);         // rdub += RC1^H C2b
+  o.mm(OP_C, OP_N, ONE, RC1, NN, C2b.p, NN, ONE, rdub, sb);         // rdub += RC1^H C2b
   // RC1 = Rud C1
-  o.mm(OP_N, OP_C, ONE, RC1b.p, NN, C1.p, NN, ZERO, Rudn.p, NN);      // Rudb = RC1b C1^H
+  o.mm(OP_N, OP_C, ONE, RC1b.p, NN, C1, NN, ZERO, Rudn.p, NN);      // Rudb = RC1b C1^H
   o.mm(OP_C, OP_N, ONE, S0.Rud, NN, RC1b.p, NN, ONE, C1b.p, NN);      // C1b += Rud^H RC1b
   // C1 = D^{-1}, D = I - rdu Rud: Db = -C1^H C1b C1^H ; rdub -= Db Rud^H ; Rudb -= rdu^H Db
-  o.mm(OP_N, OP_C, ONE, C1b.p, NN, C1.p, NN, ZERO, Wk.p, NN);
-  o.mm(OP_C, OP_N, MONE, C1.p, NN, Wk.p, NN, ZERO, U1.p, NN);         // Db
+  o.mm(OP_N, OP_C, ONE, C1b.p, NN, C1, NN, ZERO, Wk.p, NN);
+  o.mm(OP_C, OP_N, MONE, C1, NN, Wk.p, NN, ZERO, U1.p, NN);         // Db
   o.mm(OP_N, OP_C, MONE, U1.p, NN, S0.Rud, NN, ONE, rdub, sb);
   o.mm(OP_C, OP_N, MONE, rdu, ss, U1.p, NN, ONE, Rudn.p, NN);
   // Rdub of the prefix is Rdu'b itself (unchanged)

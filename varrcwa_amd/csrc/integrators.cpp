@@ -710,7 +710,7 @@ void stack_tape_vjp(StackTape& t, const cd* Sb_in, cd* epsb, int chunk) {
   std::vector<long> off((size_t)I * L + 1, 0);
   for (int i = 0; i < I; ++i)
     for (int l = 0; l < L; ++l) off[(size_t)i * L + l + 1] = off[(size_t)i * L + l] + (long)t.slab(i, l)->eps.size();
-  if (chunk <= 0) chunk = std::max(1, std::min(I, 64 / L));
+  if (chunk <= 0) chunk = std::max(1, std::min(I, 128 / L));
   for (int i0 = 0; i0 < I; i0 += chunk) {
     const int B = std::min(chunk, I - i0), SL = B * L;
     // ---- P, Q of the slabs (per-instance alpha, beta, k0^2)
@@ -760,10 +760,14 @@ void stack_tape_vjp(StackTape& t, const cd* Sb_in, cd* epsb, int chunk) {
     DevBuf Wr = c.ws.get(sizeof(cd) * NN * SL), Vr = c.ws.get(sizeof(cd) * NN * SL);
     gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
     gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
-    // ---- slab S-matrices and the cascade with every prefix kept
-    DevBuf tu = c.ws.get(sizeof(cd) * NN * SL), ru = c.ws.get(sizeof(cd) * NN * SL);
-    layer_smatrix(c, N, SL, dsv.as<double>(), Ws, Vs, gs, Wr.as<cd>(), Vr.as<cd>(), NN, tu.as<cd>(), ru.as<cd>());
+    // ---- slab S-matrices (forward intermediates kept) and the cascade with every
+    // prefix and star-product intermediate kept
+    SlabSaved slab;
+    layer_smatrix_fwd_saved(c, N, SL, dsv.as<double>(), Ws, Vs, gs, Wr.as<cd>(), Vr.as<cd>(), NN, slab);
+    const cd* tuv = slab.Tuu.as<cd>();
+    const cd* ruv = slab.Rud.as<cd>();
     DevBuf pre = c.ws.get(sizeof(cd) * 4 * NN * B * L);
+    DevBuf sstar = c.ws.get(sizeof(cd) * 3 * NN * B * L);
     DevBuf Sbuf = c.ws.get(sizeof(cd) * 4 * NN * B);
     auto blocks = [&](cd* base) {
       SMat m;
@@ -779,9 +783,9 @@ void stack_tape_vjp(StackTape& t, const cd* Sb_in, cd* epsb, int chunk) {
     for (int l = 0; l < L; ++l) {
       VRC_HIP(hipMemcpyAsync(pre.as<cd>() + (long)l * 4 * B * NN, Sbuf.ptr, sizeof(cd) * 4 * NN * B,
                              hipMemcpyDeviceToDevice, st));
-      const cd* tl = tu.as<cd>() + l * NN;
-      const cd* rl = ru.as<cd>() + l * NN;
-      star_product(c, N, B, S, tl, tl, rl, rl, ss);
+      const cd* tl = tuv + l * NN;
+      const cd* rl = ruv + l * NN;
+      star_product(c, N, B, S, tl, tl, rl, rl, ss, sstar.as<cd>() + (long)l * 3 * B * NN);
     }
     // ---- cotangents: projection, then the cascade in reverse
     DevBuf Sbb = c.ws.get(sizeof(cd) * 4 * NN * B);
@@ -800,28 +804,29 @@ void stack_tape_vjp(StackTape& t, const cd* Sb_in, cd* epsb, int chunk) {
       cd* q0 = q.as<cd>();
       for (int l = L - 1; l >= 0; --l) {
         SMat S0 = blocks(pre.as<cd>() + (long)l * 4 * B * NN);
-        const cd* tl = tu.as<cd>() + l * NN;
-        const cd* rl = ru.as<cd>() + l * NN;
-        star_product_vjp(c, N, B, S0, tl, tl, rl, rl, ss, Sb, q0, q0 + B * NN, q0 + 2 * B * NN, q0 + 3 * B * NN, NN);
+        const cd* tl = tuv + l * NN;
+        const cd* rl = ruv + l * NN;
+        star_product_vjp(c, N, B, S0, tl, tl, rl, rl, ss, Sb, q0, q0 + B * NN, q0 + 2 * B * NN, q0 + 3 * B * NN, NN,
+                         sstar.as<cd>() + (long)l * 3 * B * NN);
         // the slab's Tdd and Tuu (Rud and Rdu) are one matrix: their cotangents add
         zgeam(st, N, N, ONE, q0, N, NN, ONE, q0 + B * NN, N, NN, tb.as<cd>() + l * NN, N, ss, B);
         zgeam(st, N, N, ONE, q0 + 2 * B * NN, N, NN, ONE, q0 + 3 * B * NN, N, NN, rb.as<cd>() + l * NN, N, ss, B);
       }
     }
     pre = DevBuf();
+    sstar = DevBuf();
     // ---- slab S -> (W, V, gamma) -> (P, Q) -> eps
     DevBuf Wbar = c.ws.get(sizeof(cd) * NN * SL), Vbar = c.ws.get(sizeof(cd) * NN * SL),
-           gbar = c.ws.get(sizeof(cd) * (size_t)N * SL), Wi = c.ws.get(sizeof(cd) * NN * SL);
-    layer_smatrix_vjp(c, N, SL, dsv.as<double>(), Ws, Vs, gs, Wr.as<cd>(), Vr.as<cd>(), NN, tb.as<cd>(), rb.as<cd>(),
-                      Wbar.as<cd>(), Vbar.as<cd>(), gbar.as<cd>(), Wi.as<cd>());
-    tu = DevBuf();
-    ru = DevBuf();
+           gbar = c.ws.get(sizeof(cd) * (size_t)N * SL);
+    layer_smatrix_bwd_saved(c, N, SL, dsv.as<double>(), slab, tb.as<cd>(), rb.as<cd>(), Wbar.as<cd>(), Vbar.as<cd>(),
+                            gbar.as<cd>());
+    const cd* Wi = slab.Wi.as<cd>();  // W^{-1}, unchanged by the slab reverse pass
     DevBuf& Pbar = tb;  // slab cotangents consumed
     DevBuf& Qbar = rb;
     {
       StageScope sc(ST_ADJOINT);
-      wave_eqn_vjp(c, N, SL, P.as<cd>(), Q.as<cd>(), NN, Ws, Vs, gs, Wi.as<cd>(), Wbar.as<cd>(), Vbar.as<cd>(),
-                   gbar.as<cd>(), Pbar.as<cd>(), Qbar.as<cd>());
+      wave_eqn_vjp(c, N, SL, P.as<cd>(), Q.as<cd>(), NN, Ws, Vs, gs, Wi, Wbar.as<cd>(), Vbar.as<cd>(), gbar.as<cd>(),
+                   Pbar.as<cd>(), Qbar.as<cd>());
       std::vector<long> o(off.begin() + (size_t)i0 * L, off.begin() + (size_t)i0 * L + SL);
       assemble_pq_vjp(c, a, Pbar.as<cd>(), Qbar.as<cd>(), NN, epsb, o);
     }

@@ -410,14 +410,23 @@ void homogeneous_eig(hipStream_t s, int n, int batch, const cd* alpha, const cd*
 
 // ===================================================================== (d) star product
 void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
-                  const cd* rdu, long ss) {
+                  const cd* rdu, long ss, cd* save) {
   StageScope st(ST_STAR);
   hipStream_t s = c.stream;
   const long NN = (long)N * N;
   const size_t B = sizeof(cd) * (size_t)NN * batch;
-  DevBuf C1b = c.ws.get(B), RC1b = c.ws.get(B), C2b = c.ws.get(B), T1b = c.ws.get(B), T2b = c.ws.get(B);
+  DevBuf C1b, RC1b, C2b;
+  if (!save) {
+    C1b = c.ws.get(B);
+    RC1b = c.ws.get(B);
+    C2b = c.ws.get(B);
+  }
+  DevBuf T1b = c.ws.get(B), T2b = c.ws.get(B);
   DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * batch), infb = c.ws.get(sizeof(int) * (size_t)batch);
-  cd *C1 = C1b.as<cd>(), *RC1 = RC1b.as<cd>(), *C2 = C2b.as<cd>(), *T1 = T1b.as<cd>(), *T2 = T2b.as<cd>();
+  cd* C1 = save ? save : C1b.as<cd>();
+  cd* RC1 = save ? save + (long)batch * NN : RC1b.as<cd>();
+  cd* C2 = save ? save + 2L * batch * NN : C2b.as<cd>();
+  cd *T1 = T1b.as<cd>(), *T2 = T2b.as<cd>();
   fill_int(s, infb.as<int>(), 0, batch);
   // invC1 = I - rdu Rud ; C1 = invC1^{-1}  (src/core/RCWAIntegrator.cpp:57-60)
   zset(s, N, N, ONE, ZERO, T1, N, NN, batch);

@@ -96,8 +96,9 @@ void layer_smatrix_factors(vrc_ctx& c, int N, int batch, const double* d_s, cons
                            const cd* X, const cd* dbal, const cd* gam, const cd* W0, const cd* V0, long sRef,
                            cd* Tuu, cd* Rud);
 // Star product of src/core/RCWAIntegrator.cpp:57-67 (S updated in place; slice stride ss).
+// save (optional, 3 * batch * N*N): receives C1, Rud C1 and C2 for the reverse pass.
 void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
-                  const cd* rdu, long ss);
+                  const cd* rdu, long ss, cd* save = nullptr);
 void project_left(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LU1, const cd* W0, const cd* V0, long sRef);
 void project_right(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LUl, const cd* W1, const cd* V1,
                    long sRef);
