@@ -319,7 +319,7 @@ void invert(vrc_ctx& c, const Ops& o, const cd* A, long sA, cd* Ainv) {
   DevBuf piv = c.ws.get(sizeof(int) * (size_t)o.N * o.batch), inf = c.ws.get(sizeof(int) * (size_t)o.batch);
   fill_int(o.s, inf.as<int>(), 0, o.batch);
   o.copy(A, sA, LU.as<cd>(), o.NN);
-  zgetrf(o.s, o.N, LU.as<cd>(), o.N, o.NN, piv.as<int>(), o.N, inf.as<int>(), o.batch);
+  zgetrf(o.s, o.N, LU.as<cd>(), o.N, o.NN, piv.as<int>(), o.N, inf.as<int>(), o.batch, c.ws);
   zgetri(o.s, o.N, LU.as<cd>(), o.N, o.NN, piv.as<int>(), o.N, Ainv, o.N, o.NN, o.batch, c.ws);
 }
 

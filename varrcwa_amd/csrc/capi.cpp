@@ -67,6 +67,10 @@ int vrc_tune_set(const char* key, long long value) {
     zgemm_set_big_min_k((int)value);
     return 0;
   }
+  if (std::strcmp(key, "zgeev.bulges") == 0) {
+    zgeev_set_bulges((int)value);
+    return 0;
+  }
   if (std::strcmp(key, "zgeev.aed_window") == 0) {
     zgeev_set_aed((int)value);
     return 0;
@@ -127,7 +131,7 @@ int vrc_zgetrf(vrc_ctx* c, int n, void* A, long lda, long sA, int* ipiv, long sP
     need(n >= 0 && batch >= 0 && lda >= n, "bad size");
     need(A && ipiv, "null pointer");
     if (info) VRC_HIP(hipMemsetAsync(info, 0, sizeof(int) * batch, c->stream));
-    zgetrf(c->stream, n, (cd*)A, lda, sA, ipiv, sP, info, batch);
+    zgetrf(c->stream, n, (cd*)A, lda, sA, ipiv, sP, info, batch, c->ws);
   });
 }
 
@@ -280,8 +284,8 @@ int vrc_project_left(vrc_ctx* c, int N, int batch, void* Tdd, void* Tuu, void* R
           batch);
     zgeam(c->stream, N, N, mk(1, 0), (const cd*)V1, N, ref_stride, mk(0, 0), nullptr, 0, 0, LV.as<cd>(), N, NN,
           batch);
-    zgetrf(c->stream, N, LW.as<cd>(), N, NN, pW.as<int>(), N, inf.as<int>(), batch);
-    zgetrf(c->stream, N, LV.as<cd>(), N, NN, pV.as<int>(), N, inf.as<int>(), batch);
+    zgetrf(c->stream, N, LW.as<cd>(), N, NN, pW.as<int>(), N, inf.as<int>(), batch, c->ws);
+    zgetrf(c->stream, N, LV.as<cd>(), N, NN, pV.as<int>(), N, inf.as<int>(), batch, c->ws);
     LUPair lu;
     lu.W = LW.as<cd>();
     lu.V = LV.as<cd>();
@@ -305,8 +309,8 @@ int vrc_redheffer_segment(vrc_ctx* c, int N, double lambda, double z0, double z1
     fill_int(c->stream, inf.as<int>(), 0, 1);
     zgeam(c->stream, N, N, mk(1, 0), (const cd*)W, N, 0, mk(0, 0), nullptr, 0, 0, LW.as<cd>(), N, 0, 1);
     zgeam(c->stream, N, N, mk(1, 0), (const cd*)V, N, 0, mk(0, 0), nullptr, 0, 0, LV.as<cd>(), N, 0, 1);
-    zgetrf(c->stream, N, LW.as<cd>(), N, 0, pW.as<int>(), N, inf.as<int>(), 1);
-    zgetrf(c->stream, N, LV.as<cd>(), N, 0, pV.as<int>(), N, inf.as<int>(), 1);
+    zgetrf(c->stream, N, LW.as<cd>(), N, 0, pW.as<int>(), N, inf.as<int>(), 1, c->ws);
+    zgetrf(c->stream, N, LV.as<cd>(), N, 0, pV.as<int>(), N, inf.as<int>(), 1, c->ws);
     RefSys R;
     R.W = (const cd*)W;
     R.V = (const cd*)V;
@@ -731,8 +735,8 @@ int vrc_project_left_vjp(vrc_ctx* c, int N, int batch, const void* Tdd, const vo
           batch);
     zgeam(c->stream, N, N, mk(1, 0), (const cd*)V1, N, ref_stride, mk(0, 0), nullptr, 0, 0, LV.as<cd>(), N, NN,
           batch);
-    zgetrf(c->stream, N, LW.as<cd>(), N, NN, pW.as<int>(), N, inf.as<int>(), batch);
-    zgetrf(c->stream, N, LV.as<cd>(), N, NN, pV.as<int>(), N, inf.as<int>(), batch);
+    zgetrf(c->stream, N, LW.as<cd>(), N, NN, pW.as<int>(), N, inf.as<int>(), batch, c->ws);
+    zgetrf(c->stream, N, LV.as<cd>(), N, NN, pV.as<int>(), N, inf.as<int>(), batch, c->ws);
     LUPair lu;
     lu.W = LW.as<cd>();
     lu.V = LV.as<cd>();

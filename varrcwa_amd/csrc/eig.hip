@@ -26,7 +26,7 @@ constexpr double SAFMIN = 2.2250738585072014e-308;
 constexpr int HB = 32;         // Hessenberg panel width
 constexpr int PT = 1024;       // threads of the Hessenberg panel workgroup (16 waves)
 constexpr int QW = 64;         // maximal QR window
-constexpr int NBUL = 10;       // bulges per sweep (2 shifts each)
+constexpr int NBUL = 16;       // most bulges per sweep (2 shifts each)
 constexpr int QR_NMIN = 48;    // active blocks this small are finished in LDS
 constexpr int EXC_SWEEPS = 6;  // sweeps without deflation before exceptional shifts
 constexpr int AED_NW = 24;     // default deflation window (measured best of 16..64 at N = 882)
@@ -907,9 +907,11 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
 
 // stats (nullable): per matrix {sweeps, windows, window-chase ticks, off-window-apply ticks,
 // small-block ticks} of the constant-rate device clock.
-// use_aed: deflation-window size (0 = no aggressive early deflation).
+// use_aed: deflation-window size (0 = no aggressive early deflation); max_bul: bulges
+// per sweep (<= NBUL).
 __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long sH, cd* Z, long ldz, long sZ,
-                                                  cd* wout, long sw, int* info, long long* stats, int use_aed) {
+                                                  cd* wout, long sw, int* info, long long* stats, int use_aed,
+                                                  int max_bul) {
   long long st[5] = {0, 0, 0, 0, 0};
   __shared__ QrLds L;
   const long b = blockIdx.x;
@@ -951,7 +953,7 @@ __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long s
       break;
     }
     int nbul = (m - 2) / 6;
-    if (nbul > NBUL) nbul = NBUL;
+    if (nbul > max_bul) nbul = max_bul;
     if (nbul < 1) nbul = 1;
     // aggressive early deflation on the trailing window
     int nsa = 0;
@@ -973,7 +975,7 @@ __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long s
       if (nd < 0) nsa = 0;
       const int mm = ihi - lo + 1;
       nbul = (mm - 2) / 6;
-      if (nbul > NBUL) nbul = NBUL;
+      if (nbul > max_bul) nbul = max_bul;
       if (nbul < 1) nbul = 1;
     }
     int ns = 2 * nbul;
@@ -1250,7 +1252,9 @@ __global__ __launch_bounds__(256) void balance_kernel(int N, cd* A, long lda, lo
 }
 
 static int g_aed_window = AED_NW;
+static int g_bulges = 12;
 void zgeev_set_aed(int window) { g_aed_window = window < 0 ? 0 : (window > QW ? QW : window); }
+void zgeev_set_bulges(int nb) { g_bulges = nb < 1 ? 1 : (nb > NBUL ? NBUL : nb); }
 
 // Schur factors of a strided batch (A destroyed): A = D Z T Z^H D^{-1} with D = diag(dbal)
 // (balancing), Z unitary (ld n), T upper triangular (left in A); X (ld n, unit upper
@@ -1278,7 +1282,7 @@ void zgeev_factors(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
     DevBuf sb;
     if (want) sb = ws.get(sizeof(long long) * 5 * batch);
     hqr_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, want ? sb.as<long long>() : nullptr,
-                                     g_aed_window);
+                                     g_aed_window, g_bulges);
     launch_check("hqr");
     if (want) {
       std::vector<long long> h(5 * (size_t)batch);

@@ -214,8 +214,8 @@ struct LUStore {
     fill_int(c.stream, inf.as<int>(), 0, batch);
     zgeam(c.stream, N, N, ONE, Wsrc, N, sSrc, ZERO, nullptr, 0, 0, W.as<cd>(), N, NN, batch);
     zgeam(c.stream, N, N, ONE, Vsrc, N, sSrc, ZERO, nullptr, 0, 0, V.as<cd>(), N, NN, batch);
-    zgetrf(c.stream, N, W.as<cd>(), N, NN, pW.as<int>(), N, inf.as<int>(), batch);
-    zgetrf(c.stream, N, V.as<cd>(), N, NN, pV.as<int>(), N, inf.as<int>(), batch);
+    zgetrf(c.stream, N, W.as<cd>(), N, NN, pW.as<int>(), N, inf.as<int>(), batch, c.ws);
+    zgetrf(c.stream, N, V.as<cd>(), N, NN, pV.as<int>(), N, inf.as<int>(), batch, c.ws);
     pair.W = W.as<cd>();
     pair.V = V.as<cd>();
     pair.pW = pW.as<int>();
@@ -468,7 +468,7 @@ void Integrator::differential(int Nsteps) {
     mm(MONE, A22, V1, ONE, T22);
     // tdd = T22^{-1}; rud = T12 tdd; rdu = -T22^{-1} T21; tuu = T11 + T12 rdu
     fill_int(st, inf.as<int>(), 0, 1);
-    zgetrf(st, N, T22, N, 0, piv.as<int>(), N, inf.as<int>(), 1);
+    zgetrf(st, N, T22, N, 0, piv.as<int>(), N, inf.as<int>(), 1, c.ws);
     zgetri(st, N, T22, N, 0, piv.as<int>(), N, tdd, N, 0, 1, c.ws);
     mm(ONE, T12, tdd, ZERO, rud);
     zgetrs(st, N, N, T22, N, 0, piv.as<int>(), N, T21, N, 0, rdu, N, 0, 1, c.ws);

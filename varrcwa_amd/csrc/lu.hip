@@ -1,8 +1,10 @@
 // Batched LU factorization with partial pivoting and triangular solves (complex128).
 //
-// getrf: right-looking, panel width LU_NB. Panel factored by one workgroup per
-// matrix (pivot = first index of max |re|+|im|, as LAPACK izamax); row swaps and the
-// unit-lower solve for U12 in a column-parallel kernel; trailing update on MFMA ZGEMM.
+// getrf: right-looking, two levels: LU_NB2-column blocks factored by LU_NB-column
+// panels (one workgroup per matrix and panel; pivot = first index of max |re|+|im|, as
+// LAPACK izamax), each followed by its row interchanges, a column-parallel unit-lower
+// solve and a ZGEMM update inside the block; per block, the interchanges on the other
+// columns, the blocked triangular solve for U12 and one trailing ZGEMM with k = LU_NB2.
 // getrs: rows gathered through the final permutation, then blocked forward/backward
 // substitution (64-row diagonal blocks applied through their inverses, off-diagonal
 // updates on ZGEMM).
@@ -11,7 +13,8 @@
 
 namespace vrc {
 
-constexpr int LU_NB = 32;
+constexpr int LU_NB = 32;    // panel width
+constexpr int LU_NB2 = 128;  // outer block width
 constexpr int TRS_NB = 64;
 
 __global__ __launch_bounds__(256) void getrf_panel_kernel(int n, int k, int kb, cd* A, long lda, long sA, int* ipiv,
@@ -79,35 +82,46 @@ __global__ __launch_bounds__(256) void getrf_panel_kernel(int n, int k, int kb, 
   }
 }
 
-// Row swaps of the panel applied to all other columns; for columns right of the
-// panel also U12 = L11^{-1} A12 (unit lower kb x kb).
-__global__ __launch_bounds__(256) void getrf_swap_trsm_kernel(int n, int k, int kb, cd* A, long lda, long sA,
-                                                              const int* ipiv, long sP) {
+// Row interchanges ipiv[k0 .. k0+nswap) (in order) applied to columns [c0, c1); one
+// thread per column.
+__global__ void laswp_kernel(int k0, int nswap, cd* A, long lda, long sA, const int* ipiv, long sP, int c0, int c1) {
+  const long b = blockIdx.y;
+  const int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= c1) return;
+  cd* ac = A + b * sA + (long)col * lda;
+  const int* pv = ipiv + b * sP;
+  for (int j = 0; j < nswap; ++j) {
+    const int p = pv[k0 + j];
+    if (p != k0 + j) {
+      const cd tmp = ac[k0 + j];
+      ac[k0 + j] = ac[p];
+      ac[p] = tmp;
+    }
+  }
+}
+
+static void laswp(hipStream_t s, int k0, int nswap, cd* A, long lda, long sA, const int* ipiv, long sP, int c0,
+                  int c1, int batch) {
+  if (c1 <= c0 || nswap <= 0) return;
+  dim3 g(cdiv_up(c1 - c0, 256), batch);
+  laswp_kernel<<<g, 256, 0, s>>>(k0, nswap, A, lda, sA, ipiv, sP, c0, c1);
+  launch_check("laswp");
+}
+
+// U12 = L11^{-1} A12 for the kb x kb unit-lower panel block at (k, k), columns [c0, c1).
+__global__ __launch_bounds__(256) void trsm_panel_kernel(int k, int kb, cd* A, long lda, long sA, int c0, int c1) {
   const long b = blockIdx.y;
   A += b * sA;
-  ipiv += b * sP;
   __shared__ cd L[LU_NB][LU_NB];
-  __shared__ int pv[LU_NB];
   const int t = threadIdx.x;
   for (int e = t; e < kb * kb; e += 256) {
     int i = e % kb, j = e / kb;
     L[i][j] = A[(k + i) + (long)(k + j) * lda];
   }
-  if (t < kb) pv[t] = ipiv[k + t];
   __syncthreads();
-  const int idx = blockIdx.x * 256 + t;
-  if (idx >= n - kb) return;
-  const int col = idx < k ? idx : idx + kb;
+  const int col = c0 + blockIdx.x * 256 + t;
+  if (col >= c1) return;
   cd* ac = A + (long)col * lda;
-  for (int j = 0; j < kb; ++j) {
-    int p = pv[j];
-    if (p != k + j) {
-      cd tmp = ac[k + j];
-      ac[k + j] = ac[p];
-      ac[p] = tmp;
-    }
-  }
-  if (col < k + kb) return;
   cd x[LU_NB];
 #pragma unroll
   for (int i = 0; i < LU_NB; ++i)
@@ -125,22 +139,44 @@ __global__ __launch_bounds__(256) void getrf_swap_trsm_kernel(int n, int k, int 
     if (i < kb) ac[k + i] = x[i];
 }
 
-void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, int* info, int batch) {
+static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* LU, long lda, long sA, cd* Y,
+                           long ldy, long sY, int batch, Workspace& ws);
+
+// Two-level right-looking LU: 128-column blocks, each factored by 32-column panels with
+// updates inside the block; the block's interchanges and U12 = L11^{-1} A12 (blocked
+// triangular solve) are then applied to the rest, and the trailing matrix is updated
+// by one ZGEMM with k = 128.
+void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, int* info, int batch, Workspace& ws) {
   if (n <= 0 || batch <= 0) return;
-  for (int k = 0; k < n; k += LU_NB) {
-    const int kb = n - k < LU_NB ? n - k : LU_NB;
-    getrf_panel_kernel<<<batch, 256, 0, s>>>(n, k, kb, A, lda, sA, ipiv, sP, info);
-    launch_check("getrf_panel");
-    if (n - kb > 0) {
-      dim3 g(cdiv_up(n - kb, 256), batch);
-      getrf_swap_trsm_kernel<<<g, 256, 0, s>>>(n, k, kb, A, lda, sA, ipiv, sP);
-      launch_check("getrf_swap_trsm");
+  const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0);
+  for (int K = 0; K < n; K += LU_NB2) {
+    const int KB = n - K < LU_NB2 ? n - K : LU_NB2;
+    for (int k = K; k < K + KB; k += LU_NB) {
+      const int kb = K + KB - k < LU_NB ? K + KB - k : LU_NB;
+      getrf_panel_kernel<<<batch, 256, 0, s>>>(n, k, kb, A, lda, sA, ipiv, sP, info);
+      launch_check("getrf_panel");
+      // the panel's interchanges on the block's other columns
+      laswp(s, k, kb, A, lda, sA, ipiv, sP, K, k, batch);
+      laswp(s, k, kb, A, lda, sA, ipiv, sP, k + kb, K + KB, batch);
+      const int c0 = k + kb, c1 = K + KB;
+      if (c1 > c0) {
+        dim3 g(cdiv_up(c1 - c0, 256), batch);
+        trsm_panel_kernel<<<g, 256, 0, s>>>(k, kb, A, lda, sA, c0, c1);
+        launch_check("trsm_panel");
+        if (n - c0 > 0)
+          zgemm_strided(s, OP_N, OP_N, n - c0, c1 - c0, kb, mone, A + c0 + (long)k * lda, lda, sA,
+                        A + k + (long)c0 * lda, lda, sA, one, A + c0 + (long)c0 * lda, lda, sA, batch);
+      }
     }
-    const int rest = n - k - kb;
+    // the block's interchanges on the columns left and right of the block
+    laswp(s, K, KB, A, lda, sA, ipiv, sP, 0, K, batch);
+    laswp(s, K, KB, A, lda, sA, ipiv, sP, K + KB, n, batch);
+    const int rest = n - K - KB;
     if (rest > 0) {
-      zgemm_strided(s, OP_N, OP_N, rest, rest, kb, mk(-1.0, 0.0), A + (k + kb) + (long)k * lda, lda, sA,
-                    A + k + (long)(k + kb) * lda, lda, sA, mk(1.0, 0.0), A + (k + kb) + (long)(k + kb) * lda, lda,
-                    sA, batch);
+      trsm_lu_factor(s, 0, KB, rest, A + K + (long)K * lda, lda, sA, A + K + (long)(K + KB) * lda, lda, sA, batch,
+                     ws);
+      zgemm_strided(s, OP_N, OP_N, rest, rest, KB, mone, A + (K + KB) + (long)K * lda, lda, sA,
+                    A + K + (long)(K + KB) * lda, lda, sA, one, A + (K + KB) + (long)(K + KB) * lda, lda, sA, batch);
     }
   }
 }

@@ -148,7 +148,7 @@ void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q) {
   DevBuf piv = c.ws.get(sizeof(int) * (size_t)n * L);
   DevBuf inf = c.ws.get(sizeof(int) * (size_t)L);
   fill_int(s, inf.as<int>(), 0, L);
-  zgetrf(s, n, fe33, n, nn, piv.as<int>(), n, inf.as<int>(), L);
+  zgetrf(s, n, fe33, n, nn, piv.as<int>(), n, inf.as<int>(), L, c.ws);
   if (a.periodic) {
     DevBuf inv = c.ws.get(sizeof(cd) * (size_t)nn * L);
     zgetri(s, n, fe33, n, nn, piv.as<int>(), n, inv.as<cd>(), n, nn, L, c.ws);
@@ -218,10 +218,10 @@ void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W,
   fill_int(s, infb.as<int>(), 0, batch);
   // F0 = W^{-1} W0 ; G0 = V^{-1} V0
   copy(s, N, N, W, LU, NN, batch);
-  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch, c.ws);
   zgetrs(s, N, N, LU, N, NN, piv, N, W0, N, sRef, F0, N, NN, batch, c.ws);
   copy(s, N, N, V, LU, NN, batch);
-  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch, c.ws);
   zgetrs(s, N, N, LU, N, NN, piv, N, V0, N, sRef, G0, N, NN, batch, c.ws);
   // A = F0 + G0 (into LU, factored); RHS = [B | X A | X B] with B = F0 - G0
   DevBuf Xb = c.ws.get(sizeof(cd) * (size_t)N * batch);
@@ -237,7 +237,7 @@ void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W,
   zgeam(s, N, N, ONE, F0, N, NN, ONE, G0, N, NN, LU, N, NN, batch);              // A
   zgeam(s, N, N, ONE, F0, N, NN, MONE, G0, N, NN, R3, N, s3, batch);             // B
   zdscale(s, N, N, Xb.as<cd>(), N, nullptr, 0, 0, LU, N, NN, R3 + NN, N, s3, batch);      // X A
-  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch, c.ws);
   zgetrs(s, N, 2 * N, LU, N, NN, piv, N, R3, N, s3, R3, N, s3, batch, c.ws);
   const cd* invAB = R3;
   const cd* invAXA = R3 + NN;
@@ -253,7 +253,7 @@ void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W,
   zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, invAXB, N, s3, invAXA, N, s3, MONE, R2, N, s2, batch);
   zgeam(s, N, N, ONE, invAXA, N, s3, ZERO, nullptr, 0, 0, R2 + NN, N, s2, batch);
   zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, invAXB, N, s3, invAB, N, s3, ONE, R2 + NN, N, s2, batch);
-  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch, c.ws);
   zgetrs(s, N, 2 * N, LU, N, NN, piv, N, R2, N, s2, R2, N, s2, batch, c.ws);
   zgeam(s, N, N, ONE, R2, N, s2, ZERO, nullptr, 0, 0, Rud, N, NN, batch);
   zgeam(s, N, N, ONE, R2 + NN, N, s2, ZERO, nullptr, 0, 0, Tuu, N, NN, batch);
@@ -347,7 +347,7 @@ void layer_smatrix_factors(vrc_ctx& c, int N, int batch, const double* d_s, cons
   zgeam(s, N, N, ONE, F0, N, s2, ONE, G0, N, NN, LU, N, NN, batch);              // A
   zgeam(s, N, N, ONE, F0, N, s2, MONE, G0, N, NN, R3, N, s3, batch);             // B
   zdscale(s, N, N, Xb.as<cd>(), N, nullptr, 0, 0, LU, N, NN, R3 + NN, N, s3, batch);  // X A
-  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch, c.ws);
   zgetrs(s, N, 2 * N, LU, N, NN, piv, N, R3, N, s3, R3, N, s3, batch, c.ws);
   const cd* invAB = R3;
   const cd* invAXA = R3 + NN;
@@ -361,7 +361,7 @@ void layer_smatrix_factors(vrc_ctx& c, int N, int batch, const double* d_s, cons
   zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, invAXB, N, s3, invAXA, N, s3, MONE, RH, N, s2, batch);
   zgeam(s, N, N, ONE, invAXA, N, s3, ZERO, nullptr, 0, 0, RH + NN, N, s2, batch);
   zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, invAXB, N, s3, invAB, N, s3, ONE, RH + NN, N, s2, batch);
-  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch);
+  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch, c.ws);
   zgetrs(s, N, 2 * N, LU, N, NN, piv, N, RH, N, s2, RH, N, s2, batch, c.ws);
   zgeam(s, N, N, ONE, RH, N, s2, ZERO, nullptr, 0, 0, Rud, N, NN, batch);
   zgeam(s, N, N, ONE, RH + NN, N, s2, ZERO, nullptr, 0, 0, Tuu, N, NN, batch);
@@ -431,7 +431,7 @@ void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd
   // invC1 = I - rdu Rud ; C1 = invC1^{-1}  (src/core/RCWAIntegrator.cpp:57-60)
   zset(s, N, N, ONE, ZERO, T1, N, NN, batch);
   zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, rdu, N, ss, S.Rud, N, NN, ONE, T1, N, NN, batch);
-  zgetrf(s, N, T1, N, NN, pivb.as<int>(), N, infb.as<int>(), batch);
+  zgetrf(s, N, T1, N, NN, pivb.as<int>(), N, infb.as<int>(), batch, c.ws);
   zgetri(s, N, T1, N, NN, pivb.as<int>(), N, C1, N, NN, batch, c.ws);
   // RC1 = Rud C1 ; C2 = I + RC1 rdu
   mm(s, N, ONE, S.Rud, C1, ZERO, RC1, NN, batch);
@@ -480,7 +480,7 @@ void project_left(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LU1, cons
   zgeam(s, N, N, MONE, G, N, NN, ZERO, nullptr, 0, 0, R2, N, s2, batch);
   zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, S.Rdu, N, NN, F, N, NN, ONE, R2, N, s2, batch);
   zgeam(s, N, N, ONE, S.Tdd, N, NN, ZERO, nullptr, 0, 0, R2 + NN, N, s2, batch);
-  zgetrf(s, N, K, N, NN, pivb.as<int>(), N, infb.as<int>(), batch);
+  zgetrf(s, N, K, N, NN, pivb.as<int>(), N, infb.as<int>(), batch, c.ws);
   zgetrs(s, N, 2 * N, K, N, NN, pivb.as<int>(), N, R2, N, s2, R2, N, s2, batch, c.ws);
   zgeam(s, N, N, ONE, R2, N, s2, ZERO, nullptr, 0, 0, S.Rdu, N, NN, batch);
   zgeam(s, N, N, ONE, R2 + NN, N, s2, ZERO, nullptr, 0, 0, S.Tdd, N, NN, batch);
@@ -517,7 +517,7 @@ void project_right(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LUl, con
   zgeam(s, N, N, ONE, S.Tuu, N, NN, ZERO, nullptr, 0, 0, R2, N, s2, batch);
   zgeam(s, N, N, MONE, G, N, NN, ZERO, nullptr, 0, 0, R2 + NN, N, s2, batch);
   zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, S.Rud, N, NN, F, N, NN, ONE, R2 + NN, N, s2, batch);
-  zgetrf(s, N, K, N, NN, pivb.as<int>(), N, infb.as<int>(), batch);
+  zgetrf(s, N, K, N, NN, pivb.as<int>(), N, infb.as<int>(), batch, c.ws);
   zgetrs(s, N, 2 * N, K, N, NN, pivb.as<int>(), N, R2, N, s2, R2, N, s2, batch, c.ws);
   zgeam(s, N, N, ONE, R2, N, s2, ZERO, nullptr, 0, 0, S.Tuu, N, NN, batch);
   zgeam(s, N, N, ONE, R2 + NN, N, s2, ZERO, nullptr, 0, 0, S.Rud, N, NN, batch);

@@ -71,7 +71,7 @@ void zdscale(hipStream_t s, int m, int n, const cd* left, long sl, const cd* rig
 void zmaxabs(hipStream_t s, int m, int n, const cd* A, long lda, long sA, double* out, int batch);
 
 // ---------------------------------------------------------------- lu.hip
-void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, int* info, int batch);
+void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, int* info, int batch, Workspace& ws);
 void zgetrs(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, const int* ipiv, long sP, const cd* B,
             long ldb, long sB, cd* X, long ldx, long sX, int batch, Workspace& ws);
 // Y <- U^{-1} Y for upper-triangular U (non-unit diagonal), in place.
@@ -94,7 +94,8 @@ void zgeev_factors(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
                    int* info, int batch, Workspace& ws);
 void zgeev_vectors(hipStream_t s, int n, const cd* Z, const cd* X, const cd* dbal, cd* VR, long ldv, long sV,
                    int batch);
-void zgeev_set_aed(int window);  // deflation-window size of the QR iteration (0 = off)
+void zgeev_set_aed(int window);
+void zgeev_set_bulges(int nb);  // bulges per QR sweep (1..16)  // deflation-window size of the QR iteration (0 = off)
 // gamma = sqrt(lambda) with the reference's branch rule.
 void gamma_sqrt(hipStream_t s, int n, const cd* lam, long sl, cd* gam, long sg, int batch);
 
