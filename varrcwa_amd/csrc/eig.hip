@@ -99,8 +99,8 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
     const int c = p + j;
     if (t < j) vec1[t] = cconj(Vf[c + (long)(p + t) * N]);
     __syncthreads();
-    // column c of A0*Q_j
-    for (int r = t; r < N; r += PT) {
+    // column c of A0*Q_j, rows below p (rows <= p are updated after the panel)
+    for (int r = p + 1 + t; r < N; r += PT) {
       cd acc = A[r + (long)c * lda];
       for (int i = 0; i < j; ++i) acc = csub(acc, cmul(Y[r + (long)i * N], vec1[i]));
       bvec[r] = acc;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
     }
     // final entries of column c and the reflector from rows c+1..N-1
     double part = 0.0;
-    for (int r = t; r < N; r += PT) {
+    for (int r = p + 1 + t; r < N; r += PT) {
       if (r <= c) A[r + (long)c * lda] = bvec[r];
       if (r >= c + 2) part += cnorm2(bvec[r]);
     }
@@ -152,10 +152,10 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
     __syncthreads();
     // z = V_j^H v  (rows >= c+1)
     if (j > 0) panel_multi_dot(N, c + 1, p, j, Vf, vvec, red, vec2);
-    // y_j = tau*(A0[:, c+1:] v - Y_j z). The trailing-matrix product is the
-    // memory-bound part: every thread keeps 4 rows and 4 columns of independent
-    // loads in flight.
-    for (int rb = t; rb < N; rb += 4 * PT) {
+    // y_j = tau*(A0[:, c+1:] v - Y_j z) for the rows below p (the rows above are one
+    // ZGEMM after the panel). The trailing-matrix product is the memory-bound part:
+    // every thread keeps 4 rows and 4 columns of independent loads in flight.
+    for (int rb = p + 1 + t; rb < N; rb += 4 * PT) {
       double gr[4] = {0.0, 0.0, 0.0, 0.0}, gi[4] = {0.0, 0.0, 0.0, 0.0};
       int col = c + 1;
       for (; col + 3 < N; col += 4) {
@@ -237,6 +237,13 @@ static void gehrd(hipStream_t s, int n, cd* A, long lda, long sA, cd* Vf, cd* Ta
     gehrd_panel_kernel<<<batch, PT, 2 * sizeof(cd) * n, s>>>(n, p, nb, A, lda, sA, Vf, sV, Y, sY, Tp, sT,
                                                                 taub.as<cd>(), n);
     launch_check("gehrd_panel");
+    // rows 0..p: Y = A0 V T, then the panel columns' A0 Q = A0 - Y V^H
+    zgemm_strided(s, OP_N, OP_N, p + 1, nb, n - p - 1, one, A + (long)(p + 1) * lda, lda, sA,
+                  Vf + (p + 1) + (long)p * n, n, sV, zero, W, n, sW, batch);
+    zgemm_strided(s, OP_N, OP_N, p + 1, nb, nb, one, W, n, sW, Tp, HB, sT, zero, Y, n, sY, batch);
+    if (nb > 1)
+      zgemm_strided(s, OP_N, OP_C, p + 1, nb - 1, nb, mone, Y, n, sY, Vf + (p + 1) + (long)p * n, n, sV, one,
+                    A + (long)(p + 1) * lda, lda, sA, batch);
     const int c0 = p + nb, ncols = n - c0;
     if (ncols <= 0) continue;
     zgemm_strided(s, OP_N, OP_C, n, ncols, nb, mone, Y, n, sY, Vf + c0 + (long)p * n, n, sV, one,

@@ -280,23 +280,42 @@ static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* 
                   zero, Tmp, TRS_NB, sTmp, batch);
     zgeam(s, bs, nrhs, one, Tmp, TRS_NB, sTmp, zero, nullptr, 0, 0, Y + k0, ldy, sY, batch);
   };
+  // Diagonal blocks are taken in pairs: the coupling between the two is one 64-deep
+  // update, the rest of Y is updated once per pair (k up to 2*TRS_NB).
   if (!upper) {
-    for (int kb = 0; kb < nblk; ++kb) {
+    for (int kb = 0; kb < nblk; kb += 2) {
       const int k0 = kb * TRS_NB;
-      const int bs = n - k0 < TRS_NB ? n - k0 : TRS_NB;
-      diag(kb, k0, bs);
-      const int rest = n - k0 - bs;
+      const int bs0 = n - k0 < TRS_NB ? n - k0 : TRS_NB;
+      diag(kb, k0, bs0);
+      int w = bs0;
+      if (kb + 1 < nblk) {
+        const int k1 = k0 + TRS_NB;
+        const int bs1 = n - k1 < TRS_NB ? n - k1 : TRS_NB;
+        zgemm_strided(s, OP_N, OP_N, bs1, nrhs, bs0, mone, LU + k1 + (long)k0 * lda, lda, sA, Y + k0, ldy, sY, one,
+                      Y + k1, ldy, sY, batch);
+        diag(kb + 1, k1, bs1);
+        w += bs1;
+      }
+      const int rest = n - k0 - w;
       if (rest > 0)
-        zgemm_strided(s, OP_N, OP_N, rest, nrhs, bs, mone, LU + (k0 + bs) + (long)k0 * lda, lda, sA, Y + k0, ldy,
-                      sY, one, Y + (k0 + bs), ldy, sY, batch);
+        zgemm_strided(s, OP_N, OP_N, rest, nrhs, w, mone, LU + (k0 + w) + (long)k0 * lda, lda, sA, Y + k0, ldy,
+                      sY, one, Y + (k0 + w), ldy, sY, batch);
     }
   } else {
-    for (int kb = nblk - 1; kb >= 0; --kb) {
-      const int k0 = kb * TRS_NB;
-      const int bs = n - k0 < TRS_NB ? n - k0 : TRS_NB;
-      diag(kb, k0, bs);
+    for (int kb = nblk - 1; kb >= 0; kb -= 2) {
+      const int k1 = kb * TRS_NB;
+      const int bs1 = n - k1 < TRS_NB ? n - k1 : TRS_NB;
+      diag(kb, k1, bs1);
+      int k0 = k1, w = bs1;
+      if (kb >= 1) {
+        k0 = k1 - TRS_NB;
+        zgemm_strided(s, OP_N, OP_N, TRS_NB, nrhs, bs1, mone, LU + k0 + (long)k1 * lda, lda, sA, Y + k1, ldy, sY,
+                      one, Y + k0, ldy, sY, batch);
+        diag(kb - 1, k0, TRS_NB);
+        w += TRS_NB;
+      }
       if (k0 > 0)
-        zgemm_strided(s, OP_N, OP_N, k0, nrhs, bs, mone, LU + (long)k0 * lda, lda, sA, Y + k0, ldy, sY, one, Y, ldy,
+        zgemm_strided(s, OP_N, OP_N, k0, nrhs, w, mone, LU + (long)k0 * lda, lda, sA, Y + k0, ldy, sY, one, Y, ldy,
                       sY, batch);
     }
   }
