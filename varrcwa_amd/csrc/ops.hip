@@ -201,6 +201,79 @@ void wave_eqn(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* W, cd* 
 }
 
 // ===================================================================== (d) layer S-matrix
+// Systems of the mirror-symmetric slab: per instance b and thread (row r, column c),
+// with f = F0(r,c), g = G0(r,c) (G given as Gamma G0 when gam is set), x = X(r):
+//   L[2b] = (1-x) f + (1+x) g,  R[2b] = (1+x) g - (1-x) f,
+//   L[2b+1] = (1+x) f + (1-x) g,  R[2b+1] = (1-x) g - (1+x) f.
+// R may alias F0/G when R[2b] is F0[b] and R[2b+1] is G[b] (each thread reads before it writes).
+__global__ void evenodd_system_kernel(int N, const cd* F0, long sF, const cd* G, long sG, const cd* gam,
+                                      const cd* x, cd* L, cd* R) {
+  const long b = blockIdx.z;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= N) return;
+  const long o = r + (long)c * N, NN = (long)N * N;
+  const cd f = F0[b * sF + o];
+  cd g = G[b * sG + o];
+  if (gam) g = cdiv(g, gam[b * N + r]);
+  const cd xr = x[b * N + r];
+  const cd om = mk(1.0 - xr.x, -xr.y), op = mk(1.0 + xr.x, xr.y);
+  const cd P = cmul(om, f), Q = cmul(op, g), Pp = cmul(op, f), Qp = cmul(om, g);
+  L[(2 * b) * NN + o] = cadd(P, Q);
+  L[(2 * b + 1) * NN + o] = cadd(Pp, Qp);
+  R[(2 * b) * NN + o] = csub(Q, P);
+  R[(2 * b + 1) * NN + o] = csub(Qp, Pp);
+}
+
+// Rud = (E[2b] + E[2b+1]) / 2, Tuu = (E[2b] - E[2b+1]) / 2.
+__global__ void evenodd_combine_kernel(long NN, const cd* E, cd* Tuu, cd* Rud) {
+  const long b = blockIdx.y;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < NN; i += (long)gridDim.x * blockDim.x) {
+    const cd ep = E[(2 * b) * NN + i], em = E[(2 * b + 1) * NN + i];
+    Rud[b * NN + i] = mk(0.5 * (ep.x + em.x), 0.5 * (ep.y + em.y));
+    Tuu[b * NN + i] = mk(0.5 * (ep.x - em.x), 0.5 * (ep.y - em.y));
+  }
+}
+
+// Slab S-matrix from F0 = W^{-1} W0 and G0 = V^{-1} V0 (G = Gamma G0 when gam_scale is
+// set). The reference (src/rcwa/RCWAScatterMatrix.cpp:26-85) computes, with A = F0 + G0,
+// B = F0 - G0, X = diag(exp(i gamma s)) and D = A - X B A^{-1} X B,
+//   Rud = D^{-1} (X B A^{-1} X A - B),  Tuu = D^{-1} X (A - B A^{-1} B).
+// D factors both as (I + X B A^{-1})(A - X B) and as (I - X B A^{-1})(A + X B), which gives
+//   Rud + Tuu = (A - X B)^{-1} (X A - B),  Rud - Tuu = -(A + X B)^{-1} (X A + B),
+// and A -+ X B, X A -+ B are the row-scaled sums in evenodd_system_kernel. Two LU
+// factorizations with N right-hand sides each (batched together) replace the
+// reference's two factorizations with 2N right-hand sides and four products.
+// R (2 N x N per instance) may be the storage of [F0 | G] (stride sF = 2 N^2, G = F0 + N^2).
+static void slab_evenodd(vrc_ctx& c, int N, int batch, const double* d_s, const cd* gam, const cd* F0, long sF,
+                         const cd* G, long sG, bool gam_scale, cd* R, cd* Tuu, cd* Rud) {
+  hipStream_t s = c.stream;
+  const long NN = (long)N * N;
+  const size_t B2 = sizeof(cd) * (size_t)NN * 2 * batch;
+  DevBuf xb = c.ws.get(sizeof(cd) * (size_t)N * batch);
+  {
+    dim3 g(cdiv_up(N, 256), batch);
+    phase_kernel<<<g, 256, 0, s>>>(N, gam, N, d_s, xb.as<cd>(), N);
+    launch_check("phase");
+  }
+  DevBuf Lb = c.ws.get(B2), Eb = c.ws.get(B2);
+  DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * 2 * batch), infb = c.ws.get(sizeof(int) * (size_t)2 * batch);
+  {
+    dim3 g(cdiv_up(N, 256), N, batch);
+    evenodd_system_kernel<<<g, 256, 0, s>>>(N, F0, sF, G, sG, gam_scale ? gam : nullptr, xb.as<cd>(), Lb.as<cd>(),
+                                            R);
+    launch_check("evenodd_system");
+  }
+  fill_int(s, infb.as<int>(), 0, 2 * batch);
+  zgetrf(s, N, Lb.as<cd>(), N, NN, pivb.as<int>(), N, infb.as<int>(), 2 * batch, c.ws);
+  zgetrs(s, N, N, Lb.as<cd>(), N, NN, pivb.as<int>(), N, R, N, NN, Eb.as<cd>(), N, NN, 2 * batch, c.ws);
+  {
+    dim3 g(cdiv_up(NN, 256) < 1024 ? cdiv_up(NN, 256) : 1024, batch);
+    evenodd_combine_kernel<<<g, 256, 0, s>>>(NN, Eb.as<cd>(), Tuu, Rud);
+    launch_check("evenodd_combine");
+  }
+}
+
 void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
                    const cd* W0, const cd* V0, long sRef, cd* Tuu, cd* Rud) {
   StageScope st(ST_LAYER_S);
@@ -223,40 +296,8 @@ void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W,
   copy(s, N, N, V, LU, NN, batch);
   zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch, c.ws);
   zgetrs(s, N, N, LU, N, NN, piv, N, V0, N, sRef, G0, N, NN, batch, c.ws);
-  // A = F0 + G0 (into LU, factored); RHS = [B | X A | X B] with B = F0 - G0
-  DevBuf Xb = c.ws.get(sizeof(cd) * (size_t)N * batch);
-  {
-    dim3 g(cdiv_up(N, 256), batch);
-    phase_kernel<<<g, 256, 0, s>>>(N, gam, N, d_s, Xb.as<cd>(), N);
-    launch_check("phase");
-  }
-  // RHS = [B | X A]; invAXB = A^{-1} X B = (A^{-1} X A)(A^{-1} B) is then one product
-  DevBuf R3b = c.ws.get(sizeof(cd) * (size_t)NN * 3 * batch);
-  cd* R3 = R3b.as<cd>();
-  const long s3 = 3 * NN;
-  zgeam(s, N, N, ONE, F0, N, NN, ONE, G0, N, NN, LU, N, NN, batch);              // A
-  zgeam(s, N, N, ONE, F0, N, NN, MONE, G0, N, NN, R3, N, s3, batch);             // B
-  zdscale(s, N, N, Xb.as<cd>(), N, nullptr, 0, 0, LU, N, NN, R3 + NN, N, s3, batch);      // X A
-  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch, c.ws);
-  zgetrs(s, N, 2 * N, LU, N, NN, piv, N, R3, N, s3, R3, N, s3, batch, c.ws);
-  const cd* invAB = R3;
-  const cd* invAXA = R3 + NN;
-  const cd* invAXB = R3 + 2 * NN;
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, invAXA, N, s3, invAB, N, s3, ZERO, R3 + 2 * NN, N, s3, batch);
-  // M = I - invAXB^2 ; RHS2 = [invAXB invAXA - invAB | invAXA - invAXB invAB]
-  zset(s, N, N, ONE, ZERO, LU, N, NN, batch);
-  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, invAXB, N, s3, invAXB, N, s3, ONE, LU, N, NN, batch);
-  DevBuf R2b = c.ws.get(sizeof(cd) * (size_t)NN * 2 * batch);
-  cd* R2 = R2b.as<cd>();
-  const long s2 = 2 * NN;
-  zgeam(s, N, N, ONE, invAB, N, s3, ZERO, nullptr, 0, 0, R2, N, s2, batch);
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, invAXB, N, s3, invAXA, N, s3, MONE, R2, N, s2, batch);
-  zgeam(s, N, N, ONE, invAXA, N, s3, ZERO, nullptr, 0, 0, R2 + NN, N, s2, batch);
-  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, invAXB, N, s3, invAB, N, s3, ONE, R2 + NN, N, s2, batch);
-  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch, c.ws);
-  zgetrs(s, N, 2 * N, LU, N, NN, piv, N, R2, N, s2, R2, N, s2, batch, c.ws);
-  zgeam(s, N, N, ONE, R2, N, s2, ZERO, nullptr, 0, 0, Rud, N, NN, batch);
-  zgeam(s, N, N, ONE, R2 + NN, N, s2, ZERO, nullptr, 0, 0, Tuu, N, NN, batch);
+  DevBuf Rb = c.ws.get(sizeof(cd) * (size_t)NN * 2 * batch);
+  slab_evenodd(c, N, batch, d_s, gam, F0, NN, G0, NN, false, Rb.as<cd>(), Tuu, Rud);
 }
 
 // C = diag(1/d) A (row scaling by the reciprocal of d), batched.
@@ -308,63 +349,28 @@ void wave_eqn_vectors(vrc_ctx& c, int N, const cd* Q, long sQ, const cd* Z, cons
 //   W^{-1} = X^{-1} Z^H D^{-1}, and since PQ W = W Gamma^2 gives V = Q W / Gamma =
 //   P^{-1} W Gamma, V^{-1} V0 = Gamma^{-1} W^{-1} (P V0). So
 //   [F0 | Gamma G0] = X^{-1} Z^H D^{-1} [W0 | P V0]  (one triangular solve, no LU),
-// then the arithmetic of RCWAScatterMatrix::compute (src/rcwa/RCWAScatterMatrix.cpp:26-85).
+// then the even/odd split of slab_evenodd.
 void layer_smatrix_factors(vrc_ctx& c, int N, int batch, const double* d_s, const cd* P, long sP, const cd* Z,
                            const cd* X, const cd* dbal, const cd* gam, const cd* W0, const cd* V0, long sRef,
                            cd* Tuu, cd* Rud) {
   StageScope st(ST_LAYER_S);
   hipStream_t s = c.stream;
   const long NN = (long)N * N, s2 = 2 * NN;
-  DevBuf R2b = c.ws.get(sizeof(cd) * (size_t)NN * 2 * batch), Yb = c.ws.get(sizeof(cd) * (size_t)NN * 2 * batch);
-  cd* R2 = R2b.as<cd>();
+  DevBuf Yb = c.ws.get(sizeof(cd) * (size_t)NN * 2 * batch);
   cd* Y = Yb.as<cd>();
-  // R2 = D^{-1} [W0 | P V0]
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, P, N, sP, V0, N, sRef, ZERO, Y + NN, N, s2, batch);
-  zgeam(s, N, N, ONE, W0, N, sRef, ZERO, nullptr, 0, 0, Y, N, s2, batch);
-  rowscale_inv(s, N, 2 * N, dbal, N, Y, N, s2, R2, N, s2, batch);
-  // Y = Z^H R2 ; Y <- X^{-1} Y
-  zgemm_strided(s, OP_C, OP_N, N, 2 * N, N, ONE, Z, N, NN, R2, N, s2, ZERO, Y, N, s2, batch);
-  ztrsm_upper(s, N, 2 * N, X, N, NN, Y, N, s2, batch, c.ws);
-  // F0 = Y[:, :N] ; G0 = Gamma^{-1} Y[:, N:]
-  cd* F0 = Y;
-  cd* G0 = R2;  // reuse
-  rowscale_inv(s, N, N, gam, N, Y + NN, N, s2, G0, N, NN, batch);
-  DevBuf LUb = c.ws.get(sizeof(cd) * (size_t)NN * batch);
-  DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * batch);
-  DevBuf infb = c.ws.get(sizeof(int) * (size_t)batch);
-  cd* LU = LUb.as<cd>();
-  int* piv = pivb.as<int>();
-  fill_int(s, infb.as<int>(), 0, batch);
-  DevBuf Xb = c.ws.get(sizeof(cd) * (size_t)N * batch);
   {
-    dim3 g(cdiv_up(N, 256), batch);
-    phase_kernel<<<g, 256, 0, s>>>(N, gam, N, d_s, Xb.as<cd>(), N);
-    launch_check("phase");
+    DevBuf R2b = c.ws.get(sizeof(cd) * (size_t)NN * 2 * batch);
+    cd* R2 = R2b.as<cd>();
+    // R2 = D^{-1} [W0 | P V0]
+    zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, P, N, sP, V0, N, sRef, ZERO, Y + NN, N, s2, batch);
+    zgeam(s, N, N, ONE, W0, N, sRef, ZERO, nullptr, 0, 0, Y, N, s2, batch);
+    rowscale_inv(s, N, 2 * N, dbal, N, Y, N, s2, R2, N, s2, batch);
+    // Y = Z^H R2 ; Y <- X^{-1} Y
+    zgemm_strided(s, OP_C, OP_N, N, 2 * N, N, ONE, Z, N, NN, R2, N, s2, ZERO, Y, N, s2, batch);
+    ztrsm_upper(s, N, 2 * N, X, N, NN, Y, N, s2, batch, c.ws);
   }
-  DevBuf R3b = c.ws.get(sizeof(cd) * (size_t)NN * 3 * batch);
-  cd* R3 = R3b.as<cd>();
-  const long s3 = 3 * NN;
-  zgeam(s, N, N, ONE, F0, N, s2, ONE, G0, N, NN, LU, N, NN, batch);              // A
-  zgeam(s, N, N, ONE, F0, N, s2, MONE, G0, N, NN, R3, N, s3, batch);             // B
-  zdscale(s, N, N, Xb.as<cd>(), N, nullptr, 0, 0, LU, N, NN, R3 + NN, N, s3, batch);  // X A
-  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch, c.ws);
-  zgetrs(s, N, 2 * N, LU, N, NN, piv, N, R3, N, s3, R3, N, s3, batch, c.ws);
-  const cd* invAB = R3;
-  const cd* invAXA = R3 + NN;
-  const cd* invAXB = R3 + 2 * NN;
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, invAXA, N, s3, invAB, N, s3, ZERO, R3 + 2 * NN, N, s3, batch);
-  // M = I - invAXB^2 ; RHS2 = [invAXB invAXA - invAB | invAXA - invAXB invAB]
-  zset(s, N, N, ONE, ZERO, LU, N, NN, batch);
-  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, invAXB, N, s3, invAXB, N, s3, ONE, LU, N, NN, batch);
-  cd* RH = Y;  // 2N columns, F0/G0 consumed
-  zgeam(s, N, N, ONE, invAB, N, s3, ZERO, nullptr, 0, 0, RH, N, s2, batch);
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, invAXB, N, s3, invAXA, N, s3, MONE, RH, N, s2, batch);
-  zgeam(s, N, N, ONE, invAXA, N, s3, ZERO, nullptr, 0, 0, RH + NN, N, s2, batch);
-  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, invAXB, N, s3, invAB, N, s3, ONE, RH + NN, N, s2, batch);
-  zgetrf(s, N, LU, N, NN, piv, N, infb.as<int>(), batch, c.ws);
-  zgetrs(s, N, 2 * N, LU, N, NN, piv, N, RH, N, s2, RH, N, s2, batch, c.ws);
-  zgeam(s, N, N, ONE, RH, N, s2, ZERO, nullptr, 0, 0, Rud, N, NN, batch);
-  zgeam(s, N, N, ONE, RH + NN, N, s2, ZERO, nullptr, 0, 0, Tuu, N, NN, batch);
+  // F0 = Y[:, :N], Gamma G0 = Y[:, N:]; the systems' right-hand sides overwrite Y
+  slab_evenodd(c, N, batch, d_s, gam, Y, s2, Y + NN, s2, true, Y, Tuu, Rud);
 }
 
 // Closed-form eigen-system of a homogeneous periodic medium (one permittivity value):
