@@ -415,23 +415,18 @@ void homogeneous_eig(hipStream_t s, int n, int batch, const cd* alpha, const cd*
 }
 
 // ===================================================================== (d) star product
-void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
-                  const cd* rdu, long ss, cd* save) {
-  StageScope st(ST_STAR);
+// Reference arithmetic (src/core/RCWAIntegrator.cpp:57-72) with the explicit inverse C1;
+// used when the reverse pass needs C1, Rud C1 and C2 (save: 3 N x N per instance).
+static void star_product_explicit(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu,
+                                  const cd* rud, const cd* rdu, long ss, cd* save) {
   hipStream_t s = c.stream;
   const long NN = (long)N * N;
   const size_t B = sizeof(cd) * (size_t)NN * batch;
-  DevBuf C1b, RC1b, C2b;
-  if (!save) {
-    C1b = c.ws.get(B);
-    RC1b = c.ws.get(B);
-    C2b = c.ws.get(B);
-  }
   DevBuf T1b = c.ws.get(B), T2b = c.ws.get(B);
   DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * batch), infb = c.ws.get(sizeof(int) * (size_t)batch);
-  cd* C1 = save ? save : C1b.as<cd>();
-  cd* RC1 = save ? save + (long)batch * NN : RC1b.as<cd>();
-  cd* C2 = save ? save + 2L * batch * NN : C2b.as<cd>();
+  cd* C1 = save;
+  cd* RC1 = save + (long)batch * NN;
+  cd* C2 = save + 2L * batch * NN;
   cd *T1 = T1b.as<cd>(), *T2 = T2b.as<cd>();
   fill_int(s, infb.as<int>(), 0, batch);
   // invC1 = I - rdu Rud ; C1 = invC1^{-1}  (src/core/RCWAIntegrator.cpp:57-60)
@@ -458,6 +453,54 @@ void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd
   zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, tuu, N, ss, C2, N, NN, ZERO, T1, N, NN, batch);
   mm(s, N, ONE, T1, S.Tuu, ZERO, T2, NN, batch);
   copy(s, N, N, T2, S.Tuu, NN, batch);
+}
+
+// Redheffer star product S <- S * s (src/core/RCWAIntegrator.cpp:57-72):
+//   C1 = (I - rdu Rud)^{-1}, C2 = (I - Rud rdu)^{-1},
+//   Rdu' = Rdu + Tdd rdu C2 Tuu, Tdd' = Tdd C1 tdd, Rud' = rud + tuu Rud C1 tdd, Tuu' = tuu C2 Tuu.
+// With Rud C1 = C2 Rud and C1 = I + rdu C2 Rud, everything follows from one factorization
+// of M = I - Rud rdu and the left solves [Yt | Yr] = M^{-1} [Tuu | Rud]:
+//   K = Yr tdd, Rud' = rud + tuu K, Tdd' = Tdd (tdd + rdu K), Rdu' = Rdu + Tdd (rdu Yt),
+//   Tuu' = tuu Yt
+// (8 products, one LU, 2N right-hand sides; the reference form needs 12 products, one LU
+// and an inverse).
+void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
+                  const cd* rdu, long ss, cd* save) {
+  StageScope st(ST_STAR);
+  if (save) {
+    star_product_explicit(c, N, batch, S, tdd, tuu, rud, rdu, ss, save);
+    return;
+  }
+  hipStream_t s = c.stream;
+  const long NN = (long)N * N;
+  const size_t B = sizeof(cd) * (size_t)NN * batch;
+  DevBuf Mb = c.ws.get(B), Yb = c.ws.get(2 * B), Kb = c.ws.get(B), T1b = c.ws.get(B);
+  DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * batch), infb = c.ws.get(sizeof(int) * (size_t)batch);
+  cd *M = Mb.as<cd>(), *Y = Yb.as<cd>(), *K = Kb.as<cd>(), *T1 = T1b.as<cd>();
+  cd* Yt = Y;
+  cd* Yr = Y + NN;
+  const long s2 = 2 * NN;
+  int* piv = pivb.as<int>();
+  fill_int(s, infb.as<int>(), 0, batch);
+  // M = I - Rud rdu
+  zset(s, N, N, ONE, ZERO, M, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, S.Rud, N, NN, rdu, N, ss, ONE, M, N, NN, batch);
+  zgetrf(s, N, M, N, NN, piv, N, infb.as<int>(), batch, c.ws);
+  zgetrs(s, N, N, M, N, NN, piv, N, S.Tuu, N, NN, Yt, N, s2, batch, c.ws);
+  zgetrs(s, N, N, M, N, NN, piv, N, S.Rud, N, NN, Yr, N, s2, batch, c.ws);
+  // K = Yr tdd ; Rud' = rud + tuu K
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, Yr, N, s2, tdd, N, ss, ZERO, K, N, NN, batch);
+  zgeam(s, N, N, ONE, rud, N, ss, ZERO, nullptr, 0, 0, S.Rud, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, tuu, N, ss, K, N, NN, ONE, S.Rud, N, NN, batch);
+  // T1 = tdd + rdu K ; Yr <- rdu Yt (Yr consumed)
+  zgeam(s, N, N, ONE, tdd, N, ss, ZERO, nullptr, 0, 0, T1, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, rdu, N, ss, K, N, NN, ONE, T1, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, rdu, N, ss, Yt, N, s2, ZERO, Yr, N, s2, batch);
+  // Rdu' = Rdu + Tdd (rdu Yt) ; Tdd' = Tdd T1 ; Tuu' = tuu Yt
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, S.Tdd, N, NN, Yr, N, s2, ONE, S.Rdu, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, S.Tdd, N, NN, T1, N, NN, ZERO, K, N, NN, batch);
+  copy(s, N, N, K, S.Tdd, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, tuu, N, ss, Yt, N, s2, ZERO, S.Tuu, N, NN, batch);
 }
 
 // ===================================================================== (d) projections
