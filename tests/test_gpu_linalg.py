@@ -74,6 +74,44 @@ def test_getrf_getrs(ctx, n):
     assert np.abs(from_dev_colmajor(dB) - ref).max() / np.abs(ref).max() < 1e-11
 
 
+@pytest.mark.parametrize("n", [130, 1030])
+def test_getrf_matches_lapack(ctx, n):
+    """Pivot sequence and factors equal LAPACK zgetrf's (scipy lu_factor). n = 130 runs the
+    register-resident panel (n <= 1024), n = 1030 the workgroup-per-panel kernel."""
+    import torch
+    from scipy.linalg import lu_factor
+    rng = np.random.default_rng(7 + n)
+    batch = 2
+    A = crand(rng, batch, n, n)
+    dA = to_dev_colmajor(A)
+    ipiv = torch.zeros((batch, n), dtype=torch.int32, device="cuda")
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    ctx.check(ctx.lib.vrc_zgetrf(ctx.h, n, ptr(dA), n, n * n, ptr(ipiv), n, ptr(info), batch), "getrf")
+    LU = from_dev_colmajor(dA)
+    piv = ipiv.cpu().numpy()
+    assert info.cpu().numpy().tolist() == [0] * batch
+    for b in range(batch):
+        lu_ref, piv_ref = lu_factor(A[b])
+        assert np.array_equal(piv[b], piv_ref), np.nonzero(piv[b] != piv_ref)[0][:5]
+        assert np.abs(LU[b] - lu_ref).max() <= 1e-11 * np.abs(lu_ref).max()
+
+
+def test_getrf_zero_column_info(ctx):
+    """A zero column inside a panel: info names it (1-based), as LAPACK's."""
+    import torch
+    from scipy.linalg import lapack
+    rng = np.random.default_rng(3)
+    n = 70
+    A = crand(rng, n, n)
+    A[:, 37] = 0.0
+    dA = to_dev_colmajor(A[None])
+    ipiv = torch.zeros((1, n), dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ctx.check(ctx.lib.vrc_zgetrf(ctx.h, n, ptr(dA), n, n * n, ptr(ipiv), n, ptr(info), 1), "getrf")
+    _, _, info_ref = lapack.zgetrf(A)
+    assert int(info.cpu()[0]) == info_ref == 38
+
+
 def test_getrf_singular_info(ctx):
     import torch
     n = 6

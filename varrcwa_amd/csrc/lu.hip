@@ -82,6 +82,125 @@ __global__ __launch_bounds__(256) void getrf_panel_kernel(int n, int k, int kb, 
   }
 }
 
+// Register-resident panel for n - k <= LUR_ROWS: thread t holds row k + t of the
+// kb <= LUR_NB panel columns. Per column: wave arg-max (first index of the largest
+// |re|+|im|), row exchange through LDS, elimination in registers. Afterwards the
+// panel's interchanges are applied to the other columns [K, K+KB) of the block and,
+// right of the panel, U12 = L11^{-1} A12 is solved (one thread per column).
+constexpr int LUR_NB = 16;
+constexpr int LUR_ROWS = 1024;
+
+__global__ __launch_bounds__(LUR_ROWS) void getrf_panel_reg_kernel(int n, int K, int KB, int k, int kb, cd* A,
+                                                                   long lda, long sA, int* ipiv, long sP, int* info) {
+  const long b = blockIdx.x;
+  A += b * sA;
+  ipiv += b * sP;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int r = k + t;
+  const bool own = r < n;
+  __shared__ double wv[LUR_ROWS / 64];
+  __shared__ int wi[LUR_ROWS / 64];
+  __shared__ cd prow[LUR_NB], crow[LUR_NB];
+  __shared__ cd L11[LUR_NB][LUR_NB + 1];
+  __shared__ int pvs[LUR_NB];
+  cd a[LUR_NB];
+#pragma unroll
+  for (int j = 0; j < LUR_NB; ++j) a[j] = (own && j < kb) ? A[r + (long)(k + j) * lda] : mk(0.0, 0.0);
+#pragma unroll
+  for (int j = 0; j < LUR_NB; ++j) {
+    if (j < kb) {
+      const int c = k + j;
+      double v = -1.0;
+      int vi = 0x7fffffff;
+      if (own && r >= c) {
+        const double m = cabs1(a[j]);
+        if (m > v) { v = m; vi = r; }  // NaN never wins
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o, 64);
+        const int oi = __shfl_xor(vi, o, 64);
+        if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
+      }
+      if (lane == 0) { wv[wave] = v; wi[wave] = vi; }
+      __syncthreads();
+      double best = wv[0];
+      int bi = wi[0];
+      for (int w = 1; w < LUR_ROWS / 64; ++w) {
+        const double ov = wv[w];
+        const int oi = wi[w];
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
+      const int p = (bi >= c && bi < n) ? bi : c;
+      if (t == 0) { ipiv[c] = p; pvs[j] = p; }
+      const bool nonsing = best > 0.0;
+      if (!nonsing && t == 0 && info && info[b] == 0) info[b] = c + 1;  // getf2 semantics: no elimination
+      if (nonsing && r == p) {
+#pragma unroll
+        for (int jj = 0; jj < LUR_NB; ++jj) prow[jj] = a[jj];
+      }
+      if (nonsing && r == c && p != c) {
+#pragma unroll
+        for (int jj = 0; jj < LUR_NB; ++jj) crow[jj] = a[jj];
+      }
+      __syncthreads();
+      if (nonsing) {
+        if (p != c && r == c) {
+#pragma unroll
+          for (int jj = 0; jj < LUR_NB; ++jj) a[jj] = prow[jj];
+        } else if (p != c && r == p) {
+#pragma unroll
+          for (int jj = 0; jj < LUR_NB; ++jj) a[jj] = crow[jj];
+        }
+        if (own && r > c) {
+          const cd l = cmul(a[j], crecip(prow[j]));
+          a[j] = l;
+#pragma unroll
+          for (int jj = 0; jj < LUR_NB; ++jj)
+            if (jj > j && jj < kb) a[jj] = csub(a[jj], cmul(l, prow[jj]));
+        }
+      }
+    }
+  }
+  if (own) {
+#pragma unroll
+    for (int j = 0; j < LUR_NB; ++j)
+      if (j < kb) A[r + (long)(k + j) * lda] = a[j];
+  }
+  if (t < kb) {
+#pragma unroll
+    for (int j = 0; j < LUR_NB; ++j) L11[t][j] = a[j];
+  }
+  __syncthreads();
+  // the block's other columns: interchanges, and U12 right of the panel
+  const int nleft = k - K, nother = KB - kb;
+  if (t < nother) {
+    const int col = t < nleft ? K + t : k + kb + (t - nleft);
+    cd* ac = A + (long)col * lda;
+    for (int j = 0; j < kb; ++j) {
+      const int p = pvs[j];
+      if (p != k + j) {
+        const cd tmp = ac[k + j];
+        ac[k + j] = ac[p];
+        ac[p] = tmp;
+      }
+    }
+    if (col >= k + kb) {
+      cd x[LUR_NB];
+#pragma unroll
+      for (int i = 0; i < LUR_NB; ++i) {
+        if (i < kb) {
+          cd xi = ac[k + i];
+#pragma unroll
+          for (int j = 0; j < LUR_NB; ++j)
+            if (j < i) xi = csub(xi, cmul(L11[i][j], x[j]));
+          x[i] = xi;
+          ac[k + i] = xi;
+        }
+      }
+    }
+  }
+}
+
 // Row interchanges ipiv[k0 .. k0+nswap) (in order) applied to columns [c0, c1); one
 // thread per column.
 __global__ void laswp_kernel(int k0, int nswap, cd* A, long lda, long sA, const int* ipiv, long sP, int c0, int c1) {
@@ -149,9 +268,21 @@ static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* 
 void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, int* info, int batch, Workspace& ws) {
   if (n <= 0 || batch <= 0) return;
   const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0);
+  const bool reg = n <= LUR_ROWS;
   for (int K = 0; K < n; K += LU_NB2) {
     const int KB = n - K < LU_NB2 ? n - K : LU_NB2;
-    for (int k = K; k < K + KB; k += LU_NB) {
+    if (reg) {
+      for (int k = K; k < K + KB; k += LUR_NB) {
+        const int kb = K + KB - k < LUR_NB ? K + KB - k : LUR_NB;
+        getrf_panel_reg_kernel<<<batch, LUR_ROWS, 0, s>>>(n, K, KB, k, kb, A, lda, sA, ipiv, sP, info);
+        launch_check("getrf_panel_reg");
+        const int c0 = k + kb, c1 = K + KB;
+        if (c1 > c0)
+          zgemm_strided(s, OP_N, OP_N, n - c0, c1 - c0, kb, mone, A + c0 + (long)k * lda, lda, sA,
+                        A + k + (long)c0 * lda, lda, sA, one, A + c0 + (long)c0 * lda, lda, sA, batch);
+      }
+    }
+    for (int k = K; k < K + KB && !reg; k += LU_NB) {
       const int kb = K + KB - k < LU_NB ? K + KB - k : LU_NB;
       getrf_panel_kernel<<<batch, 256, 0, s>>>(n, k, kb, A, lda, sA, ipiv, sP, info);
       launch_check("getrf_panel");
