@@ -47,6 +47,22 @@ def flops_qr(N):
     return (316.0 - 40.0 - 16.0 - 28.0) / 3.0 * N ** 3
 
 
+TRAFFIC_SUMMARY = ROOT / "profiles" / "pmc_traffic__hqr_kernel_rect_stack_N882_25inst.md"
+
+
+def hqr_traffic_per_matrix():
+    """HBM bytes per matrix of hqr_kernel (FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc,
+    each scaled by the counter's reading for a plain copy of known size), from the
+    committed profile summary of the 25-instance launch; None when it is absent."""
+    import re
+    try:
+        text = TRAFFIC_SUMMARY.read_text()
+    except OSError:
+        return None
+    m = re.search(r"hbm_bytes_per_matrix:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
 def load_instances(nh: int, count: int):
     from varrcwa_amd.api import Layer
     d = json.loads((ROOT / "tests" / "golden" / "metric_geometry.json").read_text())
@@ -154,6 +170,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--adjoint", type=int, default=21, help="instances in the adjoint timing (0 = skip)")
     ap.add_argument("--tune", action="append", default=[], help="kernel switch key=value (vrc_tune_set)")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="first copy this many bytes device to device (a plain copy kernel of known size "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -168,6 +187,15 @@ def main():
         print(*a, file=sys.stderr, flush=True)
 
     ctx = Context(0)
+    if args.copy_calibration > 0:
+        nel = args.copy_calibration // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {nel * 8} bytes read, {nel * 8} bytes written")
+        del src, dst
     for kv in args.tune:
         k, v = kv.split("=")
         if ctx.lib.vrc_tune_set(k.encode(), int(v)) != 0:
@@ -184,6 +212,8 @@ def main():
         torch.cuda.synchronize()
         log(f"warmup {w + 1}/{args.warmup} done")
     read_stages(ctx)  # reset
+    qs = (C.c_longlong * 6)()
+    ctx.check(ctx.lib.vrc_qr_stats(ctx.h, qs), "qr_stats")  # reset
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -192,8 +222,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st = read_stages(ctx)
-    qs = (C.c_longlong * 6)()
-    ctx.check(ctx.lib.vrc_qr_stats(ctx.h, qs), "qr_stats")
+    ctx.check(ctx.lib.vrc_qr_stats(ctx.h, qs), "qr_stats")  # the timed steps only
     dt = t1 - t0
     slices = args.steps * len(insts) * L
     value = slices / dt
@@ -203,16 +232,21 @@ def main():
     excl = {k: v[0] for k, v in st.items()}
     excl["pq"] = max(0.0, excl["pq"] - excl["fourier"])
     dom = max(excl, key=excl.get)
-    eigs_per_step = len(insts) * (L + 2)
     qr_ms, qr_cnt = st["qr"]
     roof = None
     if qr_cnt:
+        # dense eigensolves of the timed steps, counted by the QR kernel itself (the
+        # homogeneous end media take the closed form and are not among them)
+        mats_per_launch = qs[0] / qr_cnt
         avg_ms = qr_ms / qr_cnt
-        per_launch = flops_qr(N) * eigs_per_step * args.steps / qr_cnt
+        per_launch = flops_qr(N) * mats_per_launch
         ach = per_launch / (avg_ms * 1e-3) / 1e12
+        tpm = hqr_traffic_per_matrix()
         roof = {"bound": "mfma", "kernel": "hqr_kernel (multishift QR, Schur phase)", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
-                "avg_launch_ms": avg_ms, "flops_per_launch": per_launch}
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                "traffic": tpm * mats_per_launch if tpm else None,
+                "traffic_source": TRAFFIC_SUMMARY.relative_to(ROOT).as_posix() if tpm else None,
+                "avg_launch_ms": avg_ms, "flops_per_launch": per_launch, "matrices_per_launch": mats_per_launch}
 
     plain = None
     if args.plain > 0:

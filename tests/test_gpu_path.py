@@ -1,6 +1,6 @@
 """Parity of the HIP S-matrix path with the CPU oracle (restated reference CPU path).
 
-Tolerances (north-star bar): relative 1e-9 on basis-invariant S transforms and on
+Tolerances: relative 1e-9 on basis-invariant S transforms and on
 |S| of non-degenerate modes; Fourier/P/Q stages compared elementwise at 1e-12.
 """
 import numpy as np
