@@ -23,7 +23,7 @@ namespace vrc {
 
 constexpr double ULP = 2.220446049250313e-16;      // 2^-52
 constexpr double SAFMIN = 2.2250738585072014e-308;
-constexpr int HB = 32;         // Hessenberg panel width
+constexpr int HB = 64;         // Hessenberg panel width
 constexpr int PT = 1024;       // threads of the Hessenberg panel workgroup (16 waves)
 constexpr int QW = 64;         // maximal QR window
 constexpr int NBUL = 16;       // most bulges per sweep (2 shifts each)
