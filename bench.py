@@ -223,6 +223,7 @@ def main():
     t1 = time.perf_counter()
     st = read_stages(ctx)
     ctx.check(ctx.lib.vrc_qr_stats(ctx.h, qs), "qr_stats")  # the timed steps only
+    free_b, total_b = torch.cuda.mem_get_info()
     dt = t1 - t0
     slices = args.steps * len(insts) * L
     value = slices / dt
@@ -291,6 +292,7 @@ def main():
         "batched_over_cpu": (value / cpu["value"]) if cpu else None,
         "adjoint": adj,
         "stage_ms_per_step": {k: v / args.steps for k, v in excl.items()},
+        "device_memory_in_use_gb": (total_b - free_b) / 1e9,
         "dominant_stage": dom,
         "qr_stats": {"matrices": qs[0], "sweeps_per_matrix": qs[1] / max(qs[0], 1),
                      "windows_per_matrix": qs[2] / max(qs[0], 1),

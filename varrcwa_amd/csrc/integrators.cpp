@@ -629,8 +629,12 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     S.Tuu = S.Tdd + B * NN;
     S.Rud = S.Tdd + 2 * B * NN;
     S.Rdu = S.Tdd + 3 * B * NN;
-    init_identity_S(c, N, S, B);
-    for (int l = 0; l < L; ++l) {
+    // identity * slab 0 = slab 0: the cascade starts from the first slab's S-matrix
+    zgeam(st, N, N, ONE, tu.as<cd>(), N, (long)L * NN, ZERO, nullptr, 0, 0, S.Tdd, N, NN, B);
+    zgeam(st, N, N, ONE, tu.as<cd>(), N, (long)L * NN, ZERO, nullptr, 0, 0, S.Tuu, N, NN, B);
+    zgeam(st, N, N, ONE, ru.as<cd>(), N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rud, N, NN, B);
+    zgeam(st, N, N, ONE, ru.as<cd>(), N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rdu, N, NN, B);
+    for (int l = 1; l < L; ++l) {
       const cd* t = tu.as<cd>() + l * NN;
       const cd* r = ru.as<cd>() + l * NN;
       star_product(c, N, B, S, t, t, r, r, (long)L * NN);
