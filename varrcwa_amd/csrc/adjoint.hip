@@ -468,117 +468,109 @@ void wave_eqn_vjp(vrc_ctx& c, int N, int batch, const cd* P, const cd* Q, long s
 }
 
 // ===================================================================== (d) slab S
-// Forward of RCWAScatterMatrix::compute with explicit inverses, keeping what the
-// reverse pass needs.
+// Ebar[2b] = (rb + tb) / 2, Ebar[2b+1] = (rb - tb) / 2 (reverse of evenodd_combine).
+__global__ void evenodd_combine_bwd_kernel(long NN, const cd* tb, const cd* rb, cd* Eb) {
+  const long b = blockIdx.y;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < NN; i += (long)gridDim.x * blockDim.x) {
+    const cd t = tb[b * NN + i], r = rb[b * NN + i];
+    Eb[(2 * b) * NN + i] = mk(0.5 * (r.x + t.x), 0.5 * (r.y + t.y));
+    Eb[(2 * b + 1) * NN + i] = mk(0.5 * (r.x - t.x), 0.5 * (r.y - t.y));
+  }
+}
+
+// Reverse of evenodd_system: from Lbar, Rbar (interleaved as L, R) and x, with
+// a = Lb+ - Rb+, p = Lb+ + Rb+, q = Lb- - Rb-, d = Lb- + Rb-:
+//   F0b = conj(1-x) a + conj(1+x) q,  G0b = conj(1+x) p + conj(1-x) d,
+//   D1 = q - a, D2 = p - d  (xbar_i = sum_k conj(F0_ik) D1_ik + conj(G0_ik) D2_ik).
+__global__ void evenodd_system_bwd_kernel(int N, const cd* Lb, const cd* Rb, const cd* x, cd* F0b, cd* G0b, cd* D1,
+                                          cd* D2) {
+  const long b = blockIdx.z;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= N) return;
+  const long NN = (long)N * N, o = r + (long)c * N;
+  const cd lp = Lb[(2 * b) * NN + o], lm = Lb[(2 * b + 1) * NN + o];
+  const cd rp = Rb[(2 * b) * NN + o], rm = Rb[(2 * b + 1) * NN + o];
+  const cd av = csub(lp, rp), pv = cadd(lp, rp), qv = csub(lm, rm), dv = cadd(lm, rm);
+  const cd xr = x[b * N + r];
+  const cd omc = mk(1.0 - xr.x, xr.y), opc = mk(1.0 + xr.x, -xr.y);  // conj(1-x), conj(1+x)
+  F0b[b * NN + o] = cadd(cmul(omc, av), cmul(opc, qv));
+  G0b[b * NN + o] = cadd(cmul(opc, pv), cmul(omc, dv));
+  D1[b * NN + o] = csub(qv, av);
+  D2[b * NN + o] = csub(pv, dv);
+}
+
+// Forward of the slab S-matrix in the even/odd form of slab_evenodd (ops.hip), which
+// computes the quantities of RCWAScatterMatrix::compute (src/rcwa/RCWAScatterMatrix.cpp:
+// 26-85), with the inverses W^{-1}, V^{-1}, L^{-1} formed explicitly for the reverse pass.
 void layer_smatrix_fwd_saved(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
                              const cd* W0, const cd* V0, long sRef, SlabSaved& sv) {
   StageScope st(ST_ADJOINT);
   hipStream_t s = c.stream;
-  Ops o(s, N, batch);
+  Ops o(s, N, batch), o2(s, N, 2 * batch);
   const long NN = o.NN;
   const size_t M = sizeof(cd) * (size_t)NN * batch;
-  for (DevBuf* b : {&sv.Wi, &sv.Vi, &sv.F0, &sv.G0, &sv.A, &sv.B, &sv.Ai, &sv.H, &sv.Gm, &sv.F, &sv.Mi, &sv.Rud,
-                    &sv.Tuu})
-    *b = c.ws.get(M);
+  for (DevBuf* b : {&sv.Wi, &sv.Vi, &sv.F0, &sv.G0, &sv.Rud, &sv.Tuu}) *b = c.ws.get(M);
+  sv.Li = c.ws.get(2 * M);
+  sv.E = c.ws.get(2 * M);
   sv.x = c.ws.get(sizeof(cd) * (size_t)N * batch);
-  Buf XA(c, (size_t)NN * batch), XB(c, (size_t)NN * batch), t1(c, (size_t)NN * batch), t2(c, (size_t)NN * batch);
-  cd *Wi = sv.Wi.as<cd>(), *Vi = sv.Vi.as<cd>(), *F0 = sv.F0.as<cd>(), *G0 = sv.G0.as<cd>(), *A = sv.A.as<cd>(),
-     *B = sv.B.as<cd>(), *Ai = sv.Ai.as<cd>(), *H = sv.H.as<cd>(), *Gm = sv.Gm.as<cd>(), *F = sv.F.as<cd>(),
-     *Mi = sv.Mi.as<cd>(), *x = sv.x.as<cd>();
+  cd *Wi = sv.Wi.as<cd>(), *Vi = sv.Vi.as<cd>(), *F0 = sv.F0.as<cd>(), *G0 = sv.G0.as<cd>(), *Li = sv.Li.as<cd>(),
+     *E = sv.E.as<cd>(), *x = sv.x.as<cd>();
   invert(c, o, W, NN, Wi);
   invert(c, o, V, NN, Vi);
   o.mm(OP_N, OP_N, ONE, Wi, NN, W0, sRef, ZERO, F0, NN);
   o.mm(OP_N, OP_N, ONE, Vi, NN, V0, sRef, ZERO, G0, NN);
-  o.add(ONE, F0, NN, ONE, G0, NN, A, NN);
-  o.add(ONE, F0, NN, MONE, G0, NN, B, NN);
-  invert(c, o, A, NN, Ai);
   {
     dim3 g(cdiv_up(N, 256), batch);
     phase_fwd_kernel<<<g, 256, 0, s>>>(N, gam, d_s, x, N);
     launch_check("phase_fwd");
   }
-  zdscale(s, N, N, x, N, nullptr, 0, 0, A, N, NN, XA.p, N, NN, batch);
-  zdscale(s, N, N, x, N, nullptr, 0, 0, B, N, NN, XB.p, N, NN, batch);
-  o.mm(OP_N, OP_N, ONE, Ai, NN, B, NN, ZERO, H, NN);
-  o.mm(OP_N, OP_N, ONE, Ai, NN, XA.p, NN, ZERO, Gm, NN);
-  o.mm(OP_N, OP_N, ONE, Ai, NN, XB.p, NN, ZERO, F, NN);
-  o.eye(t1.p, NN);
-  o.mm(OP_N, OP_N, MONE, F, NN, F, NN, ONE, t1.p, NN);  // M = I - F F
-  invert(c, o, t1.p, NN, Mi);
-  o.copy(H, NN, t1.p, NN, MONE);
-  o.mm(OP_N, OP_N, ONE, F, NN, Gm, NN, ONE, t1.p, NN);  // R1 = F Gm - H
-  o.copy(Gm, NN, t2.p, NN);
-  o.mm(OP_N, OP_N, MONE, F, NN, H, NN, ONE, t2.p, NN);  // T1 = Gm - F H
-  o.mm(OP_N, OP_N, ONE, Mi, NN, t1.p, NN, ZERO, sv.Rud.as<cd>(), NN);
-  o.mm(OP_N, OP_N, ONE, Mi, NN, t2.p, NN, ZERO, sv.Tuu.as<cd>(), NN);
+  Buf Lm(c, 2 * (size_t)NN * batch), Rm(c, 2 * (size_t)NN * batch);
+  evenodd_system(s, N, batch, F0, NN, G0, NN, nullptr, x, Lm.p, Rm.p);
+  invert(c, o2, Lm.p, NN, Li);
+  o2.mm(OP_N, OP_N, ONE, Li, NN, Rm.p, NN, ZERO, E, NN);
+  evenodd_combine(s, N, batch, E, sv.Tuu.as<cd>(), sv.Rud.as<cd>());
 }
 
-// Reverse of layer_smatrix_fwd_saved (consumes sv: several buffers are reused).
+// Reverse of layer_smatrix_fwd_saved (consumes sv: several buffers are reused; Wi kept).
 void layer_smatrix_bwd_saved(vrc_ctx& c, int N, int batch, const double* d_s, SlabSaved& sv, const cd* tb,
                              const cd* rb, cd* Wb, cd* Vb, cd* gb) {
   StageScope st(ST_ADJOINT);
   hipStream_t s = c.stream;
-  Ops o(s, N, batch);
+  Ops o(s, N, batch), o2(s, N, 2 * batch);
   const long NN = o.NN;
   const size_t M = (size_t)NN * batch;
-  Buf t1(c, M), t2(c, M), t3(c, M), XA(c, M), XB(c, M);
-  Buf xb(c, (size_t)N * batch), cx(c, (size_t)N * batch);
-  cd *Wi = sv.Wi.as<cd>(), *Vi = sv.Vi.as<cd>(), *F0 = sv.F0.as<cd>(), *G0 = sv.G0.as<cd>(), *A = sv.A.as<cd>(),
-     *B = sv.B.as<cd>(), *Ai = sv.Ai.as<cd>(), *H = sv.H.as<cd>(), *Gm = sv.Gm.as<cd>(), *F = sv.F.as<cd>(),
-     *Mi = sv.Mi.as<cd>(), *x = sv.x.as<cd>(), *R = sv.Rud.as<cd>(), *Tt = sv.Tuu.as<cd>();
-  cd* R1b = t1.p;
-  cd* T1b = t2.p;
-  o.mm(OP_C, OP_N, ONE, Mi, NN, rb, NN, ZERO, R1b, NN);
-  o.mm(OP_C, OP_N, ONE, Mi, NN, tb, NN, ZERO, T1b, NN);
-  cd* Mb = Mi;  // M^{-1} no longer needed
-  o.mm(OP_N, OP_C, MONE, R1b, NN, R, NN, ZERO, Mb, NN);
-  o.mm(OP_N, OP_C, MONE, T1b, NN, Tt, NN, ONE, Mb, NN);
-  cd* Fb = R;    // Rud no longer needed
-  cd* Gmb = Tt;  // Tuu no longer needed
-  cd* Hb = t3.p;
-  // Fb = -Mb F^H - F^H Mb + R1b Gm^H - T1b H^H
-  o.mm(OP_N, OP_C, MONE, Mb, NN, F, NN, ZERO, Fb, NN);
-  o.mm(OP_C, OP_N, MONE, F, NN, Mb, NN, ONE, Fb, NN);
-  o.mm(OP_N, OP_C, ONE, R1b, NN, Gm, NN, ONE, Fb, NN);
-  o.mm(OP_N, OP_C, MONE, T1b, NN, H, NN, ONE, Fb, NN);
-  // Gmb = F^H R1b + T1b ; Hb = -R1b - F^H T1b
-  o.copy(T1b, NN, Gmb, NN);
-  o.mm(OP_C, OP_N, ONE, F, NN, R1b, NN, ONE, Gmb, NN);
-  o.copy(R1b, NN, Hb, NN, MONE);
-  o.mm(OP_C, OP_N, MONE, F, NN, T1b, NN, ONE, Hb, NN);
-  // [B | XA | XB]b = A^{-H} [Hb | Gmb | Fb]; Ab = -(Bb H^H + XAb Gm^H + XBb F^H)
-  cd* Bb = t1.p;   // R1b consumed
-  cd* XAb = t2.p;  // T1b consumed
-  cd* XBb = Mb;    // Mb consumed
-  o.mm(OP_C, OP_N, ONE, Ai, NN, Hb, NN, ZERO, Bb, NN);
-  o.mm(OP_C, OP_N, ONE, Ai, NN, Gmb, NN, ZERO, XAb, NN);
-  o.mm(OP_C, OP_N, ONE, Ai, NN, Fb, NN, ZERO, XBb, NN);
-  cd* Ab = Hb;
-  o.mm(OP_N, OP_C, MONE, Bb, NN, H, NN, ZERO, Ab, NN);
-  o.mm(OP_N, OP_C, MONE, XAb, NN, Gm, NN, ONE, Ab, NN);
-  o.mm(OP_N, OP_C, MONE, XBb, NN, F, NN, ONE, Ab, NN);
-  // phases: xb_i = sum_k conj(A_ik) XAb_ik + conj(B_ik) XBb_ik ; Ab += conj(X) XAb ; Bb += conj(X) XBb
+  cd *Wi = sv.Wi.as<cd>(), *Vi = sv.Vi.as<cd>(), *F0 = sv.F0.as<cd>(), *G0 = sv.G0.as<cd>(), *Li = sv.Li.as<cd>(),
+     *E = sv.E.as<cd>(), *x = sv.x.as<cd>();
+  Buf Eb(c, 2 * M), Rbar(c, 2 * M), xb(c, (size_t)N * batch);
+  {
+    dim3 g(cdiv_up(NN, 256) < 1024 ? cdiv_up(NN, 256) : 1024, batch);
+    evenodd_combine_bwd_kernel<<<g, 256, 0, s>>>(NN, tb, rb, Eb.p);
+    launch_check("evenodd_combine_bwd");
+  }
+  // E = L^{-1} R:  Rbar = L^{-H} Ebar ; Lbar = -Rbar E^H (into Eb)
+  o2.mm(OP_C, OP_N, ONE, Li, NN, Eb.p, NN, ZERO, Rbar.p, NN);
+  o2.mm(OP_N, OP_C, MONE, Rbar.p, NN, E, NN, ZERO, Eb.p, NN);
+  // F0b, G0b into E (consumed); D1, D2 into Li (consumed)
+  cd *F0b = E, *G0b = E + M, *D1 = Li, *D2 = Li + M;
+  {
+    dim3 g(cdiv_up(N, 256), N, batch);
+    evenodd_system_bwd_kernel<<<g, 256, 0, s>>>(N, Eb.p, Rbar.p, x, F0b, G0b, D1, D2);
+    launch_check("evenodd_system_bwd");
+  }
   {
     dim3 g(cdiv_up(N, 256), batch);
-    rowdot_kernel<<<g, 256, 0, s>>>(N, A, XAb, B, XBb, NN, xb.p, N);
+    rowdot_kernel<<<g, 256, 0, s>>>(N, F0, D1, G0, D2, NN, xb.p, N);
     launch_check("rowdot");
     phase_bwd_kernel<<<g, 256, 0, s>>>(N, x, xb.p, d_s, gb, N);
     launch_check("phase_bwd");
   }
-  o.vec(x, N, cx.p, N, VEC_CONJ);
-  zdscale(s, N, N, cx.p, N, nullptr, 0, 0, XAb, N, NN, XAb, N, NN, batch);
-  zdscale(s, N, N, cx.p, N, nullptr, 0, 0, XBb, N, NN, XBb, N, NN, batch);
-  o.add(ONE, Ab, NN, ONE, XAb, NN, Ab, NN);
-  o.add(ONE, Bb, NN, ONE, XBb, NN, Bb, NN);
-  // F0b = Ab + Bb ; G0b = Ab - Bb ; Wb = -W^{-H} F0b F0^H ; Vb = -V^{-H} G0b G0^H
-  cd* F0b = Gm;
-  cd* G0b = H;
-  o.add(ONE, Ab, NN, ONE, Bb, NN, F0b, NN);
-  o.add(ONE, Ab, NN, MONE, Bb, NN, G0b, NN);
-  o.mm(OP_N, OP_C, ONE, F0b, NN, F0, NN, ZERO, XA.p, NN);
-  o.mm(OP_C, OP_N, MONE, Wi, NN, XA.p, NN, ZERO, Wb, NN);
-  o.mm(OP_N, OP_C, ONE, G0b, NN, G0, NN, ZERO, XB.p, NN);
-  o.mm(OP_C, OP_N, MONE, Vi, NN, XB.p, NN, ZERO, Vb, NN);
+  // F0 = W^{-1} W0: Wb = -W^{-H} F0b F0^H ; likewise Vb
+  cd* T = Rbar.p;
+  o.mm(OP_N, OP_C, ONE, F0b, NN, F0, NN, ZERO, T, NN);
+  o.mm(OP_C, OP_N, MONE, Wi, NN, T, NN, ZERO, Wb, NN);
+  o.mm(OP_N, OP_C, ONE, G0b, NN, G0, NN, ZERO, T, NN);
+  o.mm(OP_C, OP_N, MONE, Vi, NN, T, NN, ZERO, Vb, NN);
 }
 
 void layer_smatrix_vjp(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
@@ -592,6 +584,9 @@ void layer_smatrix_vjp(vrc_ctx& c, int N, int batch, const double* d_s, const cd
 }
 
 // ===================================================================== (d) star product
+// Reverse of the factored star product (ops.hip, star_product):
+//   M = I - Rud rdu, [Yt | Yr] = M^{-1} [Tuu | Rud], K = Yr tdd, Rud' = rud + tuu K,
+//   T1 = tdd + rdu K, Z2 = rdu Yt, Rdu' = Rdu + Tdd Z2, Tdd' = Tdd T1, Tuu' = tuu Yt.
 void star_product_vjp(vrc_ctx& c, int N, int batch, const SMat& S0, const cd* tdd, const cd* tuu, const cd* rud,
                       const cd* rdu, long ss, SMat& Sb, cd* tddb, cd* tuub, cd* rudb, cd* rdub, long sb,
                       const cd* saved) {
@@ -600,64 +595,57 @@ void star_product_vjp(vrc_ctx& c, int N, int batch, const SMat& S0, const cd* td
   Ops o(s, N, batch);
   const long NN = o.NN;
   const size_t M = (size_t)NN * batch;
-  Buf C1o(c, saved ? 1 : M), RC1o(c, saved ? 1 : M), C2o(c, saved ? 1 : M);
-  Buf U1(c, M), U2(c, M), Wk(c, M), C1b(c, M), C2b(c, M), RC1b(c, M);
-  Buf Tddn(c, M), Tuun(c, M), Rudn(c, M);
-  const cd* C1 = saved ? saved : C1o.p;
-  const cd* RC1 = saved ? saved + M : RC1o.p;
-  const cd* C2 = saved ? saved + 2 * M : C2o.p;
+  Buf own(c, saved ? 1 : STAR_SAVE * M);
   if (!saved) {
-    // recompute C1 = (I - rdu Rud)^{-1}, RC1 = Rud C1, C2 = I + RC1 rdu
-    o.eye(Wk.p, NN);
-    o.mm(OP_N, OP_N, MONE, rdu, ss, S0.Rud, NN, ONE, Wk.p, NN);
-    invert(c, o, Wk.p, NN, C1o.p);
-    o.mm(OP_N, OP_N, ONE, S0.Rud, NN, C1o.p, NN, ZERO, RC1o.p, NN);
-    o.eye(C2o.p, NN);
-    o.mm(OP_N, OP_N, ONE, RC1o.p, NN, rdu, ss, ONE, C2o.p, NN);
+    // recompute the forward intermediates on a copy of the prefix
+    Buf S4(c, 4 * M);
+    SMat T;
+    T.Tdd = S4.p;
+    T.Tuu = S4.p + M;
+    T.Rud = S4.p + 2 * M;
+    T.Rdu = S4.p + 3 * M;
+    o.copy(S0.Tdd, NN, T.Tdd, NN);
+    o.copy(S0.Tuu, NN, T.Tuu, NN);
+    o.copy(S0.Rud, NN, T.Rud, NN);
+    o.copy(S0.Rdu, NN, T.Rdu, NN);
+    star_product(c, N, batch, T, tdd, tuu, rud, rdu, ss, own.p);
+    saved = own.p;
   }
-  // Rdu' = Rdu + (Tdd rdu) C2 Tuu
-  o.mm(OP_N, OP_N, ONE, S0.Tdd, NN, rdu, ss, ZERO, U1.p, NN);  // U1 = Tdd rdu
-  o.mm(OP_N, OP_N, ONE, U1.p, NN, C2, NN, ZERO, U2.p, NN);    // U2 = U1 C2
-  o.mm(OP_C, OP_N, ONE, U2.p, NN, Sb.Rdu, NN, ZERO, Tuun.p, NN);      // Tuub = U2^H Rdu'b
-  o.mm(OP_N, OP_C, ONE, Sb.Rdu, NN, S0.Tuu, NN, ZERO, Wk.p, NN);      // U2b = Rdu'b Tuu^H
-  o.mm(OP_C, OP_N, ONE, U1.p, NN, Wk.p, NN, ZERO, C2b.p, NN);         // C2b = U1^H U2b
-  o.mm(OP_N, OP_C, ONE, Wk.p, NN, C2, NN, ZERO, U2.p, NN);          // U1b = U2b C2^H (into U2)
-  o.mm(OP_N, OP_C, ONE, U2.p, NN, rdu, ss, ZERO, Tddn.p, NN);         // Tddb = U1b rdu^H
-  o.mm(OP_C, OP_N, ONE, S0.Tdd, NN, U2.p, NN, ZERO, rdub, sb);        // rdub = Tdd^H U1b
-  // Tdd' = (Tdd C1) tdd
-  o.mm(OP_N, OP_N, ONE, S0.Tdd, NN, C1, NN, ZERO, U1.p, NN);        // U3 = Tdd C1
-  o.mm(OP_C, OP_N, ONE, U1.p, NN, Sb.Tdd, NN, ZERO, tddb, sb);        // tddb = U3^H Tdd'b
-  o.mm(OP_N, OP_C, ONE, Sb.Tdd, NN, tdd, ss, ZERO, Wk.p, NN);         // U3b = Tdd'b tdd^H
-  o.mm(OP_N, OP_C, ONE, Wk.p, NN, C1, NN, ONE, Tddn.p, NN);         // Tddb += U3b C1^H
-  o.mm(OP_C, OP_N, ONE, S0.Tdd, NN, Wk.p, NN, ZERO, C1b.p, NN);       // C1b = Tdd^H U3b
-  // Rud' = rud + (tuu RC1) tdd
-  o.copy(Sb.Rud, NN, rudb, sb);                                        // rudb = Rud'b
-  o.mm(OP_N, OP_N, ONE, tuu, ss, RC1, NN, ZERO, U1.p, NN);          // U5 = tuu RC1
-  o.mm(OP_C, OP_N, ONE, U1.p, NN, Sb.Rud, NN, ONE, tddb, sb);         // tddb += U5^H Rud'b
-  o.mm(OP_N, OP_C, ONE, Sb.Rud, NN, tdd, ss, ZERO, Wk.p, NN);         // U5b = Rud'b tdd^H
-  o.mm(OP_C, OP_N, ONE, tuu, ss, Wk.p, NN, ZERO, RC1b.p, NN);         // RC1b = tuu^H U5b
-  o.mm(OP_N, OP_C, ONE, Wk.p, NN, RC1, NN, ZERO, tuub, sb);         // tuub = U5b RC1^H
-  // Tuu' = (tuu C2) Tuu
-  o.mm(OP_N, OP_N, ONE, tuu, ss, C2, NN, ZERO, U1.p, NN);           // U6 = tuu C2
-  o.mm(OP_C, OP_N, ONE, U1.p, NN, Sb.Tuu, NN, ONE, Tuun.p, NN);       // Tuub += U6^H Tuu'b
-  o.mm(OP_N, OP_C, ONE, Sb.Tuu, NN, S0.Tuu, NN, ZERO, Wk.p, NN);      // U6b = Tuu'b Tuu^H
-  o.mm(OP_N, OP_C, ONE, Wk.p, NN, C2, NN, ONE, tuub, sb);           // tuub += U6b C2^H
-  o.mm(OP_C, OP_N, ONE, tuu, ss, Wk.p, NN, ONE, C2b.p, NN);           // C2b += tuu^H U6b
-  // C2 = I + RC1 rdu
-  o.mm(OP_N, OP_C, ONE, C2b.p, NN, rdu, ss, ONE, RC1b.p, NN);         // RC1b += C2b rdu^H
-  o.mm(OP_C, OP_N, ONE, RC1, NN, C2b.p, NN, ONE, rdub, sb);         // rdub += RC1^H C2b
-  // RC1 = Rud C1
-  o.mm(OP_N, OP_C, ONE, RC1b.p, NN, C1, NN, ZERO, Rudn.p, NN);      // Rudb = RC1b C1^H
-  o.mm(OP_C, OP_N, ONE, S0.Rud, NN, RC1b.p, NN, ONE, C1b.p, NN);      // C1b += Rud^H RC1b
-  // C1 = D^{-1}, D = I - rdu Rud: Db = -C1^H C1b C1^H ; rdub -= Db Rud^H ; Rudb -= rdu^H Db
-  o.mm(OP_N, OP_C, ONE, C1b.p, NN, C1, NN, ZERO, Wk.p, NN);
-  o.mm(OP_C, OP_N, MONE, C1, NN, Wk.p, NN, ZERO, U1.p, NN);         // Db
-  o.mm(OP_N, OP_C, MONE, U1.p, NN, S0.Rud, NN, ONE, rdub, sb);
-  o.mm(OP_C, OP_N, MONE, rdu, ss, U1.p, NN, ONE, Rudn.p, NN);
-  // Rdub of the prefix is Rdu'b itself (unchanged)
-  o.copy(Tddn.p, NN, Sb.Tdd, NN);
-  o.copy(Tuun.p, NN, Sb.Tuu, NN);
-  o.copy(Rudn.p, NN, Sb.Rud, NN);
+  const cd *Mi = saved, *Yt = saved + M, *Yr = saved + 2 * M, *K = saved + 3 * M, *T1 = saved + 4 * M,
+           *Z2 = saved + 5 * M;
+  Buf Ytb(c, M), T1b(c, M), Z2b(c, M), Tn(c, M), Kb(c, M);
+  // Tuu' = tuu Yt
+  o.mm(OP_N, OP_C, ONE, Sb.Tuu, NN, Yt, NN, ZERO, tuub, sb);
+  o.mm(OP_C, OP_N, ONE, tuu, ss, Sb.Tuu, NN, ZERO, Ytb.p, NN);
+  // Tdd' = Tdd T1 ; Rdu' = Rdu + Tdd Z2 (Rdu's cotangent passes through unchanged)
+  o.mm(OP_C, OP_N, ONE, S0.Tdd, NN, Sb.Tdd, NN, ZERO, T1b.p, NN);
+  o.mm(OP_C, OP_N, ONE, S0.Tdd, NN, Sb.Rdu, NN, ZERO, Z2b.p, NN);
+  o.mm(OP_N, OP_C, ONE, Sb.Tdd, NN, T1, NN, ZERO, Tn.p, NN);
+  o.mm(OP_N, OP_C, ONE, Sb.Rdu, NN, Z2, NN, ONE, Tn.p, NN);
+  // Z2 = rdu Yt ; T1 = tdd + rdu K
+  o.mm(OP_C, OP_N, ONE, rdu, ss, Z2b.p, NN, ONE, Ytb.p, NN);
+  o.mm(OP_C, OP_N, ONE, rdu, ss, T1b.p, NN, ZERO, Kb.p, NN);
+  o.mm(OP_N, OP_C, ONE, T1b.p, NN, K, NN, ZERO, rdub, sb);
+  o.mm(OP_N, OP_C, ONE, Z2b.p, NN, Yt, NN, ONE, rdub, sb);
+  o.copy(T1b.p, NN, tddb, sb);
+  // Rud' = rud + tuu K
+  o.copy(Sb.Rud, NN, rudb, sb);
+  o.mm(OP_N, OP_C, ONE, Sb.Rud, NN, K, NN, ONE, tuub, sb);
+  o.mm(OP_C, OP_N, ONE, tuu, ss, Sb.Rud, NN, ONE, Kb.p, NN);
+  // K = Yr tdd
+  cd* Yrb = Z2b.p;  // consumed
+  o.mm(OP_N, OP_C, ONE, Kb.p, NN, tdd, ss, ZERO, Yrb, NN);
+  o.mm(OP_C, OP_N, ONE, Yr, NN, Kb.p, NN, ONE, tddb, sb);
+  // [Yt | Yr] = M^{-1} [Tuu | Rud]: [Tuub | Rudb] = M^{-H} [Ytb | Yrb], Mb = -[Tuub | Rudb] [Yt | Yr]^H
+  o.mm(OP_C, OP_N, ONE, Mi, NN, Ytb.p, NN, ZERO, Sb.Tuu, NN);
+  o.mm(OP_C, OP_N, ONE, Mi, NN, Yrb, NN, ZERO, Sb.Rud, NN);
+  cd* Mb = T1b.p;  // consumed
+  o.mm(OP_N, OP_C, MONE, Sb.Tuu, NN, Yt, NN, ZERO, Mb, NN);
+  o.mm(OP_N, OP_C, MONE, Sb.Rud, NN, Yr, NN, ONE, Mb, NN);
+  // M = I - Rud rdu
+  o.mm(OP_N, OP_C, MONE, Mb, NN, rdu, ss, ONE, Sb.Rud, NN);
+  o.mm(OP_C, OP_N, MONE, S0.Rud, NN, Mb, NN, ONE, rdub, sb);
+  o.copy(Tn.p, NN, Sb.Tdd, NN);
 }
 
 // ===================================================================== (d) left projection

@@ -771,7 +771,7 @@ void stack_tape_vjp(StackTape& t, const cd* Sb_in, cd* epsb, int chunk) {
     const cd* tuv = slab.Tuu.as<cd>();
     const cd* ruv = slab.Rud.as<cd>();
     DevBuf pre = c.ws.get(sizeof(cd) * 4 * NN * B * L);
-    DevBuf sstar = c.ws.get(sizeof(cd) * 3 * NN * B * L);
+    DevBuf sstar = c.ws.get(sizeof(cd) * STAR_SAVE * NN * B * L);
     DevBuf Sbuf = c.ws.get(sizeof(cd) * 4 * NN * B);
     auto blocks = [&](cd* base) {
       SMat m;
@@ -789,7 +789,7 @@ void stack_tape_vjp(StackTape& t, const cd* Sb_in, cd* epsb, int chunk) {
                              hipMemcpyDeviceToDevice, st));
       const cd* tl = tuv + l * NN;
       const cd* rl = ruv + l * NN;
-      star_product(c, N, B, S, tl, tl, rl, rl, ss, sstar.as<cd>() + (long)l * 3 * B * NN);
+      star_product(c, N, B, S, tl, tl, rl, rl, ss, sstar.as<cd>() + (long)l * STAR_SAVE * B * NN);
     }
     // ---- cotangents: projection, then the cascade in reverse
     DevBuf Sbb = c.ws.get(sizeof(cd) * 4 * NN * B);
@@ -811,7 +811,7 @@ void stack_tape_vjp(StackTape& t, const cd* Sb_in, cd* epsb, int chunk) {
         const cd* tl = tuv + l * NN;
         const cd* rl = ruv + l * NN;
         star_product_vjp(c, N, B, S0, tl, tl, rl, rl, ss, Sb, q0, q0 + B * NN, q0 + 2 * B * NN, q0 + 3 * B * NN, NN,
-                         sstar.as<cd>() + (long)l * 3 * B * NN);
+                         sstar.as<cd>() + (long)l * STAR_SAVE * B * NN);
         // the slab's Tdd and Tuu (Rud and Rdu) are one matrix: their cotangents add
         zgeam(st, N, N, ONE, q0, N, NN, ONE, q0 + B * NN, N, NN, tb.as<cd>() + l * NN, N, ss, B);
         zgeam(st, N, N, ONE, q0 + 2 * B * NN, N, NN, ONE, q0 + 3 * B * NN, N, NN, rb.as<cd>() + l * NN, N, ss, B);

@@ -235,6 +235,20 @@ __global__ void evenodd_combine_kernel(long NN, const cd* E, cd* Tuu, cd* Rud) {
   }
 }
 
+void evenodd_system(hipStream_t s, int N, int batch, const cd* F0, long sF, const cd* G, long sG, const cd* gam,
+                    const cd* x, cd* L, cd* R) {
+  dim3 g(cdiv_up(N, 256), N, batch);
+  evenodd_system_kernel<<<g, 256, 0, s>>>(N, F0, sF, G, sG, gam, x, L, R);
+  launch_check("evenodd_system");
+}
+
+void evenodd_combine(hipStream_t s, int N, int batch, const cd* E, cd* Tuu, cd* Rud) {
+  const long NN = (long)N * N;
+  dim3 g(cdiv_up(NN, 256) < 1024 ? cdiv_up(NN, 256) : 1024, batch);
+  evenodd_combine_kernel<<<g, 256, 0, s>>>(NN, E, Tuu, Rud);
+  launch_check("evenodd_combine");
+}
+
 // Slab S-matrix from F0 = W^{-1} W0 and G0 = V^{-1} V0 (G = Gamma G0 when gam_scale is
 // set). The reference (src/rcwa/RCWAScatterMatrix.cpp:26-85) computes, with A = F0 + G0,
 // B = F0 - G0, X = diag(exp(i gamma s)) and D = A - X B A^{-1} X B,
@@ -258,20 +272,11 @@ static void slab_evenodd(vrc_ctx& c, int N, int batch, const double* d_s, const 
   }
   DevBuf Lb = c.ws.get(B2), Eb = c.ws.get(B2);
   DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * 2 * batch), infb = c.ws.get(sizeof(int) * (size_t)2 * batch);
-  {
-    dim3 g(cdiv_up(N, 256), N, batch);
-    evenodd_system_kernel<<<g, 256, 0, s>>>(N, F0, sF, G, sG, gam_scale ? gam : nullptr, xb.as<cd>(), Lb.as<cd>(),
-                                            R);
-    launch_check("evenodd_system");
-  }
+  evenodd_system(s, N, batch, F0, sF, G, sG, gam_scale ? gam : nullptr, xb.as<cd>(), Lb.as<cd>(), R);
   fill_int(s, infb.as<int>(), 0, 2 * batch);
   zgetrf(s, N, Lb.as<cd>(), N, NN, pivb.as<int>(), N, infb.as<int>(), 2 * batch, c.ws);
   zgetrs(s, N, N, Lb.as<cd>(), N, NN, pivb.as<int>(), N, R, N, NN, Eb.as<cd>(), N, NN, 2 * batch, c.ws);
-  {
-    dim3 g(cdiv_up(NN, 256) < 1024 ? cdiv_up(NN, 256) : 1024, batch);
-    evenodd_combine_kernel<<<g, 256, 0, s>>>(NN, Eb.as<cd>(), Tuu, Rud);
-    launch_check("evenodd_combine");
-  }
+  evenodd_combine(s, N, batch, Eb.as<cd>(), Tuu, Rud);
 }
 
 void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
@@ -415,44 +420,39 @@ void homogeneous_eig(hipStream_t s, int n, int batch, const cd* alpha, const cd*
 }
 
 // ===================================================================== (d) star product
-// Reference arithmetic (src/core/RCWAIntegrator.cpp:57-72) with the explicit inverse C1;
-// used when the reverse pass needs C1, Rud C1 and C2 (save: 3 N x N per instance).
-static void star_product_explicit(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu,
-                                  const cd* rud, const cd* rdu, long ss, cd* save) {
+// The factored star product with M^{-1} formed explicitly, keeping the intermediates the
+// reverse pass needs (save: STAR_SAVE N x N per instance, in the order of vrc_ops.hpp).
+static void star_product_saved(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu,
+                               const cd* rud, const cd* rdu, long ss, cd* save) {
   hipStream_t s = c.stream;
   const long NN = (long)N * N;
-  const size_t B = sizeof(cd) * (size_t)NN * batch;
-  DevBuf T1b = c.ws.get(B), T2b = c.ws.get(B);
+  const long sv = (long)batch * NN;
+  cd *Mi = save, *Yt = save + sv, *Yr = save + 2 * sv, *K = save + 3 * sv, *T1 = save + 4 * sv,
+     *Z2 = save + 5 * sv;
+  DevBuf Mb = c.ws.get(sizeof(cd) * (size_t)sv);
   DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * batch), infb = c.ws.get(sizeof(int) * (size_t)batch);
-  cd* C1 = save;
-  cd* RC1 = save + (long)batch * NN;
-  cd* C2 = save + 2L * batch * NN;
-  cd *T1 = T1b.as<cd>(), *T2 = T2b.as<cd>();
+  cd* M = Mb.as<cd>();
   fill_int(s, infb.as<int>(), 0, batch);
-  // invC1 = I - rdu Rud ; C1 = invC1^{-1}  (src/core/RCWAIntegrator.cpp:57-60)
-  zset(s, N, N, ONE, ZERO, T1, N, NN, batch);
-  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, rdu, N, ss, S.Rud, N, NN, ONE, T1, N, NN, batch);
-  zgetrf(s, N, T1, N, NN, pivb.as<int>(), N, infb.as<int>(), batch, c.ws);
-  zgetri(s, N, T1, N, NN, pivb.as<int>(), N, C1, N, NN, batch, c.ws);
-  // RC1 = Rud C1 ; C2 = I + RC1 rdu
-  mm(s, N, ONE, S.Rud, C1, ZERO, RC1, NN, batch);
-  zset(s, N, N, ONE, ZERO, C2, N, NN, batch);
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, RC1, N, NN, rdu, N, ss, ONE, C2, N, NN, batch);
-  // Rdu += Tdd rdu C2 Tuu
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, S.Tdd, N, NN, rdu, N, ss, ZERO, T1, N, NN, batch);
-  mm(s, N, ONE, T1, C2, ZERO, T2, NN, batch);
-  mm(s, N, ONE, T2, S.Tuu, ONE, S.Rdu, NN, batch);
-  // Tdd = Tdd C1 tdd
-  mm(s, N, ONE, S.Tdd, C1, ZERO, T1, NN, batch);
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, T1, N, NN, tdd, N, ss, ZERO, S.Tdd, N, NN, batch);
-  // Rud = rud + tuu Rud C1 tdd
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, tuu, N, ss, RC1, N, NN, ZERO, T1, N, NN, batch);
+  // M = I - Rud rdu ; Mi = M^{-1}
+  zset(s, N, N, ONE, ZERO, M, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, S.Rud, N, NN, rdu, N, ss, ONE, M, N, NN, batch);
+  zgetrf(s, N, M, N, NN, pivb.as<int>(), N, infb.as<int>(), batch, c.ws);
+  zgetri(s, N, M, N, NN, pivb.as<int>(), N, Mi, N, NN, batch, c.ws);
+  // Yt = Mi Tuu ; Yr = Mi Rud ; K = Yr tdd
+  mm(s, N, ONE, Mi, S.Tuu, ZERO, Yt, NN, batch);
+  mm(s, N, ONE, Mi, S.Rud, ZERO, Yr, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, Yr, N, NN, tdd, N, ss, ZERO, K, N, NN, batch);
+  // Rud' = rud + tuu K ; T1 = tdd + rdu K ; Z2 = rdu Yt
   zgeam(s, N, N, ONE, rud, N, ss, ZERO, nullptr, 0, 0, S.Rud, N, NN, batch);
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, T1, N, NN, tdd, N, ss, ONE, S.Rud, N, NN, batch);
-  // Tuu = tuu C2 Tuu
-  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, tuu, N, ss, C2, N, NN, ZERO, T1, N, NN, batch);
-  mm(s, N, ONE, T1, S.Tuu, ZERO, T2, NN, batch);
-  copy(s, N, N, T2, S.Tuu, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, tuu, N, ss, K, N, NN, ONE, S.Rud, N, NN, batch);
+  zgeam(s, N, N, ONE, tdd, N, ss, ZERO, nullptr, 0, 0, T1, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, rdu, N, ss, K, N, NN, ONE, T1, N, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, rdu, N, ss, Yt, N, NN, ZERO, Z2, N, NN, batch);
+  // Rdu' = Rdu + Tdd Z2 ; Tdd' = Tdd T1 ; Tuu' = tuu Yt
+  mm(s, N, ONE, S.Tdd, Z2, ONE, S.Rdu, NN, batch);
+  mm(s, N, ONE, S.Tdd, T1, ZERO, M, NN, batch);
+  copy(s, N, N, M, S.Tdd, NN, batch);
+  zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, tuu, N, ss, Yt, N, NN, ZERO, S.Tuu, N, NN, batch);
 }
 
 // Redheffer star product S <- S * s (src/core/RCWAIntegrator.cpp:57-72):
@@ -468,7 +468,7 @@ void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd
                   const cd* rdu, long ss, cd* save) {
   StageScope st(ST_STAR);
   if (save) {
-    star_product_explicit(c, N, batch, S, tdd, tuu, rud, rdu, ss, save);
+    star_product_saved(c, N, batch, S, tdd, tuu, rud, rdu, ss, save);
     return;
   }
   hipStream_t s = c.stream;

@@ -25,10 +25,11 @@ void wave_eqn_vjp(vrc_ctx& c, int N, int batch, const cd* P, const cd* Q, long s
 void layer_smatrix_vjp(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
                        const cd* W0, const cd* V0, long sRef, const cd* tb, const cd* rb, cd* Wb, cd* Vb, cd* gb,
                        cd* Wi_out);
-// Forward intermediates of the slab S-matrix kept for its reverse pass (N x N per slab,
+// Forward intermediates of the slab S-matrix kept for its reverse pass (N x N per slab;
+// Li, E: the inverses of the two even/odd systems and their solutions, 2 per slab;
 // x: the N phases exp(i gamma s)); Tuu and Rud are the slab S-matrix itself.
 struct SlabSaved {
-  DevBuf Wi, Vi, F0, G0, A, B, Ai, H, Gm, F, Mi, Rud, Tuu, x;
+  DevBuf Wi, Vi, F0, G0, Li, E, Rud, Tuu, x;
 };
 void layer_smatrix_fwd_saved(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
                              const cd* W0, const cd* V0, long sRef, SlabSaved& sv);
@@ -36,7 +37,7 @@ void layer_smatrix_bwd_saved(vrc_ctx& c, int N, int batch, const double* d_s, Sl
                              const cd* rb, cd* Wb, cd* Vb, cd* gb);
 // star_product: S0 is the cascade before the product, Sb the cotangent after it on
 // entry and of S0 on return; slab cotangents are written with stride sb. saved
-// (optional): C1, Rud C1, C2 as kept by star_product(..., save).
+// (optional): the STAR_SAVE intermediates kept by star_product(..., save).
 void star_product_vjp(vrc_ctx& c, int N, int batch, const SMat& S0, const cd* tdd, const cd* tuu, const cd* rud,
                       const cd* rdu, long ss, SMat& Sb, cd* tddb, cd* tuub, cd* rudb, cd* rdub, long sb,
                       const cd* saved = nullptr);

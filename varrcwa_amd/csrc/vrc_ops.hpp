@@ -95,8 +95,18 @@ void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W,
 void layer_smatrix_factors(vrc_ctx& c, int N, int batch, const double* d_s, const cd* P, long sP, const cd* Z,
                            const cd* X, const cd* dbal, const cd* gam, const cd* W0, const cd* V0, long sRef,
                            cd* Tuu, cd* Rud);
+// Systems of the mirror-symmetric slab (see slab_evenodd in ops.hip): per instance b,
+// L[2b] = (1-x) F0 + (1+x) G0, R[2b] = (1+x) G0 - (1-x) F0, L[2b+1] = (1+x) F0 + (1-x) G0,
+// R[2b+1] = (1-x) G0 - (1+x) F0 (x = exp(i gamma s) scaling rows; G = Gamma G0 when gam is set).
+void evenodd_system(hipStream_t s, int N, int batch, const cd* F0, long sF, const cd* G, long sG, const cd* gam,
+                    const cd* x, cd* L, cd* R);
+// Rud = (E[2b] + E[2b+1]) / 2, Tuu = (E[2b] - E[2b+1]) / 2.
+void evenodd_combine(hipStream_t s, int N, int batch, const cd* E, cd* Tuu, cd* Rud);
 // Star product of src/core/RCWAIntegrator.cpp:57-67 (S updated in place; slice stride ss).
-// save (optional, 3 * batch * N*N): receives C1, Rud C1 and C2 for the reverse pass.
+// save (optional, STAR_SAVE * batch * N*N): receives M^{-1}, Yt, Yr, K, T1, Z2 of the
+// factored form (M = I - Rud rdu, [Yt | Yr] = M^{-1} [Tuu | Rud], K = Yr tdd,
+// T1 = tdd + rdu K, Z2 = rdu Yt) for the reverse pass.
+constexpr int STAR_SAVE = 6;
 void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
                   const cd* rdu, long ss, cd* save = nullptr);
 void project_left(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LU1, const cd* W0, const cd* V0, long sRef);
