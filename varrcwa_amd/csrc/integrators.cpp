@@ -610,11 +610,11 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
       if (!homog)
         wave_eqn_vectors(c, N, Q.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(), gall.as<cd>(), Wall.as<cd>(),
                          Vall.as<cd>(), 2 * B);
-      gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
+      if (!homog) gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);  // homogeneous: W1 = I
       gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
       layer_smatrix_factors(c, N, SL, dsv.as<double>(), P.as<cd>() + s0 * NN, NN, Zb.as<cd>() + s0 * NN,
                             Xb.as<cd>() + s0 * NN, Db.as<cd>() + s0 * N, gall.as<cd>() + (long)2 * B * N,
-                            Wr.as<cd>(), Vr.as<cd>(), NN, tu.as<cd>(), ru.as<cd>());
+                            Wr.as<cd>(), Vr.as<cd>(), NN, tu.as<cd>(), ru.as<cd>(), homog);
     }
     Wr = DevBuf();
     Vr = DevBuf();

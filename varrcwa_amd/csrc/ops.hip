@@ -350,6 +350,41 @@ void wave_eqn_vectors(vrc_ctx& c, int N, const cd* Q, long sQ, const cd* Z, cons
   zdscale(s, N, N, nullptr, 0, gam, N, 1, V, N, NN, V, N, NN, batch);
 }
 
+// Reference basis of a homogeneous medium (W0 = I, V0 coupling only harmonic pairs
+// (i, n+i): V0[k, c] != 0 only for k == c mod n). out = D^{-1} P V0 with two terms per
+// entry (d: the balancing diagonal of the slab).
+__global__ void pv0_pair_kernel(int n, const cd* P, long sP, const cd* V0, long sV, const cd* d, cd* out, long ldo,
+                                long so) {
+  const long b = blockIdx.z;
+  const int N = 2 * n;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= N) return;
+  const int k1 = c % n, k2 = k1 + n;
+  const cd* Pb = P + b * sP;
+  const cd* Vb = V0 + b * sV;
+  const cd v = cadd(cmul(Pb[r + (long)k1 * N], Vb[k1 + (long)c * N]), cmul(Pb[r + (long)k2 * N], Vb[k2 + (long)c * N]));
+  out[b * so + r + (long)c * ldo] = cdiv(v, d[b * N + r]);
+}
+
+// Y[i, j] = conj(Z[j, i]) / d[j] (Z^H D^{-1}), 32 x 32 tiles through LDS.
+__global__ __launch_bounds__(256) void ctrans_scale_kernel(int N, const cd* Z, long sZ, const cd* d, cd* Y, long ldy,
+                                                           long sY) {
+  const long b = blockIdx.z;
+  __shared__ cd tile[32][33];
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int row = j0 + tx, col = i0 + k;  // Z[row = j, col = i]
+    if (row < N && col < N) tile[k][tx] = cdiv(cconj(Z[b * sZ + row + (long)col * N]), d[b * N + row]);
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int i = i0 + tx, j = j0 + k;
+    if (i < N && j < N) Y[b * sY + i + (long)j * ldy] = tile[tx][k];
+  }
+}
+
 // (d) Slab S-matrix from the eigen factors, without forming W or V:
 //   W^{-1} = X^{-1} Z^H D^{-1}, and since PQ W = W Gamma^2 gives V = Q W / Gamma =
 //   P^{-1} W Gamma, V^{-1} V0 = Gamma^{-1} W^{-1} (P V0). So
@@ -357,13 +392,28 @@ void wave_eqn_vectors(vrc_ctx& c, int N, const cd* Q, long sQ, const cd* Z, cons
 // then the even/odd split of slab_evenodd.
 void layer_smatrix_factors(vrc_ctx& c, int N, int batch, const double* d_s, const cd* P, long sP, const cd* Z,
                            const cd* X, const cd* dbal, const cd* gam, const cd* W0, const cd* V0, long sRef,
-                           cd* Tuu, cd* Rud) {
+                           cd* Tuu, cd* Rud, bool ref_homog) {
   StageScope st(ST_LAYER_S);
   hipStream_t s = c.stream;
   const long NN = (long)N * N, s2 = 2 * NN;
   DevBuf Yb = c.ws.get(sizeof(cd) * (size_t)NN * 2 * batch);
   cd* Y = Yb.as<cd>();
-  {
+  if (ref_homog) {
+    // W0 = I: Y[:, :N] = Z^H D^{-1}; V0 pair-sparse: Y[:, N:] = Z^H (D^{-1} P V0)
+    DevBuf R2b = c.ws.get(sizeof(cd) * (size_t)NN * batch);
+    {
+      dim3 g(cdiv_up(N, 256), N, batch);
+      pv0_pair_kernel<<<g, 256, 0, s>>>(N / 2, P, sP, V0, sRef, dbal, R2b.as<cd>(), N, NN);
+      launch_check("pv0_pair");
+    }
+    {
+      dim3 g(cdiv_up(N, 32), cdiv_up(N, 32), batch);
+      ctrans_scale_kernel<<<g, 256, 0, s>>>(N, Z, NN, dbal, Y, N, s2);
+      launch_check("ctrans_scale");
+    }
+    zgemm_strided(s, OP_C, OP_N, N, N, N, ONE, Z, N, NN, R2b.as<cd>(), N, NN, ZERO, Y + NN, N, s2, batch);
+    ztrsm_upper(s, N, 2 * N, X, N, NN, Y, N, s2, batch, c.ws);
+  } else {
     DevBuf R2b = c.ws.get(sizeof(cd) * (size_t)NN * 2 * batch);
     cd* R2 = R2b.as<cd>();
     // R2 = D^{-1} [W0 | P V0]

@@ -91,10 +91,12 @@ void gather_mats(hipStream_t s, long nn, const cd* src, long ss, const int* d_id
 // Slab S-matrix (src/rcwa/RCWAScatterMatrix.cpp:26-85); d_s[b] = dz_b / k0_b.
 void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
                    const cd* W0, const cd* V0, long sRef, cd* Tuu, cd* Rud);
-// Slab S-matrix from the eigen factors of the slab (P per slab, stride sP).
+// Slab S-matrix from the eigen factors of the slab (P per slab, stride sP). ref_homog: the
+// reference basis is that of a homogeneous medium (W0 = I, V0 nonzero only where the row
+// and column harmonics agree), used without products by W0 and V0.
 void layer_smatrix_factors(vrc_ctx& c, int N, int batch, const double* d_s, const cd* P, long sP, const cd* Z,
                            const cd* X, const cd* dbal, const cd* gam, const cd* W0, const cd* V0, long sRef,
-                           cd* Tuu, cd* Rud);
+                           cd* Tuu, cd* Rud, bool ref_homog = false);
 // Systems of the mirror-symmetric slab (see slab_evenodd in ops.hip): per instance b,
 // L[2b] = (1-x) F0 + (1+x) G0, R[2b] = (1+x) G0 - (1-x) F0, L[2b+1] = (1+x) F0 + (1-x) G0,
 // R[2b+1] = (1-x) G0 - (1+x) F0 (x = exp(i gamma s) scaling rows; G = Gamma G0 when gam is set).
