@@ -51,7 +51,8 @@ int vrc_timing_enable(vrc_ctx* ctx, int on);
  * "zgemm.large_tile" (default 1): 128x64-tile ZGEMM for m >= 128, n >= 64, else the
  * 64x64 tile; "zgemm.large_tile_min_k" (default 65): smallest k for the 128x64 tile; "zgeev.aed_window" (default 24, 0 = off, at most 64): deflation-window
  * size of the aggressive early deflation in the QR iteration; "zgeev.bulges" (default 12,
- * 1..16): double-shift bulges chased per QR sweep.
+ * 1..16): double-shift bulges chased per QR sweep; "stack.cascade_group" (default 128):
+ * instances of vrc_batch_stack_rcwa whose star products and projections run in one batch.
  * Returns -1 for an unknown key. */
 int vrc_tune_set(const char* key, long long value);
 int vrc_timing_read(vrc_ctx* ctx, double* ms, int* counts, int nstages);

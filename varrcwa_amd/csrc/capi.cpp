@@ -75,6 +75,10 @@ int vrc_tune_set(const char* key, long long value) {
     zgeev_set_aed((int)value);
     return 0;
   }
+  if (std::strcmp(key, "stack.cascade_group") == 0) {
+    stack_set_cascade_group((int)value);
+    return 0;
+  }
   return -1;
 }
 

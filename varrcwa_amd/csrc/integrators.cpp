@@ -13,6 +13,10 @@ static constexpr double PI_REF = 3.1415926535897932;
 static const cd ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
 
 // Host -> device copy of a small array on the context stream (kept alive until done).
+// Instances whose star products and projections run in one batch (batch_stack_rcwa).
+static int g_cascade_group = 128;
+void stack_set_cascade_group(int n) { g_cascade_group = n < 1 ? 1 : n; }
+
 static DevBuf upload(vrc_ctx& c, const void* host, size_t bytes) {
   DevBuf b = c.ws.get(bytes);
   VRC_HIP(hipMemcpyAsync(b.ptr, host, bytes, hipMemcpyHostToDevice, c.stream));
@@ -513,7 +517,21 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
   const int nchunks = (ninst + chunk - 1) / chunk;
   std::vector<int> starts(nchunks + 1, 0);
   for (int ci = 0; ci < nchunks; ++ci) starts[ci + 1] = starts[ci] + ninst / nchunks + (ci < ninst % nchunks ? 1 : 0);
-  for (int ci = 0; ci < nchunks; ++ci) {
+  // Cascade groups: the slab S-matrices and end systems of consecutive chunks are kept
+  // and cascaded together, so that the star products and projections run batched over
+  // up to g_cascade_group instances (chunks split evenly over the groups).
+  const int gmax = std::max(1, g_cascade_group / std::max(1, chunk));
+  const int ngroups = (nchunks + gmax - 1) / gmax;
+  std::vector<int> gstart(ngroups + 1, 0);
+  for (int g = 0; g < ngroups; ++g) gstart[g + 1] = gstart[g] + nchunks / ngroups + (g < nchunks % ngroups ? 1 : 0);
+  for (int g = 0; g < ngroups; ++g) {
+  const int I0 = starts[gstart[g]];
+  const int GB = starts[gstart[g + 1]] - I0;
+  // group buffers: slab S [instance][slab], end systems [bottoms GB | tops GB]
+  DevBuf tuG = c.ws.get(sizeof(cd) * NN * GB * L), ruG = c.ws.get(sizeof(cd) * NN * GB * L);
+  DevBuf WeG = c.ws.get(sizeof(cd) * NN * 2 * GB), VeG = c.ws.get(sizeof(cd) * NN * 2 * GB),
+         geG = c.ws.get(sizeof(cd) * (size_t)N * 2 * GB);
+  for (int ci = gstart[g]; ci < gstart[g + 1]; ++ci) {
     const int i0 = starts[ci];
     const int B = starts[ci + 1] - i0;
     // ---- jobs: [bottoms(B) | tops(B) | slices(B*L, instance-major)]
@@ -573,7 +591,8 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
       homogeneous_eig(st, n, 2 * B, dab.as<cd>(), dab.as<cd>() + n, 2 * n, dk2.as<double>(), dev.as<cd>(),
                       Wall.as<cd>(), Vall.as<cd>(), gall.as<cd>());
     }
-    DevBuf tu = c.ws.get(sizeof(cd) * NN * SL), ru = c.ws.get(sizeof(cd) * NN * SL);
+    cd* tu = tuG.as<cd>() + (long)(i0 - I0) * L * NN;  // this chunk's slab S-matrices
+    cd* ru = ruG.as<cd>() + (long)(i0 - I0) * L * NN;
     DevBuf Wr = c.ws.get(sizeof(cd) * NN * SL), Vr = c.ws.get(sizeof(cd) * NN * SL);
     const cd* W0 = Wall.as<cd>();
     const cd* V0 = Vall.as<cd>();
@@ -600,7 +619,7 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
       gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
       gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
       layer_smatrix(c, N, SL, dsv.as<double>(), Wall.as<cd>() + 2 * B * NN, Vall.as<cd>() + 2 * B * NN,
-                    gall.as<cd>() + (long)2 * B * N, Wr.as<cd>(), Vr.as<cd>(), NN, tu.as<cd>(), ru.as<cd>());
+                    gall.as<cd>() + (long)2 * B * N, Wr.as<cd>(), Vr.as<cd>(), NN, tu, ru);
     } else {
       // slabs keep their eigenvectors factored (W = D Z X); only the end media form W, V
       DevBuf Zb = c.ws.get(sizeof(cd) * NN * JD), Xb = c.ws.get(sizeof(cd) * NN * JD),
@@ -614,15 +633,35 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
       gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
       layer_smatrix_factors(c, N, SL, dsv.as<double>(), P.as<cd>() + s0 * NN, NN, Zb.as<cd>() + s0 * NN,
                             Xb.as<cd>() + s0 * NN, Db.as<cd>() + s0 * N, gall.as<cd>() + (long)2 * B * N,
-                            Wr.as<cd>(), Vr.as<cd>(), NN, tu.as<cd>(), ru.as<cd>(), homog);
+                            Wr.as<cd>(), Vr.as<cd>(), NN, tu, ru, homog);
     }
     Wr = DevBuf();
     Vr = DevBuf();
     P = DevBuf();
     Q = DevBuf();
+    // end systems into the group buffers
+    const long og = i0 - I0;
+    zgeam(st, N, N, ONE, W0, N, NN, ZERO, nullptr, 0, 0, WeG.as<cd>() + og * NN, N, NN, B);
+    zgeam(st, N, N, ONE, V0, N, NN, ZERO, nullptr, 0, 0, VeG.as<cd>() + og * NN, N, NN, B);
+    zgeam(st, N, N, ONE, W1, N, NN, ZERO, nullptr, 0, 0, WeG.as<cd>() + (GB + og) * NN, N, NN, B);
+    zgeam(st, N, N, ONE, V1, N, NN, ZERO, nullptr, 0, 0, VeG.as<cd>() + (GB + og) * NN, N, NN, B);
+    zgeam(st, N, 1, ONE, gall.as<cd>(), N, N, ZERO, nullptr, 0, 0, geG.as<cd>() + og * N, N, N, B);
+    zgeam(st, N, 1, ONE, gall.as<cd>() + (long)B * N, N, N, ZERO, nullptr, 0, 0, geG.as<cd>() + (GB + og) * N, N, N,
+          B);
+  }
+  {
+    // ---- cascade over layers and the end projection, batched over the group's instances
+    const int B = GB, i0 = I0;
+    const cd* tu = tuG.as<cd>();
+    const cd* ru = ruG.as<cd>();
+    const cd* W0 = WeG.as<cd>();
+    const cd* V0 = VeG.as<cd>();
+    const cd* W1 = WeG.as<cd>() + (long)GB * NN;
+    const cd* V1 = VeG.as<cd>() + (long)GB * NN;
+    const cd* g0 = geG.as<cd>();
+    const cd* g1 = geG.as<cd>() + (long)GB * N;
     LUStore lu1;
     lu1.make(c, N, W1, V1, B, NN);
-    // ---- cascade over layers, batched over instances
     DevBuf Sb = c.ws.get(sizeof(cd) * 4 * NN * B);
     SMat S;
     S.Tdd = Sb.as<cd>();
@@ -630,13 +669,13 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     S.Rud = S.Tdd + 2 * B * NN;
     S.Rdu = S.Tdd + 3 * B * NN;
     // identity * slab 0 = slab 0: the cascade starts from the first slab's S-matrix
-    zgeam(st, N, N, ONE, tu.as<cd>(), N, (long)L * NN, ZERO, nullptr, 0, 0, S.Tdd, N, NN, B);
-    zgeam(st, N, N, ONE, tu.as<cd>(), N, (long)L * NN, ZERO, nullptr, 0, 0, S.Tuu, N, NN, B);
-    zgeam(st, N, N, ONE, ru.as<cd>(), N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rud, N, NN, B);
-    zgeam(st, N, N, ONE, ru.as<cd>(), N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rdu, N, NN, B);
+    zgeam(st, N, N, ONE, tu, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Tdd, N, NN, B);
+    zgeam(st, N, N, ONE, tu, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Tuu, N, NN, B);
+    zgeam(st, N, N, ONE, ru, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rud, N, NN, B);
+    zgeam(st, N, N, ONE, ru, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rdu, N, NN, B);
     for (int l = 1; l < L; ++l) {
-      const cd* t = tu.as<cd>() + l * NN;
-      const cd* r = ru.as<cd>() + l * NN;
+      const cd* t = tu + l * NN;
+      const cd* r = ru + l * NN;
       star_product(c, N, B, S, t, t, r, r, (long)L * NN);
     }
     project_left(c, N, B, S, lu1.pair, W0, V0, NN);
@@ -652,10 +691,11 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     };
     out(W0o, W0, NN);
     out(V0o, V0, NN);
-    out(g0o, gall.as<cd>(), N);
+    out(g0o, g0, N);
     out(W1o, W1, NN);
     out(V1o, V1, NN);
-    out(g1o, gall.as<cd>() + (long)B * N, N);
+    out(g1o, g1, N);
+  }
   }
 }
 

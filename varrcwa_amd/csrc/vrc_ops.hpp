@@ -88,6 +88,7 @@ struct RefSys {
 };
 
 void gather_mats(hipStream_t s, long nn, const cd* src, long ss, const int* d_idx, cd* dst, long sd, int batch);
+void stack_set_cascade_group(int n);  // instances per batched cascade of batch_stack_rcwa
 // Slab S-matrix (src/rcwa/RCWAScatterMatrix.cpp:26-85); d_s[b] = dz_b / k0_b.
 void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
                    const cd* W0, const cd* V0, long sRef, cd* Tuu, cd* Rud);
