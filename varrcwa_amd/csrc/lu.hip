@@ -227,6 +227,88 @@ static void laswp(hipStream_t s, int k0, int nswap, cd* A, long lda, long sA, co
   launch_check("laswp");
 }
 
+// Block interchanges as one row permutation. Plan (one workgroup per matrix): the rows
+// touched by ipiv[k0 .. k0+nswap) in order, with the source row of each (rows below
+// k0+nswap that end where they started are left out). Layout per matrix: count, then
+// LASWP_MAXROW destination rows, then their source rows.
+constexpr int LASWP_MAXROW = 256;  // 2 * the largest block
+
+__global__ __launch_bounds__(256) void laswp_plan_kernel(int n, int k0, int nswap, const int* ipiv, long sP,
+                                                         int* plan) {
+  const long b = blockIdx.x;
+  extern __shared__ int pm[];  // pm[r - k0]: original row now at row r
+  __shared__ int cnt;
+  const int t = threadIdx.x;
+  const int m = n - k0;
+  for (int i = t; i < m; i += 256) pm[i] = k0 + i;
+  if (t == 0) cnt = nswap;
+  __syncthreads();
+  if (t == 0) {
+    const int* pv = ipiv + b * sP;
+    for (int j = 0; j < nswap; ++j) {
+      const int p = pv[k0 + j] - k0;
+      const int tmp = pm[j];
+      pm[j] = pm[p];
+      pm[p] = tmp;
+    }
+  }
+  __syncthreads();
+  int* pl = plan + b * (1 + 2 * LASWP_MAXROW);
+  for (int i = t; i < nswap; i += 256) {
+    pl[1 + i] = k0 + i;
+    pl[1 + LASWP_MAXROW + i] = pm[i];
+  }
+  for (int i = nswap + t; i < m; i += 256) {
+    if (pm[i] != k0 + i) {
+      const int slot = atomicAdd(&cnt, 1);  // LDS atomic; at most nswap rows below
+      pl[1 + slot] = k0 + i;
+      pl[1 + LASWP_MAXROW + slot] = pm[i];
+    }
+  }
+  __syncthreads();
+  if (t == 0) pl[0] = cnt;
+}
+
+// Apply a plan to the columns [0, c_skip0) and [c_skip1, n): each workgroup takes
+// LASWP_CT columns, every thread one touched row (all reads of a column before its writes).
+constexpr int LASWP_CT = 16;
+
+__global__ __launch_bounds__(256) void laswp_apply_kernel(int n, int cs0, int cs1, cd* A, long lda, long sA,
+                                                          const int* plan) {
+  const long b = blockIdx.y;
+  const int* pl = plan + b * (1 + 2 * LASWP_MAXROW);
+  const int cnt = pl[0];
+  const int t = threadIdx.x;
+  const int ncols = n - (cs1 - cs0);
+  const int cb = blockIdx.x * LASWP_CT;
+  A += b * sA;
+  cd v[LASWP_CT];
+  int dst = 0;
+  if (t < cnt) {
+    dst = pl[1 + t];
+    const int src = pl[1 + LASWP_MAXROW + t];
+#pragma unroll
+    for (int q = 0; q < LASWP_CT; ++q) {
+      const int ci = cb + q;
+      if (ci < ncols) {
+        const int col = ci < cs0 ? ci : ci + (cs1 - cs0);
+        v[q] = A[src + (long)col * lda];
+      }
+    }
+  }
+  __syncthreads();
+  if (t < cnt) {
+#pragma unroll
+    for (int q = 0; q < LASWP_CT; ++q) {
+      const int ci = cb + q;
+      if (ci < ncols) {
+        const int col = ci < cs0 ? ci : ci + (cs1 - cs0);
+        A[dst + (long)col * lda] = v[q];
+      }
+    }
+  }
+}
+
 // U12 = L11^{-1} A12 for the kb x kb unit-lower panel block at (k, k), columns [c0, c1).
 __global__ __launch_bounds__(256) void trsm_panel_kernel(int k, int kb, cd* A, long lda, long sA, int c0, int c1) {
   const long b = blockIdx.y;
@@ -269,6 +351,7 @@ void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, 
   if (n <= 0 || batch <= 0) return;
   const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0);
   const bool reg = n <= LUR_ROWS;
+  DevBuf plan = ws.get(sizeof(int) * (size_t)(1 + 2 * LASWP_MAXROW) * batch);
   for (int K = 0; K < n; K += LU_NB2) {
     const int KB = n - K < LU_NB2 ? n - K : LU_NB2;
     if (reg) {
@@ -300,8 +383,16 @@ void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, 
       }
     }
     // the block's interchanges on the columns left and right of the block
-    laswp(s, K, KB, A, lda, sA, ipiv, sP, 0, K, batch);
-    laswp(s, K, KB, A, lda, sA, ipiv, sP, K + KB, n, batch);
+    if (n - K > 16000) {  // the plan's row map would not fit in LDS
+      laswp(s, K, KB, A, lda, sA, ipiv, sP, 0, K, batch);
+      laswp(s, K, KB, A, lda, sA, ipiv, sP, K + KB, n, batch);
+    } else if (n - KB > 0) {
+      laswp_plan_kernel<<<batch, 256, sizeof(int) * (size_t)(n - K), s>>>(n, K, KB, ipiv, sP, plan.as<int>());
+      launch_check("laswp_plan");
+      dim3 g(cdiv_up(n - KB, LASWP_CT), batch);
+      laswp_apply_kernel<<<g, 256, 0, s>>>(n, K, K + KB, A, lda, sA, plan.as<int>());
+      launch_check("laswp_apply");
+    }
     const int rest = n - K - KB;
     if (rest > 0) {
       trsm_lu_factor(s, 0, KB, rest, A + K + (long)K * lda, lda, sA, A + K + (long)(K + KB) * lda, lda, sA, batch,
