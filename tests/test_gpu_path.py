@@ -219,6 +219,22 @@ def test_batch_stack_parity(ctx):
     invariants_close(gpu_result(it), got, tol=1e-11)
 
 
+def test_batch_stack_chunks_and_groups(ctx):
+    """Launch chunks and cascade groups only regroup the batches: every instance's S is the
+    same (to rounding) for one chunk, chunks of 2 in one group, and chunks of 2 in groups
+    of 2 instances."""
+    nx = ny = 3
+    insts = [_stack_case(s) for s in range(7)]
+    ref = from_dev_colmajor(api.batch_stack_rcwa(ctx, nx, ny, insts)["S"])
+    try:
+        for group in (128, 2):
+            assert ctx.lib.vrc_tune_set(b"stack.cascade_group", group) == 0
+            got = from_dev_colmajor(api.batch_stack_rcwa(ctx, nx, ny, insts, chunk=2)["S"])
+            assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), group
+    finally:
+        ctx.lib.vrc_tune_set(b"stack.cascade_group", 128)
+
+
 def test_batch_stack_patterned_ends(ctx):
     """End media with several cells take the dense eigensolver path of the batch."""
     nx = ny = 3
