@@ -283,9 +283,12 @@ static void orghr(hipStream_t s, int n, const cd* Vf, const cd* Tall, cd* Z, lon
 // ===================================================================== LDS matrix products
 typedef cd LdsMat[QW][QW + 1];
 
-// acc = op(A) * B over K = 64 for the 64x64 output tile; CT selects op(A)(i,k) = conj(A[k][i]).
-template <bool CT>
-__device__ __forceinline__ void mm64(const LdsMat& As, const LdsMat& Bs, d4 (&accR)[2][2], d4 (&accI)[2][2]) {
+// acc = op(A) * B over K = 4 * ks (<= 64) for the 64x64 output tile; CT selects
+// op(A)(i,k) = conj(A[k][i]). Windows narrower than 64 (zero padded) take fewer k steps.
+template <bool CT, bool FULL = true>
+__device__ __forceinline__ void mm64(const LdsMat& As, const LdsMat& Bs, int ks, d4 (&accR)[2][2],
+                                     d4 (&accI)[2][2]) {
+  if (FULL) ks = QW / 4;  // compile-time trip count for the full window
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = wave >> 1, wc = wave & 1;
 #pragma unroll
@@ -295,7 +298,7 @@ __device__ __forceinline__ void mm64(const LdsMat& As, const LdsMat& Bs, d4 (&ac
       accR[i][j] = d4{0.0, 0.0, 0.0, 0.0};
       accI[i][j] = d4{0.0, 0.0, 0.0, 0.0};
     }
-  for (int s = 0; s < QW / 4; ++s) {
+  for (int s = 0; s < ks; ++s) {
     const int kr = 4 * s + (lane >> 4);
     double ar[2], ai[2], nai[2], br[2], bi[2];
 #pragma unroll
@@ -364,7 +367,10 @@ __device__ void apply_left_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, 
         pre[q] = (k < w && j0 + 4 * q < cwn) ? M[(r0 + k) + (long)(cn + j0 + 4 * q) * ld] : mk(0.0, 0.0);
     }
     d4 aR[2][2], aI[2][2];
-    mm64<true>(U, C, aR, aI);
+    if (w == QW)
+      mm64<true, true>(U, C, QW / 4, aR, aI);
+    else
+      mm64<true, false>(U, C, (w + 3) >> 2, aR, aI);
     mm64_for_each(aR, aI, [&](int i, int j, cd v) {
       if (i < w && j < cw) M[(r0 + i) + (long)(cc + j) * ld] = v;
     });
@@ -395,7 +401,10 @@ __device__ void apply_right_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld,
         pre[q] = (i < rhn && k0 + 4 * q < w) ? M[(rn + i) + (long)(c0 + k0 + 4 * q) * ld] : mk(0.0, 0.0);
     }
     d4 aR[2][2], aI[2][2];
-    mm64<false>(C, U, aR, aI);
+    if (w == QW)
+      mm64<false, true>(C, U, QW / 4, aR, aI);
+    else
+      mm64<false, false>(C, U, (w + 3) >> 2, aR, aI);
     mm64_for_each(aR, aI, [&](int ii, int j, cd v) {
       if (ii < rh && j < w) M[(rr + ii) + (long)(c0 + j) * ld] = v;
     });
@@ -1084,7 +1093,7 @@ __global__ __launch_bounds__(256) void trevc_kernel(int N, const cd* T, long ldt
     }
     __syncthreads();
     d4 aR[2][2], aI[2][2];
-    mm64<false>(Ts, Xs, aR, aI);
+    mm64<false>(Ts, Xs, QW / 4, aR, aI);
     mm64_for_each(aR, aI, [&](int i, int j, cd v) {
       if (j < bk) X[(rr + i) + (long)(k0 + j) * ldx] = cneg(v);
     });
@@ -1122,7 +1131,7 @@ __global__ __launch_bounds__(256) void trevc_kernel(int N, const cd* T, long ldt
       }
       __syncthreads();
       d4 aR[2][2], aI[2][2];
-      mm64<false>(Ts, Xs, aR, aI);
+      mm64<false>(Ts, Xs, QW / 4, aR, aI);
       mm64_for_each(aR, aI, [&](int i, int j, cd v) {
         if (j < bk) {
           cd* p = X + (rr + i) + (long)(k0 + j) * ldx;
