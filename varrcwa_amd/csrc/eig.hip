@@ -25,6 +25,7 @@ constexpr double ULP = 2.220446049250313e-16;      // 2^-52
 constexpr double SAFMIN = 2.2250738585072014e-308;
 constexpr int HB = 64;         // Hessenberg panel width
 constexpr int PT = 1024;       // threads of the Hessenberg panel workgroup (16 waves)
+constexpr int QT = 512;        // threads of the QR workgroup (8 waves; the window MFMAs use 4)
 constexpr int QW = 64;         // maximal QR window
 constexpr int NBUL = 16;       // most bulges per sweep (2 shifts each)
 constexpr int QR_NMIN = 48;    // active blocks this small are finished in LDS
@@ -346,9 +347,10 @@ __device__ __forceinline__ void mm64_for_each(const d4 (&accR)[2][2], const d4 (
 // The next 64-column chunk is fetched into registers while the current one is multiplied.
 __device__ void apply_left_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, int r0, int c0, int c1) {
   const int t = threadIdx.x;
+  const bool act = t < 256;           // the first four waves stage and multiply
   const int k = t & 63, j0 = t >> 6;  // element (k, j0 + 4q) of a chunk, q < 16
   cd pre[16];
-  if (c0 < c1) {
+  if (act && c0 < c1) {
     const int cw = c1 - c0 < QW ? c1 - c0 : QW;
 #pragma unroll
     for (int q = 0; q < 16; ++q)
@@ -356,33 +358,38 @@ __device__ void apply_left_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, 
   }
   for (int cc = c0; cc < c1; cc += QW) {
     const int cw = c1 - cc < QW ? c1 - cc : QW;
+    if (act) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) C[k][j0 + 4 * q] = pre[q];
-    __syncthreads();
-    const int cn = cc + QW;
-    if (cn < c1) {
-      const int cwn = c1 - cn < QW ? c1 - cn : QW;
-#pragma unroll
-      for (int q = 0; q < 16; ++q)
-        pre[q] = (k < w && j0 + 4 * q < cwn) ? M[(r0 + k) + (long)(cn + j0 + 4 * q) * ld] : mk(0.0, 0.0);
+      for (int q = 0; q < 16; ++q) C[k][j0 + 4 * q] = pre[q];
     }
-    d4 aR[2][2], aI[2][2];
-    if (w == QW)
-      mm64<true, true>(U, C, QW / 4, aR, aI);
-    else
-      mm64<true, false>(U, C, (w + 3) >> 2, aR, aI);
-    mm64_for_each(aR, aI, [&](int i, int j, cd v) {
-      if (i < w && j < cw) M[(r0 + i) + (long)(cc + j) * ld] = v;
-    });
+    __syncthreads();
+    if (act) {
+      const int cn = cc + QW;
+      if (cn < c1) {
+        const int cwn = c1 - cn < QW ? c1 - cn : QW;
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+          pre[q] = (k < w && j0 + 4 * q < cwn) ? M[(r0 + k) + (long)(cn + j0 + 4 * q) * ld] : mk(0.0, 0.0);
+      }
+      d4 aR[2][2], aI[2][2];
+      if (w == QW)
+        mm64<true, true>(U, C, QW / 4, aR, aI);
+      else
+        mm64<true, false>(U, C, (w + 3) >> 2, aR, aI);
+      mm64_for_each(aR, aI, [&](int i, int j, cd v) {
+        if (i < w && j < cw) M[(r0 + i) + (long)(cc + j) * ld] = v;
+      });
+    }
     __syncthreads();
   }
 }
 // M[r0:r1, c0:c0+w] <- M[...] * U, 64-row chunks, next chunk prefetched into registers.
 __device__ void apply_right_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, int r0, int r1, int c0) {
   const int t = threadIdx.x;
+  const bool act = t < 256;           // the first four waves stage and multiply
   const int i = t & 63, k0 = t >> 6;  // element (i, k0 + 4q) of a chunk, q < 16
   cd pre[16];
-  if (r0 < r1) {
+  if (act && r0 < r1) {
     const int rh = r1 - r0 < QW ? r1 - r0 : QW;
 #pragma unroll
     for (int q = 0; q < 16; ++q)
@@ -390,24 +397,28 @@ __device__ void apply_right_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld,
   }
   for (int rr = r0; rr < r1; rr += QW) {
     const int rh = r1 - rr < QW ? r1 - rr : QW;
+    if (act) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) C[i][k0 + 4 * q] = pre[q];
-    __syncthreads();
-    const int rn = rr + QW;
-    if (rn < r1) {
-      const int rhn = r1 - rn < QW ? r1 - rn : QW;
-#pragma unroll
-      for (int q = 0; q < 16; ++q)
-        pre[q] = (i < rhn && k0 + 4 * q < w) ? M[(rn + i) + (long)(c0 + k0 + 4 * q) * ld] : mk(0.0, 0.0);
+      for (int q = 0; q < 16; ++q) C[i][k0 + 4 * q] = pre[q];
     }
-    d4 aR[2][2], aI[2][2];
-    if (w == QW)
-      mm64<false, true>(C, U, QW / 4, aR, aI);
-    else
-      mm64<false, false>(C, U, (w + 3) >> 2, aR, aI);
-    mm64_for_each(aR, aI, [&](int ii, int j, cd v) {
-      if (ii < rh && j < w) M[(rr + ii) + (long)(c0 + j) * ld] = v;
-    });
+    __syncthreads();
+    if (act) {
+      const int rn = rr + QW;
+      if (rn < r1) {
+        const int rhn = r1 - rn < QW ? r1 - rn : QW;
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+          pre[q] = (i < rhn && k0 + 4 * q < w) ? M[(rn + i) + (long)(c0 + k0 + 4 * q) * ld] : mk(0.0, 0.0);
+      }
+      d4 aR[2][2], aI[2][2];
+      if (w == QW)
+        mm64<false, true>(C, U, QW / 4, aR, aI);
+      else
+        mm64<false, false>(C, U, (w + 3) >> 2, aR, aI);
+      mm64_for_each(aR, aI, [&](int ii, int j, cd v) {
+        if (ii < rh && j < w) M[(rr + ii) + (long)(c0 + j) * ld] = v;
+      });
+    }
     __syncthreads();
   }
 }
@@ -419,8 +430,7 @@ struct QrLds {
   cd rv1[NBUL], rv2[NBUL], rtau[NBUL];
   double rbeta[NBUL];
   int rsize[NBUL], rrow[NBUL], rcol[NBUL];
-  double redv[256];
-  int redi[256];
+  int redi[QT];
   cd aedv[QW];
   int ctl[4];
 };
@@ -737,7 +747,7 @@ __device__ int aed(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int lo, in
 }
 
 __device__ void lds_load_block(QrLds& L, const cd* H, long ldh, int r0, int w) {
-  for (int e = threadIdx.x; e < QW * QW; e += 256) {
+  for (int e = threadIdx.x; e < QW * QW; e += QT) {
     int i = e & 63, j = e >> 6;
     L.Hw[i][j] = (i < w && j < w) ? H[(r0 + i) + (long)(r0 + j) * ldh] : mk(0.0, 0.0);
     L.U[i][j] = (i == j && i < w) ? mk(1.0, 0.0) : mk(0.0, 0.0);
@@ -747,7 +757,7 @@ __device__ void lds_load_block(QrLds& L, const cd* H, long ldh, int r0, int w) {
 
 __device__ void lds_store_block(QrLds& L, cd* H, long ldh, int r0, int w) {
   __syncthreads();
-  for (int e = threadIdx.x; e < w * w; e += 256) {
+  for (int e = threadIdx.x; e < w * w; e += QT) {
     int i = e % w, j = e / w;
     H[(r0 + i) + (long)(r0 + j) * ldh] = L.Hw[i][j];
   }
@@ -767,7 +777,7 @@ __device__ int find_lo(QrLds& L, cd* H, long ldh, int N, int hi) {
   const int t = threadIdx.x;
   const double smlnum = SAFMIN * ((double)N / ULP);
   int best = 0;
-  for (int k = hi - t; k >= 1; k -= 256) {
+  for (int k = hi - t; k >= 1; k -= QT) {
     const cd hs = H[k + (long)(k - 1) * ldh];
     const double ahs = cabs1(hs);
     bool ng = ahs <= smlnum;
@@ -794,7 +804,7 @@ __device__ int find_lo(QrLds& L, cd* H, long ldh, int N, int hi) {
   }
   L.redi[t] = best;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = QT / 2; w > 0; w >>= 1) {
     if (t < w && L.redi[t + w] > L.redi[t]) L.redi[t] = L.redi[t + w];
     __syncthreads();
   }
@@ -868,7 +878,7 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
       }
       __syncthreads();
       // row operations: items (bulge, column), spread over the workgroup
-      for (int it = t; it < nbul * w; it += 256) {
+      for (int it = t; it < nbul * w; it += QT) {
         const int bb = it / w, j = it - bb * w;
         const int sz = L.rsize[bb];
         if (sz == 0) continue;
@@ -891,7 +901,7 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
       }
       __syncthreads();
       // column operations on the window and on U: items (matrix, bulge, row)
-      for (int it = t; it < 2 * nbul * w; it += 256) {
+      for (int it = t; it < 2 * nbul * w; it += QT) {
         const int mat = it / (nbul * w);
         const int rem = it - mat * nbul * w;
         const int bb = rem / w, i = rem - bb * w;
@@ -925,7 +935,7 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
 // small-block ticks} of the constant-rate device clock.
 // use_aed: deflation-window size (0 = no aggressive early deflation); max_bul: bulges
 // per sweep (<= NBUL).
-__global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long sH, cd* Z, long ldz, long sZ,
+__global__ __launch_bounds__(QT) void hqr_kernel(int N, cd* H, long ldh, long sH, cd* Z, long ldz, long sZ,
                                                   cd* wout, long sw, int* info, long long* stats, int use_aed,
                                                   int max_bul) {
   long long st[5] = {0, 0, 0, 0, 0};
@@ -1014,7 +1024,7 @@ __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long s
       __syncthreads();
     } else {
       const int r0 = ihi - ns + 1;
-      for (int e = t; e < QW * QW; e += 256) {
+      for (int e = t; e < QW * QW; e += QT) {
         int i = e & 63, j = e >> 6;
         L.Hw[i][j] = (i < ns && j < ns) ? H[(r0 + i) + (long)(r0 + j) * ldh] : mk(0.0, 0.0);
       }
@@ -1028,7 +1038,7 @@ __global__ __launch_bounds__(256) void hqr_kernel(int N, cd* H, long ldh, long s
     ++sweeps;
   }
   __syncthreads();
-  for (int i = t; i < N; i += 256) wout[i] = H[i + (long)i * ldh];
+  for (int i = t; i < N; i += QT) wout[i] = H[i + (long)i * ldh];
   if (t == 0) {
     info[b] = fail;
     if (stats) {
@@ -1297,7 +1307,7 @@ void zgeev_factors(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
     const bool want = c && c->timing;
     DevBuf sb;
     if (want) sb = ws.get(sizeof(long long) * 5 * batch);
-    hqr_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, want ? sb.as<long long>() : nullptr,
+    hqr_kernel<<<batch, QT, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, want ? sb.as<long long>() : nullptr,
                                      g_aed_window, g_bulges);
     launch_check("hqr");
     if (want) {
