@@ -343,82 +343,123 @@ __device__ __forceinline__ void mm64_for_each(const d4 (&accR)[2][2], const d4 (
       }
 }
 
-// M[r0:r0+w, c0:c1] <- U^H * M[...]  (U: w x w in LDS, zero padded), chunk buffer C.
-// The next 64-column chunk is fetched into registers while the current one is multiplied.
+// The same product on eight waves (QR workgroup): wave w owns rows 32*(w>>2) .. +32 and
+// columns 16*(w&3) .. +16 of the 64x64 output (two 16x16 blocks).
+static_assert(QT == 512, "the QR window multiply and its staging assume eight waves");
+template <bool CT, bool FULL>
+__device__ __forceinline__ void mm64w8(const LdsMat& As, const LdsMat& Bs, int ks, d4 (&accR)[2], d4 (&accI)[2]) {
+  if (FULL) ks = QW / 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 2, wc = wave & 3;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    accR[i] = d4{0.0, 0.0, 0.0, 0.0};
+    accI[i] = d4{0.0, 0.0, 0.0, 0.0};
+  }
+  for (int s = 0; s < ks; ++s) {
+    const int kr = 4 * s + (lane >> 4);
+    double ar[2], ai[2], nai[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int i = wr * 32 + q * 16 + (lane & 15);
+      cd a = CT ? cconj(As[kr][i]) : As[i][kr];
+      ar[q] = a.x;
+      ai[q] = a.y;
+      nai[q] = -a.y;
+    }
+    const cd bb = Bs[kr][wc * 16 + (lane & 15)];
+    const double br = bb.x, bi = bb.y;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      accR[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br, accR[i], 0, 0, 0);
+      accR[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi, accR[i], 0, 0, 0);
+      accI[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi, accI[i], 0, 0, 0);
+      accI[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br, accI[i], 0, 0, 0);
+    }
+  }
+}
+
+template <class F>
+__device__ __forceinline__ void mm64w8_for_each(const d4 (&accR)[2], const d4 (&accI)[2], F f) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 2, wc = wave & 3;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ii = wr * 32 + i * 16 + (lane >> 4) + 4 * r;
+      const int jj = wc * 16 + (lane & 15);
+      f(ii, jj, mk(accR[i][r], accI[i][r]));
+    }
+}
+
+// M[r0:r0+w, c0:c1] <- U^H * M[...]  (U: w x w in LDS, zero padded), chunk buffer C;
+// QT threads. The next 64-column chunk is fetched into registers while the current one
+// is multiplied.
 __device__ void apply_left_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, int r0, int c0, int c1) {
   const int t = threadIdx.x;
-  const bool act = t < 256;           // the first four waves stage and multiply
-  const int k = t & 63, j0 = t >> 6;  // element (k, j0 + 4q) of a chunk, q < 16
-  cd pre[16];
-  if (act && c0 < c1) {
+  const int k = t & 63, j0 = t >> 6;  // element (k, j0 + 8q) of a chunk, q < 8
+  cd pre[8];
+  if (c0 < c1) {
     const int cw = c1 - c0 < QW ? c1 - c0 : QW;
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
-      pre[q] = (k < w && j0 + 4 * q < cw) ? M[(r0 + k) + (long)(c0 + j0 + 4 * q) * ld] : mk(0.0, 0.0);
+    for (int q = 0; q < 8; ++q)
+      pre[q] = (k < w && j0 + 8 * q < cw) ? M[(r0 + k) + (long)(c0 + j0 + 8 * q) * ld] : mk(0.0, 0.0);
   }
   for (int cc = c0; cc < c1; cc += QW) {
     const int cw = c1 - cc < QW ? c1 - cc : QW;
-    if (act) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) C[k][j0 + 4 * q] = pre[q];
-    }
+    for (int q = 0; q < 8; ++q) C[k][j0 + 8 * q] = pre[q];
     __syncthreads();
-    if (act) {
-      const int cn = cc + QW;
-      if (cn < c1) {
-        const int cwn = c1 - cn < QW ? c1 - cn : QW;
+    const int cn = cc + QW;
+    if (cn < c1) {
+      const int cwn = c1 - cn < QW ? c1 - cn : QW;
 #pragma unroll
-        for (int q = 0; q < 16; ++q)
-          pre[q] = (k < w && j0 + 4 * q < cwn) ? M[(r0 + k) + (long)(cn + j0 + 4 * q) * ld] : mk(0.0, 0.0);
-      }
-      d4 aR[2][2], aI[2][2];
-      if (w == QW)
-        mm64<true, true>(U, C, QW / 4, aR, aI);
-      else
-        mm64<true, false>(U, C, (w + 3) >> 2, aR, aI);
-      mm64_for_each(aR, aI, [&](int i, int j, cd v) {
-        if (i < w && j < cw) M[(r0 + i) + (long)(cc + j) * ld] = v;
-      });
+      for (int q = 0; q < 8; ++q)
+        pre[q] = (k < w && j0 + 8 * q < cwn) ? M[(r0 + k) + (long)(cn + j0 + 8 * q) * ld] : mk(0.0, 0.0);
     }
+    d4 aR[2], aI[2];
+    if (w == QW)
+      mm64w8<true, true>(U, C, QW / 4, aR, aI);
+    else
+      mm64w8<true, false>(U, C, (w + 3) >> 2, aR, aI);
+    mm64w8_for_each(aR, aI, [&](int i, int j, cd v) {
+      if (i < w && j < cw) M[(r0 + i) + (long)(cc + j) * ld] = v;
+    });
     __syncthreads();
   }
 }
 // M[r0:r1, c0:c0+w] <- M[...] * U, 64-row chunks, next chunk prefetched into registers.
 __device__ void apply_right_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld, int r0, int r1, int c0) {
   const int t = threadIdx.x;
-  const bool act = t < 256;           // the first four waves stage and multiply
-  const int i = t & 63, k0 = t >> 6;  // element (i, k0 + 4q) of a chunk, q < 16
-  cd pre[16];
-  if (act && r0 < r1) {
+  const int i = t & 63, k0 = t >> 6;  // element (i, k0 + 8q) of a chunk, q < 8
+  cd pre[8];
+  if (r0 < r1) {
     const int rh = r1 - r0 < QW ? r1 - r0 : QW;
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
-      pre[q] = (i < rh && k0 + 4 * q < w) ? M[(r0 + i) + (long)(c0 + k0 + 4 * q) * ld] : mk(0.0, 0.0);
+    for (int q = 0; q < 8; ++q)
+      pre[q] = (i < rh && k0 + 8 * q < w) ? M[(r0 + i) + (long)(c0 + k0 + 8 * q) * ld] : mk(0.0, 0.0);
   }
   for (int rr = r0; rr < r1; rr += QW) {
     const int rh = r1 - rr < QW ? r1 - rr : QW;
-    if (act) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) C[i][k0 + 4 * q] = pre[q];
-    }
+    for (int q = 0; q < 8; ++q) C[i][k0 + 8 * q] = pre[q];
     __syncthreads();
-    if (act) {
-      const int rn = rr + QW;
-      if (rn < r1) {
-        const int rhn = r1 - rn < QW ? r1 - rn : QW;
+    const int rn = rr + QW;
+    if (rn < r1) {
+      const int rhn = r1 - rn < QW ? r1 - rn : QW;
 #pragma unroll
-        for (int q = 0; q < 16; ++q)
-          pre[q] = (i < rhn && k0 + 4 * q < w) ? M[(rn + i) + (long)(c0 + k0 + 4 * q) * ld] : mk(0.0, 0.0);
-      }
-      d4 aR[2][2], aI[2][2];
-      if (w == QW)
-        mm64<false, true>(C, U, QW / 4, aR, aI);
-      else
-        mm64<false, false>(C, U, (w + 3) >> 2, aR, aI);
-      mm64_for_each(aR, aI, [&](int ii, int j, cd v) {
-        if (ii < rh && j < w) M[(rr + ii) + (long)(c0 + j) * ld] = v;
-      });
+      for (int q = 0; q < 8; ++q)
+        pre[q] = (i < rhn && k0 + 8 * q < w) ? M[(rn + i) + (long)(c0 + k0 + 8 * q) * ld] : mk(0.0, 0.0);
     }
+    d4 aR[2], aI[2];
+    if (w == QW)
+      mm64w8<false, true>(C, U, QW / 4, aR, aI);
+    else
+      mm64w8<false, false>(C, U, (w + 3) >> 2, aR, aI);
+    mm64w8_for_each(aR, aI, [&](int ii, int j, cd v) {
+      if (ii < rh && j < w) M[(rr + ii) + (long)(c0 + j) * ld] = v;
+    });
     __syncthreads();
   }
 }
