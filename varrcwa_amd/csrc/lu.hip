@@ -480,15 +480,21 @@ __global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const cd* T, l
 // In-place triangular solve on Y (n x nrhs): UPPER=0 -> L (unit) ; UPPER=1 -> U (non-unit).
 // Blocked by TRS_NB rows; each diagonal block is applied as a product with its inverse
 // (computed once per call), the off-diagonal blocks as ZGEMM updates.
+// Xo (optional): the solution goes to Xo block by block (each diagonal product written
+// there directly) and Y is only the work space of the updates; otherwise in place in Y.
 static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* LU, long lda, long sA, cd* Y,
-                           long ldy, long sY, int batch, Workspace& ws) {
+                           long ldy, long sY, int batch, Workspace& ws, cd* Xo, long ldxo, long sXo) {
   const int nblk = cdiv_up(n, TRS_NB);
   const long sD = (long)nblk * TRS_NB * TRS_NB;
   DevBuf Db = ws.get(sizeof(cd) * (size_t)sD * batch);
-  DevBuf Tb = ws.get(sizeof(cd) * (size_t)TRS_NB * nrhs * batch);
+  DevBuf Tb;
   cd* Dinv = Db.as<cd>();
-  cd* Tmp = Tb.as<cd>();
+  cd* Tmp = nullptr;
   const long sTmp = (long)TRS_NB * nrhs;
+  if (!Xo) {
+    Tb = ws.get(sizeof(cd) * (size_t)TRS_NB * nrhs * batch);
+    Tmp = Tb.as<cd>();
+  }
   dim3 g(nblk, batch);
   if (!upper)
     trtri_blocks_kernel<0><<<g, 256, 0, s>>>(n, LU, lda, sA, Dinv, sD);
@@ -496,11 +502,20 @@ static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* 
     trtri_blocks_kernel<1><<<g, 256, 0, s>>>(n, LU, lda, sA, Dinv, sD);
   launch_check("trtri_blocks");
   const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0), zero = mk(0.0, 0.0);
+  // solved rows k0.. live in X (= Xo or Y)
+  cd* X = Xo ? Xo : Y;
+  const long ldx = Xo ? ldxo : ldy, sX = Xo ? sXo : sY;
   auto diag = [&](int kb, int k0, int bs) {
-    // Y[k0:k0+bs] <- inv(T_kk) Y[k0:k0+bs]  (through Tmp)
-    zgemm_strided(s, OP_N, OP_N, bs, nrhs, bs, one, Dinv + (long)kb * TRS_NB * TRS_NB, TRS_NB, sD, Y + k0, ldy, sY,
-                  zero, Tmp, TRS_NB, sTmp, batch);
-    zgeam(s, bs, nrhs, one, Tmp, TRS_NB, sTmp, zero, nullptr, 0, 0, Y + k0, ldy, sY, batch);
+    const cd* Dk = Dinv + (long)kb * TRS_NB * TRS_NB;
+    if (Xo) {
+      zgemm_strided(s, OP_N, OP_N, bs, nrhs, bs, one, Dk, TRS_NB, sD, Y + k0, ldy, sY, zero, Xo + k0, ldxo, sXo,
+                    batch);
+    } else {
+      // Y[k0:k0+bs] <- inv(T_kk) Y[k0:k0+bs]  (through Tmp)
+      zgemm_strided(s, OP_N, OP_N, bs, nrhs, bs, one, Dk, TRS_NB, sD, Y + k0, ldy, sY, zero, Tmp, TRS_NB, sTmp,
+                    batch);
+      zgeam(s, bs, nrhs, one, Tmp, TRS_NB, sTmp, zero, nullptr, 0, 0, Y + k0, ldy, sY, batch);
+    }
   };
   // Diagonal blocks are taken in pairs: the coupling between the two is one 64-deep
   // update, the rest of Y is updated once per pair (k up to 2*TRS_NB).
@@ -513,15 +528,15 @@ static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* 
       if (kb + 1 < nblk) {
         const int k1 = k0 + TRS_NB;
         const int bs1 = n - k1 < TRS_NB ? n - k1 : TRS_NB;
-        zgemm_strided(s, OP_N, OP_N, bs1, nrhs, bs0, mone, LU + k1 + (long)k0 * lda, lda, sA, Y + k0, ldy, sY, one,
+        zgemm_strided(s, OP_N, OP_N, bs1, nrhs, bs0, mone, LU + k1 + (long)k0 * lda, lda, sA, X + k0, ldx, sX, one,
                       Y + k1, ldy, sY, batch);
         diag(kb + 1, k1, bs1);
         w += bs1;
       }
       const int rest = n - k0 - w;
       if (rest > 0)
-        zgemm_strided(s, OP_N, OP_N, rest, nrhs, w, mone, LU + (k0 + w) + (long)k0 * lda, lda, sA, Y + k0, ldy,
-                      sY, one, Y + (k0 + w), ldy, sY, batch);
+        zgemm_strided(s, OP_N, OP_N, rest, nrhs, w, mone, LU + (k0 + w) + (long)k0 * lda, lda, sA, X + k0, ldx,
+                      sX, one, Y + (k0 + w), ldy, sY, batch);
     }
   } else {
     for (int kb = nblk - 1; kb >= 0; kb -= 2) {
@@ -531,16 +546,21 @@ static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* 
       int k0 = k1, w = bs1;
       if (kb >= 1) {
         k0 = k1 - TRS_NB;
-        zgemm_strided(s, OP_N, OP_N, TRS_NB, nrhs, bs1, mone, LU + k0 + (long)k1 * lda, lda, sA, Y + k1, ldy, sY,
+        zgemm_strided(s, OP_N, OP_N, TRS_NB, nrhs, bs1, mone, LU + k0 + (long)k1 * lda, lda, sA, X + k1, ldx, sX,
                       one, Y + k0, ldy, sY, batch);
         diag(kb - 1, k0, TRS_NB);
         w += TRS_NB;
       }
       if (k0 > 0)
-        zgemm_strided(s, OP_N, OP_N, k0, nrhs, w, mone, LU + (long)k0 * lda, lda, sA, Y + k0, ldy, sY, one, Y, ldy,
+        zgemm_strided(s, OP_N, OP_N, k0, nrhs, w, mone, LU + (long)k0 * lda, lda, sA, X + k0, ldx, sX, one, Y, ldy,
                       sY, batch);
     }
   }
+}
+
+static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* LU, long lda, long sA, cd* Y,
+                           long ldy, long sY, int batch, Workspace& ws) {
+  trsm_lu_factor(s, upper, n, nrhs, LU, lda, sA, Y, ldy, sY, batch, ws, nullptr, 0, 0);
 }
 
 void ztrsm_upper(hipStream_t s, int n, int nrhs, const cd* U, long ldu, long sU, cd* Y, long ldy, long sY,
@@ -556,22 +576,43 @@ void zgetrs(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, con
   int* perm = (int*)permb.ptr;
   ipiv_to_perm_kernel<<<batch, 256, sizeof(int) * n, s>>>(n, ipiv, sP, perm, n);
   launch_check("ipiv_to_perm");
-  // Gather into X directly when X does not alias B, otherwise through a temporary.
-  DevBuf tmpb;
-  cd* Y = X;
-  long ldy = ldx, sY = sX;
-  if (X == B) {
-    tmpb = ws.get(sizeof(cd) * (size_t)n * nrhs * batch);
-    Y = (cd*)tmpb.ptr;
-    ldy = n;
-    sY = (long)n * nrhs;
-  }
+  // X not aliasing B: gather into X, forward substitution X -> T, back substitution
+  // T -> X. Aliased: gather into T, forward substitution in place, back substitution
+  // T -> X.
+  DevBuf Tb = ws.get(sizeof(cd) * (size_t)n * nrhs * batch);
+  cd* T = Tb.as<cd>();
+  const long sT = (long)n * nrhs;
   dim3 g(cdiv_up(n, 256), nrhs, batch);
-  gather_rows_kernel<<<g, 256, 0, s>>>(n, nrhs, perm, n, B, ldb, sB, Y, ldy, sY);
+  if (X != B) {
+    gather_rows_kernel<<<g, 256, 0, s>>>(n, nrhs, perm, n, B, ldb, sB, X, ldx, sX);
+    launch_check("gather_rows");
+    trsm_lu_factor(s, 0, n, nrhs, LU, lda, sA, X, ldx, sX, batch, ws, T, n, sT);
+  } else {
+    gather_rows_kernel<<<g, 256, 0, s>>>(n, nrhs, perm, n, B, ldb, sB, T, n, sT);
+    launch_check("gather_rows");
+    trsm_lu_factor(s, 0, n, nrhs, LU, lda, sA, T, n, sT, batch, ws);
+  }
+  trsm_lu_factor(s, 1, n, nrhs, LU, lda, sA, T, n, sT, batch, ws, X, ldx, sX);
+}
+
+void zgetrs_pair(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, const int* ipiv, long sP,
+                 const cd* B1, long ldb1, long sB1, const cd* B2, long ldb2, long sB2, cd* X, long ldx, long sX,
+                 int batch, Workspace& ws) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return;
+  DevBuf permb = ws.get(sizeof(int) * (size_t)n * batch);
+  int* perm = (int*)permb.ptr;
+  ipiv_to_perm_kernel<<<batch, 256, sizeof(int) * n, s>>>(n, ipiv, sP, perm, n);
+  launch_check("ipiv_to_perm");
+  dim3 g(cdiv_up(n, 256), nrhs, batch);
+  gather_rows_kernel<<<g, 256, 0, s>>>(n, nrhs, perm, n, B1, ldb1, sB1, X, ldx, sX);
   launch_check("gather_rows");
-  trsm_lu_factor(s, 0, n, nrhs, LU, lda, sA, Y, ldy, sY, batch, ws);
-  trsm_lu_factor(s, 1, n, nrhs, LU, lda, sA, Y, ldy, sY, batch, ws);
-  if (Y != X) zgeam(s, n, nrhs, mk(1.0, 0.0), Y, ldy, sY, mk(0.0, 0.0), nullptr, 0, 0, X, ldx, sX, batch);
+  gather_rows_kernel<<<g, 256, 0, s>>>(n, nrhs, perm, n, B2, ldb2, sB2, X + (long)nrhs * ldx, ldx, sX);
+  launch_check("gather_rows");
+  // forward substitution X -> T, back substitution T -> X (no block copies)
+  DevBuf Tb = ws.get(sizeof(cd) * (size_t)n * 2 * nrhs * batch);
+  const long sT = (long)n * 2 * nrhs;
+  trsm_lu_factor(s, 0, n, 2 * nrhs, LU, lda, sA, X, ldx, sX, batch, ws, Tb.as<cd>(), n, sT);
+  trsm_lu_factor(s, 1, n, 2 * nrhs, LU, lda, sA, Tb.as<cd>(), n, sT, batch, ws, X, ldx, sX);
 }
 
 void zgetri(hipStream_t s, int n, const cd* LU, long lda, long sA, const int* ipiv, long sP, cd* Ainv, long ldi,

@@ -536,8 +536,7 @@ void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd
   zset(s, N, N, ONE, ZERO, M, N, NN, batch);
   zgemm_strided(s, OP_N, OP_N, N, N, N, MONE, S.Rud, N, NN, rdu, N, ss, ONE, M, N, NN, batch);
   zgetrf(s, N, M, N, NN, piv, N, infb.as<int>(), batch, c.ws);
-  zgetrs(s, N, N, M, N, NN, piv, N, S.Tuu, N, NN, Yt, N, s2, batch, c.ws);
-  zgetrs(s, N, N, M, N, NN, piv, N, S.Rud, N, NN, Yr, N, s2, batch, c.ws);
+  zgetrs_pair(s, N, N, M, N, NN, piv, N, S.Tuu, N, NN, S.Rud, N, NN, Y, N, s2, batch, c.ws);
   // K = Yr tdd ; Rud' = rud + tuu K
   zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, Yr, N, s2, tdd, N, ss, ZERO, K, N, NN, batch);
   zgeam(s, N, N, ONE, rud, N, ss, ZERO, nullptr, 0, 0, S.Rud, N, NN, batch);

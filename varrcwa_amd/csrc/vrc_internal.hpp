@@ -72,6 +72,10 @@ void zmaxabs(hipStream_t s, int m, int n, const cd* A, long lda, long sA, double
 
 // ---------------------------------------------------------------- lu.hip
 void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, int* info, int batch, Workspace& ws);
+// X = A^{-1} [B1 | B2] (nrhs columns each; X holds 2 nrhs columns and aliases neither B).
+void zgetrs_pair(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, const int* ipiv, long sP,
+                 const cd* B1, long ldb1, long sB1, const cd* B2, long ldb2, long sB2, cd* X, long ldx, long sX,
+                 int batch, Workspace& ws);
 void zgetrs(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA, const int* ipiv, long sP, const cd* B,
             long ldb, long sB, cd* X, long ldx, long sX, int batch, Workspace& ws);
 // Y <- U^{-1} Y for upper-triangular U (non-unit diagonal), in place.
