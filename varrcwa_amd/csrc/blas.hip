@@ -1,10 +1,10 @@
 // Complex128 dense kernels for gfx950: FP64-MFMA ZGEMM (strided and descriptor
 // batched) and the elementwise matrix kernels used between the factorizations.
 //
-// ZGEMM tile: 64x64 output per workgroup (4 wavefronts, 32x32 each = 2x2 blocks of
-// v_mfma_f64_16x16x4f64), K staged through LDS in chunks of 16 complex values,
-// real and imaginary planes kept separate so that one complex multiply-add is four
-// real MFMAs (standard 4M product, no 3M/Gauss trick).
+// ZGEMM tiles: 64x64 and 128x64 output per workgroup (8 wavefronts by default; 4-wave
+// variants selectable) from v_mfma_f64_16x16x4f64 blocks, K staged through LDS in
+// chunks of 16 complex values, real and imaginary planes kept separate so that one
+// complex multiply-add is four real MFMAs (standard 4M product, no 3M/Gauss trick).
 #include "vrc_common.hpp"
 #include "vrc_internal.hpp"
 
@@ -132,6 +132,116 @@ __device__ __forceinline__ void zgemm_tile(int m, int n, int k, cd alpha, const 
         if (!beta0) out = cadd(out, cmul(beta, *cp));
         *cp = out;
       }
+    }
+}
+
+// The 64x64 tile on eight waves (512 threads): wave w owns rows 32*(w>>2) .. +32 and
+// columns 16*(w&3) .. +16 (2x1 blocks); staging as above, 2 elements per thread.
+template <int OPA, int OPB>
+__device__ __forceinline__ void zgemm_tile8(int m, int n, int k, cd alpha, const cd* __restrict__ A, long lda,
+                                            const cd* __restrict__ B, long ldb, cd beta, cd* __restrict__ C,
+                                            long ldc, int tm, int tn) {
+  __shared__ double As_re[GT_K][GT_M + 1];
+  __shared__ double As_im[GT_K][GT_M + 1];
+  __shared__ double Bs_re[GT_K][GT_N + 1];
+  __shared__ double Bs_im[GT_K][GT_N + 1];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wr = wave >> 2, wc = wave & 3;
+  const int row0 = tm * GT_M, col0 = tn * GT_N;
+  d4 accR[2], accI[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    accR[i] = d4{0.0, 0.0, 0.0, 0.0};
+    accI[i] = d4{0.0, 0.0, 0.0, 0.0};
+  }
+  // element q of this thread: A (row t&63, k (t>>6) + 8q) or (row (t>>4) + 32q, k t&15);
+  // B (k t&15, col (t>>4) + 32q) or (k (t>>6) + 8q, col t&63)
+  cd pa[2], pb[2];
+  auto fetch = [&](int kk) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      cd v = mk(0.0, 0.0);
+      if (OPA == OP_N) {
+        const int gr = row0 + (t & 63), gc = kk + (t >> 6) + 8 * i;
+        if (gr < m && gc < k) v = A[gr + (long)gc * lda];
+      } else {
+        const int gr = row0 + (t >> 4) + 32 * i, gc = kk + (t & 15);
+        if (gr < m && gc < k) v = A[gc + (long)gr * lda];
+        if (OPA == OP_C) v.y = -v.y;
+      }
+      pa[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      cd v = mk(0.0, 0.0);
+      if (OPB == OP_N) {
+        const int gc = kk + (t & 15), gj = col0 + (t >> 4) + 32 * i;
+        if (gc < k && gj < n) v = B[gc + (long)gj * ldb];
+      } else {
+        const int gc = kk + (t >> 6) + 8 * i, gj = col0 + (t & 63);
+        if (gc < k && gj < n) v = B[gj + (long)gc * ldb];
+        if (OPB == OP_C) v.y = -v.y;
+      }
+      pb[i] = v;
+    }
+  };
+  fetch(0);
+  for (int kk = 0; kk < k; kk += GT_K) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      if (OPA == OP_N) {
+        As_re[(t >> 6) + 8 * i][t & 63] = pa[i].x;
+        As_im[(t >> 6) + 8 * i][t & 63] = pa[i].y;
+      } else {
+        As_re[t & 15][(t >> 4) + 32 * i] = pa[i].x;
+        As_im[t & 15][(t >> 4) + 32 * i] = pa[i].y;
+      }
+      if (OPB == OP_N) {
+        Bs_re[t & 15][(t >> 4) + 32 * i] = pb[i].x;
+        Bs_im[t & 15][(t >> 4) + 32 * i] = pb[i].y;
+      } else {
+        Bs_re[(t >> 6) + 8 * i][t & 63] = pb[i].x;
+        Bs_im[(t >> 6) + 8 * i][t & 63] = pb[i].y;
+      }
+    }
+    __syncthreads();
+    if (kk + GT_K < k) fetch(kk + GT_K);
+#pragma unroll
+    for (int s = 0; s < GT_K / 4; ++s) {
+      const int kr = s * 4 + (lane >> 4);
+      double ar[2], ai[2], nai[2];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        ar[b] = As_re[kr][wr * 32 + b * 16 + (lane & 15)];
+        ai[b] = As_im[kr][wr * 32 + b * 16 + (lane & 15)];
+        nai[b] = -ai[b];
+      }
+      const double br = Bs_re[kr][wc * 16 + (lane & 15)], bi = Bs_im[kr][wc * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        accR[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br, accR[i], 0, 0, 0);
+        accR[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi, accR[i], 0, 0, 0);
+        accI[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi, accI[i], 0, 0, 0);
+        accI[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br, accI[i], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  const bool beta0 = (beta.x == 0.0 && beta.y == 0.0);
+  const int gc = col0 + wc * 16 + (lane & 15);
+  if (gc >= n) return;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gr = row0 + wr * 32 + i * 16 + (lane >> 4) + 4 * r;
+      if (gr >= m) continue;
+      cd out = cmul(alpha, mk(accR[i][r], accI[i][r]));
+      cd* cp = C + gr + (long)gc * ldc;
+      if (!beta0) out = cadd(out, cmul(beta, *cp));
+      *cp = out;
     }
 }
 
@@ -443,6 +553,15 @@ __global__ __launch_bounds__(GT_THREADS) void zgemm_strided_kernel(int m, int n,
 }
 
 template <int OPA, int OPB>
+__global__ __launch_bounds__(512) void zgemm_strided8_kernel(int m, int n, int k, cd alpha, const cd* A, long lda,
+                                                            long sA, const cd* B, long ldb, long sB, cd beta, cd* C,
+                                                            long ldc, long sC) {
+  const int tm = blockIdx.x, tn = blockIdx.y;
+  const long b = blockIdx.z;
+  zgemm_tile8<OPA, OPB>(m, n, k, alpha, A + b * sA, lda, B + b * sB, ldb, beta, C + b * sC, ldc, tm, tn);
+}
+
+template <int OPA, int OPB>
 __global__ __launch_bounds__(GT_THREADS) void zgemm_desc_kernel(const GemmDesc* __restrict__ desc) {
   const GemmDesc d = desc[blockIdx.z];
   const int tm = blockIdx.x, tn = blockIdx.y;
@@ -458,7 +577,9 @@ __global__ __launch_bounds__(GT_THREADS) void zgemm_desc_kernel(const GemmDesc* 
 static bool g_big_gemm = true;  // large-tile kernel for m >= 128, n >= 64 (switchable for comparison)
 static int g_big_waves = 8;     // wavefronts of the large-tile kernel (4 or 8)
 void zgemm_set_big_waves(int w) { g_big_waves = w == 8 ? 8 : 4; }
-static int g_big_min_k = 65;    // ... and k at least this (short-k updates run better on the 64x64 tile)
+static int g_small_waves = 8;   // wavefronts of the 64x64-tile kernel (4 or 8)
+void zgemm_set_small_waves(int w) { g_small_waves = w == 8 ? 8 : 4; }
+static int g_big_min_k = 129;   // ... and k at least this (short-k updates run better on the 64x64 tile)
 void zgemm_set_big(bool on) { g_big_gemm = on; }
 void zgemm_set_big_min_k(int k) { g_big_min_k = k; }
 
@@ -492,6 +613,24 @@ void zgemm_strided(hipStream_t s, int opA, int opB, int m, int n, int k, cd alph
     return;
   }
   dim3 grid(cdiv_up(m, GT_M), cdiv_up(n, GT_N), batch);
+  if (g_small_waves == 8 && k > 0) {
+#define VRC_S8(OA, OB) \
+  zgemm_strided8_kernel<OA, OB><<<grid, 512, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC)
+    switch (opA * 3 + opB) {
+      case 0: VRC_S8(OP_N, OP_N); break;
+      case 1: VRC_S8(OP_N, OP_T); break;
+      case 2: VRC_S8(OP_N, OP_C); break;
+      case 3: VRC_S8(OP_T, OP_N); break;
+      case 4: VRC_S8(OP_T, OP_T); break;
+      case 5: VRC_S8(OP_T, OP_C); break;
+      case 6: VRC_S8(OP_C, OP_N); break;
+      case 7: VRC_S8(OP_C, OP_T); break;
+      default: VRC_S8(OP_C, OP_C); break;
+    }
+#undef VRC_S8
+    launch_check("zgemm_strided8");
+    return;
+  }
   switch (opA * 3 + opB) {
     case 0: zgemm_strided_kernel<OP_N, OP_N><<<grid, GT_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC); break;
     case 1: zgemm_strided_kernel<OP_N, OP_T><<<grid, GT_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC); break;
