@@ -579,7 +579,9 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     a.sab = 2 * n;
     a.d_k0sq = dk2.as<double>() + je;
     DevBuf P = c.ws.get(sizeof(cd) * NN * JD), Q = c.ws.get(sizeof(cd) * NN * JD);
-    assemble_pq(c, a, P.as<cd>(), Q.as<cd>());
+    DevBuf PQ;  // P Q formed with P and Q (periodic cells: without the N x N product)
+    if (!tape) PQ = c.ws.get(sizeof(cd) * NN * JD);
+    assemble_pq(c, a, P.as<cd>(), Q.as<cd>(), PQ.as<cd>());
     const int JW = tape ? J : 2 * B;  // jobs whose W, V are formed explicitly
     DevBuf Wall = c.ws.get(sizeof(cd) * NN * JW), Vall = c.ws.get(sizeof(cd) * NN * JW),
            gall = c.ws.get(sizeof(cd) * (size_t)N * J), infall = c.ws.get(sizeof(int) * J);
@@ -625,7 +627,8 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
       DevBuf Zb = c.ws.get(sizeof(cd) * NN * JD), Xb = c.ws.get(sizeof(cd) * NN * JD),
              Db = c.ws.get(sizeof(cd) * (size_t)N * JD);
       wave_eqn_factors(c, N, P.as<cd>(), Q.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(),
-                       gall.as<cd>() + (long)je * N, infall.as<int>() + je, JD);
+                       gall.as<cd>() + (long)je * N, infall.as<int>() + je, JD, PQ.as<cd>());
+      PQ = DevBuf();
       if (!homog)
         wave_eqn_vectors(c, N, Q.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(), gall.as<cd>(), Wall.as<cd>(),
                          Vall.as<cd>(), 2 * B);

@@ -50,6 +50,62 @@ __global__ void pq_periodic_kernel(int n, const cd* inv33, const cd* fe11, const
   Q[o] = q;
 }
 
+// Periodic P Q without the N x N product. With a = diag(alpha), b = diag(beta), E = fe33^{-1}
+// and the blocks of pq_periodic_kernel (P11 = a E b, P12 = -a E a + k^2 I, P21 = b E b - k^2 I,
+// P22 = -b E a; Q11 = -a b, Q12 = a^2 - k^2 fe22, Q21 = k^2 fe11 - b^2, Q22 = b a):
+//   PQ11 = a M1 + k^2 Q21, PQ21 = b M1 - k^2 Q11, PQ12 = a M2 + k^2 Q22, PQ22 = b M2 - k^2 Q12
+// with [M1 | M2] = E [b Q11 - a Q21 | b Q12 - a Q22] (one n x 2n product per layer).
+// First kernel: R = [b Q11 - a Q21 | b Q12 - a Q22] (n x 2n per layer).
+__global__ void pq_rhs_kernel(int n, const cd* fe11, const cd* fe22, long sfe, const cd* alpha, const cd* beta,
+                              long sab, const double* k0sq, cd* R) {
+  const long l = blockIdx.z;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;  // 0 .. 2n-1
+  if (r >= n) return;
+  const int cb = c / n, ci = c % n;
+  const cd* al = alpha + l * sab;
+  const cd* be = beta + l * sab;
+  const double k2 = k0sq[l];
+  const cd ar = al[r], br = be[r];
+  const bool dg = (r == ci);
+  const long e = l * sfe + r + (long)ci * n;
+  cd v;
+  if (cb == 0) {
+    // b Q11 - a Q21 = -b a b (diagonal) - a (k^2 fe11 - b^2)
+    cd q21 = cscale(fe11[e], k2);
+    if (dg) q21 = csub(q21, cmul(br, br));
+    v = cneg(cmul(ar, q21));
+    if (dg) v = csub(v, cmul(br, cmul(ar, br)));
+  } else {
+    // b Q12 - a Q22 = b (a^2 - k^2 fe22) - a b a (diagonal)
+    cd q12 = cscale(fe22[e], -k2);
+    if (dg) q12 = cadd(q12, cmul(ar, ar));
+    v = cmul(br, q12);
+    if (dg) v = csub(v, cmul(ar, cmul(br, ar)));
+  }
+  R[l * 2 * (long)n * n + r + (long)c * n] = v;
+}
+
+// Second kernel: PQ from M = [M1 | M2] and Q (N x N per layer, stride sPQ).
+__global__ void pq_combine_kernel(int n, const cd* M, const cd* alpha, const cd* beta, long sab, const double* k0sq,
+                                  const cd* Q, long sQ, cd* PQ, long sPQ) {
+  const long l = blockIdx.z;
+  const int N = 2 * n;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= N) return;
+  const int rb = r / n, ri = r % n, cb = c / n, ci = c % n;
+  const double k2 = k0sq[l];
+  const cd m = M[l * 2 * (long)n * n + ri + (long)c * n];  // M1 for cb = 0, M2 for cb = 1
+  const cd d = rb == 0 ? alpha[l * sab + ri] : beta[l * sab + ri];
+  // k^2 Q21 / k^2 Q22 (top) and -k^2 Q11 / -k^2 Q12 (bottom): the Q block one row block
+  // below (top) or above (bottom) in the same column
+  const int qr = rb == 0 ? n + ri : ri;
+  const cd q = Q[l * sQ + qr + (long)c * N];
+  const cd add = rb == 0 ? cscale(q, k2) : cscale(q, -k2);
+  PQ[l * sPQ + r + (long)c * N] = cadd(cmul(d, m), add);
+}
+
 // X_i = exp(i * gamma_i * s) with s = dz/k0 per batch entry (src/rcwa/RCWAScatterMatrix.cpp:36-40).
 __global__ void phase_kernel(int n, const cd* gam, long sg, const double* s, cd* X, long sx) {
   const long b = blockIdx.y;
@@ -135,7 +191,7 @@ static void copy(hipStream_t s, int m, int n, const cd* A, cd* C, long stride, i
 }
 
 // ===================================================================== (b) assembly
-void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q) {
+void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q, cd* PQ) {
   StageScope st(ST_PQ);
   hipStream_t s = c.stream;
   const int n = a.nx * a.ny, N = 2 * n, L = a.nlayers;
@@ -156,6 +212,16 @@ void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q) {
     pq_periodic_kernel<<<g, 256, 0, s>>>(n, inv.as<cd>(), fe11, fe22, nn, a.d_alpha, a.d_beta, a.sab, a.d_k0sq, P,
                                          Q, NN);
     launch_check("pq_periodic");
+    if (PQ) {
+      DevBuf Rb = c.ws.get(sizeof(cd) * (size_t)nn * 2 * L), Mb = c.ws.get(sizeof(cd) * (size_t)nn * 2 * L);
+      dim3 g1(cdiv_up(n, 256), 2 * n, L);
+      pq_rhs_kernel<<<g1, 256, 0, s>>>(n, fe11, fe22, nn, a.d_alpha, a.d_beta, a.sab, a.d_k0sq, Rb.as<cd>());
+      launch_check("pq_rhs");
+      zgemm_strided(s, OP_N, OP_N, n, 2 * n, n, ONE, inv.as<cd>(), n, nn, Rb.as<cd>(), n, 2 * nn, ZERO, Mb.as<cd>(), n,
+                    2 * nn, L);
+      pq_combine_kernel<<<g, 256, 0, s>>>(n, Mb.as<cd>(), a.d_alpha, a.d_beta, a.sab, a.d_k0sq, Q, NN, PQ, NN);
+      launch_check("pq_combine");
+    }
   } else {
     // F = fe33^{-1} [Kx | Ky]  (n x 2n), then P blocks by ZGEMM against the dense K
     DevBuf Fb = c.ws.get(sizeof(cd) * (size_t)nn * 2 * L);
@@ -180,6 +246,7 @@ void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q) {
     zgeam(s, n, n, ONE, a.d_KxKx, n, 0, mk(-k2, 0.0), fe22, n, nn, Q + (long)n * N, N, NN, L);
     zgeam(s, n, n, MONE, a.d_KyKy, n, 0, mk(k2, 0.0), fe11, n, nn, Q + n, N, NN, L);
     zgeam(s, n, n, ONE, a.d_KyKx, n, 0, ZERO, nullptr, 0, 0, Q + n + (long)n * N, N, NN, L);
+    if (PQ) zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, P, N, NN, Q, N, NN, ZERO, PQ, N, NN, L);
   }
 }
 
@@ -325,16 +392,18 @@ static void rowscale_inv(hipStream_t s, int m, int n, const cd* d, long sd, cons
 // (c) as wave_eqn, but the eigenvectors stay factored: PQ = D Z T Z^H D^{-1},
 // W = D Z X (unnormalized; every S-matrix is invariant under the column scaling).
 void wave_eqn_factors(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* Z, cd* X, cd* dbal, cd* gam,
-                      int* info, int batch) {
+                      int* info, int batch, cd* PQ) {
   hipStream_t s = c.stream;
   const long NN = (long)N * N;
-  DevBuf PQb = c.ws.get(sizeof(cd) * (size_t)NN * batch);
+  DevBuf PQb;
   DevBuf lamb = c.ws.get(sizeof(cd) * (size_t)N * batch);
-  {
+  if (!PQ) {
+    PQb = c.ws.get(sizeof(cd) * (size_t)NN * batch);
+    PQ = PQb.as<cd>();
     StageScope st(ST_EIGPOST);
-    zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, P, N, sPQ, Q, N, sPQ, ZERO, PQb.as<cd>(), N, NN, batch);
+    zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, P, N, sPQ, Q, N, sPQ, ZERO, PQ, N, NN, batch);
   }
-  zgeev_factors(s, N, PQb.as<cd>(), N, NN, lamb.as<cd>(), N, Z, X, dbal, info, batch, c.ws);
+  zgeev_factors(s, N, PQ, N, NN, lamb.as<cd>(), N, Z, X, dbal, info, batch, c.ws);
   StageScope st(ST_EIGPOST);
   gamma_sqrt(s, N, lamb.as<cd>(), N, gam, N, batch);
 }

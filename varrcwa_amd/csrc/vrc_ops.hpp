@@ -55,15 +55,18 @@ struct FourierStage {
   int L = 0;
 };
 FourierStage fourier_prepare(vrc_ctx& c, int nx, int ny, const std::vector<const HostLayer*>& layers);
-void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q);
+// P, Q of every layer (N x N, stride N^2); PQ (optional) also receives P Q, for periodic
+// cells without the N x N product.
+void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q, cd* PQ = nullptr);
 
 // ---------------------------------------------------------------- (c)
 // PQ = P Q; eig; gamma = sqrt branch; V = Q W diag(1/gamma)  (src/rcwa/WaveEquationSolver.cpp:24-37)
 void wave_eqn(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* W, cd* V, cd* gam, int* info, int batch);
 
 // Eigen factors instead of W, V: W = diag(dbal) Z X (unnormalized), and the vectors.
+// PQ (optional): P Q already formed (stride N^2); consumed as the eigensolver's work matrix.
 void wave_eqn_factors(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* Z, cd* X, cd* dbal, cd* gam,
-                      int* info, int batch);
+                      int* info, int batch, cd* PQ = nullptr);
 void wave_eqn_vectors(vrc_ctx& c, int N, const cd* Q, long sQ, const cd* Z, const cd* X, const cd* dbal,
                       const cd* gam, cd* W, cd* V, int batch);
 
