@@ -25,6 +25,7 @@ constexpr double ULP = 2.220446049250313e-16;      // 2^-52
 constexpr double SAFMIN = 2.2250738585072014e-308;
 constexpr int HB = 64;         // Hessenberg panel width
 constexpr int PT = 1024;       // threads of the Hessenberg panel workgroup (16 waves)
+constexpr int GV_MAXG = 8;     // most column groups of the panel's matrix-vector product
 constexpr int QT = 512;        // threads of the QR workgroup (8 waves; the window MFMAs use 4)
 constexpr int QW = 64;         // maximal QR window
 constexpr int NBUL = 16;       // most bulges per sweep (2 shifts each)
@@ -94,7 +95,12 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
   __shared__ cd red[PT / 64][HB];
   __shared__ cd vec1[HB], vec2[HB];
   __shared__ double rn[PT / 64];
+  __shared__ cd gpart[PT];  // per-thread partial sums of the column-split product
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  // rows below the panel; when they fit in the workgroup, the columns of the trailing
+  // product are split over G groups of R threads (partial sums reduced through LDS)
+  const int R = N - p - 1;
+  const int G = (R > 0 && R <= PT) ? (PT / R < GV_MAXG ? PT / R : GV_MAXG) : 1;
 
   for (int j = 0; j < nb; ++j) {
     const int c = p + j;
@@ -154,8 +160,38 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
     // z = V_j^H v  (rows >= c+1)
     if (j > 0) panel_multi_dot(N, c + 1, p, j, Vf, vvec, red, vec2);
     // y_j = tau*(A0[:, c+1:] v - Y_j z) for the rows below p (the rows above are one
-    // ZGEMM after the panel). The trailing-matrix product is the memory-bound part:
-    // every thread keeps 4 rows and 4 columns of independent loads in flight.
+    // ZGEMM after the panel). The trailing-matrix product is the memory-bound part.
+    if (R > 0 && R <= PT) {
+      // thread (g, i): row p+1+i, columns c+1+8g+8Gk .. +8 (eight loads in flight)
+      const int g = t / R, i = t - g * R;
+      double sr = 0.0, si = 0.0;
+      if (g < G) {
+        const int r = p + 1 + i;
+        for (int col0 = c + 1 + 8 * g; col0 < N; col0 += 8 * G) {
+          cd a[8], v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const bool in = col0 + u < N;
+            a[u] = in ? A[r + (long)(col0 + u) * lda] : mk(0.0, 0.0);
+            v[u] = in ? vvec[col0 + u] : mk(0.0, 0.0);
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            sr += a[u].x * v[u].x - a[u].y * v[u].y;
+            si += a[u].x * v[u].y + a[u].y * v[u].x;
+          }
+        }
+      }
+      gpart[t] = mk(sr, si);
+      __syncthreads();
+      if (t < R) {
+        const int r = p + 1 + t;
+        cd acc = gpart[t];
+        for (int q = 1; q < G; ++q) acc = cadd(acc, gpart[q * R + t]);
+        for (int i2 = 0; i2 < j; ++i2) acc = csub(acc, cmul(Y[r + (long)i2 * N], vec2[i2]));
+        Y[r + (long)j * N] = cmul(tv, acc);
+      }
+    } else
     for (int rb = p + 1 + t; rb < N; rb += 4 * PT) {
       double gr[4] = {0.0, 0.0, 0.0, 0.0}, gi[4] = {0.0, 0.0, 0.0, 0.0};
       int col = c + 1;
