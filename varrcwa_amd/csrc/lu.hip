@@ -83,10 +83,11 @@ __global__ __launch_bounds__(256) void getrf_panel_kernel(int n, int k, int kb, 
 }
 
 // Register-resident panel for n - k <= LUR_ROWS: thread t holds row k + t of the
-// kb <= LUR_NB panel columns. Per column: wave arg-max (first index of the largest
-// |re|+|im|), row exchange through LDS, elimination in registers. Afterwards the
-// panel's interchanges are applied to the other columns [K, K+KB) of the block and,
-// right of the panel, U12 = L11^{-1} A12 is solved (one thread per column).
+// kb <= LUR_NB panel columns. Per column (one barrier): wave arg-max (first index of the
+// largest |re|+|im|) posted with the candidate row, row exchange through LDS, elimination
+// in registers. Afterwards the panel's interchanges are applied to the other columns
+// [K, K+KB) of the block and, right of the panel, U12 = L11^{-1} A12 is solved (one
+// thread per column).
 constexpr int LUR_NB = 16;
 constexpr int LUR_ROWS = 1024;
 
@@ -98,9 +99,13 @@ __global__ __launch_bounds__(LUR_ROWS) void getrf_panel_reg_kernel(int n, int K,
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int r = k + t;
   const bool own = r < n;
-  __shared__ double wv[LUR_ROWS / 64];
-  __shared__ int wi[LUR_ROWS / 64];
-  __shared__ cd prow[LUR_NB], crow[LUR_NB];
+  // One barrier per column: every wave posts its best candidate and that candidate's
+  // row, the owner of row c posts row c; buffers alternate between columns.
+  constexpr int NW = LUR_ROWS / 64;
+  __shared__ double wv[2][NW];
+  __shared__ int wi[2][NW];
+  __shared__ cd wrow[2][NW][LUR_NB];
+  __shared__ cd crow[2][LUR_NB];
   __shared__ cd L11[LUR_NB][LUR_NB + 1];
   __shared__ int pvs[LUR_NB];
   cd a[LUR_NB];
@@ -109,6 +114,7 @@ __global__ __launch_bounds__(LUR_ROWS) void getrf_panel_reg_kernel(int n, int K,
 #pragma unroll
   for (int j = 0; j < LUR_NB; ++j) {
     if (j < kb) {
+      const int par = j & 1;
       const int c = k + j;
       double v = -1.0;
       int vi = 0x7fffffff;
@@ -121,35 +127,35 @@ __global__ __launch_bounds__(LUR_ROWS) void getrf_panel_reg_kernel(int n, int K,
         const int oi = __shfl_xor(vi, o, 64);
         if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
       }
-      if (lane == 0) { wv[wave] = v; wi[wave] = vi; }
+      if (lane == 0) { wv[par][wave] = v; wi[par][wave] = vi; }
+      if (own && r == vi) {
+#pragma unroll
+        for (int jj = 0; jj < LUR_NB; ++jj) wrow[par][wave][jj] = a[jj];
+      }
+      if (own && r == c) {
+#pragma unroll
+        for (int jj = 0; jj < LUR_NB; ++jj) crow[par][jj] = a[jj];
+      }
       __syncthreads();
-      double best = wv[0];
-      int bi = wi[0];
-      for (int w = 1; w < LUR_ROWS / 64; ++w) {
-        const double ov = wv[w];
-        const int oi = wi[w];
-        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      double best = wv[par][0];
+      int bi = wi[par][0], bw = 0;
+      for (int w = 1; w < NW; ++w) {
+        const double ov = wv[par][w];
+        const int oi = wi[par][w];
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; bw = w; }
       }
       const int p = (bi >= c && bi < n) ? bi : c;
       if (t == 0) { ipiv[c] = p; pvs[j] = p; }
       const bool nonsing = best > 0.0;
       if (!nonsing && t == 0 && info && info[b] == 0) info[b] = c + 1;  // getf2 semantics: no elimination
-      if (nonsing && r == p) {
-#pragma unroll
-        for (int jj = 0; jj < LUR_NB; ++jj) prow[jj] = a[jj];
-      }
-      if (nonsing && r == c && p != c) {
-#pragma unroll
-        for (int jj = 0; jj < LUR_NB; ++jj) crow[jj] = a[jj];
-      }
-      __syncthreads();
       if (nonsing) {
+        const cd* prow = wrow[par][bw];  // row p
         if (p != c && r == c) {
 #pragma unroll
           for (int jj = 0; jj < LUR_NB; ++jj) a[jj] = prow[jj];
         } else if (p != c && r == p) {
 #pragma unroll
-          for (int jj = 0; jj < LUR_NB; ++jj) a[jj] = crow[jj];
+          for (int jj = 0; jj < LUR_NB; ++jj) a[jj] = crow[par][jj];
         }
         if (own && r > c) {
           const cd l = cmul(a[j], crecip(prow[j]));
