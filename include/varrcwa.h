@@ -51,7 +51,7 @@ int vrc_timing_enable(vrc_ctx* ctx, int on);
  * "zgemm.large_tile" (default 1): 128x64-tile ZGEMM for m >= 128, n >= 64, else the
  * 64x64 tile; "zgemm.large_tile_min_k" (default 129): smallest k for the 128x64 tile;
  * "zgemm.big_waves", "zgemm.small_waves" (default 8, or 4): wavefronts per workgroup of the
- * 128x64 and 64x64 tiles; "zgeev.aed_window" (default 24, 0 = off, at most 64): deflation-window
+ * 128x64 and 64x64 tiles; "zgemm.big_n" (default 128, or 64): a 128x128 tile for n >= 128; "zgeev.aed_window" (default 24, 0 = off, at most 64): deflation-window
  * size of the aggressive early deflation in the QR iteration; "zgeev.bulges" (default 12,
  * 1..16): double-shift bulges chased per QR sweep; "stack.cascade_group" (default 128):
  * instances of vrc_batch_stack_rcwa whose star products and projections run in one batch.

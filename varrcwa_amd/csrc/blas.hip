@@ -492,6 +492,128 @@ __device__ __forceinline__ void zgemm_tile_big8(int m, int n, int k, cd alpha, c
     }
 }
 
+// 128x128 tile on eight waves: wave w owns rows 64*(w>>2) .. +64 and columns 32*(w&3) .. +32
+// (4x2 blocks of 16x16); K chunk of 16 staged through LDS as above.
+constexpr int GQ_N = 128;
+
+template <int OPA, int OPB>
+__device__ __forceinline__ void zgemm_tile_sq(int m, int n, int k, cd alpha, const cd* __restrict__ A, long lda,
+                                              const cd* __restrict__ B, long ldb, cd beta, cd* __restrict__ C,
+                                              long ldc, int tm, int tn) {
+  __shared__ double As_re[GB_K][GB_M + 1];
+  __shared__ double As_im[GB_K][GB_M + 1];
+  __shared__ double Bs_re[GB_K][GQ_N + 1];
+  __shared__ double Bs_im[GB_K][GQ_N + 1];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wr = wave >> 2, wc = wave & 3;
+  const int row0 = tm * GB_M, col0 = tn * GQ_N;
+  d4 accR[4][2], accI[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      accI[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+  // element e = t + 512*i of the A tile (4 per thread) and of the B tile (4 per thread)
+  auto a_coord = [&](int e, int& r, int& c) {
+    if (OPA == OP_N) { r = e & 127; c = e >> 7; } else { c = e & 15; r = e >> 4; }
+  };
+  auto b_coord = [&](int e, int& c, int& j) {
+    if (OPB == OP_N) { c = e & 15; j = e >> 4; } else { j = e & 127; c = e >> 7; }
+  };
+  cd ra[4], rb[4];
+  auto fetch = [&](int kk) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int r, c;
+      a_coord(t + GB8_THREADS * i, r, c);
+      const int gr = row0 + r, gc = kk + c;
+      cd v = mk(0.0, 0.0);
+      if (gr < m && gc < k) v = (OPA == OP_N) ? A[gr + (long)gc * lda] : A[gc + (long)gr * lda];
+      if (OPA == OP_C) v.y = -v.y;
+      ra[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int c, j;
+      b_coord(t + GB8_THREADS * i, c, j);
+      const int gc = kk + c, gj = col0 + j;
+      cd v = mk(0.0, 0.0);
+      if (gc < k && gj < n) v = (OPB == OP_N) ? B[gc + (long)gj * ldb] : B[gj + (long)gc * ldb];
+      if (OPB == OP_C) v.y = -v.y;
+      rb[i] = v;
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int r, c;
+      a_coord(t + GB8_THREADS * i, r, c);
+      As_re[c][r] = ra[i].x;
+      As_im[c][r] = ra[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int c, j;
+      b_coord(t + GB8_THREADS * i, c, j);
+      Bs_re[c][j] = rb[i].x;
+      Bs_im[c][j] = rb[i].y;
+    }
+  };
+  fetch(0);
+  for (int kk = 0; kk < k; kk += GB_K) {
+    stash();
+    __syncthreads();
+    if (kk + GB_K < k) fetch(kk + GB_K);
+#pragma unroll
+    for (int s = 0; s < GB_K / 4; ++s) {
+      const int kr = s * 4 + (lane >> 4);
+      double ar[4], ai[4], nai[4], br[2], bi[2];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        ar[b] = As_re[kr][wr * 64 + b * 16 + (lane & 15)];
+        ai[b] = As_im[kr][wr * 64 + b * 16 + (lane & 15)];
+        nai[b] = -ai[b];
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        br[b] = Bs_re[kr][wc * 32 + b * 16 + (lane & 15)];
+        bi[b] = Bs_im[kr][wc * 32 + b * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], accR[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  const bool beta0 = (beta.x == 0.0 && beta.y == 0.0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int gc = col0 + wc * 32 + j * 16 + (lane & 15);
+      if (gc >= n) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = row0 + wr * 64 + i * 16 + (lane >> 4) + 4 * r;
+        if (gr >= m) continue;
+        cd out = cmul(alpha, mk(accR[i][j][r], accI[i][j][r]));
+        cd* cp = C + gr + (long)gc * ldc;
+        if (!beta0) out = cadd(out, cmul(beta, *cp));
+        *cp = out;
+      }
+    }
+}
+
 // 1-D grid over (batch, n tile, m tile) with m tiles fastest. Workgroup ids are dealt
 // to the 8 XCDs round-robin; the id is remapped so that every XCD works on one
 // contiguous range of tiles (tiles of the same matrices share that XCD's L2).
@@ -524,6 +646,21 @@ __global__ __launch_bounds__(GB8_THREADS) void zgemm_big8_kernel(int m, int n, i
   const int rem = (int)(logical - b * per);
   const int tn = rem / gm, tm = rem - tn * gm;
   zgemm_tile_big8<OPA, OPB>(m, n, k, alpha, A + b * sA, lda, B + b * sB, ldb, beta, C + b * sC, ldc, tm, tn);
+}
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(GB8_THREADS) void zgemm_sq_kernel(int m, int n, int k, cd alpha, const cd* A, long lda,
+                                                               long sA, const cd* B, long ldb, long sB, cd beta,
+                                                               cd* C, long ldc, long sC, int gm, int gn, long total) {
+  const long id = blockIdx.x;
+  const long q = total / 8, rr = total % 8;
+  const long x = id % 8, slot = id / 8;
+  const long logical = x * q + (x < rr ? x : rr) + slot;
+  const long per = (long)gm * gn;
+  const long b = logical / per;
+  const int rem = (int)(logical - b * per);
+  const int tn = rem / gm, tm = rem - tn * gm;
+  zgemm_tile_sq<OPA, OPB>(m, n, k, alpha, A + b * sA, lda, B + b * sB, ldb, beta, C + b * sC, ldc, tm, tn);
 }
 
 // Scale-only path for k == 0 (C = beta*C), shared by both batch modes.
@@ -578,6 +715,8 @@ static bool g_big_gemm = true;  // large-tile kernel for m >= 128, n >= 64 (swit
 static int g_big_waves = 8;     // wavefronts of the large-tile kernel (4 or 8)
 void zgemm_set_big_waves(int w) { g_big_waves = w == 8 ? 8 : 4; }
 static int g_small_waves = 8;   // wavefronts of the 64x64-tile kernel (4 or 8)
+static int g_big_n = 128;       // columns of the large tile (128 for n >= 128, else 64)
+void zgemm_set_big_n(int nn) { g_big_n = nn == 128 ? 128 : 64; }
 void zgemm_set_small_waves(int w) { g_small_waves = w == 8 ? 8 : 4; }
 static int g_big_min_k = 129;   // ... and k at least this (short-k updates run better on the 64x64 tile)
 void zgemm_set_big(bool on) { g_big_gemm = on; }
@@ -586,6 +725,27 @@ void zgemm_set_big_min_k(int k) { g_big_min_k = k; }
 void zgemm_strided(hipStream_t s, int opA, int opB, int m, int n, int k, cd alpha, const cd* A, long lda,
                    long sA, const cd* B, long ldb, long sB, cd beta, cd* C, long ldc, long sC, int batch) {
   if (m <= 0 || n <= 0 || batch <= 0) return;
+  if (g_big_gemm && g_big_n == 128 && k > 0 && k >= g_big_min_k && m >= GB_M && n >= GQ_N) {
+    const int gm = cdiv_up(m, GB_M), gn = cdiv_up(n, GQ_N);
+    const long total = (long)gm * gn * batch;
+    dim3 grid((unsigned)total);
+#define VRC_SQ(OA, OB) \
+  zgemm_sq_kernel<OA, OB><<<grid, GB8_THREADS, 0, s>>>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, gm, gn, total)
+    switch (opA * 3 + opB) {
+      case 0: VRC_SQ(OP_N, OP_N); break;
+      case 1: VRC_SQ(OP_N, OP_T); break;
+      case 2: VRC_SQ(OP_N, OP_C); break;
+      case 3: VRC_SQ(OP_T, OP_N); break;
+      case 4: VRC_SQ(OP_T, OP_T); break;
+      case 5: VRC_SQ(OP_T, OP_C); break;
+      case 6: VRC_SQ(OP_C, OP_N); break;
+      case 7: VRC_SQ(OP_C, OP_T); break;
+      default: VRC_SQ(OP_C, OP_C); break;
+    }
+#undef VRC_SQ
+    launch_check("zgemm_sq");
+    return;
+  }
   if (g_big_gemm && k > 0 && k >= g_big_min_k && m >= GB_M && n >= GB_N) {
     const int gm = cdiv_up(m, GB_M), gn = cdiv_up(n, GB_N);
     const long total = (long)gm * gn * batch;

@@ -75,6 +75,10 @@ int vrc_tune_set(const char* key, long long value) {
     zgemm_set_small_waves((int)value);
     return 0;
   }
+  if (std::strcmp(key, "zgemm.big_n") == 0) {
+    zgemm_set_big_n((int)value);
+    return 0;
+  }
   if (std::strcmp(key, "zgeev.bulges") == 0) {
     zgeev_set_bulges((int)value);
     return 0;

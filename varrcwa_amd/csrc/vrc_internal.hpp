@@ -63,6 +63,7 @@ void zgemm_set_big(bool on);  // large-tile kernel on/off (default on)
 void zgemm_set_big_min_k(int k);  // smallest k for the large-tile kernel
 void zgemm_set_big_waves(int w);  // wavefronts of the large-tile kernel (4 or 8)
 void zgemm_set_small_waves(int w);  // wavefronts of the 64x64-tile kernel (4 or 8)
+void zgemm_set_big_n(int n);        // columns of the large tile (64 or 128)
 void zgemm_desc(hipStream_t s, int opA, int opB, const GemmDesc* d_desc, int count, int max_m, int max_n);
 void zgeam(hipStream_t s, int m, int n, cd a, const cd* A, long lda, long sA, cd b, const cd* B, long ldb, long sB,
            cd* C, long ldc, long sC, int batch);
