@@ -664,7 +664,7 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     const cd* g0 = geG.as<cd>();
     const cd* g1 = geG.as<cd>() + (long)GB * N;
     LUStore lu1;
-    lu1.make(c, N, W1, V1, B, NN);
+    if (!homog) lu1.make(c, N, W1, V1, B, NN);
     DevBuf Sb = c.ws.get(sizeof(cd) * 4 * NN * B);
     SMat S;
     S.Tdd = Sb.as<cd>();
@@ -681,7 +681,10 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
       const cd* r = ru + l * NN;
       star_product(c, N, B, S, t, t, r, r, (long)L * NN);
     }
-    project_left(c, N, B, S, lu1.pair, W0, V0, NN);
+    if (homog)
+      project_left_homog(c, N, B, S, V0, V1, NN);  // W0 = W1 = I
+    else
+      project_left(c, N, B, S, lu1.pair, W0, V0, NN);
     // ---- outputs
     const long so = 4 * NN;
     cd* So = S_out + (long)i0 * so;

@@ -622,6 +622,107 @@ void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd
 }
 
 // ===================================================================== (d) projections
+// Homogeneous end media (W0 = W1 = I, V0 and V1 coupling only the harmonic pair (i, i+n)):
+// every matrix of the projection built from the end bases is "pair-sparse", a 2x2 block
+// per harmonic i on the indices (i, i+n), stored compactly as 4 values per harmonic
+// (s00, s01, s10, s11). ends_pair_kernel: G_i = V1_i^{-1} V0_i, A_i = (I + G_i)/2,
+// B_i = (I - G_i)/2 (the A, B of project_left with F = W1^{-1} W0 = I).
+__global__ void ends_pair_kernel(int n, const cd* V0, const cd* V1, long sV, cd* Ap, cd* Bp) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int N = 2 * n;
+  const cd* v0 = V0 + b * sV;
+  const cd* v1 = V1 + b * sV;
+  auto at = [&](const cd* M, int r, int c) { return M[(r ? i + n : i) + (long)(c ? i + n : i) * N]; };
+  const cd a = at(v1, 0, 0), bb = at(v1, 0, 1), cc = at(v1, 1, 0), d = at(v1, 1, 1);
+  const cd det = csub(cmul(a, d), cmul(bb, cc));
+  const cd idet = crecip(det);
+  // V1_i^{-1} = [[d, -b], [-c, a]] / det
+  const cd i00 = cmul(d, idet), i01 = cneg(cmul(bb, idet)), i10 = cneg(cmul(cc, idet)), i11 = cmul(a, idet);
+  const cd p00 = at(v0, 0, 0), p01 = at(v0, 0, 1), p10 = at(v0, 1, 0), p11 = at(v0, 1, 1);
+  const cd g00 = cadd(cmul(i00, p00), cmul(i01, p10)), g01 = cadd(cmul(i00, p01), cmul(i01, p11));
+  const cd g10 = cadd(cmul(i10, p00), cmul(i11, p10)), g11 = cadd(cmul(i10, p01), cmul(i11, p11));
+  cd* A = Ap + (b * n + i) * 4;
+  cd* B = Bp + (b * n + i) * 4;
+  A[0] = mk(0.5 * (1.0 + g00.x), 0.5 * g00.y);
+  A[1] = cscale(g01, 0.5);
+  A[2] = cscale(g10, 0.5);
+  A[3] = mk(0.5 * (1.0 + g11.x), 0.5 * g11.y);
+  B[0] = mk(0.5 * (1.0 - g00.x), -0.5 * g00.y);
+  B[1] = cscale(g01, -0.5);
+  B[2] = cscale(g10, -0.5);
+  B[3] = mk(0.5 * (1.0 - g11.x), -0.5 * g11.y);
+}
+
+// out = s1 * (X P) + s2 * Y (X dense N x N, P, Y pair-sparse), or with left = true
+// out = s1 * (P X) + s2 * Y.
+__global__ void pair_mul_kernel(int n, bool left, const cd* X, long sX, const cd* P, cd s1, const cd* Y, cd s2,
+                                cd* out, long so) {
+  const long b = blockIdx.z;
+  const int N = 2 * n;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= N) return;
+  const cd* x = X + b * sX;
+  cd v;
+  if (!left) {
+    const int i = c % n, cb = c / n;
+    const cd* p = P + (b * n + i) * 4;
+    v = cadd(cmul(x[r + (long)i * N], p[cb]), cmul(x[r + (long)(i + n) * N], p[2 + cb]));
+  } else {
+    const int i = r % n, rb = r / n;
+    const cd* p = P + (b * n + i) * 4;
+    v = cadd(cmul(p[2 * rb], x[i + (long)c * N]), cmul(p[2 * rb + 1], x[(i + n) + (long)c * N]));
+  }
+  v = cmul(s1, v);
+  if (Y && (r % n) == (c % n)) v = cadd(v, cmul(s2, Y[(b * n + r % n) * 4 + 2 * (r / n) + c / n]));
+  out[b * so + r + (long)c * N] = v;
+}
+
+static void pair_mul(hipStream_t s, int n, int batch, bool left, const cd* X, long sX, const cd* P, cd s1,
+                     const cd* Y, cd s2, cd* out, long so) {
+  const int N = 2 * n;
+  dim3 g(cdiv_up(N, 256), N, batch);
+  pair_mul_kernel<<<g, 256, 0, s>>>(n, left, X, sX, P, s1, Y, s2, out, so);
+  launch_check("pair_mul");
+}
+
+// project_left for homogeneous end media (same formulas, pair-sparse A and B).
+void project_left_homog(vrc_ctx& c, int N, int batch, SMat& S, const cd* V0, const cd* V1, long sRef) {
+  StageScope st(ST_PROJECT);
+  hipStream_t s = c.stream;
+  const int n = N / 2;
+  const long NN = (long)N * N, s2 = 2 * NN;
+  const size_t B = sizeof(cd) * (size_t)NN * batch;
+  DevBuf Apb = c.ws.get(sizeof(cd) * (size_t)4 * n * batch), Bpb = c.ws.get(sizeof(cd) * (size_t)4 * n * batch);
+  DevBuf Kb = c.ws.get(B), R2b = c.ws.get(2 * B), Tb = c.ws.get(B);
+  DevBuf pivb = c.ws.get(sizeof(int) * (size_t)N * batch), infb = c.ws.get(sizeof(int) * (size_t)batch);
+  cd *Ap = Apb.as<cd>(), *Bp = Bpb.as<cd>(), *K = Kb.as<cd>(), *R2 = R2b.as<cd>(), *T = Tb.as<cd>();
+  {
+    dim3 g(cdiv_up(n, 256), batch);
+    ends_pair_kernel<<<g, 256, 0, s>>>(n, V0, V1, sRef, Ap, Bp);
+    launch_check("ends_pair");
+  }
+  fill_int(s, infb.as<int>(), 0, batch);
+  // K = A - Rdu B ; RHS = [Rdu A - B | Tdd]
+  pair_mul(s, n, batch, false, S.Rdu, NN, Bp, MONE, Ap, ONE, K, NN);
+  pair_mul(s, n, batch, false, S.Rdu, NN, Ap, ONE, Bp, MONE, R2, s2);
+  zgeam(s, N, N, ONE, S.Tdd, N, NN, ZERO, nullptr, 0, 0, R2 + NN, N, s2, batch);
+  zgetrf(s, N, K, N, NN, pivb.as<int>(), N, infb.as<int>(), batch, c.ws);
+  zgetrs(s, N, 2 * N, K, N, NN, pivb.as<int>(), N, R2, N, s2, R2, N, s2, batch, c.ws);
+  zgeam(s, N, N, ONE, R2, N, s2, ZERO, nullptr, 0, 0, S.Rdu, N, NN, batch);
+  zgeam(s, N, N, ONE, R2 + NN, N, s2, ZERO, nullptr, 0, 0, S.Tdd, N, NN, batch);
+  // Rud = (Tuu B) Tdd + Rud
+  pair_mul(s, n, batch, false, S.Tuu, NN, Bp, ONE, nullptr, ZERO, T, NN);
+  mm(s, N, ONE, T, S.Tdd, ONE, S.Rud, NN, batch);
+  // Tuu = Tuu (B Rdu + A)
+  pair_mul(s, n, batch, true, S.Rdu, NN, Bp, ONE, Ap, ONE, K, NN);
+  mm(s, N, ONE, S.Tuu, K, ZERO, T, NN, batch);
+  copy(s, N, N, T, S.Tuu, NN, batch);
+}
+
+
 // Left end (src/core/RCWAIntegrator.cpp:73-86): S expressed with basis (W1,V1) on the left
 // is re-expressed with (W0,V0). LU1 holds the factorizations of W1 and V1.
 void project_left(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LU1, const cd* W0, const cd* V0,

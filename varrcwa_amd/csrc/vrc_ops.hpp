@@ -116,6 +116,8 @@ constexpr int STAR_SAVE = 6;
 void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
                   const cd* rdu, long ss, cd* save = nullptr);
 void project_left(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LU1, const cd* W0, const cd* V0, long sRef);
+// project_left for homogeneous end media (W0 = W1 = I; V0, V1 couple only harmonic pairs).
+void project_left_homog(vrc_ctx& c, int N, int batch, SMat& S, const cd* V0, const cd* V1, long sRef);
 void project_right(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LUl, const cd* W1, const cd* V1,
                    long sRef);
 double redheffer_perturbation(vrc_ctx& c, int N, double k0, double z0, double z1, const cd* Pr, const cd* Qr,
