@@ -227,6 +227,17 @@ typedef struct vrc_stack_instance {
 } vrc_stack_instance;
 int vrc_batch_stack_rcwa(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
                          void* S_out, void* W0, void* V0, void* gam0, void* W1, void* V1, void* gam1);
+/* Plane-wave diffraction efficiencies of solved stacks; replaces the power bookkeeping of
+ * RCWASolver::scatterPlaneWave (src/rcwa/RCWASolver.cpp:1636-1733) for the integrator's
+ * S-matrix. S, W0, W1: device outputs of vrc_batch_stack_rcwa for the same instances;
+ * e_inc: device, per instance the incident field harmonics [Ex; Ey] (2 nx ny complex) in
+ * the bottom medium (modal coefficients W0^{-1} e_inc; the wave travels up). REF, TRN:
+ * device, nx*ny doubles per instance: |Re Kz| / (sqrt(Re eps) k0) (|Ex|^2 + |Ey|^2 + |Ez|^2)
+ * per harmonic of the reflected (bottom, eps of its first cell) and transmitted (top)
+ * field, Ez from the plane-wave divergence condition; NaN for an instance with a
+ * transversely moving component (Kz = 0), which the reference rejects. */
+int vrc_stack_efficiencies(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, const void* S,
+                           const void* W0, const void* W1, const void* e_inc, double* REF, double* TRN);
 
 /* ------------------------------------------------------ adjoint (gradients) */
 /* No reference counterpart: the reference differentiates nothing on this path.

@@ -537,6 +537,35 @@ class SResult:
     tree: List[Tuple[float, float]] = field(default_factory=list)
 
 
+def scatter_plane_wave(r: "SResult", alpha: np.ndarray, beta: np.ndarray, k0: float, eps_ref: complex,
+                       eps_trn: complex, e_inc: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """Diffraction efficiencies per harmonic, the power bookkeeping of
+    RCWASolver::scatterPlaneWave (src/rcwa/RCWASolver.cpp:1636-1733) applied to the
+    integrator's S-matrix with the wave incident from the bottom medium.
+
+    e_inc: incident field harmonics [Ex; Ey] (2n) in the bottom medium; its modal
+    coefficients are c = W0^{-1} e_inc. Reflected coefficients Rdu c (bottom basis),
+    transmitted Tuu c (top basis); fields E = W c per medium, Ez from the plane-wave
+    divergence condition with Kz = -/+ sqrt(k0^2 eps - kx^2 - ky^2) (reflected / transmitted,
+    src/rcwa/RCWASolver.cpp:1671-1700), and per harmonic
+    REF_i = |Re Kz_i| / (sqrt(Re eps_ref) k0) * (|Ex|^2 + |Ey|^2 + |Ez|^2), TRN likewise with
+    the top medium (src/rcwa/RCWASolver.cpp:1706-1732)."""
+    n = alpha.shape[0]
+    c = np.linalg.solve(r.W0, e_inc)
+    rxy = r.W0 @ (r.Rdu @ c)
+    txy = r.W1 @ (r.Tuu @ c)
+
+    def power(xy, eps, sign):
+        kz = sign * np.sqrt(k0 * k0 * eps - alpha * alpha - beta * beta)
+        if np.any(kz == 0):
+            raise ValueError("a plane-wave component moves transversely")
+        ex, ey = xy[:n], xy[n:]
+        ez = (alpha * ex + beta * ey) / kz
+        return np.abs(kz.real) / (np.sqrt(np.real(eps)) * k0) * (np.abs(ex) ** 2 + np.abs(ey) ** 2 + np.abs(ez) ** 2)
+
+    return power(rxy, eps_ref, -1.0), power(txy, eps_trn, 1.0)
+
+
 def rcwa_integrate(sampler: LayerSampler, z0: float, z1: float, N: int,
                    z_edges: Optional[Sequence[float]] = None) -> SResult:
     """RCWAIntegrator::compute(N) (src/core/RCWAIntegrator.cpp:20-87).

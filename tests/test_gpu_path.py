@@ -257,6 +257,71 @@ def test_batch_stack_patterned_ends(ctx):
         invariants_close(got, ref)
 
 
+def _efficiency_case(seed, cell=1.4, lam=0.9):
+    rng = np.random.default_rng(seed)
+    layers = [G.random_rect_layer(rng, cell) for _ in range(2)]
+    return dict(lam=lam, layers=layers, thickness=np.array([0.13, 0.18]), bottom=G.homogeneous(cell, 2.1),
+                top=G.homogeneous(cell, 1.0))
+
+
+def _efficiency_check(ctx, nx, ny, insts, e_inc, tol=1e-10):
+    import torch
+    n = nx * ny
+    out = api.batch_stack_rcwa(ctx, nx, ny, insts, want_bases=True)
+    REF, TRN = api.stack_efficiencies(ctx, nx, ny, insts, out,
+                                      torch.from_numpy(e_inc).to(f"cuda:{ctx.device}").contiguous())
+    REF, TRN = REF.cpu().numpy(), TRN.cpu().numpy()
+    for i, inst in enumerate(insts):
+        ze = np.concatenate([[0.0], np.cumsum(inst["thickness"])])
+        so = O.StackLayerSampler(inst["lam"], nx, ny, inst["layers"], ze, inst["bottom"], inst["top"])
+        ref = O.rcwa_integrate(so, ze[0], ze[-1], len(inst["layers"]), z_edges=ze)
+        cell = inst["bottom"][0][-1] - inst["bottom"][0][0]
+        al, be = O.alpha_beta(nx, ny, cell, cell)
+        r0, t0 = O.scatter_plane_wave(ref, al, be, 2 * O.PI / inst["lam"], inst["bottom"][2][0, 0],
+                                      inst["top"][2][0, 0], e_inc[i])
+        scale = max(np.abs(r0).max(), np.abs(t0).max())
+        assert np.abs(REF[i] - r0).max() <= tol * scale, (i, np.abs(REF[i] - r0).max())
+        assert np.abs(TRN[i] - t0).max() <= tol * scale, (i, np.abs(TRN[i] - t0).max())
+    return REF, TRN
+
+
+def test_stack_efficiencies(ctx):
+    """Plane-wave diffraction efficiencies of batched stacks (RCWASolver::scatterPlaneWave)
+    against the oracle's bookkeeping on the oracle's S-matrix, to 1e-10 of the largest
+    efficiency: x- and y-polarised normally incident waves and a random field over all
+    harmonics. The lossless stacks with several propagating orders keep the power balance
+    for the plane waves; each side is normalised by its own medium's sqrt(Re eps) k0
+    (src/rcwa/RCWASolver.cpp:1732-1733), so sum(REF) + sum(TRN) sqrt(eps_trn / eps_ref) = 1."""
+    nx, ny = 5, 3
+    n = nx * ny
+    insts = [_efficiency_case(s) for s in range(3)]
+    e = np.zeros((3, 2 * n), dtype=complex)
+    c0 = (nx // 2) * ny + ny // 2
+    e[0, c0] = 1.0
+    e[1, n + c0] = 1.0
+    rng = np.random.default_rng(5)
+    e[2] = rng.standard_normal(2 * n) + 1j * rng.standard_normal(2 * n)
+    REF, TRN = _efficiency_check(ctx, nx, ny, insts, e)
+    for i in (0, 1):
+        assert (TRN[i] > 1e-6).sum() >= 3  # several transmitted orders carry power
+        bal = REF[i].sum() + TRN[i].sum() * np.sqrt(1.0 / 2.1)
+        assert abs(bal - 1.0) < 1e-10, (i, bal)
+
+
+def test_stack_efficiencies_patterned_bottom(ctx):
+    """A patterned bottom medium gives a dense end basis W0: the incident coefficients then
+    come from the LU solve W0 c = e_inc."""
+    nx = ny = 3
+    rng = np.random.default_rng(21)
+    insts = []
+    for s in range(2):
+        c = _efficiency_case(s, cell=0.6, lam=1.1)
+        c["bottom"] = G.random_rect_layer(rng, 0.6)
+        insts.append(c)
+    e = rng.standard_normal((2, 2 * nx * ny)) + 1j * rng.standard_normal((2, 2 * nx * ny))
+    _efficiency_check(ctx, nx, ny, insts, e, tol=1e-9)
+
+
 def test_homogeneous_closed_form(ctx):
     """Single-value end media: W = I, PQ W = W Gamma^2 and V = Q W / Gamma against the
     oracle's P and Q (the closed form of a homogeneous medium)."""

@@ -276,6 +276,83 @@ def test_energy_conservation_lossless_stack():
         energy_balance(bad, 2 * O.PI / lam, 1e-10)
 
 
+def _plane_wave_case(nx, ny, layers, ze, lam, bottom, top, pol):
+    cell = bottom[0][-1] - bottom[0][0]
+    s = O.StackLayerSampler(lam, nx, ny, layers, ze, bottom, top)
+    r = O.rcwa_integrate(s, ze[0], ze[-1], len(layers), z_edges=ze)
+    al, be = O.alpha_beta(nx, ny, cell, cell)
+    n = nx * ny
+    e = np.zeros(2 * n, dtype=complex)
+    e[(nx // 2) * ny + ny // 2 + (0 if pol == "x" else n)] = 1.0
+    return O.scatter_plane_wave(r, al, be, 2 * O.PI / lam, bottom[2][0, 0], top[2][0, 0], e)
+
+
+def test_plane_wave_homogeneous_slab_airy():
+    """A homogeneous slab (index ns, thickness d) in air at normal incidence: only the
+    (0,0) order carries power, T = 1 / (1 + F sin^2(k0 ns d)) with F = ((ns^2 - 1) / (2 ns))^2
+    (Airy), and R = 1 - T."""
+    nx = ny = 3
+    cell, lam, d, eps = 0.7, 1.3, 0.37, 5.2
+    slab = G.homogeneous(cell, eps)
+    for pol in ("x", "y"):
+        REF, TRN = _plane_wave_case(nx, ny, [slab], np.array([0.0, d]), lam, G.homogeneous(cell, 1.0),
+                                    G.homogeneous(cell, 1.0), pol)
+        c = (nx // 2) * ny + ny // 2
+        ns, k0 = np.sqrt(eps), 2 * O.PI / lam
+        F = ((ns * ns - 1) / (2 * ns)) ** 2
+        T = 1.0 / (1.0 + F * np.sin(k0 * ns * d) ** 2)
+        assert abs(TRN[c] - T) < 1e-12 and abs(REF[c] - (1 - T)) < 1e-12, (pol, TRN[c], T)
+        others = np.ones(nx * ny, bool)
+        others[c] = False
+        assert np.abs(TRN[others]).max() < 1e-20 and np.abs(REF[others]).max() < 1e-20
+
+
+def test_plane_wave_energy_conservation():
+    """Lossless patterned stack between equal media: the efficiencies of all orders
+    (propagating ones carry power, evanescent ones have Re Kz = 0) sum to 1; dropping Ez
+    from the bookkeeping breaks the balance (oblique diffraction orders carry it)."""
+    rng = np.random.default_rng(11)
+    nx = ny = 5
+    cell, lam = 1.4, 0.9
+    layers = [G.random_rect_layer(rng, cell) for _ in range(2)]
+    ze = np.array([0.0, 0.13, 0.31])
+    air = G.homogeneous(cell, 1.0)
+    for pol in ("x", "y"):
+        REF, TRN = _plane_wave_case(nx, ny, layers, ze, lam, air, air, pol)
+        assert (TRN > 1e-6).sum() >= 3  # several propagating diffraction orders
+        assert abs(REF.sum() + TRN.sum() - 1.0) < 1e-10, (pol, REF.sum() + TRN.sum())
+    # planted mistake: Ez left out
+    s = O.StackLayerSampler(lam, nx, ny, layers, ze, air, air)
+    r = O.rcwa_integrate(s, ze[0], ze[-1], 2, z_edges=ze)
+    al, be = O.alpha_beta(nx, ny, cell, cell)
+    n = nx * ny
+    e = np.zeros(2 * n, dtype=complex)
+    e[(nx // 2) * ny + ny // 2] = 1.0
+    k0 = 2 * O.PI / lam
+    txy = r.W1 @ (r.Tuu @ np.linalg.solve(r.W0, e))
+    rxy = r.W0 @ (r.Rdu @ np.linalg.solve(r.W0, e))
+    kz = np.sqrt(k0 * k0 - al * al - be * be)
+    noz = sum((np.abs(kz.real) / k0 * (np.abs(v[:n]) ** 2 + np.abs(v[n:]) ** 2)).sum() for v in (rxy, txy))
+    assert abs(noz - 1.0) > 1e-5  # far outside the 1e-10 balance above
+
+
+def test_plane_wave_unequal_media_normalisation():
+    """Incident from a substrate (eps 2.1) into air: the reference divides each side by its
+    own medium's sqrt(Re eps) k0 (src/rcwa/RCWASolver.cpp:1732-1733), so the physical balance
+    is sum(REF) + sum(TRN) sqrt(eps_trn / eps_ref) = 1; dividing TRN by the incident medium's
+    index instead (the plausible slip) misses it by the index ratio."""
+    rng = np.random.default_rng(12)
+    nx, ny = 5, 3
+    cell, lam = 1.4, 0.9
+    layers = [G.random_rect_layer(rng, cell) for _ in range(2)]
+    ze = np.array([0.0, 0.13, 0.31])
+    for pol in ("x", "y"):
+        REF, TRN = _plane_wave_case(nx, ny, layers, ze, lam, G.homogeneous(cell, 2.1), G.homogeneous(cell, 1.0),
+                                    pol)
+        assert abs(REF.sum() + TRN.sum() * np.sqrt(1.0 / 2.1) - 1.0) < 1e-10
+        assert abs(REF.sum() + TRN.sum() / np.sqrt(2.1) * np.sqrt(1.0 / 2.1) - 1.0) > 1e-2
+
+
 def test_convergence_rcwa_varrcwa_differential():
     g = G.metasurface(1)
     s = O.VarLayerSampler(g["lam"], g["nx"], g["ny"], g["x"], g["y"], g["eps"], False)

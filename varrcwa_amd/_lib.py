@@ -89,6 +89,7 @@ def load():
         "vrc_integrator_counts": (i, vp, c_int_p, c_int_p),
         "vrc_integrator_tree": (i, vp, i, c_double_p),
         "vrc_batch_stack_rcwa": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp),
+        "vrc_stack_efficiencies": (i, vp, i, i, i, C.POINTER(VrcStackInstance), vp, vp, vp, vp, vp, vp),
         "vrc_batch_stack_forward": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp,
                                     C.POINTER(vp)),
         "vrc_tape_cells": (C.c_longlong, vp),

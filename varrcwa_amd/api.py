@@ -316,6 +316,33 @@ def batch_stack_rcwa(ctx: Context, nx: int, ny: int, instances, chunk: int = 0, 
     return out
 
 
+def stack_efficiencies(ctx: Context, nx: int, ny: int, instances, solved, e_inc):
+    """Plane-wave diffraction efficiencies (RCWASolver::scatterPlaneWave,
+    src/rcwa/RCWASolver.cpp:1636-1733) of stacks solved by ``batch_stack_rcwa`` with
+    ``want_bases=True``. e_inc: device complex128 (ninst, 2 nx ny), the incident field
+    harmonics [Ex; Ey] in the bottom medium, travelling up. Returns (REF, TRN), device
+    float64 (ninst, nx ny): efficiency per order of the reflected and transmitted field."""
+    torch = ctx.torch
+    n = nx * ny
+    B = len(instances)
+    if "W0" not in solved or "W1" not in solved:
+        raise ValueError("stack_efficiencies needs the end bases (batch_stack_rcwa(..., want_bases=True))")
+    if tuple(e_inc.shape) != (B, 2 * n) or e_inc.dtype != torch.complex128 or not e_inc.is_contiguous():
+        raise ValueError("e_inc must be a contiguous complex128 tensor of shape (ninst, 2 nx ny)")
+    for name in ("S", "W0", "W1"):
+        if solved[name].shape[0] != B:
+            raise ValueError(f"{name} holds {solved[name].shape[0]} instances, expected {B}")
+    arr, keep = _stack_array(instances)
+    dev = f"cuda:{ctx.device}"
+    REF = torch.empty((B, n), dtype=torch.float64, device=dev)
+    TRN = torch.empty((B, n), dtype=torch.float64, device=dev)
+    ctx.check(ctx.lib.vrc_stack_efficiencies(ctx.h, nx, ny, B, arr, ptr(solved["S"]), ptr(solved["W0"]),
+                                             ptr(solved["W1"]), ptr(e_inc), ptr(REF), ptr(TRN)),
+              "stack_efficiencies")
+    del keep
+    return REF, TRN
+
+
 # ------------------------------------------------------------------ stage entry points
 def assemble_toeplitz(ctx: Context, nx: int, ny: int, layers):
     torch = ctx.torch

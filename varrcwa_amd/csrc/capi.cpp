@@ -566,6 +566,16 @@ int vrc_batch_stack_rcwa(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_
   return vrc_batch_stack_forward(c, nx, ny, ninst, inst, chunk, S_out, W0, V0, g0, W1, V1, g1, nullptr);
 }
 
+int vrc_stack_efficiencies(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, const void* S,
+                           const void* W0, const void* W1, const void* e_inc, double* REF, double* TRN) {
+  return guarded(c, [&] {
+    check_harmonics(nx, ny);
+    need(ninst >= 0 && (ninst == 0 || (inst && S && W0 && W1 && e_inc && REF && TRN)), "bad arguments");
+    HostStack hs(ninst, inst);
+    stack_efficiencies(*c, nx, ny, hs.si, (const cd*)S, (const cd*)W0, (const cd*)W1, (const cd*)e_inc, REF, TRN);
+  });
+}
+
 int vrc_batch_stack_forward(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
                             void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1,
                             vrc_tape** tape) {

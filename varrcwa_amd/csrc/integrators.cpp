@@ -705,6 +705,49 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
   }
 }
 
+// The power bookkeeping of RCWASolver::scatterPlaneWave (src/rcwa/RCWASolver.cpp:1636-1733)
+// on the batched stack S-matrices: c = W0^{-1} e_inc, reflected W0 Rdu c, transmitted W1 Tuu c.
+void stack_efficiencies(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, const cd* S,
+                        const cd* W0, const cd* W1, const cd* e_inc, double* REF, double* TRN) {
+  hipStream_t st = c.stream;
+  const int n = nx * ny, N = 2 * n, B = (int)inst.size();
+  if (B == 0) return;
+  const long NN = (long)N * N;
+  std::vector<cd> ab((size_t)B * 2 * n), eps2((size_t)2 * B);
+  std::vector<double> k0v(B);
+  for (int b = 0; b < B; ++b) {
+    const StackInstance& I = inst[b];
+    if (!(I.lambda > 0)) throw ArgFailure("wavelength must be positive");
+    std::vector<cd> al, be;
+    alpha_beta(nx, ny, I.bottom->x.back() - I.bottom->x.front(), I.bottom->y.back() - I.bottom->y.front(), al, be);
+    std::copy(al.begin(), al.end(), ab.begin() + (size_t)b * 2 * n);
+    std::copy(be.begin(), be.end(), ab.begin() + (size_t)b * 2 * n + n);
+    k0v[b] = 2 * PI_REF / I.lambda;
+    eps2[2 * b] = I.bottom->eps[0];  // eps(0, 0) of the medium, as the reference reads it
+    eps2[2 * b + 1] = I.top->eps[0];
+  }
+  DevBuf dab = upload(c, ab.data(), sizeof(cd) * ab.size());
+  DevBuf deps = upload(c, eps2.data(), sizeof(cd) * eps2.size());
+  DevBuf dk0 = upload(c, k0v.data(), sizeof(double) * k0v.size());
+  // c = W0^{-1} e_inc
+  DevBuf LU = c.ws.get(sizeof(cd) * NN * B), piv = c.ws.get(sizeof(int) * (size_t)N * B),
+         inf = c.ws.get(sizeof(int) * (size_t)B);
+  DevBuf cb = c.ws.get(sizeof(cd) * (size_t)N * B), cr = c.ws.get(sizeof(cd) * (size_t)N * B),
+         ctr = c.ws.get(sizeof(cd) * (size_t)N * B), fr = c.ws.get(sizeof(cd) * (size_t)N * B),
+         ft = c.ws.get(sizeof(cd) * (size_t)N * B);
+  fill_int(st, inf.as<int>(), 0, B);
+  zgeam(st, N, N, ONE, W0, N, NN, ZERO, nullptr, 0, 0, LU.as<cd>(), N, NN, B);
+  zgetrf(st, N, LU.as<cd>(), N, NN, piv.as<int>(), N, inf.as<int>(), B, c.ws);
+  zgetrs(st, N, 1, LU.as<cd>(), N, NN, piv.as<int>(), N, e_inc, N, N, cb.as<cd>(), N, N, B, c.ws);
+  // reflected Rdu c, transmitted Tuu c; fields W0 (Rdu c), W1 (Tuu c)
+  const long sS = 4 * NN;
+  zgemm_strided(st, OP_N, OP_N, N, 1, N, ONE, S + 3 * NN, N, sS, cb.as<cd>(), N, N, ZERO, cr.as<cd>(), N, N, B);
+  zgemm_strided(st, OP_N, OP_N, N, 1, N, ONE, S + NN, N, sS, cb.as<cd>(), N, N, ZERO, ctr.as<cd>(), N, N, B);
+  zgemm_strided(st, OP_N, OP_N, N, 1, N, ONE, W0, N, NN, cr.as<cd>(), N, N, ZERO, fr.as<cd>(), N, N, B);
+  zgemm_strided(st, OP_N, OP_N, N, 1, N, ONE, W1, N, NN, ctr.as<cd>(), N, N, ZERO, ft.as<cd>(), N, N, B);
+  efficiencies(st, n, B, fr.as<cd>(), ft.as<cd>(), dab.as<cd>(), dk0.as<double>(), deps.as<cd>(), REF, TRN);
+}
+
 }  // namespace vrc
 
 // ===================================================================== adjoint of batched stacks

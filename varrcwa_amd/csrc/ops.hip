@@ -621,6 +621,43 @@ void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd
   zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, tuu, N, ss, Yt, N, s2, ZERO, S.Tuu, N, NN, batch);
 }
 
+// ===================================================================== plane-wave efficiencies
+// Per instance b and harmonic i (RCWASolver::scatterPlaneWave, src/rcwa/RCWASolver.cpp:1671-1732):
+// Kz = sqrt(k0^2 eps - kx^2 - ky^2), Ez = (kx Ex + ky Ey) / (-/+ Kz) for the reflected /
+// transmitted field, efficiency = |Re Kz| / (sqrt(Re eps) k0) (|Ex|^2 + |Ey|^2 + |Ez|^2).
+// A component with Kz == 0 (a plane wave moving transversely, which the reference rejects)
+// makes the instance's efficiencies NaN.
+__global__ void efficiency_kernel(int n, const cd* rxy, const cd* txy, const cd* ab, const double* k0v,
+                                  const cd* eps2, double* REF, double* TRN) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double k0 = k0v[b];
+  const cd kx = ab[b * 2 * n + i], ky = ab[b * 2 * n + n + i];
+  const cd t2 = cadd(cmul(kx, kx), cmul(ky, ky));
+  for (int side = 0; side < 2; ++side) {
+    const cd eps = eps2[2 * b + side];
+    const cd* f = (side == 0 ? rxy : txy) + b * 2 * n;
+    const cd kz = csqrt_principal(csub(cscale(eps, k0 * k0), t2));
+    double out;
+    if (kz.x == 0.0 && kz.y == 0.0) {
+      out = nan("");
+    } else {
+      const cd ex = f[i], ey = f[n + i];
+      const cd ez = cdiv(cadd(cmul(kx, ex), cmul(ky, ey)), kz);  // the sign of Kz drops out of |Ez|
+      out = fabs(kz.x) / (sqrt(eps.x) * k0) * (cnorm2(ex) + cnorm2(ey) + cnorm2(ez));
+    }
+    (side == 0 ? REF : TRN)[b * n + i] = out;
+  }
+}
+
+void efficiencies(hipStream_t s, int n, int batch, const cd* rxy, const cd* txy, const cd* ab, const double* k0v,
+                  const cd* eps2, double* REF, double* TRN) {
+  dim3 g(cdiv_up(n, 256), batch);
+  efficiency_kernel<<<g, 256, 0, s>>>(n, rxy, txy, ab, k0v, eps2, REF, TRN);
+  launch_check("efficiency");
+}
+
 // ===================================================================== (d) projections
 // Homogeneous end media (W0 = W1 = I, V0 and V1 coupling only the harmonic pair (i, i+n)):
 // every matrix of the projection built from the end bases is "pair-sparse", a 2x2 block

@@ -213,6 +213,15 @@ struct StackInstance {
   const HostLayer* top;
 };
 struct StackTape;
+// Plane-wave diffraction efficiencies per instance (n per instance) from the reflected and
+// transmitted field harmonics rxy, txy ([Ex; Ey], 2n per instance).
+void efficiencies(hipStream_t s, int n, int batch, const cd* rxy, const cd* txy, const cd* ab, const double* k0v,
+                  const cd* eps2, double* REF, double* TRN);
+// For instances solved by batch_stack_rcwa (S, end bases W0, W1 in its output layout): the
+// wave e_inc ([Ex; Ey] harmonics, 2n per instance) incident from the bottom medium;
+// REF, TRN: efficiency per harmonic of the reflected / transmitted orders.
+void stack_efficiencies(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, const cd* S,
+                        const cd* W0, const cd* W1, const cd* e_inc, double* REF, double* TRN);
 // Out: S[inst] = [Tdd|Tuu|Rud|Rdu] (4 N x N); optional W0/V0/gam0/W1/V1/gam1 per instance.
 // With a tape, the eigen-systems of every instance are kept for the reverse pass.
 void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int chunk, cd* S_out,
