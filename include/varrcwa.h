@@ -238,6 +238,24 @@ int vrc_batch_stack_rcwa(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stac
  * transversely moving component (Kz = 0), which the reference rejects. */
 int vrc_stack_efficiencies(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, const void* S,
                            const void* W0, const void* W1, const void* e_inc, double* REF, double* TRN);
+/* Power carried by modes through a window; replaces evaluateEnergy
+ * (src/tests/test_cpu.cpp:335-412) for nmodes modes in one call. W, V: device, column b
+ * (at W + b*ldw, V + b*ldv, ld >= 2 nx ny) holds one mode's field harmonics [Ex; Ey] and
+ * [Hx; Hy] (index j + i*ny, i the x-harmonic), as the columns of the integrator's W0/V0 or
+ * W1/V1. energy: device, nmodes doubles: 0.5 Re(Ex Hy* - Ey Hx*) dx dy summed over the
+ * midpoints of an nres_x x nres_y grid on [x0, x1] x [y0, y1], fields from the wavelets
+ * exp(-i 2 pi m x / Lx), exp(-i 2 pi n y / Ly). The same sum is formed exactly through the
+ * window Gram matrices of the wavelets, so the cost does not grow with the grid. */
+int vrc_modal_power(vrc_ctx* ctx, int nx, int ny, int nmodes, const void* W, long long ldw, const void* V,
+                    long long ldv, double Lx, double Ly, int nres_x, int nres_y, double x0, double x1, double y0,
+                    double y1, double* energy);
+/* findGuidedModes (src/tests/test_cpu.cpp:311-333), host only: the modes whose gamma/k0^2
+ * has real part in [re_min, re_max] and imaginary part below imag_coeff, by decreasing real
+ * part (ties keep index order). gamma: host, n complex; index_out (n ints) and neff_out
+ * (n complex gamma/k0^2): host; *count: number found. Makes no device call; ctx may be
+ * NULL (errors then carry no text). */
+int vrc_find_guided_modes(vrc_ctx* ctx, int n, const void* gamma, double k0, double imag_coeff, double re_min,
+                          double re_max, int* index_out, void* neff_out, int* count);
 
 /* ------------------------------------------------------ adjoint (gradients) */
 /* No reference counterpart: the reference differentiates nothing on this path.

@@ -566,6 +566,46 @@ def scatter_plane_wave(r: "SResult", alpha: np.ndarray, beta: np.ndarray, k0: fl
     return power(rxy, eps_ref, -1.0), power(txy, eps_trn, 1.0)
 
 
+def evaluate_energy(e: np.ndarray, h: np.ndarray, nx: int, ny: int, Lx: float, Ly: float, nres_x: int,
+                    nres_y: int, x0: float, x1: float, y0: float, y1: float) -> float:
+    """Power of one mode through a window, evaluateEnergy (src/tests/test_cpu.cpp:335-412):
+    the field harmonics e = [Ex; Ey], h = [Hx; Hy] (index j + i ny, i the x-harmonic,
+    :352-361) are summed to fields at the midpoints of an nres_x x nres_y grid over
+    [x0, x1] x [y0, y1] with wavelets exp(-i 2 pi m x / Lx), exp(-i 2 pi n y / Ly)
+    (:379-399), and 0.5 Re(Ex Hy* - Ey Hx*) dx dy is added up over the grid (:401-408)."""
+    n = nx * ny
+    kex, key = e[:n].reshape(nx, ny), e[n:].reshape(nx, ny)
+    khx, khy = h[:n].reshape(nx, ny), h[n:].reshape(nx, ny)
+    xstep, ystep = (x1 - x0) / nres_x, (y1 - y0) / nres_y
+    xs = x0 + xstep * (np.arange(nres_x) + 0.5)
+    ys = y0 + ystep * (np.arange(nres_y) + 0.5)
+    mx = np.arange(nx) - (nx - 1) // 2
+    my = np.arange(ny) - (ny - 1) // 2
+    wx = np.exp(-2j * PI * np.outer(xs, mx) / Lx)  # (nres_x, nx)
+    wy = np.exp(-2j * PI * np.outer(ys, my) / Ly)  # (nres_y, ny)
+
+    def field(k):
+        return wx @ k @ wy.T  # (nres_x, nres_y)
+
+    ex, ey, hx, hy = field(kex), field(key), field(khx), field(khy)
+    return float(0.5 * np.real(ex * np.conj(hy) - ey * np.conj(hx)).sum() * xstep * ystep)
+
+
+def find_guided_modes(gamma: np.ndarray, k0: float, imag_coeff: float, re_min: float,
+                      re_max: float) -> List[Tuple[int, complex]]:
+    """findGuidedModes (src/tests/test_cpu.cpp:311-333): the modes whose gamma / k0^2 has real
+    part in [re_min, re_max] and imaginary part below imag_coeff, as (index, gamma / k0^2),
+    sorted by decreasing real part (ties keep index order; the reference's std::sort leaves
+    their order unspecified)."""
+    rec = []
+    for i, g in enumerate(np.asarray(gamma)):
+        gi = complex(g.real / k0 / k0, g.imag / k0 / k0)  # std::complex / double: per component
+        if re_min <= gi.real <= re_max and gi.imag < imag_coeff:
+            rec.append((i, complex(gi)))
+    rec.sort(key=lambda r: -r[1].real)
+    return rec
+
+
 def rcwa_integrate(sampler: LayerSampler, z0: float, z1: float, N: int,
                    z_edges: Optional[Sequence[float]] = None) -> SResult:
     """RCWAIntegrator::compute(N) (src/core/RCWAIntegrator.cpp:20-87).

@@ -39,6 +39,26 @@ def var12(nhx=2, nhy=2):
                 z0=-0.1, z1=10.1)
 
 
+def guided_benchmark(nhx=2, nhy=2):
+    """src/tests/test_cpu.cpp:416-451 (PML waveguide whose core edge moves 3.6 -> 4.7 over
+    z in [0, 10]; the sampler's enablePML defaults to true, src/core/VarLayerSampler.h:19-37)."""
+    x = np.array([0.0, 0.5, 2.5, 3.6, 4.7, 6.7, 7.2])
+    y = np.array([0.0, 0.5, 2.0, 2.08, 2.3, 3.8, 4.3])
+    cl, co = 1.445 * 1.445, 3.48 * 3.48
+    eps = np.full((6, 6), cl, dtype=complex)
+    eps[3, 2] = co
+
+    def rule(z):
+        if z < 0.0:
+            return 3.6
+        if z > 10.0:
+            return 4.7
+        return (1.0 - 0.1 * z) * 3.6 + 0.1 * z * 4.7
+
+    return dict(lam=1.55, nx=2 * nhx + 1, ny=2 * nhy + 1, x=x, y=y, eps=eps, pml=True, rules={3: rule},
+                z0=-0.1, z1=10.1)
+
+
 def oracle_sampler(g):
     from oracle import rcwa_oracle as O
     s = O.VarLayerSampler(g["lam"], g["nx"], g["ny"], g["x"], g["y"], g["eps"], g["pml"])

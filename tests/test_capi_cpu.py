@@ -27,3 +27,21 @@ def test_null_context_is_rejected():
     out = C.c_void_p()
     assert lib.vrc_ctx_create(0, None, None) == -1
     assert out.value is None
+
+
+def test_find_guided_modes_host_entry_point():
+    """vrc_find_guided_modes runs on the host (NULL context): the oracle's selection and order
+    on random spectra with repeated real parts, and rejection of k0 <= 0."""
+    import numpy as np
+
+    from oracle import rcwa_oracle as O
+    from varrcwa_amd import api
+    rng = np.random.default_rng(4)
+    k0 = 2 * O.PI / 1.55
+    g = k0 * k0 * (rng.uniform(0.5, 4.0, 200) + 1j * rng.uniform(-0.5, 0.5, 200))
+    g[10] = g[20] = g[30]  # ties keep index order
+    assert api.find_guided_modes(None, g, k0, 0.2, 1.445, 3.48) == O.find_guided_modes(g, k0, 0.2, 1.445, 3.48)
+    assert api.find_guided_modes(None, g[:0], k0, 0.2, 1.445, 3.48) == []
+    lib = _lib.load()
+    cnt = C.c_int(0)
+    assert lib.vrc_find_guided_modes(None, 1, g.ctypes.data, -1.0, 0.2, 1.0, 2.0, None, None, C.byref(cnt)) == -1

@@ -259,7 +259,7 @@ def test_stack_gradient_small(ctx):
             jm = float(_objective(ctx, prob, torch.from_numpy(em).to("cuda")))
             fd[k] = (jp - jm) / (2 * h)
     err = np.linalg.norm(g - fd) / np.linalg.norm(fd)
-    print(f"J={float(J):.10f} |g|={np.linalg.norm(g):.3e} rel err vs GPU FD {err:.2e}", flush=True)
+    print(f"J={float(J.detach()):.10f} |g|={np.linalg.norm(g):.3e} rel err vs GPU FD {err:.2e}", flush=True)
     assert err < TOL_GRAD
     # CPU oracle: J itself and finite differences on 4 cells (instance 0, slab 0)
     j_or = sum(_oracle_objective(i, nx, ny) for i in insts)

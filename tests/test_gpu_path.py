@@ -322,6 +322,62 @@ def test_stack_efficiencies_patterned_bottom(ctx):
     _efficiency_check(ctx, nx, ny, insts, e, tol=1e-9)
 
 
+def test_modal_power_guided_modes(ctx):
+    """The guided-mode benchmark of src/tests/test_cpu.cpp:416-507 at 7 x 5 harmonics: the
+    guided modes of both ends (findGuidedModes), the power of every end mode on the driver's
+    500 x 500 grid over [0.5, 6.7] x [0.5, 3.8] against the oracle's direct field sum
+    (1e-10 of the largest |power|, 1e-9 relative for guided modes), the driver's normalised
+    transmissions |Tuu(i,j)|^2 e1_i / e0_j, Parseval on the full cell, and a padded leading
+    dimension."""
+    import torch
+    from varrcwa_amd._lib import ptr
+    g = G.guided_benchmark(3, 2)
+    nx, ny = g["nx"], g["ny"]
+    n, N = nx * ny, 2 * nx * ny
+    it = api.RCWAIntegrator(g["z0"], g["z1"], G.gpu_sampler(g, ctx))
+    it.compute(6)
+    k0 = 2 * O.PI / g["lam"]
+    dev = f"cuda:{ctx.device}"
+    win = (7.2, 4.3, 500, 500, 0.5, 6.7, 0.5, 3.8)
+    energies, guided = [], []
+    for W, V, gam in ((it.W0(), it.V0(), it.gam0()), (it.W1(), it.V1(), it.gam1())):
+        rec = api.find_guided_modes(ctx, gam, k0, 0.2, 1.445, 3.48)
+        assert rec == O.find_guided_modes(gam, k0, 0.2, 1.445, 3.48)
+        assert len(rec) >= 1 and 1.445 < rec[0][1].real < 3.48
+        Wd = torch.from_numpy(np.ascontiguousarray(W.T)).to(dev)
+        Vd = torch.from_numpy(np.ascontiguousarray(V.T)).to(dev)
+        e_gpu = api.modal_power(ctx, nx, ny, Wd, Vd, *win).cpu().numpy()
+        e_ref = np.array([O.evaluate_energy(W[:, i], V[:, i], nx, ny, *win) for i in range(N)])
+        assert np.abs(e_gpu - e_ref).max() <= 1e-10 * np.abs(e_ref).max()
+        for i, _ in rec:
+            assert abs(e_gpu[i] - e_ref[i]) <= 1e-9 * abs(e_ref[i]), (i, e_gpu[i], e_ref[i])
+        # Parseval on the full cell (the sampled sum is exact there)
+        full = api.modal_power(ctx, nx, ny, Wd, Vd, 7.2, 4.3, 2 * nx, 2 * ny, 0.0, 7.2, 0.0, 4.3).cpu().numpy()
+        pars = 0.5 * 7.2 * 4.3 * np.real(np.sum(W[:n] * np.conj(V[n:]) - W[n:] * np.conj(V[:n]), axis=0))
+        assert np.abs(full - pars).max() <= 1e-11 * np.abs(pars).max()
+        # leading dimension larger than 2 nx ny
+        ld = N + 3
+        Wp = torch.zeros((N, ld), dtype=torch.complex128, device=dev)
+        Vp = torch.zeros((N, ld), dtype=torch.complex128, device=dev)
+        Wp[:, :N] = Wd
+        Vp[:, :N] = Vd
+        ep = torch.empty(N, dtype=torch.float64, device=dev)
+        ctx.check(ctx.lib.vrc_modal_power(ctx.h, nx, ny, N, ptr(Wp), ld, ptr(Vp), ld, *[float(v) for v in win[:2]],
+                                          win[2], win[3], *[float(v) for v in win[4:]], ptr(ep)), "modal_power")
+        assert np.abs(ep.cpu().numpy() - e_gpu).max() <= 1e-14 * np.abs(e_gpu).max()
+        energies.append(e_gpu)
+        guided.append(rec)
+    # the driver's normalised transmission table |Tuu(i,j)|^2 e1 / e0 over guided modes
+    Tuu = it.Tuu()
+    tab = np.array([[abs(Tuu[i, j]) ** 2 * energies[1][i] / energies[0][j] for j, _ in guided[0]]
+                    for i, _ in guided[1]])
+    e0 = {j: O.evaluate_energy(it.W0()[:, j], it.V0()[:, j], nx, ny, *win) for j, _ in guided[0]}
+    e1 = {i: O.evaluate_energy(it.W1()[:, i], it.V1()[:, i], nx, ny, *win) for i, _ in guided[1]}
+    ref = np.array([[abs(Tuu[i, j]) ** 2 * e1[i] / e0[j] for j, _ in guided[0]] for i, _ in guided[1]])
+    assert np.abs(tab - ref).max() <= 1e-9 * np.abs(ref).max()
+    assert np.all(np.isfinite(tab))
+
+
 def test_homogeneous_closed_form(ctx):
     """Single-value end media: W = I, PQ W = W Gamma^2 and V = Q W / Gamma against the
     oracle's P and Q (the closed form of a homogeneous medium)."""

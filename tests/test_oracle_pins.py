@@ -353,6 +353,61 @@ def test_plane_wave_unequal_media_normalisation():
         assert abs(REF.sum() + TRN.sum() / np.sqrt(2.1) * np.sqrt(1.0 / 2.1) - 1.0) > 1e-2
 
 
+def test_evaluate_energy_parseval():
+    """Full-cell window with more sample points than harmonic differences: the midpoint sums
+    of the wavelet products are exactly orthogonal, so the sampled power is
+    0.5 Lx Ly Re sum(Ex Hy* - Ey Hx*) over the harmonics (Parseval); a dropped 1/2, dx dy or
+    a swapped component fails it."""
+    rng = np.random.default_rng(3)
+    nx, ny, Lx, Ly = 5, 3, 1.3, 0.7
+    n = nx * ny
+    for nres in ((5, 3), (7, 4), (16, 9)):
+        e = rng.standard_normal(2 * n) + 1j * rng.standard_normal(2 * n)
+        h = rng.standard_normal(2 * n) + 1j * rng.standard_normal(2 * n)
+        got = O.evaluate_energy(e, h, nx, ny, Lx, Ly, nres[0], nres[1], 0.0, Lx, 0.0, Ly)
+        want = 0.5 * Lx * Ly * np.real(np.sum(e[:n] * np.conj(h[n:]) - e[n:] * np.conj(h[:n])))
+        assert abs(got - want) < 1e-12 * abs(want), (nres, got, want)
+        swapped = 0.5 * Lx * Ly * np.real(np.sum(e[:n] * np.conj(h[:n]) - e[n:] * np.conj(h[n:])))
+        assert abs(swapped - want) > 1e-3 * abs(want)
+
+
+def test_evaluate_energy_partial_window_closed_form():
+    """Ex only in the harmonic (m, n) = (1, 0), Hy only in (-1, 0), window partial in x and
+    full in y: the power is 0.5 dx Ly Re(a b* sum_p exp(-i 4 pi x_p / Lx)), a geometric sum.
+    This fixes the harmonic layout (index j + i ny, i the x-harmonic): the transposed
+    layout moves both harmonics onto the y axis, where the full window cancels the term."""
+    nx, ny, Lx, Ly = 5, 3, 1.3, 0.7
+    n = nx * ny
+    a, b = 0.7 - 0.4j, -0.3 + 1.1j
+    x0, x1, P = 0.1 * Lx, 0.45 * Lx, 37
+
+    def case(idx_e, idx_h):
+        e = np.zeros(2 * n, complex)
+        h = np.zeros(2 * n, complex)
+        e[idx_e] = a
+        h[n + idx_h] = b
+        return O.evaluate_energy(e, h, nx, ny, Lx, Ly, P, 11, x0, x1, 0.0, Ly)
+
+    i_p1, i_m1, j0 = (nx - 1) // 2 + 1, (nx - 1) // 2 - 1, (ny - 1) // 2
+    got = case(j0 + i_p1 * ny, j0 + i_m1 * ny)
+    s, th = (x1 - x0) / P, 4 * O.PI / Lx
+    geo = np.exp(-1j * th * (x0 + s / 2)) * (1 - np.exp(-1j * th * s * P)) / (1 - np.exp(-1j * th * s))
+    want = 0.5 * s * Ly * np.real(a * np.conj(b) * geo)
+    assert abs(want) > 1e-2 and abs(got - want) < 1e-13, (got, want)
+    transposed = case(i_p1 + j0 * nx, i_m1 + j0 * nx)
+    assert abs(transposed - want) > 1e-2
+
+
+def test_find_guided_modes_known_answer():
+    """Hand-worked case: gamma / k0^2 = 2 + 0.1i (kept), 3 + 0.3i (imaginary part too large),
+    1.2 (below the range), 2.5 - 0.01i (kept), 3.48 (range end, kept): sorted by real part."""
+    k0 = 2 * O.PI / 1.55
+    g = k0 * k0 * np.array([2 + 0.1j, 3 + 0.3j, 1.2, 2.5 - 0.01j, 3.48])
+    rec = O.find_guided_modes(g, k0, 0.2, 1.445, 3.48)
+    assert [r[0] for r in rec] == [4, 3, 0]
+    assert abs(rec[1][1] - (2.5 - 0.01j)) < 1e-14
+
+
 def test_convergence_rcwa_varrcwa_differential():
     g = G.metasurface(1)
     s = O.VarLayerSampler(g["lam"], g["nx"], g["ny"], g["x"], g["y"], g["eps"], False)

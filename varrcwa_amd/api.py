@@ -343,6 +343,47 @@ def stack_efficiencies(ctx: Context, nx: int, ny: int, instances, solved, e_inc)
     return REF, TRN
 
 
+def modal_power(ctx: Context, nx: int, ny: int, W, V, Lx: float, Ly: float, nres_x: int, nres_y: int,
+                x0: float, x1: float, y0: float, y1: float):
+    """Power of modes through a window, evaluateEnergy (src/tests/test_cpu.cpp:335-412).
+
+    W, V: device complex128 tensors (nmodes, 2 nx ny); row b holds mode b's field harmonics
+    [Ex; Ey] and [Hx; Hy] (a column of the integrator's W/V; the column-major device storage
+    of batch_stack_rcwa's W0[k] is exactly this layout). Returns device float64 (nmodes,)."""
+    torch = ctx.torch
+    n2 = 2 * nx * ny
+    for name, t in (("W", W), ("V", V)):
+        if t.dim() != 2 or t.shape[1] != n2 or t.dtype != torch.complex128 or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous complex128 tensor of shape (nmodes, 2 nx ny)")
+    if W.shape[0] != V.shape[0]:
+        raise ValueError("W and V must hold the same number of modes")
+    energy = torch.empty((W.shape[0],), dtype=torch.float64, device=f"cuda:{ctx.device}")
+    ctx.check(ctx.lib.vrc_modal_power(ctx.h, nx, ny, W.shape[0], ptr(W), n2, ptr(V), n2, float(Lx), float(Ly),
+                                      int(nres_x), int(nres_y), float(x0), float(x1), float(y0), float(y1),
+                                      ptr(energy)), "modal_power")
+    return energy
+
+
+def find_guided_modes(ctx: Optional[Context], gamma, k0: float, imag_coeff: float, re_min: float, re_max: float):
+    """findGuidedModes (src/tests/test_cpu.cpp:311-333): [(index, gamma / k0^2)] of the modes with
+    re_min <= Re <= re_max and Im < imag_coeff, by decreasing real part. Host only: ctx may be
+    None (no device is touched)."""
+    g = np.ascontiguousarray(np.asarray(gamma, dtype=np.complex128))
+    n = g.shape[0]
+    idx = np.zeros(max(n, 1), dtype=np.int32)
+    neff = np.zeros(max(n, 1), dtype=np.complex128)
+    cnt = C.c_int(0)
+    lib = ctx.lib if ctx is not None else load()
+    rc = lib.vrc_find_guided_modes(ctx.h if ctx is not None else None, n, g.ctypes.data, float(k0),
+                                   float(imag_coeff), float(re_min), float(re_max), idx.ctypes.data,
+                                   neff.ctypes.data, C.byref(cnt))
+    if ctx is not None:
+        ctx.check(rc, "find_guided_modes")
+    elif rc != 0:
+        raise VrcError(f"find_guided_modes failed ({rc}): bad arguments")
+    return [(int(idx[k]), complex(neff[k])) for k in range(cnt.value)]
+
+
 # ------------------------------------------------------------------ stage entry points
 def assemble_toeplitz(ctx: Context, nx: int, ny: int, layers):
     torch = ctx.torch

@@ -1,6 +1,7 @@
 // extern "C" entry points (see include/varrcwa.h for the contract of each).
 #include "../../include/varrcwa.h"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 
@@ -208,6 +209,21 @@ void check_harmonics(int nx, int ny) {
   need(nx <= 63 && ny <= 63, "nx, ny above 63 not supported");
 }
 }  // namespace
+
+// Host-only entry points: no device call; ctx may be NULL (then no error text is kept).
+template <class F>
+static int host_guarded(vrc_ctx* c, F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const ArgFailure& e) {
+    if (c) c->last_error = e.what;
+    return -1;
+  } catch (const std::exception& e) {
+    if (c) c->last_error = e.what();
+    return -3;
+  }
+}
 
 extern "C" {
 
@@ -564,6 +580,43 @@ struct vrc_tape {
 int vrc_batch_stack_rcwa(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
                          void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1) {
   return vrc_batch_stack_forward(c, nx, ny, ninst, inst, chunk, S_out, W0, V0, g0, W1, V1, g1, nullptr);
+}
+
+int vrc_modal_power(vrc_ctx* c, int nx, int ny, int nmodes, const void* W, long long ldw, const void* V,
+                    long long ldv, double Lx, double Ly, int nres_x, int nres_y, double x0, double x1, double y0,
+                    double y1, double* energy) {
+  return guarded(c, [&] {
+    check_harmonics(nx, ny);
+    const long n2 = 2L * nx * ny;
+    need(nmodes >= 0 && (nmodes == 0 || (W && V && energy)), "bad arguments");
+    need(ldw >= n2 && ldv >= n2, "leading dimensions must be at least 2 nx ny");
+    need(nres_x > 0 && nres_y > 0, "the sampling grid needs at least one point per axis");
+    need(Lx > 0 && Ly > 0, "cell periods must be positive");
+    modal_power(*c, nx, ny, nmodes, (const cd*)W, (long)ldw, (const cd*)V, (long)ldv, Lx, Ly, nres_x, nres_y, x0,
+                x1, y0, y1, energy);
+  });
+}
+
+int vrc_find_guided_modes(vrc_ctx* c, int n, const void* gamma, double k0, double imag_coeff, double re_min,
+                          double re_max, int* index_out, void* neff_out, int* count) {
+  return host_guarded(c, [&] {
+    need(n >= 0 && (n == 0 || (gamma && index_out && neff_out)) && count, "bad arguments");
+    need(k0 > 0, "k0 must be positive");
+    const cd* g = (const cd*)gamma;
+    std::vector<std::pair<int, cd>> rec;
+    for (int i = 0; i < n; ++i) {
+      const cd gi = mk(g[i].x / k0 / k0, g[i].y / k0 / k0);
+      if (re_min <= gi.x && gi.x <= re_max && gi.y < imag_coeff) rec.emplace_back(i, gi);
+    }
+    std::stable_sort(rec.begin(), rec.end(), [](const std::pair<int, cd>& a, const std::pair<int, cd>& b) {
+      return a.second.x > b.second.x;
+    });
+    for (size_t k = 0; k < rec.size(); ++k) {
+      index_out[k] = rec[k].first;
+      ((cd*)neff_out)[k] = rec[k].second;
+    }
+    *count = (int)rec.size();
+  });
 }
 
 int vrc_stack_efficiencies(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, const void* S,

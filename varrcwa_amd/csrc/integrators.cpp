@@ -748,6 +748,28 @@ void stack_efficiencies(vrc_ctx& c, int nx, int ny, const std::vector<StackInsta
   efficiencies(st, n, B, fr.as<cd>(), ft.as<cd>(), dab.as<cd>(), dk0.as<double>(), deps.as<cd>(), REF, TRN);
 }
 
+// evaluateEnergy (src/tests/test_cpu.cpp:335-412) for many modes: with the window Gram
+// matrices Gx (nx x nx) and Gy (ny x ny) of the sampled wavelets, the grid sum of
+// E conj(H) of one mode is sum(K o conj(Gy^T L Gx)) in the (ny x nx) layout of a mode
+// column, so each mode costs two small products and a dot product instead of a field
+// reconstruction on the whole grid.
+void modal_power(vrc_ctx& c, int nx, int ny, int nmodes, const cd* W, long ldw, const cd* V, long ldv, double Lx,
+                 double Ly, int nres_x, int nres_y, double x0, double x1, double y0, double y1, double* energy) {
+  hipStream_t st = c.stream;
+  const int n = nx * ny;
+  if (nmodes == 0) return;
+  DevBuf gx = c.ws.get(sizeof(cd) * (size_t)nx * nx), gyt = c.ws.get(sizeof(cd) * (size_t)ny * ny);
+  DevBuf T = c.ws.get(sizeof(cd) * (size_t)2 * n * nmodes), Z = c.ws.get(sizeof(cd) * (size_t)2 * n * nmodes);
+  window_gram(st, nx, nres_x, x0, x1, Lx, -1.0, gx.as<cd>());
+  window_gram(st, ny, nres_y, y0, y1, Ly, +1.0, gyt.as<cd>());
+  // T_b = Gy^T [Hx_b | Hy_b] (ny x 2nx per mode), Z = T Gx per (ny x nx) block
+  zgemm_strided(st, OP_N, OP_N, ny, 2 * nx, ny, ONE, gyt.as<cd>(), ny, 0, V, ny, ldv, ZERO, T.as<cd>(), ny, 2 * n,
+                nmodes);
+  zgemm_strided(st, OP_N, OP_N, ny, nx, nx, ONE, T.as<cd>(), ny, n, gx.as<cd>(), nx, 0, ZERO, Z.as<cd>(), ny, n,
+                2 * nmodes);
+  modal_power_reduce(st, n, nmodes, W, ldw, Z.as<cd>(), (x1 - x0) / nres_x * ((y1 - y0) / nres_y), energy);
+}
+
 }  // namespace vrc
 
 // ===================================================================== adjoint of batched stacks

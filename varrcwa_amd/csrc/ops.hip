@@ -658,6 +658,63 @@ void efficiencies(hipStream_t s, int n, int batch, const cd* rxy, const cd* txy,
   launch_check("efficiency");
 }
 
+// ===================================================================== modal power through a window
+// The sampled flux sum of evaluateEnergy (src/tests/test_cpu.cpp:335-412),
+//   sum_{p,q} E(x_p, y_q) conj(H(x_p, y_q)),  E = sum_{i,j} K(i,j) a_i(x) b_j(y),
+// with a_i(x) = exp(-i 2 pi m_i x / Lx) at midpoints x_p, factors exactly through the window
+// Gram matrices Gx(i,i') = sum_p a_i(x_p) conj(a_i'(x_p)) and Gy likewise:
+//   sum = sum_{i,j} K(i,j) (Gx conj(L) Gy^T)(i,j).
+// G[i + i' nh] = sum_p exp(sgn i 2 pi (m_i - m_i') x_p / L), x_p = x0 + step (p + 1/2);
+// sgn = -1 gives the Gram matrix, +1 its transpose.
+constexpr double PI_D = 3.1415926535897932;  // src/rcwa/defns.h:29
+__global__ void window_gram_kernel(int nh, int nres, double x0, double step, double L, double sgn, cd* G) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nh * nh) return;
+  const int i = t % nh, ip = t / nh;
+  const double w = sgn * 2.0 * PI_D * (double)(i - ip) / L;
+  double re = 0.0, im = 0.0;
+  for (int p = 0; p < nres; ++p) {
+    double sn, cs;
+    sincos(w * (x0 + step * (p + 0.5)), &sn, &cs);
+    re += cs;
+    im += sn;
+  }
+  G[t] = mk(re, im);
+}
+
+// energy[b] = 0.5 dx dy Re sum_r (Ex_b[r] conj(Z_b[n + r]) - Ey_b[r] conj(Z_b[r])), where
+// Z_b = [Gy^T Hx_b Gx; Gy^T Hy_b Gx] in the (ny x nx) layout of a mode column.
+__global__ void __launch_bounds__(256) modal_power_kernel(int n, const cd* W, long ldw, const cd* Z, double dxdy,
+                                                          double* energy) {
+  __shared__ double red[256];
+  const long b = blockIdx.x;
+  const cd* e = W + b * ldw;
+  const cd* z = Z + b * 2 * n;
+  double acc = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const cd ex = e[r], ey = e[n + r], zx = z[r], zy = z[n + r];
+    acc += (ex.x * zy.x + ex.y * zy.y) - (ey.x * zx.x + ey.y * zx.y);  // Re(ex conj(zy)) - Re(ey conj(zx))
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) energy[b] = 0.5 * dxdy * red[0];
+}
+
+void window_gram(hipStream_t s, int nh, int nres, double x0, double x1, double L, double sgn, cd* G) {
+  window_gram_kernel<<<cdiv_up((long)nh * nh, 256), 256, 0, s>>>(nh, nres, x0, (x1 - x0) / nres, L, sgn, G);
+  launch_check("window_gram");
+}
+
+void modal_power_reduce(hipStream_t s, int n, int nmodes, const cd* W, long ldw, const cd* Z, double dxdy,
+                        double* energy) {
+  modal_power_kernel<<<nmodes, 256, 0, s>>>(n, W, ldw, Z, dxdy, energy);
+  launch_check("modal_power");
+}
+
 // ===================================================================== (d) projections
 // Homogeneous end media (W0 = W1 = I, V0 and V1 coupling only the harmonic pair (i, i+n)):
 // every matrix of the projection built from the end bases is "pair-sparse", a 2x2 block

@@ -217,6 +217,15 @@ struct StackTape;
 // transmitted field harmonics rxy, txy ([Ex; Ey], 2n per instance).
 void efficiencies(hipStream_t s, int n, int batch, const cd* rxy, const cd* txy, const cd* ab, const double* k0v,
                   const cd* eps2, double* REF, double* TRN);
+// Window Gram matrix of the harmonic wavelets over nres midpoints of [x0, x1] (sgn -1), or
+// its transpose (sgn +1); nh x nh column-major.
+void window_gram(hipStream_t s, int nh, int nres, double x0, double x1, double L, double sgn, cd* G);
+void modal_power_reduce(hipStream_t s, int n, int nmodes, const cd* W, long ldw, const cd* Z, double dxdy,
+                        double* energy);
+// Power of nmodes modes (field harmonics W [Ex; Ey], V [Hx; Hy] per column) through the window
+// [x0, x1] x [y0, y1] sampled at nres_x x nres_y midpoints.
+void modal_power(vrc_ctx& c, int nx, int ny, int nmodes, const cd* W, long ldw, const cd* V, long ldv, double Lx,
+                 double Ly, int nres_x, int nres_y, double x0, double x1, double y0, double y1, double* energy);
 // For instances solved by batch_stack_rcwa (S, end bases W0, W1 in its output layout): the
 // wave e_inc ([Ex; Ey] harmonics, 2n per instance) incident from the bottom medium;
 // REF, TRN: efficiency per harmonic of the reflected / transmitted orders.
