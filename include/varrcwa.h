@@ -122,6 +122,20 @@ int vrc_assemble_toeplitz(vrc_ctx* ctx, int nx, int ny, int nlayers, const vrc_l
 int vrc_assemble_pq(vrc_ctx* ctx, int nx, int ny, int nlayers, const vrc_layer* layers, double lambda,
                     int enable_pml, const vrc_layer* region, void* P, void* Q, long stride);
 
+/* Shape and wavelength derivatives of P and Q for one layer: the derivative outputs of
+ * Layer::waveEqnCoeff (src/rcwa/Layer.cpp:36-162) with the tangents of FourierSolver2D::solve
+ * (src/rcwa/FourierSolver2D.cpp:12-351; FourierSolver1D::dSolve / dSolve_v2,
+ * src/rcwa/FourierSolver1D.cpp:93-321). Parameterization (src/rcwa/Parameterization.h:5-25):
+ * x[var_index[k]] = sum_p var_coeff[k*npara + p] * param_p + const for nvar interior x
+ * coordinates (y coordinates are not varied, as in the reference); x must be non-decreasing.
+ * K (from region, with enable_pml as in vrc_assemble_pq) is held fixed. Device outputs:
+ * dP, dQ: npara matrices N x N (stride N*N); dfe11, dfe22, dfe33: npara matrices n x n
+ * (stride n*n) or NULL; dPl, dQl: d/d lambda, N x N, or NULL. */
+int vrc_wave_eqn_derivatives(vrc_ctx* ctx, int nx, int ny, const vrc_layer* layer, double lambda, int enable_pml,
+                             const vrc_layer* region, int nvar, const int* var_index, const double* var_coeff,
+                             int npara, void* dP, void* dQ, void* dfe11, void* dfe22, void* dfe33, void* dPl,
+                             void* dQl);
+
 /* (c) PQ = P Q; eigen-decomposition; gamma = sqrt(lambda) with the reference branch
  * rule; V = Q W diag(1/gamma). Replaces WaveEquationSolver::solve
  * (src/rcwa/WaveEquationSolver.cpp:24-37) and evaluateEigenvalueAndSetReference

@@ -209,6 +209,166 @@ def fourier2d(coordX: np.ndarray, coordY: np.ndarray, f: np.ndarray,
 
 
 # ---------------------------------------------------------------------------
+# Shape and wavelength derivatives (tangents per parameter)
+# ---------------------------------------------------------------------------
+
+def dsolve_1d(x: np.ndarray, d: float, f: np.ndarray, M: int, layout: int) -> np.ndarray:
+    """FourierSolver1D::dSolve (src/rcwa/FourierSolver1D.cpp:93-143): derivative of the M
+    coefficients of a piecewise constant f with respect to the coordinate x[layout]:
+    (f[layout-1] - f[layout]) / d * exp(-i 2 m pi x[layout] / d), a side absent at the ends."""
+    out = np.zeros(M, dtype=np.complex128)
+    for i in range(M):
+        m = i - (M - 1) // 2
+        e = 1.0 if m == 0 else np.exp(-1j * (2 * m * PI * x[layout] / d))
+        if layout > 0:
+            out[i] += f[layout - 1] / d * e
+        if layout < x.size - 1:
+            out[i] += -f[layout] / d * e
+    return out
+
+
+def dsolve_v2_1d(x: np.ndarray, d: float, f: np.ndarray, M: int, layout: int) -> np.ndarray:
+    """FourierSolver1D::dSolve_v2 (src/rcwa/FourierSolver1D.cpp:195-249): dSolve on the
+    painter's-order (refilled) values, neighbours taken at the coordinate's sorted position;
+    only interior coordinates (0 < layout < len(x) - 1, sorted position likewise)."""
+    _, rindex = _rindex(x)
+    r = int(rindex[layout])
+    if not (0 < layout < x.size - 1) or not (0 < r < x.size - 1):
+        raise ValueError("only interior coordinates can be varied")
+    newf = refill(x, np.asarray(f, dtype=np.complex128), rindex)
+    out = np.zeros(M, dtype=np.complex128)
+    for i in range(M):
+        m = i - (M - 1) // 2
+        e = 1.0 if m == 0 else np.exp(-1j * (2 * m * PI * x[layout] / d))
+        out[i] += newf[r - 1] / d * e
+        out[i] += -newf[r] / d * e
+    return out
+
+
+def _para_jacobian(para: Dict[int, Sequence[float]], npara: int) -> Tuple[List[int], np.ndarray]:
+    """Parameterization (src/rcwa/Parameterization.h:5-25): x_k = sum_p a_kp p_p + x_k0.
+    Coefficients beyond a coordinate's vector are 0 (the header's stated default)."""
+    keys = list(para.keys())
+    J = np.zeros((len(keys), npara))
+    for r, k in enumerate(keys):
+        v = np.asarray(para[k], dtype=np.float64)
+        J[r, :v.size] = v
+    return keys, J
+
+
+def dsolve_para(x, d, f, M, para, npara, v2: bool) -> np.ndarray:
+    """FourierSolver1D::dSolve / dSolve_v2 over a Parameterization
+    (src/rcwa/FourierSolver1D.cpp:252-321): (M x nvar gradient) @ (nvar x npara jacobian)."""
+    keys, J = _para_jacobian(para, npara)
+    g = np.zeros((M, len(keys)), dtype=np.complex128)
+    for c, k in enumerate(keys):
+        g[:, c] = (dsolve_v2_1d if v2 else dsolve_1d)(x, d, f, M, k)
+    return g @ J
+
+
+def fourier2d_tangents(coordX, coordY, f, nx, ny, opt, para_x: Dict[int, Sequence[float]],
+                       npara: int) -> List[np.ndarray]:
+    """The dFfs outputs of FourierSolver2D::solve (src/rcwa/FourierSolver2D.cpp:12-351):
+    per parameter the derivative of the convolution matrix, x coordinates only (the
+    reference rejects y parameterizations, :19-23).
+
+    DiscontinuousX: d inv(T) = -inv(T) dT inv(T) per strip with dT from dSolve_v2 of 1/f
+    (:61-80); DiscontinuousY: the x-cell coefficient of each column differentiated at its
+    two ends with dSolve (:188-212); ContinuousXY: dSolve_v2 of f (:295-321)."""
+    coordX = np.asarray(coordX, dtype=np.float64)
+    coordY = np.asarray(coordY, dtype=np.float64)
+    f = np.asarray(f, dtype=np.complex128)
+    npara = max(npara, max((len(v) for v in para_x.values()), default=0))
+    dx = coordX[-1] - coordX[0]
+    dy = coordY[-1] - coordY[0]
+    nCy, nCx = coordY.size, coordX.size
+    n = nx * ny
+    out = [np.zeros((n, n), dtype=np.complex128) for _ in range(npara)]
+    if npara == 0:
+        return out
+    index, rindex = _rindex(coordX)
+    ii = np.repeat(np.arange(nx), ny)
+    jj = np.tile(np.arange(ny), nx)
+    if opt == DISCONTINUOUS_X:
+        for i in range(nCy - 1):
+            newf = refill(coordX, f[i, :], rindex)
+            tFF = np.linalg.inv(toeplitz_dense(fourier1d_vec(coordX, dx, 1.0 / newf, 2 * nx - 1)))
+            dfi = dsolve_para(coordX, dx, 1.0 / f[i, :], 2 * nx - 1, para_x, npara, True)
+            fY1 = fourier1d_vec(np.array([coordY[i], coordY[i + 1]]), dy, np.array([1.0 + 0j]), 2 * ny - 1)
+            for p in range(npara):
+                dtFF = -tFF @ toeplitz_dense(dfi[:, p]) @ tFF
+                out[p] += fY1[jj[:, None] - jj[None, :] + ny - 1] * dtFF[ii[:, None], ii[None, :]]
+    elif opt == DISCONTINUOUS_Y:
+        newf = np.stack([refill(coordX, f[i, :], rindex) for i in range(f.shape[0])])
+        keys, J = _para_jacobian(para_x, npara)
+        row = {k: J[r] for r, k in enumerate(keys)}
+        for i in range(nCx - 1):
+            tFF = np.linalg.inv(toeplitz_dense(fourier1d_vec(coordY, dy, 1.0 / newf[:, i], 2 * ny - 1)))
+            xs = np.array([coordX[index[i]], coordX[index[i + 1]]])
+            one = np.array([1.0 + 0j])
+            dfX1 = np.zeros((2 * nx - 1, npara), dtype=np.complex128)
+            if int(index[i]) in row:
+                dfX1 += np.outer(dsolve_1d(xs, dx, one, 2 * nx - 1, 0), row[int(index[i])])
+            if int(index[i + 1]) in row:
+                dfX1 += np.outer(dsolve_1d(xs, dx, one, 2 * nx - 1, 1), row[int(index[i + 1])])
+            for p in range(npara):
+                out[p] += dfX1[ii[:, None] - ii[None, :] + nx - 1, p] * tFF[jj[:, None], jj[None, :]]
+    elif opt == CONTINUOUS_XY:
+        four = [np.zeros((2 * nx - 1, 2 * ny - 1), dtype=np.complex128) for _ in range(npara)]
+        for i in range(nCy - 1):
+            dfi = dsolve_para(coordX, dx, f[i, :], 2 * nx - 1, para_x, npara, True)
+            fY1 = fourier1d_vec(np.array([coordY[i], coordY[i + 1]]), dy, np.array([1.0 + 0j]), 2 * ny - 1)
+            for p in range(npara):
+                four[p] += np.outer(dfi[:, p], fY1)
+        out = [block_toeplitz_dense(fp) for fp in four]
+    else:
+        raise ValueError("bad Fourier option")
+    return out
+
+
+def wave_eqn_derivatives(coordX, coordY, eps, Kx, Ky, lam, nx, ny, para_x: Dict[int, Sequence[float]],
+                         npara: int = 0):
+    """The derivative outputs of Layer::waveEqnCoeff (src/rcwa/Layer.cpp:36-162): per
+    parameter dP = [[-Kx T F2h, Kx T F1h], [-Ky T F2h, Ky T F1h]] with T = fe33^-1 dfe33,
+    dQ = [[0, -k0^2 dfe22], [k0^2 dfe11, 0]] (K held fixed); and the wavelength derivatives
+    dPl, dQl with d(k0^2)/d lambda = -8 pi^2 / lambda^3. Returns (dP, dQ, dPl, dQl, dfe)
+    with dfe = (dfe11, dfe22, dfe33), lists over parameters."""
+    npara = max(npara, max((len(v) for v in para_x.values()), default=0))
+    fe11 = fourier2d(coordX, coordY, eps, nx, ny, DISCONTINUOUS_X)
+    fe22 = fourier2d(coordX, coordY, eps, nx, ny, DISCONTINUOUS_Y)
+    fe33 = fourier2d(coordX, coordY, eps, nx, ny, CONTINUOUS_XY)
+    d11 = fourier2d_tangents(coordX, coordY, eps, nx, ny, DISCONTINUOUS_X, para_x, npara)
+    d22 = fourier2d_tangents(coordX, coordY, eps, nx, ny, DISCONTINUOUS_Y, para_x, npara)
+    d33 = fourier2d_tangents(coordX, coordY, eps, nx, ny, CONTINUOUS_XY, para_x, npara)
+    n = nx * ny
+    F1h = np.linalg.solve(fe33, Kx)
+    F2h = np.linalg.solve(fe33, Ky)
+    k02 = (2 * PI / lam) ** 2
+    dk02 = -8.0 * PI * PI / (lam * lam * lam)
+    dP, dQ = [], []
+    for p in range(npara):
+        T = np.linalg.solve(fe33, d33[p])
+        m = np.zeros((2 * n, 2 * n), dtype=np.complex128)
+        m[:n, :n] = Kx @ (-T @ F2h)
+        m[:n, n:] = -Kx @ (-T @ F1h)
+        m[n:, :n] = Ky @ (-T @ F2h)
+        m[n:, n:] = -Ky @ (-T @ F1h)
+        dP.append(m)
+        q = np.zeros((2 * n, 2 * n), dtype=np.complex128)
+        q[:n, n:] = -k02 * d22[p]
+        q[n:, :n] = k02 * d11[p]
+        dQ.append(q)
+    I = np.eye(n)
+    dPl = np.zeros((2 * n, 2 * n), dtype=np.complex128)
+    dPl[:n, n:] = dk02 * I
+    dPl[n:, :n] = -dk02 * I
+    dQl = np.zeros((2 * n, 2 * n), dtype=np.complex128)
+    dQl[:n, n:] = -dk02 * fe22
+    dQl[n:, :n] = dk02 * fe11
+    return dP, dQ, dPl, dQl, (d11, d22, d33)
+
+
+# ---------------------------------------------------------------------------
 # PML stretch and K matrices
 # ---------------------------------------------------------------------------
 

@@ -378,6 +378,48 @@ def test_modal_power_guided_modes(ctx):
     assert np.all(np.isfinite(tab))
 
 
+def _shape_parity(ctx, x, y, eps, nx, ny, lam, pml, para, npara, Kx, Ky):
+    layer = (x, y, eps)
+    out = api.wave_eqn_derivatives(ctx, nx, ny, layer, lam, pml, layer, para, npara)
+    dP, dQ, dPl, dQl, dfe = O.wave_eqn_derivatives(x, y, eps, Kx, Ky, lam, nx, ny, para, npara)
+    for name, ref in (("dP", dP), ("dQ", dQ), ("dfe11", dfe[0]), ("dfe22", dfe[1]), ("dfe33", dfe[2])):
+        got = from_dev_colmajor(out[name])
+        for p in range(npara):
+            assert np.abs(got[p] - ref[p]).max() <= 1e-11 * np.abs(ref[p]).max(), (name, p)
+    for name, ref in (("dPl", dPl), ("dQl", dQl)):
+        got = from_dev_colmajor(out[name])
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), name
+
+
+def test_wave_eqn_derivatives(ctx):
+    """Shape (two-parameter Parameterization of x coordinates) and wavelength derivatives of
+    fe11/fe22/fe33, P and Q against the oracle's restatement of the reference's tangent path
+    (itself pinned to finite differences): periodic metasurface with complex permittivity
+    at 5 x 3 harmonics, and the PML waveguide taper (dense K held fixed) at 5 x 5."""
+    g = G.metasurface(1)
+    x, y = g["x"].copy(), g["y"].copy()
+    eps = g["eps"].copy()
+    eps[0, 2], eps[2, 0] = 2.0 + 0.1j, 5.0
+    a, b = O.alpha_beta(5, 3, x[-1] - x[0], y[-1] - y[0])
+    _shape_parity(ctx, x, y, eps, 5, 3, 1.55, False, {2: [1.0], 1: [0.3, -0.7]}, 2, np.diag(a), np.diag(b))
+    v = G.var12(2, 2)
+    s = O.VarLayerSampler(v["lam"], v["nx"], v["ny"], v["x"], v["y"], v["eps"], True)
+    _shape_parity(ctx, v["x"].copy(), v["y"], v["eps"], v["nx"], v["ny"], v["lam"], True,
+                  {3: [1.0, 0.5], 4: [0.0, 1.0], 2: [0.25]}, 3, s.Kx, s.Ky)
+
+
+def test_wave_eqn_derivatives_rejects(ctx):
+    """Crossing coordinates (the reference's own derivative there is marked unfinished and is
+    not a derivative of its forward map), boundary coordinates and repeated coordinates are
+    refused with an error, before any kernel runs."""
+    g = G.metasurface(1)
+    y, eps = g["y"], g["eps"]
+    xc = np.array([0.0, 0.4, 0.2, 0.65])
+    for x, para in ((xc, {1: [1.0]}), (g["x"], {0: [1.0]}), (g["x"], {3: [1.0]})):
+        with pytest.raises(api.VrcError):
+            api.wave_eqn_derivatives(ctx, 3, 3, (x, y, eps), 1.55, False, (x, y, eps), para)
+
+
 def test_homogeneous_closed_form(ctx):
     """Single-value end media: W = I, PQ W = W Gamma^2 and V = Q W / Gamma against the
     oracle's P and Q (the closed form of a homogeneous medium)."""

@@ -13,8 +13,14 @@ wave_eqn_coeff (homogeneous closed form), wave_equation_solve (eigen residual,
 homogeneous eigenvalues), gamma_sqrt (branch table), rcwa_scatter_matrix
 (homogeneous slab), star_product (identity, associativity), rcwa_integrate
 (z-invariance, energy conservation), redheffer_integrate (single segment for a
-z-invariant structure, convergence to RCWA), differential_integrate (convergence).
-Parity unpinned: svd_solve rank threshold for singular fe33 (never exercised).
+z-invariant structure, convergence to RCWA), differential_integrate (convergence),
+scatter_plane_wave (Airy slab, power balance in equal and unequal media), evaluate_energy
+(Parseval, closed-form partial window), find_guided_modes (known answer), dsolve_1d /
+dsolve_v2_1d / dsolve_para / fourier2d_tangents / wave_eqn_derivatives (central differences
+of the forward maps, monotone coordinates).
+Parity unpinned: svd_solve rank threshold for singular fe33 (never exercised); the
+derivative formulas for crossing coordinates (the reference's own is unfinished there, and
+the library refuses that case).
 """
 import math
 
@@ -406,6 +412,56 @@ def test_find_guided_modes_known_answer():
     rec = O.find_guided_modes(g, k0, 0.2, 1.445, 3.48)
     assert [r[0] for r in rec] == [4, 3, 0]
     assert abs(rec[1][1] - (2.5 - 0.01j)) < 1e-14
+
+
+def _fd_shape_case(x, y, eps, nx, ny, lam, Kx, Ky, para, npara):
+    dP, dQ, dPl, dQl, dfe = O.wave_eqn_derivatives(x, y, eps, Kx, Ky, lam, nx, ny, para, npara)
+    h = 1e-6
+    for p in range(npara):
+        xp, xm = x.copy(), x.copy()
+        for k, v in para.items():
+            c = v[p] if p < len(v) else 0.0
+            xp[k] += h * c
+            xm[k] -= h * c
+        Pp, Qp, fp = O.wave_eqn_coeff(xp, y, eps, Kx, Ky, lam, nx, ny, "lu", return_fe=True)
+        Pm, Qm, fm = O.wave_eqn_coeff(xm, y, eps, Kx, Ky, lam, nx, ny, "lu", return_fe=True)
+        for r in range(3):
+            fd = (fp[r] - fm[r]) / (2 * h)
+            assert np.abs(dfe[r][p] - fd).max() < 1e-7 * np.abs(fd).max(), (p, r)
+        for got, a, b in ((dP[p], Pp, Pm), (dQ[p], Qp, Qm)):
+            fd = (a - b) / (2 * h)
+            assert np.abs(got - fd).max() < 1e-7 * np.abs(fd).max(), p
+    Pp, Qp = O.wave_eqn_coeff(x, y, eps, Kx, Ky, lam + h, nx, ny, "lu")
+    Pm, Qm = O.wave_eqn_coeff(x, y, eps, Kx, Ky, lam - h, nx, ny, "lu")
+    assert np.abs(dPl - (Pp - Pm) / (2 * h)).max() < 1e-7 * np.abs(dPl).max()
+    assert np.abs(dQl - (Qp - Qm) / (2 * h)).max() < 1e-7 * np.abs(dQl).max()
+    return dP, dfe
+
+
+def test_shape_derivatives_match_finite_differences():
+    """The tangents of FourierSolver1D::dSolve/dSolve_v2, FourierSolver2D::solve's dFfs and
+    Layer::waveEqnCoeff's dP/dQ/dPl/dQl are derivatives of the (separately pinned) forward
+    maps: central differences (h = 1e-6) of fe11/fe22/fe33, P and Q in a two-parameter
+    Parameterization (one coordinate in both parameters, coefficients missing = 0) and in
+    lambda agree to 1e-7 relative, periodic and with dense PML K held fixed; a sign slip
+    in d inv(T) = -inv(T) dT inv(T) would fail (planted below)."""
+    g = G.metasurface(1)
+    x, y = g["x"].copy(), g["y"].copy()
+    eps = g["eps"].copy()
+    eps[0, 2], eps[2, 0] = 2.0 + 0.1j, 5.0
+    nx, ny = 5, 3
+    a, b = O.alpha_beta(nx, ny, x[-1] - x[0], y[-1] - y[0])
+    para = {2: [1.0], 1: [0.3, -0.7]}
+    dP, dfe = _fd_shape_case(x, y, eps, nx, ny, 1.55, np.diag(a), np.diag(b), para, 2)
+    v = G.var12(2, 1)
+    s = O.VarLayerSampler(v["lam"], v["nx"], v["ny"], v["x"], v["y"], v["eps"], True)
+    _fd_shape_case(v["x"].copy(), v["y"], v["eps"], v["nx"], v["ny"], v["lam"], s.Kx, s.Ky, {3: [1.0, 0.5], 4: [0.0, 1.0]},
+                   2)
+    # planted mistake: the inverse-rule tangent with the sign of d inv(T) flipped
+    fe11_p, fe11_m = (O.fourier2d(x + dx, y, eps, nx, ny, O.DISCONTINUOUS_X) for dx in
+                      (np.array([0, 0, 1e-6, 0]), np.array([0, 0, -1e-6, 0])))
+    fd = (fe11_p - fe11_m) / 2e-6
+    assert np.abs(-dfe[0][0] - fd).max() > 1e-2 * np.abs(fd).max()
 
 
 def test_convergence_rcwa_varrcwa_differential():

@@ -409,6 +409,38 @@ def assemble_pq(ctx: Context, nx: int, ny: int, layers, lam: float, enable_pml: 
     return P, Q
 
 
+def wave_eqn_derivatives(ctx: Context, nx: int, ny: int, layer, lam: float, enable_pml: bool, region,
+                         para_x: dict, npara: int = 0):
+    """Shape and wavelength derivatives of P and Q (Layer::waveEqnCoeff's dP, dQ, dPl, dQl,
+    src/rcwa/Layer.cpp:36-162). para_x: Parameterization {x-coordinate index: coefficients}
+    (src/rcwa/Parameterization.h:5-25; missing coefficients are 0). Returns device tensors in
+    column-major storage (read with from_dev_colmajor): dP, dQ (npara, N, N), dfe11, dfe22,
+    dfe33 (npara, n, n), dPl, dQl (N, N)."""
+    torch = ctx.torch
+    npara = max(int(npara), max((len(v) for v in para_x.values()), default=0))
+    n, N = nx * ny, 2 * nx * ny
+    keys = sorted(int(k) for k in para_x)
+    idx = np.ascontiguousarray(keys, dtype=np.int32)
+    coef = np.zeros((len(keys), max(npara, 1)), dtype=np.float64)
+    for r, k in enumerate(keys):
+        v = np.asarray(para_x[k], dtype=np.float64)
+        coef[r, :v.size] = v
+    coef = np.ascontiguousarray(coef[:, :npara])
+    dev = f"cuda:{ctx.device}"
+    out = {name: torch.empty((npara, N, N), dtype=torch.complex128, device=dev) for name in ("dP", "dQ")}
+    for name in ("dfe11", "dfe22", "dfe33"):
+        out[name] = torch.empty((npara, n, n), dtype=torch.complex128, device=dev)
+    for name in ("dPl", "dQl"):
+        out[name] = torch.empty((N, N), dtype=torch.complex128, device=dev)
+    L, R = _as_layer(layer), _as_layer(region)
+    ctx.check(ctx.lib.vrc_wave_eqn_derivatives(ctx.h, nx, ny, C.byref(L.struct), float(lam), int(bool(enable_pml)),
+                                               C.byref(R.struct), len(keys), idx.ctypes.data, coef.ctypes.data,
+                                               npara, ptr(out["dP"]), ptr(out["dQ"]), ptr(out["dfe11"]),
+                                               ptr(out["dfe22"]), ptr(out["dfe33"]), ptr(out["dPl"]),
+                                               ptr(out["dQl"])), "wave_eqn_derivatives")
+    return out
+
+
 def eig_pq(ctx: Context, P, Q):
     torch = ctx.torch
     B, N, _ = P.shape

@@ -258,6 +258,20 @@ int vrc_assemble_pq(vrc_ctx* c, int nx, int ny, int nlayers, const vrc_layer* la
   });
 }
 
+int vrc_wave_eqn_derivatives(vrc_ctx* c, int nx, int ny, const vrc_layer* layer, double lambda, int enable_pml,
+                             const vrc_layer* region, int nvar, const int* var_index, const double* var_coeff,
+                             int npara, void* dP, void* dQ, void* dfe11, void* dfe22, void* dfe33, void* dPl,
+                             void* dQl) {
+  return guarded(c, [&] {
+    check_harmonics(nx, ny);
+    need(layer && region && npara >= 0 && nvar >= 0, "bad arguments");
+    need(npara == 0 || (dP && dQ), "dP and dQ are required when there are parameters");
+    VarSampler s(c, lambda, nx, ny, to_host(region), enable_pml != 0);
+    s.shape_derivatives(to_host(layer), nvar, var_index, var_coeff, npara, (cd*)dP, (cd*)dQ, (cd*)dfe11,
+                        (cd*)dfe22, (cd*)dfe33, (cd*)dPl, (cd*)dQl);
+  });
+}
+
 int vrc_eig_pq(vrc_ctx* c, int N, int batch, const void* P, const void* Q, long stride, void* W, void* V, void* gam,
                int* info) {
   return guarded(c, [&] {

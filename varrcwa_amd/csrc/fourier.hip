@@ -224,4 +224,109 @@ void fourier_assemble(hipStream_t s, const FourierDev& g, int nlayers, cd* fe11,
   launch_check("fourier_expand");
 }
 
+// ===================================================================== shape tangents
+// Derivative of a 1D coefficient at the coordinate x moving (FourierSolver1D::dSolve,
+// src/rcwa/FourierSolver1D.cpp:93-143): one side's value / d * exp(-i 2 m pi x / d).
+__device__ inline cd dterm(cd f, double x, int m, double d) {
+  const cd t = mk(f.x / d, f.y / d);
+  if (m == 0) return t;
+  double sn, cs;
+  sincos((2.0 * m * PI_REF) * x / d, &sn, &cs);
+  return cmul(t, mk(cs, -sn));
+}
+
+// Per parameter p (layer 0 of g; cfs[r * npara + p]: coefficient of the coordinate at
+// sorted position r, 0 when it is not varied):
+//  dFX [p][i][m] = sum_r cfs[r][p] (val[i][r-1] - val[i][r]) term at xs[r]  (dSolve_v2 of f,
+//                  src/rcwa/FourierSolver1D.cpp:195-249, for the Laurent rule)
+//  dFXi[p][i][m]   likewise for 1 / val (the inverse rule along x)
+//  dFX1[p][c][m] = cfs[c][p] (-term at xs[c]) + cfs[c+1][p] (term at xs[c+1])  (dSolve of the
+//                  x-cell indicator at its two ends, src/rcwa/FourierSolver2D.cpp:188-212)
+__global__ void shape_coef_kernel(FourierDev g, int npara, const double* cfs, cd* dFX, cd* dFXi, cd* dFX1) {
+  const int ncx = g.ncx[0], ncy = g.ncy[0];
+  const double Lx = g.Lx[0];
+  const int M1 = 2 * g.nx - 1, hx = g.nx - 1;
+  const long MX = g.MX, MY = g.MY;
+  const long nFX = MY * M1, nFX1 = MX * M1, per = 2 * nFX + nFX1;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < per * npara; e += (long)gridDim.x * blockDim.x) {
+    const int p = (int)(e / per);
+    const long ee = e % per;
+    if (ee < 2 * nFX) {
+      const bool inv = ee >= nFX;
+      const long k = inv ? ee - nFX : ee;
+      const int i = (int)(k / M1), m = (int)(k % M1) - hx;
+      cd acc = mk(0.0, 0.0);
+      if (i < ncy) {
+        for (int r = 1; r < ncx; ++r) {
+          const double a = cfs[(long)r * npara + p];
+          if (a == 0.0) continue;
+          cd fl = g.val[(long)i * MX + r - 1], fr = g.val[(long)i * MX + r];
+          if (inv) {
+            fl = crecip(fl);
+            fr = crecip(fr);
+          }
+          const cd d = cadd(dterm(fl, g.xs[r], m, Lx), dterm(mk(-fr.x, -fr.y), g.xs[r], m, Lx));
+          acc = cadd(acc, cscale(d, a));
+        }
+      }
+      (inv ? dFXi : dFX)[(long)p * nFX + k] = acc;
+    } else {
+      const long k = ee - 2 * nFX;
+      const int c = (int)(k / M1), m = (int)(k % M1) - hx;
+      cd acc = mk(0.0, 0.0);
+      if (c < ncx) {
+        const double a0 = cfs[(long)c * npara + p], a1 = cfs[(long)(c + 1) * npara + p];
+        if (a0 != 0.0) acc = cadd(acc, cscale(dterm(mk(-1.0, 0.0), g.xs[c], m, Lx), a0));
+        if (a1 != 0.0) acc = cadd(acc, cscale(dterm(mk(1.0, 0.0), g.xs[c + 1], m, Lx), a1));
+      }
+      dFX1[(long)p * nFX1 + k] = acc;
+    }
+  }
+}
+
+// D[b] = Toeplitz(v[b]) (k x k column-major, D(r,c) = v[k-1 + r - c]).
+__global__ void toeplitz_dense_kernel(int k, const cd* v, long sv, cd* D, long sD) {
+  const long b = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k * k) return;
+  const int r = e % k, c = e / k;
+  D[b * sD + e] = v[b * sv + k - 1 + r - c];
+}
+
+void fourier_tangents(hipStream_t s, const FourierDev& g, int npara, const double* cfs, const ShapeWork& w,
+                      cd* dfe11, cd* dfe22, cd* dfe33, long sfe) {
+  const int nx = g.nx, ny = g.ny, n = nx * ny, M1 = 2 * nx - 1;
+  const long MX = g.MX, MY = g.MY, nxx = (long)nx * nx;
+  shape_coef_kernel<<<64, 256, 0, s>>>(g, npara, cfs, w.dFX, w.dFXi, w.dFX1);
+  launch_check("shape_coef");
+  // inverse rule along x: d inv(T) = -inv(T) dT inv(T) per strip (src/rcwa/FourierSolver2D.cpp:76-79)
+  dim3 tg(cdiv_up(nxx, 256), (unsigned)(npara * MY));
+  toeplitz_dense_kernel<<<tg, 256, 0, s>>>(nx, w.dFXi, M1, w.D, nxx);
+  launch_check("toeplitz_dense");
+  for (int p = 0; p < npara; ++p) {
+    cd* Dp = w.D + (long)p * MY * nxx;
+    cd* Yp = w.Y + (long)p * MY * nxx;
+    cd* Tp = w.dtFF + (long)p * MY * nxx;
+    zgemm_strided(s, OP_N, OP_N, nx, nx, nx, mk(1.0, 0.0), Dp, nx, nxx, g.tFFx, nx, nxx, mk(0.0, 0.0), Yp, nx, nxx,
+                  (int)MY);
+    zgemm_strided(s, OP_N, OP_N, nx, nx, nx, mk(-1.0, 0.0), g.tFFx, nx, nxx, Yp, nx, nxx, mk(0.0, 0.0), Tp, nx, nxx,
+                  (int)MY);
+  }
+  // the forward contraction and expansion over the tangent tables
+  for (int p = 0; p < npara; ++p) {
+    FourierDev gp = g;
+    gp.FX = w.dFX + (long)p * MY * M1;
+    gp.tFFx = w.dtFF + (long)p * MY * nxx;
+    gp.FX1 = w.dFX1 + (long)p * MX * M1;
+    gp.F2 = w.F2;
+    gp.G11 = w.G11;
+    gp.G22 = w.G22;
+    fourier_contract_kernel<<<dim3(64, 1), 256, 0, s>>>(gp);
+    launch_check("fourier_contract");
+    dim3 grid(cdiv_up(n, 256), n, 1);
+    fourier_expand_kernel<<<grid, 256, 0, s>>>(gp, dfe11 + p * sfe, dfe22 + p * sfe, dfe33 + p * sfe, 0);
+    launch_check("fourier_expand");
+  }
+}
+
 }  // namespace vrc

@@ -108,6 +108,110 @@ void Sampler::sample_layers(const std::vector<HostLayer>& ls, cd* P, cd* Q) {
   assemble_pq(*c_, a, P, Q);
 }
 
+// Derivative outputs of Layer::waveEqnCoeff (src/rcwa/Layer.cpp:36-162) with the
+// tangents of FourierSolver2D::solve (src/rcwa/FourierSolver2D.cpp:12-351):
+//   dP = [[Kx Y1, Kx Y2], [Ky Y1, Ky Y2]],  [Y1 | Y2] = fe33^-1 dfe33 fe33^-1 [-Ky | Kx]
+//   dQ = [[0, -k0^2 dfe22], [k0^2 dfe11, 0]],  dPl, dQl with d(k0^2)/d lambda = -8 pi^2 / lambda^3.
+void Sampler::shape_derivatives(const HostLayer& L, int nvar, const int* var_index, const double* var_coeff,
+                                int npara, cd* dP, cd* dQ, cd* dfe11, cd* dfe22, cd* dfe33, cd* dPl, cd* dQl) {
+  hipStream_t st = c_->stream;
+  const int nCx = (int)L.x.size();
+  if (npara < 0 || nvar < 0 || (nvar > 0 && (!var_index || !var_coeff))) throw ArgFailure("bad parameterization");
+  for (int i = 0; i + 1 < nCx; ++i)
+    if (L.x[i] > L.x[i + 1])
+      throw ArgFailure("shape derivatives need non-decreasing x coordinates (crossing coordinates have none)");
+  std::vector<char> seen(nCx, 0);
+  for (int k = 0; k < nvar; ++k) {
+    const int v = var_index[k];
+    if (v <= 0 || v >= nCx - 1) throw ArgFailure("only interior x coordinates can be varied");
+    if (seen[v]) throw ArgFailure("a coordinate appears twice in the parameterization");
+    seen[v] = 1;
+  }
+  const int n = nx_ * ny_, N = 2 * n;
+  const long nn = (long)n * n, NN = (long)N * N;
+  const HostLayer* lp = &L;
+  FourierStage fs = fourier_prepare(*c_, nx_, ny_, {lp});
+  DevBuf fe = c_->ws.get(sizeof(cd) * 3 * nn);
+  cd* fe11 = fe.as<cd>();
+  cd* fe22 = fe11 + nn;
+  cd* fe33 = fe22 + nn;
+  fourier_assemble(st, fs.g, 1, fe11, fe22, fe33, nn);
+  const double k2 = k0() * k0();
+  const double dk02 = -8.0 * PI_REF * PI_REF / (lambda_ * lambda_ * lambda_);
+  if (dPl) {
+    zset(st, N, N, ZERO, ZERO, dPl, N, 0, 1);
+    zset(st, n, n, mk(dk02, 0.0), ZERO, dPl + (long)n * N, N, 0, 1);
+    zset(st, n, n, mk(-dk02, 0.0), ZERO, dPl + n, N, 0, 1);
+  }
+  if (dQl) {
+    zset(st, N, N, ZERO, ZERO, dQl, N, 0, 1);
+    zgeam(st, n, n, mk(-dk02, 0.0), fe22, n, 0, ZERO, nullptr, 0, 0, dQl + (long)n * N, N, 0, 1);
+    zgeam(st, n, n, mk(dk02, 0.0), fe11, n, 0, ZERO, nullptr, 0, 0, dQl + n, N, 0, 1);
+  }
+  if (npara == 0) return;
+  // coefficient of every coordinate (sorted = given order for non-decreasing coordinates)
+  const long MX = fs.g.MX, MY = fs.g.MY;
+  std::vector<double> cfs((size_t)(MX + 1) * npara, 0.0);
+  for (int k = 0; k < nvar; ++k)
+    for (int p = 0; p < npara; ++p) cfs[(size_t)var_index[k] * npara + p] = var_coeff[(size_t)k * npara + p];
+  DevBuf dcfs = upload(*c_, cfs.data(), sizeof(double) * cfs.size());
+  const int M1 = 2 * nx_ - 1, M2 = 2 * ny_ - 1;
+  const long nxx = (long)nx_ * nx_;
+  DevBuf wk = c_->ws.get(sizeof(cd) * ((size_t)npara * (2 * MY * M1 + MX * M1 + 3 * MY * nxx) + M1 * M2 +
+                                       nxx * M2 + (long)ny_ * ny_ * M1));
+  ShapeWork w;
+  w.dFX = wk.as<cd>();
+  w.dFXi = w.dFX + npara * MY * M1;
+  w.dFX1 = w.dFXi + npara * MY * M1;
+  w.D = w.dFX1 + npara * MX * M1;
+  w.Y = w.D + npara * MY * nxx;
+  w.dtFF = w.Y + npara * MY * nxx;
+  w.F2 = w.dtFF + npara * MY * nxx;
+  w.G11 = w.F2 + (long)M1 * M2;
+  w.G22 = w.G11 + nxx * M2;
+  DevBuf dfe;
+  if (!dfe11 || !dfe22 || !dfe33) {
+    dfe = c_->ws.get(sizeof(cd) * 3 * nn * npara);
+    if (!dfe11) dfe11 = dfe.as<cd>();
+    if (!dfe22) dfe22 = dfe.as<cd>() + nn * npara;
+    if (!dfe33) dfe33 = dfe.as<cd>() + 2 * nn * npara;
+  }
+  fourier_tangents(st, fs.g, npara, dcfs.as<double>(), w, dfe11, dfe22, dfe33, nn);
+  // dQ
+  zset(st, N, N, ZERO, ZERO, dQ, N, NN, npara);
+  zgeam(st, n, n, mk(-k2, 0.0), dfe22, n, nn, ZERO, nullptr, 0, 0, dQ + (long)n * N, N, NN, npara);
+  zgeam(st, n, n, mk(k2, 0.0), dfe11, n, nn, ZERO, nullptr, 0, 0, dQ + n, N, NN, npara);
+  // F = fe33^-1 [-Ky | Kx]; X = fe33^-1 dfe33 (all parameters as one right-hand side block)
+  DevBuf lu = c_->ws.get(sizeof(cd) * nn), piv = c_->ws.get(sizeof(int) * n), inf = c_->ws.get(sizeof(int));
+  DevBuf F = c_->ws.get(sizeof(cd) * 2 * nn), X = c_->ws.get(sizeof(cd) * nn * npara),
+         Y = c_->ws.get(sizeof(cd) * 2 * nn * npara);
+  fill_int(st, inf.as<int>(), 0, 1);
+  zgeam(st, n, n, ONE, fe33, n, 0, ZERO, nullptr, 0, 0, lu.as<cd>(), n, 0, 1);
+  zgetrf(st, n, lu.as<cd>(), n, 0, piv.as<int>(), 0, inf.as<int>(), 1, c_->ws);
+  if (pml_) {
+    const cd* K = d_K_.as<cd>();
+    zgeam(st, n, n, mk(-1.0, 0.0), K + nn, n, 0, ZERO, nullptr, 0, 0, F.as<cd>(), n, 0, 1);
+    zgeam(st, n, n, ONE, K, n, 0, ZERO, nullptr, 0, 0, F.as<cd>() + nn, n, 0, 1);
+  } else {
+    zset(st, n, 2 * n, ZERO, ZERO, F.as<cd>(), n, 0, 1);
+    zadd_diag(st, n, mk(-1.0, 0.0), d_alpha_.as<cd>() + n, 0, F.as<cd>(), n, 0, 1);
+    zadd_diag(st, n, ONE, d_alpha_.as<cd>(), 0, F.as<cd>() + nn, n, 0, 1);
+  }
+  zgetrs(st, n, 2 * n, lu.as<cd>(), n, 0, piv.as<int>(), 0, F.as<cd>(), n, 0, F.as<cd>(), n, 0, 1, c_->ws);
+  zgetrs(st, n, n * npara, lu.as<cd>(), n, 0, piv.as<int>(), 0, dfe33, n, 0, X.as<cd>(), n, 0, 1, c_->ws);
+  zgemm_strided(st, OP_N, OP_N, n, 2 * n, n, ONE, X.as<cd>(), n, nn, F.as<cd>(), n, 0, ZERO, Y.as<cd>(), n, 2 * nn,
+                npara);
+  if (pml_) {
+    const cd* K = d_K_.as<cd>();
+    zgemm_strided(st, OP_N, OP_N, n, 2 * n, n, ONE, K, n, 0, Y.as<cd>(), n, 2 * nn, ZERO, dP, N, NN, npara);
+    zgemm_strided(st, OP_N, OP_N, n, 2 * n, n, ONE, K + nn, n, 0, Y.as<cd>(), n, 2 * nn, ZERO, dP + n, N, NN,
+                  npara);
+  } else {
+    zdscale(st, n, 2 * n, d_alpha_.as<cd>(), 0, nullptr, 0, 0, Y.as<cd>(), n, 2 * nn, dP, N, NN, npara);
+    zdscale(st, n, 2 * n, d_alpha_.as<cd>() + n, 0, nullptr, 0, 0, Y.as<cd>(), n, 2 * nn, dP + n, N, NN, npara);
+  }
+}
+
 VarSampler::VarSampler(vrc_ctx* c, double lambda, int nx, int ny, HostLayer base, bool pml)
     : Sampler(c, lambda, nx, ny, pml), base_(std::move(base)) {
   prepare_layer(base_);  // validates shapes

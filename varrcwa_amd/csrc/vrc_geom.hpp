@@ -37,5 +37,16 @@ struct FourierDev {
 void fourier_tables(hipStream_t s, const FourierDev& g, int nlayers);
 // fourier_tables + contraction + expansion into fe11, fe22, fe33 (n x n, stride sfe).
 void fourier_assemble(hipStream_t s, const FourierDev& g, int nlayers, cd* fe11, cd* fe22, cd* fe33, long sfe);
+// Workspace of the shape tangents of one layer (npara parameters; MX cells, MY strips).
+struct ShapeWork {
+  cd *dFX, *dFXi, *dFX1;  // [p][MY][2nx-1], [p][MY][2nx-1], [p][MX][2nx-1]
+  cd *D, *Y, *dtFF;       // [p][MY][nx][nx]
+  cd *F2, *G11, *G22;     // one layer's contraction tables
+};
+// Derivatives of fe11, fe22, fe33 (n x n each, stride sfe per parameter) of layer 0 of g
+// (after fourier_tables) with respect to npara parameters of its x coordinates; cfs:
+// device [(MX+1) x npara], the coefficient of each sorted coordinate position.
+void fourier_tangents(hipStream_t s, const FourierDev& g, int npara, const double* cfs, const ShapeWork& w,
+                      cd* dfe11, cd* dfe22, cd* dfe33, long sfe);
 
 }  // namespace vrc

@@ -143,6 +143,12 @@ class Sampler {
   void sample_layers(const std::vector<HostLayer>& ls, cd* P, cd* Q);
   const std::vector<cd>& Kx() const { return hKx_; }
   const std::vector<cd>& Ky() const { return hKy_; }
+  // Tangents of P and Q of layer L (this sampler's wavelength and K, held fixed) with
+  // respect to npara parameters of its x coordinates: x[var_index[k]] moves with
+  // var_coeff[k * npara + p]. Device outputs, npara matrices each: dP, dQ (N x N);
+  // dfe11/dfe22/dfe33 (n x n, may be null); dPl, dQl: d/d lambda (N x N, may be null).
+  void shape_derivatives(const HostLayer& L, int nvar, const int* var_index, const double* var_coeff, int npara,
+                         cd* dP, cd* dQ, cd* dfe11, cd* dfe22, cd* dfe33, cd* dPl, cd* dQl);
 
  protected:
   void post_init(const std::vector<double>& x, const std::vector<double>& y);
