@@ -185,6 +185,23 @@ int vrc_sampler_add_rule(vrc_sampler* s, int axis, int layout, double (*rule)(do
 int vrc_stack_sampler_create(vrc_ctx* ctx, double lambda, int nx, int ny, int nlayers, const vrc_layer* layers,
                              const double* z_edges, const vrc_layer* bottom, const vrc_layer* top, int enable_pml,
                              vrc_sampler** out);
+/* GDSIISampler (src/core/GDSIISampler.cpp:17-146): x-z cross-sections of the polygons
+ * (BOUNDARY/BOX elements) of the first cell of a GDSII file, read as gdstk::read_gds does
+ * with coordinates in micrometers (src/gdstk/library.cpp:160-420); PML always on. shift:
+ * x, y, z (NULL = 0); pml_space: lPML, rPML, lspace, rspace, bPML, uPML, bspace, uspace
+ * (NULL = the reference defaults 0.5, 0.5, 1.5, 1.5, 0.5, 0.5, 1.08, 1.0); eps_background,
+ * eps_device: complex permittivities (re, im) at this wavelength (Material::permittivity). */
+int vrc_gds_sampler_create(vrc_ctx* ctx, double lambda, int nx, int ny, const char* filename, const double* shift,
+                           const double* pml_space, const double* eps_background, const double* eps_device,
+                           double thickness, vrc_sampler** out);
+/* GDSIISampler::minX/maxX/minZ/maxZ (src/core/GDSIISampler.h:41-44) into out4. */
+int vrc_gds_sampler_bounds(const vrc_sampler* s, double* out4);
+/* The layer a sampler sees at z (GDSIISampler::sampleLayout, src/core/GDSIISampler.cpp:94-146;
+ * VarLayerSampler's moved coordinates for the others). Host only. Writes the counts always;
+ * x (ncoord_x), y (ncoord_y) and eps ((ncoord_y-1) x (ncoord_x-1) complex, column-major)
+ * only when they fit max_coord_x / max_coord_y. */
+int vrc_sampler_layout(vrc_sampler* s, double z, int max_coord_x, double* x, int* ncoord_x, int max_coord_y,
+                       double* y, int* ncoord_y, double* eps);
 int vrc_sampler_destroy(vrc_sampler* s);
 /* LayerSampler::nDim (src/core/LayerSampler.h:44) */
 int vrc_sampler_ndim(const vrc_sampler* s);

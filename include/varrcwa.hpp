@@ -204,6 +204,66 @@ class StackLayerSampler : public LayerSampler {
   }
 };
 
+// Material (src/core/Material.h:5-18): a constant permittivity or a function of wavelength.
+class Material {
+ public:
+  Material(scalex permittivity) : f_([permittivity](scalar) { return permittivity; }) {}
+  Material(double permittivity) : Material(scalex(permittivity, 0.0)) {}
+  Material(std::function<scalex(scalar)> permittivity) : f_(std::move(permittivity)) {}
+  scalex permittivity(scalar lambda) const { return f_(lambda); }
+
+ private:
+  std::function<scalex(scalar)> f_;
+};
+inline const Material m_SiO2{1.445 * 1.445};
+inline const Material m_Si{3.48 * 3.48};
+
+// GDSIISampler (src/core/GDSIISampler.h:9-57): x-z cross-sections of the first cell of a
+// GDSII file (micrometers), PML always on; shift = {x, y, z}.
+class GDSIISampler : public LayerSampler {
+ public:
+  GDSIISampler(scalar lambda, int nx, int ny, const std::string& filename, const std::vector<scalar>& shift = {0, 0, 0},
+               scalar lPML = 0.5, scalar rPML = 0.5, scalar lspace = 1.5, scalar rspace = 1.5, scalar bPML = 0.5,
+               scalar uPML = 0.5, scalar bspace = 1.08, scalar uspace = 1., const Material& permBackground = m_SiO2,
+               const Material& permDevice = m_Si, scalar thickness = 0.22, std::shared_ptr<Context> ctx = nullptr)
+      : LayerSampler(lambda, std::move(ctx)) {
+    if (shift.size() != 3) throw std::invalid_argument("GDSIISampler: shift needs x, y, z");
+    const double ps[8] = {lPML, rPML, lspace, rspace, bPML, uPML, bspace, uspace};
+    const scalex eb = permBackground.permittivity(lambda), ed = permDevice.permittivity(lambda);
+    const double b[2] = {eb.real(), eb.imag()}, d[2] = {ed.real(), ed.imag()};
+    vrc_sampler* s = nullptr;
+    detail::check(vrc_gds_sampler_create(ctx_->get(), lambda, nx, ny, filename.c_str(), shift.data(), ps, b, d,
+                                         thickness, &s),
+                  ctx_->get(), "GDSIISampler");
+    finish(s, nx * ny);
+    detail::check(vrc_gds_sampler_bounds(s_, bounds_), ctx_->get(), "GDSIISampler bounds");
+    VectorXs x;
+    MatrixXcs eps;
+    sampleLayout(minZ(), x, eps);  // y does not depend on z
+  }
+  scalar minX() const { return bounds_[0]; }
+  scalar maxX() const { return bounds_[1]; }
+  scalar minZ() const { return bounds_[2]; }
+  scalar maxZ() const { return bounds_[3]; }
+  // GDSIISampler::sampleLayout (src/core/GDSIISampler.cpp:94-146); eps is (ny-1) x (nx-1).
+  void sampleLayout(scalar z, VectorXs& x, MatrixXcs& eps) {
+    int ncx = 0, ncy = 0;
+    detail::check(vrc_sampler_layout(s_, z, 0, nullptr, &ncx, 0, nullptr, &ncy, nullptr), ctx_->get(),
+                  "sampleLayout");
+    x.resize(ncx);
+    y_.resize(ncy);
+    eps.resize(ncy - 1, ncx - 1);
+    detail::check(vrc_sampler_layout(s_, z, ncx, x.data(), &ncx, ncy, y_.data(), &ncy,
+                                     reinterpret_cast<double*>(eps.data())),
+                  ctx_->get(), "sampleLayout");
+  }
+  const VectorXs& y() const { return y_; }
+
+ private:
+  double bounds_[4] = {0, 0, 0, 0};
+  VectorXs y_;
+};
+
 namespace detail {
 // Common state of the three integrators: handle, sampler reference, host copies of
 // the results (filled after each compute).

@@ -557,6 +557,109 @@ class VarLayerSampler(LayerSampler):
         return x, y, self.eps
 
 
+def gdsii_real(r: int) -> float:
+    """GDSII 8-byte real (gdsii_real_to_double, src/gdstk/utils.cpp:46-51): sign bit,
+    excess-64 base-16 exponent, 56-bit mantissa."""
+    e = ((r >> 56) & 0x7F) - 64
+    m = (r & 0x00FFFFFFFFFFFFFF) / 72057594037927936.0
+    v = math.ldexp(m, 4 * e)
+    return -v if r >> 63 else v
+
+
+def read_gds_polygons(path: str, unit: float = 1e-6) -> List[List[List[Tuple[float, float]]]]:
+    """gdstk::read_gds (src/gdstk/library.cpp:160-420) restricted to what GDSIISampler reads:
+    per cell (in file order) its BOUNDARY/BOX polygons as point lists, coordinates
+    factor * int32 with factor = db_in_meters / unit (UNITS record), the closing point of
+    each polygon dropped at ENDEL (:386-391); PATH, SREF, AREF and TEXT elements are not
+    polygons."""
+    import struct
+    data = open(path, "rb").read()
+    cells: List[List[List[Tuple[float, float]]]] = []
+    factor = 1.0
+    cur = None
+    elem = None
+    o = 0
+    while o + 4 <= len(data):
+        ln, rec = struct.unpack(">HB", data[o:o + 3])
+        if ln < 4:
+            break
+        body = data[o + 4:o + ln]
+        if rec == 0x03:
+            db_in_m = gdsii_real(struct.unpack(">Q", body[8:16])[0])
+            factor = db_in_m / unit
+        elif rec == 0x04:
+            break
+        elif rec == 0x05:
+            cells.append([])
+        elif rec in (0x08, 0x2D):
+            elem = []
+            cells[-1].append(elem)
+        elif rec in (0x09, 0x0A, 0x0B, 0x0C, 0x15):
+            elem = None
+        elif rec == 0x10 and elem is not None:
+            v = struct.unpack(">%di" % (len(body) // 4), body)
+            elem.extend((factor * v[k], factor * v[k + 1]) for k in range(0, len(v) - 1, 2))
+        elif rec == 0x11:
+            if elem:
+                elem.pop()
+            elem = None
+        o += ln
+    return cells
+
+
+class GDSIISampler(LayerSampler):
+    """GDSIISampler (src/core/GDSIISampler.cpp:17-146): x-z cross-sections of the first cell's
+    polygons (GDSII y is the propagation axis z), PML always on."""
+
+    def __init__(self, lam, nx, ny, filename, shift=(0.0, 0.0, 0.0), lPML=0.5, rPML=0.5, lspace=1.5,
+                 rspace=1.5, bPML=0.5, uPML=0.5, bspace=1.08, uspace=1.0, permBackground=1.445 * 1.445,
+                 permDevice=3.48 * 3.48, thickness=0.22):
+        super().__init__(lam, nx, ny, True)
+        self.polys = read_gds_polygons(filename, 1e-6)[0]
+        self.shift = [float(v) for v in shift]
+        self.eps_bg, self.eps_dev = complex(permBackground), complex(permDevice)
+        # assignSimulationRegion (src/core/GDSIISampler.cpp:49-92)
+        pts = [p for poly in self.polys for p in poly]
+        self.minX = min(p[0] for p in pts) + self.shift[0]
+        self.maxX = max(p[0] for p in pts) + self.shift[0]
+        self.minZ = min(p[1] for p in pts) + self.shift[2]
+        self.maxZ = max(p[1] for p in pts) + self.shift[2]
+        b0 = self.shift[1]
+        b1 = b0 + bPML
+        u0 = b1 + bspace + thickness + uspace
+        u1 = u0 + uPML
+        l1 = self.minX - lspace
+        l0 = l1 - lPML
+        r0 = self.maxX + rspace
+        r1 = r0 + rPML
+        self.xr = (l0, l1, r0, r1)
+        self.y = np.array([b0, b1, b1 + bspace, b1 + bspace + thickness, u0, u1])
+        self._post_init(np.array(self.xr), np.array([b0, b1, u0, u1]))
+
+    def layer_at(self, z):
+        """sampleLayout (src/core/GDSIISampler.cpp:94-146)."""
+        xs = []
+        for poly in self.polys:
+            m = len(poly)
+            for i in range(m):
+                (x0, y0), (x1, y1) = poly[i], poly[(i + 1) % m]
+                if (y0 <= z < y1) or (y1 <= z < y0):
+                    h = y1 - y0
+                    if abs(h) < 1e-12:
+                        h += 2e-12
+                    a = (z - y0) / h
+                    xs.append((1 - a) * x0 + a * x1)
+        xs.sort()
+        Nx = len(xs) + 4
+        x = np.empty(Nx)
+        x[0], x[1], x[-2], x[-1] = self.xr
+        x[2:Nx - 2] = np.array(xs) + self.shift[0]
+        eps = np.full((self.y.size - 1, Nx - 1), self.eps_bg, dtype=np.complex128)
+        for j in range(2, Nx - 3, 2):
+            eps[2, j] = self.eps_dev
+        return x, self.y.copy(), eps
+
+
 class StackLayerSampler(LayerSampler):
     """Piecewise-constant-in-z stack of Layer(x, y, eps) slabs (extension).
 

@@ -29,7 +29,8 @@ static MatrixXcs grid(int rows, int cols, double bg) {
 
 int main(int argc, char** argv) {
   if (argc != 6) {
-    std::fprintf(stderr, "usage: %s <metasurface|var12> <rcwa|redheffer|differential> <nh> <N|pe> <out>\n", argv[0]);
+    std::fprintf(stderr, "usage: %s <metasurface|var12|gds:FILE> <rcwa|redheffer|differential> <nh> <N|pe> <out>\n",
+                 argv[0]);
     return 2;
   }
   const std::string scen = argv[1], method = argv[2];
@@ -37,29 +38,37 @@ int main(int argc, char** argv) {
   const double param = std::atof(argv[4]);
   try {
     double z0, z1;
-    std::shared_ptr<VarLayerSampler> sampler;
+    std::shared_ptr<LayerSampler> sampler;
     if (scen == "metasurface") {
       z0 = 0.0;
       z1 = 2.0;
       const double co = 3.48 * 3.48;
       MatrixXcs eps = grid(3, 3, 1.0);
       eps(1, 1) = scalex(co, 0.0);
-      sampler = std::make_shared<VarLayerSampler>(1.55, 2 * nh + 1, 2 * nh + 1, VectorXs{0.0, 0.1, 0.5, 0.65},
+      auto vs = std::make_shared<VarLayerSampler>(1.55, 2 * nh + 1, 2 * nh + 1, VectorXs{0.0, 0.1, 0.5, 0.65},
                                                   VectorXs{0.0, 0.1, 0.5, 0.65}, eps, false);
-      sampler->addXRule(2, [](scalar z) { return z <= 1.0 ? 0.5 : 0.2 * std::cos(0.5 * kPi * (z - 1.0)) + 0.3; });
+      vs->addXRule(2, [](scalar z) { return z <= 1.0 ? 0.5 : 0.2 * std::cos(0.5 * kPi * (z - 1.0)) + 0.3; });
+      sampler = vs;
     } else if (scen == "var12") {
       z0 = -0.1;
       z1 = 10.1;
       MatrixXcs eps = grid(6, 7, 1.445 * 1.445);
       eps(3, 2) = scalex(3.48 * 3.48, 0.0);
-      sampler = std::make_shared<VarLayerSampler>(1.55, 2 * nh + 1, 2 * nh + 1,
+      auto vs = std::make_shared<VarLayerSampler>(1.55, 2 * nh + 1, 2 * nh + 1,
                                                   VectorXs{0.0, 0.5, 1.5, 2.6, 3.7, 3.9, 5.0, 5.5},
                                                   VectorXs{0.0, 0.5, 1.5, 1.58, 1.8, 2.8, 3.3}, eps);
-      sampler->addXRule(3, [](scalar z) {
+      vs->addXRule(3, [](scalar z) {
         if (z < 0.0) return 2.6;
         if (z > 10.0) return 3.7;
         return (1.0 - 0.1 * z) * 2.6 + 0.1 * z * 3.7;
       });
+      sampler = vs;
+    } else if (scen.rfind("gds:", 0) == 0) {
+      // src/tests/test_gds_cpu.cpp:129-145: the contour layout, ny = 2 nh - 1
+      z0 = -1.63;
+      z1 = 1.36;
+      sampler = std::make_shared<GDSIISampler>(1.55, 2 * nh + 1, 2 * nh - 1, scen.substr(4),
+                                               std::vector<scalar>{2.3647, 0.0, 0.0}, 0.5, 0.5, 1.0, 1.3);
     } else {
       std::fprintf(stderr, "unknown scenario %s\n", scen.c_str());
       return 2;

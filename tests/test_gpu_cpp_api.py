@@ -21,7 +21,8 @@ TOL = 1e-9
 def run_cpp(tmp_path, scen, method, nh, param):
     if not EXE.exists():
         raise FileNotFoundError(f"{EXE} is not built; run `python -m varrcwa_amd.build`")
-    out = tmp_path / f"{scen}_{method}.bin"
+    tag = "".join(ch if ch.isalnum() else "_" for ch in scen)[-40:]
+    out = tmp_path / f"{tag}_{method}.bin"
     print(f"cpp_api_check {scen} {method} nh={nh} param={param}", flush=True)
     r = subprocess.run([str(EXE), scen, method, str(nh), repr(param), str(out)], capture_output=True, text=True,
                        timeout=100)
@@ -76,6 +77,16 @@ def test_cpp_differential_metasurface(tmp_path):
     g = G.metasurface(1)
     ref = O.differential_integrate(G.oracle_sampler(g), g["z0"], g["z1"], 32)
     close(run_cpp(tmp_path, "metasurface", "differential", 1, 32), ref)
+
+
+@pytest.mark.gpu
+def test_cpp_rcwa_gds_contour(tmp_path):
+    """GDSIISampler through the C++ classes on the reference's layout file
+    (src/tests/test_gds_cpu.cpp:129-145 at 5 x 3 harmonics, 10 slices)."""
+    f = ROOT / "tests" / "golden" / "contour_extract.gds"
+    o = O.GDSIISampler(1.55, 5, 3, str(f), shift=(2.3647, 0.0, 0.0), lPML=0.5, rPML=0.5, lspace=1.0, rspace=1.3)
+    ref = O.rcwa_integrate(o, -1.63, 1.36, 10)
+    close(run_cpp(tmp_path, f"gds:{f}", "rcwa", 2, 10), ref)
 
 
 def test_cpp_header_compiles():

@@ -420,6 +420,38 @@ def test_wave_eqn_derivatives_rejects(ctx):
             api.wave_eqn_derivatives(ctx, 3, 3, (x, y, eps), 1.55, False, (x, y, eps), para)
 
 
+def test_gds_sampler(ctx, tmp_path):
+    """GDSIISampler from a written file and from the reference's layout file: bounds and the
+    cross-section layouts equal the oracle's exactly, P/Q at a few z and the RCWA S-matrix of
+    the contour device (src/tests/test_gds_cpu.cpp:129-145 at 5 x 3 harmonics, 12 slices)
+    against the oracle at the usual 1e-9 bar."""
+    import os
+    from tests.gds_files import synthetic_cells, write_gds
+    f = tmp_path / "t.gds"
+    write_gds(f, synthetic_cells())
+    contour = os.path.join(os.path.dirname(__file__), "golden", "contour_extract.gds")
+    kw = dict(shift=(2.3647, 0.0, 0.0), lPML=0.5, rPML=0.5, lspace=1.0, rspace=1.3)
+    fac = 1e-9 / 1e-6
+    for path, args, zs in ((str(f), dict(shift=(0.25, 0.1, 0.0)), (fac * -500, 0.1, 0.6999, fac * 700)),
+                           (contour, kw, (-1.63, -1.0, 0.0, 0.7, 1.35))):
+        g = api.GDSIISampler(1.55, 5, 3, path, ctx=ctx, **args)
+        o = O.GDSIISampler(1.55, 5, 3, path, **args)
+        assert (g.minX(), g.maxX(), g.minZ(), g.maxZ()) == (o.minX, o.maxX, o.minZ, o.maxZ)
+        for z in zs:
+            x, y, e = g.sampleLayout(z)
+            xo, yo, eo = o.layer_at(z)
+            assert np.array_equal(x, xo) and np.array_equal(y, yo) and np.array_equal(e, eo), z
+            P, Q = g.sample(z)
+            Po, Qo = o.sample(z)
+            assert np.abs(P - Po).max() <= 1e-11 * np.abs(Po).max()
+            assert np.abs(Q - Qo).max() <= 1e-11 * np.abs(Qo).max()
+    g = api.GDSIISampler(1.55, 5, 3, contour, ctx=ctx, **kw)
+    o = O.GDSIISampler(1.55, 5, 3, contour, **kw)
+    it = api.RCWAIntegrator(-1.63, 1.36, g)
+    it.compute(12)
+    invariants_close(gpu_result(it), O.rcwa_integrate(o, -1.63, 1.36, 12))
+
+
 def test_homogeneous_closed_form(ctx):
     """Single-value end media: W = I, PQ W = W Gamma^2 and V = Q W / Gamma against the
     oracle's P and Q (the closed form of a homogeneous medium)."""

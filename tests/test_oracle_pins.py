@@ -14,6 +14,8 @@ homogeneous eigenvalues), gamma_sqrt (branch table), rcwa_scatter_matrix
 (homogeneous slab), star_product (identity, associativity), rcwa_integrate
 (z-invariance, energy conservation), redheffer_integrate (single segment for a
 z-invariant structure, convergence to RCWA), differential_integrate (convergence),
+gdsii_real (known answers), read_gds_polygons / GDSIISampler (written files with known
+coordinates, the reference's layout file),
 scatter_plane_wave (Airy slab, power balance in equal and unequal media), evaluate_energy
 (Parseval, closed-form partial window), find_guided_modes (known answer), dsolve_1d /
 dsolve_v2_1d / dsolve_para / fourier2d_tangents / wave_eqn_derivatives (central differences
@@ -462,6 +464,64 @@ def test_shape_derivatives_match_finite_differences():
                       (np.array([0, 0, 1e-6, 0]), np.array([0, 0, -1e-6, 0])))
     fd = (fe11_p - fe11_m) / 2e-6
     assert np.abs(-dfe[0][0] - fd).max() > 1e-2 * np.abs(fd).max()
+
+
+def test_gdsii_real_known_answers():
+    """The GDSII 8-byte real: 1.0 = 0x4110000000000000 and -2.5 = 0xC128000000000000 exactly
+    (mantissa 1/16 and 5/32 at exponent 65), and the database units most layout tools write,
+    0x3E4189374BC6A7EF = 1e-3 and 0x3944B82FA09B5A53 = 1e-9."""
+    assert O.gdsii_real(0x4110000000000000) == 1.0
+    assert O.gdsii_real(0xC128000000000000) == -2.5
+    assert abs(O.gdsii_real(0x3E4189374BC6A7EF) - 1e-3) < 1e-18
+    assert abs(O.gdsii_real(0x3944B82FA09B5A53) - 1e-9) < 1e-24
+
+
+def test_gds_reader_and_cross_sections(tmp_path):
+    """A written file with known integer coordinates (1 nm database unit, read in um):
+    exact polygon points with the closing point dropped, boxes read as polygons, paths and
+    the second cell left out; cross-sections: the rectangle's sides, the trapezoid's sloped
+    edge by linear interpolation, the half-open rule y0 <= z < y1 at the top edge, and the
+    device material in every other x cell of the device strip."""
+    from tests.gds_files import synthetic_cells, write_gds
+    f = tmp_path / "t.gds"
+    write_gds(f, synthetic_cells())
+    cells = O.read_gds_polygons(str(f))
+    assert len(cells) == 2 and len(cells[0]) == 3
+    fac = 1e-9 / 1e-6  # db_in_meters / unit, the reference's scale factor
+    assert cells[0][0] == [(fac * 1000, fac * -500), (fac * 1600, fac * -500), (fac * 1600, fac * 700),
+                           (fac * 1000, fac * 700)]
+    assert cells[0][2] == [(fac * 3000, fac * -200), (fac * 3300, fac * -200), (fac * 3300, fac * 200),
+                           (fac * 3000, fac * 200)]
+    s = O.GDSIISampler(1.55, 3, 3, str(f), shift=(0.25, 0.1, 0.0))
+    assert np.allclose([s.minX, s.maxX, s.minZ, s.maxZ], [1.25, 3.55, -0.5, 0.7], atol=1e-14, rtol=0)
+    x, y, eps = s.layer_at(0.1)
+    # sloped edge (2.6, -0.4) -> (2.3, 0.6) at z = 0.1: a = 0.5, x = 2.45
+    want = [1.0, 1.6, 2.0, 2.45, 3.0, 3.3]
+    assert np.allclose(x[2:-2], np.array(want) + 0.25, atol=1e-12, rtol=0)
+    assert np.allclose(x[:2], [1.25 - 1.5 - 0.5, 1.25 - 1.5]) and np.allclose(x[-2:], [3.55 + 1.5, 3.55 + 2.0])
+    assert np.allclose(y, [0.1, 0.6, 1.68, 1.9, 2.9, 3.4])
+    dev = np.zeros(eps.shape, bool)
+    dev[2, [2, 4, 6]] = True
+    assert np.all(eps[dev] == 3.48 ** 2) and np.all(eps[~dev] == 1.445 ** 2)
+    x_top, _, _ = s.layer_at(fac * 700)  # the rectangle's top edge is excluded: nothing crosses
+    assert x_top.size == 4
+    x_bot, _, _ = s.layer_at(fac * -500)  # its bottom edge is included (rectangle only)
+    assert np.allclose(x_bot[2:-2], np.array([1.0, 1.6]) + 0.25)
+
+
+def test_gds_fixture_cross_sections():
+    """The reference's layout file (tests/golden/contour_extract.gds, the input of
+    src/tests/test_gds_cpu.cpp): 8 closed polygons, so every generic cross-section crosses an
+    even number of edges, inside the polygons' x range."""
+    import os
+    f = os.path.join(os.path.dirname(__file__), "golden", "contour_extract.gds")
+    s = O.GDSIISampler(1.55, 3, 3, f, shift=(2.3647, 0.0, 0.0), lPML=0.5, rPML=0.5, lspace=1.0, rspace=1.3)
+    assert len(s.polys) == 8
+    for z in np.linspace(-1.6, 1.35, 23):
+        x, _, _ = s.layer_at(z)
+        inner = x[2:-2]
+        assert inner.size % 2 == 0 and inner.size >= 2
+        assert np.all(np.diff(inner) >= 0) and s.minX - 1e-12 <= inner[0] and inner[-1] <= s.maxX + 1e-12
 
 
 def test_convergence_rcwa_varrcwa_differential():

@@ -92,6 +92,9 @@ def load():
         "vrc_stack_efficiencies": (i, vp, i, i, i, C.POINTER(VrcStackInstance), vp, vp, vp, vp, vp, vp),
         "vrc_wave_eqn_derivatives": (i, vp, i, i, C.POINTER(VrcLayer), d, i, C.POINTER(VrcLayer), i, vp, vp, i, vp,
                                      vp, vp, vp, vp, vp, vp),
+        "vrc_gds_sampler_create": (i, vp, d, i, i, C.c_char_p, vp, vp, vp, vp, d, C.POINTER(vp)),
+        "vrc_gds_sampler_bounds": (i, vp, vp),
+        "vrc_sampler_layout": (i, vp, d, i, vp, vp, i, vp, vp, vp),
         "vrc_modal_power": (i, vp, i, i, i, vp, C.c_longlong, vp, C.c_longlong, d, d, i, i, d, d, d, d, vp),
         "vrc_find_guided_modes": (i, vp, i, vp, d, d, d, d, vp, vp, vp),
         "vrc_batch_stack_forward": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp,

@@ -151,6 +151,54 @@ class StackLayerSampler(_Sampler):
             int(bool(enablePML)), C.byref(self.h)), "StackLayerSampler")
 
 
+class GDSIISampler(_Sampler):
+    """GDSIISampler (src/core/GDSIISampler.cpp:17-146): x-z cross-sections of the first cell's
+    polygons in a GDSII file (micrometers), PML always on. Defaults follow
+    src/core/GDSIISampler.h:27-35 (permittivities: SiO2 1.445^2 background, Si 3.48^2 device)."""
+
+    def __init__(self, lam, nx, ny, filename, shift=(0.0, 0.0, 0.0), lPML=0.5, rPML=0.5, lspace=1.5, rspace=1.5,
+                 bPML=0.5, uPML=0.5, bspace=1.08, uspace=1.0, permBackground=1.445 * 1.445,
+                 permDevice=3.48 * 3.48, thickness=0.22, ctx: Optional[Context] = None):
+        super().__init__(ctx or default_context())
+        sh = np.ascontiguousarray(shift, dtype=np.float64)
+        ps = np.ascontiguousarray([lPML, rPML, lspace, rspace, bPML, uPML, bspace, uspace], dtype=np.float64)
+        eb = np.array([complex(permBackground).real, complex(permBackground).imag])
+        ed = np.array([complex(permDevice).real, complex(permDevice).imag])
+        self.ctx.check(self.ctx.lib.vrc_gds_sampler_create(
+            self.ctx.h, float(lam), int(nx), int(ny), str(filename).encode(), sh.ctypes.data, ps.ctypes.data,
+            eb.ctypes.data, ed.ctypes.data, float(thickness), C.byref(self.h)), "GDSIISampler")
+
+    def _bounds(self):
+        b = np.zeros(4)
+        self.ctx.check(self.ctx.lib.vrc_gds_sampler_bounds(self.h, b.ctypes.data), "bounds")
+        return b
+
+    def minX(self):
+        return float(self._bounds()[0])
+
+    def maxX(self):
+        return float(self._bounds()[1])
+
+    def minZ(self):
+        return float(self._bounds()[2])
+
+    def maxZ(self):
+        return float(self._bounds()[3])
+
+    def sampleLayout(self, z: float):
+        """(x, y, eps) of the cross-section at z; eps[i, j] for y strip i and x cell j."""
+        nxc, nyc = C.c_int(0), C.c_int(0)
+        lib = self.ctx.lib
+        self.ctx.check(lib.vrc_sampler_layout(self.h, float(z), 0, None, C.byref(nxc), 0, None, C.byref(nyc), None),
+                       "sampleLayout")
+        x = np.zeros(nxc.value)
+        y = np.zeros(nyc.value)
+        e = np.zeros((nyc.value - 1) * (nxc.value - 1), dtype=np.complex128)
+        self.ctx.check(lib.vrc_sampler_layout(self.h, float(z), nxc.value, x.ctypes.data, C.byref(nxc), nyc.value,
+                                              y.ctypes.data, C.byref(nyc), e.ctypes.data), "sampleLayout")
+        return x, y, e.reshape((nyc.value - 1, nxc.value - 1), order="F")
+
+
 _GETTERS = ["Tdd", "Tuu", "Rud", "Rdu", "W0", "V0", "gam0", "W1", "V1", "gam1"]
 
 

@@ -414,6 +414,56 @@ int vrc_stack_sampler_create(vrc_ctx* c, double lambda, int nx, int ny, int nlay
   });
 }
 
+int vrc_gds_sampler_create(vrc_ctx* c, double lambda, int nx, int ny, const char* filename, const double* shift,
+                           const double* pml_space, const double* eps_background, const double* eps_device,
+                           double thickness, vrc_sampler** out) {
+  return guarded(c, [&] {
+    need(out != nullptr && filename != nullptr && eps_background && eps_device, "bad arguments");
+    *out = nullptr;
+    check_harmonics(nx, ny);
+    const double zero3[3] = {0.0, 0.0, 0.0};
+    // the reference's defaults (src/core/GDSIISampler.h:27-35)
+    const double dflt[8] = {0.5, 0.5, 1.5, 1.5, 0.5, 0.5, 1.08, 1.0};
+    auto s = std::make_unique<vrc_sampler>();
+    s->ctx = c;
+    s->s = std::make_unique<GdsSampler>(c, lambda, nx, ny, std::string(filename), shift ? shift : zero3,
+                                        pml_space ? pml_space : dflt, mk(eps_background[0], eps_background[1]),
+                                        mk(eps_device[0], eps_device[1]), thickness);
+    ctx_retain(c);
+    *out = s.release();
+  });
+}
+
+int vrc_gds_sampler_bounds(const vrc_sampler* s, double* out4) {
+  if (!s || !out4) return -1;
+  const GdsSampler* g = dynamic_cast<const GdsSampler*>(s->s.get());
+  if (!g) {
+    s->ctx->last_error = "not a GDSII sampler";
+    return -1;
+  }
+  out4[0] = g->minX();
+  out4[1] = g->maxX();
+  out4[2] = g->minZ();
+  out4[3] = g->maxZ();
+  return 0;
+}
+
+int vrc_sampler_layout(vrc_sampler* s, double z, int max_coord_x, double* x, int* ncoord_x, int max_coord_y,
+                       double* y, int* ncoord_y, double* eps) {
+  if (!s) return -1;
+  return host_guarded(s->ctx, [&] {
+    need(ncoord_x && ncoord_y, "bad arguments");
+    HostLayer L = s->s->layer_at(z);
+    *ncoord_x = (int)L.x.size();
+    *ncoord_y = (int)L.y.size();
+    if ((int)L.x.size() > max_coord_x || (int)L.y.size() > max_coord_y || !x || !y || !eps)
+      return;  // sizes only
+    std::copy(L.x.begin(), L.x.end(), x);
+    std::copy(L.y.begin(), L.y.end(), y);
+    std::copy(L.eps.begin(), L.eps.end(), (cd*)eps);
+  });
+}
+
 int vrc_sampler_destroy(vrc_sampler* s) {
   if (!s) return 0;
   vrc_ctx* c = s->ctx;

@@ -5,6 +5,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "vrc_context.hpp"
@@ -181,6 +182,45 @@ class StackSampler : public Sampler {
   std::vector<HostLayer> layers_;
   std::vector<double> ze_;
   HostLayer bottom_, top_;
+};
+
+// GDSII polygons (gdstk::read_gds subset: BOUNDARY and BOX elements of every cell,
+// src/gdstk/library.cpp:160-420); coordinates in units of `unit` meters.
+struct GdsPolygon {
+  std::vector<double> x, y;
+  int layer = 0, datatype = 0;
+};
+struct GdsCell {
+  std::string name;
+  std::vector<GdsPolygon> polygons;
+};
+struct GdsLibrary {
+  std::string name;
+  double unit = 0, precision = 0;
+  std::vector<GdsCell> cells;
+};
+double gdsii_real(uint64_t r);
+GdsLibrary read_gds(const std::string& path, double unit);
+
+// GDSIISampler (src/core/GDSIISampler.cpp:17-146): x-z cross-sections of the first cell's
+// polygons, always with PML. pml_space: lPML, rPML, lspace, rspace, bPML, uPML, bspace, uspace.
+class GdsSampler : public Sampler {
+ public:
+  GdsSampler(vrc_ctx* c, double lambda, int nx, int ny, const std::string& file, const double shift[3],
+             const double pml_space[8], cd eps_background, cd eps_device, double thickness);
+  HostLayer layer_at(double z) const override;
+  double minX() const { return minX_; }
+  double maxX() const { return maxX_; }
+  double minZ() const { return minZ_; }
+  double maxZ() const { return maxZ_; }
+
+ private:
+  std::vector<GdsPolygon> polys_;
+  double shift_[3];
+  cd eps_bg_, eps_dev_;
+  double thickness_;
+  double minX_, maxX_, minZ_, maxZ_;
+  std::vector<double> xr_, y_;
 };
 
 // ---------------------------------------------------------------- integrators
