@@ -203,6 +203,27 @@ int vrc_gds_sampler_bounds(const vrc_sampler* s, double* out4);
 int vrc_sampler_layout(vrc_sampler* s, double z, int max_coord_x, double* x, int* ncoord_x, int max_coord_y,
                        double* y, int* ncoord_y, double* eps);
 int vrc_sampler_destroy(vrc_sampler* s);
+
+/* ----------------------------------------------------------- legacy stacked-layer solver */
+/* RCWASolver (src/rcwa/RCWASolver.h:60-80): layer types added once, then a stack of type
+ * indices with thicknesses (the first and last entries are semi-infinite; their thickness is
+ * not used). nX, nY: harmonic orders (nx = 2 nX + 1). PML always on, region from the first
+ * layer. */
+typedef struct vrc_rcwa_solver vrc_rcwa_solver;
+int vrc_rcwa_solver_create(vrc_ctx* ctx, int nX, int nY, vrc_rcwa_solver** out);
+/* RCWASolver::addLayer (src/rcwa/RCWASolver.cpp:73-102): same cell and PML region as the
+ * first layer, or an error. */
+int vrc_rcwa_solver_add_layer(vrc_rcwa_solver* s, const vrc_layer* layer);
+/* RCWASolver::solve (src/rcwa/RCWASolver.cpp:104-113): Layer::solve of every type,
+ * interfaces (:460-580; translation: NULL or one x shift per stack entry,
+ * Layer::permuteEigVecX) and the RTCM recursion (:343-457) for Tdd and Rud. */
+int vrc_rcwa_solver_solve(vrc_rcwa_solver* s, double lambda, int nstack, const int* stack, const double* thickness,
+                          const double* translation);
+/* Host copies: which 0 Tdd, 1 Rud (N x N, column-major); for layer type layer_type: 2 eigvecE,
+ * 3 eigvecH (N x N), 4 gamma (N, Layer::gamma = sqrt(eig) / k0). */
+int vrc_rcwa_solver_get(vrc_rcwa_solver* s, int which, int layer_type, double* host);
+int vrc_rcwa_solver_dims(const vrc_rcwa_solver* s, int* N, int* ntypes);
+int vrc_rcwa_solver_destroy(vrc_rcwa_solver* s);
 /* LayerSampler::nDim (src/core/LayerSampler.h:44) */
 int vrc_sampler_ndim(const vrc_sampler* s);
 /* LayerSampler::sample (src/core/LayerSampler.h:49-51): P, Q into host N x N buffers. */

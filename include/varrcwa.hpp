@@ -264,6 +264,61 @@ class GDSIISampler : public LayerSampler {
   VectorXs y_;
 };
 
+// The legacy stacked-layer solver RCWASolver (src/rcwa/RCWASolver.h:60-80): nX, nY are
+// harmonic orders; layers are LayerDesc (the reference's Layer(x, y, eps)).
+class RCWASolver {
+ public:
+  RCWASolver(int nX, int nY, std::shared_ptr<Context> ctx = nullptr) : ctx_(ctx ? std::move(ctx) : Context::thread_default()) {
+    detail::check(vrc_rcwa_solver_create(ctx_->get(), nX, nY, &h_), ctx_->get(), "RCWASolver");
+  }
+  ~RCWASolver() {
+    if (h_) vrc_rcwa_solver_destroy(h_);
+  }
+  RCWASolver(const RCWASolver&) = delete;
+  RCWASolver& operator=(const RCWASolver&) = delete;
+
+  void addLayer(const LayerDesc& layer) {
+    vrc_layer l = layer.view();
+    detail::check(vrc_rcwa_solver_add_layer(h_, &l), ctx_->get(), "addLayer");
+    ++ntypes_;
+  }
+  int nLayerTypes() const { return ntypes_; }
+  void solve(scalar lambda, const std::vector<int>& layerStack, const std::vector<scalar>& thickness,
+             const std::vector<scalar>* translation = nullptr) {
+    if (translation && translation->size() != layerStack.size())
+      throw Error("translation needs one value per stack entry", -1);
+    detail::check(vrc_rcwa_solver_solve(h_, lambda, (int)layerStack.size(), layerStack.data(), thickness.data(),
+                                        translation ? translation->data() : nullptr),
+                  ctx_->get(), "solve");
+    int N = 0, t = 0;
+    vrc_rcwa_solver_dims(h_, &N, &t);
+    Tdd_.resize(N, N);
+    Rud_.resize(N, N);
+    detail::check(vrc_rcwa_solver_get(h_, 0, 0, reinterpret_cast<double*>(Tdd_.data())), ctx_->get(), "Tdd");
+    detail::check(vrc_rcwa_solver_get(h_, 1, 0, reinterpret_cast<double*>(Rud_.data())), ctx_->get(), "Rud");
+  }
+  const MatrixXcs& Tdd() const { return Tdd_; }
+  const MatrixXcs& Rud() const { return Rud_; }
+  scalex tdd(int row, int col) const { return Tdd_(row, col); }
+  scalex rud(int row, int col) const { return Rud_(row, col); }
+  // src/rcwa/RCWASolver.cpp:582-670
+  scalar transmittance(int inputMode, int outputMode) const { return std::norm(Tdd_(outputMode, inputMode)); }
+  scalar reflectance(int inputMode, int outputMode) const { return std::norm(Rud_(outputMode, inputMode)); }
+  scalar transmittance(int inputMode) const { return colsum(Tdd_, inputMode); }
+  scalar reflectance(int inputMode) const { return colsum(Rud_, inputMode); }
+
+ private:
+  static scalar colsum(const MatrixXcs& M, int c) {
+    scalar s = 0;
+    for (long r = 0; r < M.rows(); ++r) s += std::norm(M(r, c));
+    return s;
+  }
+  std::shared_ptr<Context> ctx_;
+  vrc_rcwa_solver* h_ = nullptr;
+  int ntypes_ = 0;
+  MatrixXcs Tdd_, Rud_;
+};
+
 namespace detail {
 // Common state of the three integrators: handle, sampler reference, host copies of
 // the results (filled after each compute).

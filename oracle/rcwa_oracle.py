@@ -782,6 +782,81 @@ def left_end_projection(S, W1, V1, W0, V0):
     return Tdd_n, Tuu_n, Rud_n, Rdu_n
 
 
+class RCWASolver:
+    """The legacy stacked-layer solver RCWASolver (src/rcwa/RCWASolver.h:60-80; solve,
+    src/rcwa/RCWASolver.cpp:104-113): layer types solved once (Layer::solve,
+    src/rcwa/Layer.cpp:173-190: P, Q with PML K from the first layer's region, eigenvectors
+    E = W, H = V, gamma = sqrt(eig) / k0), interface matrices E = W1^-1 W2, H = V1^-1 V2 per
+    pair of neighbouring types (:460-580; with a translation per stack entry the eigenvector
+    rows of x-harmonic m are scaled by exp(-i 2 pi m t / Lx) first, Layer::permuteEigVecX,
+    src/rcwa/Layer.cpp:193-238), and the RTCM recursion from the last layer to the first
+    (:343-457). Complete orthogonal decomposition solves are LU solves here (the same for the
+    non-singular eigenvector matrices)."""
+
+    def __init__(self, nX: int, nY: int):
+        self.nx, self.ny = 2 * nX + 1, 2 * nY + 1
+        self.layers = []
+
+    def addLayer(self, x, y, eps):
+        self.layers.append((np.asarray(x, np.float64), np.asarray(y, np.float64),
+                            np.asarray(eps, np.complex128)))
+
+    def solve(self, lam: float, stack: Sequence[int], thickness: Sequence[float],
+              translation: Optional[Sequence[float]] = None):
+        nx, ny = self.nx, self.ny
+        n, N = nx * ny, 2 * nx * ny
+        x0, y0, _ = self.layers[0]
+        region = dict(Lx=x0[-1] - x0[0], Ly=y0[-1] - y0[0], b0=y0[0], b1=y0[1], u0=y0[-2], u1=y0[-1],
+                      l0=x0[0], l1=x0[1], r0=x0[-2], r1=x0[-1])
+        Kx, Ky = k_matrices(nx, ny, region, True)
+        k0 = 2 * PI / lam
+        self.E, self.H, self.gamma = [], [], []
+        for (x, y, eps) in self.layers:
+            P, Q = wave_eqn_coeff(x, y, eps, Kx, Ky, lam, nx, ny, "lu")
+            W, V, gam, _ = wave_equation_solve(P, Q)
+            self.E.append(W)
+            self.H.append(V)
+            self.gamma.append(gam / k0)
+        nL = len(stack)
+
+        def rows(t):
+            m = np.repeat(np.arange(nx) - (nx - 1) // 2, ny)
+            d = np.exp(-1j * (2.0 * PI * m / region["Lx"] * t))
+            return np.concatenate([d, d])
+
+        ifE, ifH = [], []
+        for i in range(nL - 1):
+            a, b = stack[i], stack[i + 1]
+            if translation is None and a == b:
+                ifE.append(np.eye(N, dtype=complex))
+                ifH.append(np.eye(N, dtype=complex))
+                continue
+            Ea, Ha, Eb, Hb = self.E[a], self.H[a], self.E[b], self.H[b]
+            if translation is not None:
+                da, db = rows(translation[i])[:, None], rows(translation[i + 1])[:, None]
+                Ea, Ha, Eb, Hb = da * Ea, da * Ha, db * Eb, db * Hb
+            ifE.append(np.linalg.solve(Ea, Eb))
+            ifH.append(np.linalg.solve(Ha, Hb))
+        Tdd = np.eye(N, dtype=complex)
+        Rud = np.zeros((N, N), dtype=complex)
+        for i in range(nL - 1):
+            index = nL - i - 1
+            if i == 0:
+                phi = np.ones(N, dtype=complex)
+            else:
+                phi = np.exp(1j * self.gamma[stack[index]] * thickness[index])
+            waveRud = phi[:, None] * Rud * phi[None, :]
+            waveTdd = Tdd * phi[None, :]
+            E, H = ifE[index - 1], ifH[index - 1]
+            F = E + E @ waveRud
+            G = H - H @ waveRud
+            tau = np.linalg.inv(F + G)
+            Rud = -2.0 * G @ tau + np.eye(N)
+            Tdd = 2.0 * waveTdd @ tau
+        self.Tdd, self.Rud = Tdd, Rud
+        return Tdd, Rud
+
+
 @dataclass
 class SResult:
     """Outputs of an integrator (the getters of src/core/RCWAIntegrator.h:28-38)."""

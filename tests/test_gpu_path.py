@@ -452,6 +452,51 @@ def test_gds_sampler(ctx, tmp_path):
     invariants_close(gpu_result(it), O.rcwa_integrate(o, -1.63, 1.36, 12))
 
 
+def _legacy_layer(eps_core, shift=0.0, pml=0.1):
+    x = np.array([0.0, pml, 0.3 + shift, 0.55 + shift, 1.0 - pml, 1.0])
+    y = np.array([0.0, pml, 0.45, 0.55, 1.0 - pml, 1.0])
+    e = np.full((5, 5), 2.1, dtype=complex)
+    e[2, 2] = eps_core
+    return (x, y, e)
+
+
+def test_legacy_rcwa_solver(ctx):
+    """The legacy stacked-layer solver against the oracle's RTCM restatement: a stack with
+    repeated types (shared interface matrices) and a lossy layer, and the translation
+    option; compared through W_last Tdd W_first^-1 and W_first Rud W_first^-1 with each
+    side's own eigenvectors (1e-9), the layer spectra gamma (sorted), the transmittance
+    helpers on the device result, and the addLayer region check."""
+    nX, nY = 2, 1
+    types = [_legacy_layer(2.1), _legacy_layer(12.0), _legacy_layer(4.0 + 0.2j), _legacy_layer(12.0, 0.05)]
+    g = api.RCWASolver(nX, nY, ctx=ctx)
+    o = O.RCWASolver(nX, nY)
+    for L in types:
+        g.addLayer(*L)
+        o.addLayer(*L)
+    cases = (([0, 1, 2, 1, 2, 0], [0.0, 0.37, 0.21, 0.37, 0.21, 0.0], None),
+             ([0, 1, 2, 3, 0], [0.0, 0.3, 0.2, 0.25, 0.0], [0.0, 0.11, -0.07, 0.05, 0.0]))
+    for stack, th, tr in cases:
+        g.solve(1.3, stack, th, tr)
+        To, Ro = o.solve(1.3, stack, th, tr)
+        first, last = stack[0], stack[-1]
+        Wg0, Wg1 = g.eigvecE(first), g.eigvecE(last)
+        Xg = Wg1 @ g.Tdd() @ np.linalg.inv(Wg0)
+        Yg = Wg0 @ g.Rud() @ np.linalg.inv(Wg0)
+        Xo = o.E[last] @ To @ np.linalg.inv(o.E[first])
+        Yo = o.E[first] @ Ro @ np.linalg.inv(o.E[first])
+        assert np.abs(Xg - Xo).max() <= 1e-9 * np.abs(Xo).max(), stack
+        assert np.abs(Yg - Yo).max() <= 1e-9 * np.abs(Yo).max(), stack
+        for k in range(len(types)):
+            assert spectrum_rel(g.gamma(k), o.gamma[k]) <= 1e-10
+        T = g.Tdd()
+        assert abs(g.transmittance(3) - np.sum(np.abs(T[:, 3]) ** 2)) < 1e-12
+        assert abs(g.transmittance(3, 5) - abs(T[5, 3]) ** 2) < 1e-15
+        assert g.tdd(5, 3) == T[5, 3]
+    x, y, e = _legacy_layer(2.1)
+    with pytest.raises(api.VrcError):
+        g.addLayer(x * 1.1, y, e)
+
+
 def test_homogeneous_closed_form(ctx):
     """Single-value end media: W = I, PQ W = W Gamma^2 and V = Q W / Gamma against the
     oracle's P and Q (the closed form of a homogeneous medium)."""

@@ -17,7 +17,9 @@ z-invariant structure, convergence to RCWA), differential_integrate (convergence
 gdsii_real (known answers), read_gds_polygons / GDSIISampler (written files with known
 coordinates, the reference's layout file),
 scatter_plane_wave (Airy slab, power balance in equal and unequal media), evaluate_energy
-(Parseval, closed-form partial window), find_guided_modes (known answer), dsolve_1d /
+(Parseval, closed-form partial window), find_guided_modes (known answer), RCWASolver (the
+legacy RTCM recursion against the star-product cascade, homogeneous closed form,
+translation = shifted geometry), dsolve_1d /
 dsolve_v2_1d / dsolve_para / fourier2d_tangents / wave_eqn_derivatives (central differences
 of the forward maps, monotone coordinates).
 Parity unpinned: svd_solve rank threshold for singular fe33 (never exercised); the
@@ -522,6 +524,67 @@ def test_gds_fixture_cross_sections():
         inner = x[2:-2]
         assert inner.size % 2 == 0 and inner.size >= 2
         assert np.all(np.diff(inner) >= 0) and s.minX - 1e-12 <= inner[0] and inner[-1] <= s.maxX + 1e-12
+
+
+def _legacy_layer(eps_core, shift=0.0, pml=0.1):
+    x = np.array([0.0, pml, 0.3 + shift, 0.55 + shift, 1.0 - pml, 1.0])
+    y = np.array([0.0, pml, 0.45, 0.55, 1.0 - pml, 1.0])
+    e = np.full((5, 5), 2.1, dtype=complex)
+    e[2, 2] = eps_core
+    return (x, y, e)
+
+
+def test_legacy_rtcm_matches_star_product_cascade():
+    """The RTCM recursion of the legacy solver and the star-product cascade of the integrator
+    (pinned separately) are different algorithms for the same stack: with the first stack
+    entry as the bottom medium, W_last Tdd W_first^-1 equals W1 Tuu W0^-1 and
+    W_first Rud W_first^-1 equals W0 Rdu W0^-1 (PML K of the first layer, two slabs)."""
+    A, B, C = _legacy_layer(2.1), _legacy_layer(12.0), _legacy_layer(4.0 + 0.2j)
+    s = O.RCWASolver(1, 1)
+    for L in (A, B, C):
+        s.addLayer(*L)
+    Tdd, Rud = s.solve(1.3, [0, 1, 2, 0], [0.0, 0.37, 0.21, 0.0])
+    so = O.StackLayerSampler(1.3, 3, 3, [B, C], [0.0, 0.37, 0.58], A, A, enable_pml=True)
+    r = O.rcwa_integrate(so, 0.0, 0.58, 2, z_edges=[0.0, 0.37, 0.58])
+    WA = s.E[0]
+    X = WA @ Tdd @ np.linalg.inv(WA)
+    Y = WA @ Rud @ np.linalg.inv(WA)
+    inv = O.invariant_transforms(r)
+    tk = [k for k in inv if "Tuu" in k][0]
+    rk = [k for k in inv if "Rdu" in k][0]
+    assert np.abs(X - inv[tk]).max() < 1e-11 * np.abs(inv[tk]).max()
+    assert np.abs(Y - inv[rk]).max() < 1e-11 * np.abs(inv[rk]).max()
+    # planted mistake: Rud without the 2 (tau taken as (F + G)^-1 / 2) breaks the match
+    Tdd2, Rud2 = s.solve(1.3, [0, 1, 2, 0], [0.0, 0.37, 0.21, 0.0])
+    bad = WA @ (Rud2 - np.eye(Rud2.shape[0])) @ np.linalg.inv(WA) * 0.5 + np.eye(Rud2.shape[0])
+    assert np.abs(bad - inv[rk]).max() > 1e-3
+
+
+def test_legacy_homogeneous_and_translation():
+    """One homogeneous type repeated: Rud = 0 and Tdd = exp(i gamma t) over the inner
+    thicknesses (the interfaces are identities). With zero-width PML (K periodic), shifting
+    a layer's geometry by t equals the translation option with t for that stack entry
+    (Layer::permuteEigVecX's phase exp(-i 2 pi m t / Lx)); the opposite sign does not."""
+    s = O.RCWASolver(1, 1)
+    s.addLayer(*_legacy_layer(2.1, pml=0.0))
+    T, R = s.solve(1.3, [0, 0, 0, 0], [0.0, 0.2, 0.3, 0.0])
+    assert np.abs(R).max() == 0.0
+    assert np.abs(T - np.diag(np.exp(1j * s.gamma[0] * 0.5))).max() < 1e-14
+    t = 0.17
+    s = O.RCWASolver(1, 1)
+    for L in (_legacy_layer(2.1, pml=0.0), _legacy_layer(12.0, pml=0.0), _legacy_layer(4.0, pml=0.0),
+              _legacy_layer(12.0, t, pml=0.0)):
+        s.addLayer(*L)
+
+    def inv_t(T, R):
+        WA, WC = s.E[0], s.E[2]
+        return WC @ T @ np.linalg.inv(WA), WA @ R @ np.linalg.inv(WA)
+
+    X1, Y1 = inv_t(*s.solve(1.3, [0, 3, 2], [0, 0.37, 0]))
+    X2, Y2 = inv_t(*s.solve(1.3, [0, 1, 2], [0, 0.37, 0], translation=[0.0, t, 0.0]))
+    X3, _ = inv_t(*s.solve(1.3, [0, 1, 2], [0, 0.37, 0], translation=[0.0, -t, 0.0]))
+    assert np.abs(X1 - X2).max() < 1e-12 * np.abs(X1).max() and np.abs(Y1 - Y2).max() < 1e-12 * np.abs(Y1).max()
+    assert np.abs(X1 - X3).max() > 1e-2 * np.abs(X1).max()
 
 
 def test_convergence_rcwa_varrcwa_differential():

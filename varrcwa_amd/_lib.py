@@ -95,6 +95,12 @@ def load():
         "vrc_gds_sampler_create": (i, vp, d, i, i, C.c_char_p, vp, vp, vp, vp, d, C.POINTER(vp)),
         "vrc_gds_sampler_bounds": (i, vp, vp),
         "vrc_sampler_layout": (i, vp, d, i, vp, vp, i, vp, vp, vp),
+        "vrc_rcwa_solver_create": (i, vp, i, i, C.POINTER(vp)),
+        "vrc_rcwa_solver_add_layer": (i, vp, C.POINTER(VrcLayer)),
+        "vrc_rcwa_solver_solve": (i, vp, d, i, vp, vp, vp),
+        "vrc_rcwa_solver_get": (i, vp, i, i, vp),
+        "vrc_rcwa_solver_dims": (i, vp, c_int_p, c_int_p),
+        "vrc_rcwa_solver_destroy": (i, vp),
         "vrc_modal_power": (i, vp, i, i, i, vp, C.c_longlong, vp, C.c_longlong, d, d, i, i, d, d, d, d, vp),
         "vrc_find_guided_modes": (i, vp, i, vp, d, d, d, d, vp, vp, vp),
         "vrc_batch_stack_forward": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp,

@@ -199,6 +199,84 @@ class GDSIISampler(_Sampler):
         return x, y, e.reshape((nyc.value - 1, nxc.value - 1), order="F")
 
 
+class RCWASolver:
+    """The legacy stacked-layer solver RCWASolver (src/rcwa/RCWASolver.h:60-80): addLayer,
+    solve(lambda, layerStack, thickness, translation=None), then Tdd/Rud and the
+    transmittance/reflectance helpers (src/rcwa/RCWASolver.cpp:582-670). nX, nY are
+    harmonic orders (nx = 2 nX + 1); PML always on."""
+
+    def __init__(self, nX: int, nY: int, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.h = C.c_void_p()
+        self._layers = []
+        self.ctx.check(self.ctx.lib.vrc_rcwa_solver_create(self.ctx.h, int(nX), int(nY), C.byref(self.h)),
+                       "RCWASolver")
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.ctx.lib.vrc_rcwa_solver_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def addLayer(self, x, y=None, eps=None):
+        L = _as_layer(x if y is None else (x, y, eps))
+        self._layers.append(L)
+        self.ctx.check(self.ctx.lib.vrc_rcwa_solver_add_layer(self.h, C.byref(L.struct)), "addLayer")
+
+    def nLayerTypes(self):
+        return len(self._layers)
+
+    def solve(self, lam: float, layerStack, thickness, translation=None):
+        st = np.ascontiguousarray(layerStack, dtype=np.int32)
+        th = np.ascontiguousarray(thickness, dtype=np.float64)
+        tr = None if translation is None else np.ascontiguousarray(translation, dtype=np.float64)
+        if tr is not None and tr.size != st.size:
+            raise ValueError("translation needs one value per stack entry")
+        self.ctx.check(self.ctx.lib.vrc_rcwa_solver_solve(self.h, float(lam), st.size, st.ctypes.data, th.ctypes.data,
+                                                          None if tr is None else tr.ctypes.data), "solve")
+
+    def _dims(self):
+        N, t = C.c_int(), C.c_int()
+        self.ctx.lib.vrc_rcwa_solver_dims(self.h, C.byref(N), C.byref(t))
+        return N.value
+
+    def _get(self, which, k=0):
+        N = self._dims()
+        cnt = N if which == 4 else N * N
+        buf = np.empty(cnt, np.complex128)
+        self.ctx.check(self.ctx.lib.vrc_rcwa_solver_get(self.h, which, int(k), buf.ctypes.data), "get")
+        return buf if which == 4 else buf.reshape((N, N), order="F")
+
+    def Tdd(self):
+        return self._get(0)
+
+    def Rud(self):
+        return self._get(1)
+
+    def eigvecE(self, layerType):
+        return self._get(2, layerType)
+
+    def eigvecH(self, layerType):
+        return self._get(3, layerType)
+
+    def gamma(self, layerType):
+        return self._get(4, layerType)
+
+    def tdd(self, row, col):
+        return complex(self.Tdd()[row, col])
+
+    def rud(self, row, col):
+        return complex(self.Rud()[row, col])
+
+    def transmittance(self, inputMode, outputMode=None):
+        """|Tdd(out, in)|^2, or the column's power sum without outputMode (:582-594, :632-650)."""
+        col = self.Tdd()[:, inputMode]
+        return float(np.sum(np.abs(col) ** 2)) if outputMode is None else float(abs(col[outputMode]) ** 2)
+
+    def reflectance(self, inputMode, outputMode=None):
+        col = self.Rud()[:, inputMode]
+        return float(np.sum(np.abs(col) ** 2)) if outputMode is None else float(abs(col[outputMode]) ** 2)
+
+
 _GETTERS = ["Tdd", "Tuu", "Rud", "Rdu", "W0", "V0", "gam0", "W1", "V1", "gam1"]
 
 

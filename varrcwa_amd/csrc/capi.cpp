@@ -179,6 +179,10 @@ struct vrc_sampler {
   vrc_ctx* ctx;
   std::unique_ptr<vrc::Sampler> s;
 };
+struct vrc_rcwa_solver {
+  vrc_ctx* ctx;
+  std::unique_ptr<vrc::LegacySolver> s;
+};
 struct vrc_integrator {
   vrc_ctx* ctx;
   int kind;
@@ -462,6 +466,62 @@ int vrc_sampler_layout(vrc_sampler* s, double z, int max_coord_x, double* x, int
     std::copy(L.y.begin(), L.y.end(), y);
     std::copy(L.eps.begin(), L.eps.end(), (cd*)eps);
   });
+}
+
+int vrc_rcwa_solver_create(vrc_ctx* c, int nX, int nY, vrc_rcwa_solver** out) {
+  return guarded(c, [&] {
+    need(out != nullptr, "bad arguments");
+    *out = nullptr;
+    check_harmonics(2 * nX + 1, 2 * nY + 1);
+    auto s = std::make_unique<vrc_rcwa_solver>();
+    s->ctx = c;
+    s->s = std::make_unique<LegacySolver>(c, nX, nY);
+    ctx_retain(c);
+    *out = s.release();
+  });
+}
+
+int vrc_rcwa_solver_add_layer(vrc_rcwa_solver* s, const vrc_layer* layer) {
+  if (!s) return -1;
+  return host_guarded(s->ctx, [&] {
+    need(layer != nullptr, "bad arguments");
+    s->s->add_layer(to_host(layer));
+  });
+}
+
+int vrc_rcwa_solver_solve(vrc_rcwa_solver* s, double lambda, int nstack, const int* stack, const double* thickness,
+                          const double* translation) {
+  if (!s) return -1;
+  return guarded(s->ctx, [&] {
+    need(nstack >= 0 && (nstack == 0 || (stack && thickness)), "bad arguments");
+    s->s->solve(lambda, std::vector<int>(stack, stack + nstack), std::vector<double>(thickness, thickness + nstack),
+                translation);
+  });
+}
+
+int vrc_rcwa_solver_get(vrc_rcwa_solver* s, int which, int layer_type, double* host) {
+  if (!s) return -1;
+  return guarded(s->ctx, [&] {
+    need(host != nullptr, "bad arguments");
+    s->s->get(which, layer_type, (cd*)host);
+  });
+}
+
+int vrc_rcwa_solver_dims(const vrc_rcwa_solver* s, int* N, int* ntypes) {
+  if (!s || !N || !ntypes) return -1;
+  *N = s->s->N();
+  *ntypes = s->s->ntypes();
+  return 0;
+}
+
+int vrc_rcwa_solver_destroy(vrc_rcwa_solver* s) {
+  if (!s) return 0;
+  vrc_ctx* c = s->ctx;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  delete s;
+  ctx_release(c);
+  return 0;
 }
 
 int vrc_sampler_destroy(vrc_sampler* s) {

@@ -117,6 +117,13 @@ __global__ void phase_kernel(int n, const cd* gam, long sg, const double* s, cd*
   X[b * sx + i] = cexp(mk(-g.y * ss, g.x * ss));
 }
 
+void phases(hipStream_t s, int n, const cd* gam, long sg, const double* d_s, cd* X, long sx, int batch) {
+  if (n <= 0 || batch <= 0) return;
+  dim3 g(cdiv_up(n, 256), batch);
+  phase_kernel<<<g, 256, 0, s>>>(n, gam, sg, d_s, X, sx);
+  launch_check("phase");
+}
+
 // dst[b] = src[idx[b]] (n x n, ld n), used to give every slice its instance's reference basis.
 __global__ void gather_mats_kernel(long nn, const cd* src, long ss, const int* idx, cd* dst, long sd) {
   const long b = blockIdx.y;

@@ -63,6 +63,31 @@ void assemble_pq(vrc_ctx& c, const AssemblyBatch& a, cd* P, cd* Q, cd* PQ = null
 // ---------------------------------------------------------------- (c)
 // PQ = P Q; eig; gamma = sqrt branch; V = Q W diag(1/gamma)  (src/rcwa/WaveEquationSolver.cpp:24-37)
 void wave_eqn(vrc_ctx& c, int N, const cd* P, const cd* Q, long sPQ, cd* W, cd* V, cd* gam, int* info, int batch);
+// X[b][i] = exp(i gam[b][i] s[b]) (d_s: device, one value per batch entry).
+void phases(hipStream_t s, int n, const cd* gam, long sg, const double* d_s, cd* X, long sx, int batch);
+
+// The legacy stacked-layer solver RCWASolver (src/rcwa/RCWASolver.cpp:73-580): layer types,
+// interface matrices and the RTCM recursion; Tdd and Rud stay on the device.
+class LegacySolver {
+ public:
+  LegacySolver(vrc_ctx* c, int nX, int nY);
+  void add_layer(const HostLayer& L);
+  void solve(double lambda, const std::vector<int>& stack, const std::vector<double>& thickness,
+             const double* translation);
+  int N() const { return 2 * nx_ * ny_; }
+  int ntypes() const { return (int)layers_.size(); }
+  // which: 0 Tdd, 1 Rud (N x N); layer type k: 2 E, 3 H (N x N), 4 gamma (N) (gamma = sqrt(eig) / k0)
+  void get(int which, int k, cd* host) const;
+  bool solved() const { return solved_; }
+
+ private:
+  vrc_ctx* c_;
+  int nx_, ny_;
+  std::vector<HostLayer> layers_;
+  DevBuf W_, V_, gam_, Tdd_, Rud_;
+  double k0_ = 0.0;
+  bool solved_ = false;
+};
 
 // Eigen factors instead of W, V: W = diag(dbal) Z X (unnormalized), and the vectors.
 // PQ (optional): P Q already formed (stride N^2); consumed as the eigensolver's work matrix.
