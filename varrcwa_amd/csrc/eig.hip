@@ -56,27 +56,52 @@ __device__ inline void house(cd alpha, double xnorm, double& beta, cd& tau, cd& 
 
 // ===================================================================== Hessenberg
 // out[i] = sum_{r>=r0} conj(Vf[r, p+i]) * x[r] for i < j, x in LDS.
-__device__ void panel_multi_dot(int N, int r0, int p, int j, const cd* Vf, const cd* x, cd (*red)[HB], cd* out) {
+constexpr int GV_W = 8;  // trailing-product loads in flight per thread
+
+// out[i] = sum_{r >= r0} conj(Vf[r, p+i]) x[r] for i < j (x in LDS): wave w reduces the
+// columns i = w, w + PT/64, ... (the columns proceed in parallel, four loads in flight per
+// lane, rows summed in increasing order within a lane).
+__device__ void panel_multi_dot(int N, int r0, int p, int j, const cd* Vf, const cd* x, cd* out) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int i = 0; i < j; ++i) {
+  for (int i = wave; i < j; i += PT / 64) {
     const cd* vc = Vf + (long)(p + i) * N;
     double sr = 0.0, si = 0.0;
-    for (int r = r0 + t; r < N; r += PT) {
-      cd a = vc[r], xb = x[r];
+    int r = r0 + lane;
+    for (; r + 192 < N; r += 256) {
+      cd a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = vc[r + 64 * u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const cd xb = x[r + 64 * u];
+        sr += a[u].x * xb.x + a[u].y * xb.y;
+        si += a[u].x * xb.y - a[u].y * xb.x;
+      }
+    }
+    for (; r < N; r += 64) {
+      const cd a = vc[r], xb = x[r];
       sr += a.x * xb.x + a.y * xb.y;
       si += a.x * xb.y - a.y * xb.x;
     }
     sr = wsum(sr);
     si = wsum(si);
-    if (lane == 0) red[wave][i] = mk(sr, si);
+    if (lane == 0) out[i] = mk(sr, si);
   }
   __syncthreads();
-  if (t < j) {
-    cd acc = mk(0.0, 0.0);
-    for (int w = 0; w < PT / 64; ++w) acc = cadd(acc, red[w][t]);
-    out[t] = acc;
+}
+
+// acc - sum_{i < j} M[r + i ld] v[i], summed in increasing i with eight loads in flight.
+__device__ inline cd sub_row_dot(cd acc, const cd* M, long ld, int r, int j, const cd* v) {
+  int i = 0;
+  for (; i + 8 <= j; i += 8) {
+    cd m[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m[u] = M[r + (long)(i + u) * ld];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = csub(acc, cmul(m[u], v[i + u]));
   }
-  __syncthreads();
+  for (; i < j; ++i) acc = csub(acc, cmul(M[r + (long)i * ld], v[i]));
+  return acc;
 }
 
 __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, cd* A, long lda, long sA, cd* Vf,
@@ -92,7 +117,6 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
   cd* bvec = dyn;
   cd* vvec = dyn + N;
   __shared__ cd Tl[HB][HB + 1];
-  __shared__ cd red[PT / 64][HB];
   __shared__ cd vec1[HB], vec2[HB];
   __shared__ double rn[PT / 64];
   __shared__ cd gpart[PT];  // per-thread partial sums of the column-split product
@@ -107,26 +131,18 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
     if (t < j) vec1[t] = cconj(Vf[c + (long)(p + t) * N]);
     __syncthreads();
     // column c of A0*Q_j, rows below p (rows <= p are updated after the panel)
-    for (int r = p + 1 + t; r < N; r += PT) {
-      cd acc = A[r + (long)c * lda];
-      for (int i = 0; i < j; ++i) acc = csub(acc, cmul(Y[r + (long)i * N], vec1[i]));
-      bvec[r] = acc;
-    }
+    for (int r = p + 1 + t; r < N; r += PT) bvec[r] = sub_row_dot(A[r + (long)c * lda], Y, N, r, j, vec1);
     __syncthreads();
     if (j > 0) {
       // left transform by Q_j^H on rows >= p+1
-      panel_multi_dot(N, p + 1, p, j, Vf, bvec, red, vec2);
+      panel_multi_dot(N, p + 1, p, j, Vf, bvec, vec2);
       if (t < j) {
         cd s = mk(0.0, 0.0);
         for (int l = 0; l <= t; ++l) s = cadd(s, cmul(cconj(Tl[l][t]), vec2[l]));
         vec1[t] = s;
       }
       __syncthreads();
-      for (int r = p + 1 + t; r < N; r += PT) {
-        cd acc = bvec[r];
-        for (int i = 0; i < j; ++i) acc = csub(acc, cmul(Vf[r + (long)(p + i) * N], vec1[i]));
-        bvec[r] = acc;
-      }
+      for (int r = p + 1 + t; r < N; r += PT) bvec[r] = sub_row_dot(bvec[r], Vf + (long)p * N, N, r, j, vec1);
       __syncthreads();
     }
     // final entries of column c and the reflector from rows c+1..N-1
@@ -158,27 +174,27 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
     if (t == 0) tau[c] = tv;
     __syncthreads();
     // z = V_j^H v  (rows >= c+1)
-    if (j > 0) panel_multi_dot(N, c + 1, p, j, Vf, vvec, red, vec2);
+    if (j > 0) panel_multi_dot(N, c + 1, p, j, Vf, vvec, vec2);
     // y_j = tau*(A0[:, c+1:] v - Y_j z) for the rows below p (the rows above are one
     // ZGEMM after the panel). The trailing-matrix product is the memory-bound part.
     if (R > 0 && R <= PT) {
-      // thread (g, i): row p+1+i, columns c+1+8g+8Gk .. +8 (eight loads in flight)
+      // thread (g, i): row p+1+i, columns c+1+GV_W g+GV_W Gk .. +GV_W (GV_W loads in flight)
       const int g = t / R, i = t - g * R;
       double sr = 0.0, si = 0.0;
       if (g < G) {
         const int r = p + 1 + i;
-        for (int col0 = c + 1 + 8 * g; col0 < N; col0 += 8 * G) {
-          cd a[8], v[8];
+        for (int col0 = c + 1 + GV_W * g; col0 < N; col0 += GV_W * G) {
+          cd a[GV_W];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
+          for (int u = 0; u < GV_W; ++u) {
             const bool in = col0 + u < N;
             a[u] = in ? A[r + (long)(col0 + u) * lda] : mk(0.0, 0.0);
-            v[u] = in ? vvec[col0 + u] : mk(0.0, 0.0);
           }
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            sr += a[u].x * v[u].x - a[u].y * v[u].y;
-            si += a[u].x * v[u].y + a[u].y * v[u].x;
+          for (int u = 0; u < GV_W; ++u) {
+            const cd v = col0 + u < N ? vvec[col0 + u] : mk(0.0, 0.0);
+            sr += a[u].x * v.x - a[u].y * v.y;
+            si += a[u].x * v.y + a[u].y * v.x;
           }
         }
       }
@@ -188,7 +204,7 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
         const int r = p + 1 + t;
         cd acc = gpart[t];
         for (int q = 1; q < G; ++q) acc = cadd(acc, gpart[q * R + t]);
-        for (int i2 = 0; i2 < j; ++i2) acc = csub(acc, cmul(Y[r + (long)i2 * N], vec2[i2]));
+        acc = sub_row_dot(acc, Y, N, r, j, vec2);
         Y[r + (long)j * N] = cmul(tv, acc);
       }
     } else
