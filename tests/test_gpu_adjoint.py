@@ -263,7 +263,7 @@ def test_stack_gradient_small(ctx):
     assert err < TOL_GRAD
     # CPU oracle: J itself and finite differences on 4 cells (instance 0, slab 0)
     j_or = sum(_oracle_objective(i, nx, ny) for i in insts)
-    assert abs(j_or - float(J)) < 1e-10 * abs(j_or)
+    assert abs(j_or - float(J.detach())) < 1e-10 * abs(j_or)
     for k in rng.choice(16, 4, replace=False):
         vals = []
         for sgn in (1, -1):

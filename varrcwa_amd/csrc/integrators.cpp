@@ -595,6 +595,159 @@ void Integrator::differential(int Nsteps) {
 }
 
 // ===================================================================== batched stacks
+// Cascade of one group of GB instances (slab S-matrices tu/ru [instance][slab], end systems
+// We/Ve/ge [bottoms | tops]) and the end projection, batched over the group; results into
+// the outputs at instance offset I0.
+static void cascade_group(vrc_ctx& c, int N, int L, int GB, int I0, bool homog, const cd* tu, const cd* ru,
+                          const cd* We, const cd* Ve, const cd* ge, cd* S_out, cd* W0o, cd* V0o, cd* g0o, cd* W1o,
+                          cd* V1o, cd* g1o) {
+  hipStream_t st = c.stream;
+  const long NN = (long)N * N;
+  const int B = GB, i0 = I0;
+  const cd* W0 = We;
+  const cd* V0 = Ve;
+  const cd* W1 = We + (long)GB * NN;
+  const cd* V1 = Ve + (long)GB * NN;
+  const cd* g0 = ge;
+  const cd* g1 = ge + (long)GB * N;
+  LUStore lu1;
+  if (!homog) lu1.make(c, N, W1, V1, B, NN);
+  DevBuf Sb = c.ws.get(sizeof(cd) * 4 * NN * B);
+  SMat S;
+  S.Tdd = Sb.as<cd>();
+  S.Tuu = S.Tdd + B * NN;
+  S.Rud = S.Tdd + 2 * B * NN;
+  S.Rdu = S.Tdd + 3 * B * NN;
+  // identity * slab 0 = slab 0: the cascade starts from the first slab's S-matrix
+  zgeam(st, N, N, ONE, tu, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Tdd, N, NN, B);
+  zgeam(st, N, N, ONE, tu, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Tuu, N, NN, B);
+  zgeam(st, N, N, ONE, ru, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rud, N, NN, B);
+  zgeam(st, N, N, ONE, ru, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rdu, N, NN, B);
+  for (int l = 1; l < L; ++l) {
+    const cd* t = tu + l * NN;
+    const cd* r = ru + l * NN;
+    star_product(c, N, B, S, t, t, r, r, (long)L * NN);
+  }
+  if (homog)
+    project_left_homog(c, N, B, S, V0, V1, NN);  // W0 = W1 = I
+  else
+    project_left(c, N, B, S, lu1.pair, W0, V0, NN);
+  // ---- outputs
+  const long so = 4 * NN;
+  cd* So = S_out + (long)i0 * so;
+  zgeam(st, N, N, ONE, S.Tdd, N, NN, ZERO, nullptr, 0, 0, So, N, so, B);
+  zgeam(st, N, N, ONE, S.Tuu, N, NN, ZERO, nullptr, 0, 0, So + NN, N, so, B);
+  zgeam(st, N, N, ONE, S.Rud, N, NN, ZERO, nullptr, 0, 0, So + 2 * NN, N, so, B);
+  zgeam(st, N, N, ONE, S.Rdu, N, NN, ZERO, nullptr, 0, 0, So + 3 * NN, N, so, B);
+  auto out = [&](cd* dst, const cd* src, long sz) {
+    if (dst) zgeam(st, (int)sz, 1, ONE, src, sz, sz, ZERO, nullptr, 0, 0, dst + (long)i0 * sz, sz, sz, B);
+  };
+  out(W0o, W0, NN);
+  out(V0o, V0, NN);
+  out(g0o, g0, N);
+  out(W1o, W1, NN);
+  out(V1o, V1, NN);
+  out(g1o, g1, N);
+}
+
+// Forward path for instances whose end media are single-valued (closed-form eigen-systems):
+// the slab eigensolves of a cascade group are split evenly into launches of at most
+// `per_launch` slabs, regardless of instance boundaries (the eigen kernels run one matrix
+// per CU, so full launches matter more than whole instances).
+static void batch_stack_homog(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int per_launch,
+                              cd* S_out, cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o) {
+  hipStream_t st = c.stream;
+  const int n = nx * ny, N = 2 * n;
+  const long NN = (long)N * N;
+  const int ninst = (int)inst.size();
+  const int L = (int)inst[0].layers.size();
+  const int gsz = std::max(1, g_cascade_group);
+  const int ngroups = (ninst + gsz - 1) / gsz;
+  for (int g = 0; g < ngroups; ++g) {
+    const int I0 = (int)((long)ninst * g / ngroups), GB = (int)((long)ninst * (g + 1) / ngroups) - I0;
+    DevBuf tuG = c.ws.get(sizeof(cd) * NN * GB * L), ruG = c.ws.get(sizeof(cd) * NN * GB * L);
+    DevBuf WeG = c.ws.get(sizeof(cd) * NN * 2 * GB), VeG = c.ws.get(sizeof(cd) * NN * 2 * GB),
+           geG = c.ws.get(sizeof(cd) * (size_t)N * 2 * GB);
+    // per instance: alpha/beta, k0^2; end media [bottoms | tops] in closed form
+    std::vector<cd> abI((size_t)GB * 2 * n), ev(2 * GB);
+    std::vector<double> k0I(GB), k2I(GB), k2E(2 * GB);
+    for (int b = 0; b < GB; ++b) {
+      const StackInstance& I = inst[I0 + b];
+      const double k0 = 2 * PI_REF / I.lambda;
+      std::vector<cd> al, be;
+      alpha_beta(nx, ny, I.bottom->x.back() - I.bottom->x.front(), I.bottom->y.back() - I.bottom->y.front(), al, be);
+      std::copy(al.begin(), al.end(), abI.begin() + (size_t)b * 2 * n);
+      std::copy(be.begin(), be.end(), abI.begin() + (size_t)b * 2 * n + n);
+      k0I[b] = k0;
+      k2I[b] = k0 * k0;
+      k2E[b] = k2E[GB + b] = k0 * k0;
+      ev[b] = I.bottom->eps[0];
+      ev[GB + b] = I.top->eps[0];
+    }
+    {
+      std::vector<cd> abE((size_t)2 * GB * 2 * n);
+      std::copy(abI.begin(), abI.end(), abE.begin());
+      std::copy(abI.begin(), abI.end(), abE.begin() + (size_t)GB * 2 * n);
+      DevBuf dabE = upload(c, abE.data(), sizeof(cd) * abE.size());
+      DevBuf dk2E = upload(c, k2E.data(), sizeof(double) * k2E.size());
+      DevBuf dev = upload(c, ev.data(), sizeof(cd) * ev.size());
+      homogeneous_eig(st, n, 2 * GB, dabE.as<cd>(), dabE.as<cd>() + n, 2 * n, dk2E.as<double>(), dev.as<cd>(),
+                      WeG.as<cd>(), VeG.as<cd>(), geG.as<cd>());
+    }
+    const cd* V1 = VeG.as<cd>() + (long)GB * NN;
+    // slab launches: the group's GB * L slabs (instance-major) split evenly
+    const int SLall = GB * L;
+    const int nch = (SLall + per_launch - 1) / per_launch;
+    for (int ch = 0; ch < nch; ++ch) {
+      const int s0 = (int)((long)SLall * ch / nch), SL = (int)((long)SLall * (ch + 1) / nch) - s0;
+      std::vector<const HostLayer*> jl(SL);
+      std::vector<double> k2(SL), sv(SL);
+      std::vector<cd> ab((size_t)SL * 2 * n);
+      std::vector<int> owner(SL);
+      for (int q = 0; q < SL; ++q) {
+        const int b = (s0 + q) / L, l = (s0 + q) % L;
+        const StackInstance& I = inst[I0 + b];
+        jl[q] = I.layers[l];
+        k2[q] = k2I[b];
+        sv[q] = I.thickness[l] / k0I[b];
+        owner[q] = b;
+        std::copy(abI.begin() + (size_t)b * 2 * n, abI.begin() + (size_t)(b + 1) * 2 * n, ab.begin() + (size_t)q * 2 * n);
+      }
+      DevBuf dk2 = upload(c, k2.data(), sizeof(double) * SL);
+      DevBuf dab = upload(c, ab.data(), sizeof(cd) * ab.size());
+      DevBuf dsv = upload(c, sv.data(), sizeof(double) * SL);
+      DevBuf down = upload(c, owner.data(), sizeof(int) * SL);
+      AssemblyBatch a;
+      a.nx = nx;
+      a.ny = ny;
+      a.nlayers = SL;
+      a.layers = jl;
+      a.periodic = true;
+      a.d_alpha = dab.as<cd>();
+      a.d_beta = a.d_alpha + n;
+      a.sab = 2 * n;
+      a.d_k0sq = dk2.as<double>();
+      DevBuf P = c.ws.get(sizeof(cd) * NN * SL), Q = c.ws.get(sizeof(cd) * NN * SL),
+             PQ = c.ws.get(sizeof(cd) * NN * SL);
+      assemble_pq(c, a, P.as<cd>(), Q.as<cd>(), PQ.as<cd>());
+      DevBuf gam = c.ws.get(sizeof(cd) * (size_t)N * SL), inf = c.ws.get(sizeof(int) * SL);
+      fill_int(st, inf.as<int>(), 0, SL);
+      DevBuf Zb = c.ws.get(sizeof(cd) * NN * SL), Xb = c.ws.get(sizeof(cd) * NN * SL),
+             Db = c.ws.get(sizeof(cd) * (size_t)N * SL);
+      wave_eqn_factors(c, N, P.as<cd>(), Q.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(), gam.as<cd>(),
+                       inf.as<int>(), SL, PQ.as<cd>());
+      PQ = DevBuf();
+      DevBuf Vr = c.ws.get(sizeof(cd) * NN * SL);
+      gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);  // W1 = I
+      layer_smatrix_factors(c, N, SL, dsv.as<double>(), P.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(),
+                            gam.as<cd>(), nullptr, Vr.as<cd>(), NN, tuG.as<cd>() + (long)s0 * NN,
+                            ruG.as<cd>() + (long)s0 * NN, true);
+    }
+    cascade_group(c, N, L, GB, I0, true, tuG.as<cd>(), ruG.as<cd>(), WeG.as<cd>(), VeG.as<cd>(), geG.as<cd>(), S_out,
+                  W0o, V0o, g0o, W1o, V1o, g1o);
+  }
+}
+
 void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int chunk, cd* S_out,
                       cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o, StackTape* tape) {
   hipStream_t st = c.stream;
@@ -614,6 +767,10 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
   for (auto& I : inst)
     homog = homog && I.bottom->x.size() == 2 && I.bottom->y.size() == 2 && I.top->x.size() == 2 &&
             I.top->y.size() == 2;
+  if (homog && !tape) {
+    batch_stack_homog(c, nx, ny, inst, chunk > 0 ? chunk * L : 256, S_out, W0o, V0o, g0o, W1o, V1o, g1o);
+    return;
+  }
   // Default chunk: at most 256 dense eigensolves (L slices, + 2 end media per instance
   // unless closed form) per launch, one matrix per CU of the QR kernel; the instances
   // are then split evenly over the chunks so that no launch runs a short tail.
@@ -756,56 +913,8 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     zgeam(st, N, 1, ONE, gall.as<cd>() + (long)B * N, N, N, ZERO, nullptr, 0, 0, geG.as<cd>() + (GB + og) * N, N, N,
           B);
   }
-  {
-    // ---- cascade over layers and the end projection, batched over the group's instances
-    const int B = GB, i0 = I0;
-    const cd* tu = tuG.as<cd>();
-    const cd* ru = ruG.as<cd>();
-    const cd* W0 = WeG.as<cd>();
-    const cd* V0 = VeG.as<cd>();
-    const cd* W1 = WeG.as<cd>() + (long)GB * NN;
-    const cd* V1 = VeG.as<cd>() + (long)GB * NN;
-    const cd* g0 = geG.as<cd>();
-    const cd* g1 = geG.as<cd>() + (long)GB * N;
-    LUStore lu1;
-    if (!homog) lu1.make(c, N, W1, V1, B, NN);
-    DevBuf Sb = c.ws.get(sizeof(cd) * 4 * NN * B);
-    SMat S;
-    S.Tdd = Sb.as<cd>();
-    S.Tuu = S.Tdd + B * NN;
-    S.Rud = S.Tdd + 2 * B * NN;
-    S.Rdu = S.Tdd + 3 * B * NN;
-    // identity * slab 0 = slab 0: the cascade starts from the first slab's S-matrix
-    zgeam(st, N, N, ONE, tu, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Tdd, N, NN, B);
-    zgeam(st, N, N, ONE, tu, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Tuu, N, NN, B);
-    zgeam(st, N, N, ONE, ru, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rud, N, NN, B);
-    zgeam(st, N, N, ONE, ru, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rdu, N, NN, B);
-    for (int l = 1; l < L; ++l) {
-      const cd* t = tu + l * NN;
-      const cd* r = ru + l * NN;
-      star_product(c, N, B, S, t, t, r, r, (long)L * NN);
-    }
-    if (homog)
-      project_left_homog(c, N, B, S, V0, V1, NN);  // W0 = W1 = I
-    else
-      project_left(c, N, B, S, lu1.pair, W0, V0, NN);
-    // ---- outputs
-    const long so = 4 * NN;
-    cd* So = S_out + (long)i0 * so;
-    zgeam(st, N, N, ONE, S.Tdd, N, NN, ZERO, nullptr, 0, 0, So, N, so, B);
-    zgeam(st, N, N, ONE, S.Tuu, N, NN, ZERO, nullptr, 0, 0, So + NN, N, so, B);
-    zgeam(st, N, N, ONE, S.Rud, N, NN, ZERO, nullptr, 0, 0, So + 2 * NN, N, so, B);
-    zgeam(st, N, N, ONE, S.Rdu, N, NN, ZERO, nullptr, 0, 0, So + 3 * NN, N, so, B);
-    auto out = [&](cd* dst, const cd* src, long sz) {
-      if (dst) zgeam(st, (int)sz, 1, ONE, src, sz, sz, ZERO, nullptr, 0, 0, dst + (long)i0 * sz, sz, sz, B);
-    };
-    out(W0o, W0, NN);
-    out(V0o, V0, NN);
-    out(g0o, g0, N);
-    out(W1o, W1, NN);
-    out(V1o, V1, NN);
-    out(g1o, g1, N);
-  }
+  cascade_group(c, N, L, GB, I0, homog, tuG.as<cd>(), ruG.as<cd>(), WeG.as<cd>(), VeG.as<cd>(), geG.as<cd>(), S_out,
+                W0o, V0o, g0o, W1o, V1o, g1o);
   }
 }
 
