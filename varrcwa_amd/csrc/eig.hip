@@ -58,6 +58,14 @@ __device__ inline void house(cd alpha, double xnorm, double& beta, cd& tau, cd& 
 // out[i] = sum_{r>=r0} conj(Vf[r, p+i]) * x[r] for i < j, x in LDS.
 constexpr int GV_W = 8;  // trailing-product loads in flight per thread
 
+// One complex value read with the non-temporal hint (streamed once per column, not reused
+// from cache).
+typedef double vrc_d2 __attribute__((ext_vector_type(2)));
+__device__ inline cd load_stream(const cd* p) {
+  const vrc_d2 v = __builtin_nontemporal_load(reinterpret_cast<const vrc_d2*>(p));
+  return mk(v.x, v.y);
+}
+
 // out[i] = sum_{r >= r0} conj(Vf[r, p+i]) x[r] for i < j (x in LDS): wave w reduces the
 // columns i = w, w + PT/64, ... (the columns proceed in parallel, four loads in flight per
 // lane, rows summed in increasing order within a lane).
@@ -188,7 +196,7 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
 #pragma unroll
           for (int u = 0; u < GV_W; ++u) {
             const bool in = col0 + u < N;
-            a[u] = in ? A[r + (long)(col0 + u) * lda] : mk(0.0, 0.0);
+            a[u] = in ? load_stream(A + r + (long)(col0 + u) * lda) : mk(0.0, 0.0);
           }
 #pragma unroll
           for (int u = 0; u < GV_W; ++u) {
