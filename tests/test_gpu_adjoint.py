@@ -336,6 +336,6 @@ def test_metasurface_inverse_design_gradient(ctx):
             fd6[t] = (Jdir(ek, 1e-6) - Jdir(ek, -1e-6)) / 2e-6
     err = np.linalg.norm(g[pick] - fdp) / np.linalg.norm(fdp)
     err6 = np.linalg.norm(g[pick] - fd6) / np.linalg.norm(fd6)
-    print(f"J={float(J):.12f} |g|={np.linalg.norm(g):.3e}; 32 pixels: rel err vs 4th-order FD (h=5e-2) {err:.2e}, "
+    print(f"J={float(J.detach()):.12f} |g|={np.linalg.norm(g):.3e}; 32 pixels: rel err vs 4th-order FD (h=5e-2) {err:.2e}, "
           f"vs central FD (h=1e-6, noise-dominated) {err6:.2e}", flush=True)
     assert err < 1e-4
