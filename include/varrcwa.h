@@ -268,7 +268,9 @@ int vrc_integrator_tree(vrc_integrator* it, int cap, double* host_out);
  * and nlayers. S_out: per instance [Tdd|Tuu|Rud|Rdu] (4 N x N, device); the
  * optional W0/V0/gam0/W1/V1/gam1 outputs may be NULL. Instances are solved in chunks
  * of at most `chunk` (<= 0: as many as keep one launch within 256 eigensolves), split
- * evenly over the chunks. */
+ * evenly over the chunks; with single-valued half-spaces (and no tape) the slab eigensolves
+ * of each cascade group go in even launches of at most chunk * nlayers (<= 0: 256) slabs,
+ * across instance boundaries. ninst = 0 does nothing (S_out may then be NULL). */
 typedef struct vrc_stack_instance {
   double lambda;
   int nlayers;

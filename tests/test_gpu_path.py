@@ -497,6 +497,63 @@ def test_legacy_rcwa_solver(ctx):
         g.addLayer(x * 1.1, y, e)
 
 
+def test_edge_cases_and_argument_errors(ctx, tmp_path):
+    """Empty batches and zero-sized requests return empty results; a zero-thickness slab is
+    transparent (Tuu = Tdd = I, Rud = Rdu = 0 between equal media); malformed requests are
+    refused with an error before any kernel runs."""
+    import torch
+    from varrcwa_amd._lib import ptr
+    nx = ny = 3
+    n, N = nx * ny, 2 * nx * ny
+    dev = f"cuda:{ctx.device}"
+    out = api.batch_stack_rcwa(ctx, nx, ny, [], want_bases=True)
+    assert tuple(out["S"].shape) == (0, 4, N, N)
+    REF, TRN = api.stack_efficiencies(ctx, nx, ny, [], out, torch.zeros((0, 2 * n), dtype=torch.complex128,
+                                                                        device=dev))
+    assert REF.numel() == 0 and TRN.numel() == 0
+    # zero-thickness patterned slab between equal media
+    inst = _stack_case(4)
+    inst["thickness"] = np.array([0.0, 0.0])
+    inst["bottom"] = inst["top"] = G.homogeneous(0.6, 1.0)
+    S = from_dev_colmajor(api.batch_stack_rcwa(ctx, nx, ny, [inst])["S"])[0]
+    eye = np.eye(N)
+    assert np.abs(S[0] - eye).max() < 1e-11 and np.abs(S[1] - eye).max() < 1e-11
+    assert np.abs(S[2]).max() < 1e-11 and np.abs(S[3]).max() < 1e-11
+    # instances with different slab counts in one batch, even / too many harmonics
+    bad = [_stack_case(0, 2), _stack_case(1, 3)]
+    with pytest.raises(api.VrcError):
+        api.batch_stack_rcwa(ctx, nx, ny, bad)
+    for hx in (4, 65):
+        with pytest.raises(api.VrcError):
+            api.batch_stack_rcwa(ctx, hx, 3, [_stack_case(0)])
+    # modal power: no modes; an empty sampling grid
+    W = torch.zeros((0, N), dtype=torch.complex128, device=dev)
+    assert api.modal_power(ctx, nx, ny, W, W, 1.0, 1.0, 10, 10, 0.0, 1.0, 0.0, 1.0).numel() == 0
+    W1 = torch.eye(N, dtype=torch.complex128, device=dev)
+    e = torch.empty(N, dtype=torch.float64, device=dev)
+    assert ctx.lib.vrc_modal_power(ctx.h, nx, ny, N, ptr(W1), N, ptr(W1), N, 1.0, 1.0, 0, 10, 0.0, 1.0, 0.0, 1.0,
+                                   ptr(e)) != 0
+    assert ctx.lib.vrc_modal_power(ctx.h, nx, ny, N, ptr(W1), N - 1, ptr(W1), N, 1.0, 1.0, 4, 4, 0.0, 1.0, 0.0, 1.0,
+                                   ptr(e)) != 0
+    # no shape parameters: only the wavelength derivatives
+    g = G.metasurface(1)
+    lay = (g["x"], g["y"], g["eps"])
+    d = api.wave_eqn_derivatives(ctx, 5, 3, lay, 1.55, False, lay, {})
+    a, b = O.alpha_beta(5, 3, g["x"][-1] - g["x"][0], g["y"][-1] - g["y"][0])
+    _, _, dPl, dQl, _ = O.wave_eqn_derivatives(g["x"], g["y"], g["eps"], np.diag(a), np.diag(b), 1.55, 5, 3, {})
+    assert d["dP"].shape[0] == 0
+    assert np.abs(from_dev_colmajor(d["dQl"]) - dQl).max() <= 1e-12 * np.abs(dQl).max()
+    assert np.abs(from_dev_colmajor(d["dPl"]) - dPl).max() <= 1e-12 * np.abs(dPl).max()
+    # GDSII: missing file; legacy solver: stack entry out of range, a one-entry stack
+    with pytest.raises(api.VrcError):
+        api.GDSIISampler(1.55, 3, 3, str(tmp_path / "missing.gds"), ctx=ctx)
+    s = api.RCWASolver(1, 1, ctx=ctx)
+    s.addLayer(*_legacy_layer(2.1))
+    for stack, th in (([0, 1], [0.0, 0.0]), ([0], [0.0])):
+        with pytest.raises(api.VrcError):
+            s.solve(1.3, stack, th)
+
+
 def test_homogeneous_closed_form(ctx):
     """Single-value end media: W = I, PQ W = W Gamma^2 and V = Q W / Gamma against the
     oracle's P and Q (the closed form of a homogeneous medium)."""

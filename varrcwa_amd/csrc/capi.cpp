@@ -759,7 +759,11 @@ int vrc_batch_stack_forward(vrc_ctx* c, int nx, int ny, int ninst, const vrc_sta
   if (tape) *tape = nullptr;
   return guarded(c, [&] {
     check_harmonics(nx, ny);
-    need(ninst >= 0 && (ninst == 0 || inst) && S_out, "bad arguments");
+    need(ninst >= 0 && (ninst == 0 || (inst && S_out)), "bad arguments");
+    if (ninst == 0) {  // nothing to solve
+      need(tape == nullptr, "a gradient tape needs at least one instance");
+      return;
+    }
     HostStack hs(ninst, inst);
     std::unique_ptr<vrc_tape> tp;
     if (tape) {
