@@ -29,6 +29,24 @@ def spectrum_rel(a, b):
     return d[r, c].max() / np.abs(b).max()
 
 
+def test_eig_pq_large_N(ctx):
+    """Deep-stack harmonic count (25 x 25 harmonics, N = 1250): the Hessenberg panel's rows no
+    longer fit one row per thread, the LU panel leaves the register path; eigen residual,
+    branch rule, V = Q W / gamma and the spectrum against LAPACK."""
+    g = G.metasurface(12)
+    s_or = G.oracle_sampler(g)
+    P, Q = s_or.sample(0.7)
+    assert P.shape == (1250, 1250)
+    W, V, gam = _eig_system(ctx, P, Q)
+    PQ = P @ Q
+    lam = gam * gam
+    res = np.abs(PQ @ W - W * lam[None, :]).max() / np.abs(PQ).max()
+    assert res < 1e-12 * PQ.shape[0], res
+    assert np.allclose(gam, O.gamma_sqrt(lam), rtol=1e-12, atol=0)
+    assert rel(V, (Q @ W) / gam[None, :]) < 1e-11
+    assert spectrum_rel(lam, np.linalg.eigvals(PQ)) < 1e-11
+
+
 def layer_set():
     rng = np.random.default_rng(3)
     g = G.metasurface(2)
@@ -217,6 +235,22 @@ def test_batch_stack_parity(ctx):
     got = O.SResult(S[0, 0], S[0, 1], S[0, 2], S[0, 3], bases["W0"][0], bases["V0"][0], bases["gam0"][0],
                     bases["W1"][0], bases["V1"][0], bases["gam1"][0])
     invariants_close(gpu_result(it), got, tol=1e-11)
+
+
+def test_batch_stack_large_N(ctx):
+    """One stack at 25 x 25 harmonics (N = 1250, the deep-stack configuration's size): LU panels
+    beyond the register path in the slab and star-product solves, against the oracle."""
+    nx = ny = 25
+    inst = _stack_case(7)
+    out = api.batch_stack_rcwa(ctx, nx, ny, [inst], want_bases=True)
+    S = from_dev_colmajor(out["S"])[0]
+    b = {k: (from_dev_colmajor(out[k])[0] if not k.startswith("gam") else out[k].cpu().numpy()[0])
+         for k in ["W0", "V0", "gam0", "W1", "V1", "gam1"]}
+    ze = np.concatenate([[0.0], np.cumsum(inst["thickness"])])
+    so = O.StackLayerSampler(inst["lam"], nx, ny, inst["layers"], ze, inst["bottom"], inst["top"])
+    ref = O.rcwa_integrate(so, ze[0], ze[-1], len(inst["layers"]), z_edges=ze)
+    got = O.SResult(S[0], S[1], S[2], S[3], b["W0"], b["V0"], b["gam0"], b["W1"], b["V1"], b["gam1"])
+    invariants_close(got, ref)
 
 
 def test_batch_stack_chunks_and_groups(ctx):
