@@ -194,6 +194,13 @@ int vrc_stack_sampler_create(vrc_ctx* ctx, double lambda, int nx, int ny, int nl
 int vrc_gds_sampler_create(vrc_ctx* ctx, double lambda, int nx, int ny, const char* filename, const double* shift,
                            const double* pml_space, const double* eps_background, const double* eps_device,
                            double thickness, vrc_sampler** out);
+/* The polygons (BOUNDARY/BOX elements, closing point dropped) of cell `cell` of a GDSII file
+ * as gdstk::read_gds reads them (src/gdstk/library.cpp:160-420), coordinates scaled to `unit`
+ * meters. Host only, no context. Writes *npolygons always; counts (points per polygon) when
+ * npolygons <= max_polygons; xy (x, y pairs, polygon after polygon) when the total point count
+ * is <= max_points. */
+int vrc_gds_polygons(const char* filename, double unit, int cell, int max_polygons, int* npolygons, int* counts,
+                     int max_points, double* xy);
 /* GDSIISampler::minX/maxX/minZ/maxZ (src/core/GDSIISampler.h:41-44) into out4. */
 int vrc_gds_sampler_bounds(const vrc_sampler* s, double* out4);
 /* The layer a sampler sees at z (GDSIISampler::sampleLayout, src/core/GDSIISampler.cpp:94-146;

@@ -45,3 +45,25 @@ def test_find_guided_modes_host_entry_point():
     lib = _lib.load()
     cnt = C.c_int(0)
     assert lib.vrc_find_guided_modes(None, 1, g.ctypes.data, -1.0, 0.2, 1.0, 2.0, None, None, C.byref(cnt)) == -1
+
+
+def test_gds_reader_host_entry_point(tmp_path):
+    """vrc_gds_polygons runs the library's GDSII reader on the host: the same polygons, bit for
+    bit, as the oracle's reader on the reference's layout file and on a written file (closing
+    point, BOX, PATH and second cell), and an error for a missing file."""
+    import os
+
+    import pytest
+
+    from oracle import rcwa_oracle as O
+    from tests.gds_files import synthetic_cells, write_gds
+    from varrcwa_amd import api
+    f = tmp_path / "t.gds"
+    write_gds(f, synthetic_cells())
+    fixture = os.path.join(os.path.dirname(__file__), "golden", "contour_extract.gds")
+    for path in (str(f), fixture):
+        ref = O.read_gds_polygons(path)
+        for cell in range(len(ref)):
+            assert api.gds_polygons(path, 1e-6, cell) == ref[cell]
+    with pytest.raises(api.VrcError):
+        api.gds_polygons(str(tmp_path / "missing.gds"))

@@ -94,6 +94,7 @@ def load():
                                      vp, vp, vp, vp, vp, vp),
         "vrc_gds_sampler_create": (i, vp, d, i, i, C.c_char_p, vp, vp, vp, vp, d, C.POINTER(vp)),
         "vrc_gds_sampler_bounds": (i, vp, vp),
+        "vrc_gds_polygons": (i, C.c_char_p, d, i, i, c_int_p, vp, i, vp),
         "vrc_sampler_layout": (i, vp, d, i, vp, vp, i, vp, vp, vp),
         "vrc_rcwa_solver_create": (i, vp, i, i, C.POINTER(vp)),
         "vrc_rcwa_solver_add_layer": (i, vp, C.POINTER(VrcLayer)),

@@ -277,6 +277,29 @@ class RCWASolver:
         return float(np.sum(np.abs(col) ** 2)) if outputMode is None else float(abs(col[outputMode]) ** 2)
 
 
+def gds_polygons(filename, unit: float = 1e-6, cell: int = 0):
+    """The polygons of one cell of a GDSII file as the library reads them (gdstk::read_gds,
+    src/gdstk/library.cpp:160-420): a list of [(x, y), ...] in units of `unit` meters. Host only."""
+    lib = load()
+    n = C.c_int(0)
+    fn = str(filename).encode()
+    rc = lib.vrc_gds_polygons(fn, float(unit), int(cell), 0, C.byref(n), None, 0, None)
+    if rc != 0:
+        raise VrcError(f"gds_polygons failed ({rc}) for {filename}")
+    counts = np.zeros(max(n.value, 1), dtype=np.int32)
+    lib.vrc_gds_polygons(fn, float(unit), int(cell), n.value, C.byref(n), counts.ctypes.data, 0, None)
+    total = int(counts[:n.value].sum())
+    xy = np.zeros(2 * max(total, 1))
+    if lib.vrc_gds_polygons(fn, float(unit), int(cell), n.value, C.byref(n), counts.ctypes.data, total,
+                            xy.ctypes.data) != 0:
+        raise VrcError(f"gds_polygons failed for {filename}")
+    out, o = [], 0
+    for k in range(n.value):
+        out.append([(float(xy[2 * (o + i)]), float(xy[2 * (o + i) + 1])) for i in range(int(counts[k]))])
+        o += int(counts[k])
+    return out
+
+
 _GETTERS = ["Tdd", "Tuu", "Rud", "Rdu", "W0", "V0", "gam0", "W1", "V1", "gam1"]
 
 

@@ -438,6 +438,29 @@ int vrc_gds_sampler_create(vrc_ctx* c, double lambda, int nx, int ny, const char
   });
 }
 
+int vrc_gds_polygons(const char* filename, double unit, int cell, int max_polygons, int* npolygons, int* counts,
+                     int max_points, double* xy) {
+  return host_guarded(nullptr, [&] {
+    need(filename && npolygons, "bad arguments");
+    GdsLibrary lib = read_gds(filename, unit);
+    need(cell >= 0 && cell < (int)lib.cells.size(), "no such cell");
+    const auto& polys = lib.cells[cell].polygons;
+    *npolygons = (int)polys.size();
+    if ((int)polys.size() > max_polygons || !counts) return;  // count only
+    long total = 0;
+    for (const auto& p : polys) total += (long)p.x.size();
+    for (size_t k = 0; k < polys.size(); ++k) counts[k] = (int)polys[k].x.size();
+    if (total > max_points || !xy) return;
+    long o = 0;
+    for (const auto& p : polys)
+      for (size_t i = 0; i < p.x.size(); ++i) {
+        xy[2 * o] = p.x[i];
+        xy[2 * o + 1] = p.y[i];
+        ++o;
+      }
+  });
+}
+
 int vrc_gds_sampler_bounds(const vrc_sampler* s, double* out4) {
   if (!s || !out4) return -1;
   const GdsSampler* g = dynamic_cast<const GdsSampler*>(s->s.get());
