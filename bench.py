@@ -10,33 +10,48 @@ into the cascade; every instance also solves its two end media and the end proje
 (counted as overhead, not as slices). The library splits the batch into launches of at
 most 256 eigensolves.
 
-Also reported: the plain per-instance path (RCWAIntegrator over one instance's stack,
-one slice after another), live stage times from HIP events on the library stream, the
-roofline of the dominant kernel, and the CPU oracle timed on one whole instance.
+The run is bounded by a wall-clock budget (``--budget`` seconds from process start):
+the short legs run first (plain per-instance path, CPU oracle on instance 0, adjoint),
+then at most one warm-up step, then timed steps until the next step would overrun the
+budget; ``steps`` in the output is the number of steps actually timed and
+``steps_requested`` the number asked for. Instance 0 of the last timed step is checked
+against the CPU oracle (basis-invariant transforms), and the error goes into the line.
+
+Also reported: live stage times from HIP events on the library stream, the roofline of
+the dominant kernel, and the CPU oracle timed on one whole instance.
 Prints ONE JSON line.
 """
 from __future__ import annotations
 
-import argparse
-import ctypes as C
-import json
-import os
-import sys
 import time
-from pathlib import Path
 
-import numpy as np
+T_START = time.perf_counter()
+
+import argparse  # noqa: E402
+import ctypes as C  # noqa: E402
+import json  # noqa: E402
+import os  # noqa: E402
+import sys  # noqa: E402
+from pathlib import Path  # noqa: E402
+
+import numpy as np  # noqa: E402
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "S-matrix solves/s @441 harmonics c128, batched+adjoint vs plain; FP64 MFMA %"
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 STAGES = ["fourier", "pq", "hessenberg", "qr", "eigvec", "eigpost", "layer_s", "star", "project", "adjoint"]
 
 
+def log(*a):
+    print(f"[{time.perf_counter() - T_START:7.1f}s]", *a, file=sys.stderr, flush=True)
+
+
 def flops_slice(N):
-    """Algorithmic real flops of one slice solve (F(N) = 305 N^3)."""
+    """Algorithmic real flops of one slice solve of the reference's operation sequence
+    (F(N) = 305 N^3)."""
     return 305.0 * N ** 3
 
 
@@ -45,6 +60,13 @@ def flops_qr(N):
     the eig total (316/3) N^3 minus Hessenberg (40/3), Q formation (16/3) and
     eigenvectors + back-transform (28/3)."""
     return (316.0 - 40.0 - 16.0 - 28.0) / 3.0 * N ** 3
+
+
+def fourier_bytes(nh, cells):
+    """Algorithmic HBM bytes of the Fourier stage for one layer: the eps cells read once
+    (16 B each) and fe11, fe22, fe33 (n x n complex each) written once."""
+    n = nh * nh
+    return 16.0 * cells + 48.0 * n * n
 
 
 TRAFFIC_SUMMARY = ROOT / "profiles" / "pmc_traffic__hqr_kernel_rect_stack_N882_25inst.md"
@@ -63,9 +85,13 @@ def hqr_traffic_per_matrix():
     return float(m.group(1)) if m else None
 
 
-def load_instances(nh: int, count: int):
+def load_geometry():
+    return json.loads((ROOT / "tests" / "golden" / "metric_geometry.json").read_text())
+
+
+def load_instances(count: int):
     from varrcwa_amd.api import Layer
-    d = json.loads((ROOT / "tests" / "golden" / "metric_geometry.json").read_text())
+    d = load_geometry()
     cell = d["cell"]
     lams = d["wavelengths"]
     geoms = d["geometries"]
@@ -86,15 +112,16 @@ def read_stages(ctx):
     return {s: (ms[i], cnt[i]) for i, s in enumerate(STAGES)}
 
 
-def cpu_baseline(nh: int):
+def oracle_instance0(nh: int):
     """Oracle (restated reference CPU path, numpy/LAPACK) on instance 0 of the sweep:
-    RCWAIntegrator over its 10 slabs (end media, slices, star products, projection)."""
+    RCWAIntegrator over its 10 slabs (end media, slices, star products, projection).
+    Returns (SResult, seconds, threads)."""
     from oracle import rcwa_oracle as O
     try:
         from threadpoolctl import threadpool_limits
     except ImportError:  # pragma: no cover
         threadpool_limits = None
-    d = json.loads((ROOT / "tests" / "golden" / "metric_geometry.json").read_text())
+    d = load_geometry()
     g = d["geometries"][0]
     cell = d["cell"]
     lam = d["wavelengths"][0]
@@ -106,18 +133,40 @@ def cpu_baseline(nh: int):
     ctxm = threadpool_limits(limits=cores) if threadpool_limits else None
     try:
         t0 = time.perf_counter()
-        O.rcwa_integrate(s, ze[0], ze[-1], len(layers), z_edges=ze)
+        ref = O.rcwa_integrate(s, ze[0], ze[-1], len(layers), z_edges=ze)
         t = time.perf_counter() - t0
     finally:
         if ctxm is not None:
             ctxm.__exit__(None, None, None)
-    nsl = len(layers)
-    return dict(value=nsl / t, unit="slice solves/s", cores=cores, kind="port",
-                sample=f"instance 0 of the sweep ({nsl} slabs + 2 end media + projection, N={2 * nh * nh}; "
-                       f"numpy/OpenBLAS LAPACK, {cores} threads, {t:.1f} s)")
+    return ref, t, cores, len(layers)
 
 
-def adjoint_bench(ctx, nh: int, insts, log):
+def parity_instance0(ref, S0, W0, V0, g0, W1, V1, g1, nh):
+    """Relative deviation of the GPU's instance 0 from the oracle: the four basis-invariant
+    transforms (W1 Tuu W0^-1, W0 Tdd W1^-1, W0 Rdu W0^-1, W1 Rud W1^-1; independent of the
+    eigenvectors' order, phase and rotation in degenerate eigenspaces), max |diff| / max |ref|,
+    and the same on the zeroth-order block (the propagating orders of this sweep: the cell is
+    smaller than every wavelength), entrywise relative on the moduli."""
+    from oracle import rcwa_oracle as O
+    from varrcwa_amd._lib import from_dev_colmajor
+    S = from_dev_colmajor(S0)
+    got = O.SResult(S[0], S[1], S[2], S[3], from_dev_colmajor(W0), from_dev_colmajor(V0), g0.cpu().numpy(),
+                    from_dev_colmajor(W1), from_dev_colmajor(V1), g1.cpu().numpy())
+    a, b = O.invariant_transforms(got), O.invariant_transforms(ref)
+    err = max(float(np.abs(a[k] - b[k]).max() / np.abs(b[k]).max()) for k in a)
+    n = nh * nh
+    z = (nh // 2) * nh + nh // 2  # harmonic (0, 0): index m*ny + n with m = n = order
+    idx = np.array([z, n + z])
+    err0 = 0.0
+    for k in a:
+        ga, gb = np.abs(a[k][np.ix_(idx, idx)]), np.abs(b[k][np.ix_(idx, idx)])
+        big = gb > 1e-3 * gb.max()
+        if big.any():
+            err0 = max(err0, float((np.abs(ga - gb)[big] / gb[big]).max()))
+    return err, err0
+
+
+def adjoint_bench(ctx, nh: int, insts):
     """Forward S-matrices with the adjoint tape + the reverse pass for the objective
     sum_i zeroth-order transmission (TE+TM), timed against the plain batched forward
     on the same instances (one launch chunk)."""
@@ -161,12 +210,14 @@ def adjoint_bench(ctx, nh: int, insts, log):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--budget", type=float, default=400.0,
+                    help="wall-clock seconds from process start; the timed loop stops before it would overrun")
     ap.add_argument("--instances", type=int, default=512, help="instances per step (8 geometries x 64 wavelengths)")
     ap.add_argument("--nh", type=int, default=21, help="harmonics per axis (2*order+1)")
     ap.add_argument("--plain", type=int, default=1, help="instances timed through the plain path (0 = skip)")
-    ap.add_argument("--cpu", type=int, default=1, help="time the CPU oracle on one instance (0 = skip)")
+    ap.add_argument("--cpu", type=int, default=1, help="time the CPU oracle on instance 0 and check parity (0 = skip)")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--adjoint", type=int, default=21, help="instances in the adjoint timing (0 = skip)")
     ap.add_argument("--tune", action="append", default=[], help="kernel switch key=value (vrc_tune_set)")
@@ -183,9 +234,7 @@ def main():
     from varrcwa_amd import api
     from varrcwa_amd._lib import Context
 
-    def log(*a):
-        print(*a, file=sys.stderr, flush=True)
-
+    log("torch and the library loaded")
     ctx = Context(0)
     if args.copy_calibration > 0:
         nel = args.copy_calibration // 8
@@ -203,31 +252,89 @@ def main():
     nh = args.nh
     n = nh * nh
     N = 2 * n
-    insts = load_instances(nh, args.instances)
+    insts = load_instances(args.instances)
     L = len(insts[0]["layers"])
-    ctx.check(ctx.lib.vrc_timing_enable(ctx.h, 1), "timing")
+    cells_per_step = sum(int(np.asarray(l.eps).size) for i in insts for l in i["layers"])
 
-    for w in range(args.warmup):
+    # ---- bounded legs first: plain path, CPU oracle (+ parity reference), adjoint
+    plain = None
+    if args.plain > 0:
+        inst = insts[0]
+        ze = np.concatenate([[0.0], np.cumsum(inst["thickness"])])
+        s = api.StackLayerSampler(inst["lam"], nh, nh, inst["layers"], ze, inst["bottom"], inst["top"], ctx=ctx)
+        it = api.RCWAIntegrator(ze[0], ze[-1], s)
+        it.compute_edges(ze)  # warm
+        torch.cuda.synchronize()
+        tp0 = time.perf_counter()
+        for _ in range(args.plain):
+            it.compute_edges(ze)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - tp0
+        plain = args.plain * L / tp
+        log(f"plain: {args.plain * L} slices in {tp:.2f} s -> {plain:.3f} slices/s")
+        del it, s
+
+    ref = cpu = None
+    if args.cpu > 0:
+        ref, tcpu, cores, nsl = oracle_instance0(nh)
+        cpu = dict(value=nsl / tcpu, unit="slice solves/s", cores=cores, kind="port",
+                   sample=f"instance 0 of the sweep ({nsl} slabs + 2 end media + projection, N={N}; "
+                          f"numpy/OpenBLAS LAPACK, {cores} threads, {tcpu:.1f} s)")
+        log(f"cpu: {cpu}")
+
+    adj = None
+    if args.adjoint > 0:
+        adj = adjoint_bench(ctx, nh, insts[:args.adjoint])
+
+    # ---- batched: at most one warm-up step, then timed steps within the budget
+    ctx.check(ctx.lib.vrc_timing_enable(ctx.h, 1), "timing")
+    warm = min(args.warmup, 1)
+    t_step = None
+    for w in range(warm):
+        tw0 = time.perf_counter()
         api.batch_stack_rcwa(ctx, nh, nh, insts, chunk=args.chunk)
         torch.cuda.synchronize()
-        log(f"warmup {w + 1}/{args.warmup} done")
+        t_step = time.perf_counter() - tw0
+        log(f"warmup {w + 1}/{warm} done ({t_step:.1f} s)")
     read_stages(ctx)  # reset
     qs = (C.c_longlong * 6)()
     ctx.check(ctx.lib.vrc_qr_stats(ctx.h, qs), "qr_stats")  # reset
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    steps = 0
+    out = None
+    while steps < max(1, args.steps):
+        if steps >= 1:
+            est = (time.perf_counter() - t0) / steps
+            if time.perf_counter() - T_START + 1.1 * est > args.budget:
+                break
+        out = None  # the previous step's S-matrices are released before the next step allocates
         out = api.batch_stack_rcwa(ctx, nh, nh, insts, chunk=args.chunk)
-        del out
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        steps += 1
+        log(f"timed step {steps} done ({time.perf_counter() - t0:.1f} s so far)")
     t1 = time.perf_counter()
     st = read_stages(ctx)
     ctx.check(ctx.lib.vrc_qr_stats(ctx.h, qs), "qr_stats")  # the timed steps only
     free_b, total_b = torch.cuda.mem_get_info()
     dt = t1 - t0
-    slices = args.steps * len(insts) * L
+    slices = steps * len(insts) * L
     value = slices / dt
-    log(f"batched: {slices} slices in {dt:.2f} s -> {value:.2f} slices/s")
+    log(f"batched: {slices} slices in {dt:.2f} s ({steps} steps) -> {value:.2f} slices/s")
+
+    parity = None
+    if ref is not None:
+        S0 = out["S"][0].clone()
+        del out
+        b = api.batch_stack_rcwa(ctx, nh, nh, insts[:1], want_bases=True)
+        err, err0 = parity_instance0(ref, S0, b["W0"][0], b["V0"][0], b["gam0"][0], b["W1"][0], b["V1"][0],
+                                     b["gam1"][0], nh)
+        parity = {"max_rel_err_vs_oracle": err, "zeroth_order_rel_err_vs_oracle": err0,
+                  "instance": 0, "measure": "basis-invariant transforms W1 Tuu W0^-1, W0 Tdd W1^-1, W0 Rdu W0^-1, "
+                                            "W1 Rud W1^-1 of the last timed step's instance 0; max|diff|/max|ref|"}
+        log(f"parity: {parity['max_rel_err_vs_oracle']:.2e} (zeroth order {err0:.2e})")
+    else:
+        del out
 
     # dominant stage (stage 1 includes stage 0; report exclusive times)
     excl = {k: v[0] for k, v in st.items()}
@@ -248,50 +355,38 @@ def main():
                 "traffic": tpm * mats_per_launch if tpm else None,
                 "traffic_source": TRAFFIC_SUMMARY.relative_to(ROOT).as_posix() if tpm else None,
                 "avg_launch_ms": avg_ms, "flops_per_launch": per_launch, "matrices_per_launch": mats_per_launch}
-
-    plain = None
-    if args.plain > 0:
-        inst = insts[0]
-        ze = np.concatenate([[0.0], np.cumsum(inst["thickness"])])
-        s = api.StackLayerSampler(inst["lam"], nh, nh, inst["layers"], ze, inst["bottom"], inst["top"], ctx=ctx)
-        it = api.RCWAIntegrator(ze[0], ze[-1], s)
-        torch.cuda.synchronize()
-        tp0 = time.perf_counter()
-        for _ in range(args.plain):
-            it.compute_edges(ze)
-        torch.cuda.synchronize()
-        tp = time.perf_counter() - tp0
-        plain = args.plain * L / tp
-        read_stages(ctx)
-        log(f"plain: {args.plain * L} slices in {tp:.2f} s -> {plain:.3f} slices/s")
-        del it, s
-
-    adj = None
-    if args.adjoint > 0:
-        adj = adjoint_bench(ctx, nh, insts[:args.adjoint], log)
-        read_stages(ctx)
-
-    cpu = None
-    if args.cpu > 0:
-        cpu = cpu_baseline(nh)
-        log(f"cpu: {cpu}")
+    fourier = None
+    if st["fourier"][1]:
+        fb = fourier_bytes(nh, cells_per_step / (len(insts) * L)) * len(insts) * L * steps
+        fourier = {"bound": "hbm", "achieved_gbs": fb / (st["fourier"][0] * 1e-3) / 1e9, "peak_gbs": HBM_PEAK_GBS,
+                   "frac": fb / (st["fourier"][0] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                   "bytes_per_layer": fb / (len(insts) * L * steps),
+                   "ms_per_step": st["fourier"][0] / steps,
+                   "definition": "16 B per eps cell read + 48 n^2 B (fe11, fe22, fe33) written per layer, "
+                                 "over the Fourier stage's HIP-event time"}
 
     frac = flops_slice(N) * value / (FP64_PEAK_TFLOPS * 1e12)
     line = {
-        "metric": METRIC, "value": value, "unit": "slice solves/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": value, "unit": "slice solves/s", "n_gpus": 1, "steps": steps,
+        "steps_requested": args.steps, "warmup": warm, "warmup_requested": args.warmup,
+        "ms_per_step": dt / steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "c128", "data": "synthetic",
         "config": {"workload": f"rectangle stacks, {nh}x{nh} harmonics (N={N}), {L} slabs/instance, "
                                f"{len(insts)} instances/step (8 seeded geometries x 64 wavelengths in 0.8-1.6 um)",
                    "instances_per_step": len(insts), "slices_per_step": len(insts) * L, "N": N,
-                   "parallelism": "single GPU", "tuning": args.tune},
-        "fp64_matrix_frac": frac,
-        "path_tflops": flops_slice(N) * value / 1e12,
+                   "parallelism": "single GPU", "tuning": args.tune, "budget_s": args.budget},
+        "fp64_matrix_frac_reference_equivalent": frac,
+        "reference_equivalent_tflops": flops_slice(N) * value / 1e12,
+        "reference_equivalent_definition": "the reference operation sequence's 305 N^3 flops per slice times the "
+                                           "measured rate; this build does fewer flops per slice (closed-form end "
+                                           "media, even/odd slab systems, factored star product)",
         "plain_slices_per_s": plain,
         "batched_over_plain": (value / plain) if plain else None,
         "batched_over_cpu": (value / cpu["value"]) if cpu else None,
+        "parity": parity,
         "adjoint": adj,
-        "stage_ms_per_step": {k: v / args.steps for k, v in excl.items()},
+        "fourier_stage": fourier,
+        "stage_ms_per_step": {k: v / steps for k, v in excl.items()},
         "device_memory_in_use_gb": (total_b - free_b) / 1e9,
         "dominant_stage": dom,
         "qr_stats": {"matrices": qs[0], "sweeps_per_matrix": qs[1] / max(qs[0], 1),
@@ -301,8 +396,9 @@ def main():
                      "small_block_ms_per_matrix": qs[5] / max(qs[0], 1) / 1e5},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "wall_s": time.perf_counter() - T_START,
     }
-    print(json.dumps(line))
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

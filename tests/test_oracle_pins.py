@@ -8,12 +8,14 @@ mistake (dropped term, wrong sign or index, transposed operand) fails.
 Pinned: fourier1d/fourier1d_vec (quadrature + DFT closed form), refill (painter's
 order), toeplitz_dense/block_toeplitz_dense (convolution identities), fourier2d
 (homogeneous, single-strip inverse rules, Laurent = separable product), k_matrices
-periodic (closed form) and PML (gamma=1 stretch reduces to the unstretched cell),
-wave_eqn_coeff (homogeneous closed form), wave_equation_solve (eigen residual,
+periodic (closed form) and PML (zero-width limit; non-zero widths against quadrature of the
+stretch profile), pml_solve (quadrature of the stretch profile), svd_solve (minimum-norm
+least squares with the rank threshold), wave_eqn_coeff (homogeneous closed form), wave_equation_solve (eigen residual,
 homogeneous eigenvalues), gamma_sqrt (branch table), rcwa_scatter_matrix
 (homogeneous slab), star_product (identity, associativity), rcwa_integrate
 (z-invariance, energy conservation), redheffer_integrate (single segment for a
-z-invariant structure, convergence to RCWA), differential_integrate (convergence),
+z-invariant structure, convergence to RCWA), differential_integrate (convergence, and
+fourth order: successive-difference ratio 16),
 gdsii_real (known answers), read_gds_polygons / GDSIISampler (written files with known
 coordinates, the reference's layout file),
 scatter_plane_wave (Airy slab, power balance in equal and unequal media), evaluate_energy
@@ -22,9 +24,10 @@ legacy RTCM recursion against the star-product cascade, homogeneous closed form,
 translation = shifted geometry), dsolve_1d /
 dsolve_v2_1d / dsolve_para / fourier2d_tangents / wave_eqn_derivatives (central differences
 of the forward maps, monotone coordinates).
-Parity unpinned: svd_solve rank threshold for singular fe33 (never exercised); the
-derivative formulas for crossing coordinates (the reference's own is unfinished there, and
-the library refuses that case).
+Each pin added for PML, the differential method and svd_solve is also run on planted
+mistakes (the oracle's own source with one fragment changed), which it must reject.
+Parity unpinned: the derivative formulas for crossing coordinates (the reference's own is
+unfinished there, and the library refuses that case).
 """
 import math
 
@@ -600,3 +603,166 @@ def test_convergence_rcwa_varrcwa_differential():
     ed = [np.abs(O.invariant_transforms(O.differential_integrate(s, 0.0, 2.0, N))["Tuu"] - ref).max()
           for N in (32, 128)]
     assert ed[1] < ed[0]
+
+
+# ---------------------------------------------------------------------------
+# Planted mistakes: a pin is only as good as the slips it catches.  planted()
+# rewrites one fragment of an oracle function's own source text and returns the
+# mutated function, so every pin below is shown to fail on a plausible mistake.
+# ---------------------------------------------------------------------------
+
+def planted(func, old, new):
+    import inspect
+    import textwrap
+    src = textwrap.dedent(inspect.getsource(func))
+    assert old in src, f"planted fragment {old!r} not found in {func.__name__}"
+    ns = dict(vars(O))
+    exec(compile(src.replace(old, new), f"<planted {func.__name__}>", "exec"), ns)
+    return ns[func.__name__]
+
+
+def pml_profile(u, coeff, gamma):
+    """Stretch profile whose Fourier coefficients PMLSolver1D::solve (src/rcwa/PMLSolver1D.cpp:15-30)
+    returns: sin^2(pi u / 2) - (gamma / 4) sin^2(pi u) on the side whose outer end is u = 0
+    (coeff = +1) and cos^2(pi u / 2) - (gamma / 4) sin^2(pi u) on the mirrored side
+    (coeff = -1), u the position across the absorbing region.  Both equal 1 where they meet the
+    interior and vanish at the outer end."""
+    base = np.sin(0.5 * np.pi * u) ** 2 if coeff > 0 else np.cos(0.5 * np.pi * u) ** 2
+    return base - 0.25 * gamma * np.sin(np.pi * u) ** 2
+
+
+def quad_profile(b0, b1, L, n, fn, npts=200000):
+    """(1/L) * integral over [b0, b1] of fn(u(x)) exp(-2 pi i n x / L), midpoint rule."""
+    t = (np.arange(npts) + 0.5) / npts
+    x = b0 + t * (b1 - b0)
+    return np.sum(fn(t) * np.exp(-2j * np.pi * n * x / L)) * (b1 - b0) / npts / L
+
+
+PML_GAMMA = 1.0 / complex(1.0, -1.0)  # src/core/LayerSampler.cpp:121-122
+
+
+def pml_solve_error(fn):
+    """Largest deviation of fn (pml_solve's signature) from quadrature of the stretch profile,
+    over both sides, several region placements and orders."""
+    worst = 0.0
+    for (b0, b1, L) in [(0.0, 0.5, 2.5), (0.3, 0.9, 3.1), (2.0, 2.5, 2.5)]:
+        for coeff in (1.0, -1.0):
+            for n in range(-6, 7):
+                ref = quad_profile(b0, b1, L, n, lambda u: pml_profile(u, coeff, PML_GAMMA))
+                worst = max(worst, abs(fn(b0, b1, L, PML_GAMMA, n, coeff) - ref))
+    return worst
+
+
+def test_pml_solve_matches_stretch_profile_quadrature():
+    assert pml_solve_error(O.pml_solve) < 1e-10
+    # planted mistakes: conjugated stretch, flipped side, negated result, dropped term
+    for old, new in [("r += gamma * sinc(n * bm - 1.0)", "r += np.conj(gamma) * sinc(n * bm - 1.0)"),
+                     ("np.exp(1j * PI * (-(n * bp + coeff * 0.5)))", "np.exp(1j * PI * (-(n * bp - coeff * 0.5)))"),
+                     ("return r * (bm / 16.0)", "return -r * (bm / 16.0)"),
+                     ("r += -2.0 * gamma", "r += -0.0 * gamma"),
+                     ("bp = (b1 + b0) / L", "bp = (b1 - b0) / L")]:
+        assert pml_solve_error(planted(O.pml_solve, old, new)) > 1e-4, (old, new)
+
+
+def kmat_from_quadrature(nx, ny, x, y):
+    """Kx, Ky of a PML cell from the stretched-coordinate profile: s(x) = 1 inside, the
+    stretch profiles over [x0, x1] and [x_-2, x_-1]; Kx[(m,n),(j,l)] = s_hat(m-j) delta(n,l)
+    alpha_(j,l) (harmonic index m*ny+n), Ky likewise along y (src/core/LayerSampler.cpp:34-132)."""
+    def coeffs(c, nh):
+        L = c[-1] - c[0]
+        out = {}
+        for m in range(-(nh - 1), nh):
+            v = quad_profile(c[0], c[1], L, m, lambda u: pml_profile(u, 1.0, PML_GAMMA))
+            v += quad_profile(c[1], c[-2], L, m, lambda u: np.ones_like(u))
+            v += quad_profile(c[-2], c[-1], L, m, lambda u: pml_profile(u, -1.0, PML_GAMMA))
+            out[m] = v
+        return out
+    fx, fy = coeffs(x, nx), coeffs(y, ny)
+    Tx = np.array([[fx[i - j] for j in range(nx)] for i in range(nx)])
+    Ty = np.array([[fy[i - j] for j in range(ny)] for i in range(ny)])
+    a, b = O.alpha_beta(nx, ny, x[-1] - x[0], y[-1] - y[0])
+    return np.kron(Tx, np.eye(ny)) * a[None, :], np.kron(np.eye(nx), Ty) * b[None, :]
+
+
+def test_pml_k_matrices_match_quadrature():
+    nx, ny = 5, 3
+    x = np.array([0.3, 0.8, 2.0, 2.4, 2.8])  # PML widths 0.5 and 0.4, cell offset from 0
+    y = np.array([-0.2, 0.4, 1.5, 2.0])
+    reg = dict(Lx=x[-1] - x[0], Ly=y[-1] - y[0], b0=y[0], b1=y[1], u0=y[-2], u1=y[-1],
+               l0=x[0], l1=x[1], r0=x[-2], r1=x[-1])
+    Kxr, Kyr = kmat_from_quadrature(nx, ny, x, y)
+    scale = max(np.abs(Kxr).max(), np.abs(Kyr).max())
+
+    def err(fn):
+        Kx, Ky = fn(nx, ny, reg, True)
+        return max(np.abs(Kx - Kxr).max(), np.abs(Ky - Kyr).max()) / scale
+
+    assert err(O.k_matrices) < 1e-9
+    # the absorbing regions really act: far from the unstretched diag(alpha)
+    a, _ = O.alpha_beta(nx, ny, reg["Lx"], reg["Ly"])
+    assert np.abs(Kxr - np.diag(a)).max() > 0.05 * scale
+    for old, new in [("gamma = 1.0 / complex(1.0, -1.0)", "gamma = 1.0 / complex(1.0, 1.0)"),
+                     ("fxx[i] += pml_solve(l0, l1, Lx, gamma, m, 1.0)", "fxx[i] += pml_solve(l0, l1, Lx, gamma, m, -1.0)"),
+                     ("return Fx * alpha[None, :], Fy * beta[None, :]", "return alpha[:, None] * Fx, beta[:, None] * Fy"),
+                     ("fyy[i] += pml_solve(u0, u1, Ly, gamma, n, -1.0)", "fyy[i] += pml_solve(u0, u1, Lx, gamma, n, -1.0)"),
+                     ("/ np.exp(1j * (PI * m * (l1 + r0) / Lx)))", "* np.exp(1j * (PI * m * (l1 + r0) / Lx)))")]:
+        assert err(planted(O.k_matrices, old, new)) > 1e-4, (old, new)
+
+
+def differential_order_ratios(fn):
+    """Successive-difference ratios |D(N) - D(2N)| / |D(2N) - D(4N)| of the differential method's
+    W1 Tuu W0^-1 on a smoothly tapering pillar (z in [1, 2] of the metasurface rule); a method of
+    order p gives 2^p."""
+    g = G.metasurface(1)
+    s = O.VarLayerSampler(g["lam"], g["nx"], g["ny"], g["x"], g["y"], g["eps"], False)
+    s.addXRule(2, g["rules"][2])
+    D = {N: O.invariant_transforms(fn(s, 1.0, 2.0, N))["Tuu"] for N in (8, 16, 32, 64)}
+    return [np.abs(D[a] - D[2 * a]).max() / np.abs(D[2 * a] - D[4 * a]).max() for a in (8, 16)]
+
+
+def test_differential_integrate_is_fourth_order():
+    """DifferentialIntegrator's step (src/core/DifferentialIntegrator.cpp:58-65) is a 4th-order
+    Magnus-type update: halving dz divides the error by 16.  Dropping its correction terms
+    leaves a 2nd-order method (ratio 4)."""
+    r = differential_order_ratios(O.differential_integrate)
+    assert all(12.0 < x < 20.0 for x in r), r
+    for old, new in [(" - b * (P2 @ Q1 @ P1 @ Q0))", ")"),
+                     (" - c * (P1 @ Q1 @ P0 + P2 @ Q1 @ P1))", ")"),
+                     ("(P0 + 4.0 * P1 + P2", "(P0 + 2.0 * P1 + P2"),
+                     ("rdu = -np.linalg.solve(T22, T21)", "rdu = np.linalg.solve(T22, T21)")]:
+        bad = differential_order_ratios(planted(O.differential_integrate, old, new))
+        assert not all(12.0 < x < 20.0 for x in bad), (old, new, bad)
+
+
+def test_svd_solve_least_squares_minimum_norm():
+    """Eigen BDCSVD(A).solve(B) (src/rcwa/Layer.cpp:84-86): the minimum-norm least-squares
+    solution over the singular values at or above max(rows, cols) * eps * s_max
+    (Eigen SVDBase::rank and _solve_impl, Eigen >= 3.3)."""
+    rng = np.random.default_rng(3)
+    n = 12
+    U, _ = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    V, _ = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    B = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    # non-singular: equals the LU solve
+    s = np.linspace(1.0, 3.0, n)
+    A = (U * s) @ V.conj().T
+    assert np.abs(O.svd_solve(A, B) - np.linalg.solve(A, B)).max() < 1e-12
+    # rank-deficient, with one singular value below the threshold (dropped) and one above (kept)
+    s = np.concatenate([np.linspace(1.0, 3.0, n - 3), [1e-9, 1e-20, 0.0]])[::-1]
+    s = np.sort(s)[::-1]
+    A = (U * s) @ V.conj().T
+
+    def check(fn):
+        x = fn(A, B)
+        keep = s >= n * np.finfo(float).eps * s[0]
+        # x lies in the span of the kept right singular vectors, with the exact coefficients
+        c = V.conj().T @ x
+        want = (U.conj().T @ B)[keep] / s[keep, None]
+        return max(np.abs(c[~keep]).max() / np.abs(want).max(), np.abs(c[keep] - want).max() / np.abs(want).max())
+
+    assert check(O.svd_solve) < 1e-5  # the kept 1e-9 direction amplifies rounding by 1e9
+    for old, new in [("r = int(np.sum(s > thr))", "r = int(np.sum(s > 0))"),
+                     ("return Vh[:r, :].conj().T @ tmp", "return Vh[:r, :].T @ tmp"),
+                     ("thr = max(A.shape) * np.finfo(np.float64).eps * (s[0] if s.size else 0.0)",
+                      "thr = 1e-8 * (s[0] if s.size else 0.0)")]:
+        assert not check(planted(O.svd_solve, old, new)) < 1e-5, (old, new)
