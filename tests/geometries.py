@@ -117,3 +117,63 @@ def rects_to_layer(rects, cell):
 
 def homogeneous(cell, eps):
     return np.array([0.0, cell]), np.array([0.0, cell]), np.array([[eps]], dtype=complex)
+
+
+def pillar_taper_c1(grid=256, nh=11, nlayers=20):
+    """Staircased tapered pillar on a substrate (plain RCWA, CPU-runnable case): square cell
+    of period 0.5 um, lambda = 1.0 um, circular pillar of eps = 12.25 in air whose radius
+    tapers linearly from 0.10 to 0.20 um over 1.0 um height, staircased into `nlayers`
+    layers (radius at each layer's mid-height), each a grid x grid pixel map (pixel centres
+    inside the circle); substrate eps = 2.1 below, air above; nh x nh harmonics."""
+    cell, h = 0.5, 1.0
+    x = np.linspace(0.0, cell, grid + 1)
+    c = 0.5 * (x[1:] + x[:-1]) - 0.5 * cell
+    r2 = c[None, :] ** 2 + c[:, None] ** 2
+    layers, thick = [], []
+    for k in range(nlayers):
+        r = 0.10 + 0.10 * (k + 0.5) / nlayers
+        eps = np.where(r2 < r * r, 12.25, 1.0).astype(complex)
+        layers.append((x, x.copy(), eps))
+        thick.append(h / nlayers)
+    return dict(lam=1.0, nx=nh, ny=nh, cell=cell, layers=layers, thickness=thick,
+                bottom=homogeneous(cell, 2.1), top=homogeneous(cell, 1.0))
+
+
+def mode_converter_c2(nh=15):
+    """PML waveguide mode converter (varRCWA case): a rectangular core of eps = 12.1 and
+    height 0.22 um in eps = 2.1 cladding, computational window 4 x 2 um with 0.5 um
+    complex-stretched PML on every side, lambda = 1.55 um, nh x nh harmonics; over
+    z in [0, 10] um the core width w(z) = 0.4 + 0.8 (3 s^2 - 2 s^3), s = z / 10 (cubic
+    profile, centred at x = 2 um), constant before and after."""
+    x = np.array([0.0, 0.5, 1.8, 2.2, 3.5, 4.0])
+    y = np.array([0.0, 0.5, 0.89, 1.11, 1.5, 2.0])
+    cl, co = 2.1, 12.1
+    eps = np.full((5, 5), cl, dtype=complex)
+    eps[2, 2] = co
+
+    def width(z):
+        s = min(max(z / 10.0, 0.0), 1.0)
+        return 0.4 + 0.8 * (3 * s * s - 2 * s * s * s)
+
+    def left(z):
+        return 2.0 - 0.5 * width(z)
+
+    def right(z):
+        return 2.0 + 0.5 * width(z)
+
+    return dict(lam=1.55, nx=nh, ny=nh, x=x, y=y, eps=eps, pml=True, rules={2: left, 3: right},
+                z0=0.0, z1=10.0)
+
+
+def stack_oracle_sampler(g):
+    from oracle import rcwa_oracle as O
+    ze = np.concatenate([[0.0], np.cumsum(g["thickness"])])
+    return O.StackLayerSampler(g["lam"], g["nx"], g["ny"], g["layers"], ze, g["bottom"], g["top"]), ze
+
+
+def stack_gpu_sampler(g, ctx=None):
+    from varrcwa_amd.api import Layer, StackLayerSampler
+    ze = np.concatenate([[0.0], np.cumsum(g["thickness"])])
+    layers = [Layer(*l) for l in g["layers"]]
+    return StackLayerSampler(g["lam"], g["nx"], g["ny"], layers, ze, Layer(*g["bottom"]), Layer(*g["top"]),
+                             ctx=ctx), ze
