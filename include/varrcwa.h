@@ -10,7 +10,17 @@
  *    unless documented (results read back to _host buffers synchronize).
  *  - status: 0 ok, -1 bad argument, -2 HIP runtime failure, -3 other failure;
  *    vrc_last_error() gives the text. Per-matrix numerical failures are reported
- *    LAPACK-style through info arrays (0 ok, >0 failing index).
+ *    LAPACK-style through info arrays (0 ok, >0 failing index) by the dense kernels;
+ *    the calls that solve whole problems (integrator computes, batched stacks, the
+ *    legacy solver) read them back once at the end and return a positive status on a
+ *    numerical failure, with the results written as computed:
+ *      1 the eigensolver did not converge, 2 an LU met an exactly zero pivot,
+ *      3 a zero propagation constant (Rayleigh/Wood anomaly: the reference divides by
+ *        gamma in V = Q W / gamma, src/rcwa/WaveEquationSolver.cpp:36),
+ *      4 non-finite results, 5 varRCWA reached its subdivision depth limit (40 levels of
+ *        trisection) with the segment error still above pe.
+ *    Non-finite or zero permittivity values are bad arguments (-1): the inverse Fourier
+ *    rules need 1/eps of every cell.
  *  - one context per host thread; calls on one context are not re-entrant.
  *
  * Harmonic ordering follows the reference: index m*ny + n (x order outer), field
@@ -54,7 +64,11 @@ int vrc_timing_enable(vrc_ctx* ctx, int on);
  * 128x64 and 64x64 tiles; "zgemm.big_n" (default 128, or 64): a 128x128 tile for n >= 128; "zgeev.aed_window" (default 24, 0 = off, at most 64): deflation-window
  * size of the aggressive early deflation in the QR iteration; "zgeev.bulges" (default 12,
  * 1..16): double-shift bulges chased per QR sweep; "stack.cascade_group" (default 128):
- * instances of vrc_batch_stack_rcwa whose star products and projections run in one batch.
+ * instances of vrc_batch_stack_rcwa whose star products and projections run in one batch;
+ * "zgeev.max_sweeps" (default 0 = 30 * max(N, 10)): QR sweeps per matrix before the
+ * eigensolver gives up and reports info > 0; "redheffer.max_depth" (default 40, 0..60):
+ * trisection depth at which vrc_redheffer_compute accepts a segment regardless of pe
+ * (status 5).
  * Returns -1 for an unknown key. */
 int vrc_tune_set(const char* key, long long value);
 int vrc_timing_read(vrc_ctx* ctx, double* ms, int* counts, int nstages);
@@ -180,6 +194,28 @@ int vrc_var_sampler_create(vrc_ctx* ctx, double lambda, int nx, int ny, const vr
                            vrc_sampler** out);
 /* axis 0 = x (addXRule), 1 = y (addYRule). rule(z, user) returns the coordinate. */
 int vrc_sampler_add_rule(vrc_sampler* s, int axis, int layout, double (*rule)(double z, void* user), void* user);
+/* The simulation region of a sampler: LayerSampler's Lx_, Ly_ (periods) and the absorbing
+ * regions [b0, b1], [u0, u1] along y and [l0, l1], [r0, r1] along x that a subclass's
+ * assignSimulationRegion() sets (src/core/LayerSampler.h:7-28, src/core/LayerSampler.h:57-62). */
+typedef struct vrc_region {
+  double Lx, Ly, b0, b1, u0, u1, l0, l1, r0, r1;
+} vrc_region;
+/* The sampler plugin boundary: a user subclass of the abstract LayerSampler
+ * (src/core/LayerSampler.h:5-77) overriding sample(z, P, Q) and/or sampleOnGPU(z, P, Q).
+ * host_fn(z, P_host, Q_host, user) fills host N x N column-major complex P and Q (N = 2 nx ny);
+ * device_fn(z, P, Q, stream, user) fills device N x N buffers, enqueuing its work on `stream`
+ * (the context's hipStream_t); it may return 1 to defer to host_fn. Any other non-zero return
+ * is a failure (status -3). Either function may be NULL, not both; device_fn is preferred when
+ * the integrators sample. Kx, Ky come from region and enable_pml as
+ * LayerSampler::postInitialization computes them (src/core/LayerSampler.cpp:145-153). The
+ * callbacks run on the calling thread, inside the integrator calls. */
+typedef int (*vrc_sample_host_fn)(double z, double* P_host, double* Q_host, void* user);
+typedef int (*vrc_sample_device_fn)(double z, void* P, void* Q, void* stream, void* user);
+int vrc_callback_sampler_create(vrc_ctx* ctx, double lambda, int nx, int ny, const vrc_region* region,
+                                int enable_pml, vrc_sample_host_fn host_fn, vrc_sample_device_fn device_fn,
+                                void* user, vrc_sampler** out);
+/* LayerSampler::Lx(), Ly() and the region scalars of any sampler (src/core/LayerSampler.h:42-43). */
+int vrc_sampler_region(const vrc_sampler* s, vrc_region* out);
 /* Piecewise-constant stack (no reference counterpart; extension): layer k occupies
  * (z_edges[k], z_edges[k+1]); bottom at z <= z_edges[0], top at z >= z_edges[nlayers]. */
 int vrc_stack_sampler_create(vrc_ctx* ctx, double lambda, int nx, int ny, int nlayers, const vrc_layer* layers,
@@ -254,7 +290,10 @@ int vrc_rcwa_compute_batched(vrc_integrator* it, int N);
 /* Extension: explicit slice edges (midpoint sampling, per-slice width). */
 int vrc_rcwa_compute_edges(vrc_integrator* it, int nedges, const double* z_edges);
 /* RedhefferIntegrator::compute(p, maxN, pe) (src/core/RedhefferIntegrator.cpp:174-224);
- * p and maxN are accepted and unused, as in the reference. */
+ * p and maxN are accepted and unused, as in the reference. The reference recurses until
+ * every segment meets pe; here trisection stops at depth 40 (segments of 3^-40 of the
+ * interval; "redheffer.max_depth" of vrc_tune_set), accepts the segment and returns
+ * status 5. */
 int vrc_redheffer_compute(vrc_integrator* it, int p, int maxN, double pe);
 /* DifferentialIntegrator::compute(N) (src/core/DifferentialIntegrator.cpp:21-134). */
 int vrc_differential_compute(vrc_integrator* it, int N);
@@ -277,7 +316,11 @@ int vrc_integrator_tree(vrc_integrator* it, int cap, double* host_out);
  * of at most `chunk` (<= 0: as many as keep one launch within 256 eigensolves), split
  * evenly over the chunks; with single-valued half-spaces (and no tape) the slab eigensolves
  * of each cascade group go in even launches of at most chunk * nlayers (<= 0: 256) slabs,
- * across instance boundaries. ninst = 0 does nothing (S_out may then be NULL). */
+ * across instance boundaries. ninst = 0 does nothing (S_out may then be NULL).
+ * info_out (host, ninst ints, or NULL) receives per instance 0, 1 (an eigensolve of the
+ * instance did not converge) or 4 (its S-matrix holds NaN or infinity, e.g. a zero
+ * propagation constant or a singular system); the status is then the first flagged
+ * instance's code, and vrc_last_error names that instance. */
 typedef struct vrc_stack_instance {
   double lambda;
   int nlayers;
@@ -287,7 +330,8 @@ typedef struct vrc_stack_instance {
   const vrc_layer* top;
 } vrc_stack_instance;
 int vrc_batch_stack_rcwa(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
-                         void* S_out, void* W0, void* V0, void* gam0, void* W1, void* V1, void* gam1);
+                         void* S_out, void* W0, void* V0, void* gam0, void* W1, void* V1, void* gam1,
+                         int* info_out);
 /* Plane-wave diffraction efficiencies of solved stacks; replaces the power bookkeeping of
  * RCWASolver::scatterPlaneWave (src/rcwa/RCWASolver.cpp:1636-1733) for the integrator's
  * S-matrix. S, W0, W1: device outputs of vrc_batch_stack_rcwa for the same instances;
@@ -336,7 +380,7 @@ int vrc_find_guided_modes(vrc_ctx* ctx, int n, const void* gamma, double k0, dou
 typedef struct vrc_tape vrc_tape;
 int vrc_batch_stack_forward(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
                             void* S_out, void* W0, void* V0, void* gam0, void* W1, void* V1, void* gam1,
-                            vrc_tape** tape);
+                            vrc_tape** tape, int* info_out);
 long long vrc_tape_cells(const vrc_tape* tape);
 int vrc_batch_stack_vjp(vrc_tape* tape, const void* S_bar, void* eps_bar, int chunk);
 int vrc_tape_destroy(vrc_tape* tape);

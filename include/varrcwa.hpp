@@ -15,6 +15,7 @@
 // device's default stream unless a Context is passed explicitly.
 #pragma once
 
+#include <algorithm>
 #include <complex>
 #include <functional>
 #include <memory>
@@ -108,7 +109,13 @@ struct LayerDesc {
   }
 };
 
-// LayerSampler (src/core/LayerSampler.h:5-77): common part of the geometric samplers.
+// LayerSampler (src/core/LayerSampler.h:5-77). The library's samplers (VarLayerSampler,
+// StackLayerSampler, GDSIISampler below) derive from it; so can a caller's own sampler, as
+// with the reference: derive, override sample() (and, optionally, sampleOnGPU()) and
+// assignSimulationRegion() -- which sets Lx_, Ly_, b0_ .. r1_ -- and call
+// postInitialization() at the end of the constructor. The integrators then call back into
+// the subclass for every P, Q they need (sampleOnGPU when overridden, else sample() and a
+// host-to-device copy).
 class LayerSampler {
  public:
   virtual ~LayerSampler() {
@@ -119,20 +126,28 @@ class LayerSampler {
 
   scalar lambda() const { return lambda_; }
   scalar k0() const { return 2 * 3.1415926535897932 / lambda_; }
-  int nDim() const { return vrc_sampler_ndim(s_); }
+  scalar Lx() const { return Lx_; }
+  scalar Ly() const { return Ly_; }
+  int nDim() const { return 2 * nx_ * ny_; }
   const MatrixXcs& Kx() const { return Kx_; }
   const MatrixXcs& Ky() const { return Ky_; }
 
   // LayerSampler::sample (src/core/LayerSampler.h:49-51): host P, Q (resized).
-  void sample(scalar z, MatrixXcs& P, MatrixXcs& Q) {
+  virtual void sample(scalar z, MatrixXcs& P, MatrixXcs& Q) {
+    if (user_) throw Error("LayerSampler subclass must override sample()", -1);
     const int N = nDim();
     P.resize(N, N);
     Q.resize(N, N);
     detail::check(vrc_sampler_sample(s_, z, reinterpret_cast<double*>(P.data()), reinterpret_cast<double*>(Q.data())),
                   ctx_->get(), "sample");
   }
-  // LayerSampler::sampleOnGPU (src/core/LayerSampler.h:53-55): device N x N buffers.
-  void sampleOnGPU(scalar z, void* P, void* Q) {
+  // LayerSampler::sampleOnGPU (src/core/LayerSampler.h:53-55): device N x N buffers, work on
+  // the context's stream. A subclass that does not override it is sampled on the host.
+  virtual void sampleOnGPU(scalar z, void* P, void* Q) {
+    if (user_) {
+      gpu_default_ = true;
+      return;
+    }
     detail::check(vrc_sampler_sample_gpu(s_, z, P, Q), ctx_->get(), "sampleOnGPU");
   }
 
@@ -140,19 +155,78 @@ class LayerSampler {
   const std::shared_ptr<Context>& context() const { return ctx_; }
 
  protected:
+  // A caller's sampler (src/core/LayerSampler.h:31-37).
+  LayerSampler(scalar lambda, int nx, int ny, bool enablePML = true, std::shared_ptr<Context> ctx = nullptr)
+      : lambda_(lambda), ctx_(ctx ? std::move(ctx) : Context::thread_default()), nx_(nx), ny_(ny),
+        enablePML_(enablePML), user_(true) {}
+  // Put this at the end of the subclass constructor (src/core/LayerSampler.cpp:145-153).
+  void postInitialization() {
+    assignSimulationRegion();
+    const vrc_region r{Lx_, Ly_, b0_, b1_, u0_, u1_, l0_, l1_, r0_, r1_};
+    vrc_sampler* s = nullptr;
+    detail::check(vrc_callback_sampler_create(ctx_->get(), lambda_, nx_, ny_, &r, enablePML_ ? 1 : 0, &host_cb,
+                                              &device_cb, this, &s),
+                  ctx_->get(), "postInitialization");
+    finish(s, nx_, ny_);
+  }
+  // Assign Lx_, Ly_, b0_, b1_, u0_, u1_, l0_, l1_, r0_, r1_ (src/core/LayerSampler.h:57-62).
+  virtual void assignSimulationRegion() {}
+
+  // The library's own samplers.
   LayerSampler(scalar lambda, std::shared_ptr<Context> ctx)
       : lambda_(lambda), ctx_(ctx ? std::move(ctx) : Context::thread_default()) {}
-  void finish(vrc_sampler* s, int n) {
+  void finish(vrc_sampler* s, int nx, int ny) {
     s_ = s;
+    nx_ = nx;
+    ny_ = ny;
+    const int n = nx * ny;
     Kx_.resize(n, n);
     Ky_.resize(n, n);
     detail::check(vrc_sampler_k(s_, reinterpret_cast<double*>(Kx_.data()), reinterpret_cast<double*>(Ky_.data())),
                   ctx_->get(), "Kx/Ky");
+    vrc_region r{};
+    detail::check(vrc_sampler_region(s_, &r), ctx_->get(), "region");
+    Lx_ = r.Lx, Ly_ = r.Ly, b0_ = r.b0, b1_ = r.b1, u0_ = r.u0, u1_ = r.u1, l0_ = r.l0, l1_ = r.l1, r0_ = r.r0,
+    r1_ = r.r1;
   }
+
   scalar lambda_;
   std::shared_ptr<Context> ctx_;
   vrc_sampler* s_ = nullptr;
   MatrixXcs Kx_, Ky_;
+  int nx_ = 0, ny_ = 0;
+  bool enablePML_ = true;
+  scalar Lx_ = 0, Ly_ = 0, b0_ = 0, b1_ = 0, u0_ = 0, u1_ = 0, l0_ = 0, l1_ = 0, r0_ = 0, r1_ = 0;
+
+ private:
+  // C-ABI trampolines into the subclass; exceptions become non-zero returns.
+  static int host_cb(double z, double* P, double* Q, void* user) {
+    auto* self = static_cast<LayerSampler*>(user);
+    try {
+      MatrixXcs p, q;
+      self->sample(z, p, q);
+      const long N = self->nDim();
+      if (p.rows() != N || p.cols() != N || q.rows() != N || q.cols() != N) return 3;
+      std::copy(p.data(), p.data() + N * N, reinterpret_cast<scalex*>(P));
+      std::copy(q.data(), q.data() + N * N, reinterpret_cast<scalex*>(Q));
+      return 0;
+    } catch (...) {
+      return 2;
+    }
+  }
+  static int device_cb(double z, void* P, void* Q, void* stream, void* user) {
+    (void)stream;
+    auto* self = static_cast<LayerSampler*>(user);
+    try {
+      self->gpu_default_ = false;
+      self->sampleOnGPU(z, P, Q);
+      return self->gpu_default_ ? 1 : 0;  // 1: not overridden, sample on the host
+    } catch (...) {
+      return 2;
+    }
+  }
+  bool user_ = false;
+  bool gpu_default_ = false;
 };
 
 // VarLayerSampler (src/core/VarLayerSampler.h:8-39).
@@ -168,7 +242,7 @@ class VarLayerSampler : public LayerSampler {
     vrc_sampler* s = nullptr;
     detail::check(vrc_var_sampler_create(ctx_->get(), lambda, nx, ny, &l, enablePML ? 1 : 0, &s), ctx_->get(),
                   "VarLayerSampler");
-    finish(s, nx * ny);
+    finish(s, nx, ny);
   }
   // x(layout) = rule(z) at every sample (VarLayerSampler::addXRule, src/core/VarLayerSampler.cpp:42-50).
   void addXRule(int layout, std::function<scalar(scalar)> rule) { add(0, layout, std::move(rule)); }
@@ -200,7 +274,7 @@ class StackLayerSampler : public LayerSampler {
     detail::check(vrc_stack_sampler_create(ctx_->get(), lambda, nx, ny, (int)ls.size(), ls.data(), z_edges.data(), &b,
                                            &t, enablePML ? 1 : 0, &s),
                   ctx_->get(), "StackLayerSampler");
-    finish(s, nx * ny);
+    finish(s, nx, ny);
   }
 };
 
@@ -235,7 +309,7 @@ class GDSIISampler : public LayerSampler {
     detail::check(vrc_gds_sampler_create(ctx_->get(), lambda, nx, ny, filename.c_str(), shift.data(), ps, b, d,
                                          thickness, &s),
                   ctx_->get(), "GDSIISampler");
-    finish(s, nx * ny);
+    finish(s, nx, ny);
     detail::check(vrc_gds_sampler_bounds(s_, bounds_), ctx_->get(), "GDSIISampler bounds");
     VectorXs x;
     MatrixXcs eps;

@@ -2,10 +2,11 @@
 // reference's C++ classes does, on the two scenario geometries of the reference
 // drivers, and writes the ten results as raw complex128 for the Python parity test.
 //
-//   cpp_api_check <metasurface|var12> <rcwa|redheffer|differential> <nh> <N or pe> <out.bin>
+//   cpp_api_check <metasurface|var12|user|usergpu|gds:FILE> <rcwa|redheffer|differential> <nh> <N or pe> <out.bin>
 //
 // Scenarios (geometry as data): metasurface cell of src/tests/test_metasurface_cpu.cpp:129-164
-// (periodic, x[2] follows the pillar rule) and the PML waveguide taper of
+// (periodic, x[2] follows the pillar rule; "user"/"usergpu": the same cell through a
+// caller-written LayerSampler subclass overriding sample() / sampleOnGPU()) and the PML waveguide taper of
 // src/tests/test_cpu.cpp:128-176 (rule "var12" on x[3]). Output: int32 N, nEigVal, nDim,
 // then Tdd, Tuu, Rud, Rdu, W0, V0, gam0, W1, V1, gam1 (column-major complex128).
 #include <cmath>
@@ -20,6 +21,45 @@
 using namespace varrcwa;
 
 static constexpr double kPi = 3.1415926535897932;
+
+// A caller's own sampler, written the way a reference user subclasses LayerSampler
+// (src/core/LayerSampler.h:5-77): it assigns the simulation region, calls
+// postInitialization(), and produces P, Q itself -- here by moving the metasurface pillar
+// edge and asking a private layer description for the operator at that edge position. With
+// on_gpu it overrides sampleOnGPU (device buffers on the context stream), else sample().
+class PillarSampler : public LayerSampler {
+ public:
+  PillarSampler(int nx, int ny, bool on_gpu)
+      : LayerSampler(1.55, nx, ny, false), on_gpu_(on_gpu) {
+    postInitialization();
+  }
+  void sample(scalar z, MatrixXcs& P, MatrixXcs& Q) override { layer(z)->sample(0.0, P, Q); }
+  void sampleOnGPU(scalar z, void* P, void* Q) override {
+    if (!on_gpu_) return LayerSampler::sampleOnGPU(z, P, Q);
+    layer(z)->sampleOnGPU(0.0, P, Q);
+  }
+
+ protected:
+  void assignSimulationRegion() override {
+    Lx_ = Ly_ = 0.65;
+    l0_ = b0_ = 0.0;
+    l1_ = b1_ = 0.1;
+    r0_ = u0_ = 0.5;
+    r1_ = u1_ = 0.65;
+  }
+
+ private:
+  // the pillar cell with its right edge at the metasurface rule's position for z
+  std::shared_ptr<VarLayerSampler> layer(scalar z) {
+    const double xe = z <= 1.0 ? 0.5 : 0.2 * std::cos(0.5 * 3.1415926535897932 * (z - 1.0)) + 0.3;
+    MatrixXcs eps(3, 3);
+    for (long k = 0; k < eps.size(); ++k) eps(k) = scalex(1.0, 0.0);
+    eps(1, 1) = scalex(3.48 * 3.48, 0.0);
+    return std::make_shared<VarLayerSampler>(lambda_, nx_, ny_, VectorXs{0.0, 0.1, xe, 0.65},
+                                             VectorXs{0.0, 0.1, 0.5, 0.65}, eps, false, ctx_);
+  }
+  bool on_gpu_;
+};
 
 static MatrixXcs grid(int rows, int cols, double bg) {
   MatrixXcs m(rows, cols);
@@ -63,6 +103,15 @@ int main(int argc, char** argv) {
         return (1.0 - 0.1 * z) * 2.6 + 0.1 * z * 3.7;
       });
       sampler = vs;
+    } else if (scen == "user" || scen == "usergpu") {
+      // the metasurface scenario through a caller-written LayerSampler subclass
+      z0 = 0.0;
+      z1 = 2.0;
+      sampler = std::make_shared<PillarSampler>(2 * nh + 1, 2 * nh + 1, scen == "usergpu");
+      if (sampler->Lx() != 0.65 || sampler->Ly() != 0.65) {
+        std::fprintf(stderr, "Lx()/Ly() do not report the assigned region\n");
+        return 1;
+      }
     } else if (scen.rfind("gds:", 0) == 0) {
       // src/tests/test_gds_cpu.cpp:129-145: the contour layout, ny = 2 nh - 1
       z0 = -1.63;

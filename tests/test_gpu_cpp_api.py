@@ -80,6 +80,25 @@ def test_cpp_differential_metasurface(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("scen", ["user", "usergpu"])
+def test_cpp_user_sampler_subclass(tmp_path, scen):
+    """The sampler plugin boundary (src/core/LayerSampler.h:5-77): a LayerSampler subclass
+    written by the caller (overriding sample(), or sampleOnGPU()) drives all three
+    integrators and matches the oracle on the metasurface scenario."""
+    g = G.metasurface(2)
+    ref = O.rcwa_integrate(G.oracle_sampler(g), g["z0"], g["z1"], 6)
+    close(run_cpp(tmp_path, scen, "rcwa", 2, 6), ref)
+    pe = 5e-2
+    ref = O.redheffer_integrate(G.oracle_sampler(g), g["z0"], g["z1"], pe)
+    got = run_cpp(tmp_path, scen, "redheffer", 2, pe)
+    assert got.N == ref.N
+    close(got, ref)
+    g1 = G.metasurface(1)
+    ref = O.differential_integrate(G.oracle_sampler(g1), g1["z0"], g1["z1"], 16)
+    close(run_cpp(tmp_path, scen, "differential", 1, 16), ref)
+
+
+@pytest.mark.gpu
 def test_cpp_rcwa_gds_contour(tmp_path):
     """GDSIISampler through the C++ classes on the reference's layout file
     (src/tests/test_gds_cpu.cpp:129-145 at 5 x 3 harmonics, 10 slices)."""

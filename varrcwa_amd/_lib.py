@@ -31,6 +31,16 @@ class VrcStackInstance(C.Structure):
                 ("thickness", c_double_p), ("bottom", C.POINTER(VrcLayer)), ("top", C.POINTER(VrcLayer))]
 
 
+class VrcRegion(C.Structure):
+    """``vrc_region``: LayerSampler's Lx_, Ly_ and absorbing-region boundaries."""
+    _fields_ = [(k, C.c_double) for k in ("Lx", "Ly", "b0", "b1", "u0", "u1", "l0", "l1", "r0", "r1")]
+
+
+# vrc_sample_host_fn / vrc_sample_device_fn of the callback sampler
+SAMPLE_HOST_FN = C.CFUNCTYPE(C.c_int, C.c_double, c_double_p, c_double_p, C.c_void_p)
+SAMPLE_DEVICE_FN = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
+
+
 def _sig(name, restype, *args):
     f = getattr(_lib, name)
     f.restype = restype
@@ -71,6 +81,9 @@ def load():
         "vrc_redheffer_segment": (i, vp, i, d, d, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                   vp, vp, vp, c_double_p),
         "vrc_var_sampler_create": (i, vp, d, i, i, C.POINTER(VrcLayer), i, C.POINTER(vp)),
+        "vrc_callback_sampler_create": (i, vp, d, i, i, C.POINTER(VrcRegion), i, SAMPLE_HOST_FN, SAMPLE_DEVICE_FN,
+                                        vp, C.POINTER(vp)),
+        "vrc_sampler_region": (i, vp, C.POINTER(VrcRegion)),
         "vrc_stack_sampler_create": (i, vp, d, i, i, i, C.POINTER(VrcLayer), c_double_p,
                                      C.POINTER(VrcLayer), C.POINTER(VrcLayer), i, C.POINTER(vp)),
         "vrc_sampler_add_rule": (i, vp, i, i, RULE_FN, vp),
@@ -88,7 +101,7 @@ def load():
         "vrc_integrator_get": (i, vp, i, c_double_p),
         "vrc_integrator_counts": (i, vp, c_int_p, c_int_p),
         "vrc_integrator_tree": (i, vp, i, c_double_p),
-        "vrc_batch_stack_rcwa": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp),
+        "vrc_batch_stack_rcwa": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp, c_int_p),
         "vrc_stack_efficiencies": (i, vp, i, i, i, C.POINTER(VrcStackInstance), vp, vp, vp, vp, vp, vp),
         "vrc_wave_eqn_derivatives": (i, vp, i, i, C.POINTER(VrcLayer), d, i, C.POINTER(VrcLayer), i, vp, vp, i, vp,
                                      vp, vp, vp, vp, vp, vp),
@@ -105,7 +118,7 @@ def load():
         "vrc_modal_power": (i, vp, i, i, i, vp, C.c_longlong, vp, C.c_longlong, d, d, i, i, d, d, d, d, vp),
         "vrc_find_guided_modes": (i, vp, i, vp, d, d, d, d, vp, vp, vp),
         "vrc_batch_stack_forward": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp,
-                                    C.POINTER(vp)),
+                                    C.POINTER(vp), c_int_p),
         "vrc_tape_cells": (C.c_longlong, vp),
         "vrc_batch_stack_vjp": (i, vp, vp, vp, i),
         "vrc_tape_destroy": (i, vp),

@@ -13,10 +13,10 @@ from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
-from ._lib import (RULE_FN, Context, VrcError, VrcLayer, VrcStackInstance, c_double_p, cplx, from_dev_colmajor,
-                   load, ptr, to_dev_colmajor)
+from ._lib import (RULE_FN, SAMPLE_DEVICE_FN, SAMPLE_HOST_FN, Context, VrcError, VrcLayer, VrcRegion,
+                   VrcStackInstance, c_double_p, cplx, from_dev_colmajor, load, ptr, to_dev_colmajor)
 
-__all__ = ["Context", "Layer", "VarLayerSampler", "StackLayerSampler", "RCWAIntegrator", "RedhefferIntegrator",
+__all__ = ["Context", "Layer", "LayerSampler", "VarLayerSampler", "StackLayerSampler", "RCWAIntegrator", "RedhefferIntegrator",
            "DifferentialIntegrator", "batch_stack_rcwa", "StackTape", "assemble_toeplitz", "assemble_pq", "eig_pq",
            "layer_smatrix", "star_product", "project_left", "redheffer_segment", "assemble_toeplitz_vjp",
            "assemble_pq_vjp", "eig_pq_vjp", "layer_smatrix_vjp", "star_product_vjp", "project_left_vjp",
@@ -133,6 +133,74 @@ class VarLayerSampler(_Sampler):
 
     def addYRule(self, layout: int, rule: Callable[[float], float]):
         self._rule(1, layout, rule)
+
+
+class LayerSampler(_Sampler):
+    """Base class of a caller's own sampler -- the reference's abstract LayerSampler
+    (src/core/LayerSampler.h:5-77). Subclass it, override ``sample(z) -> (P, Q)`` (numpy
+    N x N complex) and ``assignSimulationRegion()`` (set Lx_, Ly_, b0_ .. r1_), optionally
+    ``sampleOnGPU(z, P, Q)`` (fill the device tensors, column-major N x N, on the context's
+    stream), and call ``postInitialization()`` at the end of ``__init__``. The integrators
+    then call back into the subclass for every P and Q."""
+
+    def __init__(self, lam, nx, ny, enablePML=True, ctx: Optional[Context] = None):
+        super().__init__(ctx or default_context())
+        self.lam, self.nx, self.ny, self.enablePML = float(lam), int(nx), int(ny), bool(enablePML)
+        self.Lx_ = self.Ly_ = self.b0_ = self.b1_ = self.u0_ = self.u1_ = 0.0
+        self.l0_ = self.l1_ = self.r0_ = self.r1_ = 0.0
+
+    def assignSimulationRegion(self):
+        raise NotImplementedError("a LayerSampler subclass assigns its simulation region")
+
+    def lambda_(self):
+        return self.lam
+
+    def k0(self):
+        return 2 * 3.1415926535897932 / self.lam
+
+    def Lx(self):
+        return self.Lx_
+
+    def Ly(self):
+        return self.Ly_
+
+    def postInitialization(self):
+        self.assignSimulationRegion()
+        r = VrcRegion(self.Lx_, self.Ly_, self.b0_, self.b1_, self.u0_, self.u1_, self.l0_, self.l1_, self.r0_,
+                      self.r1_)
+        N = 2 * self.nx * self.ny
+
+        def host(z, Pp, Qp, user):
+            try:
+                P, Q = self.sample(z)
+                for src, dst in ((P, Pp), (Q, Qp)):
+                    a = np.asarray(src, dtype=np.complex128)
+                    if a.shape != (N, N):
+                        return 3
+                    out = np.ctypeslib.as_array(C.cast(dst, C.POINTER(C.c_double)), shape=(2 * N * N,))
+                    out[:] = np.asfortranarray(a).reshape(-1, order="F").view(np.float64)
+                return 0
+            except Exception:
+                return 2
+
+        def device(z, Pd, Qd, stream, user):
+            try:
+                return 1 if self.sampleOnGPU(z, Pd, Qd) is NotImplemented else 0
+            except Exception:
+                return 2
+
+        self._cbs = (SAMPLE_HOST_FN(host), SAMPLE_DEVICE_FN(device))
+        self.ctx.check(self.ctx.lib.vrc_callback_sampler_create(
+            self.ctx.h, self.lam, self.nx, self.ny, C.byref(r), int(self.enablePML), self._cbs[0], self._cbs[1],
+            None, C.byref(self.h)), "postInitialization")
+
+    def sample(self, z: float):
+        raise NotImplementedError("a LayerSampler subclass computes P and Q in sample(z)")
+
+    def sampleOnGPU(self, z: float, P, Q):
+        """Default: not provided (the integrators call sample() and copy). An override
+        receives raw device pointers (ints) of the N x N column-major complex buffers."""
+        return NotImplemented
 
 
 class StackLayerSampler(_Sampler):
@@ -433,12 +501,15 @@ class StackTape:
 
 
 def batch_stack_rcwa(ctx: Context, nx: int, ny: int, instances, chunk: int = 0, want_bases: bool = False,
-                     tape: bool = False):
+                     tape: bool = False, return_info: bool = False):
     """instances: list of dicts with keys lam, layers (Layer list), thickness, bottom, top.
 
     Returns a dict of device tensors: S (ninst, 4, N, N) as column-major storage
     (use ``from_dev_colmajor``), W0, V0, gam0, W1, V1, gam1 when requested, and a
-    StackTape under "tape" when tape=True.
+    StackTape under "tape" when tape=True. A numerical failure of any instance (an
+    unconverged eigensolve, NaN or infinity in its S-matrix) raises VrcError naming the
+    instance, unless return_info=True: then the per-instance codes (0 ok, 1 eigensolver,
+    4 non-finite) are returned under "info" (numpy int array) and nothing is raised for them.
     """
     torch = ctx.torch
     n = nx * ny
@@ -453,14 +524,22 @@ def batch_stack_rcwa(ctx: Context, nx: int, ny: int, instances, chunk: int = 0, 
             shape = (len(instances), N) if name.startswith("gam") else (len(instances), N, N)
             out[name] = torch.empty(shape, dtype=torch.complex128, device=dev)
             ptrs[k] = ptr(out[name])
+    info = np.zeros(max(1, len(instances)), dtype=np.int32)
+    info_p = info.ctypes.data_as(C.POINTER(C.c_int))
     if tape:
         h = C.c_void_p()
-        ctx.check(ctx.lib.vrc_batch_stack_forward(ctx.h, nx, ny, len(instances), arr, int(chunk), ptr(S), *ptrs,
-                                                  C.byref(h)), "batch_stack_forward")
-        out["tape"] = StackTape(ctx, h)
+        rc = ctx.lib.vrc_batch_stack_forward(ctx.h, nx, ny, len(instances), arr, int(chunk), ptr(S), *ptrs,
+                                             C.byref(h), info_p)
+        if h:
+            out["tape"] = StackTape(ctx, h)
+        what = "batch_stack_forward"
     else:
-        ctx.check(ctx.lib.vrc_batch_stack_rcwa(ctx.h, nx, ny, len(instances), arr, int(chunk), ptr(S), *ptrs),
-                  "batch_stack_rcwa")
+        rc = ctx.lib.vrc_batch_stack_rcwa(ctx.h, nx, ny, len(instances), arr, int(chunk), ptr(S), *ptrs, info_p)
+        what = "batch_stack_rcwa"
+    if not (return_info and rc > 0):
+        ctx.check(rc, what)
+    if return_info:
+        out["info"] = info[:len(instances)].copy()
     del keep
     return out
 

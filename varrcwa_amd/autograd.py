@@ -80,8 +80,9 @@ def _autograd_function():
         def backward(fctx, gS, gW0, gW1):
             if gS is None:
                 return None, None
+            # the tape is read, not consumed: it stays with the graph node (released with the
+            # graph), so a second backward with retain_graph=True sees the same eigen-systems
             g = fctx.tape.vjp(gS, chunk=fctx.chunk)
-            fctx.tape.close()
             return (g.real if fctx.real_input else g), None
 
     return StackSMatrix

@@ -15,7 +15,27 @@ inline cd c2(const double* p) { return p ? mk(p[0], p[1]) : mk(0.0, 0.0); }
 inline void need(bool ok, const char* what) {
   if (!ok) throw ArgFailure(what);
 }
+
+// guarded() for the calls that solve whole problems: the context's numerical-failure
+// record is reset on entry and read back (one small synchronizing copy) on exit; a
+// recorded failure returns its positive code (H_EIG .. H_DEPTH) with the text in
+// vrc_last_error. The results are written as computed.
+template <class F>
+int checked(vrc_ctx* c, F&& f) {
+  int num = 0;
+  const int rc = guarded(c, [&] {
+    health_reset(*c);
+    f();
+    std::string m;
+    num = health_read(*c, m);
+    if (num) c->last_error = m;
+  });
+  return rc != 0 ? rc : num;
+}
 }  // namespace
+
+// Trisection depth limit of the varRCWA recursion (vrc_tune_set "redheffer.max_depth").
+static int g_redheffer_depth = 40;
 
 extern "C" {
 
@@ -28,7 +48,12 @@ int vrc_ctx_create(int device, void* stream, vrc_ctx** out) {
   c->device = device;
   // NULL selects the device's default (null) stream, which orders the library's work
   // with every other default-stream user in the process (e.g. torch's default stream).
-  int rc = guarded(c, [&] { c->stream = (hipStream_t)stream; });
+  int rc = guarded(c, [&] {
+    c->stream = (hipStream_t)stream;
+    VRC_HIP(hipMalloc(&c->health, sizeof(int) * HEALTH_INTS));
+    fill_int(c->stream, c->health, 0, HEALTH_INTS);
+    VRC_HIP(hipStreamSynchronize(c->stream));
+  });
   if (rc != 0) {
     delete c;
     return rc;
@@ -78,6 +103,15 @@ int vrc_tune_set(const char* key, long long value) {
   }
   if (std::strcmp(key, "zgemm.big_n") == 0) {
     zgemm_set_big_n((int)value);
+    return 0;
+  }
+  if (std::strcmp(key, "redheffer.max_depth") == 0) {
+    if (value < 0 || value > 60) return -1;
+    g_redheffer_depth = (int)value;
+    return 0;
+  }
+  if (std::strcmp(key, "zgeev.max_sweeps") == 0) {
+    zgeev_set_max_sweeps((int)value);
     return 0;
   }
   if (std::strcmp(key, "zgeev.bulges") == 0) {
@@ -401,6 +435,39 @@ int vrc_sampler_add_rule(vrc_sampler* s, int axis, int layout, double (*rule)(do
   });
 }
 
+int vrc_callback_sampler_create(vrc_ctx* c, double lambda, int nx, int ny, const vrc_region* region,
+                                int enable_pml, vrc_sample_host_fn host_fn, vrc_sample_device_fn device_fn,
+                                void* user, vrc_sampler** out) {
+  if (out) *out = nullptr;
+  return guarded(c, [&] {
+    need(out && region, "bad arguments");
+    check_harmonics(nx, ny);
+    Region r;
+    r.Lx = region->Lx;
+    r.Ly = region->Ly;
+    r.b0 = region->b0;
+    r.b1 = region->b1;
+    r.u0 = region->u0;
+    r.u1 = region->u1;
+    r.l0 = region->l0;
+    r.l1 = region->l1;
+    r.r0 = region->r0;
+    r.r1 = region->r1;
+    auto h = std::make_unique<vrc_sampler>();
+    h->ctx = c;
+    h->s = std::make_unique<CallbackSampler>(c, lambda, nx, ny, r, enable_pml != 0, host_fn, device_fn, user);
+    ctx_retain(c);
+    *out = h.release();
+  });
+}
+
+int vrc_sampler_region(const vrc_sampler* s, vrc_region* out) {
+  if (!s || !out) return -1;
+  const Region& r = s->s->region();
+  *out = vrc_region{r.Lx, r.Ly, r.b0, r.b1, r.u0, r.u1, r.l0, r.l1, r.r0, r.r1};
+  return 0;
+}
+
 int vrc_stack_sampler_create(vrc_ctx* c, double lambda, int nx, int ny, int nlayers, const vrc_layer* layers,
                              const double* z_edges, const vrc_layer* bottom, const vrc_layer* top, int enable_pml,
                              vrc_sampler** out) {
@@ -515,7 +582,7 @@ int vrc_rcwa_solver_add_layer(vrc_rcwa_solver* s, const vrc_layer* layer) {
 int vrc_rcwa_solver_solve(vrc_rcwa_solver* s, double lambda, int nstack, const int* stack, const double* thickness,
                           const double* translation) {
   if (!s) return -1;
-  return guarded(s->ctx, [&] {
+  return checked(s->ctx, [&] {
     need(nstack >= 0 && (nstack == 0 || (stack && thickness)), "bad arguments");
     s->s->solve(lambda, std::vector<int>(stack, stack + nstack), std::vector<double>(thickness, thickness + nstack),
                 translation);
@@ -563,12 +630,7 @@ int vrc_sampler_sample(vrc_sampler* s, double z, double* Ph, double* Qh) {
   if (!s) return -1;
   return guarded(s->ctx, [&] {
     need(Ph && Qh, "null output");
-    const long NN = (long)s->s->nDim() * s->s->nDim();
-    DevBuf P = s->ctx->ws.get(sizeof(cd) * NN), Q = s->ctx->ws.get(sizeof(cd) * NN);
-    s->s->sample_gpu({z}, P.as<cd>(), Q.as<cd>());
-    VRC_HIP(hipMemcpyAsync(Ph, P.ptr, sizeof(cd) * NN, hipMemcpyDeviceToHost, s->ctx->stream));
-    VRC_HIP(hipMemcpyAsync(Qh, Q.ptr, sizeof(cd) * NN, hipMemcpyDeviceToHost, s->ctx->stream));
-    VRC_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->s->sample_host(z, (cd*)Ph, (cd*)Qh);
   });
 }
 
@@ -622,7 +684,7 @@ static std::vector<std::pair<double, double>> uniform_slices(double z0, double z
 
 int vrc_rcwa_compute(vrc_integrator* it, int N) {
   if (!it) return -1;
-  return guarded(it->ctx, [&] {
+  return checked(it->ctx, [&] {
     need(it->kind == 0 && N >= 1, "RCWA integrator with N >= 1 expected");
     it->it->rcwa(uniform_slices(it->it->it_z0(), it->it->it_z1(), N), false);
   });
@@ -630,7 +692,7 @@ int vrc_rcwa_compute(vrc_integrator* it, int N) {
 
 int vrc_rcwa_compute_batched(vrc_integrator* it, int N) {
   if (!it) return -1;
-  return guarded(it->ctx, [&] {
+  return checked(it->ctx, [&] {
     need(it->kind == 0 && N >= 1, "RCWA integrator with N >= 1 expected");
     it->it->rcwa(uniform_slices(it->it->it_z0(), it->it->it_z1(), N), true);
   });
@@ -638,7 +700,7 @@ int vrc_rcwa_compute_batched(vrc_integrator* it, int N) {
 
 int vrc_rcwa_compute_edges(vrc_integrator* it, int ne, const double* ze) {
   if (!it) return -1;
-  return guarded(it->ctx, [&] {
+  return checked(it->ctx, [&] {
     need(it->kind == 0 && ne >= 2 && ze, "RCWA integrator with >= 2 edges expected");
     std::vector<std::pair<double, double>> v;
     for (int i = 0; i + 1 < ne; ++i) {
@@ -653,15 +715,15 @@ int vrc_redheffer_compute(vrc_integrator* it, int p, int maxN, double pe) {
   if (!it) return -1;
   (void)p;
   (void)maxN;
-  return guarded(it->ctx, [&] {
+  return checked(it->ctx, [&] {
     need(it->kind == 1 && pe > 0, "Redheffer integrator with pe > 0 expected");
-    it->it->redheffer(pe, 40);
+    it->it->redheffer(pe, g_redheffer_depth);
   });
 }
 
 int vrc_differential_compute(vrc_integrator* it, int N) {
   if (!it) return -1;
-  return guarded(it->ctx, [&] {
+  return checked(it->ctx, [&] {
     need(it->kind == 2 && N >= 1, "differential integrator with N >= 1 expected");
     it->it->differential(N);
   });
@@ -725,8 +787,8 @@ struct vrc_tape {
 };
 
 int vrc_batch_stack_rcwa(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
-                         void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1) {
-  return vrc_batch_stack_forward(c, nx, ny, ninst, inst, chunk, S_out, W0, V0, g0, W1, V1, g1, nullptr);
+                         void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1, int* info_out) {
+  return vrc_batch_stack_forward(c, nx, ny, ninst, inst, chunk, S_out, W0, V0, g0, W1, V1, g1, nullptr, info_out);
 }
 
 int vrc_modal_power(vrc_ctx* c, int nx, int ny, int nmodes, const void* W, long long ldw, const void* V,
@@ -778,9 +840,10 @@ int vrc_stack_efficiencies(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stac
 
 int vrc_batch_stack_forward(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
                             void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1,
-                            vrc_tape** tape) {
+                            vrc_tape** tape, int* info_out) {
   if (tape) *tape = nullptr;
-  return guarded(c, [&] {
+  int num = 0;
+  const int rc = guarded(c, [&] {
     check_harmonics(nx, ny);
     need(ninst >= 0 && (ninst == 0 || (inst && S_out)), "bad arguments");
     if (ninst == 0) {  // nothing to solve
@@ -794,13 +857,29 @@ int vrc_batch_stack_forward(vrc_ctx* c, int nx, int ny, int ninst, const vrc_sta
       tp->ctx = c;
       tp->t = make_stack_tape(*c, nx, ny, hs.si);
     }
+    health_reset(*c);
+    DevBuf flags = c->ws.get(sizeof(int) * (size_t)ninst);
+    fill_int(c->stream, flags.as<int>(), 0, ninst);
     batch_stack_rcwa(*c, nx, ny, hs.si, chunk, (cd*)S_out, (cd*)W0, (cd*)V0, (cd*)g0, (cd*)W1, (cd*)V1, (cd*)g1,
-                     tp ? tp->t.get() : nullptr);
+                     tp ? tp->t.get() : nullptr, flags.as<int>());
+    std::vector<int> fl(ninst);
+    VRC_HIP(hipMemcpyAsync(fl.data(), flags.ptr, sizeof(int) * ninst, hipMemcpyDeviceToHost, c->stream));
+    std::string m;
+    num = health_read(*c, m);  // synchronizes
+    if (info_out) std::copy(fl.begin(), fl.end(), info_out);
+    for (int i = 0; i < ninst; ++i)
+      if (fl[i]) {
+        num = fl[i];
+        m = "instance " + std::to_string(i) + ": " + health_text(fl[i]) + (m.empty() ? "" : "; " + m);
+        break;
+      }
+    if (num) c->last_error = m;
     if (tape) {
       ctx_retain(c);
       *tape = tp.release();
     }
   });
+  return rc != 0 ? rc : num;
 }
 
 long long vrc_tape_cells(const vrc_tape* t) { return (t && t->t) ? (long long)t->t->cells() : -1; }

@@ -1038,7 +1038,7 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
 // per sweep (<= NBUL).
 __global__ __launch_bounds__(QT) void hqr_kernel(int N, cd* H, long ldh, long sH, cd* Z, long ldz, long sZ,
                                                   cd* wout, long sw, int* info, long long* stats, int use_aed,
-                                                  int max_bul) {
+                                                  int max_bul, int max_sweeps) {
   long long st[5] = {0, 0, 0, 0, 0};
   __shared__ QrLds L;
   const long b = blockIdx.x;
@@ -1048,7 +1048,7 @@ __global__ __launch_bounds__(QT) void hqr_kernel(int N, cd* H, long ldh, long sH
   const int t = threadIdx.x;
   int ihi = N - 1;
   int sweeps = 0, last_progress = 0, prev_lo = -1, prev_hi = N;
-  const int itmax = 30 * (N > 10 ? N : 10);
+  const int itmax = max_sweeps > 0 ? max_sweeps : 30 * (N > 10 ? N : 10);
   int fail = 0;
   while (ihi >= 0) {
     const int lo = find_lo(L, H, ldh, N, ihi);
@@ -1380,6 +1380,8 @@ __global__ __launch_bounds__(256) void balance_kernel(int N, cd* A, long lda, lo
 
 static int g_aed_window = AED_NW;
 static int g_bulges = 12;
+static int g_max_sweeps = 0;  // 0: 30 * max(N, 10), LAPACK zlahqr's iteration budget
+void zgeev_set_max_sweeps(int n) { g_max_sweeps = n < 0 ? 0 : n; }
 void zgeev_set_aed(int window) { g_aed_window = window < 0 ? 0 : (window > QW ? QW : window); }
 void zgeev_set_bulges(int nb) { g_bulges = nb < 1 ? 1 : (nb > NBUL ? NBUL : nb); }
 
@@ -1409,8 +1411,9 @@ void zgeev_factors(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
     DevBuf sb;
     if (want) sb = ws.get(sizeof(long long) * 5 * batch);
     hqr_kernel<<<batch, QT, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, want ? sb.as<long long>() : nullptr,
-                                     g_aed_window, g_bulges);
+                                     g_aed_window, g_bulges, g_max_sweeps);
     launch_check("hqr");
+    health_note_info(s, info, batch, H_EIG);
     if (want) {
       std::vector<long long> h(5 * (size_t)batch);
       VRC_HIP(hipMemcpyAsync(h.data(), sb.ptr, sizeof(long long) * h.size(), hipMemcpyDeviceToHost, s));
@@ -1462,6 +1465,7 @@ void gamma_sqrt(hipStream_t s, int n, const cd* lam, long sl, cd* gam, long sg, 
   dim3 g(cdiv_up(n, 256), batch);
   gamma_sqrt_kernel<<<g, 256, 0, s>>>(n, lam, sl, gam, sg);
   launch_check("gamma_sqrt");
+  if (sg == n) health_note_gamma(s, gam, (long)n * batch);
 }
 
 }  // namespace vrc

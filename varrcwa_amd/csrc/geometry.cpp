@@ -19,6 +19,11 @@ PreparedLayer prepare_layer(const HostLayer& L) {
   const int nCx = (int)L.x.size(), nCy = (int)L.y.size();
   if (nCx < 2 || nCy < 2) throw ArgFailure("layer needs at least 2 coordinates per axis");
   if ((int)L.eps.size() != (nCx - 1) * (nCy - 1)) throw ArgFailure("eps size does not match coordinates");
+  for (const cd& e : L.eps) {
+    if (!std::isfinite(e.x) || !std::isfinite(e.y)) throw ArgFailure("non-finite permittivity (NaN or infinity)");
+    // the inverse Fourier rules (fe11, fe22) are built from 1/eps of every cell
+    if (e.x == 0.0 && e.y == 0.0) throw ArgFailure("zero permittivity in a cell: the inverse Fourier rules need 1/eps");
+  }
   PreparedLayer p;
   p.ncx = nCx - 1;
   p.ncy = nCy - 1;
@@ -75,12 +80,35 @@ static cd pml1d(double b0, double b1, double L, cd gamma, int n, double coeff) {
   return cscale(r, bm / 16.0);
 }
 
+// VarLayerSampler::assignSimulationRegion (src/core/VarLayerSampler.cpp:23-39): the cell
+// and the absorbing regions from the first/last two coordinates of each axis.
+Region region_from_coords(const std::vector<double>& x, const std::vector<double>& y) {
+  const int nCx = (int)x.size(), nCy = (int)y.size();
+  Region r;
+  r.Lx = x[nCx - 1] - x[0];
+  r.Ly = y[nCy - 1] - y[0];
+  r.b0 = y[0];
+  r.b1 = nCy > 1 ? y[1] : y[0];
+  r.u0 = nCy > 1 ? y[nCy - 2] : y[0];
+  r.u1 = y[nCy - 1];
+  r.l0 = x[0];
+  r.l1 = nCx > 1 ? x[1] : x[0];
+  r.r0 = nCx > 1 ? x[nCx - 2] : x[0];
+  r.r1 = x[nCx - 1];
+  return r;
+}
+
 void pml_k_matrices(int nx, int ny, const std::vector<double>& x, const std::vector<double>& y,
                     std::vector<cd>& Kx, std::vector<cd>& Ky) {
-  const int nCx = (int)x.size(), nCy = (int)y.size();
-  const double Lx = x[nCx - 1] - x[0], Ly = y[nCy - 1] - y[0];
-  const double b0 = y[0], b1 = y[1], u0 = y[nCy - 2], u1 = y[nCy - 1];
-  const double l0 = x[0], l1 = x[1], r0 = x[nCx - 2], r1 = x[nCx - 1];
+  pml_k_matrices(nx, ny, region_from_coords(x, y), Kx, Ky);
+}
+
+// LayerSampler::evaluateKMatrices + evaluatePML (src/core/LayerSampler.cpp:34-132) for a
+// region given by its scalars.
+void pml_k_matrices(int nx, int ny, const Region& R, std::vector<cd>& Kx, std::vector<cd>& Ky) {
+  const double Lx = R.Lx, Ly = R.Ly;
+  const double b0 = R.b0, b1 = R.b1, u0 = R.u0, u1 = R.u1;
+  const double l0 = R.l0, l1 = R.l1, r0 = R.r0, r1 = R.r1;
   std::vector<cd> alpha, beta;
   alpha_beta(nx, ny, Lx, Ly, alpha, beta);
   const cd gamma = crecip(mk(1.0, -1.0));

@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
+#include <string>
 
 #include "vrc_adjoint.hpp"
 
@@ -35,13 +37,20 @@ Sampler::Sampler(vrc_ctx* c, double lambda, int nx, int ny, bool pml)
 double Sampler::k0() const { return 2 * PI_REF / lambda_; }
 
 void Sampler::post_init(const std::vector<double>& x, const std::vector<double>& y) {
+  if (pml_ && (x.size() < 4 || y.size() < 4)) throw ArgFailure("PML needs at least 4 coordinates per axis");
+  post_init(region_from_coords(x, y));
+}
+
+// LayerSampler::postInitialization (src/core/LayerSampler.cpp:145-153): K matrices of the
+// region, periodic (diagonal) or PML-stretched (dense, with the products P/Q assembly needs).
+void Sampler::post_init(const Region& r) {
   const int n = nx_ * ny_;
   const long nn = (long)n * n;
-  const double Lx = x.back() - x.front(), Ly = y.back() - y.front();
+  const double Lx = r.Lx, Ly = r.Ly;
   if (!(Lx > 0) || !(Ly > 0)) throw ArgFailure("cell size must be positive");
+  region_ = r;
   if (pml_) {
-    if (x.size() < 4 || y.size() < 4) throw ArgFailure("PML needs at least 4 coordinates per axis");
-    pml_k_matrices(nx_, ny_, x, y, hKx_, hKy_);
+    pml_k_matrices(nx_, ny_, r, hKx_, hKy_);
     std::vector<cd> both(2 * nn);
     std::copy(hKx_.begin(), hKx_.end(), both.begin());
     std::copy(hKy_.begin(), hKy_.end(), both.begin() + nn);
@@ -75,6 +84,62 @@ void Sampler::sample_gpu(const std::vector<double>& zs, cd* P, cd* Q) {
   ls.reserve(zs.size());
   for (double z : zs) ls.push_back(layer_at(z));
   sample_layers(ls, P, Q);
+}
+
+void Sampler::sample_host(double z, cd* Ph, cd* Qh) {
+  const long NN = (long)nDim() * nDim();
+  DevBuf P = c_->ws.get(sizeof(cd) * NN), Q = c_->ws.get(sizeof(cd) * NN);
+  sample_gpu({z}, P.as<cd>(), Q.as<cd>());
+  VRC_HIP(hipMemcpyAsync(Ph, P.ptr, sizeof(cd) * NN, hipMemcpyDeviceToHost, c_->stream));
+  VRC_HIP(hipMemcpyAsync(Qh, Q.ptr, sizeof(cd) * NN, hipMemcpyDeviceToHost, c_->stream));
+  VRC_HIP(hipStreamSynchronize(c_->stream));
+}
+
+CallbackSampler::CallbackSampler(vrc_ctx* c, double lambda, int nx, int ny, const Region& r, bool pml,
+                                 SampleHostFn hf, SampleDeviceFn df, void* user)
+    : Sampler(c, lambda, nx, ny, pml), hf_(hf), df_(df), user_(user) {
+  if (!hf && !df) throw ArgFailure("a callback sampler needs a host or a device sampling function");
+  if (pml && !(r.l0 <= r.l1 && r.l1 <= r.r0 && r.r0 <= r.r1 && r.b0 <= r.b1 && r.b1 <= r.u0 && r.u0 <= r.u1))
+    throw ArgFailure("PML region boundaries must be ordered l0 <= l1 <= r0 <= r1 and b0 <= b1 <= u0 <= u1");
+  post_init(r);
+}
+
+HostLayer CallbackSampler::layer_at(double) const {
+  throw ArgFailure("a callback sampler has no layer description (its P and Q come from the caller)");
+}
+
+static void callback_failed(int rc, const char* which) {
+  throw std::runtime_error(std::string("sampler ") + which + " callback returned " + std::to_string(rc));
+}
+
+void CallbackSampler::sample_gpu(const std::vector<double>& zs, cd* P, cd* Q) {
+  const long NN = (long)nDim() * nDim();
+  std::vector<cd> hP, hQ;
+  for (size_t k = 0; k < zs.size(); ++k) {
+    cd* Pk = P + (long)k * NN;
+    cd* Qk = Q + (long)k * NN;
+    if (df_) {
+      const int rc = df_(zs[k], Pk, Qk, (void*)c_->stream, user_);
+      if (rc == 0) continue;
+      if (rc != 1 || !hf_) callback_failed(rc, "device");
+    }
+    hP.resize(NN);
+    hQ.resize(NN);
+    const int rc = hf_(zs[k], (double*)hP.data(), (double*)hQ.data(), user_);
+    if (rc != 0) callback_failed(rc, "host");
+    VRC_HIP(hipMemcpyAsync(Pk, hP.data(), sizeof(cd) * NN, hipMemcpyHostToDevice, c_->stream));
+    VRC_HIP(hipMemcpyAsync(Qk, hQ.data(), sizeof(cd) * NN, hipMemcpyHostToDevice, c_->stream));
+    VRC_HIP(hipStreamSynchronize(c_->stream));  // the host buffers are reused
+  }
+}
+
+void CallbackSampler::sample_host(double z, cd* Ph, cd* Qh) {
+  if (hf_) {
+    const int rc = hf_(z, (double*)Ph, (double*)Qh, user_);
+    if (rc != 0) callback_failed(rc, "host");
+    return;
+  }
+  Sampler::sample_host(z, Ph, Qh);
 }
 
 void Sampler::sample_layers(const std::vector<HostLayer>& ls, cd* P, cd* Q) {
@@ -458,6 +523,12 @@ void Integrator::redheffer(double pe, int max_depth) {
         const double err =
             redheffer_perturbation(c, N, k0, za, zb, Pr, Qr, R, P3, Q3, T.Tdd, T.Tuu, T.Rud, T.Rdu);
         if (err <= pe || depth >= max_depth) {
+          if (err > pe && !c.host_code) {
+            c.host_code = H_DEPTH;
+            c.host_msg = std::string(health_text(H_DEPTH)) + " (segment [" + std::to_string(za) + ", " +
+                         std::to_string(zb) + "], error " + std::to_string(err) + ", depth " +
+                         std::to_string(max_depth) + ")";
+          }
           ++N_;
           tree_.push_back({za, zb});
           project_left(c, N, 1, T, R.LU, leftW.as<cd>(), leftV.as<cd>(), 0);
@@ -655,7 +726,7 @@ static void cascade_group(vrc_ctx& c, int N, int L, int GB, int I0, bool homog, 
 // `per_launch` slabs, regardless of instance boundaries (the eigen kernels run one matrix
 // per CU, so full launches matter more than whole instances).
 static void batch_stack_homog(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int per_launch,
-                              cd* S_out, cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o) {
+                              cd* S_out, cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o, int* flags) {
   hipStream_t st = c.stream;
   const int n = nx * ny, N = 2 * n;
   const long NN = (long)N * N;
@@ -736,6 +807,7 @@ static void batch_stack_homog(vrc_ctx& c, int nx, int ny, const std::vector<Stac
              Db = c.ws.get(sizeof(cd) * (size_t)N * SL);
       wave_eqn_factors(c, N, P.as<cd>(), Q.as<cd>(), NN, Zb.as<cd>(), Xb.as<cd>(), Db.as<cd>(), gam.as<cd>(),
                        inf.as<int>(), SL, PQ.as<cd>());
+      if (flags) info_to_owner(st, inf.as<int>(), down.as<int>(), SL, H_EIG, flags + I0);
       PQ = DevBuf();
       DevBuf Vr = c.ws.get(sizeof(cd) * NN * SL);
       gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);  // W1 = I
@@ -748,8 +820,20 @@ static void batch_stack_homog(vrc_ctx& c, int nx, int ny, const std::vector<Stac
   }
 }
 
+static void batch_stack_solve(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int chunk,
+                              cd* S_out, cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o, StackTape* tape,
+                              int* flags);
+
 void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int chunk, cd* S_out,
-                      cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o, StackTape* tape) {
+                      cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o, StackTape* tape, int* flags) {
+  batch_stack_solve(c, nx, ny, inst, chunk, S_out, W0o, V0o, g0o, W1o, V1o, g1o, tape, flags);
+  const long NN = 4L * nx * ny * nx * ny;
+  if (flags) nonfinite_flags(c.stream, S_out, 4 * NN, 4 * NN, (int)inst.size(), flags, H_NONFINITE);
+}
+
+static void batch_stack_solve(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int chunk,
+                              cd* S_out, cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o, StackTape* tape,
+                              int* flags) {
   hipStream_t st = c.stream;
   const int n = nx * ny, N = 2 * n;
   const long NN = (long)N * N;
@@ -761,6 +845,8 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
       throw ArgFailure("all instances of one batch must have the same layer count");
     if (!I.bottom || !I.top) throw ArgFailure("instance without bottom/top medium");
     if (!(I.lambda > 0)) throw ArgFailure("wavelength must be positive");
+    prepare_layer(*I.bottom);  // validates the half-space media (their eps may take the closed form)
+    prepare_layer(*I.top);
   }
   // End media given by one permittivity value get their eigen-system in closed form.
   bool homog = true;
@@ -768,7 +854,7 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
     homog = homog && I.bottom->x.size() == 2 && I.bottom->y.size() == 2 && I.top->x.size() == 2 &&
             I.top->y.size() == 2;
   if (homog && !tape) {
-    batch_stack_homog(c, nx, ny, inst, chunk > 0 ? chunk * L : 256, S_out, W0o, V0o, g0o, W1o, V1o, g1o);
+    batch_stack_homog(c, nx, ny, inst, chunk > 0 ? chunk * L : 256, S_out, W0o, V0o, g0o, W1o, V1o, g1o, flags);
     return;
   }
   // Default chunk: at most 256 dense eigensolves (L slices, + 2 end media per instance
@@ -821,10 +907,13 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
         owner[b * L + l] = b;
       }
     }
+    std::vector<int> job_owner(J);  // instance (within the chunk) of every job
+    for (int j = 0; j < J; ++j) job_owner[j] = j < B ? j : (j < 2 * B ? j - B : owner[j - 2 * B]);
     DevBuf dk2 = upload(c, k2.data(), sizeof(double) * J);
     DevBuf dab = upload(c, ab.data(), sizeof(cd) * ab.size());
     DevBuf dsv = upload(c, sv.data(), sizeof(double) * sv.size());
     DevBuf down = upload(c, owner.data(), sizeof(int) * owner.size());
+    DevBuf djo = upload(c, job_owner.data(), sizeof(int) * J);
     // ---- one eigen-solve launch for end media (unless closed form) and slices
     const int SL = B * L;
     const int je = homog ? 2 * B : 0;  // first job solved by the dense eigensolver
@@ -899,6 +988,7 @@ void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstanc
                             Xb.as<cd>() + s0 * NN, Db.as<cd>() + s0 * N, gall.as<cd>() + (long)2 * B * N,
                             Wr.as<cd>(), Vr.as<cd>(), NN, tu, ru, homog);
     }
+    if (flags) info_to_owner(st, infall.as<int>(), djo.as<int>(), J, H_EIG, flags + i0);
     Wr = DevBuf();
     Vr = DevBuf();
     P = DevBuf();

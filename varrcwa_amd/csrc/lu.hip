@@ -17,6 +17,11 @@ constexpr int LU_NB = 32;    // panel width
 constexpr int LU_NB2 = 128;  // outer block width
 constexpr int TRS_NB = 64;
 
+__global__ void info_zero_kernel(int* info, int batch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < batch) info[b] = 0;
+}
+
 __global__ __launch_bounds__(256) void getrf_panel_kernel(int n, int k, int kb, cd* A, long lda, long sA, int* ipiv,
                                                           long sP, int* info) {
   const long b = blockIdx.x;
@@ -355,6 +360,10 @@ static void trsm_lu_factor(hipStream_t s, int upper, int n, int nrhs, const cd* 
 // by one ZGEMM with k = 128.
 void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, int* info, int batch, Workspace& ws) {
   if (n <= 0 || batch <= 0) return;
+  if (info) {  // LAPACK semantics: info is an output (0, or the first zero pivot's column + 1)
+    info_zero_kernel<<<cdiv_up(batch, 256), 256, 0, s>>>(info, batch);
+    launch_check("info_zero");
+  }
   const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0);
   const bool reg = n <= LUR_ROWS;
   DevBuf plan = ws.get(sizeof(int) * (size_t)(1 + 2 * LASWP_MAXROW) * batch);
@@ -407,6 +416,7 @@ void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, 
                     A + K + (long)(K + KB) * lda, lda, sA, one, A + (K + KB) + (long)(K + KB) * lda, lda, sA, batch);
     }
   }
+  health_note_info(s, info, batch, H_LU);
 }
 
 // perm[i] = original row that ends up in row i after applying ipiv[0..n-1] in order.

@@ -543,6 +543,7 @@ void homogeneous_eig(hipStream_t s, int n, int batch, const cd* alpha, const cd*
   dim3 g(cdiv_up(N, 256), N, batch);
   homogeneous_eig_kernel<<<g, 256, 0, s>>>(n, alpha, beta, sab, k0sq, eps, W, V, gam, (long)N * N);
   launch_check("homogeneous_eig");
+  health_note_gamma(s, gam, (long)N * batch);
 }
 
 // ===================================================================== (d) star product

@@ -26,7 +26,13 @@ struct vrc_ctx {
   // QR iteration statistics accumulated while timing: matrices, sweeps, windows,
   // chase ticks, apply ticks, small-block ticks
   long long qr_stats[6] = {0, 0, 0, 0, 0, 0};
+  // numerical-failure record (device, HEALTH_INTS ints; see health.hip) and the host-side
+  // failure noted by the integrators themselves (code 0 = none)
+  int* health = nullptr;
+  int host_code = 0;
+  std::string host_msg;
   ~vrc_ctx() {
+    if (health) (void)hipFree(health);
     for (auto& r : recs) {
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);

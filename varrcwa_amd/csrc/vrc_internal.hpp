@@ -56,6 +56,17 @@ inline DevBuf::~DevBuf() {
   if (owner) owner->put(ptr, bytes);
 }
 
+// ---------------------------------------------------------------- numerical failures (health.hip)
+// Positive status codes of the top-level calls (and the per-instance info of batched
+// stacks): 1 eigensolver not converged, 2 zero pivot in an LU, 3 zero propagation
+// constant, 4 non-finite results, 5 varRCWA depth limit with the error above pe.
+enum HealthCode : int { H_OK = 0, H_EIG = 1, H_LU = 2, H_GAMMA0 = 3, H_NONFINITE = 4, H_DEPTH = 5 };
+constexpr int HEALTH_INTS = 4;
+// Record the first non-zero info / zero gamma of a launch in the active context's record
+// (no-op without an active context).
+void health_note_info(hipStream_t s, const int* info, int batch, int code);
+void health_note_gamma(hipStream_t s, const cd* gam, long count);
+
 // ---------------------------------------------------------------- blas.hip
 void zgemm_strided(hipStream_t s, int opA, int opB, int m, int n, int k, cd alpha, const cd* A, long lda, long sA,
                    const cd* B, long ldb, long sB, cd beta, cd* C, long ldc, long sC, int batch);
@@ -102,6 +113,7 @@ void zgeev_factors(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
 void zgeev_vectors(hipStream_t s, int n, const cd* Z, const cd* X, const cd* dbal, cd* VR, long ldv, long sV,
                    int batch);
 void zgeev_set_aed(int window);
+void zgeev_set_max_sweeps(int n);  // QR sweep budget per matrix (0 = automatic)
 void zgeev_set_bulges(int nb);  // bulges per QR sweep (1..16)  // deflation-window size of the QR iteration (0 = off)
 // gamma = sqrt(lambda) with the reference's branch rule.
 void gamma_sqrt(hipStream_t s, int n, const cd* lam, long sl, cd* gam, long sg, int batch);

@@ -16,6 +16,17 @@ namespace vrc {
 
 void fill_int(hipStream_t s, int* p, int v, long n);
 
+// ---------------------------------------------------------------- numerical failures
+// (codes and the per-launch notes in vrc_internal.hpp)
+void health_reset(vrc_ctx& c);
+// Synchronizes; returns the first recorded code (0 none) and its description.
+int health_read(vrc_ctx& c, std::string& msg);
+const char* health_text(int code);
+// flags[i] = code (if still 0) for each instance i whose `per` values at A + i*stride hold a
+// NaN or an infinity; flags[owner[q]] = code for each job q with info[q] != 0.
+void nonfinite_flags(hipStream_t s, const cd* A, long per, long stride, int ninst, int* flags, int code);
+void info_to_owner(hipStream_t s, const int* info, const int* owner, int njobs, int code, int* flags);
+
 // ---------------------------------------------------------------- geometry (host)
 // FourierSolver1D::reorder/refill semantics (src/rcwa/FourierSolver1D.cpp:145-193)
 // applied to one layer: painter-order values and the two interval lists.
@@ -29,9 +40,16 @@ PreparedLayer prepare_layer(const HostLayer& L);
 
 // alpha/beta of LayerSampler::evaluateAlphaBeta (src/core/LayerSampler.cpp:9-32).
 void alpha_beta(int nx, int ny, double Lx, double Ly, std::vector<cd>& alpha, std::vector<cd>& beta);
+// Simulation region of a sampler (LayerSampler's Lx_, Ly_, b0_ .. r1_,
+// src/core/LayerSampler.h:7-28).
+struct Region {
+  double Lx = 0, Ly = 0, b0 = 0, b1 = 0, u0 = 0, u1 = 0, l0 = 0, l1 = 0, r0 = 0, r1 = 0;
+};
+Region region_from_coords(const std::vector<double>& x, const std::vector<double>& y);
 // Dense PML-stretched K matrices (src/core/LayerSampler.cpp:34-131), n x n column-major.
 void pml_k_matrices(int nx, int ny, const std::vector<double>& x, const std::vector<double>& y,
                     std::vector<cd>& Kx, std::vector<cd>& Ky);
+void pml_k_matrices(int nx, int ny, const Region& r, std::vector<cd>& Kx, std::vector<cd>& Ky);
 
 // ---------------------------------------------------------------- (a) + (b)
 struct AssemblyBatch {
@@ -157,6 +175,7 @@ class Sampler {
  public:
   Sampler(vrc_ctx* c, double lambda, int nx, int ny, bool pml);
   virtual ~Sampler() = default;
+  // The layer at z (geometric samplers; others throw).
   virtual HostLayer layer_at(double z) const = 0;
   int nDim() const { return 2 * nx_ * ny_; }
   int n() const { return nx_ * ny_; }
@@ -164,7 +183,10 @@ class Sampler {
   double k0() const;
   vrc_ctx* ctx() const { return c_; }
   // P, Q (N x N, stride N*N) for every z (device).
-  void sample_gpu(const std::vector<double>& zs, cd* P, cd* Q);
+  virtual void sample_gpu(const std::vector<double>& zs, cd* P, cd* Q);
+  // P, Q (N x N) into host buffers.
+  virtual void sample_host(double z, cd* P, cd* Q);
+  const Region& region() const { return region_; }
   // P, Q of explicit layers with this sampler's wavelength and K matrices.
   void sample_layers(const std::vector<HostLayer>& ls, cd* P, cd* Q);
   const std::vector<cd>& Kx() const { return hKx_; }
@@ -178,7 +200,9 @@ class Sampler {
 
  protected:
   void post_init(const std::vector<double>& x, const std::vector<double>& y);
+  void post_init(const Region& r);
   vrc_ctx* c_;
+  Region region_;
   double lambda_;
   int nx_, ny_;
   bool pml_;
@@ -195,6 +219,26 @@ class VarSampler : public Sampler {
  private:
   HostLayer base_;
   std::map<int, std::pair<RuleFn, void*>> xr_, yr_;
+};
+
+// A sampler whose P and Q come from the caller (the reference's user subclasses of
+// LayerSampler, src/core/LayerSampler.h:5-77): host_fn fills host N x N column-major P, Q;
+// device_fn fills device buffers on the given stream (returns 1 to defer to host_fn).
+// Either may be null, not both. K matrices from the given region (postInitialization).
+typedef int (*SampleHostFn)(double z, double* P, double* Q, void* user);
+typedef int (*SampleDeviceFn)(double z, void* P, void* Q, void* stream, void* user);
+class CallbackSampler : public Sampler {
+ public:
+  CallbackSampler(vrc_ctx* c, double lambda, int nx, int ny, const Region& r, bool pml, SampleHostFn hf,
+                  SampleDeviceFn df, void* user);
+  HostLayer layer_at(double z) const override;
+  void sample_gpu(const std::vector<double>& zs, cd* P, cd* Q) override;
+  void sample_host(double z, cd* P, cd* Q) override;
+
+ private:
+  SampleHostFn hf_;
+  SampleDeviceFn df_;
+  void* user_;
 };
 
 class StackSampler : public Sampler {
@@ -304,7 +348,10 @@ void stack_efficiencies(vrc_ctx& c, int nx, int ny, const std::vector<StackInsta
                         const cd* W0, const cd* W1, const cd* e_inc, double* REF, double* TRN);
 // Out: S[inst] = [Tdd|Tuu|Rud|Rdu] (4 N x N); optional W0/V0/gam0/W1/V1/gam1 per instance.
 // With a tape, the eigen-systems of every instance are kept for the reverse pass.
+// flags (device, one int per instance, zeroed by the caller, or NULL): H_EIG for an instance
+// whose eigensolves failed, H_NONFINITE for one whose S-matrix holds NaN or infinity.
 void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int chunk, cd* S_out,
-                      cd* W0, cd* V0, cd* g0, cd* W1, cd* V1, cd* g1, StackTape* tape = nullptr);
+                      cd* W0, cd* V0, cd* g0, cd* W1, cd* V1, cd* g1, StackTape* tape = nullptr,
+                      int* flags = nullptr);
 
 }  // namespace vrc
