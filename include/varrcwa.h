@@ -172,6 +172,21 @@ int vrc_layer_smatrix(vrc_ctx* ctx, int N, int batch, const double* lambda_host,
 int vrc_star_product(vrc_ctx* ctx, int N, int batch, void* Tdd, void* Tuu, void* Rud, void* Rdu,
                      const void* tdd, const void* tuu, const void* rud, const void* rdu, long slice_stride);
 
+/* Star product without device allocation. No counterpart in the reference: this is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same arguments, kernels and results
+ * as vrc_star_product, but every temporary comes from a fixed device array of
+ * VRC_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on until the
+ * process ends; the call makes no hipMalloc/hipFree. Limit: the temporaries, about
+ * 7 N^2 batch complex values, must fit (N = 882 up to batch 1, N = 450 up to batch 5,
+ * N = 242 up to batch 19); larger requests return -1 with the size in the error text.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, contexts or threads) race and must be serialized by the caller. */
+#define VRC_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int vrc_star_product_noalloc(vrc_ctx* ctx, int N, int batch, void* Tdd, void* Tuu, void* Rud, void* Rdu,
+                             const void* tdd, const void* tuu, const void* rud, const void* rdu,
+                             long slice_stride);
+
 /* (d) End projection of the left side from basis (W1, V1) to (W0, V0)
  * (src/core/RCWAIntegrator.cpp:73-86; reprojectLeftEnd,
  * src/gpu/RedhefferIntegratorGPU.cpp:500-627). */

@@ -77,6 +77,7 @@ def load():
         "vrc_eig_pq": (i, vp, i, i, vp, vp, l, vp, vp, vp, vp),
         "vrc_layer_smatrix": (i, vp, i, i, c_double_p, c_double_p, vp, vp, vp, vp, vp, l, vp, vp),
         "vrc_star_product": (i, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, l),
+        "vrc_star_product_noalloc": (i, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, l),
         "vrc_project_left": (i, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, l),
         "vrc_redheffer_segment": (i, vp, i, d, d, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                   vp, vp, vp, c_double_p),

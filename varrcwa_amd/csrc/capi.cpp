@@ -50,6 +50,7 @@ int vrc_ctx_create(int device, void* stream, vrc_ctx** out) {
   // with every other default-stream user in the process (e.g. torch's default stream).
   int rc = guarded(c, [&] {
     c->stream = (hipStream_t)stream;
+    noalloc_init_device(device);
     VRC_HIP(hipMalloc(&c->health, sizeof(int) * HEALTH_INTS));
     fill_int(c->stream, c->health, 0, HEALTH_INTS);
     VRC_HIP(hipStreamSynchronize(c->stream));

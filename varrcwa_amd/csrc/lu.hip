@@ -631,6 +631,26 @@ void zgetrs_pair(hipStream_t s, int n, int nrhs, const cd* LU, long lda, long sA
   trsm_lu_factor(s, 1, n, 2 * nrhs, LU, lda, sA, Tb.as<cd>(), n, sT, batch, ws, X, ldx, sX);
 }
 
+static size_t ws_round(size_t b) { return b == 0 ? 256 : (b + 255) & ~(size_t)255; }
+
+// Largest workspace zgetrf holds at once: the interchange plan, plus the diagonal-block
+// inverses and staging of the first block's U12 solve (the widest).
+size_t zgetrf_scratch_bytes(int n, int batch) {
+  size_t b = ws_round(sizeof(int) * (size_t)(1 + 2 * LASWP_MAXROW) * batch);
+  if (n > LU_NB2)
+    b += ws_round(sizeof(cd) * (size_t)2 * TRS_NB * TRS_NB * batch) +
+         ws_round(sizeof(cd) * (size_t)TRS_NB * (n - LU_NB2) * batch);
+  return b;
+}
+
+// Largest workspace zgetrs_pair holds at once: the permutation, the forward-substitution
+// result and the diagonal-block inverses.
+size_t zgetrs_pair_scratch_bytes(int n, int nrhs, int batch) {
+  const size_t nblk = (size_t)cdiv_up(n, TRS_NB);
+  return ws_round(sizeof(int) * (size_t)n * batch) + ws_round(sizeof(cd) * (size_t)n * 2 * nrhs * batch) +
+         ws_round(sizeof(cd) * nblk * TRS_NB * TRS_NB * batch);
+}
+
 void zgetri(hipStream_t s, int n, const cd* LU, long lda, long sA, const int* ipiv, long sP, cd* Ainv, long ldi,
             long sI, int batch, Workspace& ws) {
   DevBuf idb = ws.get(sizeof(cd) * (size_t)n * n * batch);

@@ -629,6 +629,16 @@ void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd
   zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, tuu, N, ss, Yt, N, s2, ZERO, S.Tuu, N, NN, batch);
 }
 
+// Peak workspace bytes of star_product (without the saved intermediates): its own five
+// N x N blocks, pivots and info, plus the larger of the LU's and the solve's peaks.
+size_t star_product_scratch_bytes(int N, int batch) {
+  auto r = [](size_t b) { return b == 0 ? (size_t)256 : (b + 255) & ~(size_t)255; };
+  const size_t B = sizeof(cd) * (size_t)N * N * batch;
+  const size_t own = r(B) + r(2 * B) + r(B) + r(B) + r(sizeof(int) * (size_t)N * batch) + r(sizeof(int) * (size_t)batch);
+  const size_t f = zgetrf_scratch_bytes(N, batch), g = zgetrs_pair_scratch_bytes(N, N, batch);
+  return own + (f > g ? f : g);
+}
+
 // ===================================================================== plane-wave efficiencies
 // Per instance b and harmonic i (RCWASolver::scatterPlaneWave, src/rcwa/RCWASolver.cpp:1671-1732):
 // Kz = sqrt(k0^2 eps - kx^2 - ky^2), Ez = (kx Ex + ky Ey) / (-/+ Kz) for the reflected /

@@ -38,10 +38,22 @@ class Workspace {
   size_t reserved() const { return reserved_; }
   void trim();  // release cached blocks (caller must have synchronized the stream)
 
+  // Arena mode: between arena_begin and arena_end every get() is carved from the
+  // caller's region [base, base + bytes) as a stack (a block is reclaimed once it and
+  // every block above it are released) and nothing is allocated; a request that does
+  // not fit throws ArgFailure. Blocks of the region returned after arena_end are
+  // ignored. arena_end returns the high-water mark in bytes.
+  void arena_begin(void* base, size_t bytes);
+  size_t arena_end();
+
  private:
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> owned_;
   size_t reserved_ = 0;
+  char* arena_ = nullptr;       // active region (nullptr: pool mode)
+  char* arena_lo_ = nullptr;    // last region used, for put() after arena_end
+  size_t arena_bytes_ = 0, arena_top_ = 0, arena_peak_ = 0;
+  std::vector<std::pair<size_t, bool>> arena_blocks_;  // (offset, released), bottom to top
 };
 
 inline DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
@@ -97,6 +109,9 @@ void ztrsm_upper(hipStream_t s, int n, int nrhs, const cd* U, long ldu, long sU,
                  int batch, Workspace& ws);
 void zgetri(hipStream_t s, int n, const cd* LU, long lda, long sA, const int* ipiv, long sP, cd* Ainv, long ldi,
             long sI, int batch, Workspace& ws);
+// Peak workspace bytes of zgetrf and zgetrs_pair for these sizes.
+size_t zgetrf_scratch_bytes(int n, int batch);
+size_t zgetrs_pair_scratch_bytes(int n, int nrhs, int batch);
 
 // ---------------------------------------------------------------- eig.hip
 // Right eigen-decomposition of a strided batch of general n x n matrices

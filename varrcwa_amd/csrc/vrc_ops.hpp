@@ -15,6 +15,9 @@
 namespace vrc {
 
 void fill_int(hipStream_t s, int* p, int v, long n);
+// Looks up the device address of the allocation-free entry point's scratch array on this
+// device (current) once per device.
+void noalloc_init_device(int device);
 
 // ---------------------------------------------------------------- numerical failures
 // (codes and the per-launch notes in vrc_internal.hpp)
@@ -159,6 +162,8 @@ void evenodd_combine(hipStream_t s, int N, int batch, const cd* E, cd* Tuu, cd* 
 constexpr int STAR_SAVE = 6;
 void star_product(vrc_ctx& c, int N, int batch, SMat& S, const cd* tdd, const cd* tuu, const cd* rud,
                   const cd* rdu, long ss, cd* save = nullptr);
+// Peak workspace bytes of star_product without save.
+size_t star_product_scratch_bytes(int N, int batch);
 void project_left(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LU1, const cd* W0, const cd* V0, long sRef);
 // project_left for homogeneous end media (W0 = W1 = I; V0, V1 couple only harmonic pairs).
 void project_left_homog(vrc_ctx& c, int N, int batch, SMat& S, const cd* V0, const cd* V1, long sRef);

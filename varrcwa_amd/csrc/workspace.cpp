@@ -10,6 +10,16 @@ Workspace::~Workspace() {
 DevBuf Workspace::get(size_t bytes) {
   if (bytes == 0) bytes = 256;
   bytes = (bytes + 255) & ~(size_t)255;
+  if (arena_) {
+    if (bytes > arena_bytes_ - arena_top_)
+      throw ArgFailure("scratch region exhausted: a request of " + std::to_string(bytes) + " bytes with " +
+                       std::to_string(arena_top_) + " of " + std::to_string(arena_bytes_) + " bytes in use");
+    void* p = arena_ + arena_top_;
+    arena_blocks_.push_back({arena_top_, false});
+    arena_top_ += bytes;
+    if (arena_top_ > arena_peak_) arena_peak_ = arena_top_;
+    return DevBuf(p, bytes, this);
+  }
   auto it = free_.lower_bound(bytes);
   if (it != free_.end() && it->first <= bytes + bytes / 2 + (1u << 20)) {
     void* p = it->second;
@@ -37,7 +47,36 @@ DevBuf Workspace::get(size_t bytes) {
 }
 
 void Workspace::put(void* p, size_t bytes) {
-  if (p) free_.emplace(bytes, p);
+  if (!p) return;
+  if (arena_lo_ && (char*)p >= arena_lo_ && (char*)p < arena_lo_ + arena_bytes_) {
+    if (!arena_) return;  // released after arena_end: the region is not pooled
+    const size_t off = (size_t)((char*)p - arena_);
+    for (auto it = arena_blocks_.rbegin(); it != arena_blocks_.rend(); ++it)
+      if (it->first == off) {
+        it->second = true;
+        break;
+      }
+    while (!arena_blocks_.empty() && arena_blocks_.back().second) {
+      arena_top_ = arena_blocks_.back().first;
+      arena_blocks_.pop_back();
+    }
+    return;
+  }
+  free_.emplace(bytes, p);
+}
+
+void Workspace::arena_begin(void* base, size_t bytes) {
+  arena_ = arena_lo_ = (char*)base;
+  arena_bytes_ = bytes;
+  arena_top_ = arena_peak_ = 0;
+  arena_blocks_.clear();
+}
+
+size_t Workspace::arena_end() {
+  arena_ = nullptr;
+  arena_blocks_.clear();
+  arena_top_ = 0;
+  return arena_peak_;
 }
 
 void Workspace::trim() {
