@@ -207,6 +207,58 @@ def adjoint_bench(ctx, nh: int, insts):
             "stage_ms_forward_and_reverse": {k: v[0] for k, v in st.items()}}
 
 
+def varrcwa_bench(ctx, lams):
+    """RedhefferIntegrator on the PML mode converter (15 x 15 harmonics, N = 450, pe = 1e-2):
+    one wavelength through compute(), then the wavelength sweep through compute_many()."""
+    import torch
+    from tests import geometries as G
+    from varrcwa_amd import api
+    g = G.mode_converter_c2()
+
+    def integ(lam):
+        gg = dict(g, lam=lam)
+        return api.RedhefferIntegrator(gg["z0"], gg["z1"], G.gpu_sampler(gg, ctx))
+
+    one = integ(lams[0])
+    one.compute(0, 10, 1e-2)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    one.compute(0, 10, 1e-2)
+    t1 = time.perf_counter()
+    its = [integ(l) for l in lams]
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    api.RedhefferIntegrator.compute_many(its, 0, 10, 1e-2)
+    t3 = time.perf_counter()
+    segs = [it.N() for it in its]
+    log(f"varRCWA C2: one wavelength {1e3 * (t1 - t0):.0f} ms ({one.N()} segments); "
+        f"{len(lams)} wavelengths in one call {1e3 * (t3 - t2):.0f} ms")
+    return {"workload": "PML mode converter, 15x15 harmonics (N=450), pe=1e-2", "single_ms": 1e3 * (t1 - t0),
+            "single_segments": one.N(), "sweep_wavelengths": len(lams), "sweep_ms": 1e3 * (t3 - t2),
+            "sweep_segments": segs, "sweep_over_single_calls": (t1 - t0) * len(lams) / (t3 - t2)}
+
+
+def mixed_bench(ctx, count: int):
+    """The mixed-order sweep: instances of the workload with harmonics per instance from
+    {7, 11, 15, 21} squared (c3_orders of the geometry file), one batched call."""
+    import torch
+    from varrcwa_amd import api
+    orders = load_geometry()["c3_orders"]
+    insts = load_instances(count)
+    harm = [(orders[i], orders[i]) for i in range(count)]
+    api.batch_stack_rcwa_mixed(ctx, harm, insts[:16])  # warm the small classes
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    api.batch_stack_rcwa_mixed(ctx, harm, insts)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    L = len(insts[0]["layers"])
+    from collections import Counter
+    log(f"mixed orders: {count} instances ({dict(Counter(orders[:count]))}) in {t:.2f} s")
+    return {"instances": count, "slices": count * L, "seconds": t, "slices_per_s": count * L / t,
+            "orders": {str(k): v for k, v in sorted(Counter(orders[:count]).items())}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +272,9 @@ def main():
     ap.add_argument("--cpu", type=int, default=1, help="time the CPU oracle on instance 0 and check parity (0 = skip)")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--adjoint", type=int, default=21, help="instances in the adjoint timing (0 = skip)")
+    ap.add_argument("--varrcwa", type=int, default=4, help="wavelengths of the varRCWA sweep leg (0 = skip)")
+    ap.add_argument("--mixed", type=int, default=512, help="instances of the mixed-order leg (0 = skip)")
+    ap.add_argument("--graph", type=int, default=1, help="also time replays of the recorded step (0 = skip)")
     ap.add_argument("--tune", action="append", default=[], help="kernel switch key=value (vrc_tune_set)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first copy this many bytes device to device (a plain copy kernel of known size "
@@ -285,6 +340,13 @@ def main():
     adj = None
     if args.adjoint > 0:
         adj = adjoint_bench(ctx, nh, insts[:args.adjoint])
+    var = None
+    if args.varrcwa > 0:
+        var = varrcwa_bench(ctx, list(np.linspace(1.50, 1.60, args.varrcwa)))
+    mixed = None
+    if args.mixed > 0:
+        mixed = mixed_bench(ctx, args.mixed)
+    ctx.check(ctx.lib.vrc_pool_trim(ctx.h), "pool_trim")
 
     # ---- batched: at most one warm-up step, then timed steps within the budget
     ctx.check(ctx.lib.vrc_timing_enable(ctx.h, 1), "timing")
@@ -301,12 +363,15 @@ def main():
     ctx.check(ctx.lib.vrc_qr_stats(ctx.h, qs), "qr_stats")  # reset
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    # with graph replays to time as well, the eager loop takes half of what is left
+    left = args.budget - (t0 - T_START)
+    eager_end = args.budget - (0.5 * left if args.graph else 0.0)
     steps = 0
     out = None
     while steps < max(1, args.steps):
         if steps >= 1:
             est = (time.perf_counter() - t0) / steps
-            if time.perf_counter() - T_START + 1.1 * est > args.budget:
+            if time.perf_counter() - T_START + 1.1 * est > eager_end:
                 break
         out = None  # the previous step's S-matrices are released before the next step allocates
         out = api.batch_stack_rcwa(ctx, nh, nh, insts, chunk=args.chunk)
@@ -323,8 +388,8 @@ def main():
     log(f"batched: {slices} slices in {dt:.2f} s ({steps} steps) -> {value:.2f} slices/s")
 
     parity = None
+    S0 = out["S"][0].clone()
     if ref is not None:
-        S0 = out["S"][0].clone()
         del out
         b = api.batch_stack_rcwa(ctx, nh, nh, insts[:1], want_bases=True)
         err, err0 = parity_instance0(ref, S0, b["W0"][0], b["V0"][0], b["gam0"][0], b["W1"][0], b["V1"][0],
@@ -333,8 +398,41 @@ def main():
                   "instance": 0, "measure": "basis-invariant transforms W1 Tuu W0^-1, W0 Tdd W1^-1, W0 Rdu W0^-1, "
                                             "W1 Rud W1^-1 of the last timed step's instance 0; max|diff|/max|ref|"}
         log(f"parity: {parity['max_rel_err_vs_oracle']:.2e} (zeroth order {err0:.2e})")
+        del b
     else:
         del out
+
+    graph = None
+    if args.graph:
+        ctx.check(ctx.lib.vrc_timing_enable(ctx.h, 0), "timing")
+        torch.cuda.synchronize()
+        tr0 = time.perf_counter()
+        plan = api.BatchPlan(ctx, nh, nh, insts, chunk=args.chunk)
+        t_rec = time.perf_counter() - tr0
+        log(f"recorded the step as a graph in {t_rec:.1f} s")
+        tg0 = time.perf_counter()
+        reps = 0
+        gout = None
+        while reps < max(1, args.steps):
+            if reps >= 1:
+                est = (time.perf_counter() - tg0) / reps
+                if time.perf_counter() - T_START + 1.1 * est > args.budget:
+                    break
+            gout = plan.launch()
+            reps += 1
+            log(f"graph replay {reps} done ({time.perf_counter() - tg0:.1f} s so far)")
+        tg = time.perf_counter() - tg0
+        dev0 = float((gout["S"][0] - S0).abs().max() / S0.abs().max())
+        graph = {"replays": reps, "seconds": tg, "slices_per_s": reps * len(insts) * L / tg,
+                 "record_s": t_rec, "over_eager": reps * len(insts) * L / tg / value,
+                 "instance0_rel_dev_from_eager": dev0,
+                 "definition": "the whole step (uploads, every kernel, per-instance status read-back) recorded "
+                               "once as a HIP graph and replayed; replays include the first"}
+        log(f"graph replays: {graph['slices_per_s']:.2f} slices/s ({reps} replays), "
+            f"instance 0 vs eager {dev0:.1e}")
+        del gout
+        plan.close()
+    del S0
 
     # dominant stage (stage 1 includes stage 0; report exclusive times)
     excl = {k: v[0] for k, v in st.items()}
@@ -380,6 +478,9 @@ def main():
         "reference_equivalent_definition": "the reference operation sequence's 305 N^3 flops per slice times the "
                                            "measured rate; this build does fewer flops per slice (closed-form end "
                                            "media, even/odd slab systems, factored star product)",
+        "graph_replay": graph,
+        "varrcwa": var,
+        "mixed_orders": mixed,
         "plain_slices_per_s": plain,
         "batched_over_plain": (value / plain) if plain else None,
         "batched_over_cpu": (value / cpu["value"]) if cpu else None,

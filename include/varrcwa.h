@@ -72,6 +72,22 @@ int vrc_timing_enable(vrc_ctx* ctx, int on);
  * Returns -1 for an unknown key. */
 int vrc_tune_set(const char* key, long long value);
 int vrc_timing_read(vrc_ctx* ctx, double* ms, int* counts, int nstages);
+/* Autotuner (no reference counterpart). vrc_autotune_stack times vrc_batch_stack_rcwa on
+ * the given instances (a sample of the batch to be solved; a few hundred slabs make full
+ * launches) under candidate kernel switch settings, one switch at a time (the QR
+ * iteration's deflation window and bulge count, the ZGEMM tile shape, wavefronts and
+ * large-tile threshold), keeping a candidate only when it is >= 1 % faster, until
+ * budget_s seconds have been spent. The winners are kept for N = 2 nx ny and put in
+ * place before every later call that solves N x N systems (other N keep the values set
+ * by vrc_tune_set); with cache_path the entry (and the report as a comment) is appended
+ * to that text file. report (optional) receives a one-line summary. Synchronizes.
+ * vrc_autotune_load applies a cache file's entries (returns how many, -1 on error);
+ * vrc_autotune_clear forgets every entry. The switches are process-wide. */
+struct vrc_stack_instance;
+int vrc_autotune_stack(vrc_ctx* ctx, int nx, int ny, int ninst, const struct vrc_stack_instance* inst,
+                       double budget_s, const char* cache_path, char* report, int report_len);
+int vrc_autotune_load(const char* path);
+int vrc_autotune_clear(void);
 /* QR-iteration statistics gathered while timing is enabled, then reset: out[6] =
  * {matrices, sweeps, windows, window-chase ticks, off-window-apply ticks,
  * small-block ticks} (ticks of the device's constant-rate clock, 100 MHz). */
@@ -310,6 +326,13 @@ int vrc_rcwa_compute_edges(vrc_integrator* it, int nedges, const double* z_edges
  * interval; "redheffer.max_depth" of vrc_tune_set), accepts the segment and returns
  * status 5. */
 int vrc_redheffer_compute(vrc_integrator* it, int p, int maxN, double pe);
+/* RedhefferIntegrator::compute for several integrators at once (a wavelength or geometry
+ * sweep: one sampler each, one context, equal N). Results per integrator are those of
+ * vrc_redheffer_compute; the subdivision trees grow level by level, and every new
+ * reference eigensystem of a level, across segments and integrators, is solved in one
+ * batched launch (the reference solves them one at a time, depth first:
+ * src/core/RedhefferIntegrator.cpp:279-313). Status as vrc_redheffer_compute. */
+int vrc_redheffer_compute_many(int count, vrc_integrator* const* its, int p, int maxN, double pe);
 /* DifferentialIntegrator::compute(N) (src/core/DifferentialIntegrator.cpp:21-134). */
 int vrc_differential_compute(vrc_integrator* it, int N);
 /* Getters Tdd/Tuu/Rud/Rdu/W0/V0/gam0/W1/V1/gam1 (src/core/RCWAIntegrator.h:28-38):
@@ -347,6 +370,34 @@ typedef struct vrc_stack_instance {
 int vrc_batch_stack_rcwa(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
                          void* S_out, void* W0, void* V0, void* gam0, void* W1, void* V1, void* gam1,
                          int* info_out);
+
+/* vrc_batch_stack_rcwa with a harmonic count per instance (no reference counterpart; the
+ * reference solves one instance at a time, src/gpu/RCWAIntegratorGPU.cpp:191-232):
+ * instance i uses nx[i] x ny[i] harmonics, N_i = 2 nx[i] ny[i]. Outputs are packed in
+ * instance order: instance i's [Tdd|Tuu|Rud|Rdu] starts at complex offset
+ * 4 * sum_{j<i} N_j^2 of S_out, its W0/V0/W1/V1 at sum_{j<i} N_j^2 of those arrays and
+ * its gam0/gam1 at sum_{j<i} N_j (optional outputs may be NULL). Instances of one
+ * harmonic class are solved together (largest class first), with the launches of
+ * vrc_batch_stack_rcwa; results per instance equal that call's. chunk, info_out and
+ * the status as vrc_batch_stack_rcwa. */
+int vrc_batch_stack_rcwa_mixed(vrc_ctx* ctx, int ninst, const int* nx, const int* ny, const vrc_stack_instance* inst,
+                               int chunk, void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1,
+                               int* info_out);
+
+/* Recorded batches (no reference counterpart): vrc_batch_plan_create records the work of
+ * vrc_batch_stack_rcwa(ctx, nx, ny, ninst, inst, chunk, S_out, W0, ..., g1) once into a
+ * HIP graph (nothing runs yet); each vrc_batch_plan_launch replays it on the context's
+ * stream, solving the same instances again into the same output buffers (which must stay
+ * valid while the plan exists), then reads the per-instance codes back (synchronizes):
+ * info_out and the status as vrc_batch_stack_rcwa. The plan owns the device work space
+ * and host staging the recorded work uses, until vrc_batch_plan_destroy. Launches of one
+ * plan, and other work on the context, are ordered by the context's stream. */
+typedef struct vrc_batch_plan vrc_batch_plan;
+int vrc_batch_plan_create(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
+                          void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1,
+                          vrc_batch_plan** out);
+int vrc_batch_plan_launch(vrc_batch_plan* plan, int* info_out);
+int vrc_batch_plan_destroy(vrc_batch_plan* plan);
 /* Plane-wave diffraction efficiencies of solved stacks; replaces the power bookkeeping of
  * RCWASolver::scatterPlaneWave (src/rcwa/RCWASolver.cpp:1636-1733) for the integrator's
  * S-matrix. S, W0, W1: device outputs of vrc_batch_stack_rcwa for the same instances;

@@ -466,6 +466,17 @@ class RedhefferIntegrator : public detail::IntegratorBase {
     detail::check(vrc_redheffer_compute(it_, p, maxN, pe), ctx(), "RedhefferIntegrator::compute");
     fetch();
   }
+  // Several integrators (one context, equal N; e.g. one per wavelength) at once: the
+  // results of compute() on each, with the new reference eigensystems of each
+  // subdivision level solved in one batched launch.
+  static void computeMany(const std::vector<RedhefferIntegrator*>& its, int p, int maxN = 10, scalar pe = 1e-2) {
+    if (its.empty()) return;
+    std::vector<vrc_integrator*> h;
+    for (auto* x : its) h.push_back(x->it_);
+    detail::check(vrc_redheffer_compute_many((int)h.size(), h.data(), p, maxN, pe), its[0]->ctx(),
+                  "RedhefferIntegrator::computeMany");
+    for (auto* x : its) x->fetch();
+  }
   int N() const { return N_; }
   int nEigVal() const { return nEig_; }
 };

@@ -6,7 +6,11 @@ rectangle: width, height ~ U(0.05, 0.3) um, corner x0, y0 ~ U(0, 0.6) um, eps in
 ~ integers(0, 2) into {4.0, 12.25}. Rectangles are wrapped periodically into the
 0.6 um cell, painted in order on an eps = 1 background and stored as exact
 Layer(x, y, eps) grids. Wavelengths: linspace(0.8, 1.6, 64) um. Half-space media:
-eps = 1 above and below.
+eps = 1 above and below. Instance i of the 512-instance sweep is geometry i % 8 at
+wavelength i // 8.
+
+The mixed-order variant of the sweep gives instance i the harmonics c3_orders[i] x
+c3_orders[i], drawn with numpy.random.default_rng(100).choice([7, 11, 15, 21], 512).
 
 Writes tests/golden/metric_geometry.json (run: python tests/golden/make_metric_geometry.py).
 """
@@ -62,7 +66,8 @@ def geometry(seed):
 
 def main():
     out = dict(cell=CELL, wavelengths=np.linspace(0.8, 1.6, 64).tolist(), eps_background=1.0,
-               geometries=[geometry(s) for s in SEEDS])
+               geometries=[geometry(s) for s in SEEDS],
+               c3_orders=[int(v) for v in np.random.default_rng(100).choice([7, 11, 15, 21], 512)])
     path = Path(__file__).with_name("metric_geometry.json")
     path.write_text(json.dumps(out))
     print(f"wrote {path}")

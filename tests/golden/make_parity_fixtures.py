@@ -13,6 +13,8 @@ for varRCWA, the accepted segments in cascade order. Scenarios:
   var12_25x21         the reference's PML waveguide taper at its own 25 x 21 harmonics
                       (src/tests/test_cpu.cpp:128-176), RCWAIntegrator with 4 slices
   c2_converter        PML mode converter, 15 x 15 harmonics, RedhefferIntegrator at pe = 1e-2
+  c3_mixed            the mixed-order sweep's instances 0, 2, 3 and 5 (21 x 21, 7 x 7, 15 x 15
+                      and 11 x 11 harmonics: one of each class), keys prefixed "i<instance>_"
 
 Run in the build container: python tests/golden/make_parity_fixtures.py [name ...]
 (minutes of CPU; the GPU tests only read the .npz files).
@@ -55,7 +57,7 @@ def reduce_result(r, nx, ny, seed=0):
     return out
 
 
-def metric(i):
+def metric(i, nh=21):
     d = json.loads((OUT / "metric_geometry.json").read_text())
     cell = d["cell"]
     geoms, lams = d["geometries"], d["wavelengths"]
@@ -64,8 +66,21 @@ def metric(i):
     air = G.homogeneous(cell, 1.0)
     layers = [(np.array(l["x"]), np.array(l["y"]), np.array(l["eps"], dtype=complex)) for l in g["layers"]]
     ze = np.concatenate([[0.0], np.cumsum(g["thickness"])])
-    s = O.StackLayerSampler(lam, 21, 21, layers, ze, air, air)
-    return reduce_result(O.rcwa_integrate(s, ze[0], ze[-1], len(layers), z_edges=ze), 21, 21)
+    s = O.StackLayerSampler(lam, nh, nh, layers, ze, air, air)
+    return reduce_result(O.rcwa_integrate(s, ze[0], ze[-1], len(layers), z_edges=ze), nh, nh)
+
+
+C3_INSTANCES = (0, 2, 3, 5)
+
+
+def c3():
+    orders = json.loads((OUT / "metric_geometry.json").read_text())["c3_orders"]
+    out = {}
+    for i in C3_INSTANCES:
+        for k, v in metric(i, orders[i]).items():
+            out[f"i{i}_{k}"] = v
+    out["N"] = np.array(max(int(out[f"i{i}_N"]) for i in C3_INSTANCES))
+    return out
 
 
 def c1():
@@ -86,7 +101,7 @@ def c2():
 
 
 SCENARIOS = {"metric_inst0": lambda: metric(0), "metric_inst511": lambda: metric(511), "c1_pillar": c1,
-             "var12_25x21": var12, "c2_converter": c2}
+             "var12_25x21": var12, "c2_converter": c2, "c3_mixed": c3}
 
 
 def main(names):

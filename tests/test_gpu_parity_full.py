@@ -9,7 +9,9 @@ workload define, against fixtures the oracle wrote in the build container
 * the reference's PML waveguide taper var12 at its own 25 x 21 harmonics
   (src/tests/test_cpu.cpp:128-176), N = 1050, through RCWAIntegrator;
 * the PML mode converter (15 x 15 harmonics, N = 450) through RedhefferIntegrator, which
-  must take the oracle's subdivision tree segment for segment.
+  must take the oracle's subdivision tree segment for segment;
+* the mixed-order sweep (harmonics per instance from {7, 11, 15, 21} squared) in one
+  vrc_batch_stack_rcwa_mixed call.
 
 Bar: 512 seeded entries of each basis-invariant transform (W1 Tuu W0^-1, W0 Tdd W1^-1,
 W0 Rdu W0^-1, W1 Rud W1^-1) within 1e-9 of the transform's largest modulus, the zeroth
@@ -128,3 +130,24 @@ def test_c2_converter_varrcwa_same_tree(ctx):
     assert tree.shape == fx["tree"].shape, (tree.shape, fx["tree"].shape)
     assert np.abs(tree - fx["tree"]).max() < 1e-12
     compare(integrator_result(it), fx, "C2 converter")
+
+
+def test_c3_mixed_orders_one_call(ctx):
+    """The mixed-order sweep (orders per instance from {7, 11, 15, 21}): instances 0..5 in
+    one vrc_batch_stack_rcwa_mixed call, the four with fixtures (one per harmonic class)
+    against the oracle, each also equal to the single-class call on its own."""
+    fx = fixture("c3_mixed")
+    orders = json.loads((GOLDEN / "metric_geometry.json").read_text())["c3_orders"]
+    insts = metric_instances(0, 6)
+    harm = [(orders[i], orders[i]) for i in range(6)]
+    out = api.batch_stack_rcwa_mixed(ctx, harm, insts, want_bases=True)
+    for i in (0, 2, 3, 5):
+        sub = {k[len(f"i{i}_"):]: v for k, v in fx.items() if k.startswith(f"i{i}_")}
+        S = from_dev_colmajor(out["S"][i])
+        got = O.SResult(S[0], S[1], S[2], S[3], from_dev_colmajor(out["W0"][i]), from_dev_colmajor(out["V0"][i]),
+                        out["gam0"][i].cpu().numpy(), from_dev_colmajor(out["W1"][i]),
+                        from_dev_colmajor(out["V1"][i]), out["gam1"][i].cpu().numpy())
+        assert got.Tdd.shape[0] == 2 * orders[i] ** 2
+        compare(got, sub, f"C3 instance {i} ({orders[i]}x{orders[i]})")
+        one = from_dev_colmajor(api.batch_stack_rcwa(ctx, orders[i], orders[i], [insts[i]])["S"][0])
+        assert np.abs(one - S).max() <= 1e-12 * np.abs(one).max()

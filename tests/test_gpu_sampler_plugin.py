@@ -79,9 +79,14 @@ def test_python_user_sampler_drives_all_integrators(ctx, on_gpu):
     ref = O.redheffer_integrate(G.oracle_sampler(g), g["z0"], g["z1"], pe)
     assert it.N() == ref.N
     close(result(it), ref)
-    it = api.DifferentialIntegrator(g["z0"], g["z1"], s)
-    it.compute(8)
-    close(result(it), O.differential_integrate(G.oracle_sampler(g), g["z0"], g["z1"], 8))
+    # the differential method in its well-conditioned regime (3 x 3 harmonics, 32 steps): at
+    # 5 x 5 harmonics and 8 steps the oracle itself moves by 1e-3 under 1e-15 relative
+    # perturbations of P and Q (evanescent growth over one step amplifies rounding)
+    g1 = G.metasurface(1)
+    s1 = PillarSampler(1, on_gpu, ctx)
+    it = api.DifferentialIntegrator(g1["z0"], g1["z1"], s1)
+    it.compute(32)
+    close(result(it), O.differential_integrate(G.oracle_sampler(g1), g1["z0"], g1["z1"], 32))
 
 
 def test_python_user_sampler_errors(ctx):

@@ -98,11 +98,21 @@ def load():
         "vrc_rcwa_compute": (i, vp, i),
         "vrc_rcwa_compute_edges": (i, vp, i, c_double_p),
         "vrc_redheffer_compute": (i, vp, i, i, d),
+        "vrc_redheffer_compute_many": (i, i, C.POINTER(vp), i, i, d),
         "vrc_differential_compute": (i, vp, i),
         "vrc_integrator_get": (i, vp, i, c_double_p),
         "vrc_integrator_counts": (i, vp, c_int_p, c_int_p),
         "vrc_integrator_tree": (i, vp, i, c_double_p),
         "vrc_batch_stack_rcwa": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp, c_int_p),
+        "vrc_batch_stack_rcwa_mixed": (i, vp, i, c_int_p, c_int_p, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp,
+                                       vp, vp, vp, c_int_p),
+        "vrc_batch_plan_create": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp,
+                                  C.POINTER(vp)),
+        "vrc_batch_plan_launch": (i, vp, c_int_p),
+        "vrc_autotune_stack": (i, vp, i, i, i, C.POINTER(VrcStackInstance), d, C.c_char_p, C.c_char_p, i),
+        "vrc_autotune_load": (i, C.c_char_p),
+        "vrc_autotune_clear": (i,),
+        "vrc_batch_plan_destroy": (i, vp),
         "vrc_stack_efficiencies": (i, vp, i, i, i, C.POINTER(VrcStackInstance), vp, vp, vp, vp, vp, vp),
         "vrc_wave_eqn_derivatives": (i, vp, i, i, C.POINTER(VrcLayer), d, i, C.POINTER(VrcLayer), i, vp, vp, i, vp,
                                      vp, vp, vp, vp, vp, vp),

@@ -439,6 +439,18 @@ class RedhefferIntegrator(_Integrator):
         self.ctx.check(self.ctx.lib.vrc_redheffer_compute(self.h, int(p), int(maxN), float(pe)),
                        "RedhefferIntegrator.compute")
 
+    @staticmethod
+    def compute_many(integrators, p: int, maxN: int = 10, pe: float = 1e-2):
+        """compute(p, maxN, pe) of several integrators (one context, equal N) in one call;
+        each new reference eigensystem level is solved in one batched launch."""
+        its = list(integrators)
+        if not its:
+            return
+        arr = (C.c_void_p * len(its))(*[it.h for it in its])
+        ctx = its[0].ctx
+        ctx.check(ctx.lib.vrc_redheffer_compute_many(len(its), arr, int(p), int(maxN), float(pe)),
+                  "RedhefferIntegrator.compute_many")
+
     def tree(self):
         n = self.ctx.lib.vrc_integrator_tree(self.h, 0, None)
         buf = np.empty(2 * max(n, 1), np.float64)
@@ -541,6 +553,102 @@ def batch_stack_rcwa(ctx: Context, nx: int, ny: int, instances, chunk: int = 0, 
     if return_info:
         out["info"] = info[:len(instances)].copy()
     del keep
+    return out
+
+
+class BatchPlan:
+    """A recorded batch (vrc_batch_plan_*): the work of batch_stack_rcwa(ctx, nx, ny,
+    instances, chunk, want_bases) recorded once as a HIP graph; launch() replays it into
+    the plan's output tensors (returned as the same dict batch_stack_rcwa gives)."""
+
+    def __init__(self, ctx: Context, nx: int, ny: int, instances, chunk: int = 0, want_bases: bool = False):
+        torch = ctx.torch
+        self.ctx = ctx
+        self.ninst = len(instances)
+        N = 2 * nx * ny
+        dev = f"cuda:{ctx.device}"
+        self.out = {"S": torch.empty((self.ninst, 4, N, N), dtype=torch.complex128, device=dev)}
+        ptrs = [None] * 6
+        if want_bases:
+            for k, name in enumerate(["W0", "V0", "gam0", "W1", "V1", "gam1"]):
+                shape = (self.ninst, N) if name.startswith("gam") else (self.ninst, N, N)
+                self.out[name] = torch.empty(shape, dtype=torch.complex128, device=dev)
+                ptrs[k] = ptr(self.out[name])
+        arr, keep = _stack_array(instances)
+        h = C.c_void_p()
+        torch.cuda.synchronize()
+        ctx.check(ctx.lib.vrc_batch_plan_create(ctx.h, nx, ny, self.ninst, arr, int(chunk), ptr(self.out["S"]), *ptrs,
+                                                C.byref(h)), "batch_plan_create")
+        del keep
+        self.h = h
+
+    def launch(self, return_info: bool = False):
+        info = np.zeros(max(1, self.ninst), dtype=np.int32)
+        rc = self.ctx.lib.vrc_batch_plan_launch(self.h, info.ctypes.data_as(C.POINTER(C.c_int)))
+        if not (return_info and rc > 0):
+            self.ctx.check(rc, "batch_plan_launch")
+        out = dict(self.out)
+        if return_info:
+            out["info"] = info[:self.ninst].copy()
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.vrc_batch_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_stack_rcwa_mixed(ctx: Context, harmonics, instances, chunk: int = 0, want_bases: bool = False,
+                           return_info: bool = False):
+    """batch_stack_rcwa with a harmonic count per instance: harmonics[i] = (nx, ny).
+
+    Returns a dict with "S": a list of per-instance device tensors (4, N_i, N_i)
+    (column-major storage, views into one packed buffer), the end bases as lists when
+    requested, and "info" when return_info=True (see batch_stack_rcwa)."""
+    torch = ctx.torch
+    B = len(instances)
+    if len(harmonics) != B:
+        raise ValueError("one (nx, ny) pair per instance expected")
+    nxa = np.ascontiguousarray([int(h[0]) for h in harmonics] or [1], dtype=np.int32)
+    nya = np.ascontiguousarray([int(h[1]) for h in harmonics] or [1], dtype=np.int32)
+    Ns = [2 * int(a) * int(b) for a, b in zip(nxa[:B], nya[:B])]
+    dev = f"cuda:{ctx.device}"
+    Sflat = torch.empty(max(1, sum(4 * N * N for N in Ns)), dtype=torch.complex128, device=dev)
+    mats = {k: torch.empty(max(1, sum(N * N for N in Ns)), dtype=torch.complex128, device=dev)
+            for k in (["W0", "V0", "W1", "V1"] if want_bases else [])}
+    vecs = {k: torch.empty(max(1, sum(Ns)), dtype=torch.complex128, device=dev)
+            for k in (["gam0", "gam1"] if want_bases else [])}
+    ptrs = [ptr(mats["W0"]), ptr(mats["V0"]), ptr(vecs["gam0"]), ptr(mats["W1"]), ptr(mats["V1"]),
+            ptr(vecs["gam1"])] if want_bases else [None] * 6
+    arr, keep = _stack_array(instances)
+    info = np.zeros(max(1, B), dtype=np.int32)
+    rc = ctx.lib.vrc_batch_stack_rcwa_mixed(ctx.h, B, nxa.ctypes.data_as(C.POINTER(C.c_int)),
+                                            nya.ctypes.data_as(C.POINTER(C.c_int)), arr, int(chunk), ptr(Sflat),
+                                            *ptrs, info.ctypes.data_as(C.POINTER(C.c_int)))
+    del keep
+    if not (return_info and rc > 0):
+        ctx.check(rc, "batch_stack_rcwa_mixed")
+
+    def split(flat, per):
+        out, o = [], 0
+        for N in Ns:
+            out.append(flat[o:o + per(N)])
+            o += per(N)
+        return out
+
+    out = {"S": [t.view(4, N, N) for t, N in zip(split(Sflat, lambda N: 4 * N * N), Ns)]}
+    for k, t in mats.items():
+        out[k] = [u.view(N, N) for u, N in zip(split(t, lambda N: N * N), Ns)]
+    for k, t in vecs.items():
+        out[k] = split(t, lambda N: N)
+    if return_info:
+        out["info"] = info[:B].copy()
     return out
 
 

@@ -721,6 +721,12 @@ void zgemm_set_small_waves(int w) { g_small_waves = w == 8 ? 8 : 4; }
 static int g_big_min_k = 129;   // ... and k at least this (short-k updates run better on the 64x64 tile)
 void zgemm_set_big(bool on) { g_big_gemm = on; }
 void zgemm_set_big_min_k(int k) { g_big_min_k = k; }
+void zgemm_get_knobs(int* big_n, int* big_waves, int* small_waves, int* big_min_k) {
+  *big_n = g_big_n;
+  *big_waves = g_big_waves;
+  *small_waves = g_small_waves;
+  *big_min_k = g_big_min_k;
+}
 
 void zgemm_strided(hipStream_t s, int opA, int opB, int m, int n, int k, cd alpha, const cd* A, long lda,
                    long sA, const cd* B, long ldb, long sB, cd beta, cd* C, long ldc, long sC, int batch) {

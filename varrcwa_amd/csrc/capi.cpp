@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
 
 #include "vrc_adjoint.hpp"
@@ -86,6 +87,7 @@ int vrc_pool_trim(vrc_ctx* c) {
 
 int vrc_tune_set(const char* key, long long value) {
   if (!key) return -1;
+  tune_manual_begin();
   if (std::strcmp(key, "zgemm.large_tile") == 0) {
     zgemm_set_big(value != 0);
     return 0;
@@ -722,6 +724,23 @@ int vrc_redheffer_compute(vrc_integrator* it, int p, int maxN, double pe) {
   });
 }
 
+int vrc_redheffer_compute_many(int count, vrc_integrator* const* its, int p, int maxN, double pe) {
+  if (count < 1 || !its || !its[0]) return -1;
+  (void)p;
+  (void)maxN;
+  vrc_ctx* c = its[0]->ctx;
+  return checked(c, [&] {
+    need(pe > 0, "pe > 0 expected");
+    std::vector<Integrator*> v;
+    for (int i = 0; i < count; ++i) {
+      need(its[i] && its[i]->ctx == c, "every integrator of the batch must exist and share the first one's context");
+      need(its[i]->kind == 1, "Redheffer integrators expected");
+      v.push_back(its[i]->it.get());
+    }
+    Integrator::redheffer_many(v, pe, g_redheffer_depth);
+  });
+}
+
 int vrc_differential_compute(vrc_integrator* it, int N) {
   if (!it) return -1;
   return checked(it->ctx, [&] {
@@ -790,6 +809,232 @@ struct vrc_tape {
 int vrc_batch_stack_rcwa(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
                          void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1, int* info_out) {
   return vrc_batch_stack_forward(c, nx, ny, ninst, inst, chunk, S_out, W0, V0, g0, W1, V1, g1, nullptr, info_out);
+}
+
+int vrc_batch_stack_rcwa_mixed(vrc_ctx* c, int ninst, const int* nx, const int* ny, const vrc_stack_instance* inst,
+                               int chunk, void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1,
+                               int* info_out) {
+  int num = 0;
+  const int rc = guarded(c, [&] {
+    need(ninst >= 0 && (ninst == 0 || (nx && ny && inst && S_out)), "bad arguments");
+    if (ninst == 0) return;
+    for (int i = 0; i < ninst; ++i) check_harmonics(nx[i], ny[i]);
+    // packed offsets (complex elements) of every instance's outputs
+    std::vector<long> oS(ninst), oM(ninst), oG(ninst);
+    long aS = 0, aM = 0, aG = 0;
+    for (int i = 0; i < ninst; ++i) {
+      const long N = 2L * nx[i] * ny[i];
+      oS[i] = aS;
+      oM[i] = aM;
+      oG[i] = aG;
+      aS += 4 * N * N;
+      aM += N * N;
+      aG += N;
+    }
+    // harmonic classes, largest first
+    std::map<std::pair<int, int>, std::vector<int>> cls;
+    for (int i = 0; i < ninst; ++i) cls[{nx[i], ny[i]}].push_back(i);
+    std::vector<std::pair<std::pair<int, int>, std::vector<int>>> order(cls.begin(), cls.end());
+    std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) {
+      return a.first.first * a.first.second > b.first.first * b.first.second;
+    });
+    health_reset(*c);
+    std::vector<int> fl(ninst, 0);
+    std::string m;
+    for (auto& [hk, idx] : order) {
+      const int cx = hk.first, cy = hk.second, k = (int)idx.size();
+      const long N = 2L * cx * cy, NN = N * N;
+      std::vector<vrc_stack_instance> sub(k);
+      for (int q = 0; q < k; ++q) sub[q] = inst[idx[q]];
+      HostStack hs(k, sub.data());
+      DevBuf So = c->ws.get(sizeof(cd) * 4 * NN * k);
+      const bool wantM = W0 || V0 || W1 || V1, wantG = g0 || g1;
+      DevBuf Mo = c->ws.get(wantM ? sizeof(cd) * 4 * NN * k : 256), Go = c->ws.get(wantG ? sizeof(cd) * 2 * N * k : 256);
+      cd* M = Mo.as<cd>();
+      cd* G = Go.as<cd>();
+      DevBuf flags = c->ws.get(sizeof(int) * (size_t)k);
+      fill_int(c->stream, flags.as<int>(), 0, k);
+      batch_stack_rcwa(*c, cx, cy, hs.si, chunk, So.as<cd>(), W0 ? M : nullptr, V0 ? M + NN * k : nullptr,
+                       g0 ? G : nullptr, W1 ? M + 2 * NN * k : nullptr, V1 ? M + 3 * NN * k : nullptr,
+                       g1 ? G + N * k : nullptr, nullptr, flags.as<int>());
+      // class outputs (instance-major) into the packed per-instance layout
+      auto put = [&](void* dst, const cd* src, long per, const std::vector<long>& off) {
+        for (int q = 0; q < k; ++q)
+          VRC_HIP(hipMemcpyAsync((cd*)dst + off[idx[q]], src + per * q, sizeof(cd) * per, hipMemcpyDeviceToDevice,
+                                 c->stream));
+      };
+      put(S_out, So.as<cd>(), 4 * NN, oS);
+      if (W0) put(W0, M, NN, oM);
+      if (V0) put(V0, M + NN * k, NN, oM);
+      if (W1) put(W1, M + 2 * NN * k, NN, oM);
+      if (V1) put(V1, M + 3 * NN * k, NN, oM);
+      if (g0) put(g0, G, N, oG);
+      if (g1) put(g1, G + N * k, N, oG);
+      std::vector<int> f(k);
+      VRC_HIP(hipMemcpyAsync(f.data(), flags.ptr, sizeof(int) * k, hipMemcpyDeviceToHost, c->stream));
+      VRC_HIP(hipStreamSynchronize(c->stream));
+      for (int q = 0; q < k; ++q) fl[idx[q]] = f[q];
+    }
+    num = health_read(*c, m);  // synchronizes
+    if (info_out) std::copy(fl.begin(), fl.end(), info_out);
+    for (int i = 0; i < ninst; ++i)
+      if (fl[i]) {
+        num = fl[i];
+        m = "instance " + std::to_string(i) + ": " + health_text(fl[i]) + (m.empty() ? "" : "; " + m);
+        break;
+      }
+    if (num) c->last_error = m;
+  });
+  return rc != 0 ? rc : num;
+}
+
+// ------------------------------------------------------------------ recorded batches
+struct vrc_batch_plan {
+  vrc_ctx* ctx = nullptr;
+  int ninst = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<std::pair<void*, size_t>> dev;  // device blocks the graph reads and writes
+  std::vector<void*> pinned;                  // host staging the graph's copies read
+  int* flags = nullptr;                       // per-instance codes (one of the device blocks)
+  ~vrc_batch_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& b : dev) (void)hipFree(b.first);
+    for (void* p : pinned) (void)hipHostFree(p);
+  }
+};
+
+int vrc_batch_plan_create(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
+                          void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1,
+                          vrc_batch_plan** out) {
+  if (out) *out = nullptr;
+  return guarded(c, [&] {
+    check_harmonics(nx, ny);
+    need(out && ninst >= 1 && inst && S_out, "bad arguments");
+    HostStack hs(ninst, inst);
+    auto plan = std::make_unique<vrc_batch_plan>();
+    plan->ctx = c;
+    plan->ninst = ninst;
+    VRC_HIP(hipStreamSynchronize(c->stream));
+    hipStream_t cap = nullptr;
+    VRC_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    // the context records on the capture stream, without stage timers, its pool noting
+    // every block the recorded work touches and its uploads staged in blocks the plan keeps
+    struct Recording {
+      vrc_ctx* c;
+      hipStream_t prev, cap;
+      bool timing;
+      bool capturing = false;
+      Recording(vrc_ctx* cc, hipStream_t cs, std::vector<void*>* hold)
+          : c(cc), prev(cc->stream), cap(cs), timing(cc->timing) {
+        c->stream = cs;
+        c->timing = false;
+        c->pinned.hold = hold;
+        c->ws.track_begin();
+      }
+      ~Recording() {
+        if (capturing) {  // an error while recording: end it and drop the partial graph
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(cap, &g);
+          if (g) (void)hipGraphDestroy(g);
+        }
+        if (c->ws.tracking()) {  // an error before the plan claimed its blocks: nothing ran, free them
+          size_t leased = 0;
+          for (auto& b : c->ws.track_end(&leased)) (void)hipFree(b.first);
+        }
+        c->stream = prev;
+        c->timing = timing;
+        c->pinned.hold = nullptr;
+        (void)hipStreamSynchronize(cap);
+        (void)hipStreamDestroy(cap);
+      }
+    } rec(c, cap, &plan->pinned);
+    VRC_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
+    rec.capturing = true;
+    {
+      DevBuf flags = c->ws.get(sizeof(int) * (size_t)ninst);
+      plan->flags = flags.as<int>();
+      fill_int(cap, flags.as<int>(), 0, ninst);
+      fill_int(cap, c->health, 0, HEALTH_INTS);
+      batch_stack_rcwa(*c, nx, ny, hs.si, chunk, (cd*)S_out, (cd*)W0, (cd*)V0, (cd*)g0, (cd*)W1, (cd*)V1, (cd*)g1,
+                       nullptr, flags.as<int>());
+    }
+    rec.capturing = false;
+    VRC_HIP(hipStreamEndCapture(cap, &plan->graph));
+    size_t leased = 0;
+    plan->dev = c->ws.track_end(&leased);
+    need(leased == 0, "internal: a block of the recorded work is still leased");
+    VRC_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+    ctx_retain(c);
+    *out = plan.release();
+  });
+}
+
+int vrc_batch_plan_launch(vrc_batch_plan* p, int* info_out) {
+  if (!p) return -1;
+  vrc_ctx* c = p->ctx;
+  int num = 0;
+  const int rc = guarded(c, [&] {
+    c->host_code = 0;
+    c->host_msg.clear();
+    VRC_HIP(hipGraphLaunch(p->exec, c->stream));
+    std::vector<int> fl(p->ninst);
+    VRC_HIP(hipMemcpyAsync(fl.data(), p->flags, sizeof(int) * p->ninst, hipMemcpyDeviceToHost, c->stream));
+    std::string m;
+    num = health_read(*c, m);  // synchronizes
+    if (info_out) std::copy(fl.begin(), fl.end(), info_out);
+    for (int i = 0; i < p->ninst; ++i)
+      if (fl[i]) {
+        num = fl[i];
+        m = "instance " + std::to_string(i) + ": " + health_text(fl[i]) + (m.empty() ? "" : "; " + m);
+        break;
+      }
+    if (num) c->last_error = m;
+  });
+  return rc != 0 ? rc : num;
+}
+
+int vrc_batch_plan_destroy(vrc_batch_plan* p) {
+  if (!p) return 0;
+  vrc_ctx* c = p->ctx;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  delete p;
+  ctx_release(c);
+  return 0;
+}
+
+// ------------------------------------------------------------------ autotuner
+int vrc_autotune_stack(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, double budget_s,
+                       const char* cache_path, char* report, int report_len) {
+  return guarded(c, [&] {
+    check_harmonics(nx, ny);
+    need(ninst >= 1 && inst && budget_s > 0, "bad arguments");
+    HostStack hs(ninst, inst);
+    TuneKnobs best;
+    const std::string rep = autotune_stack(*c, nx, ny, hs.si, budget_s, &best);
+    tune_store(2 * nx * ny, best);
+    if (cache_path) tune_save(cache_path, 2 * nx * ny, best, rep);
+    if (report && report_len > 0) {
+      std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
+      report[report_len - 1] = 0;
+    }
+  });
+}
+
+int vrc_autotune_load(const char* path) {
+  if (!path) return -1;
+  try {
+    return tune_load(path);
+  } catch (...) {
+    return -1;
+  }
+}
+
+int vrc_autotune_clear(void) {
+  tune_clear();
+  return 0;
 }
 
 int vrc_modal_power(vrc_ctx* c, int nx, int ny, int nmodes, const void* W, long long ldw, const void* V,

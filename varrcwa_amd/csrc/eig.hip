@@ -1384,6 +1384,8 @@ static int g_max_sweeps = 0;  // 0: 30 * max(N, 10), LAPACK zlahqr's iteration b
 void zgeev_set_max_sweeps(int n) { g_max_sweeps = n < 0 ? 0 : n; }
 void zgeev_set_aed(int window) { g_aed_window = window < 0 ? 0 : (window > QW ? QW : window); }
 void zgeev_set_bulges(int nb) { g_bulges = nb < 1 ? 1 : (nb > NBUL ? NBUL : nb); }
+int zgeev_get_aed() { return g_aed_window; }
+int zgeev_get_bulges() { return g_bulges; }
 
 // Schur factors of a strided batch (A destroyed): A = D Z T Z^H D^{-1} with D = diag(dbal)
 // (balancing), Z unitary (ld n), T upper triangular (left in A); X (ld n, unit upper

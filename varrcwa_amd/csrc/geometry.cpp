@@ -217,8 +217,7 @@ FourierStage fourier_prepare(vrc_ctx& c, int nx, int ny, const std::vector<const
   h = host.data();
   std::memcpy(h + oc, val.data(), bytes_c);
   fs.geo = c.ws.get(host.size());
-  VRC_HIP(hipMemcpyAsync(fs.geo.ptr, host.data(), host.size(), hipMemcpyHostToDevice, s));
-  VRC_HIP(hipStreamSynchronize(s));  // the staging vector is released on return
+  c.pinned.upload(s, fs.geo.ptr, host.data(), host.size());
   char* d = (char*)fs.geo.ptr;
   FourierDev& g = fs.g;
   g.nx = nx;

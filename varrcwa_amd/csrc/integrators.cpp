@@ -21,8 +21,7 @@ void stack_set_cascade_group(int n) { g_cascade_group = n < 1 ? 1 : n; }
 
 static DevBuf upload(vrc_ctx& c, const void* host, size_t bytes) {
   DevBuf b = c.ws.get(bytes);
-  VRC_HIP(hipMemcpyAsync(b.ptr, host, bytes, hipMemcpyHostToDevice, c.stream));
-  VRC_HIP(hipStreamSynchronize(c.stream));
+  c.pinned.upload(c.stream, b.ptr, host, bytes);
   return b;
 }
 
@@ -407,155 +406,291 @@ void init_identity_S(vrc_ctx& c, int N, SMat& S, int batch) {
 }  // namespace
 
 // RCWAIntegrator::compute (src/core/RCWAIntegrator.cpp:20-87) over explicit slices
-// (midpoint z, width). batched=false: one slice at a time as the reference does;
-// batched=true: slices sampled/solved in groups, then cascaded in order.
-void Integrator::rcwa(const std::vector<std::pair<double, double>>& slices, bool batched) {
+// (midpoint z, width). The slices are independent until the cascade, so their samples,
+// eigensolves and slab S-matrices run in groups of up to RCWA_GROUP per launch (one
+// eigensolve per workgroup), the two end media riding with the first group; the star
+// products then follow in slice order, as the reference cascades them. Each matrix's
+// result does not depend on the group it is solved in.
+constexpr int RCWA_GROUP = 64;
+void Integrator::rcwa(const std::vector<std::pair<double, double>>& slices, bool) {
   vrc_ctx& c = *c_;
   const int N = Nd_;
+  tune_apply(N);
   const long NN = (long)N * N;
   tree_.clear();
-  DevBuf P = c.ws.get(sizeof(cd) * NN), Q = c.ws.get(sizeof(cd) * NN);
-  eig_at(z1_, P.as<cd>(), Q.as<cd>(), W1_.as<cd>(), V1_.as<cd>(), g1_.as<cd>());
-  LUStore lu1;
-  lu1.make(c, N, W1_.as<cd>(), V1_.as<cd>(), 1, 0);
+  const int ns = (int)slices.size();
+  const int G = std::max(1, std::min(ns, RCWA_GROUP));
+  const int GE = G + 2;  // + the two end media in the first launch
+  DevBuf Pb = c.ws.get(sizeof(cd) * NN * GE), Qb = c.ws.get(sizeof(cd) * NN * GE);
+  DevBuf Wb = c.ws.get(sizeof(cd) * NN * GE), Vb = c.ws.get(sizeof(cd) * NN * GE), gb = c.ws.get(sizeof(cd) * N * GE);
+  DevBuf tu = c.ws.get(sizeof(cd) * NN * G), ru = c.ws.get(sizeof(cd) * NN * G);
+  DevBuf infb = c.ws.get(sizeof(int) * GE);
   SMat S = this->S();
   init_identity_S(c, N, S, 1);
   const double k0 = s_->k0();
-  const int G = batched ? std::max(1, std::min((int)slices.size(), 64)) : 1;
-  DevBuf Pb = c.ws.get(sizeof(cd) * NN * G), Qb = c.ws.get(sizeof(cd) * NN * G);
-  DevBuf Wb = c.ws.get(sizeof(cd) * NN * G), Vb = c.ws.get(sizeof(cd) * NN * G), gb = c.ws.get(sizeof(cd) * N * G);
-  DevBuf tu = c.ws.get(sizeof(cd) * NN * G), ru = c.ws.get(sizeof(cd) * NN * G);
-  DevBuf infb = c.ws.get(sizeof(int) * G);
-  for (size_t i0 = 0; i0 < slices.size(); i0 += G) {
-    const int g = (int)std::min<size_t>(G, slices.size() - i0);
-    std::vector<double> zs(g), sv(g);
+  LUStore lu1;
+  for (int i0 = 0; i0 < ns; i0 += G) {
+    const int g = std::min(G, ns - i0);
+    const int e = i0 == 0 ? 2 : 0;
+    std::vector<double> zs, sv(g);
+    if (e) zs = {z1_, z0_};
     for (int j = 0; j < g; ++j) {
-      zs[j] = slices[i0 + j].first;
+      zs.push_back(slices[i0 + j].first);
       sv[j] = slices[i0 + j].second / k0;
     }
     s_->sample_gpu(zs, Pb.as<cd>(), Qb.as<cd>());
-    wave_eqn(c, N, Pb.as<cd>(), Qb.as<cd>(), NN, Wb.as<cd>(), Vb.as<cd>(), gb.as<cd>(), infb.as<int>(), g);
+    wave_eqn(c, N, Pb.as<cd>(), Qb.as<cd>(), NN, Wb.as<cd>(), Vb.as<cd>(), gb.as<cd>(), infb.as<int>(), g + e);
+    if (e) {
+      auto cp = [&](int rows, int cols, const cd* src, DevBuf& dst) {
+        zgeam(c.stream, rows, cols, ONE, src, rows, 0, ZERO, nullptr, 0, 0, dst.as<cd>(), rows, 0, 1);
+      };
+      cp(N, N, Wb.as<cd>(), W1_);
+      cp(N, N, Vb.as<cd>(), V1_);
+      cp(N, N, Wb.as<cd>() + NN, W0_);
+      cp(N, N, Vb.as<cd>() + NN, V0_);
+      cp(N, 1, gb.as<cd>(), g1_);
+      cp(N, 1, gb.as<cd>() + N, g0_);
+      lu1.make(c, N, W1_.as<cd>(), V1_.as<cd>(), 1, 0);
+    }
     DevBuf ds = upload(c, sv.data(), sizeof(double) * g);
-    layer_smatrix(c, N, g, ds.as<double>(), Wb.as<cd>(), Vb.as<cd>(), gb.as<cd>(), W1_.as<cd>(), V1_.as<cd>(), 0,
-                  tu.as<cd>(), ru.as<cd>());
+    layer_smatrix(c, N, g, ds.as<double>(), Wb.as<cd>() + e * NN, Vb.as<cd>() + e * NN, gb.as<cd>() + e * N,
+                  W1_.as<cd>(), V1_.as<cd>(), 0, tu.as<cd>(), ru.as<cd>());
     for (int j = 0; j < g; ++j) {
       const cd* t = tu.as<cd>() + j * NN;
       const cd* r = ru.as<cd>() + j * NN;
       star_product(c, N, 1, S, t, t, r, r, 0);
     }
   }
-  eig_at(z0_, P.as<cd>(), Q.as<cd>(), W0_.as<cd>(), V0_.as<cd>(), g0_.as<cd>());
   project_left(c, N, 1, S, lu1.pair, W0_.as<cd>(), V0_.as<cd>(), 0);
-  N_ = (int)slices.size();
-  nEig_ = (int)slices.size() + 2;
+  N_ = ns;
+  nEig_ = ns + 2;
   VRC_HIP(hipStreamSynchronize(c.stream));
 }
 
 // ===================================================================== varRCWA
 namespace {
-struct RefHolder {
-  DevBuf W, V, g;
+// One batched group of samples P, Q (count matrices); the first neig of them also hold
+// their eigen systems W, V, gamma and the LU factors of W and V (reference systems).
+struct Level {
+  int count = 0, neig = 0;
+  DevBuf P, Q, W, V, g, inf;
   LUStore lu;
-  RefSys sys() const {
+};
+using LevelP = std::shared_ptr<Level>;
+struct Smp {
+  LevelP lv;
+  int j = 0;
+  const cd* P(long NN) const { return lv->P.as<cd>() + j * NN; }
+  const cd* Q(long NN) const { return lv->Q.as<cd>() + j * NN; }
+};
+struct Ref {
+  LevelP lv;
+  int j = 0;
+  RefSys sys(int N) const {
+    const long NN = (long)N * N;
     RefSys r;
-    r.W = W.as<cd>();
-    r.V = V.as<cd>();
-    r.gam = g.as<cd>();
-    r.LU = lu.pair;
+    r.W = lv->W.as<cd>() + j * NN;
+    r.V = lv->V.as<cd>() + j * NN;
+    r.gam = lv->g.as<cd>() + (long)j * N;
+    r.LU = lv->lu.pair;
+    r.LU.W += j * NN;
+    r.LU.V += j * NN;
+    r.LU.pW += (long)j * N;
+    r.LU.pV += (long)j * N;
     return r;
   }
 };
-struct Sample {
-  DevBuf P, Q;
+struct Seg {
+  int inst, depth;
+  double za, zb;
+  Ref R;
+  Smp a, m, b;
 };
+struct Done {
+  double za, zb;
+  Ref R;
+  std::shared_ptr<DevBuf> T;  // Tdd, Tuu, Rud, Rdu of the segment (in the basis of R)
+};
+
+LevelP new_level(vrc_ctx& c, int N, int count) {
+  const long NN = (long)N * N;
+  auto L = std::make_shared<Level>();
+  L->count = count;
+  L->P = c.ws.get(sizeof(cd) * NN * count);
+  L->Q = c.ws.get(sizeof(cd) * NN * count);
+  return L;
+}
+
+// Eigen systems of the first neig samples of L (one batched launch) and the LU factors
+// of W and V for the first nlu of them.
+void solve_level(vrc_ctx& c, int N, Level& L, int neig, int nlu) {
+  const long NN = (long)N * N;
+  L.neig = neig;
+  if (neig <= 0) return;
+  L.W = c.ws.get(sizeof(cd) * NN * neig);
+  L.V = c.ws.get(sizeof(cd) * NN * neig);
+  L.g = c.ws.get(sizeof(cd) * (size_t)N * neig);
+  L.inf = c.ws.get(sizeof(int) * (size_t)neig);
+  wave_eqn(c, N, L.P.as<cd>(), L.Q.as<cd>(), NN, L.W.as<cd>(), L.V.as<cd>(), L.g.as<cd>(), L.inf.as<int>(), neig);
+  if (nlu > 0) L.lu.make(c, N, L.W.as<cd>(), L.V.as<cd>(), nlu, NN);
+}
 }  // namespace
 
-// RedhefferIntegrator::compute, trisection branch (src/core/RedhefferIntegrator.cpp:174-314).
-void Integrator::redheffer(double pe, int max_depth) {
-  vrc_ctx& c = *c_;
-  const int N = Nd_;
-  const long NN = (long)N * N;
-  const double k0 = s_->k0();
-  tree_.clear();
-  int* info = info_.as<int>();
-  auto sample = [&](double z) {
-    Sample s;
-    s.P = c.ws.get(sizeof(cd) * NN);
-    s.Q = c.ws.get(sizeof(cd) * NN);
-    s_->sample_gpu({z}, s.P.as<cd>(), s.Q.as<cd>());
-    return s;
-  };
-  auto make_ref = [&](const Sample& smp) {
-    auto R = std::make_unique<RefHolder>();
-    R->W = c.ws.get(sizeof(cd) * NN);
-    R->V = c.ws.get(sizeof(cd) * NN);
-    R->g = c.ws.get(sizeof(cd) * N);
-    wave_eqn(c, N, smp.P.as<cd>(), smp.Q.as<cd>(), NN, R->W.as<cd>(), R->V.as<cd>(), R->g.as<cd>(), info, 1);
-    R->lu.make(c, N, R->W.as<cd>(), R->V.as<cd>(), 1, 0);
-    return R;
-  };
-  Sample s0 = sample(z0_);
-  wave_eqn(c, N, s0.P.as<cd>(), s0.Q.as<cd>(), NN, W0_.as<cd>(), V0_.as<cd>(), g0_.as<cd>(), info, 1);
-  Sample s1 = sample(z1_);
-  wave_eqn(c, N, s1.P.as<cd>(), s1.Q.as<cd>(), NN, W1_.as<cd>(), V1_.as<cd>(), g1_.as<cd>(), info, 1);
-  Sample sm = sample(0.5 * (z0_ + z1_));
-  auto Rm = make_ref(sm);
-  SMat S = this->S();
-  init_identity_S(c, N, S, 1);
-  DevBuf leftW = c.ws.get(sizeof(cd) * NN), leftV = c.ws.get(sizeof(cd) * NN);
-  zgeam(c.stream, N, N, ONE, W0_.as<cd>(), N, 0, ZERO, nullptr, 0, 0, leftW.as<cd>(), N, 0, 1);
-  zgeam(c.stream, N, N, ONE, V0_.as<cd>(), N, 0, ZERO, nullptr, 0, 0, leftV.as<cd>(), N, 0, 1);
-  N_ = 0;
-  nEig_ = 2;
+// RedhefferIntegrator::compute, trisection branch (src/core/RedhefferIntegrator.cpp:174-314),
+// for one integrator.
+void Integrator::redheffer(double pe, int max_depth) { redheffer_many({this}, pe, max_depth); }
 
-  std::function<void(const RefSys&, const cd*, const cd*, double, double, const Sample&, const Sample&,
-                     const Sample&, int)>
-      rec = [&](const RefSys& R, const cd* Pr, const cd* Qr, double za, double zb, const Sample& a,
-                const Sample& mid, const Sample& b, int depth) {
-        DevBuf seg = c.ws.get(sizeof(cd) * 4 * NN);
-        SMat T;
-        T.Tdd = seg.as<cd>();
-        T.Tuu = T.Tdd + NN;
-        T.Rud = T.Tdd + 2 * NN;
-        T.Rdu = T.Tdd + 3 * NN;
-        const cd* P3[3] = {a.P.as<cd>(), mid.P.as<cd>(), b.P.as<cd>()};
-        const cd* Q3[3] = {a.Q.as<cd>(), mid.Q.as<cd>(), b.Q.as<cd>()};
-        const double err =
-            redheffer_perturbation(c, N, k0, za, zb, Pr, Qr, R, P3, Q3, T.Tdd, T.Tuu, T.Rud, T.Rdu);
-        if (err <= pe || depth >= max_depth) {
-          if (err > pe && !c.host_code) {
-            c.host_code = H_DEPTH;
-            c.host_msg = std::string(health_text(H_DEPTH)) + " (segment [" + std::to_string(za) + ", " +
-                         std::to_string(zb) + "], error " + std::to_string(err) + ", depth " +
-                         std::to_string(max_depth) + ")";
-          }
-          ++N_;
-          tree_.push_back({za, zb});
-          project_left(c, N, 1, T, R.LU, leftW.as<cd>(), leftV.as<cd>(), 0);
-          zgeam(c.stream, N, N, ONE, R.W, N, 0, ZERO, nullptr, 0, 0, leftW.as<cd>(), N, 0, 1);
-          zgeam(c.stream, N, N, ONE, R.V, N, 0, ZERO, nullptr, 0, 0, leftV.as<cd>(), N, 0, 1);
-          star_product(c, N, 1, S, T.Tdd, T.Tuu, T.Rud, T.Rdu, 0);
-          return;
+// The reference recurses depth first and solves each new reference eigensystem on its
+// own. Whether a segment is accepted depends only on the segment (its reference system
+// and its three samples), so the same tree grows level by level: every pending segment
+// of a level is evaluated (perturbation and error, one readback for all), the accepted
+// ones keep their S-matrices, and the rejected ones are trisected with their new
+// samples taken together and their new reference eigensystems solved in one batched
+// launch. The accepted segments are then projected and star-multiplied in z order,
+// exactly as the recursion does (src/core/RedhefferIntegrator.cpp:248-276), and the
+// right projection follows (:211-223).
+void Integrator::redheffer_many(const std::vector<Integrator*>& its, double pe, int max_depth) {
+  if (its.empty()) return;
+  vrc_ctx& c = *its[0]->c_;
+  const int N = its[0]->Nd_;
+  tune_apply(N);
+  for (auto* it : its)
+    if (it->c_ != &c || it->Nd_ != N) throw ArgFailure("integrators of one batch need one context and equal N");
+  const long NN = (long)N * N;
+  const int I = (int)its.size();
+  // ---- level 0: the midpoint references (with LU), then both ends, of every integrator
+  LevelP L0 = new_level(c, N, 3 * I);
+  for (int i = 0; i < I; ++i) {
+    Integrator& it = *its[i];
+    it.tree_.clear();
+    it.N_ = 0;
+    it.nEig_ = 2;
+    it.s_->sample_gpu({0.5 * (it.z0_ + it.z1_)}, L0->P.as<cd>() + i * NN, L0->Q.as<cd>() + i * NN);
+    it.s_->sample_gpu({it.z0_}, L0->P.as<cd>() + (I + i) * NN, L0->Q.as<cd>() + (I + i) * NN);
+    it.s_->sample_gpu({it.z1_}, L0->P.as<cd>() + (2 * I + i) * NN, L0->Q.as<cd>() + (2 * I + i) * NN);
+  }
+  solve_level(c, N, *L0, 3 * I, I);
+  std::vector<Seg> pending;
+  for (int i = 0; i < I; ++i) {
+    Integrator& it = *its[i];
+    auto cp = [&](int rows, int cols, const cd* src, DevBuf& dst) {
+      zgeam(c.stream, rows, cols, ONE, src, rows, 0, ZERO, nullptr, 0, 0, dst.as<cd>(), rows, 0, 1);
+    };
+    cp(N, N, L0->W.as<cd>() + (I + i) * NN, it.W0_);
+    cp(N, N, L0->V.as<cd>() + (I + i) * NN, it.V0_);
+    cp(N, 1, L0->g.as<cd>() + (long)(I + i) * N, it.g0_);
+    cp(N, N, L0->W.as<cd>() + (2 * I + i) * NN, it.W1_);
+    cp(N, N, L0->V.as<cd>() + (2 * I + i) * NN, it.V1_);
+    cp(N, 1, L0->g.as<cd>() + (long)(2 * I + i) * N, it.g1_);
+    pending.push_back(Seg{i, 0, it.z0_, it.z1_, Ref{L0, i}, Smp{L0, I + i}, Smp{L0, i}, Smp{L0, 2 * I + i}});
+  }
+  std::vector<std::vector<Done>> done(I);
+  while (!pending.empty()) {
+    // ---- evaluate every pending segment, one readback
+    const int P = (int)pending.size();
+    std::vector<double> hs(2 * (size_t)P);
+    for (int q = 0; q < P; ++q) {
+      const Seg& sg = pending[q];
+      const double k0 = its[sg.inst]->s_->k0();
+      hs[2 * q] = 0.5 * (sg.zb - sg.za) / k0;
+      hs[2 * q + 1] = (sg.zb - sg.za) / k0;
+    }
+    DevBuf hsb = upload(c, hs.data(), sizeof(double) * hs.size());
+    DevBuf errb = c.ws.get(sizeof(double) * RH_ERR_BLOCKS * (size_t)P);
+    std::vector<std::shared_ptr<DevBuf>> Ts(P);
+    for (int q = 0; q < P; ++q) {
+      const Seg& sg = pending[q];
+      Ts[q] = std::make_shared<DevBuf>(c.ws.get(sizeof(cd) * 4 * NN));
+      cd* T = Ts[q]->as<cd>();
+      const cd* P3[3] = {sg.a.P(NN), sg.m.P(NN), sg.b.P(NN)};
+      const cd* Q3[3] = {sg.a.Q(NN), sg.m.Q(NN), sg.b.Q(NN)};
+      const Smp pr{sg.R.lv, sg.R.j};
+      redheffer_perturbation_enqueue(c, N, its[sg.inst]->s_->k0(), sg.za, sg.zb, pr.P(NN), pr.Q(NN), sg.R.sys(N),
+                                     P3, Q3, T, T + NN, T + 2 * NN, T + 3 * NN, hsb.as<double>() + 2 * q,
+                                     errb.as<double>() + (size_t)RH_ERR_BLOCKS * q);
+    }
+    std::vector<double> eh(RH_ERR_BLOCKS * (size_t)P);
+    VRC_HIP(hipMemcpyAsync(eh.data(), errb.as<double>(), sizeof(double) * eh.size(), hipMemcpyDeviceToHost,
+                           c.stream));
+    VRC_HIP(hipStreamSynchronize(c.stream));
+    // ---- accept, or trisect
+    std::vector<int> split;
+    for (int q = 0; q < P; ++q) {
+      const Seg& sg = pending[q];
+      double err = 0.0;
+      for (int k = 0; k < RH_ERR_BLOCKS; ++k) err = std::max(err, eh[(size_t)RH_ERR_BLOCKS * q + k]);
+      if (err <= pe || sg.depth >= max_depth) {
+        if (err > pe && !c.host_code) {
+          c.host_code = H_DEPTH;
+          c.host_msg = std::string(health_text(H_DEPTH)) + " (segment [" + std::to_string(sg.za) + ", " +
+                       std::to_string(sg.zb) + "], error " + std::to_string(err) + ", depth " +
+                       std::to_string(max_depth) + ")";
         }
-        const double zm0 = 2. / 3. * za + 1. / 3. * zb;
-        const double zm1 = 1. / 3. * za + 2. / 3. * zb;
-        {
-          Sample st = sample(0.5 * (za + zm0));
-          Sample sk = sample(zm0);
-          auto Rt = make_ref(st);
-          rec(Rt->sys(), st.P.as<cd>(), st.Q.as<cd>(), za, zm0, a, st, sk, depth + 1);
-          Sample st2 = sample(zm1);
-          rec(R, Pr, Qr, zm0, zm1, sk, mid, st2, depth + 1);
-          Sample sk2 = sample(0.5 * (zm1 + zb));
-          auto Rt2 = make_ref(sk2);
-          rec(Rt2->sys(), sk2.P.as<cd>(), sk2.Q.as<cd>(), zm1, zb, st2, sk2, b, depth + 1);
-        }
-      };
-  rec(Rm->sys(), sm.P.as<cd>(), sm.Q.as<cd>(), z0_, z1_, s0, sm, s1, 0);
-  // right projection (src/core/RedhefferIntegrator.cpp:211-223)
-  LUStore lul;
-  lul.make(c, N, leftW.as<cd>(), leftV.as<cd>(), 1, 0);
-  project_right(c, N, 1, S, lul.pair, W1_.as<cd>(), V1_.as<cd>(), 0);
+        done[sg.inst].push_back(Done{sg.za, sg.zb, sg.R, Ts[q]});
+      } else {
+        split.push_back(q);
+      }
+    }
+    Ts.clear();
+    const int S = (int)split.size();
+    std::vector<Seg> next;
+    if (S > 0) {
+      // slots: [0, S) new left references (midpoint of the first third), [S, 2S) new right
+      // references (midpoint of the last third), [2S, 3S) first trisection point,
+      // [3S, 4S) second trisection point
+      LevelP L = new_level(c, N, 4 * S);
+      for (int u = 0; u < S; ++u) {
+        const Seg& sg = pending[split[u]];
+        Sampler* smp = its[sg.inst]->s_;
+        const double zm0 = 2. / 3. * sg.za + 1. / 3. * sg.zb;
+        const double zm1 = 1. / 3. * sg.za + 2. / 3. * sg.zb;
+        const double zz[4] = {0.5 * (sg.za + zm0), 0.5 * (zm1 + sg.zb), zm0, zm1};
+        for (int k = 0; k < 4; ++k)
+          smp->sample_gpu({zz[k]}, L->P.as<cd>() + (k * S + u) * NN, L->Q.as<cd>() + (k * S + u) * NN);
+      }
+      solve_level(c, N, *L, 2 * S, 2 * S);
+      for (int u = 0; u < S; ++u) {
+        const Seg& sg = pending[split[u]];
+        const double zm0 = 2. / 3. * sg.za + 1. / 3. * sg.zb;
+        const double zm1 = 1. / 3. * sg.za + 2. / 3. * sg.zb;
+        const Smp st{L, u}, sk2{L, S + u}, sk{L, 2 * S + u}, st2{L, 3 * S + u};
+        next.push_back(Seg{sg.inst, sg.depth + 1, sg.za, zm0, Ref{L, u}, sg.a, st, sk});
+        next.push_back(Seg{sg.inst, sg.depth + 1, zm0, zm1, sg.R, sk, sg.m, st2});
+        next.push_back(Seg{sg.inst, sg.depth + 1, zm1, sg.zb, Ref{L, S + u}, st2, sk2, sg.b});
+      }
+    }
+    pending.swap(next);
+  }
+  // ---- per integrator: project each accepted segment to the left basis and cascade, in z order
+  for (int i = 0; i < I; ++i) {
+    Integrator& it = *its[i];
+    auto& D = done[i];
+    std::sort(D.begin(), D.end(), [](const Done& x, const Done& y) { return x.za < y.za; });
+    SMat S = it.S();
+    init_identity_S(c, N, S, 1);
+    const cd* leftW = it.W0_.as<cd>();
+    const cd* leftV = it.V0_.as<cd>();
+    for (auto& d : D) {
+      cd* T0 = d.T->as<cd>();
+      SMat T;
+      T.Tdd = T0;
+      T.Tuu = T0 + NN;
+      T.Rud = T0 + 2 * NN;
+      T.Rdu = T0 + 3 * NN;
+      const RefSys R = d.R.sys(N);
+      project_left(c, N, 1, T, R.LU, leftW, leftV, 0);
+      leftW = R.W;
+      leftV = R.V;
+      star_product(c, N, 1, S, T.Tdd, T.Tuu, T.Rud, T.Rdu, 0);
+      it.tree_.push_back({d.za, d.zb});
+      ++it.N_;
+      d.T.reset();
+    }
+    LUStore lul;
+    lul.make(c, N, leftW, leftV, 1, 0);
+    project_right(c, N, 1, S, lul.pair, it.W1_.as<cd>(), it.V1_.as<cd>(), 0);
+    D.clear();  // releases the segments' references once their work is queued
+  }
   VRC_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -565,6 +700,7 @@ void Integrator::differential(int Nsteps) {
   vrc_ctx& c = *c_;
   hipStream_t st = c.stream;
   const int N = Nd_;
+  tune_apply(N);
   const long NN = (long)N * N;
   const double k0 = s_->k0();
   tree_.clear();
@@ -826,6 +962,7 @@ static void batch_stack_solve(vrc_ctx& c, int nx, int ny, const std::vector<Stac
 
 void batch_stack_rcwa(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int chunk, cd* S_out,
                       cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o, StackTape* tape, int* flags) {
+  tune_apply(2 * nx * ny);
   batch_stack_solve(c, nx, ny, inst, chunk, S_out, W0o, V0o, g0o, W1o, V1o, g1o, tape, flags);
   const long NN = 4L * nx * ny * nx * ny;
   if (flags) nonfinite_flags(c.stream, S_out, 4 * NN, 4 * NN, (int)inst.size(), flags, H_NONFINITE);

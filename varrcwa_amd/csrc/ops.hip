@@ -913,9 +913,9 @@ void project_right(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LUl, con
 // ===================================================================== (d) varRCWA segment
 // Perturbation of one segment [z0, z1] around the reference (Pr, Qr, W, V, gamma):
 // produces (tdd, tuu, rud, rdu) and the max-modulus error estimate (returned on host).
-double redheffer_perturbation(vrc_ctx& c, int N, double k0, double z0, double z1, const cd* Pr, const cd* Qr,
-                              const RefSys& R, const cd* P3[3], const cd* Q3[3], cd* tdd, cd* tuu, cd* rud,
-                              cd* rdu) {
+void redheffer_perturbation_enqueue(vrc_ctx& c, int N, double k0, double z0, double z1, const cd* Pr,
+                                    const cd* Qr, const RefSys& R, const cd* P3[3], const cd* Q3[3], cd* tdd,
+                                    cd* tuu, cd* rud, cd* rdu, const double* d_hs, double* err) {
   hipStream_t s = c.stream;
   const long NN = (long)N * N;
   DevBuf dAb = c.ws.get(sizeof(cd) * 6 * (size_t)NN);
@@ -937,23 +937,30 @@ double redheffer_perturbation(vrc_ctx& c, int N, double k0, double z0, double z1
     copy(s, N, N, tmp, dA + k * NN, 0, 1);
   }
   DevBuf phb = c.ws.get(sizeof(cd) * 2 * (size_t)N);
+  {
+    dim3 g(cdiv_up(N, 256), 1);
+    phase_kernel<<<g, 256, 0, s>>>(N, R.gam, 0, d_hs, phb.as<cd>(), 0);
+    phase_kernel<<<g, 256, 0, s>>>(N, R.gam, 0, d_hs + 1, phb.as<cd>() + N, 0);
+    launch_check("phase");
+  }
+  const cd c0 = mk(0.0, (z1 - z0) / (6.0 * 2.0 * k0));
+  simpson_kernel<<<RH_ERR_BLOCKS, 256, 0, s>>>(N, dA, dA + NN, dA + 2 * NN, dB, dB + NN, dB + 2 * NN, phb.as<cd>(),
+                                              phb.as<cd>() + N, c0, tdd, tuu, rud, rdu, err);
+  launch_check("simpson");
+}
+
+double redheffer_perturbation(vrc_ctx& c, int N, double k0, double z0, double z1, const cd* Pr, const cd* Qr,
+                              const RefSys& R, const cd* P3[3], const cd* Q3[3], cd* tdd, cd* tuu, cd* rud,
+                              cd* rdu) {
+  hipStream_t s = c.stream;
   DevBuf sb = c.ws.get(sizeof(double) * 2);
   const double hs[2] = {0.5 * (z1 - z0) / k0, (z1 - z0) / k0};
   VRC_HIP(hipMemcpyAsync(sb.as<double>(), hs, sizeof(hs), hipMemcpyHostToDevice, s));
-  {
-    dim3 g(cdiv_up(N, 256), 1);
-    phase_kernel<<<g, 256, 0, s>>>(N, R.gam, 0, sb.as<double>(), phb.as<cd>(), 0);
-    phase_kernel<<<g, 256, 0, s>>>(N, R.gam, 0, sb.as<double>() + 1, phb.as<cd>() + N, 0);
-    launch_check("phase");
-  }
-  const int nblk = 256;
-  DevBuf errb = c.ws.get(sizeof(double) * nblk);
-  const cd c0 = mk(0.0, (z1 - z0) / (6.0 * 2.0 * k0));
-  simpson_kernel<<<nblk, 256, 0, s>>>(N, dA, dA + NN, dA + 2 * NN, dB, dB + NN, dB + 2 * NN, phb.as<cd>(),
-                                     phb.as<cd>() + N, c0, tdd, tuu, rud, rdu, errb.as<double>());
-  launch_check("simpson");
-  std::vector<double> h(nblk);
-  VRC_HIP(hipMemcpyAsync(h.data(), errb.as<double>(), sizeof(double) * nblk, hipMemcpyDeviceToHost, s));
+  DevBuf errb = c.ws.get(sizeof(double) * RH_ERR_BLOCKS);
+  redheffer_perturbation_enqueue(c, N, k0, z0, z1, Pr, Qr, R, P3, Q3, tdd, tuu, rud, rdu, sb.as<double>(),
+                                 errb.as<double>());
+  std::vector<double> h(RH_ERR_BLOCKS);
+  VRC_HIP(hipMemcpyAsync(h.data(), errb.as<double>(), sizeof(double) * RH_ERR_BLOCKS, hipMemcpyDeviceToHost, s));
   VRC_HIP(hipStreamSynchronize(s));
   double e = 0.0;
   for (double v : h) e = v > e ? v : e;

@@ -7,12 +7,32 @@
 
 #include "vrc_internal.hpp"
 
+namespace vrc {
+// Pinned host staging for host -> device copies: a block is reused once the copy that
+// read it has completed (an event recorded after the copy), so uploads need no host
+// synchronization. While a graph is being recorded (hold != nullptr) fresh blocks are
+// taken and handed to the recording, which keeps them as long as its graph lives.
+struct PinnedPool {
+  struct Block {
+    void* p;
+    size_t bytes;
+    hipEvent_t ev;
+  };
+  std::vector<Block> blocks;
+  std::vector<void*>* hold = nullptr;
+  ~PinnedPool();
+  // Copies host[0, bytes) into a pinned block and enqueues its copy to dst on stream s.
+  void upload(hipStream_t s, void* dst, const void* host, size_t bytes);
+};
+}  // namespace vrc
+
 // Reference counted: samplers and integrators hold a reference, so the pool and the
 // context outlive every object that leases device memory from them.
 struct vrc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   vrc::Workspace ws;
+  vrc::PinnedPool pinned;
   std::string last_error;
   std::atomic<int> refs{1};
   // stage timers (HIP events on the context stream), see vrc::StageScope

@@ -46,6 +46,14 @@ class Workspace {
   void arena_begin(void* base, size_t bytes);
   size_t arena_end();
 
+  // Graph recording: every block handed out between track_begin and track_end is noted;
+  // track_end takes the released ones out of the pool and returns them (pointer, bytes):
+  // from then on the caller owns them and frees them with hipFree. *leased counts the
+  // noted blocks that are still leased (they stay with their holders).
+  void track_begin();
+  std::vector<std::pair<void*, size_t>> track_end(size_t* leased);
+  bool tracking() const { return tracking_; }
+
  private:
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> owned_;
@@ -54,6 +62,8 @@ class Workspace {
   char* arena_lo_ = nullptr;    // last region used, for put() after arena_end
   size_t arena_bytes_ = 0, arena_top_ = 0, arena_peak_ = 0;
   std::vector<std::pair<size_t, bool>> arena_blocks_;  // (offset, released), bottom to top
+  bool tracking_ = false;
+  std::map<void*, size_t> tracked_;
 };
 
 inline DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
@@ -88,6 +98,7 @@ void zgemm_set_big_waves(int w);  // wavefronts of the large-tile kernel (4 or 8
 void zgemm_set_small_waves(int w);  // wavefronts of the 64x64-tile kernel (4 or 8)
 void zgemm_set_big_n(int n);        // columns of the large tile (64 or 128)
 void zgemm_desc(hipStream_t s, int opA, int opB, const GemmDesc* d_desc, int count, int max_m, int max_n);
+void zgemm_get_knobs(int* big_n, int* big_waves, int* small_waves, int* big_min_k);
 void zgeam(hipStream_t s, int m, int n, cd a, const cd* A, long lda, long sA, cd b, const cd* B, long ldb, long sB,
            cd* C, long ldc, long sC, int batch);
 void zset(hipStream_t s, int m, int n, cd diag, cd off, cd* C, long ldc, long sC, int batch);
@@ -129,7 +140,22 @@ void zgeev_vectors(hipStream_t s, int n, const cd* Z, const cd* X, const cd* dba
                    int batch);
 void zgeev_set_aed(int window);
 void zgeev_set_max_sweeps(int n);  // QR sweep budget per matrix (0 = automatic)
-void zgeev_set_bulges(int nb);  // bulges per QR sweep (1..16)  // deflation-window size of the QR iteration (0 = off)
+void zgeev_set_bulges(int nb);  // bulges per QR sweep (1..16)
+int zgeev_get_aed();
+int zgeev_get_bulges();
+
+// ---------------------------------------------------------------- autotune.cpp
+// The kernel switches an autotuned harmonic count overrides (process-wide values).
+struct TuneKnobs {
+  int aed, bulges, big_n, big_waves, small_waves, big_min_k;
+};
+TuneKnobs tune_get();
+void tune_put(const TuneKnobs& k);
+// Puts the autotuned switches of N in place before a call that solves N x N systems (or
+// the caller's own values back when N has no autotuned entry).
+void tune_apply(int N);
+// A manual setting of a switch becomes the caller's own value.
+void tune_manual_begin();
 // gamma = sqrt(lambda) with the reference's branch rule.
 void gamma_sqrt(hipStream_t s, int n, const cd* lam, long sl, cd* gam, long sg, int batch);
 

@@ -172,6 +172,12 @@ void project_right(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LUl, con
 double redheffer_perturbation(vrc_ctx& c, int N, double k0, double z0, double z1, const cd* Pr, const cd* Qr,
                               const RefSys& R, const cd* P3[3], const cd* Q3[3], cd* tdd, cd* tuu, cd* rud,
                               cd* rdu);
+// The same without the readback: d_hs (device) holds {(z1 - z0) / (2 k0), (z1 - z0) / k0};
+// the error estimate is the largest of the RH_ERR_BLOCKS values written to err (device).
+constexpr int RH_ERR_BLOCKS = 256;
+void redheffer_perturbation_enqueue(vrc_ctx& c, int N, double k0, double z0, double z1, const cd* Pr,
+                                    const cd* Qr, const RefSys& R, const cd* P3[3], const cd* Q3[3], cd* tdd,
+                                    cd* tuu, cd* rud, cd* rdu, const double* d_hs, double* err);
 
 // ---------------------------------------------------------------- samplers
 typedef double (*RuleFn)(double z, void* user);
@@ -311,6 +317,10 @@ class Integrator {
   double it_z0() const { return z0_; }
   double it_z1() const { return z1_; }
   void redheffer(double pe, int max_depth);
+  // RedhefferIntegrator::compute of several integrators (one context, equal N) at once:
+  // the subdivision trees are grown breadth-first, with every new reference eigensystem of
+  // a level (across segments and integrators) solved in one batched launch.
+  static void redheffer_many(const std::vector<Integrator*>& its, double pe, int max_depth);
   void differential(int N);
 
  protected:
@@ -333,6 +343,14 @@ struct StackInstance {
   const HostLayer* top;
 };
 struct StackTape;
+// Autotuner (autotune.cpp): the best switches for N = 2 nx ny on this sample of a batch
+// (returned, not stored), and a one-line report; the table and its cache file.
+std::string autotune_stack(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& sample, double budget_s,
+                           TuneKnobs* best_out);
+void tune_store(int N, const TuneKnobs& k);
+void tune_clear();
+void tune_save(const std::string& path, int N, const TuneKnobs& k, const std::string& note);
+int tune_load(const std::string& path);
 // Plane-wave diffraction efficiencies per instance (n per instance) from the reflected and
 // transmitted field harmonics rxy, txy ([Ex; Ey], 2n per instance).
 void efficiencies(hipStream_t s, int n, int batch, const cd* rxy, const cd* txy, const cd* ab, const double* k0v,

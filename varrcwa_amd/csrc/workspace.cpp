@@ -25,6 +25,7 @@ DevBuf Workspace::get(size_t bytes) {
     void* p = it->second;
     size_t b = it->first;
     free_.erase(it);
+    if (tracking_) tracked_[p] = b;
     return DevBuf(p, b, this);
   }
   void* p = nullptr;
@@ -43,7 +44,39 @@ DevBuf Workspace::get(size_t bytes) {
   }
   owned_[p] = bytes;
   reserved_ += bytes;
+  if (tracking_) tracked_[p] = bytes;
   return DevBuf(p, bytes, this);
+}
+
+void Workspace::track_begin() {
+  tracking_ = true;
+  tracked_.clear();
+}
+
+std::vector<std::pair<void*, size_t>> Workspace::track_end(size_t* leased) {
+  tracking_ = false;
+  std::vector<std::pair<void*, size_t>> out;
+  size_t held = 0;
+  for (auto& kv : tracked_) {
+    bool found = false;
+    auto range = free_.equal_range(kv.second);
+    for (auto it = range.first; it != range.second; ++it)
+      if (it->second == kv.first) {
+        free_.erase(it);
+        found = true;
+        break;
+      }
+    if (!found) {
+      ++held;
+      continue;
+    }
+    owned_.erase(kv.first);
+    reserved_ -= kv.second;
+    out.push_back(kv);
+  }
+  tracked_.clear();
+  if (leased) *leased = held;
+  return out;
 }
 
 void Workspace::put(void* p, size_t bytes) {
@@ -91,9 +124,45 @@ void Workspace::trim() {
 }  // namespace vrc
 
 // ---------------------------------------------------------------- stage timers
+#include <algorithm>
+#include <cstring>
+
 #include "vrc_context.hpp"
 
 namespace vrc {
+
+PinnedPool::~PinnedPool() {
+  for (auto& b : blocks) {
+    (void)hipEventSynchronize(b.ev);
+    (void)hipEventDestroy(b.ev);
+    (void)hipHostFree(b.p);
+  }
+}
+
+void PinnedPool::upload(hipStream_t s, void* dst, const void* host, size_t bytes) {
+  if (bytes == 0) return;
+  void* p = nullptr;
+  if (hold) {  // recording a graph: the block belongs to the recording
+    VRC_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    hold->push_back(p);
+    std::memcpy(p, host, bytes);
+    VRC_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, s));
+    return;
+  }
+  Block* blk = nullptr;
+  for (auto& b : blocks)
+    if (b.bytes >= bytes && hipEventQuery(b.ev) == hipSuccess && (!blk || b.bytes < blk->bytes)) blk = &b;
+  if (!blk) {
+    Block b{nullptr, std::max<size_t>(bytes, 4096), nullptr};
+    VRC_HIP(hipHostMalloc(&b.p, b.bytes, hipHostMallocDefault));
+    VRC_HIP(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming));
+    blocks.push_back(b);
+    blk = &blocks.back();
+  }
+  std::memcpy(blk->p, host, bytes);
+  VRC_HIP(hipMemcpyAsync(dst, blk->p, bytes, hipMemcpyHostToDevice, s));
+  VRC_HIP(hipEventRecord(blk->ev, s));
+}
 
 vrc_ctx*& tl_ctx() {
   static thread_local vrc_ctx* c = nullptr;
