@@ -246,7 +246,7 @@ def mixed_bench(ctx, count: int):
     orders = load_geometry()["c3_orders"]
     insts = load_instances(count)
     harm = [(orders[i], orders[i]) for i in range(count)]
-    api.batch_stack_rcwa_mixed(ctx, harm, insts[:16])  # warm the small classes
+    api.batch_stack_rcwa_mixed(ctx, harm[:16], insts[:16])  # warm the small classes
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     api.batch_stack_rcwa_mixed(ctx, harm, insts)
@@ -257,6 +257,31 @@ def mixed_bench(ctx, count: int):
     log(f"mixed orders: {count} instances ({dict(Counter(orders[:count]))}) in {t:.2f} s")
     return {"instances": count, "slices": count * L, "seconds": t, "slices_per_s": count * L / t,
             "orders": {str(k): v for k, v in sorted(Counter(orders[:count]).items())}}
+
+
+def pixel_fourier_bench(ctx, nlayers: int, grid: int, nh: int):
+    """Part (a) alone on uniform pixel grids (the FFT path): nlayers layers of grid x grid
+    random permittivity pixels, nh x nh harmonics; HIP-event time of the Fourier stage and
+    its algorithmic bytes (16 B per pixel read, fe11, fe22, fe33 written once)."""
+    import torch
+    from varrcwa_amd import api
+    rng = np.random.default_rng(5)
+    x = np.arange(grid + 1) / grid
+    layers = [(x, x, 1 + 11.25 * rng.random((grid, grid)) + 0j) for _ in range(nlayers)]
+    api.assemble_toeplitz(ctx, nh, nh, layers[:2])  # warm
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.vrc_timing_enable(ctx.h, 1), "timing")
+    read_stages(ctx)
+    fe = api.assemble_toeplitz(ctx, nh, nh, layers)
+    torch.cuda.synchronize()
+    ms = read_stages(ctx)["fourier"][0]
+    ctx.check(ctx.lib.vrc_timing_enable(ctx.h, 0), "timing")
+    del fe
+    b = fourier_bytes(nh, grid * grid) * nlayers
+    log(f"pixel Fourier: {nlayers} layers of {grid}x{grid} at {nh}x{nh}: {ms:.2f} ms, {b / ms / 1e6:.0f} GB/s")
+    return {"layers": nlayers, "grid": grid, "harmonics": f"{nh}x{nh}", "ms": ms, "bytes": b,
+            "achieved_gbs": b / (ms * 1e-3) / 1e9, "peak_gbs": HBM_PEAK_GBS,
+            "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def main():
@@ -275,6 +300,7 @@ def main():
     ap.add_argument("--varrcwa", type=int, default=4, help="wavelengths of the varRCWA sweep leg (0 = skip)")
     ap.add_argument("--mixed", type=int, default=512, help="instances of the mixed-order leg (0 = skip)")
     ap.add_argument("--graph", type=int, default=1, help="also time replays of the recorded step (0 = skip)")
+    ap.add_argument("--pixel", type=int, default=64, help="layers of the 256x256 pixel Fourier leg (0 = skip)")
     ap.add_argument("--tune", action="append", default=[], help="kernel switch key=value (vrc_tune_set)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first copy this many bytes device to device (a plain copy kernel of known size "
@@ -346,6 +372,9 @@ def main():
     mixed = None
     if args.mixed > 0:
         mixed = mixed_bench(ctx, args.mixed)
+    pixel = None
+    if args.pixel > 0:
+        pixel = pixel_fourier_bench(ctx, args.pixel, 256, 11)
     ctx.check(ctx.lib.vrc_pool_trim(ctx.h), "pool_trim")
 
     # ---- batched: at most one warm-up step, then timed steps within the budget
@@ -487,6 +516,7 @@ def main():
         "parity": parity,
         "adjoint": adj,
         "fourier_stage": fourier,
+        "fourier_pixel_grids": pixel,
         "stage_ms_per_step": {k: v / steps for k, v in excl.items()},
         "device_memory_in_use_gb": (total_b - free_b) / 1e9,
         "dominant_stage": dom,

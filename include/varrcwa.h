@@ -436,19 +436,42 @@ int vrc_find_guided_modes(vrc_ctx* ctx, int n, const void* gamma, double k0, dou
  * output, results are written in the layout of the forward input.
  *
  * vrc_batch_stack_forward: vrc_batch_stack_rcwa that also returns a tape (NULL tape
- * pointer = no tape) holding the geometry and every instance's eigen-systems
- * (bottom, top, slabs: 2 N x N + N complex each) for the reverse pass.
+ * pointer = no tape) for the reverse pass: the geometry, the end media's eigen-systems,
+ * the cascade before the end projection (4 N x N per instance), the cascade prefix at the
+ * start of every block of K layers (checkpoints, 4 N x N each) and, by default, every
+ * slab's eigen-system (2 N x N + N complex each). With recompute = 1 the slabs'
+ * eigen-systems are not kept and the reverse pass solves them again, one block of K
+ * layers (all instances of a pass) per launch: the tape then grows with L / K instead of
+ * L, at the cost of one more eigensolve per slab. K <= 0: min(L, 16). The reverse pass
+ * holds the intermediates of one block, not of the whole stack.
  * vrc_batch_stack_vjp: eps_bar (vrc_tape_cells complex values) receives, per
  * instance and slab in order, the cotangent of every eps cell of the slab in the
  * layout of vrc_layer.eps, for S_bar = the cotangent of S_out ([Tdd|Tuu|Rud|Rdu]
  * per instance). Slab coordinates must be non-decreasing (no painter's refill),
- * periodic cells only. chunk (<= 0: automatic) bounds the instances per pass. */
+ * periodic cells only. chunk (<= 0: automatic) bounds the instances per pass.
+ * vrc_batch_stack_vjp_shape: the same reverse pass giving (as well as, or instead of,
+ * eps_bar: may be NULL) the gradient with respect to shape parameters of the slabs'
+ * x coordinates (a Parameterization, src/rcwa/Parameterization.h:5-25, per slab;
+ * chained through the closed-form coordinate tangents of src/rcwa/Layer.cpp:104-162):
+ * slab s = instance * nlayers + layer has npara[s] parameters (0: none) and nvar[s]
+ * varied interior x coordinates, listed in slab order in var_index (coordinate
+ * indices) and coeff (nvar[s] x npara[s] per slab, row-major: coordinate var_index[k]
+ * moves by coeff[k * npara[s] + p] per unit of parameter p). shape_bar_host receives
+ * the sum over slabs of npara doubles, slab by slab (synchronizes).
+ * vrc_tape_bytes: device memory the tape holds. */
 typedef struct vrc_tape vrc_tape;
 int vrc_batch_stack_forward(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
                             void* S_out, void* W0, void* V0, void* gam0, void* W1, void* V1, void* gam1,
                             vrc_tape** tape, int* info_out);
+int vrc_batch_stack_forward_ex(vrc_ctx* ctx, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
+                               void* S_out, void* W0, void* V0, void* gam0, void* W1, void* V1, void* gam1,
+                               int recompute, int layer_block, vrc_tape** tape, int* info_out);
 long long vrc_tape_cells(const vrc_tape* tape);
+long long vrc_tape_bytes(const vrc_tape* tape);
 int vrc_batch_stack_vjp(vrc_tape* tape, const void* S_bar, void* eps_bar, int chunk);
+int vrc_batch_stack_vjp_shape(vrc_tape* tape, const void* S_bar, const int* npara, const int* nvar,
+                              const int* var_index, const double* coeff, double* shape_bar_host, void* eps_bar,
+                              int chunk);
 int vrc_tape_destroy(vrc_tape* tape);
 
 /* Stage adjoints (each the vector-Jacobian product of the forward entry point of

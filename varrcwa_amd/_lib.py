@@ -131,6 +131,10 @@ def load():
         "vrc_batch_stack_forward": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp,
                                     C.POINTER(vp), c_int_p),
         "vrc_tape_cells": (C.c_longlong, vp),
+        "vrc_tape_bytes": (C.c_longlong, vp),
+        "vrc_batch_stack_forward_ex": (i, vp, i, i, i, C.POINTER(VrcStackInstance), i, vp, vp, vp, vp, vp, vp, vp,
+                                       i, i, C.POINTER(vp), c_int_p),
+        "vrc_batch_stack_vjp_shape": (i, vp, vp, c_int_p, c_int_p, c_int_p, c_double_p, c_double_p, vp, i),
         "vrc_batch_stack_vjp": (i, vp, vp, vp, i),
         "vrc_tape_destroy": (i, vp),
         "vrc_assemble_toeplitz_vjp": (i, vp, i, i, i, C.POINTER(VrcLayer), vp, vp, vp, l, vp),

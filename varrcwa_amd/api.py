@@ -500,6 +500,49 @@ class StackTape:
         self.ctx.check(self.ctx.lib.vrc_batch_stack_vjp(self.h, ptr(S_bar), ptr(out), int(chunk)), "batch_stack_vjp")
         return out
 
+    def bytes(self) -> int:
+        """Device memory the tape holds."""
+        return int(self.ctx.lib.vrc_tape_bytes(self.h))
+
+    def vjp_shape(self, S_bar, shape, chunk: int = 0, want_eps: bool = False):
+        """Gradient with respect to shape parameters of the slabs' x coordinates.
+
+        shape: one entry per slab (instance-major, then layer): None, or (var_index,
+        coeff) with coeff of shape (len(var_index), npara) (coordinate var_index[k] moves
+        by coeff[k, p] per unit of parameter p). Returns (per-slab list of numpy arrays of
+        npara gradients (empty for None), eps cotangents or None)."""
+        torch = self.ctx.torch
+        S_bar = S_bar.contiguous()
+        npara, nvar, vidx, coef = [], [], [], []
+        for e in shape:
+            if e is None:
+                npara.append(0)
+                nvar.append(0)
+                continue
+            vi, cf = e
+            cf = np.asarray(cf, dtype=np.float64).reshape(len(vi), -1)
+            npara.append(cf.shape[1])
+            nvar.append(len(vi))
+            vidx.extend(int(v) for v in vi)
+            coef.extend(cf.reshape(-1).tolist())
+        np_a = np.ascontiguousarray(npara, dtype=np.int32)
+        nv_a = np.ascontiguousarray(nvar, dtype=np.int32)
+        vi_a = np.ascontiguousarray(vidx or [0], dtype=np.int32)
+        cf_a = np.ascontiguousarray(coef or [0.0], dtype=np.float64)
+        out = np.zeros(max(1, int(np_a.sum())), dtype=np.float64)
+        eps = torch.empty(self.cells(), dtype=torch.complex128, device=S_bar.device) if want_eps else None
+        ip = C.POINTER(C.c_int)
+        dp = C.POINTER(C.c_double)
+        self.ctx.check(self.ctx.lib.vrc_batch_stack_vjp_shape(
+            self.h, ptr(S_bar), np_a.ctypes.data_as(ip), nv_a.ctypes.data_as(ip), vi_a.ctypes.data_as(ip),
+            cf_a.ctypes.data_as(dp), out.ctypes.data_as(dp), ptr(eps) if want_eps else None, int(chunk)),
+            "batch_stack_vjp_shape")
+        per, o = [], 0
+        for k in npara:
+            per.append(out[o:o + k].copy())
+            o += k
+        return per, eps
+
     def close(self):
         if self.h:
             self.ctx.lib.vrc_tape_destroy(self.h)
@@ -513,12 +556,14 @@ class StackTape:
 
 
 def batch_stack_rcwa(ctx: Context, nx: int, ny: int, instances, chunk: int = 0, want_bases: bool = False,
-                     tape: bool = False, return_info: bool = False):
+                     tape: bool = False, return_info: bool = False, recompute: bool = False, layer_block: int = 0):
     """instances: list of dicts with keys lam, layers (Layer list), thickness, bottom, top.
 
     Returns a dict of device tensors: S (ninst, 4, N, N) as column-major storage
     (use ``from_dev_colmajor``), W0, V0, gam0, W1, V1, gam1 when requested, and a
-    StackTape under "tape" when tape=True. A numerical failure of any instance (an
+    StackTape under "tape" when tape=True (recompute=True: the tape keeps no slab
+    eigen-systems, the reverse pass solves them again; layer_block: layers per block of
+    the reverse pass, 0 = automatic). A numerical failure of any instance (an
     unconverged eigensolve, NaN or infinity in its S-matrix) raises VrcError naming the
     instance, unless return_info=True: then the per-instance codes (0 ok, 1 eigensolver,
     4 non-finite) are returned under "info" (numpy int array) and nothing is raised for them.
@@ -540,8 +585,8 @@ def batch_stack_rcwa(ctx: Context, nx: int, ny: int, instances, chunk: int = 0, 
     info_p = info.ctypes.data_as(C.POINTER(C.c_int))
     if tape:
         h = C.c_void_p()
-        rc = ctx.lib.vrc_batch_stack_forward(ctx.h, nx, ny, len(instances), arr, int(chunk), ptr(S), *ptrs,
-                                             C.byref(h), info_p)
+        rc = ctx.lib.vrc_batch_stack_forward_ex(ctx.h, nx, ny, len(instances), arr, int(chunk), ptr(S), *ptrs,
+                                                int(bool(recompute)), int(layer_block), C.byref(h), info_p)
         if h:
             out["tape"] = StackTape(ctx, h)
         what = "batch_stack_forward"

@@ -695,4 +695,43 @@ void project_left_vjp(vrc_ctx& c, int N, int batch, const SMat& S, const LUPair&
   // Rudb = Rud_nb (unchanged)
 }
 
+// ===================================================================== shape-parameter contraction
+// part[p][blk] = Re sum over this block's share of conj(A[i]) B[p][i] (i < count).
+__global__ void dot_re_partial_kernel(long count, const cd* A, const cd* B, long sB, double* part) {
+  const int p = blockIdx.y;
+  const cd* b = B + (long)p * sB;
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x) {
+    const cd x = A[i], y = b[i];
+    acc += x.x * y.x + x.y * y.y;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(long)p * gridDim.x + blockIdx.x] = red[0];
+}
+
+// out[p] += sum of the nblk partials of parameter p (in order: deterministic).
+__global__ void dot_re_finish_kernel(int npara, int nblk, const double* part, double* out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npara) return;
+  double acc = 0.0;
+  for (int k = 0; k < nblk; ++k) acc += part[(long)p * nblk + k];
+  out[p] += acc;
+}
+
+void dot_re_accumulate(vrc_ctx& c, long count, const cd* A, const cd* B, long sB, int npara, double* out) {
+  if (count <= 0 || npara <= 0) return;
+  const int nblk = (int)std::min<long>(256, (count + 255) / 256);
+  DevBuf part = c.ws.get(sizeof(double) * (size_t)nblk * npara);
+  dot_re_partial_kernel<<<dim3(nblk, npara), 256, 0, c.stream>>>(count, A, B, sB, part.as<double>());
+  launch_check("dot_re_partial");
+  dot_re_finish_kernel<<<cdiv_up(npara, 64), 64, 0, c.stream>>>(npara, nblk, part.as<double>(), out);
+  launch_check("dot_re_finish");
+}
+
 }  // namespace vrc

@@ -1087,6 +1087,13 @@ int vrc_stack_efficiencies(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stac
 int vrc_batch_stack_forward(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
                             void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1,
                             vrc_tape** tape, int* info_out) {
+  return vrc_batch_stack_forward_ex(c, nx, ny, ninst, inst, chunk, S_out, W0, V0, g0, W1, V1, g1, 0, 0, tape,
+                                    info_out);
+}
+
+int vrc_batch_stack_forward_ex(vrc_ctx* c, int nx, int ny, int ninst, const vrc_stack_instance* inst, int chunk,
+                               void* S_out, void* W0, void* V0, void* g0, void* W1, void* V1, void* g1,
+                               int recompute, int layer_block, vrc_tape** tape, int* info_out) {
   if (tape) *tape = nullptr;
   int num = 0;
   const int rc = guarded(c, [&] {
@@ -1101,7 +1108,7 @@ int vrc_batch_stack_forward(vrc_ctx* c, int nx, int ny, int ninst, const vrc_sta
     if (tape) {
       tp = std::make_unique<vrc_tape>();
       tp->ctx = c;
-      tp->t = make_stack_tape(*c, nx, ny, hs.si);
+      tp->t = make_stack_tape(*c, nx, ny, hs.si, recompute != 0, layer_block);
     }
     health_reset(*c);
     DevBuf flags = c->ws.get(sizeof(int) * (size_t)ninst);
@@ -1130,11 +1137,58 @@ int vrc_batch_stack_forward(vrc_ctx* c, int nx, int ny, int ninst, const vrc_sta
 
 long long vrc_tape_cells(const vrc_tape* t) { return (t && t->t) ? (long long)t->t->cells() : -1; }
 
+long long vrc_tape_bytes(const vrc_tape* t) { return (t && t->t) ? (long long)t->t->device_bytes() : -1; }
+
 int vrc_batch_stack_vjp(vrc_tape* t, const void* S_bar, void* eps_bar, int chunk) {
   if (!t) return -1;
   return guarded(t->ctx, [&] {
     need(S_bar && eps_bar, "bad arguments");
     stack_tape_vjp(*t->t, (const cd*)S_bar, (cd*)eps_bar, chunk);
+  });
+}
+
+int vrc_batch_stack_vjp_shape(vrc_tape* t, const void* S_bar, const int* npara, const int* nvar,
+                              const int* var_index, const double* coeff, double* shape_bar_host, void* eps_bar,
+                              int chunk) {
+  if (!t) return -1;
+  return guarded(t->ctx, [&] {
+    need(S_bar && npara && nvar && shape_bar_host, "bad arguments");
+    const StackTape& tp = *t->t;
+    const int S = tp.ninst * tp.L;
+    ShapeParams sp;
+    sp.npara.assign(npara, npara + S);
+    sp.nvar.assign(nvar, nvar + S);
+    sp.pofs.assign(S + 1, 0);
+    sp.vofs.assign(S + 1, 0);
+    sp.cofs.assign(S + 1, 0);
+    for (int s = 0; s < S; ++s) {
+      need(npara[s] >= 0 && nvar[s] >= 0, "negative parameter or coordinate count");
+      need(nvar[s] == 0 || npara[s] > 0, "varied coordinates need at least one parameter");
+      sp.pofs[s + 1] = sp.pofs[s] + npara[s];
+      sp.vofs[s + 1] = sp.vofs[s] + nvar[s];
+      sp.cofs[s + 1] = sp.cofs[s] + (long)nvar[s] * npara[s];
+    }
+    need(sp.vofs[S] == 0 || (var_index && coeff), "coordinate lists missing");
+    sp.var_index.assign(var_index, var_index + sp.vofs[S]);
+    sp.coeff.assign(coeff, coeff + sp.cofs[S]);
+    for (int s = 0; s < S; ++s)
+      for (long k = sp.vofs[s]; k < sp.vofs[s + 1]; ++k) {
+        const HostLayer* L = tp.slab(s / tp.L, s % tp.L);
+        need(sp.var_index[k] > 0 && sp.var_index[k] < (int)L->x.size() - 1,
+             "only interior x coordinates can be varied");
+      }
+    const long P = sp.pofs[S];
+    vrc_ctx* c = t->ctx;
+    DevBuf sb = c->ws.get(sizeof(double) * (size_t)std::max<long>(1, P));
+    if (P > 0) {
+      const std::vector<double> zero((size_t)P, 0.0);
+      c->pinned.upload(c->stream, sb.ptr, zero.data(), sizeof(double) * P);
+    }
+    stack_tape_vjp(*t->t, (const cd*)S_bar, (cd*)eps_bar, chunk, &sp, sb.as<double>());
+    if (P > 0) {
+      VRC_HIP(hipMemcpyAsync(shape_bar_host, sb.ptr, sizeof(double) * P, hipMemcpyDeviceToHost, c->stream));
+      VRC_HIP(hipStreamSynchronize(c->stream));
+    }
   });
 }
 

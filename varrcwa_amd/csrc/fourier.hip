@@ -49,6 +49,7 @@ __global__ void fourier_coef_kernel(FourierDev g) {
   const long total = 2 * nFX + nFY1 + nFYi + nFX1;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     if (e < 2 * nFX) {
+      if (g.fftx[l]) continue;  // fourier_fft_x_kernel
       const bool inv = e >= nFX;
       const long ee = inv ? e - nFX : e;
       const int i = (int)(ee / M1), p = (int)(ee % M1);
@@ -69,6 +70,7 @@ __global__ void fourier_coef_kernel(FourierDev g) {
       if (i < ncy) r = fterm(mk(1.0, 0.0), ys[i], ys[i + 1], q - hy, Ly);
       g.FY1[(long)l * nFY1 + ee] = r;
     } else if (e < 2 * nFX + nFY1 + nFYi) {
+      if (g.ffty[l]) continue;  // fourier_fft_y_kernel
       const long ee = e - 2 * nFX - nFY1;
       const int i = (int)(ee / M2), q = (int)(ee % M2);
       cd acc = mk(0.0, 0.0);
@@ -84,6 +86,109 @@ __global__ void fourier_coef_kernel(FourierDev g) {
       if (i < ncx) r = fterm(mk(1.0, 0.0), xs[i], xs[i + 1], p - hx, Lx);
       g.FX1[(long)l * nFX1 + ee] = r;
     }
+  }
+}
+
+// ---------------------------------------------------------------- uniform pixel grids
+// For G cells of width d/G starting at x0 the exact coefficient is a scaled DFT:
+//   F_m = sum_j f_j (1/G) exp(-i pi m (2 x0/d + (2j+1)/G)) sinc(m/G)
+//       = (1/G) sinc(m/G) exp(-i pi m (1/G + 2 x0/d)) DFT(f)_(m mod G),
+// so each strip (row or column) is transformed once by a radix-2 FFT in LDS and the
+// 2 nx - 1 (2 ny - 1) needed orders are read out with that correction.
+
+// In-place radix-2 decimation-in-time FFT of G = 2^lg points already in bit-reversed
+// order; tw[k] = exp(-2 pi i k / G), k < G / 2. Ends with a barrier.
+__device__ void lds_fft(cd* a, int G, int lg, const cd* tw) {
+  for (int st = 1; st <= lg; ++st) {
+    const int half = 1 << (st - 1);
+    const int step = G >> st;
+    for (int b = threadIdx.x; b < G / 2; b += blockDim.x) {
+      const int k = b & (half - 1);
+      const int i0 = ((b >> (st - 1)) << st) + k, i1 = i0 + half;
+      const cd v = cmul(a[i1], tw[k * step]);
+      const cd u = a[i0];
+      a[i0] = cadd(u, v);
+      a[i1] = csub(u, v);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ inline void fft_twiddles(cd* tw, int G) {
+  for (int k = threadIdx.x; k < G / 2; k += blockDim.x) {
+    double sn, cs;
+    sincospi(2.0 * k / G, &sn, &cs);
+    tw[k] = mk(cs, -sn);
+  }
+}
+
+// (1/G) sinc(m/G) exp(-i pi m (1/G + 2 x0/d)) for order m.
+__device__ inline cd fft_correction(int m, int G, double x0, double d) {
+  const double invG = 1.0 / G;
+  if (m == 0) return mk(invG, 0.0);
+  const double sx = (double)m * invG;
+  const double sc = sin(PI_REF * sx) / (PI_REF * sx);
+  double sn, cs;
+  sincos(PI_REF * (double)m * (invG + 2.0 * x0 / d), &sn, &cs);
+  return mk(invG * sc * cs, -invG * sc * sn);
+}
+
+// FX (Laurent rows) and FXi (rows of 1/eps) of every strip of the layers whose x cells
+// form a uniform power-of-two grid; one workgroup per (strip, layer). Dynamic LDS:
+// 1.5 G complex values.
+__global__ __launch_bounds__(256) void fourier_fft_x_kernel(FourierDev g) {
+  const int l = blockIdx.y, i = blockIdx.x;
+  const int lg = g.fftx[l];
+  if (!lg || i >= g.ncy[l]) return;
+  extern __shared__ cd fsm[];
+  const int G = 1 << lg;
+  cd* a = fsm;
+  cd* tw = fsm + G;
+  fft_twiddles(tw, G);
+  const long MX = g.MX, MY = g.MY;
+  const int M1 = 2 * g.nx - 1, hx = g.nx - 1;
+  const double d = g.Lx[l], x0 = g.xu[(long)l * (MX + 1)];
+  const cd* row = g.val + ((long)l * MY + i) * MX;
+  const long o = (long)l * MY * M1 + (long)i * M1;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int j = threadIdx.x; j < G; j += blockDim.x) {
+      const cd f = row[j];
+      a[__brev((unsigned)j) >> (32 - lg)] = pass ? crecip(f) : f;
+    }
+    __syncthreads();
+    lds_fft(a, G, lg, tw);
+    for (int p = threadIdx.x; p < M1; p += blockDim.x) {
+      const int m = p - hx;
+      const int k = ((m % G) + G) % G;
+      (pass ? g.FXi : g.FX)[o + p] = cmul(a[k], fft_correction(m, G, x0, d));
+    }
+    __syncthreads();
+  }
+}
+
+// FYi (columns of 1/eps along y) of the layers whose y strips form a uniform
+// power-of-two grid; one workgroup per (x cell, layer).
+__global__ __launch_bounds__(256) void fourier_fft_y_kernel(FourierDev g) {
+  const int l = blockIdx.y, i = blockIdx.x;
+  const int lg = g.ffty[l];
+  if (!lg || i >= g.ncx[l]) return;
+  extern __shared__ cd fsm[];
+  const int G = 1 << lg;
+  cd* a = fsm;
+  cd* tw = fsm + G;
+  fft_twiddles(tw, G);
+  const long MX = g.MX, MY = g.MY;
+  const int M2 = 2 * g.ny - 1, hy = g.ny - 1;
+  const double d = g.Ly[l], y0 = g.ys[(long)l * (MY + 1)];
+  const cd* col = g.val + (long)l * MY * MX + i;
+  for (int s = threadIdx.x; s < G; s += blockDim.x) a[__brev((unsigned)s) >> (32 - lg)] = crecip(col[(long)s * MX]);
+  __syncthreads();
+  lds_fft(a, G, lg, tw);
+  const long o = (long)l * MX * M2 + (long)i * M2;
+  for (int q = threadIdx.x; q < M2; q += blockDim.x) {
+    const int m = q - hy;
+    const int k = ((m % G) + G) % G;
+    g.FYi[o + q] = cmul(a[k], fft_correction(m, G, y0, d));
   }
 }
 
@@ -201,6 +306,13 @@ void fourier_tables(hipStream_t s, const FourierDev& g, int nlayers) {
     dim3 grid(64, nlayers);
     fourier_coef_kernel<<<grid, 256, 0, s>>>(g);
     launch_check("fourier_coef");
+  }
+  if (g.fft_max > 0) {
+    const size_t lds = sizeof(cd) * (size_t)g.fft_max * 3 / 2;
+    fourier_fft_x_kernel<<<dim3((unsigned)g.MY, nlayers), 256, lds, s>>>(g);
+    launch_check("fourier_fft_x");
+    fourier_fft_y_kernel<<<dim3((unsigned)g.MX, nlayers), 256, lds, s>>>(g);
+    launch_check("fourier_fft_y");
   }
   if (g.nx > 64 || g.ny > 64) throw ArgFailure("harmonic count per axis above 64 is not supported");
   toeplitz_inv_kernel<<<nlayers * (int)g.MY, 256, 0, s>>>(g.nx, g.FXi, M1, g.tFFx, (long)g.nx * g.nx, g.ncy,

@@ -178,6 +178,26 @@ FourierStage fourier_prepare(vrc_ctx& c, int nx, int ny, const std::vector<const
     MY = std::max(MY, pl.back().ncy);
   }
   const int M1 = 2 * nx - 1, M2 = 2 * ny - 1;
+  // uniform power-of-two grids along x (sorted, so no refill) and y take the FFT path
+  auto fft_log2 = [](const std::vector<double>& c, const std::vector<double>* sorted) -> int {
+    const int n = (int)c.size() - 1;
+    if (n < 2 || n > FOURIER_FFT_MAX || (n & (n - 1)) != 0) return 0;
+    if (sorted && *sorted != c) return 0;
+    const double d = c[n] - c[0];
+    if (!(d > 0)) return 0;
+    for (int j = 0; j <= n; ++j)
+      if (std::fabs(c[j] - (c[0] + d * j / n)) > 1e-12 * d) return 0;
+    int lg = 0;
+    while ((1 << lg) < n) ++lg;
+    return lg;
+  };
+  std::vector<int> fx(L), fy(L);
+  int fft_max = 0;
+  for (int l = 0; l < L; ++l) {
+    fx[l] = fft_log2(pl[l].xu, &pl[l].xs);
+    fy[l] = fft_log2(pl[l].ys, nullptr);
+    fft_max = std::max(fft_max, std::max(fx[l] ? 1 << fx[l] : 0, fy[l] ? 1 << fy[l] : 0));
+  }
   // host staging (one upload)
   std::vector<int> ncx(L), ncy(L);
   std::vector<double> Lx(L), Ly(L), xu((size_t)L * (MX + 1), 0.0), xs((size_t)L * (MX + 1), 0.0),
@@ -195,7 +215,7 @@ FourierStage fourier_prepare(vrc_ctx& c, int nx, int ny, const std::vector<const
     for (int i = 0; i < p.ncy; ++i)
       for (int j = 0; j < p.ncx; ++j) val[((size_t)l * MY + i) * MX + j] = p.val[(size_t)i * p.ncx + j];
   }
-  const size_t bytes_int = sizeof(int) * 2 * L, bytes_d = sizeof(double) * (2 * L + 2 * L * (MX + 1) + L * (MY + 1)),
+  const size_t bytes_int = sizeof(int) * 4 * L, bytes_d = sizeof(double) * (2 * L + 2 * L * (MX + 1) + L * (MY + 1)),
                bytes_c = sizeof(cd) * val.size();
   std::vector<char> host(bytes_int + bytes_d + bytes_c);
   char* h = host.data();
@@ -206,6 +226,8 @@ FourierStage fourier_prepare(vrc_ctx& c, int nx, int ny, const std::vector<const
   };
   put(ncx.data(), sizeof(int) * L);
   put(ncy.data(), sizeof(int) * L);
+  put(fx.data(), sizeof(int) * L);
+  put(fy.data(), sizeof(int) * L);
   const size_t od = o;
   put(Lx.data(), sizeof(double) * L);
   put(Ly.data(), sizeof(double) * L);
@@ -226,6 +248,9 @@ FourierStage fourier_prepare(vrc_ctx& c, int nx, int ny, const std::vector<const
   g.MY = MY;
   g.ncx = (const int*)d;
   g.ncy = (const int*)d + L;
+  g.fftx = (const int*)d + 2 * L;
+  g.ffty = (const int*)d + 3 * L;
+  g.fft_max = fft_max;
   const double* dd = (const double*)(d + od);
   g.Lx = dd;
   g.Ly = dd + L;

@@ -18,6 +18,13 @@ static const cd ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
 // Instances whose star products and projections run in one batch (batch_stack_rcwa).
 static int g_cascade_group = 128;
 void stack_set_cascade_group(int n) { g_cascade_group = n < 1 ? 1 : n; }
+// Instances per cascade group: g_cascade_group, fewer when their slab S-matrices and end
+// systems would take more than 64 GiB (deep stacks at large N).
+static int cascade_group_cap(int N, int L) {
+  const size_t per_inst = sizeof(cd) * (size_t)N * N * (2 * (size_t)L + 4);
+  const size_t cap = std::max<size_t>(1, ((size_t)64 << 30) / per_inst);
+  return (int)std::max<size_t>(1, std::min<size_t>((size_t)g_cascade_group, cap));
+}
 
 static DevBuf upload(vrc_ctx& c, const void* host, size_t bytes) {
   DevBuf b = c.ws.get(bytes);
@@ -176,9 +183,15 @@ void Sampler::sample_layers(const std::vector<HostLayer>& ls, cd* P, cd* Q) {
 // tangents of FourierSolver2D::solve (src/rcwa/FourierSolver2D.cpp:12-351):
 //   dP = [[Kx Y1, Kx Y2], [Ky Y1, Ky Y2]],  [Y1 | Y2] = fe33^-1 dfe33 fe33^-1 [-Ky | Kx]
 //   dQ = [[0, -k0^2 dfe22], [k0^2 dfe11, 0]],  dPl, dQl with d(k0^2)/d lambda = -8 pi^2 / lambda^3.
-void Sampler::shape_derivatives(const HostLayer& L, int nvar, const int* var_index, const double* var_coeff,
-                                int npara, cd* dP, cd* dQ, cd* dfe11, cd* dfe22, cd* dfe33, cd* dPl, cd* dQl) {
-  hipStream_t st = c_->stream;
+void shape_tangents(vrc_ctx& c, int nx, int ny, double lambda, bool pml, const cd* d_alpha, const cd* d_K,
+                    const HostLayer& L, int nvar, const int* var_index, const double* var_coeff, int npara, cd* dP,
+                    cd* dQ, cd* dfe11, cd* dfe22, cd* dfe33, cd* dPl, cd* dQl) {
+  hipStream_t st = c.stream;
+  vrc_ctx* c_ = &c;
+  const int nx_ = nx, ny_ = ny;
+  const double lambda_ = lambda;
+  const bool pml_ = pml;
+  auto k0 = [&] { return 2 * PI_REF / lambda; };
   const int nCx = (int)L.x.size();
   if (npara < 0 || nvar < 0 || (nvar > 0 && (!var_index || !var_coeff))) throw ArgFailure("bad parameterization");
   for (int i = 0; i + 1 < nCx; ++i)
@@ -253,27 +266,33 @@ void Sampler::shape_derivatives(const HostLayer& L, int nvar, const int* var_ind
   zgeam(st, n, n, ONE, fe33, n, 0, ZERO, nullptr, 0, 0, lu.as<cd>(), n, 0, 1);
   zgetrf(st, n, lu.as<cd>(), n, 0, piv.as<int>(), 0, inf.as<int>(), 1, c_->ws);
   if (pml_) {
-    const cd* K = d_K_.as<cd>();
+    const cd* K = d_K;
     zgeam(st, n, n, mk(-1.0, 0.0), K + nn, n, 0, ZERO, nullptr, 0, 0, F.as<cd>(), n, 0, 1);
     zgeam(st, n, n, ONE, K, n, 0, ZERO, nullptr, 0, 0, F.as<cd>() + nn, n, 0, 1);
   } else {
     zset(st, n, 2 * n, ZERO, ZERO, F.as<cd>(), n, 0, 1);
-    zadd_diag(st, n, mk(-1.0, 0.0), d_alpha_.as<cd>() + n, 0, F.as<cd>(), n, 0, 1);
-    zadd_diag(st, n, ONE, d_alpha_.as<cd>(), 0, F.as<cd>() + nn, n, 0, 1);
+    zadd_diag(st, n, mk(-1.0, 0.0), d_alpha + n, 0, F.as<cd>(), n, 0, 1);
+    zadd_diag(st, n, ONE, d_alpha, 0, F.as<cd>() + nn, n, 0, 1);
   }
   zgetrs(st, n, 2 * n, lu.as<cd>(), n, 0, piv.as<int>(), 0, F.as<cd>(), n, 0, F.as<cd>(), n, 0, 1, c_->ws);
   zgetrs(st, n, n * npara, lu.as<cd>(), n, 0, piv.as<int>(), 0, dfe33, n, 0, X.as<cd>(), n, 0, 1, c_->ws);
   zgemm_strided(st, OP_N, OP_N, n, 2 * n, n, ONE, X.as<cd>(), n, nn, F.as<cd>(), n, 0, ZERO, Y.as<cd>(), n, 2 * nn,
                 npara);
   if (pml_) {
-    const cd* K = d_K_.as<cd>();
+    const cd* K = d_K;
     zgemm_strided(st, OP_N, OP_N, n, 2 * n, n, ONE, K, n, 0, Y.as<cd>(), n, 2 * nn, ZERO, dP, N, NN, npara);
     zgemm_strided(st, OP_N, OP_N, n, 2 * n, n, ONE, K + nn, n, 0, Y.as<cd>(), n, 2 * nn, ZERO, dP + n, N, NN,
                   npara);
   } else {
-    zdscale(st, n, 2 * n, d_alpha_.as<cd>(), 0, nullptr, 0, 0, Y.as<cd>(), n, 2 * nn, dP, N, NN, npara);
-    zdscale(st, n, 2 * n, d_alpha_.as<cd>() + n, 0, nullptr, 0, 0, Y.as<cd>(), n, 2 * nn, dP + n, N, NN, npara);
+    zdscale(st, n, 2 * n, d_alpha, 0, nullptr, 0, 0, Y.as<cd>(), n, 2 * nn, dP, N, NN, npara);
+    zdscale(st, n, 2 * n, d_alpha + n, 0, nullptr, 0, 0, Y.as<cd>(), n, 2 * nn, dP + n, N, NN, npara);
   }
+}
+
+void Sampler::shape_derivatives(const HostLayer& L, int nvar, const int* var_index, const double* var_coeff,
+                                int npara, cd* dP, cd* dQ, cd* dfe11, cd* dfe22, cd* dfe33, cd* dPl, cd* dQl) {
+  shape_tangents(*c_, nx_, ny_, lambda_, pml_, d_alpha_.as<cd>(), pml_ ? d_K_.as<cd>() : nullptr, L, nvar, var_index,
+                 var_coeff, npara, dP, dQ, dfe11, dfe22, dfe33, dPl, dQl);
 }
 
 VarSampler::VarSampler(vrc_ctx* c, double lambda, int nx, int ny, HostLayer base, bool pml)
@@ -805,9 +824,11 @@ void Integrator::differential(int Nsteps) {
 // Cascade of one group of GB instances (slab S-matrices tu/ru [instance][slab], end systems
 // We/Ve/ge [bottoms | tops]) and the end projection, batched over the group; results into
 // the outputs at instance offset I0.
+// tape (optional): receives the prefix before every K-th layer and the cascade before the
+// projection of the group's instances.
 static void cascade_group(vrc_ctx& c, int N, int L, int GB, int I0, bool homog, const cd* tu, const cd* ru,
                           const cd* We, const cd* Ve, const cd* ge, cd* S_out, cd* W0o, cd* V0o, cd* g0o, cd* W1o,
-                          cd* V1o, cd* g1o) {
+                          cd* V1o, cd* g1o, StackTape* tape = nullptr) {
   hipStream_t st = c.stream;
   const long NN = (long)N * N;
   const int B = GB, i0 = I0;
@@ -830,11 +851,22 @@ static void cascade_group(vrc_ctx& c, int N, int L, int GB, int I0, bool homog, 
   zgeam(st, N, N, ONE, tu, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Tuu, N, NN, B);
   zgeam(st, N, N, ONE, ru, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rud, N, NN, B);
   zgeam(st, N, N, ONE, ru, N, (long)L * NN, ZERO, nullptr, 0, 0, S.Rdu, N, NN, B);
+  // tape layout: 4 blocks [Tdd|Tuu|Rud|Rdu] per instance
+  auto keep_S = [&](cd* dst, long per_inst) {
+    const SMat* src = &S;
+    const cd* blk[4] = {src->Tdd, src->Tuu, src->Rud, src->Rdu};
+    for (int q = 0; q < 4; ++q)
+      zgeam(st, N, N, ONE, blk[q], N, NN, ZERO, nullptr, 0, 0, dst + (long)i0 * per_inst + q * NN, N, per_inst, B);
+  };
+  const int nb = tape ? tape->nblocks() : 0;
   for (int l = 1; l < L; ++l) {
+    if (tape && l % tape->K == 0)
+      keep_S(tape->ckpt.as<cd>() + (long)(l / tape->K - 1) * 4 * NN, (long)(nb - 1) * 4 * NN);
     const cd* t = tu + l * NN;
     const cd* r = ru + l * NN;
     star_product(c, N, B, S, t, t, r, r, (long)L * NN);
   }
+  if (tape) keep_S(tape->Sfin.as<cd>(), 4 * NN);
   if (homog)
     project_left_homog(c, N, B, S, V0, V1, NN);  // W0 = W1 = I
   else
@@ -868,7 +900,7 @@ static void batch_stack_homog(vrc_ctx& c, int nx, int ny, const std::vector<Stac
   const long NN = (long)N * N;
   const int ninst = (int)inst.size();
   const int L = (int)inst[0].layers.size();
-  const int gsz = std::max(1, g_cascade_group);
+  const int gsz = cascade_group_cap(N, L);
   const int ngroups = (ninst + gsz - 1) / gsz;
   for (int g = 0; g < ngroups; ++g) {
     const int I0 = (int)((long)ninst * g / ngroups), GB = (int)((long)ninst * (g + 1) / ngroups) - I0;
@@ -1004,7 +1036,7 @@ static void batch_stack_solve(vrc_ctx& c, int nx, int ny, const std::vector<Stac
   // Cascade groups: the slab S-matrices and end systems of consecutive chunks are kept
   // and cascaded together, so that the star products and projections run batched over
   // up to g_cascade_group instances (chunks split evenly over the groups).
-  const int gmax = std::max(1, g_cascade_group / std::max(1, chunk));
+  const int gmax = std::max(1, cascade_group_cap(N, L) / std::max(1, chunk));
   const int ngroups = (nchunks + gmax - 1) / gmax;
   std::vector<int> gstart(ngroups + 1, 0);
   for (int g = 0; g < ngroups; ++g) gstart[g + 1] = gstart[g] + nchunks / ngroups + (g < nchunks % ngroups ? 1 : 0);
@@ -1102,9 +1134,11 @@ static void batch_stack_solve(vrc_ctx& c, int nx, int ny, const std::vector<Stac
       keep(nn2, Wall.as<cd>() + B * NN, tape->Wt.as<cd>() + (long)i0 * NN, B);
       keep(nn2, Vall.as<cd>() + B * NN, tape->Vt.as<cd>() + (long)i0 * NN, B);
       keep(N, gall.as<cd>() + (long)B * N, tape->gt.as<cd>() + (long)i0 * N, B);
-      keep(nn2, Wall.as<cd>() + 2 * B * NN, tape->Ws.as<cd>() + (long)i0 * L * NN, B * L);
-      keep(nn2, Vall.as<cd>() + 2 * B * NN, tape->Vs.as<cd>() + (long)i0 * L * NN, B * L);
-      keep(N, gall.as<cd>() + (long)2 * B * N, tape->gs.as<cd>() + (long)i0 * L * N, B * L);
+      if (!tape->recompute) {
+        keep(nn2, Wall.as<cd>() + 2 * B * NN, tape->Ws.as<cd>() + (long)i0 * L * NN, B * L);
+        keep(nn2, Vall.as<cd>() + 2 * B * NN, tape->Vs.as<cd>() + (long)i0 * L * NN, B * L);
+        keep(N, gall.as<cd>() + (long)2 * B * N, tape->gs.as<cd>() + (long)i0 * L * N, B * L);
+      }
       gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
       gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
       layer_smatrix(c, N, SL, dsv.as<double>(), Wall.as<cd>() + 2 * B * NN, Vall.as<cd>() + 2 * B * NN,
@@ -1141,7 +1175,7 @@ static void batch_stack_solve(vrc_ctx& c, int nx, int ny, const std::vector<Stac
           B);
   }
   cascade_group(c, N, L, GB, I0, homog, tuG.as<cd>(), ruG.as<cd>(), WeG.as<cd>(), VeG.as<cd>(), geG.as<cd>(), S_out,
-                W0o, V0o, g0o, W1o, V1o, g1o);
+                W0o, V0o, g0o, W1o, V1o, g1o, tape);
   }
 }
 
@@ -1222,7 +1256,13 @@ long StackTape::cells() const {
   return n;
 }
 
-std::unique_ptr<StackTape> make_stack_tape(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst) {
+size_t StackTape::device_bytes() const {
+  return Wb.bytes + Vb.bytes + gb.bytes + Wt.bytes + Vt.bytes + gt.bytes + Ws.bytes + Vs.bytes + gs.bytes +
+         ckpt.bytes + Sfin.bytes;
+}
+
+std::unique_ptr<StackTape> make_stack_tape(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst,
+                                           bool recompute, int K) {
   auto t = std::make_unique<StackTape>();
   t->c = &c;
   t->nx = nx;
@@ -1231,6 +1271,8 @@ std::unique_ptr<StackTape> make_stack_tape(vrc_ctx& c, int nx, int ny, const std
   t->L = inst.empty() ? 0 : (int)inst[0].layers.size();
   const int n = nx * ny, N = 2 * n, L = t->L, I = t->ninst;
   const size_t NN = (size_t)N * N;
+  t->recompute = recompute;
+  t->K = std::max(1, std::min(L, K > 0 ? K : 16));
   t->layers.reserve((size_t)I * (L + 2));
   for (auto& s : inst) {
     if ((int)s.layers.size() != L) throw ArgFailure("all instances of one batch must have the same layer count");
@@ -1246,84 +1288,69 @@ std::unique_ptr<StackTape> make_stack_tape(vrc_ctx& c, int nx, int ny, const std
   t->Wt = c.ws.get(sizeof(cd) * NN * I);
   t->Vt = c.ws.get(sizeof(cd) * NN * I);
   t->gt = c.ws.get(sizeof(cd) * (size_t)N * I);
-  t->Ws = c.ws.get(sizeof(cd) * NN * I * L);
-  t->Vs = c.ws.get(sizeof(cd) * NN * I * L);
-  t->gs = c.ws.get(sizeof(cd) * (size_t)N * I * L);
+  if (!recompute) {
+    t->Ws = c.ws.get(sizeof(cd) * NN * I * L);
+    t->Vs = c.ws.get(sizeof(cd) * NN * I * L);
+    t->gs = c.ws.get(sizeof(cd) * (size_t)N * I * L);
+  }
+  if (t->nblocks() > 1) t->ckpt = c.ws.get(sizeof(cd) * 4 * NN * I * (size_t)(t->nblocks() - 1));
+  t->Sfin = c.ws.get(sizeof(cd) * 4 * NN * I);
   return t;
 }
 
-// Reverse pass of batch_stack_rcwa: per chunk of instances, recompute P/Q and the slab
-// S-matrices from the taped eigen-systems, rerun the cascade keeping every prefix,
-// then propagate the cotangents back through the projection, the star products (last
-// slab first), the slab S-matrices, the eigen-decompositions and the Fourier assembly.
-void stack_tape_vjp(StackTape& t, const cd* Sb_in, cd* epsb, int chunk) {
+// Shape-parameter gradients of a block's slabs: dJ/dtheta = Re sum conj(Pbar) dP/dtheta +
+// conj(Qbar) dQ/dtheta, the tangents dP, dQ from the closed-form edge terms of the Fourier
+// tables (shape_tangents) of each parameterized slab.
+static void shape_vjp(vrc_ctx& c, const StackTape& t, const AssemblyBatch& a, const std::vector<int>& sidx,
+                      const cd* Pbar, const cd* Qbar, long sPQ, const ShapeParams& sp, double* shape_bar) {
+  const int nx = t.nx, ny = t.ny, n = nx * ny, N = 2 * n;
+  const long NN = (long)N * N;
+  for (size_t j = 0; j < sidx.size(); ++j) {
+    const int s = sidx[j];
+    const int np = sp.npara[s];
+    if (np <= 0) continue;
+    const int i = s / t.L;
+    std::vector<cd> al, be;
+    const HostLayer* bot = t.bottom(i);
+    alpha_beta(nx, ny, bot->x.back() - bot->x.front(), bot->y.back() - bot->y.front(), al, be);
+    al.insert(al.end(), be.begin(), be.end());
+    DevBuf dab = upload(c, al.data(), sizeof(cd) * al.size());
+    DevBuf dP = c.ws.get(sizeof(cd) * NN * np), dQ = c.ws.get(sizeof(cd) * NN * np);
+    shape_tangents(c, nx, ny, t.lambda[i], false, dab.as<cd>(), nullptr, *a.layers[j], sp.nvar[s],
+                   sp.var_index.data() + sp.vofs[s], sp.coeff.data() + sp.cofs[s], np, dP.as<cd>(),
+                   dQ.as<cd>(), nullptr, nullptr, nullptr, nullptr, nullptr);
+    dot_re_accumulate(c, NN, Pbar + (long)j * sPQ, dP.as<cd>(), NN, np, shape_bar + sp.pofs[s]);
+    dot_re_accumulate(c, NN, Qbar + (long)j * sPQ, dQ.as<cd>(), NN, np, shape_bar + sp.pofs[s]);
+  }
+}
+
+// Reverse pass of batch_stack_rcwa, per chunk of instances: the projection's adjoint from
+// the taped cascade, then the layers block by block from the top. Per block of K layers:
+// P/Q, the eigen-systems (taped, or solved again in one batched launch), the slab
+// S-matrices with their intermediates, the cascade prefixes from the block's checkpoint,
+// then the cotangents back through the block's star products (last layer first), the
+// slab S-matrices, the eigen-decompositions and the Fourier assembly. Device memory
+// grows with K, not with the layer count.
+void stack_tape_vjp(StackTape& t, const cd* Sb_in, cd* epsb, int chunk, const ShapeParams* shape,
+                    double* shape_bar) {
   vrc_ctx& c = *t.c;
   hipStream_t st = c.stream;
-  const int nx = t.nx, ny = t.ny, n = nx * ny, N = 2 * n, L = t.L, I = t.ninst;
+  const int nx = t.nx, ny = t.ny, n = nx * ny, N = 2 * n, L = t.L, I = t.ninst, K = t.K, NB = t.nblocks();
   const long NN = (long)N * N;
   if (I == 0 || L == 0) return;
+  tune_apply(N);
   std::vector<long> off((size_t)I * L + 1, 0);
   for (int i = 0; i < I; ++i)
     for (int l = 0; l < L; ++l) off[(size_t)i * L + l + 1] = off[(size_t)i * L + l] + (long)t.slab(i, l)->eps.size();
-  if (chunk <= 0) chunk = std::max(1, std::min(I, 128 / L));
+  if (chunk <= 0) chunk = std::max(1, std::min(I, 128 / K));
   for (int i0 = 0; i0 < I; i0 += chunk) {
-    const int B = std::min(chunk, I - i0), SL = B * L;
-    // ---- P, Q of the slabs (per-instance alpha, beta, k0^2)
-    std::vector<double> k2(SL), sv(SL);
-    std::vector<cd> ab((size_t)SL * 2 * n);
-    std::vector<int> owner(SL);
-    AssemblyBatch a;
-    a.nx = nx;
-    a.ny = ny;
-    a.nlayers = SL;
-    a.periodic = true;
-    for (int b = 0; b < B; ++b) {
-      const int i = i0 + b;
-      const double k0 = 2 * PI_REF / t.lambda[i];
-      const HostLayer* bot = t.bottom(i);
-      std::vector<cd> al, be;
-      alpha_beta(nx, ny, bot->x.back() - bot->x.front(), bot->y.back() - bot->y.front(), al, be);
-      for (int l = 0; l < L; ++l) {
-        const int j = b * L + l;
-        a.layers.push_back(t.slab(i, l));
-        k2[j] = k0 * k0;
-        sv[j] = t.thickness[(size_t)i * L + l] / k0;
-        owner[j] = b;
-        std::copy(al.begin(), al.end(), ab.begin() + (size_t)j * 2 * n);
-        std::copy(be.begin(), be.end(), ab.begin() + (size_t)j * 2 * n + n);
-      }
-    }
-    DevBuf dk2 = upload(c, k2.data(), sizeof(double) * SL);
-    DevBuf dab = upload(c, ab.data(), sizeof(cd) * ab.size());
-    DevBuf dsv = upload(c, sv.data(), sizeof(double) * SL);
-    DevBuf down = upload(c, owner.data(), sizeof(int) * SL);
-    a.d_alpha = dab.as<cd>();
-    a.d_beta = a.d_alpha + n;
-    a.sab = 2 * n;
-    a.d_k0sq = dk2.as<double>();
-    DevBuf P = c.ws.get(sizeof(cd) * NN * SL), Q = c.ws.get(sizeof(cd) * NN * SL);
-    assemble_pq(c, a, P.as<cd>(), Q.as<cd>());
-    const cd* Ws = t.Ws.as<cd>() + (long)i0 * L * NN;
-    const cd* Vs = t.Vs.as<cd>() + (long)i0 * L * NN;
-    const cd* gs = t.gs.as<cd>() + (long)i0 * L * N;
+    const int B = std::min(chunk, I - i0);
     const cd* W0 = t.Wb.as<cd>() + (long)i0 * NN;
     const cd* V0 = t.Vb.as<cd>() + (long)i0 * NN;
     const cd* W1 = t.Wt.as<cd>() + (long)i0 * NN;
     const cd* V1 = t.Vt.as<cd>() + (long)i0 * NN;
     LUStore lu1;
     lu1.make(c, N, W1, V1, B, NN);
-    DevBuf Wr = c.ws.get(sizeof(cd) * NN * SL), Vr = c.ws.get(sizeof(cd) * NN * SL);
-    gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
-    gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
-    // ---- slab S-matrices (forward intermediates kept) and the cascade with every
-    // prefix and star-product intermediate kept
-    SlabSaved slab;
-    layer_smatrix_fwd_saved(c, N, SL, dsv.as<double>(), Ws, Vs, gs, Wr.as<cd>(), Vr.as<cd>(), NN, slab);
-    const cd* tuv = slab.Tuu.as<cd>();
-    const cd* ruv = slab.Rud.as<cd>();
-    DevBuf pre = c.ws.get(sizeof(cd) * 4 * NN * B * L);
-    DevBuf sstar = c.ws.get(sizeof(cd) * STAR_SAVE * NN * B * L);
-    DevBuf Sbuf = c.ws.get(sizeof(cd) * 4 * NN * B);
     auto blocks = [&](cd* base) {
       SMat m;
       m.Tdd = base;
@@ -1332,58 +1359,138 @@ void stack_tape_vjp(StackTape& t, const cd* Sb_in, cd* epsb, int chunk) {
       m.Rdu = base + 3 * B * NN;
       return m;
     };
-    SMat S = blocks(Sbuf.as<cd>());
-    init_identity_S(c, N, S, B);
-    const long ss = (long)L * NN;
-    for (int l = 0; l < L; ++l) {
-      VRC_HIP(hipMemcpyAsync(pre.as<cd>() + (long)l * 4 * B * NN, Sbuf.ptr, sizeof(cd) * 4 * NN * B,
-                             hipMemcpyDeviceToDevice, st));
-      const cd* tl = tuv + l * NN;
-      const cd* rl = ruv + l * NN;
-      star_product(c, N, B, S, tl, tl, rl, rl, ss, sstar.as<cd>() + (long)l * STAR_SAVE * B * NN);
-    }
-    // ---- cotangents: projection, then the cascade in reverse
+    // tape cascades ([inst][4 N x N]) -> block-major SMat buffers ([4][B] N x N)
+    auto load_S = [&](cd* dst, const cd* src, long per_inst) {
+      for (int q = 0; q < 4; ++q)
+        zgeam(st, N, N, ONE, src + (long)i0 * per_inst + q * NN, N, per_inst, ZERO, nullptr, 0, 0,
+              dst + (long)q * B * NN, N, NN, B);
+    };
+    // ---- cotangent of the cascade before the projection
     DevBuf Sbb = c.ws.get(sizeof(cd) * 4 * NN * B);
     SMat Sb = blocks(Sbb.as<cd>());
+    load_S(Sbb.as<cd>(), Sb_in, 4 * NN);
     {
-      const cd* src = Sb_in + (long)i0 * 4 * NN;
-      zgeam(st, N, N, ONE, src, N, 4 * NN, ZERO, nullptr, 0, 0, Sb.Tdd, N, NN, B);
-      zgeam(st, N, N, ONE, src + NN, N, 4 * NN, ZERO, nullptr, 0, 0, Sb.Tuu, N, NN, B);
-      zgeam(st, N, N, ONE, src + 2 * NN, N, 4 * NN, ZERO, nullptr, 0, 0, Sb.Rud, N, NN, B);
-      zgeam(st, N, N, ONE, src + 3 * NN, N, 4 * NN, ZERO, nullptr, 0, 0, Sb.Rdu, N, NN, B);
+      DevBuf Sf = c.ws.get(sizeof(cd) * 4 * NN * B);
+      load_S(Sf.as<cd>(), t.Sfin.as<cd>(), 4 * NN);
+      SMat S = blocks(Sf.as<cd>());
+      project_left_vjp(c, N, B, S, lu1.pair, W0, V0, NN, Sb);
     }
-    project_left_vjp(c, N, B, S, lu1.pair, W0, V0, NN, Sb);
-    DevBuf tb = c.ws.get(sizeof(cd) * NN * SL), rb = c.ws.get(sizeof(cd) * NN * SL);
-    {
-      DevBuf q = c.ws.get(sizeof(cd) * 4 * NN * B);
-      cd* q0 = q.as<cd>();
-      for (int l = L - 1; l >= 0; --l) {
-        SMat S0 = blocks(pre.as<cd>() + (long)l * 4 * B * NN);
+    for (int bk = NB - 1; bk >= 0; --bk) {
+      const int l0 = bk * K, l1 = std::min(L, l0 + K), KB = l1 - l0, SL = B * KB;
+      // ---- the block's slabs, instance-major: P, Q (per-instance alpha, beta, k0^2)
+      std::vector<double> k2(SL), sv(SL);
+      std::vector<cd> ab((size_t)SL * 2 * n);
+      std::vector<int> owner(SL), sidx(SL);
+      AssemblyBatch a;
+      a.nx = nx;
+      a.ny = ny;
+      a.nlayers = SL;
+      a.periodic = true;
+      for (int b = 0; b < B; ++b) {
+        const int i = i0 + b;
+        const double k0 = 2 * PI_REF / t.lambda[i];
+        const HostLayer* bot = t.bottom(i);
+        std::vector<cd> al, be;
+        alpha_beta(nx, ny, bot->x.back() - bot->x.front(), bot->y.back() - bot->y.front(), al, be);
+        for (int l = l0; l < l1; ++l) {
+          const int j = b * KB + (l - l0);
+          a.layers.push_back(t.slab(i, l));
+          k2[j] = k0 * k0;
+          sv[j] = t.thickness[(size_t)i * L + l] / k0;
+          owner[j] = b;
+          sidx[j] = i * L + l;
+          std::copy(al.begin(), al.end(), ab.begin() + (size_t)j * 2 * n);
+          std::copy(be.begin(), be.end(), ab.begin() + (size_t)j * 2 * n + n);
+        }
+      }
+      DevBuf dk2 = upload(c, k2.data(), sizeof(double) * SL);
+      DevBuf dab = upload(c, ab.data(), sizeof(cd) * ab.size());
+      DevBuf dsv = upload(c, sv.data(), sizeof(double) * SL);
+      DevBuf down = upload(c, owner.data(), sizeof(int) * SL);
+      a.d_alpha = dab.as<cd>();
+      a.d_beta = a.d_alpha + n;
+      a.sab = 2 * n;
+      a.d_k0sq = dk2.as<double>();
+      DevBuf P = c.ws.get(sizeof(cd) * NN * SL), Q = c.ws.get(sizeof(cd) * NN * SL);
+      assemble_pq(c, a, P.as<cd>(), Q.as<cd>());
+      // ---- eigen-systems of the block: taped, or solved again
+      DevBuf Ws = c.ws.get(sizeof(cd) * NN * SL), Vs = c.ws.get(sizeof(cd) * NN * SL),
+             gs = c.ws.get(sizeof(cd) * (size_t)N * SL);
+      if (t.recompute) {
+        DevBuf inf = c.ws.get(sizeof(int) * SL);
+        wave_eqn(c, N, P.as<cd>(), Q.as<cd>(), NN, Ws.as<cd>(), Vs.as<cd>(), gs.as<cd>(), inf.as<int>(), SL);
+      } else {
+        DevBuf dsi = upload(c, sidx.data(), sizeof(int) * SL);
+        gather_mats(st, NN, t.Ws.as<cd>(), NN, dsi.as<int>(), Ws.as<cd>(), NN, SL);
+        gather_mats(st, NN, t.Vs.as<cd>(), NN, dsi.as<int>(), Vs.as<cd>(), NN, SL);
+        gather_mats(st, N, t.gs.as<cd>(), N, dsi.as<int>(), gs.as<cd>(), N, SL);
+      }
+      DevBuf Wr = c.ws.get(sizeof(cd) * NN * SL), Vr = c.ws.get(sizeof(cd) * NN * SL);
+      gather_mats(st, NN, W1, NN, down.as<int>(), Wr.as<cd>(), NN, SL);
+      gather_mats(st, NN, V1, NN, down.as<int>(), Vr.as<cd>(), NN, SL);
+      SlabSaved slab;
+      layer_smatrix_fwd_saved(c, N, SL, dsv.as<double>(), Ws.as<cd>(), Vs.as<cd>(), gs.as<cd>(), Wr.as<cd>(),
+                              Vr.as<cd>(), NN, slab);
+      Wr = DevBuf();
+      Vr = DevBuf();
+      const cd* tuv = slab.Tuu.as<cd>();
+      const cd* ruv = slab.Rud.as<cd>();
+      // ---- prefixes of the block from its checkpoint (identity before the first layer)
+      DevBuf pre = c.ws.get(sizeof(cd) * 4 * NN * B * KB);
+      DevBuf sstar = c.ws.get(sizeof(cd) * STAR_SAVE * NN * B * KB);
+      DevBuf Sbuf = c.ws.get(sizeof(cd) * 4 * NN * B);
+      SMat S = blocks(Sbuf.as<cd>());
+      if (bk == 0)
+        init_identity_S(c, N, S, B);
+      else
+        load_S(Sbuf.as<cd>(), t.ckpt.as<cd>() + (long)(bk - 1) * 4 * NN, (long)(NB - 1) * 4 * NN);
+      const long ss = (long)KB * NN;
+      for (int l = 0; l < KB; ++l) {
+        VRC_HIP(hipMemcpyAsync(pre.as<cd>() + (long)l * 4 * B * NN, Sbuf.ptr, sizeof(cd) * 4 * NN * B,
+                               hipMemcpyDeviceToDevice, st));
         const cd* tl = tuv + l * NN;
         const cd* rl = ruv + l * NN;
-        star_product_vjp(c, N, B, S0, tl, tl, rl, rl, ss, Sb, q0, q0 + B * NN, q0 + 2 * B * NN, q0 + 3 * B * NN, NN,
-                         sstar.as<cd>() + (long)l * STAR_SAVE * B * NN);
-        // the slab's Tdd and Tuu (Rud and Rdu) are one matrix: their cotangents add
-        zgeam(st, N, N, ONE, q0, N, NN, ONE, q0 + B * NN, N, NN, tb.as<cd>() + l * NN, N, ss, B);
-        zgeam(st, N, N, ONE, q0 + 2 * B * NN, N, NN, ONE, q0 + 3 * B * NN, N, NN, rb.as<cd>() + l * NN, N, ss, B);
+        star_product(c, N, B, S, tl, tl, rl, rl, ss, sstar.as<cd>() + (long)l * STAR_SAVE * B * NN);
       }
-    }
-    pre = DevBuf();
-    sstar = DevBuf();
-    // ---- slab S -> (W, V, gamma) -> (P, Q) -> eps
-    DevBuf Wbar = c.ws.get(sizeof(cd) * NN * SL), Vbar = c.ws.get(sizeof(cd) * NN * SL),
-           gbar = c.ws.get(sizeof(cd) * (size_t)N * SL);
-    layer_smatrix_bwd_saved(c, N, SL, dsv.as<double>(), slab, tb.as<cd>(), rb.as<cd>(), Wbar.as<cd>(), Vbar.as<cd>(),
-                            gbar.as<cd>());
-    const cd* Wi = slab.Wi.as<cd>();  // W^{-1}, unchanged by the slab reverse pass
-    DevBuf& Pbar = tb;  // slab cotangents consumed
-    DevBuf& Qbar = rb;
-    {
-      StageScope sc(ST_ADJOINT);
-      wave_eqn_vjp(c, N, SL, P.as<cd>(), Q.as<cd>(), NN, Ws, Vs, gs, Wi, Wbar.as<cd>(), Vbar.as<cd>(), gbar.as<cd>(),
-                   Pbar.as<cd>(), Qbar.as<cd>());
-      std::vector<long> o(off.begin() + (size_t)i0 * L, off.begin() + (size_t)i0 * L + SL);
-      assemble_pq_vjp(c, a, Pbar.as<cd>(), Qbar.as<cd>(), NN, epsb, o);
+      Sbuf = DevBuf();
+      // ---- cotangents back through the block's star products
+      DevBuf tb = c.ws.get(sizeof(cd) * NN * SL), rb = c.ws.get(sizeof(cd) * NN * SL);
+      {
+        DevBuf q = c.ws.get(sizeof(cd) * 4 * NN * B);
+        cd* q0 = q.as<cd>();
+        for (int l = KB - 1; l >= 0; --l) {
+          SMat S0 = blocks(pre.as<cd>() + (long)l * 4 * B * NN);
+          const cd* tl = tuv + l * NN;
+          const cd* rl = ruv + l * NN;
+          star_product_vjp(c, N, B, S0, tl, tl, rl, rl, ss, Sb, q0, q0 + B * NN, q0 + 2 * B * NN, q0 + 3 * B * NN,
+                           NN, sstar.as<cd>() + (long)l * STAR_SAVE * B * NN);
+          // the slab's Tdd and Tuu (Rud and Rdu) are one matrix: their cotangents add
+          zgeam(st, N, N, ONE, q0, N, NN, ONE, q0 + B * NN, N, NN, tb.as<cd>() + l * NN, N, ss, B);
+          zgeam(st, N, N, ONE, q0 + 2 * B * NN, N, NN, ONE, q0 + 3 * B * NN, N, NN, rb.as<cd>() + l * NN, N, ss,
+                B);
+        }
+      }
+      pre = DevBuf();
+      sstar = DevBuf();
+      // ---- slab S -> (W, V, gamma) -> (P, Q) -> eps (and shape parameters)
+      DevBuf Wbar = c.ws.get(sizeof(cd) * NN * SL), Vbar = c.ws.get(sizeof(cd) * NN * SL),
+             gbar = c.ws.get(sizeof(cd) * (size_t)N * SL);
+      layer_smatrix_bwd_saved(c, N, SL, dsv.as<double>(), slab, tb.as<cd>(), rb.as<cd>(), Wbar.as<cd>(),
+                              Vbar.as<cd>(), gbar.as<cd>());
+      const cd* Wi = slab.Wi.as<cd>();  // W^{-1}, unchanged by the slab reverse pass
+      DevBuf& Pbar = tb;  // slab cotangents consumed
+      DevBuf& Qbar = rb;
+      {
+        StageScope sc(ST_ADJOINT);
+        wave_eqn_vjp(c, N, SL, P.as<cd>(), Q.as<cd>(), NN, Ws.as<cd>(), Vs.as<cd>(), gs.as<cd>(), Wi, Wbar.as<cd>(),
+                     Vbar.as<cd>(), gbar.as<cd>(), Pbar.as<cd>(), Qbar.as<cd>());
+        if (epsb) {
+          std::vector<long> o(SL);
+          for (int j = 0; j < SL; ++j) o[j] = off[sidx[j]];
+          assemble_pq_vjp(c, a, Pbar.as<cd>(), Qbar.as<cd>(), NN, epsb, o);
+        }
+        if (shape && shape_bar) shape_vjp(c, t, a, sidx, Pbar.as<cd>(), Qbar.as<cd>(), NN, *shape, shape_bar);
+      }
     }
   }
   VRC_HIP(hipStreamSynchronize(st));

@@ -18,6 +18,8 @@ struct HostLayer {
   cd at(int i, int j) const { return eps[(size_t)i + (size_t)j * ncy()]; }
 };
 
+constexpr int FOURIER_FFT_MAX = 2048;  // longest grid axis transformed by FFT (1.5 x that in LDS: 48 KB)
+
 // Device arrays for a batch of layers (capacities MX cells, MY strips per layer).
 struct FourierDev {
   int nx, ny;
@@ -30,6 +32,11 @@ struct FourierDev {
   const double* xs;  // [l][MX+1] sorted coordinates (mode Y intervals)
   const double* ys;  // [l][MY+1]
   const cd* val;     // [l][MY][MX] painter-order (refilled) permittivity, sorted cell order
+  // log2 of the cell count when the layer is a uniform grid of a power-of-two number of
+  // cells along x (fftx) / y (ffty), else 0: those 1D tables come from an FFT
+  const int* fftx;
+  const int* ffty;
+  int fft_max;  // largest FFT length of the batch (0: none)
   cd *FX, *FXi, *FY1, *FYi, *FX1, *tFFx, *tFFy, *F2, *G11, *G22;
 };
 

@@ -179,6 +179,14 @@ void redheffer_perturbation_enqueue(vrc_ctx& c, int N, double k0, double z0, dou
                                     const cd* Qr, const RefSys& R, const cd* P3[3], const cd* Q3[3], cd* tdd,
                                     cd* tuu, cd* rud, cd* rdu, const double* d_hs, double* err);
 
+// Tangents of P and Q (and fe11, fe22, fe33; dP/dlambda, dQ/dlambda) of one layer with
+// respect to npara parameters of its interior x coordinates (Sampler::shape_derivatives
+// without a sampler object): d_alpha holds alpha then beta (n each, periodic cells), d_K
+// the six dense K products (absorbing regions, else null).
+void shape_tangents(vrc_ctx& c, int nx, int ny, double lambda, bool pml, const cd* d_alpha, const cd* d_K,
+                    const HostLayer& L, int nvar, const int* var_index, const double* var_coeff, int npara, cd* dP,
+                    cd* dQ, cd* dfe11, cd* dfe22, cd* dfe33, cd* dPl, cd* dQl);
+
 // ---------------------------------------------------------------- samplers
 typedef double (*RuleFn)(double z, void* user);
 
