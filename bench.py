@@ -338,7 +338,7 @@ def main():
     cells_per_step = sum(int(np.asarray(l.eps).size) for i in insts for l in i["layers"])
 
     # ---- bounded legs first: plain path, CPU oracle (+ parity reference), adjoint
-    plain = None
+    plain = plain_seq = None
     if args.plain > 0:
         inst = insts[0]
         ze = np.concatenate([[0.0], np.cumsum(inst["thickness"])])
@@ -353,6 +353,15 @@ def main():
         tp = time.perf_counter() - tp0
         plain = args.plain * L / tp
         log(f"plain: {args.plain * L} slices in {tp:.2f} s -> {plain:.3f} slices/s")
+        # the same call with one slice per eigensolve launch (the reference GPU integrator's pattern)
+        ctx.check(ctx.lib.vrc_tune_set(b"rcwa.group", 1), "tune")
+        torch.cuda.synchronize()
+        tp0 = time.perf_counter()
+        it.compute_edges(ze)
+        torch.cuda.synchronize()
+        plain_seq = L / (time.perf_counter() - tp0)
+        ctx.check(ctx.lib.vrc_tune_set(b"rcwa.group", 64), "tune")
+        log(f"plain, one slice per launch: {plain_seq:.3f} slices/s")
         del it, s
 
     ref = cpu = None
@@ -511,7 +520,10 @@ def main():
         "varrcwa": var,
         "mixed_orders": mixed,
         "plain_slices_per_s": plain,
+        "plain_definition": "RCWAIntegrator::compute over instance 0 (its slices' eigensolves grouped in one launch)",
         "batched_over_plain": (value / plain) if plain else None,
+        "plain_one_slice_per_launch_slices_per_s": plain_seq,
+        "batched_over_plain_one_slice_per_launch": (value / plain_seq) if plain_seq else None,
         "batched_over_cpu": (value / cpu["value"]) if cpu else None,
         "parity": parity,
         "adjoint": adj,

@@ -63,7 +63,9 @@ int vrc_timing_enable(vrc_ctx* ctx, int on);
  * "zgemm.big_waves", "zgemm.small_waves" (default 8, or 4): wavefronts per workgroup of the
  * 128x64 and 64x64 tiles; "zgemm.big_n" (default 128, or 64): a 128x128 tile for n >= 128; "zgeev.aed_window" (default 24, 0 = off, at most 64): deflation-window
  * size of the aggressive early deflation in the QR iteration; "zgeev.bulges" (default 12,
- * 1..16): double-shift bulges chased per QR sweep; "stack.cascade_group" (default 128):
+ * 1..16): double-shift bulges chased per QR sweep; "rcwa.group" (default 64): slices of
+ * RCWAIntegrator::compute solved per eigensolve launch (1: one slice at a time, as the
+ * reference's GPU integrator); "stack.cascade_group" (default 128):
  * instances of vrc_batch_stack_rcwa whose star products and projections run in one batch;
  * "zgeev.max_sweeps" (default 0 = 30 * max(N, 10)): QR sweeps per matrix before the
  * eigensolver gives up and reports info > 0; "redheffer.max_depth" (default 40, 0..60):
@@ -296,6 +298,15 @@ int vrc_rcwa_solver_solve(vrc_rcwa_solver* s, double lambda, int nstack, const i
 /* Host copies: which 0 Tdd, 1 Rud (N x N, column-major); for layer type layer_type: 2 eigvecE,
  * 3 eigvecH (N x N), 4 gamma (N, Layer::gamma = sqrt(eig) / k0). */
 int vrc_rcwa_solver_get(vrc_rcwa_solver* s, int which, int layer_type, double* host);
+/* RCWASolver::evaluateIntermediateField (src/rcwa/RCWASolver.cpp:1373-1621) for the stack
+ * of the last solve: for ncols incident amplitude vectors of the first stack entry
+ * (input_host: N x ncols, column-major; a unit vector e_m is the inputMode variant),
+ * c_m_host and c_p_host (host, (nstack - 2) x N x ncols, inner layer by inner layer, each
+ * N x ncols column-major) receive the reference's c_m and c_p: c_p[j] the forward
+ * amplitudes of inner layer j + 1 at its first interface, c_m[j] its backward amplitudes at
+ * its second interface. The stack needs an inner layer (nstack >= 3). */
+int vrc_rcwa_solver_intermediate_field(vrc_rcwa_solver* s, int ncols, const double* input_host, double* c_m_host,
+                                       double* c_p_host);
 int vrc_rcwa_solver_dims(const vrc_rcwa_solver* s, int* N, int* ntypes);
 int vrc_rcwa_solver_destroy(vrc_rcwa_solver* s);
 /* LayerSampler::nDim (src/core/LayerSampler.h:44) */

@@ -381,6 +381,40 @@ class RCWASolver {
   scalar transmittance(int inputMode) const { return colsum(Tdd_, inputMode); }
   scalar reflectance(int inputMode) const { return colsum(Rud_, inputMode); }
 
+  // RCWASolver::evaluateIntermediateField (src/rcwa/RCWASolver.cpp:1373-1621): modal
+  // amplitudes of every inner layer of the solved stack (layerStack and thickness must be
+  // those of the last solve) for an input mode or a list of (mode, coefficient) pairs.
+  void evaluateIntermediateField(std::vector<VectorXcs>& c_m, std::vector<VectorXcs>& c_p, int inputMode,
+                                 const std::vector<int>& layerStack, const std::vector<scalar>& thickness) const {
+    std::vector<std::pair<int, scalex>> one{{inputMode, scalex(1.0, 0.0)}};
+    evaluateIntermediateField(c_m, c_p, one, layerStack, thickness);
+  }
+  void evaluateIntermediateField(std::vector<VectorXcs>& c_m, std::vector<VectorXcs>& c_p,
+                                 const std::vector<std::pair<int, scalex>>& inputCoeffs,
+                                 const std::vector<int>& layerStack, const std::vector<scalar>& thickness) const {
+    if (layerStack.size() != thickness.size()) throw Error("thickness needs one value per stack entry", -1);
+    const long N = Tdd_.rows();
+    const int nin = (int)layerStack.size() - 2;
+    if (nin < 1) throw Error("the stack has no inner layer", -1);
+    VectorXcs in(N, 1);
+    for (auto& [m, v] : inputCoeffs) {
+      if (m < 0 || m >= N) throw Error("input mode out of range", -1);
+      in(m, 0) += v;
+    }
+    std::vector<scalex> bm((size_t)N * nin), bp((size_t)N * nin);
+    detail::check(vrc_rcwa_solver_intermediate_field(h_, 1, reinterpret_cast<const double*>(in.data()),
+                                                     reinterpret_cast<double*>(bm.data()),
+                                                     reinterpret_cast<double*>(bp.data())),
+                  ctx_->get(), "evaluateIntermediateField");
+    c_m.assign(nin, VectorXcs(N, 1));
+    c_p.assign(nin, VectorXcs(N, 1));
+    for (int j = 0; j < nin; ++j)
+      for (long r = 0; r < N; ++r) {
+        c_m[j](r, 0) = bm[(size_t)j * N + r];
+        c_p[j](r, 0) = bp[(size_t)j * N + r];
+      }
+  }
+
  private:
   static scalar colsum(const MatrixXcs& M, int c) {
     scalar s = 0;

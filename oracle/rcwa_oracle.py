@@ -839,6 +839,7 @@ class RCWASolver:
             ifH.append(np.linalg.solve(Ha, Hb))
         Tdd = np.eye(N, dtype=complex)
         Rud = np.zeros((N, N), dtype=complex)
+        self.taus = []
         for i in range(nL - 1):
             index = nL - i - 1
             if i == 0:
@@ -851,10 +852,71 @@ class RCWASolver:
             F = E + E @ waveRud
             G = H - H @ waveRud
             tau = np.linalg.inv(F + G)
+            self.taus.append(tau)
             Rud = -2.0 * G @ tau + np.eye(N)
             Tdd = 2.0 * waveTdd @ tau
         self.Tdd, self.Rud = Tdd, Rud
+        self.ifE, self.ifH = ifE, ifH
+        self.stack, self.thickness = list(stack), list(thickness)
         return Tdd, Rud
+
+    def evaluateIntermediateField(self, input_vec: np.ndarray):
+        """RCWASolver::evaluateIntermediateField (src/rcwa/RCWASolver.cpp:1373-1500; the
+        inputMode variant at :1503-1621 is input_vec = e_mode): modal amplitudes c_m, c_p
+        of every inner layer (nL - 2 of each) for the incident amplitudes input_vec of the
+        first layer, from the interface matrices (t11 = (E + H) / 2, t12 = (E - H) / 2 in
+        the scattering form S11 = t11 - t12 t11^-1 t12, S12 = t12 t11^-1, S21 = -t11^-1 t12,
+        S22 = t11^-1), the RTCM taus and the stack's Rud, Tdd. input_vec may hold several
+        columns (one solve per column). Complete orthogonal decomposition solves are LU
+        solves here (the same for the non-singular S11)."""
+        stack, thickness = self.stack, self.thickness
+        nL = len(stack)
+        N = self.Tdd.shape[0]
+        v = np.asarray(input_vec, dtype=complex)
+        S11, S12, S21, S22, phases, REF = [], [], [], [], [], [np.zeros((N, N), dtype=complex)]
+        Rud = REF[0]
+        for_inv_s22 = None
+        for i in range(nL - 1):
+            index = nL - i - 1
+            if i == 0:
+                phi = np.ones(N, dtype=complex)
+            else:
+                phi = np.exp(1j * self.gamma[stack[index]] * thickness[index])
+            phases.append(phi)
+            E, H = self.ifE[index - 1], self.ifH[index - 1]
+            t11 = 0.5 * (E + H)
+            t12 = 0.5 * (E - H)
+            inv_t22 = np.linalg.inv(t11)
+            inv_t22_t12 = np.linalg.solve(t11, t12)
+            S11.append(t11 - t12 @ inv_t22_t12)
+            S12.append(t12 @ inv_t22)
+            S21.append(-inv_t22_t12)
+            S22.append(inv_t22)
+            if i == 0:
+                for_inv_s22 = t11
+            waveRud = phi[:, None] * Rud * phi[None, :]
+            G = H - H @ waveRud
+            Rud = -2.0 * G @ self.taus[i]
+            Rud = Rud + np.eye(N)
+            REF.append(Rud)
+        R = self.Rud @ v
+        T = self.Tdd @ v
+        nin = nL - 2
+        c_m = [None] * nin
+        c_p = [None] * nin
+        d = [None] * nin
+        c_m[nin - 1] = S12[0] @ for_inv_s22 @ T
+        temp = np.linalg.solve(S11[nL - 2], R - S12[nL - 2] @ v)
+        c_p[0] = S21[nL - 2] @ temp + S22[nL - 2] @ v
+        ph = phases[nL - 2] if v.ndim == 1 else phases[nL - 2][:, None]
+        d[nin - 1] = ph * c_p[0]
+        for i in range(nin - 2, -1, -1):
+            c_m[nin - i - 2] = REF[i + 2] @ d[i + 1]
+            tempi = np.linalg.solve(S11[i + 1], c_m[nin - i - 2] - S12[i + 1] @ d[i + 1])
+            c_p[nin - i - 1] = S21[i + 1] @ tempi + S22[i + 1] @ d[i + 1]
+            ph = phases[i + 1] if v.ndim == 1 else phases[i + 1][:, None]
+            d[i] = ph * c_p[nin - i - 1]
+        return c_m, c_p
 
 
 @dataclass

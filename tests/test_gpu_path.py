@@ -553,6 +553,43 @@ def test_legacy_rcwa_solver(ctx):
         g.addLayer(x * 1.1, y, e)
 
 
+def test_legacy_intermediate_field(ctx):
+    """RCWASolver::evaluateIntermediateField on the device against the oracle (pinned by
+    interface continuity): the amplitudes depend on each side's eigenvector basis, so the
+    comparison is on the fields they describe, E_type(j) c_m[j] and E_type(j) c_p[j] of every
+    inner layer, for one incident harmonic field per column (input = E_first^-1 e in each
+    side's own basis), 1e-9 of the largest; a unit input mode through the C++-style call."""
+    nX, nY = 2, 1
+    types = [_legacy_layer(2.1), _legacy_layer(12.0), _legacy_layer(4.0 + 0.2j)]
+    g = api.RCWASolver(nX, nY, ctx=ctx)
+    o = O.RCWASolver(nX, nY)
+    for L in types:
+        g.addLayer(*L)
+        o.addLayer(*L)
+    stack, th = [0, 1, 2, 1, 2, 0], [0.0, 0.37, 0.21, 0.15, 0.3, 0.0]
+    g.solve(1.3, stack, th)
+    o.solve(1.3, stack, th)
+    N = 2 * (2 * nX + 1) * (2 * nY + 1)
+    rng = np.random.default_rng(4)
+    e = rng.normal(size=(N, 3)) + 1j * rng.normal(size=(N, 3))
+    cmg, cpg = g.evaluateIntermediateField(np.linalg.solve(g.eigvecE(stack[0]), e), len(stack))
+    cmo, cpo = o.evaluateIntermediateField(np.linalg.solve(o.E[stack[0]], e))
+    assert len(cmg) == len(cmo) == len(stack) - 2
+    for j in range(len(stack) - 2):
+        t = stack[j + 1]
+        for a, b in ((g.eigvecE(t) @ cmg[j], o.E[t] @ cmo[j]), (g.eigvecE(t) @ cpg[j], o.E[t] @ cpo[j])):
+            assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max(), (j, np.abs(a - b).max() / np.abs(b).max())
+    # inputMode variant: the unit vector of mode 3 in the device's own basis
+    cm3, cp3 = g.evaluateIntermediateField(3, len(stack))
+    e3 = np.zeros(N, complex)
+    e3[3] = 1.0
+    cmv, cpv = g.evaluateIntermediateField(e3, len(stack))
+    assert all(np.array_equal(a, b) for a, b in zip(cm3, cmv))
+    with pytest.raises(api.VrcError):
+        g.solve(1.3, [0, 1], [0.0, 0.0])
+        g.evaluateIntermediateField(0, 2)
+
+
 def test_edge_cases_and_argument_errors(ctx, tmp_path):
     """Empty batches and zero-sized requests return empty results; a zero-thickness slab is
     transparent (Tuu = Tdd = I, Rud = Rdu = 0 between equal media); malformed requests are

@@ -590,6 +590,63 @@ def test_legacy_homogeneous_and_translation():
     assert np.abs(X1 - X3).max() > 1e-2 * np.abs(X1).max()
 
 
+def _interface_residuals(s, stack, th, v, cm, cp):
+    """Largest relative mismatch of the field continuity at every interface of the stack:
+    with t11 = (E + H) / 2, t12 = (E - H) / 2 of the interface between entries j and j + 1,
+    the left side's backward / forward amplitudes a+, a- and the right side's b+, b- must
+    satisfy a+ = t11 b+ + t12 b-, a- = t12 b+ + t11 b- (E a (a+ + a-) = E b (b+ + b-),
+    H a (a+ - a-) = H b (b+ - b-)). Layer 0: a- = v (incident), a+ = Rud v; inner layer
+    j >= 1: forward c_p[j-1] at its first interface, backward c_m[j-1] at its second,
+    carried across the layer by exp(i gamma t); last entry: b- = Tdd v, b+ = 0."""
+    nL = len(stack)
+    nin = nL - 2
+    N = s.Tdd.shape[0]
+
+    def phi(j):
+        return np.exp(1j * s.gamma[stack[j]] * th[j])
+
+    def res(ap, am, bp, bm, j):
+        E, H = s.ifE[j], s.ifH[j]
+        t11, t12 = 0.5 * (E + H), 0.5 * (E - H)
+        return max(np.abs(ap - (t11 @ bp + t12 @ bm)).max() / np.abs(ap).max(),
+                   np.abs(am - (t12 @ bp + t11 @ bm)).max() / np.abs(am).max())
+
+    out = [res(s.Rud @ v, v, phi(1) * cm[0], cp[0], 0)]
+    for j in range(1, nin):
+        out.append(res(cm[j - 1], phi(j) * cp[j - 1], phi(j + 1) * cm[j], cp[j], j))
+    out.append(res(cm[nin - 1], phi(nin) * cp[nin - 1], np.zeros(N), s.Tdd @ v, nin))
+    return max(out)
+
+
+def test_legacy_intermediate_field_interface_continuity():
+    """RCWASolver::evaluateIntermediateField's amplitudes satisfy the tangential-field
+    continuity at every interface of a 5-entry stack with three layer types (PML, lossy
+    layer), for a random incident vector and for an input mode; planted mistake: one RTCM
+    tau scaled by 1.01 breaks continuity at the inner interfaces."""
+    A, B, C = _legacy_layer(2.1), _legacy_layer(12.0), _legacy_layer(4.0 + 0.2j)
+    s = O.RCWASolver(1, 1)
+    for L in (A, B, C):
+        s.addLayer(*L)
+    stack, th = [0, 1, 2, 1, 0], [0.0, 0.37, 0.21, 0.15, 0.0]
+    s.solve(1.3, stack, th)
+    N = s.Tdd.shape[0]
+    rng = np.random.default_rng(0)
+    v = rng.normal(size=N) + 1j * rng.normal(size=N)
+    cm, cp = s.evaluateIntermediateField(v)
+    assert len(cm) == len(cp) == 3
+    assert _interface_residuals(s, stack, th, v, cm, cp) < 1e-12
+    e4 = np.zeros(N, dtype=complex)
+    e4[4] = 1.0
+    cm4, cp4 = s.evaluateIntermediateField(e4)
+    assert _interface_residuals(s, stack, th, e4, cm4, cp4) < 1e-12
+    # columns at once equal one column at a time
+    cmb, cpb = s.evaluateIntermediateField(np.stack([v, e4], axis=1))
+    assert np.abs(cmb[1][:, 0] - cm[1]).max() < 1e-13 * np.abs(cm[1]).max()
+    s.taus[1] = s.taus[1] * 1.01
+    cmx, cpx = s.evaluateIntermediateField(v)
+    assert _interface_residuals(s, stack, th, v, cmx, cpx) > 1e-4
+
+
 def test_convergence_rcwa_varrcwa_differential():
     g = G.metasurface(1)
     s = O.VarLayerSampler(g["lam"], g["nx"], g["ny"], g["x"], g["y"], g["eps"], False)

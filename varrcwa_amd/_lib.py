@@ -124,6 +124,7 @@ def load():
         "vrc_rcwa_solver_add_layer": (i, vp, C.POINTER(VrcLayer)),
         "vrc_rcwa_solver_solve": (i, vp, d, i, vp, vp, vp),
         "vrc_rcwa_solver_get": (i, vp, i, i, vp),
+        "vrc_rcwa_solver_intermediate_field": (i, vp, i, c_double_p, c_double_p, c_double_p),
         "vrc_rcwa_solver_dims": (i, vp, c_int_p, c_int_p),
         "vrc_rcwa_solver_destroy": (i, vp),
         "vrc_modal_power": (i, vp, i, i, i, vp, C.c_longlong, vp, C.c_longlong, d, d, i, i, d, d, d, d, vp),

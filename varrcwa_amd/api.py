@@ -329,6 +329,36 @@ class RCWASolver:
     def gamma(self, layerType):
         return self._get(4, layerType)
 
+    def evaluateIntermediateField(self, input_vec, nstack: int):
+        """RCWASolver::evaluateIntermediateField (src/rcwa/RCWASolver.cpp:1373-1621) for the
+        stack of the last solve (nstack entries): input_vec (N,) or (N, k) incident amplitudes
+        of the first stack entry, or an int (inputMode: that unit vector). Returns (c_m, c_p),
+        lists of nstack - 2 arrays shaped like input_vec."""
+        N = self._dims()
+        if isinstance(input_vec, (int, np.integer)):
+            v = np.zeros(N, np.complex128)
+            v[int(input_vec)] = 1.0
+        else:
+            v = np.asarray(input_vec, np.complex128)
+        one = v.ndim == 1
+        V = np.asfortranarray(v.reshape(N, -1))
+        k = V.shape[1]
+        nin = int(nstack) - 2
+        cm = np.empty((max(nin, 1), N, k), np.complex128, order="C")
+        cp = np.empty_like(cm)
+        bufm = np.empty(max(nin, 1) * N * k, np.complex128)
+        bufp = np.empty_like(bufm)
+        dp = C.POINTER(C.c_double)
+        self.ctx.check(self.ctx.lib.vrc_rcwa_solver_intermediate_field(
+            self.h, k, V.ctypes.data_as(dp), bufm.ctypes.data_as(dp), bufp.ctypes.data_as(dp)),
+            "evaluateIntermediateField")
+        for j in range(nin):
+            cm[j] = bufm[j * N * k:(j + 1) * N * k].reshape((N, k), order="F")
+            cp[j] = bufp[j * N * k:(j + 1) * N * k].reshape((N, k), order="F")
+        out_m = [cm[j][:, 0] if one else cm[j].copy() for j in range(nin)]
+        out_p = [cp[j][:, 0] if one else cp[j].copy() for j in range(nin)]
+        return out_m, out_p
+
     def tdd(self, row, col):
         return complex(self.Tdd()[row, col])
 

@@ -10,6 +10,11 @@ The reference computes no derivatives on this path; the objective helper
 ``zeroth_order_transmission`` is the 0th-order power objective used for
 metasurface inverse design (TE+TM, field-basis transmission between the (0,0)
 harmonics of the end media).
+
+``stack_smatrix_shape(theta, problem)`` is the same for geometric parameters (e.g. one
+pillar radius per layer): ``ShapeProblem`` builds the stacks from theta and says how the
+slabs' x coordinates move with it, and the backward pass chains the closed-form
+coordinate tangents of P and Q into the reverse pass (vrc_batch_stack_vjp_shape).
 """
 from __future__ import annotations
 
@@ -120,3 +125,86 @@ def zeroth_order_transmission(S, W0, W1, nx: int, ny: int):
         R = W1m[:, idx, :].contiguous()
     T = R @ (Tuu @ X)
     return (T.abs() ** 2).sum(dim=(-2, -1))
+
+
+# ------------------------------------------------------------------ shape parameters
+def pillar_layer(r: float, cell: float, nstrips: int, eps_pillar: complex, eps_bg: complex):
+    """A circular pillar of radius r centred in a square cell, staircased on nstrips fixed
+    y strips (uniform over the cell; even, so the strips pair up about the centre): strip j
+    holds the pillar over |x - c| < w_j = sqrt(r^2 - (y_j - c)^2) (y_j its midpoint), none
+    when |y_j - c| >= r. Returns (Layer, var_index, coeff): the interior x coordinates that
+    move with r and dx/dr for each (-+ r / w at c -+ w)."""
+    if nstrips < 2 or nstrips % 2:
+        raise ValueError("nstrips must be even and >= 2")
+    c = 0.5 * cell
+    if not 0.0 < r < c:
+        raise ValueError("the radius must lie in (0, cell / 2)")
+    y = np.linspace(0.0, cell, nstrips + 1)
+    dy = np.abs(0.5 * (y[1:] + y[:-1]) - c)
+    w = np.where(dy < r, np.sqrt(np.maximum(r * r - dy * dy, 0.0)), 0.0)
+    ws = np.unique(w[w > 0])  # ascending
+    x = np.concatenate([[0.0], c - ws[::-1], c + ws, [cell]])
+    xm = 0.5 * (x[1:] + x[:-1])
+    eps = np.where(np.abs(xm[None, :] - c) < w[:, None], eps_pillar, eps_bg).astype(complex)
+    U = ws.size
+    var_index = list(range(1, 2 * U + 1))
+    coeff = np.concatenate([-r / ws[::-1], r / ws]).reshape(-1, 1)
+    return Layer(x, y, eps), var_index, coeff
+
+
+class ShapeProblem:
+    """Stacks whose slab geometry is a function of parameters theta.
+
+    build(theta) -> (instances, shape), with instances as for ``batch_stack_rcwa`` and
+    shape one entry per slab (instance-major, then layer): None, or (var_index, coeff,
+    theta_index): coordinate var_index[k] moves by coeff[k, p] per unit of theta[
+    theta_index[p]]. recompute / layer_block: the tape options of batch_stack_rcwa."""
+
+    def __init__(self, ctx: Context, nx: int, ny: int, build, chunk: int = 0, recompute: bool = False,
+                 layer_block: int = 0):
+        self.ctx, self.nx, self.ny, self.build = ctx, int(nx), int(ny), build
+        self.chunk, self.recompute, self.layer_block = int(chunk), bool(recompute), int(layer_block)
+
+
+def _shape_function():
+    import torch
+
+    class StackSMatrixShape(torch.autograd.Function):
+        @staticmethod
+        def forward(fctx, theta, problem):
+            insts, shape = problem.build(theta.detach().cpu().numpy())
+            out = batch_stack_rcwa(problem.ctx, problem.nx, problem.ny, insts, chunk=problem.chunk, want_bases=True,
+                                   tape=True, recompute=problem.recompute, layer_block=problem.layer_block)
+            fctx.tape = out["tape"]
+            fctx.shape = shape
+            fctx.chunk = problem.chunk
+            fctx.ntheta = theta.numel()
+            fctx.dtype, fctx.device = theta.dtype, theta.device
+            fctx.mark_non_differentiable(out["W0"], out["W1"])
+            return out["S"], out["W0"], out["W1"]
+
+        @staticmethod
+        def backward(fctx, gS, gW0, gW1):
+            if gS is None:
+                return None, None
+            per, _ = fctx.tape.vjp_shape(gS, [None if e is None else (e[0], e[1]) for e in fctx.shape],
+                                         chunk=fctx.chunk)
+            g = np.zeros(fctx.ntheta)
+            for e, gs in zip(fctx.shape, per):
+                if e is not None:
+                    np.add.at(g, np.asarray(e[2], dtype=np.int64), gs)
+            return torch.from_numpy(g).to(dtype=fctx.dtype, device=fctx.device), None
+
+    return StackSMatrixShape
+
+
+_FN_SHAPE = None
+
+
+def stack_smatrix_shape(theta, problem: ShapeProblem):
+    """S, W0, W1 as ``stack_smatrix``, differentiable with respect to the real shape
+    parameters theta (1-D tensor) through ``problem.build``."""
+    global _FN_SHAPE
+    if _FN_SHAPE is None:
+        _FN_SHAPE = _shape_function()
+    return _FN_SHAPE.apply(theta, problem)

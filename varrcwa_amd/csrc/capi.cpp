@@ -125,6 +125,10 @@ int vrc_tune_set(const char* key, long long value) {
     zgeev_set_aed((int)value);
     return 0;
   }
+  if (std::strcmp(key, "rcwa.group") == 0) {
+    rcwa_set_group((int)value);
+    return 0;
+  }
   if (std::strcmp(key, "stack.cascade_group") == 0) {
     stack_set_cascade_group((int)value);
     return 0;
@@ -597,6 +601,15 @@ int vrc_rcwa_solver_get(vrc_rcwa_solver* s, int which, int layer_type, double* h
   return guarded(s->ctx, [&] {
     need(host != nullptr, "bad arguments");
     s->s->get(which, layer_type, (cd*)host);
+  });
+}
+
+int vrc_rcwa_solver_intermediate_field(vrc_rcwa_solver* s, int ncols, const double* input_host, double* c_m_host,
+                                       double* c_p_host) {
+  if (!s) return -1;
+  return guarded(s->ctx, [&] {
+    need(input_host && c_m_host && c_p_host, "null buffer");
+    s->s->intermediate_field(ncols, (const cd*)input_host, (cd*)c_m_host, (cd*)c_p_host);
   });
 }
 

@@ -192,6 +192,72 @@ __global__ __launch_bounds__(256) void fourier_fft_y_kernel(FourierDev g) {
   }
 }
 
+// The same inverse for k <= TINV_SMALL_K with one wavefront per matrix and LDS sized to
+// the matrix (k x (2k + 1) complex): many matrices share a CU (the fixed 64 x 129 array of
+// the kernel below allows one). Pivot: first maximum of |re| + |im| in the column, found
+// by a wave reduction (the same row the serial search picks).
+constexpr int TINV_SMALL_K = 44;  // k (2k + 1) * 16 B <= 64 KB
+__global__ __launch_bounds__(64) void toeplitz_inv_small_kernel(int k, const cd* v, long sv, cd* out, long so,
+                                                                const int* count, int per_layer) {
+  extern __shared__ cd tsm[];
+  const int ld = 2 * k + 1;
+  const long id = blockIdx.x;
+  const int layer = (int)(id / per_layer), slot = (int)(id % per_layer);
+  if (slot >= count[layer]) return;
+  v += id * sv;
+  out += id * so;
+  const int t = threadIdx.x;
+  for (int e = t; e < k * 2 * k; e += 64) {
+    const int r = e % k, c = e / k;
+    tsm[r * ld + c] = (c < k) ? v[k - 1 + r - c] : ((c - k == r) ? mk(1.0, 0.0) : mk(0.0, 0.0));
+  }
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    double best = -1.0;
+    int p = 0x7fffffff;
+    for (int r = j + t; r < k; r += 64) {
+      const double a = cabs1(tsm[r * ld + j]);
+      if (a > best) {
+        best = a;
+        p = r;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int op = __shfl_xor(p, o, 64);
+      if (ob > best || (ob == best && op < p)) {
+        best = ob;
+        p = op;
+      }
+    }
+    if (!(p >= j && p < k)) p = j;  // no comparable candidate (NaN column): keep the diagonal
+    if (p != j) {
+      for (int c = t; c < 2 * k; c += 64) {
+        const cd tmp = tsm[j * ld + c];
+        tsm[j * ld + c] = tsm[p * ld + c];
+        tsm[p * ld + c] = tmp;
+      }
+    }
+    __syncthreads();
+    const cd inv = crecip(tsm[j * ld + j]);
+    __syncthreads();
+    for (int c = t; c < 2 * k; c += 64) tsm[j * ld + c] = cmul(tsm[j * ld + c], inv);
+    __syncthreads();
+    for (int e = t; e < k * 2 * k; e += 64) {
+      const int r = e % k, c = e / k;
+      if (r != j && c != j) tsm[r * ld + c] = csub(tsm[r * ld + c], cmul(tsm[r * ld + j], tsm[j * ld + c]));
+    }
+    __syncthreads();
+    for (int r = t; r < k; r += 64)
+      if (r != j) tsm[r * ld + j] = mk(0.0, 0.0);
+    __syncthreads();
+  }
+  for (int e = t; e < k * k; e += 64) {
+    const int r = e % k, c = e / k;
+    out[r + (long)c * k] = tsm[r * ld + k + c];
+  }
+}
+
 // Inverse of the k x k Toeplitz matrix T(r,c) = v[k-1 + r - c] (ToeplitzMatrixXcs,
 // src/rcwa/ToeplitzMatrixXcs.cpp:8-44) by Gauss-Jordan elimination with partial
 // pivoting in LDS; one workgroup per matrix. Inactive slots (beyond the layer's
@@ -247,6 +313,19 @@ __global__ __launch_bounds__(256) void toeplitz_inv_kernel(int k, const cd* v, l
   for (int e = t; e < k * k; e += 256) {
     int r = e % k, c = e / k;
     out[r + (long)c * k] = M[r][k + c];
+  }
+}
+
+static void toeplitz_inverses(hipStream_t s, int k, const cd* v, long sv, cd* out, long so, const int* count,
+                              int per_layer, int nmat) {
+  if (nmat <= 0) return;
+  if (k <= TINV_SMALL_K) {
+    const size_t lds = sizeof(cd) * (size_t)k * (2 * k + 1);
+    toeplitz_inv_small_kernel<<<nmat, 64, lds, s>>>(k, v, sv, out, so, count, per_layer);
+    launch_check("toeplitz_inv_small");
+  } else {
+    toeplitz_inv_kernel<<<nmat, 256, 0, s>>>(k, v, sv, out, so, count, per_layer);
+    launch_check("toeplitz_inv");
   }
 }
 
@@ -315,12 +394,8 @@ void fourier_tables(hipStream_t s, const FourierDev& g, int nlayers) {
     launch_check("fourier_fft_y");
   }
   if (g.nx > 64 || g.ny > 64) throw ArgFailure("harmonic count per axis above 64 is not supported");
-  toeplitz_inv_kernel<<<nlayers * (int)g.MY, 256, 0, s>>>(g.nx, g.FXi, M1, g.tFFx, (long)g.nx * g.nx, g.ncy,
-                                                           (int)g.MY);
-  launch_check("toeplitz_inv_x");
-  toeplitz_inv_kernel<<<nlayers * (int)g.MX, 256, 0, s>>>(g.ny, g.FYi, M2, g.tFFy, (long)g.ny * g.ny, g.ncx,
-                                                           (int)g.MX);
-  launch_check("toeplitz_inv_y");
+  toeplitz_inverses(s, g.nx, g.FXi, M1, g.tFFx, (long)g.nx * g.nx, g.ncy, (int)g.MY, nlayers * (int)g.MY);
+  toeplitz_inverses(s, g.ny, g.FYi, M2, g.tFFy, (long)g.ny * g.ny, g.ncx, (int)g.MX, nlayers * (int)g.MX);
 }
 
 void fourier_assemble(hipStream_t s, const FourierDev& g, int nlayers, cd* fe11, cd* fe22, cd* fe33, long sfe) {

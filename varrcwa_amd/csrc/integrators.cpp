@@ -426,11 +426,12 @@ void init_identity_S(vrc_ctx& c, int N, SMat& S, int batch) {
 
 // RCWAIntegrator::compute (src/core/RCWAIntegrator.cpp:20-87) over explicit slices
 // (midpoint z, width). The slices are independent until the cascade, so their samples,
-// eigensolves and slab S-matrices run in groups of up to RCWA_GROUP per launch (one
+// eigensolves and slab S-matrices run in groups of up to g_rcwa_group per launch (one
 // eigensolve per workgroup), the two end media riding with the first group; the star
 // products then follow in slice order, as the reference cascades them. Each matrix's
 // result does not depend on the group it is solved in.
-constexpr int RCWA_GROUP = 64;
+static int g_rcwa_group = 64;
+void rcwa_set_group(int g) { g_rcwa_group = g < 1 ? 1 : g; }
 void Integrator::rcwa(const std::vector<std::pair<double, double>>& slices, bool) {
   vrc_ctx& c = *c_;
   const int N = Nd_;
@@ -438,7 +439,7 @@ void Integrator::rcwa(const std::vector<std::pair<double, double>>& slices, bool
   const long NN = (long)N * N;
   tree_.clear();
   const int ns = (int)slices.size();
-  const int G = std::max(1, std::min(ns, RCWA_GROUP));
+  const int G = std::max(1, std::min(ns, g_rcwa_group));
   const int GE = G + 2;  // + the two end media in the first launch
   DevBuf Pb = c.ws.get(sizeof(cd) * NN * GE), Qb = c.ws.get(sizeof(cd) * NN * GE);
   DevBuf Wb = c.ws.get(sizeof(cd) * NN * GE), Vb = c.ws.get(sizeof(cd) * NN * GE), gb = c.ws.get(sizeof(cd) * N * GE);
@@ -528,7 +529,8 @@ struct Seg {
 struct Done {
   double za, zb;
   Ref R;
-  std::shared_ptr<DevBuf> T;  // Tdd, Tuu, Rud, Rdu of the segment (in the basis of R)
+  std::shared_ptr<DevBuf> T;  // the level's segment S-matrices
+  long off;                   // this segment's Tdd, Tuu, Rud, Rdu (basis of R) at T + off
 };
 
 LevelP new_level(vrc_ctx& c, int N, int count) {
@@ -605,28 +607,32 @@ void Integrator::redheffer_many(const std::vector<Integrator*>& its, double pe, 
   }
   std::vector<std::vector<Done>> done(I);
   while (!pending.empty()) {
-    // ---- evaluate every pending segment, one readback
+    // ---- evaluate every pending segment in one batch, one readback
     const int P = (int)pending.size();
-    std::vector<double> hs(2 * (size_t)P);
+    std::vector<RhJob> jobs(P);
     for (int q = 0; q < P; ++q) {
       const Seg& sg = pending[q];
-      const double k0 = its[sg.inst]->s_->k0();
-      hs[2 * q] = 0.5 * (sg.zb - sg.za) / k0;
-      hs[2 * q + 1] = (sg.zb - sg.za) / k0;
-    }
-    DevBuf hsb = upload(c, hs.data(), sizeof(double) * hs.size());
-    DevBuf errb = c.ws.get(sizeof(double) * RH_ERR_BLOCKS * (size_t)P);
-    std::vector<std::shared_ptr<DevBuf>> Ts(P);
-    for (int q = 0; q < P; ++q) {
-      const Seg& sg = pending[q];
-      Ts[q] = std::make_shared<DevBuf>(c.ws.get(sizeof(cd) * 4 * NN));
-      cd* T = Ts[q]->as<cd>();
-      const cd* P3[3] = {sg.a.P(NN), sg.m.P(NN), sg.b.P(NN)};
-      const cd* Q3[3] = {sg.a.Q(NN), sg.m.Q(NN), sg.b.Q(NN)};
+      RhJob& j = jobs[q];
+      j.k0 = its[sg.inst]->s_->k0();
+      j.z0 = sg.za;
+      j.z1 = sg.zb;
       const Smp pr{sg.R.lv, sg.R.j};
-      redheffer_perturbation_enqueue(c, N, its[sg.inst]->s_->k0(), sg.za, sg.zb, pr.P(NN), pr.Q(NN), sg.R.sys(N),
-                                     P3, Q3, T, T + NN, T + 2 * NN, T + 3 * NN, hsb.as<double>() + 2 * q,
-                                     errb.as<double>() + (size_t)RH_ERR_BLOCKS * q);
+      j.Pr = pr.P(NN);
+      j.Qr = pr.Q(NN);
+      j.R = sg.R.sys(N);
+      j.P3[0] = sg.a.P(NN);
+      j.P3[1] = sg.m.P(NN);
+      j.P3[2] = sg.b.P(NN);
+      j.Q3[0] = sg.a.Q(NN);
+      j.Q3[1] = sg.m.Q(NN);
+      j.Q3[2] = sg.b.Q(NN);
+    }
+    auto Tl = std::make_shared<DevBuf>(c.ws.get(sizeof(cd) * 4 * NN * P));
+    DevBuf errb = c.ws.get(sizeof(double) * RH_ERR_BLOCKS * (size_t)P);
+    for (int q0 = 0; q0 < P; q0 += 128) {  // bounded work space per batch
+      const int q1 = std::min(P, q0 + 128);
+      redheffer_perturbation_batch(c, N, std::vector<RhJob>(jobs.begin() + q0, jobs.begin() + q1),
+                                   Tl->as<cd>() + 4 * NN * q0, errb.as<double>() + (size_t)RH_ERR_BLOCKS * q0);
     }
     std::vector<double> eh(RH_ERR_BLOCKS * (size_t)P);
     VRC_HIP(hipMemcpyAsync(eh.data(), errb.as<double>(), sizeof(double) * eh.size(), hipMemcpyDeviceToHost,
@@ -645,12 +651,12 @@ void Integrator::redheffer_many(const std::vector<Integrator*>& its, double pe, 
                        std::to_string(sg.zb) + "], error " + std::to_string(err) + ", depth " +
                        std::to_string(max_depth) + ")";
         }
-        done[sg.inst].push_back(Done{sg.za, sg.zb, sg.R, Ts[q]});
+        done[sg.inst].push_back(Done{sg.za, sg.zb, sg.R, Tl, 4 * NN * q});
       } else {
         split.push_back(q);
       }
     }
-    Ts.clear();
+    Tl.reset();
     const int S = (int)split.size();
     std::vector<Seg> next;
     if (S > 0) {
@@ -690,7 +696,7 @@ void Integrator::redheffer_many(const std::vector<Integrator*>& its, double pe, 
     const cd* leftW = it.W0_.as<cd>();
     const cd* leftV = it.V0_.as<cd>();
     for (auto& d : D) {
-      cd* T0 = d.T->as<cd>();
+      cd* T0 = d.T->as<cd>() + d.off;
       SMat T;
       T.Tdd = T0;
       T.Tuu = T0 + NN;

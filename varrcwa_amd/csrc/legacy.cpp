@@ -132,6 +132,7 @@ void LegacySolver::solve(double lambda, const std::vector<int>& stack, const std
   }
   LUw = DevBuf();
   LUv = DevBuf();
+  REF_ = c_->ws.get(sizeof(cd) * NN * (nL - 1));
   // RTCM (src/rcwa/RCWASolver.cpp:343-457), from the last layer to the first
   Tdd_ = c_->ws.get(sizeof(cd) * NN);
   Rud_ = c_->ws.get(sizeof(cd) * NN);
@@ -167,9 +168,103 @@ void LegacySolver::solve(double lambda, const std::vector<int>& stack, const std
     zgemm_strided(st, OP_N, OP_N, N, N, N, mk(-2.0, 0.0), G.as<cd>(), N, 0, tau.as<cd>(), N, 0, ZERO, Rud, N, 0, 1);
     zadd_diag(st, N, ONE, nullptr, 0, Rud, N, 0, 1);
     zgemm_strided(st, OP_N, OP_N, N, N, N, mk(2.0, 0.0), wT.as<cd>(), N, 0, tau.as<cd>(), N, 0, ZERO, Tdd, N, 0, 1);
+    // Rud of this step: the reflection REFMatrices[i + 1] of the intermediate-field pass
+    zgeam(st, N, N, ONE, Rud, N, 0, ZERO, nullptr, 0, 0, REF_.as<cd>() + (long)i * NN, N, 0, 1);
   }
+  IE_ = std::move(IE);
+  IH_ = std::move(IH);
+  ds_ = std::move(ds);
+  stack_ = stack;
+  imap_ = imap;
   VRC_HIP(hipStreamSynchronize(st));
   solved_ = true;
+}
+
+// The amplitude pass of RCWASolver::evaluateIntermediateField (src/rcwa/RCWASolver.cpp:1478-1500)
+// on all input columns at once. Interface i (between stack entries index - 1 and index,
+// index = nL - 1 - i) in scattering form: t11 = (E + H) / 2, t12 = (E - H) / 2,
+// S11 = t11 - t12 t11^-1 t12, S12 = t12 t11^-1, S21 = -t11^-1 t12, S22 = t11^-1, applied
+// as solves with t11 and S11; REFMatrices[i + 1] is the RTCM step's Rud. The first value,
+// S12[0] t11 T, is t12 T (the same product).
+void LegacySolver::intermediate_field(int ncols, const cd* input, cd* c_m, cd* c_p) {
+  if (!solved_) throw ArgFailure("the scatter matrix has not been solved");
+  const int nL = (int)stack_.size();
+  if (nL < 3) throw ArgFailure("the stack has no inner layer");
+  if (ncols < 1) throw ArgFailure("at least one input column expected");
+  hipStream_t st = c_->stream;
+  const int N = 2 * nx_ * ny_, K = ncols, nin = nL - 2;
+  const long NN = (long)N * N, NK = (long)N * K;
+  DevBuf vb = c_->ws.get(sizeof(cd) * NK);
+  c_->pinned.upload(st, vb.ptr, input, sizeof(cd) * NK);
+  const cd* v = vb.as<cd>();
+  DevBuf outm = c_->ws.get(sizeof(cd) * NK * nin), outp = c_->ws.get(sizeof(cd) * NK * nin);
+  cd* cm = outm.as<cd>();
+  cd* cp = outp.as<cd>();
+  DevBuf t11b = c_->ws.get(sizeof(cd) * NN), t12b = c_->ws.get(sizeof(cd) * NN), Xb = c_->ws.get(sizeof(cd) * NN),
+         S11b = c_->ws.get(sizeof(cd) * NN), piv1 = c_->ws.get(sizeof(int) * N), piv2 = c_->ws.get(sizeof(int) * N),
+         inf = c_->ws.get(sizeof(int) * 2), phi = c_->ws.get(sizeof(cd) * N), y = c_->ws.get(sizeof(cd) * NK),
+         z = c_->ws.get(sizeof(cd) * NK), d = c_->ws.get(sizeof(cd) * NK);
+  cd *t11 = t11b.as<cd>(), *t12 = t12b.as<cd>(), *X = Xb.as<cd>(), *S11 = S11b.as<cd>();
+  const cd half = mk(0.5, 0.0), mhalf = mk(-0.5, 0.0), mone = mk(-1.0, 0.0);
+  // t11 (LU in place), t12, X = t11^-1 t12, S11 = t11 - t12 X (LU in place) of interface i
+  auto interface = [&](int i) {
+    const int index = nL - 1 - i;
+    const cd* E = IE_.as<cd>() + (long)imap_[index - 1] * NN;
+    const cd* H = IH_.as<cd>() + (long)imap_[index - 1] * NN;
+    zgeam(st, N, N, half, E, N, 0, half, H, N, 0, t11, N, 0, 1);
+    zgeam(st, N, N, half, E, N, 0, mhalf, H, N, 0, t12, N, 0, 1);
+    zgeam(st, N, N, ONE, t11, N, 0, ZERO, nullptr, 0, 0, S11, N, 0, 1);
+    fill_int(st, inf.as<int>(), 0, 2);
+    zgetrf(st, N, t11, N, 0, piv1.as<int>(), 0, inf.as<int>(), 1, c_->ws);
+    zgetrs(st, N, N, t11, N, 0, piv1.as<int>(), 0, t12, N, 0, X, N, 0, 1, c_->ws);
+    zgemm_strided(st, OP_N, OP_N, N, N, N, mone, t12, N, 0, X, N, 0, ONE, S11, N, 0, 1);
+    zgetrf(st, N, S11, N, 0, piv2.as<int>(), 0, inf.as<int>() + 1, 1, c_->ws);
+  };
+  auto mm = [&](cd a, const cd* A, const cd* B, cd b, cd* C) {
+    zgemm_strided(st, OP_N, OP_N, N, K, N, a, A, N, 0, B, N, 0, b, C, N, 0, 1);
+  };
+  // phase of stack entry `index` (RCWASolver uses exp(i gamma t), gamma = sqrt(eig) / k0)
+  auto phase = [&](int index) {
+    phases(st, N, gam_.as<cd>() + (long)stack_[index] * N, 0, ds_.as<double>() + index, phi.as<cd>(), 0, 1);
+  };
+  // one interface of the pass: forward amplitude a (in), backward am (in): temp = S11^-1 (am -
+  // S12 a), out = S21 temp + S22 a = t11^-1 (a - t12 temp)
+  auto through = [&](int i, const cd* a, const cd* am, cd* out) {
+    interface(i);
+    zgetrs(st, N, K, t11, N, 0, piv1.as<int>(), 0, a, N, 0, y.as<cd>(), N, 0, 1, c_->ws);  // t11^-1 a
+    zgeam(st, N, K, ONE, am, N, 0, ZERO, nullptr, 0, 0, z.as<cd>(), N, 0, 1);
+    mm(mone, t12, y.as<cd>(), ONE, z.as<cd>());                                          // am - S12 a
+    zgetrs(st, N, K, S11, N, 0, piv2.as<int>(), 0, z.as<cd>(), N, 0, y.as<cd>(), N, 0, 1, c_->ws);  // temp
+    zgeam(st, N, K, ONE, a, N, 0, ZERO, nullptr, 0, 0, z.as<cd>(), N, 0, 1);
+    mm(mone, t12, y.as<cd>(), ONE, z.as<cd>());                                          // a - t12 temp
+    zgetrs(st, N, K, t11, N, 0, piv1.as<int>(), 0, z.as<cd>(), N, 0, out, N, 0, 1, c_->ws);
+  };
+  // c_m[nin - 1] = t12_0 Tdd v (interface 0: the last inner layer and the last layer)
+  {
+    const int index = nL - 1;
+    const cd* E = IE_.as<cd>() + (long)imap_[index - 1] * NN;
+    const cd* H = IH_.as<cd>() + (long)imap_[index - 1] * NN;
+    zgeam(st, N, N, half, E, N, 0, mhalf, H, N, 0, t12, N, 0, 1);
+    mm(ONE, Tdd_.as<cd>(), v, ZERO, z.as<cd>());
+    mm(ONE, t12, z.as<cd>(), ZERO, cm + (long)(nin - 1) * NK);
+  }
+  // first inner layer: forward amplitude from the incident v and reflected Rud v
+  DevBuf Rv = c_->ws.get(sizeof(cd) * NK);
+  mm(ONE, Rud_.as<cd>(), v, ZERO, Rv.as<cd>());
+  through(nL - 2, v, Rv.as<cd>(), cp);
+  phase(1);
+  zdscale(st, N, K, phi.as<cd>(), 0, nullptr, 0, 0, cp, N, 0, d.as<cd>(), N, 0, 1);
+  for (int i = nin - 2; i >= 0; --i) {
+    cd* cmi = cm + (long)(nin - i - 2) * NK;
+    cd* cpi = cp + (long)(nin - i - 1) * NK;
+    mm(ONE, REF_.as<cd>() + (long)(i + 1) * NN, d.as<cd>(), ZERO, cmi);  // REFMatrices[i + 2]
+    through(i + 1, d.as<cd>(), cmi, cpi);
+    phase(nL - 1 - (i + 1));
+    zdscale(st, N, K, phi.as<cd>(), 0, nullptr, 0, 0, cpi, N, 0, d.as<cd>(), N, 0, 1);
+  }
+  VRC_HIP(hipMemcpyAsync(c_m, cm, sizeof(cd) * NK * nin, hipMemcpyDeviceToHost, st));
+  VRC_HIP(hipMemcpyAsync(c_p, cp, sizeof(cd) * NK * nin, hipMemcpyDeviceToHost, st));
+  VRC_HIP(hipStreamSynchronize(st));
 }
 
 void LegacySolver::get(int which, int k, cd* host) const {

@@ -913,6 +913,146 @@ void project_right(vrc_ctx& c, int N, int batch, SMat& S, const LUPair& LUl, con
 // ===================================================================== (d) varRCWA segment
 // Perturbation of one segment [z0, z1] around the reference (Pr, Qr, W, V, gamma):
 // produces (tdd, tuu, rud, rdu) and the max-modulus error estimate (returned on host).
+// dst[b] = the nbytes at src[b] (one pointer per batch entry; nbytes a multiple of 8).
+__global__ void gather_ptr_kernel(long nwords, const double* const* src, double* dst) {
+  const long b = blockIdx.y;
+  const double* a = src[b];
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nwords; e += (long)gridDim.x * blockDim.x)
+    dst[b * nwords + e] = a[e];
+}
+
+static void gather_ptrs(vrc_ctx& c, const std::vector<const void*>& src, size_t nbytes, void* dst) {
+  const int B = (int)src.size();
+  if (B == 0) return;
+  DevBuf p = c.ws.get(sizeof(void*) * B);
+  c.pinned.upload(c.stream, p.ptr, src.data(), sizeof(void*) * B);
+  const long nw = (long)(nbytes / 8);
+  const long blocks = std::min<long>((nw + 255) / 256, 1024);
+  gather_ptr_kernel<<<dim3((unsigned)blocks, B), 256, 0, c.stream>>>(nw, (const double* const*)p.ptr, (double*)dst);
+  launch_check("gather_ptr");
+}
+
+// simpson_kernel over a batch of segments (blockIdx.y): inputs at stride NN (phases at
+// stride n), outputs T[b] = [tdd | tuu | rud | rdu] at stride 4 NN, c0[b] per segment,
+// RH_ERR_BLOCKS maxima per segment.
+__global__ __launch_bounds__(256) void simpson_batch_kernel(int n, const cd* dA, const cd* dB, const cd* ph,
+                                                            const cd* pf, const cd* c0v, cd* T, double* err) {
+  const long b = blockIdx.y;
+  const long NN = (long)n * n;
+  const cd* dA0 = dA + b * 3 * NN;
+  const cd* dA1 = dA0 + NN;
+  const cd* dA2 = dA1 + NN;
+  const cd* dB0 = dB + b * 3 * NN;
+  const cd* dB1 = dB0 + NN;
+  const cd* dB2 = dB1 + NN;
+  const cd* phb = ph + b * n;
+  const cd* pfb = pf + b * n;
+  const cd c0 = c0v[b];
+  cd* tdd = T + b * 4 * NN;
+  cd* tuu = tdd + NN;
+  cd* rud = tuu + NN;
+  cd* rdu = rud + NN;
+  __shared__ double red[256];
+  double m = 0.0;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < NN; e += (long)gridDim.x * 256L) {
+    const int r = (int)(e % n), cc = (int)(e / n);
+    const cd phr = phb[r], phc = phb[cc], pfr = pfb[r], pfc = pfb[cc];
+    const cd h4 = cscale(cmul(phr, phc), 4.0);
+    cd a = cmul(c0, cadd(cadd(cmul(pfr, dA0[e]), cmul(h4, dA1[e])), cmul(dA2[e], pfc)));
+    m = fmax(m, cabsv(a));
+    if (r == cc) a = cadd(a, pfr);
+    tuu[e] = a;
+    cd q = cneg(cmul(c0, cadd(cadd(cmul(cmul(pfr, dB0[e]), pfc), cmul(h4, dB1[e])), dB2[e])));
+    m = fmax(m, cabsv(q));
+    rud[e] = q;
+    cd d = cneg(cmul(c0, cadd(cadd(dB0[e], cmul(h4, dB1[e])), cmul(cmul(pfr, dB2[e]), pfc))));
+    m = fmax(m, cabsv(d));
+    rdu[e] = d;
+    cd f = cmul(c0, cadd(cadd(cmul(dA0[e], pfc), cmul(h4, dA1[e])), cmul(pfr, dA2[e])));
+    m = fmax(m, cabsv(f));
+    if (r == cc) f = cadd(f, pfr);
+    tdd[e] = f;
+  }
+  const int t = threadIdx.x;
+  red[t] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] = fmax(red[t], red[t + w]);
+    __syncthreads();
+  }
+  if (t == 0) err[b * gridDim.x + blockIdx.x] = red[0];
+}
+
+// redheffer_perturbation_enqueue of a batch of segments with their own references and
+// samples: the operands are gathered into strided batches, then every step is one
+// batched launch. T: [tdd | tuu | rud | rdu] per segment (stride 4 N^2); err:
+// RH_ERR_BLOCKS maxima per segment.
+void redheffer_perturbation_batch(vrc_ctx& c, int N, const std::vector<RhJob>& jobs, cd* T, double* err) {
+  hipStream_t s = c.stream;
+  const int B = (int)jobs.size();
+  if (B == 0) return;
+  const long NN = (long)N * N;
+  const size_t M = sizeof(cd) * NN, Vn = sizeof(cd) * N, Pn = sizeof(int) * N;
+  auto col = [&](auto f) {
+    std::vector<const void*> v(B);
+    for (int b = 0; b < B; ++b) v[b] = f(jobs[b]);
+    return v;
+  };
+  DevBuf Wb = c.ws.get(M * B), Vb = c.ws.get(M * B), LWb = c.ws.get(M * B), LVb = c.ws.get(M * B);
+  DevBuf pWb = c.ws.get(std::max<size_t>(Pn, 8) * B + 8), pVb = c.ws.get(std::max<size_t>(Pn, 8) * B + 8);
+  DevBuf gb = c.ws.get(Vn * B);
+  gather_ptrs(c, col([](const RhJob& j) { return (const void*)j.R.W; }), M, Wb.ptr);
+  gather_ptrs(c, col([](const RhJob& j) { return (const void*)j.R.V; }), M, Vb.ptr);
+  gather_ptrs(c, col([](const RhJob& j) { return (const void*)j.R.LU.W; }), M, LWb.ptr);
+  gather_ptrs(c, col([](const RhJob& j) { return (const void*)j.R.LU.V; }), M, LVb.ptr);
+  gather_ptrs(c, col([](const RhJob& j) { return (const void*)j.R.gam; }), Vn, gb.ptr);
+  // pivots: N ints (gathered as whole 8-byte words; N even as N = 2 nx ny)
+  gather_ptrs(c, col([](const RhJob& j) { return (const void*)j.R.LU.pW; }), Pn, pWb.ptr);
+  gather_ptrs(c, col([](const RhJob& j) { return (const void*)j.R.LU.pV; }), Pn, pVb.ptr);
+  DevBuf Prb = c.ws.get(M * B), Qrb = c.ws.get(M * B);
+  gather_ptrs(c, col([](const RhJob& j) { return (const void*)j.Pr; }), M, Prb.ptr);
+  gather_ptrs(c, col([](const RhJob& j) { return (const void*)j.Qr; }), M, Qrb.ptr);
+  DevBuf dAb = c.ws.get(3 * M * B), dBb = c.ws.get(3 * M * B), Xb = c.ws.get(M * B), Yb = c.ws.get(M * B);
+  cd *dA = dAb.as<cd>(), *dB = dBb.as<cd>(), *X = Xb.as<cd>(), *Y = Yb.as<cd>();
+  for (int k = 0; k < 3; ++k) {
+    // a = W^{-1} (P_k - Pr) V ; b = V^{-1} (Q_k - Qr) W   (per segment, stride 3 NN in dA, dB)
+    gather_ptrs(c, col([k](const RhJob& j) { return (const void*)j.P3[k]; }), M, X);
+    zgeam(s, N, N, ONE, X, N, NN, MONE, Prb.as<cd>(), N, NN, X, N, NN, B);
+    zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, X, N, NN, Vb.as<cd>(), N, NN, ZERO, Y, N, NN, B);
+    zgetrs(s, N, N, LWb.as<cd>(), N, NN, pWb.as<int>(), N, Y, N, NN, dA + k * NN, N, 3 * NN, B, c.ws);
+    gather_ptrs(c, col([k](const RhJob& j) { return (const void*)j.Q3[k]; }), M, X);
+    zgeam(s, N, N, ONE, X, N, NN, MONE, Qrb.as<cd>(), N, NN, X, N, NN, B);
+    zgemm_strided(s, OP_N, OP_N, N, N, N, ONE, X, N, NN, Wb.as<cd>(), N, NN, ZERO, Y, N, NN, B);
+    zgetrs(s, N, N, LVb.as<cd>(), N, NN, pVb.as<int>(), N, Y, N, NN, dB + k * NN, N, 3 * NN, B, c.ws);
+    // dA = a + b ; dB = a - b
+    zgeam(s, N, N, ONE, dA + k * NN, N, 3 * NN, ONE, dB + k * NN, N, 3 * NN, X, N, NN, B);
+    zgeam(s, N, N, ONE, dA + k * NN, N, 3 * NN, MONE, dB + k * NN, N, 3 * NN, dB + k * NN, N, 3 * NN, B);
+    zgeam(s, N, N, ONE, X, N, NN, ZERO, nullptr, 0, 0, dA + k * NN, N, 3 * NN, B);
+  }
+  std::vector<double> hh(B), hf(B);
+  std::vector<cd> c0(B);
+  for (int b = 0; b < B; ++b) {
+    const RhJob& j = jobs[b];
+    hh[b] = 0.5 * (j.z1 - j.z0) / j.k0;
+    hf[b] = (j.z1 - j.z0) / j.k0;
+    c0[b] = mk(0.0, (j.z1 - j.z0) / (6.0 * 2.0 * j.k0));
+  }
+  DevBuf dh = c.ws.get(sizeof(double) * 2 * B), dc0 = c.ws.get(sizeof(cd) * B);
+  c.pinned.upload(s, dh.ptr, hh.data(), sizeof(double) * B);
+  c.pinned.upload(s, dh.as<double>() + B, hf.data(), sizeof(double) * B);
+  c.pinned.upload(s, dc0.ptr, c0.data(), sizeof(cd) * B);
+  DevBuf phb = c.ws.get(2 * Vn * B);
+  {
+    dim3 g(cdiv_up(N, 256), B);
+    phase_kernel<<<g, 256, 0, s>>>(N, gb.as<cd>(), N, dh.as<double>(), phb.as<cd>(), N);
+    phase_kernel<<<g, 256, 0, s>>>(N, gb.as<cd>(), N, dh.as<double>() + B, phb.as<cd>() + (long)N * B, N);
+    launch_check("phase");
+  }
+  simpson_batch_kernel<<<dim3(RH_ERR_BLOCKS, B), 256, 0, s>>>(N, dA, dB, phb.as<cd>(), phb.as<cd>() + (long)N * B,
+                                                            dc0.as<cd>(), T, err);
+  launch_check("simpson_batch");
+}
+
 void redheffer_perturbation_enqueue(vrc_ctx& c, int N, double k0, double z0, double z1, const cd* Pr,
                                     const cd* Qr, const RefSys& R, const cd* P3[3], const cd* Q3[3], cd* tdd,
                                     cd* tuu, cd* rud, cd* rdu, const double* d_hs, double* err) {

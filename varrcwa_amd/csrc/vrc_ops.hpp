@@ -100,12 +100,21 @@ class LegacySolver {
   // which: 0 Tdd, 1 Rud (N x N); layer type k: 2 E, 3 H (N x N), 4 gamma (N) (gamma = sqrt(eig) / k0)
   void get(int which, int k, cd* host) const;
   bool solved() const { return solved_; }
+  int nstack() const { return (int)stack_.size(); }
+  // RCWASolver::evaluateIntermediateField (src/rcwa/RCWASolver.cpp:1373-1621) for the stack of
+  // the last solve: inputs (N x ncols, host, column-major) -> per inner layer j (nstack - 2
+  // of them) c_m[j], c_p[j] (N x ncols each, host, layer-major).
+  void intermediate_field(int ncols, const cd* input, cd* c_m, cd* c_p);
 
  private:
   vrc_ctx* c_;
   int nx_, ny_;
   std::vector<HostLayer> layers_;
   DevBuf W_, V_, gam_, Tdd_, Rud_;
+  // kept for the intermediate fields: interface matrices (unique pairs), the stack's
+  // interface map, the Rud of every RTCM step, the phase scale of every stack entry
+  DevBuf IE_, IH_, REF_, ds_;
+  std::vector<int> stack_, imap_;
   double k0_ = 0.0;
   bool solved_ = false;
 };
@@ -139,6 +148,7 @@ struct RefSys {
 
 void gather_mats(hipStream_t s, long nn, const cd* src, long ss, const int* d_idx, cd* dst, long sd, int batch);
 void stack_set_cascade_group(int n);  // instances per batched cascade of batch_stack_rcwa
+void rcwa_set_group(int g);           // slices per eigensolve launch of RCWAIntegrator (1: one at a time)
 // Slab S-matrix (src/rcwa/RCWAScatterMatrix.cpp:26-85); d_s[b] = dz_b / k0_b.
 void layer_smatrix(vrc_ctx& c, int N, int batch, const double* d_s, const cd* W, const cd* V, const cd* gam,
                    const cd* W0, const cd* V0, long sRef, cd* Tuu, cd* Rud);
@@ -175,6 +185,15 @@ double redheffer_perturbation(vrc_ctx& c, int N, double k0, double z0, double z1
 // The same without the readback: d_hs (device) holds {(z1 - z0) / (2 k0), (z1 - z0) / k0};
 // the error estimate is the largest of the RH_ERR_BLOCKS values written to err (device).
 constexpr int RH_ERR_BLOCKS = 256;
+// One varRCWA segment for redheffer_perturbation_batch.
+struct RhJob {
+  double k0, z0, z1;
+  const cd *Pr, *Qr;
+  RefSys R;
+  const cd* P3[3];
+  const cd* Q3[3];
+};
+void redheffer_perturbation_batch(vrc_ctx& c, int N, const std::vector<RhJob>& jobs, cd* T, double* err);
 void redheffer_perturbation_enqueue(vrc_ctx& c, int N, double k0, double z0, double z1, const cd* Pr,
                                     const cd* Qr, const RefSys& R, const cd* P3[3], const cd* Q3[3], cd* tdd,
                                     cd* tuu, cd* rud, cd* rdu, const double* d_hs, double* err);
