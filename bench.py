@@ -284,6 +284,62 @@ def pixel_fourier_bench(ctx, nlayers: int, grid: int, nh: int):
             "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
+def deep_stack_gradient_bench(ctx, nlam: int, nh: int = 25, nlayers: int = 200, block: int = 64):
+    """Radius gradients of a deep pillar stack: nlayers layers of 8 um / nlayers with one
+    circular pillar each (eps 12.25 in air, 1 um cell, radius r(z) = 0.2 + 0.08 sin(2 pi z / 8 +
+    phi) um, phi from default_rng(5)) on eps = 2.1, nlam wavelengths in 1.50-1.60 um, nh x nh
+    harmonics; objective: zeroth-order transmitted power (TE+TM) summed over wavelengths;
+    the tape recomputes the slab eigensystems in blocks of `block` layers."""
+    import math
+
+    import torch
+    from varrcwa_amd import api
+    from varrcwa_amd.autograd import ShapeProblem, pillar_layer, stack_smatrix_shape, zeroth_order_transmission
+    phi = float(np.random.default_rng(5).uniform(0, 2 * math.pi))
+    z = (np.arange(nlayers) + 0.5) * 8.0 / nlayers
+    radii = 0.2 + 0.08 * np.sin(2 * math.pi * z / 8.0 + phi)
+    lams = list(np.linspace(1.50, 1.60, nlam))
+    sub = api.Layer([0.0, 1.0], [0.0, 1.0], [[2.1]])
+    air = api.Layer([0.0, 1.0], [0.0, 1.0], [[1.0]])
+    tapes = []
+
+    def build(theta):
+        insts, shape = [], []
+        for lam in lams:
+            layers = []
+            for l, r in enumerate(theta):
+                L, vi, cf = pillar_layer(float(r), 1.0, 16, 12.25, 1.0)
+                layers.append(L)
+                shape.append((vi, cf, [l]) if vi else None)
+            insts.append(dict(lam=lam, layers=layers, thickness=np.full(nlayers, 8.0 / nlayers), bottom=sub,
+                              top=air))
+        return insts, shape
+
+    prob = ShapeProblem(ctx, nh, nh, build, recompute=True, layer_block=block)
+    theta = torch.from_numpy(radii.copy()).requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S, W0, W1 = stack_smatrix_shape(theta, prob)
+    J = zeroth_order_transmission(S, W0, W1, nh, nh).sum()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tape_bytes = S.grad_fn.tape.bytes() if hasattr(S.grad_fn, "tape") else None
+    J.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    pool = int(ctx.lib.vrc_pool_bytes(ctx.h))
+    g = theta.grad.numpy()
+    log(f"deep stack: {nlayers} layers x {nlam} wavelengths at {nh}x{nh}: forward {t1 - t0:.1f} s, "
+        f"gradient {t2 - t1:.1f} s, tape {tape_bytes / 1e9 if tape_bytes else float('nan'):.2f} GB, "
+        f"pool {pool / 1e9:.1f} GB")
+    return {"layers": nlayers, "wavelengths": nlam, "harmonics": f"{nh}x{nh}", "N": 2 * nh * nh,
+            "forward_s": t1 - t0, "reverse_s": t2 - t1, "gradient_over_forward": (t2 - t1) / (t1 - t0),
+            "tape_bytes": tape_bytes, "device_pool_bytes_after": pool, "layer_block": block,
+            "objective": float(J.detach()), "grad_norm": float(np.linalg.norm(g)),
+            "definition": "radius gradient (one pillar radius per layer) through a tape that keeps no slab "
+                          "eigensystems (recomputed per block of layers in the reverse pass)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,6 +357,7 @@ def main():
     ap.add_argument("--mixed", type=int, default=512, help="instances of the mixed-order leg (0 = skip)")
     ap.add_argument("--graph", type=int, default=1, help="also time replays of the recorded step (0 = skip)")
     ap.add_argument("--pixel", type=int, default=64, help="layers of the 256x256 pixel Fourier leg (0 = skip)")
+    ap.add_argument("--deep", type=int, default=2, help="wavelengths of the 200-layer radius-gradient leg (0 = skip)")
     ap.add_argument("--tune", action="append", default=[], help="kernel switch key=value (vrc_tune_set)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first copy this many bytes device to device (a plain copy kernel of known size "
@@ -384,6 +441,10 @@ def main():
     pixel = None
     if args.pixel > 0:
         pixel = pixel_fourier_bench(ctx, args.pixel, 256, 11)
+    deep = None
+    if args.deep > 0:
+        deep = deep_stack_gradient_bench(ctx, args.deep)
+    ctx.check(ctx.lib.vrc_pool_trim(ctx.h), "pool_trim")
     ctx.check(ctx.lib.vrc_pool_trim(ctx.h), "pool_trim")
 
     # ---- batched: at most one warm-up step, then timed steps within the budget
@@ -529,6 +590,7 @@ def main():
         "adjoint": adj,
         "fourier_stage": fourier,
         "fourier_pixel_grids": pixel,
+        "deep_stack_radius_gradient": deep,
         "stage_ms_per_step": {k: v / steps for k, v in excl.items()},
         "device_memory_in_use_gb": (total_b - free_b) / 1e9,
         "dominant_stage": dom,

@@ -2,7 +2,7 @@
 // reference's C++ classes does, on the two scenario geometries of the reference
 // drivers, and writes the ten results as raw complex128 for the Python parity test.
 //
-//   cpp_api_check <metasurface|var12|user|usergpu|gds:FILE> <rcwa|redheffer|differential> <nh> <N or pe> <out.bin>
+//   cpp_api_check <metasurface|var12|user|usergpu|gds:FILE> <rcwa|redheffer|redheffermany|differential> <nh> <N or pe> <out.bin>
 //
 // Scenarios (geometry as data): metasurface cell of src/tests/test_metasurface_cpu.cpp:129-164
 // (periodic, x[2] follows the pillar rule; "user"/"usergpu": the same cell through a
@@ -69,7 +69,7 @@ static MatrixXcs grid(int rows, int cols, double bg) {
 
 int main(int argc, char** argv) {
   if (argc != 6) {
-    std::fprintf(stderr, "usage: %s <metasurface|var12|gds:FILE> <rcwa|redheffer|differential> <nh> <N|pe> <out>\n",
+    std::fprintf(stderr, "usage: %s <metasurface|var12|gds:FILE> <rcwa|redheffer|redheffermany|differential> <nh> <N|pe> <out>\n",
                  argv[0]);
     return 2;
   }
@@ -135,6 +135,30 @@ int main(int argc, char** argv) {
       r->compute(0, 1000000, param);
       n_used = r->N();
       n_eig = r->nEigVal();
+      it = std::move(r);
+    } else if (method == "redheffermany") {
+      // two integrators of the same layer at once (RedhefferIntegrator::computeMany); the
+      // second must agree with the first
+      auto r = std::make_unique<RedhefferIntegrator>(z0, z1, sampler);
+      RedhefferIntegrator twin(z0, z1, sampler);
+      RedhefferIntegrator::computeMany({r.get(), &twin}, 0, 1000000, param);
+      n_used = r->N();
+      n_eig = r->nEigVal();
+      if (twin.N() != n_used) {
+        std::fprintf(stderr, "computeMany: N differs between equal integrators\n");
+        return 1;
+      }
+      const MatrixXcs& a = r->Tuu();
+      const MatrixXcs& b = twin.Tuu();
+      double dmax = 0.0, amax = 0.0;
+      for (long k = 0; k < a.size(); ++k) {
+        dmax = std::fmax(dmax, std::abs(a(k) - b(k)));
+        amax = std::fmax(amax, std::abs(a(k)));
+      }
+      if (!(dmax <= 1e-12 * amax)) {
+        std::fprintf(stderr, "computeMany: Tuu differs between equal integrators (%g)\n", dmax / amax);
+        return 1;
+      }
       it = std::move(r);
     } else if (method == "differential") {
       auto r = std::make_unique<DifferentialIntegrator>(z0, z1, sampler);

@@ -15,6 +15,9 @@ for varRCWA, the accepted segments in cascade order. Scenarios:
   c2_converter        PML mode converter, 15 x 15 harmonics, RedhefferIntegrator at pe = 1e-2
   c3_mixed            the mixed-order sweep's instances 0, 2, 3 and 5 (21 x 21, 7 x 7, 15 x 15
                       and 11 x 11 harmonics: one of each class), keys prefixed "i<instance>_"
+  metasurface_25x25   the reference's metasurface scenario at its own 25 x 25 harmonics
+                      (src/tests/test_metasurface_cpu.cpp:129-164), N = 1250, RCWAIntegrator
+                      with 4 slices
 
 Run in the build container: python tests/golden/make_parity_fixtures.py [name ...]
 (minutes of CPU; the GPU tests only read the .npz files).
@@ -100,8 +103,14 @@ def c2():
     return reduce_result(r, g["nx"], g["ny"])
 
 
+def metasurface_full():
+    g = G.metasurface(12)
+    return reduce_result(O.rcwa_integrate(G.oracle_sampler(g), g["z0"], g["z1"], 4), g["nx"], g["ny"])
+
+
 SCENARIOS = {"metric_inst0": lambda: metric(0), "metric_inst511": lambda: metric(511), "c1_pillar": c1,
-             "var12_25x21": var12, "c2_converter": c2, "c3_mixed": c3}
+             "var12_25x21": var12, "c2_converter": c2, "c3_mixed": c3,
+             "metasurface_25x25": metasurface_full}
 
 
 def main(names):

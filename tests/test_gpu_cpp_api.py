@@ -73,6 +73,18 @@ def test_cpp_redheffer_metasurface(tmp_path):
 
 
 @pytest.mark.gpu
+def test_cpp_redheffer_compute_many(tmp_path):
+    """RedhefferIntegrator::computeMany over two integrators of one layer: the same N and
+    S-matrix as the oracle's single-integrator refinement."""
+    g = G.metasurface(2)
+    pe = 5e-2
+    ref = O.redheffer_integrate(G.oracle_sampler(g), g["z0"], g["z1"], pe)
+    got = run_cpp(tmp_path, "metasurface", "redheffermany", 2, pe)
+    assert got.N == ref.N
+    close(got, ref)
+
+
+@pytest.mark.gpu
 def test_cpp_differential_metasurface(tmp_path):
     g = G.metasurface(1)
     ref = O.differential_integrate(G.oracle_sampler(g), g["z0"], g["z1"], 32)

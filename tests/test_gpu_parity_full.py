@@ -151,3 +151,35 @@ def test_c3_mixed_orders_one_call(ctx):
         compare(got, sub, f"C3 instance {i} ({orders[i]}x{orders[i]})")
         one = from_dev_colmajor(api.batch_stack_rcwa(ctx, orders[i], orders[i], [insts[i]])["S"][0])
         assert np.abs(one - S).max() <= 1e-12 * np.abs(one).max()
+
+
+def test_metasurface_reference_harmonics(ctx):
+    """The reference's metasurface scenario at its own 25 x 25 harmonics (N = 1250), four
+    slices through the drop-in RCWAIntegrator."""
+    fx = fixture("metasurface_25x25")
+    g = G.metasurface(12)
+    s = G.gpu_sampler(g, ctx)
+    assert s.nDim() == 1250
+    it = api.RCWAIntegrator(g["z0"], g["z1"], s)
+    it.compute(4)
+    compare(integrator_result(it), fx, "metasurface 25x25")
+
+
+def test_c2_converter_pe_1e6_converges_with_rcwa(ctx):
+    """The converter through varRCWA at pe = 1e-6 (a tree too large for the oracle to replay):
+    midpoint-slice RCWA converges to it at second order, |S_RCWA(n) - S_var| shrinking about
+    4x per doubling of n (n = 250, 500, 1000), the last difference small."""
+    g = G.mode_converter_c2()
+    s = G.gpu_sampler(g, ctx)
+    var = api.RedhefferIntegrator(g["z0"], g["z1"], s)
+    var.compute(0, 10, 1e-6)
+    ref = O.invariant_transforms(integrator_result(var))
+    errs = []
+    for n in (250, 500, 1000):
+        it = api.RCWAIntegrator(g["z0"], g["z1"], s)
+        it.compute(n)
+        inv = O.invariant_transforms(integrator_result(it))
+        errs.append(max(np.abs(inv[k] - ref[k]).max() / np.abs(ref[k]).max() for k in ref))
+    print(f"varRCWA pe=1e-6: {var.N()} segments; RCWA differences {errs}")
+    assert errs[0] / errs[1] > 3.0 and errs[1] / errs[2] > 3.0, errs
+    assert errs[2] < 1e-3, errs

@@ -139,7 +139,13 @@ __device__ inline cd fft_correction(int m, int G, double x0, double d) {
 __global__ __launch_bounds__(256) void fourier_fft_x_kernel(FourierDev g) {
   const int l = blockIdx.y, i = blockIdx.x;
   const int lg = g.fftx[l];
-  if (!lg || i >= g.ncy[l]) return;
+  if (!lg) return;
+  if (i >= g.ncy[l]) {  // strips beyond the layer's count: zero tables (the contraction may read them)
+    const int M1 = 2 * g.nx - 1;
+    const long o = (long)l * g.MY * M1 + (long)i * M1;
+    for (int p = threadIdx.x; p < M1; p += blockDim.x) g.FX[o + p] = g.FXi[o + p] = mk(0.0, 0.0);
+    return;
+  }
   extern __shared__ cd fsm[];
   const int G = 1 << lg;
   cd* a = fsm;
@@ -171,7 +177,13 @@ __global__ __launch_bounds__(256) void fourier_fft_x_kernel(FourierDev g) {
 __global__ __launch_bounds__(256) void fourier_fft_y_kernel(FourierDev g) {
   const int l = blockIdx.y, i = blockIdx.x;
   const int lg = g.ffty[l];
-  if (!lg || i >= g.ncx[l]) return;
+  if (!lg) return;
+  if (i >= g.ncx[l]) {
+    const int M2 = 2 * g.ny - 1;
+    const long o = (long)l * g.MX * M2 + (long)i * M2;
+    for (int q = threadIdx.x; q < M2; q += blockDim.x) g.FYi[o + q] = mk(0.0, 0.0);
+    return;
+  }
   extern __shared__ cd fsm[];
   const int G = 1 << lg;
   cd* a = fsm;
@@ -203,10 +215,13 @@ __global__ __launch_bounds__(64) void toeplitz_inv_small_kernel(int k, const cd*
   const int ld = 2 * k + 1;
   const long id = blockIdx.x;
   const int layer = (int)(id / per_layer), slot = (int)(id % per_layer);
-  if (slot >= count[layer]) return;
   v += id * sv;
   out += id * so;
   const int t = threadIdx.x;
+  if (slot >= count[layer]) {  // beyond the layer's strips: zero (the contraction may read it)
+    for (int e = t; e < k * k; e += 64) out[e] = mk(0.0, 0.0);
+    return;
+  }
   for (int e = t; e < k * 2 * k; e += 64) {
     const int r = e % k, c = e / k;
     tsm[r * ld + c] = (c < k) ? v[k - 1 + r - c] : ((c - k == r) ? mk(1.0, 0.0) : mk(0.0, 0.0));
@@ -261,17 +276,20 @@ __global__ __launch_bounds__(64) void toeplitz_inv_small_kernel(int k, const cd*
 // Inverse of the k x k Toeplitz matrix T(r,c) = v[k-1 + r - c] (ToeplitzMatrixXcs,
 // src/rcwa/ToeplitzMatrixXcs.cpp:8-44) by Gauss-Jordan elimination with partial
 // pivoting in LDS; one workgroup per matrix. Inactive slots (beyond the layer's
-// strip/column count) are left untouched.
+// strip/column count) are zeroed.
 __global__ __launch_bounds__(256) void toeplitz_inv_kernel(int k, const cd* v, long sv, cd* out, long so,
                                                            const int* count, int per_layer) {
   __shared__ cd M[64][129];
   __shared__ int piv;
   const long id = blockIdx.x;
   const int layer = (int)(id / per_layer), slot = (int)(id % per_layer);
-  if (slot >= count[layer]) return;
   v += id * sv;
   out += id * so;
   const int t = threadIdx.x;
+  if (slot >= count[layer]) {  // beyond the layer's strips: zero (the contraction may read it)
+    for (int e = t; e < k * k; e += 256) out[e] = mk(0.0, 0.0);
+    return;
+  }
   for (int e = t; e < k * 2 * k; e += 256) {
     int r = e % k, c = e / k;
     M[r][c] = (c < k) ? v[k - 1 + r - c] : ((c - k == r) ? mk(1.0, 0.0) : mk(0.0, 0.0));
@@ -398,9 +416,26 @@ void fourier_tables(hipStream_t s, const FourierDev& g, int nlayers) {
   toeplitz_inverses(s, g.ny, g.FYi, M2, g.tFFy, (long)g.ny * g.ny, g.ncx, (int)g.MX, nlayers * (int)g.MX);
 }
 
+// The contraction as three batched products over the strip (column) index, for layers with
+// many strips (pixel grids): F2 = FX^T FY1, G11 = tFFx FY1 (over strips), G22 = tFFy FX1
+// (over x cells); slots beyond a layer's count hold zeros.
+static void fourier_contract_gemm(hipStream_t s, const FourierDev& g, int nlayers) {
+  const int nx = g.nx, ny = g.ny, M1 = 2 * nx - 1, M2 = 2 * ny - 1;
+  const long MX = g.MX, MY = g.MY, nxx = (long)nx * nx, nyy = (long)ny * ny;
+  const cd one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  zgemm_strided(s, OP_N, OP_T, M1, M2, (int)MY, one, g.FX, M1, MY * M1, g.FY1, M2, MY * M2, zero, g.F2, M1,
+                (long)M1 * M2, nlayers);
+  zgemm_strided(s, OP_N, OP_T, (int)nxx, M2, (int)MY, one, g.tFFx, nxx, MY * nxx, g.FY1, M2, MY * M2, zero, g.G11,
+                nxx, nxx * M2, nlayers);
+  zgemm_strided(s, OP_N, OP_T, (int)nyy, M1, (int)MX, one, g.tFFy, nyy, MX * nyy, g.FX1, M1, MX * M1, zero, g.G22,
+                nyy, nyy * M1, nlayers);
+}
+
 void fourier_assemble(hipStream_t s, const FourierDev& g, int nlayers, cd* fe11, cd* fe22, cd* fe33, long sfe) {
   fourier_tables(s, g, nlayers);
-  {
+  if (g.MX >= 64 || g.MY >= 64) {
+    fourier_contract_gemm(s, g, nlayers);
+  } else {
     dim3 grid(64, nlayers);
     fourier_contract_kernel<<<grid, 256, 0, s>>>(g);
     launch_check("fourier_contract");

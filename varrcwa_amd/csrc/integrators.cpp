@@ -664,14 +664,24 @@ void Integrator::redheffer_many(const std::vector<Integrator*>& its, double pe, 
       // references (midpoint of the last third), [2S, 3S) first trisection point,
       // [3S, 4S) second trisection point
       LevelP L = new_level(c, N, 4 * S);
-      for (int u = 0; u < S; ++u) {
-        const Seg& sg = pending[split[u]];
-        Sampler* smp = its[sg.inst]->s_;
-        const double zm0 = 2. / 3. * sg.za + 1. / 3. * sg.zb;
-        const double zm1 = 1. / 3. * sg.za + 2. / 3. * sg.zb;
-        const double zz[4] = {0.5 * (sg.za + zm0), 0.5 * (zm1 + sg.zb), zm0, zm1};
-        for (int k = 0; k < 4; ++k)
-          smp->sample_gpu({zz[k]}, L->P.as<cd>() + (k * S + u) * NN, L->Q.as<cd>() + (k * S + u) * NN);
+      // one sampler call per slot group and run of segments of one integrator (the split
+      // segments keep the integrators' order)
+      for (int k = 0; k < 4; ++k) {
+        for (int u0 = 0; u0 < S;) {
+          const int inst = pending[split[u0]].inst;
+          int u1 = u0;
+          std::vector<double> zs;
+          while (u1 < S && pending[split[u1]].inst == inst) {
+            const Seg& sg = pending[split[u1]];
+            const double zm0 = 2. / 3. * sg.za + 1. / 3. * sg.zb;
+            const double zm1 = 1. / 3. * sg.za + 2. / 3. * sg.zb;
+            const double zz[4] = {0.5 * (sg.za + zm0), 0.5 * (zm1 + sg.zb), zm0, zm1};
+            zs.push_back(zz[k]);
+            ++u1;
+          }
+          its[inst]->s_->sample_gpu(zs, L->P.as<cd>() + (k * S + u0) * NN, L->Q.as<cd>() + (k * S + u0) * NN);
+          u0 = u1;
+        }
       }
       solve_level(c, N, *L, 2 * S, 2 * S);
       for (int u = 0; u < S; ++u) {
