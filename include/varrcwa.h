@@ -67,6 +67,10 @@ int vrc_timing_enable(vrc_ctx* ctx, int on);
  * RCWAIntegrator::compute solved per eigensolve launch (1: one slice at a time, as the
  * reference's GPU integrator); "stack.cascade_group" (default 128):
  * instances of vrc_batch_stack_rcwa whose star products and projections run in one batch;
+ * "zgeev.qr_variant" (default 0): the QR (Schur) kernel, 0: 64-wide window on 512 threads,
+ * window products formed from registers; 1: the same with the window products staged
+ * through LDS; 2: 48-wide window on 256 threads, two matrices per CU, register products;
+ * "zgeev.bulges_w48" (default 8, 1..12): bulges per sweep of variant 2;
  * "zgeev.max_sweeps" (default 0 = 30 * max(N, 10)): QR sweeps per matrix before the
  * eigensolver gives up and reports info > 0; "redheffer.max_depth" (default 40, 0..60):
  * trisection depth at which vrc_redheffer_compute accepts a segment regardless of pe

@@ -117,6 +117,15 @@ int vrc_tune_set(const char* key, long long value) {
     zgeev_set_max_sweeps((int)value);
     return 0;
   }
+  if (std::strcmp(key, "zgeev.qr_variant") == 0) {
+    if (value < 0 || value > 2) return -1;
+    zgeev_set_qr_variant((int)value);
+    return 0;
+  }
+  if (std::strcmp(key, "zgeev.bulges_w48") == 0) {
+    zgeev_set_bulges_w48((int)value);
+    return 0;
+  }
   if (std::strcmp(key, "zgeev.bulges") == 0) {
     zgeev_set_bulges((int)value);
     return 0;

@@ -54,6 +54,33 @@ __device__ inline void house(cd alpha, double xnorm, double& beta, cd& tau, cd& 
   scale = crecip(mk(alpha.x - beta, alpha.y));
 }
 
+// The same reflector for alpha and x = (y, z) (z = 0 for a 2-vector) with one square root
+// and two divisions: sqrt(|alpha|^2 + |y|^2 + |z|^2) directly and 1/(alpha - beta) as
+// conj(d) / |d|^2. Used when every component lies in [2^-400, 2^400] (no overflow or
+// underflow in the squares); otherwise the scaled form above.
+__device__ inline void house_small(cd alpha, cd y, cd z, double& beta, cd& tau, cd& scale) {
+  const double m = fmax(fmax(fmax(fabs(alpha.x), fabs(alpha.y)), fmax(fabs(y.x), fabs(y.y))),
+                        fmax(fabs(z.x), fabs(z.y)));
+  const double xn2 = cnorm2(y) + cnorm2(z);
+  if (!(m <= 0x1p400 && (m >= 0x1p-400 || m == 0.0))) {
+    house(alpha, hypot(cabsv(y), cabsv(z)), beta, tau, scale);
+    return;
+  }
+  if (xn2 == 0.0 && alpha.y == 0.0) {
+    beta = alpha.x;
+    tau = mk(0.0, 0.0);
+    scale = mk(0.0, 0.0);
+    return;
+  }
+  const double nrm = sqrt(cnorm2(alpha) + xn2);
+  beta = alpha.x >= 0.0 ? -nrm : nrm;
+  const double rb = 1.0 / beta;
+  tau = mk((beta - alpha.x) * rb, -alpha.y * rb);
+  const double dx = alpha.x - beta, dy = alpha.y;
+  const double id = 1.0 / (dx * dx + dy * dy);
+  scale = mk(dx * id, -dy * id);
+}
+
 // ===================================================================== Hessenberg
 // out[i] = sum_{r>=r0} conj(Vf[r, p+i]) * x[r] for i < j, x in LDS.
 constexpr int GV_W = 8;  // trailing-product loads in flight per thread
@@ -524,20 +551,115 @@ __device__ void apply_right_u(const LdsMat& U, LdsMat& C, int w, cd* M, long ld,
   }
 }
 // ===================================================================== QR iteration
+// The QR workgroup's LDS: the W x W window, its accumulated transform U and the bulge
+// and reduction scratch. W = 64 with T = 512 threads fills most of a CU's LDS (one
+// workgroup per CU); W = 48 with T = 256 stays under half of it, so two workgroups (two
+// matrices) share a CU and one's serial bulge chase overlaps the other's window products.
+template <int W>
+using QMat = cd[W][W + 1];
+template <int W, int T>
 struct QrLds {
-  LdsMat Hw;
-  LdsMat U;
+  QMat<W> Hw;
+  QMat<W> U;
   cd shifts[2 * NBUL];
   cd rv1[NBUL], rv2[NBUL], rtau[NBUL];
   double rbeta[NBUL];
   int rsize[NBUL], rrow[NBUL], rcol[NBUL];
-  int redi[QT];
-  cd aedv[QW];
+  int redi[T];
+  cd aedv[64];  // written by every lane of a wavefront
   int ctl[4];
 };
+static_assert(sizeof(QrLds<48, 256>) <= 80 * 1024, "two W = 48 QR workgroups must fit one CU's LDS");
+
+// Window products from registers (no LDS chunk buffer, no barrier): with the 16x16x4 FP64
+// MFMA, lane l supplies A[l & 15][l >> 4] and B[l >> 4][l & 15] and holds
+// D[(l >> 4) + 4 q][l & 15], q < 4. Both products are formed transposed so that a lane's
+// global loads and stores run along a column (16 consecutive rows per instruction).
+//
+// M[r0:r0+w, c0:c1] <- U^H M[...]: as D = M^T conj(U) per 16 columns; wave v owns columns
+// c0 + 16 v + 16 NW k, reads their w rows into registers and writes them back.
+template <int W, int NW>
+__device__ void apply_left_reg(const QMat<W>& U, int w, cd* M, long ld, int r0, int c0, int c1) {
+  constexpr int KS = W / 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kk = lane >> 4, jl = lane & 15;
+  const int ks = (w + 3) >> 2, rbn = (w + 15) >> 4;
+  for (int cc = c0 + 16 * wave; cc < c1; cc += 16 * NW) {
+    cd a[KS];
+    {
+      const int col = cc + jl;
+      const cd* src = M + r0 + (long)col * ld;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int r = 4 * s + kk;
+        a[s] = (col < c1 && r < w) ? src[r] : mk(0.0, 0.0);
+      }
+    }
+    // one 16-row block of the result at a time: a[] and one accumulator pair live
+    for (int rb = 0; rb < rbn; ++rb) {
+      d4 accR = d4{0.0, 0.0, 0.0, 0.0}, accI = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        if (s < ks) {
+          const cd u = U[4 * s + kk][16 * rb + jl];  // B[k][i] = conj(U[k][i])
+          accR = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s].x, u.x, accR, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s].x, -u.y, accI, 0, 0, 0);
+          accR = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s].y, u.y, accR, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s].y, u.x, accI, 0, 0, 0);
+        }
+      }
+      // D[j][i] = (U^H M)[i][cc + j], j = kk + 4 q, i = 16 rb + jl
+      const int i = 16 * rb + jl;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int col = cc + kk + 4 * q;
+        if (i < w && col < c1) M[(r0 + i) + (long)col * ld] = mk(accR[q], accI[q]);
+      }
+    }
+  }
+}
+
+// M[r0:r1, c0:c0+w] <- M[...] U: as D = U^T M^T per 16 rows; wave v owns rows
+// r0 + 16 v + 16 NW k, reads their w columns into registers and writes them back.
+template <int W, int NW>
+__device__ void apply_right_reg(const QMat<W>& U, int w, cd* M, long ld, int r0, int r1, int c0) {
+  constexpr int KS = W / 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kk = lane >> 4, jl = lane & 15;
+  const int ks = (w + 3) >> 2, cbn = (w + 15) >> 4;
+  for (int rr = r0 + 16 * wave; rr < r1; rr += 16 * NW) {
+    const int row = rr + jl;
+    cd b[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int k = 4 * s + kk;
+      b[s] = (row < r1 && k < w) ? M[row + (long)(c0 + k) * ld] : mk(0.0, 0.0);
+    }
+    for (int cb = 0; cb < cbn; ++cb) {
+      d4 accR = d4{0.0, 0.0, 0.0, 0.0}, accI = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        if (s < ks) {
+          const cd u = U[4 * s + kk][16 * cb + jl];  // A[j][k] = U[k][j]
+          accR = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, b[s].x, accR, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, b[s].y, accI, 0, 0, 0);
+          accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-u.y, b[s].y, accR, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f64_16x16x4f64(u.y, b[s].x, accI, 0, 0, 0);
+        }
+      }
+      // D[j][r] = (M U)[rr + r][16 cb + j], j = kk + 4 q, r = jl
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = 16 * cb + kk + 4 * q;
+        if (row < r1 && j < w) M[row + (long)(c0 + j) * ld] = mk(accR[q], accI[q]);
+      }
+    }
+  }
+}
 
 // Negligible-subdiagonal test (Ahues-Tisseur form) on an LDS block, row k of [l..i].
-__device__ inline bool negl_lds(const LdsMat& Hw, int k, int l, int i, double smlnum) {
+template <class Mat>
+__device__ inline bool negl_lds(const Mat& Hw, int k, int l, int i, double smlnum) {
   const cd hs = Hw[k][k - 1];
   const double ahs = cabs1(hs);
   if (ahs <= smlnum) return true;
@@ -593,7 +715,8 @@ __device__ __forceinline__ double wave_max(double v) {
 // transform into U when wantU (U must hold the identity or an earlier transform).
 // Subdiagonal entries are set to exact zeros on deflation, so Hw ends upper
 // triangular. Returns 0 on success, else (index of the unconverged eigenvalue + 1).
-__device__ int wave_schur(LdsMat& Hw, LdsMat& U, int s, bool wantU, int lane) {
+template <class Mat>
+__device__ int wave_schur(Mat& Hw, Mat& U, int s, bool wantU, int lane) {
   const double smlnum = SAFMIN * ((double)s / ULP);
   const int itmax = 30 * (s > 10 ? s : 10);
   int i = s - 1;
@@ -631,7 +754,7 @@ __device__ int wave_schur(LdsMat& Hw, LdsMat& U, int s, bool wantU, int lane) {
         }
         double beta;
         cd tau, scale;
-        house(x, cabsv(y), beta, tau, scale);
+        house_small(x, y, mk(0.0, 0.0), beta, tau, scale);
         const cd v1 = cmul(y, scale);
         wave_sync();
         if (tau.x == 0.0 && tau.y == 0.0) continue;
@@ -687,7 +810,8 @@ __device__ inline void givens(cd f, cd g, double& cs, cd& sn) {
 
 // Swap the adjacent diagonal entries k, k+1 of the n x n upper-triangular T by a Givens
 // rotation (LAPACK ztrexc step), updating the Schur vectors U (n rows). One wavefront.
-__device__ void wave_swap(LdsMat& T, LdsMat& U, int n, int k, int lane) {
+template <class Mat>
+__device__ void wave_swap(Mat& T, Mat& U, int n, int k, int lane) {
   const cd t11 = T[k][k], t22 = T[k + 1][k + 1];
   double cs;
   cd sn;
@@ -718,7 +842,8 @@ __device__ void wave_swap(LdsMat& T, LdsMat& U, int n, int k, int lane) {
 
 // T <- G^H T on rows r0..r0+len-1 (columns c0..n-1), then T <- T G on those columns
 // (rows 0..rows_t-1) and U <- U G (rows 0..n-1), G = I - tau v v^H with v in vv[0..len).
-__device__ void wave_reflect(LdsMat& T, LdsMat& U, const cd* vv, cd tau, int r0, int len, int c0, int n, int rows_t,
+template <class Mat>
+__device__ void wave_reflect(Mat& T, Mat& U, const cd* vv, cd tau, int r0, int len, int c0, int n, int rows_t,
                              int lane) {
   const cd ct = cconj(tau);
   if (lane >= c0 && lane < n) {
@@ -743,9 +868,53 @@ __device__ void wave_reflect(LdsMat& T, LdsMat& U, const cd* vv, cd tau, int r0,
   wave_sync();
 }
 
-__device__ void lds_load_block(QrLds& L, const cd* H, long ldh, int r0, int w);
-__device__ void lds_store_block(QrLds& L, cd* H, long ldh, int r0, int w);
-__device__ void apply_window(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int ws, int w);
+// QR kernel configuration: window W, T threads, RA: window products from registers
+// (apply_left_reg / apply_right_reg) instead of through an LDS chunk buffer.
+template <int W_, int T_, bool RA_>
+struct QrCfg {
+  static constexpr int W = W_, T = T_;
+  static constexpr bool RA = RA_;
+};
+
+template <class C>
+__device__ void lds_load_block(QrLds<C::W, C::T>& L, const cd* H, long ldh, int r0, int w) {
+  for (int e = threadIdx.x; e < C::W * C::W; e += C::T) {
+    const int i = e % C::W, j = e / C::W;
+    L.Hw[i][j] = (i < w && j < w) ? H[(r0 + i) + (long)(r0 + j) * ldh] : mk(0.0, 0.0);
+    L.U[i][j] = (i == j && i < w) ? mk(1.0, 0.0) : mk(0.0, 0.0);
+  }
+  __syncthreads();
+}
+
+template <class C>
+__device__ void lds_store_block(QrLds<C::W, C::T>& L, cd* H, long ldh, int r0, int w) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < w * w; e += C::T) {
+    int i = e % w, j = e / w;
+    H[(r0 + i) + (long)(r0 + j) * ldh] = L.Hw[i][j];
+  }
+  __syncthreads();
+}
+
+// Apply the window transform U (on indices ws..ws+w-1) to everything outside the window.
+// The three regions (rows of the window right of it, columns of the window above it, Z)
+// are disjoint; the register form needs one barrier at the end only.
+template <class C>
+__device__ void apply_window(QrLds<C::W, C::T>& L, int N, cd* H, long ldh, cd* Z, long ldz, int ws, int w) {
+  const int we = ws + w - 1;
+  if constexpr (C::RA) {
+    constexpr int NW = C::T / 64;
+    if (we + 1 < N) apply_left_reg<C::W, NW>(L.U, w, H, ldh, ws, we + 1, N);
+    if (ws > 0) apply_right_reg<C::W, NW>(L.U, w, H, ldh, 0, ws, ws);
+    apply_right_reg<C::W, NW>(L.U, w, Z, ldz, 0, N, ws);
+    __syncthreads();
+  } else {
+    static_assert(C::W == QW && C::T == QT, "the LDS-staged window products are written for W = 64, 512 threads");
+    if (we + 1 < N) apply_left_u(L.U, L.Hw, w, H, ldh, ws, we + 1, N);
+    if (ws > 0) apply_right_u(L.U, L.Hw, w, H, ldh, 0, ws, ws);
+    apply_right_u(L.U, L.Hw, w, Z, ldz, 0, N, ws);
+  }
+}
 
 // Aggressive early deflation (the deflation-window step of LAPACK zlaqr3) on the
 // trailing nw x nw window of the active block [lo, hi]: Schur form of the window,
@@ -754,17 +923,19 @@ __device__ void apply_window(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, 
 // applied to the rest of H and to Z. Returns the number of deflated eigenvalues (-1 if
 // the window's QR did not converge; nothing is changed then). The undeflated
 // eigenvalues are left in L.aedv[0..*nsa) for use as shifts.
-__device__ int aed(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int lo, int hi, int nw, int* nsa) {
+template <class C>
+__device__ int aed(QrLds<C::W, C::T>& L, int N, cd* H, long ldh, cd* Z, long ldz, int lo, int hi, int nw,
+                    int* nsa) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int kwtop = hi - nw + 1;
   const cd spike = (kwtop > lo) ? H[kwtop + (long)(kwtop - 1) * ldh] : mk(0.0, 0.0);
-  lds_load_block(L, H, ldh, kwtop, nw);
+  lds_load_block<C>(L, H, ldh, kwtop, nw);
   if (wave == 0) {
     const int info = wave_schur(L.Hw, L.U, nw, true, lane);
     if (lane == 0) L.ctl[1] = info;
     if (info == 0) {
-      LdsMat& T = L.Hw;
-      LdsMat& U = L.U;
+      auto& T = L.Hw;
+      auto& U = L.U;
       const double smlnum = SAFMIN * ((double)N / ULP);
       const double as = cabs1(spike);
       int ns = nw, ilst = 0;
@@ -839,46 +1010,21 @@ __device__ int aed(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int lo, in
   const int ns = L.ctl[2];
   const cd newspike = L.shifts[0];
   __syncthreads();
-  lds_store_block(L, H, ldh, kwtop, nw);
+  lds_store_block<C>(L, H, ldh, kwtop, nw);
   if (t == 0 && kwtop > lo) H[kwtop + (long)(kwtop - 1) * ldh] = newspike;
   __syncthreads();
-  apply_window(L, N, H, ldh, Z, ldz, kwtop, nw);
+  apply_window<C>(L, N, H, ldh, Z, ldz, kwtop, nw);
   *nsa = ns;
   return nw - ns;
 }
 
-__device__ void lds_load_block(QrLds& L, const cd* H, long ldh, int r0, int w) {
-  for (int e = threadIdx.x; e < QW * QW; e += QT) {
-    int i = e & 63, j = e >> 6;
-    L.Hw[i][j] = (i < w && j < w) ? H[(r0 + i) + (long)(r0 + j) * ldh] : mk(0.0, 0.0);
-    L.U[i][j] = (i == j && i < w) ? mk(1.0, 0.0) : mk(0.0, 0.0);
-  }
-  __syncthreads();
-}
-
-__device__ void lds_store_block(QrLds& L, cd* H, long ldh, int r0, int w) {
-  __syncthreads();
-  for (int e = threadIdx.x; e < w * w; e += QT) {
-    int i = e % w, j = e / w;
-    H[(r0 + i) + (long)(r0 + j) * ldh] = L.Hw[i][j];
-  }
-  __syncthreads();
-}
-
-// Apply the window transform U (on indices ws..ws+w-1) to everything outside the window.
-__device__ void apply_window(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int ws, int w) {
-  const int we = ws + w - 1;
-  if (we + 1 < N) apply_left_u(L.U, L.Hw, w, H, ldh, ws, we + 1, N);
-  if (ws > 0) apply_right_u(L.U, L.Hw, w, H, ldh, 0, ws, ws);
-  apply_right_u(L.U, L.Hw, w, Z, ldz, 0, N, ws);
-}
-
 // Largest k in [1, hi] whose subdiagonal is negligible (sets it to zero); 0 if none.
-__device__ int find_lo(QrLds& L, cd* H, long ldh, int N, int hi) {
+template <class C>
+__device__ int find_lo(QrLds<C::W, C::T>& L, cd* H, long ldh, int N, int hi) {
   const int t = threadIdx.x;
   const double smlnum = SAFMIN * ((double)N / ULP);
   int best = 0;
-  for (int k = hi - t; k >= 1; k -= QT) {
+  for (int k = hi - t; k >= 1; k -= C::T) {
     const cd hs = H[k + (long)(k - 1) * ldh];
     const double ahs = cabs1(hs);
     bool ng = ahs <= smlnum;
@@ -905,7 +1051,7 @@ __device__ int find_lo(QrLds& L, cd* H, long ldh, int N, int hi) {
   }
   L.redi[t] = best;
   __syncthreads();
-  for (int w = QT / 2; w > 0; w >>= 1) {
+  for (int w = C::T / 2; w > 0; w >>= 1) {
     if (t < w && L.redi[t + w] > L.redi[t]) L.redi[t] = L.redi[t + w];
     __syncthreads();
   }
@@ -917,10 +1063,11 @@ __device__ int find_lo(QrLds& L, cd* H, long ldh, int N, int hi) {
 }
 
 // One multishift sweep over the active block [lo, hi] with nbul bulges (shifts in L.shifts).
-__device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int lo, int hi, int nbul,
+template <class C>
+__device__ void qr_sweep(QrLds<C::W, C::T>& L, int N, cd* H, long ldh, cd* Z, long ldz, int lo, int hi, int nbul,
                          long long* st) {
-  const int t = threadIdx.x;
-  const int adv = QW - 3 * nbul - 1;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int adv = C::W - 3 * nbul - 1;
   const int T_end = (hi - 2) - (lo - 1) + 3 * (nbul - 1) + 1;
   for (int t0 = 0; t0 < T_end; t0 += adv) {
     const int t1 = t0 + adv < T_end ? t0 + adv : T_end;
@@ -932,7 +1079,7 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
     const int we = pmax + 4 < hi ? pmax + 4 : hi;
     const int w = we - ws + 1;
     const long long c0 = wall_clock64();
-    lds_load_block(L, H, ldh, ws, w);
+    lds_load_block<C>(L, H, ldh, ws, w);
     for (int tt = t0; tt < t1; ++tt) {
       // reflectors of all bulges at this step, from the current window state
       if (t < nbul) {
@@ -967,7 +1114,7 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
           }
           double beta;
           cd tau, scale;
-          house(x, sqrt(cnorm2(y) + cnorm2(z)), beta, tau, scale);
+          house_small(x, y, z, beta, tau, scale);
           L.rv1[bb] = cmul(y, scale);
           L.rv2[bb] = cmul(z, scale);
           L.rtau[bb] = tau;
@@ -978,13 +1125,15 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
         L.rsize[bb] = size;
       }
       __syncthreads();
-      // row operations: items (bulge, column), spread over the workgroup
-      for (int it = t; it < nbul * w; it += QT) {
-        const int bb = it / w, j = it - bb * w;
+      // row operations: wave v takes bulges v, v + T/64, ...; lane = column. Columns left
+      // of the bulge's own column hold exact zeros in its rows and are skipped.
+      for (int bb = wave; bb < nbul; bb += C::T / 64) {
         const int sz = L.rsize[bb];
         if (sz == 0) continue;
-        const int r1 = L.rrow[bb];
-        if (j == L.rcol[bb]) {
+        const int r1 = L.rrow[bb], rc = L.rcol[bb];
+        const int j = (r1 > 0 ? r1 - 1 : 0) + lane;
+        if (j >= w) continue;
+        if (j == rc) {
           L.Hw[r1][j] = mk(L.rbeta[bb], 0.0);
           L.Hw[r1 + 1][j] = mk(0.0, 0.0);
           if (sz == 3) L.Hw[r1 + 2][j] = mk(0.0, 0.0);
@@ -1001,15 +1150,17 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
         if (sz == 3) L.Hw[r1 + 2][j] = csub(h3, cmul(sm, v2));
       }
       __syncthreads();
-      // column operations on the window and on U: items (matrix, bulge, row)
-      for (int it = t; it < 2 * nbul * w; it += QT) {
-        const int mat = it / (nbul * w);
-        const int rem = it - mat * nbul * w;
-        const int bb = rem / w, i = rem - bb * w;
+      // column operations on the window (rows down to the bulge's last row + 1; below it
+      // the columns hold exact zeros) and on U (every row): pairs (matrix, bulge) over the
+      // waves, lane = row
+      for (int q = wave; q < 2 * nbul; q += C::T / 64) {
+        const int mat = q >= nbul ? 1 : 0, bb = q - mat * nbul;
         const int sz = L.rsize[bb];
         if (sz == 0) continue;
-        LdsMat& M = mat == 0 ? L.Hw : L.U;
         const int r1 = L.rrow[bb];
+        const int i = lane;
+        if (i >= w || (mat == 0 && i > r1 + 3)) continue;
+        auto& M = mat == 0 ? L.Hw : L.U;
         const cd v1 = L.rv1[bb], v2 = L.rv2[bb], tu = L.rtau[bb];
         cd c1 = M[i][r1], c2 = M[i][r1 + 1];
         cd c3 = sz == 3 ? M[i][r1 + 2] : mk(0.0, 0.0);
@@ -1022,9 +1173,9 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
       }
       __syncthreads();
     }
-    lds_store_block(L, H, ldh, ws, w);
+    lds_store_block<C>(L, H, ldh, ws, w);
     const long long c1 = wall_clock64();
-    apply_window(L, N, H, ldh, Z, ldz, ws, w);
+    apply_window<C>(L, N, H, ldh, Z, ldz, ws, w);
     const long long c2 = wall_clock64();
     st[1] += 1;
     st[2] += c1 - c0;
@@ -1036,11 +1187,10 @@ __device__ void qr_sweep(QrLds& L, int N, cd* H, long ldh, cd* Z, long ldz, int 
 // small-block ticks} of the constant-rate device clock.
 // use_aed: deflation-window size (0 = no aggressive early deflation); max_bul: bulges
 // per sweep (<= NBUL).
-__global__ __launch_bounds__(QT) void hqr_kernel(int N, cd* H, long ldh, long sH, cd* Z, long ldz, long sZ,
-                                                  cd* wout, long sw, int* info, long long* stats, int use_aed,
-                                                  int max_bul, int max_sweeps) {
+template <class C>
+__device__ void hqr_body(QrLds<C::W, C::T>& L, int N, cd* H, long ldh, long sH, cd* Z, long ldz, long sZ, cd* wout,
+                         long sw, int* info, long long* stats, int use_aed, int max_bul, int max_sweeps) {
   long long st[5] = {0, 0, 0, 0, 0};
-  __shared__ QrLds L;
   const long b = blockIdx.x;
   H += b * sH;
   Z += b * sZ;
@@ -1051,26 +1201,26 @@ __global__ __launch_bounds__(QT) void hqr_kernel(int N, cd* H, long ldh, long sH
   const int itmax = max_sweeps > 0 ? max_sweeps : 30 * (N > 10 ? N : 10);
   int fail = 0;
   while (ihi >= 0) {
-    const int lo = find_lo(L, H, ldh, N, ihi);
+    const int lo = find_lo<C>(L, H, ldh, N, ihi);
     const int m = ihi - lo + 1;
     if (lo != prev_lo || ihi != prev_hi) last_progress = sweeps;
     prev_lo = lo;
     prev_hi = ihi;
     if (m <= QR_NMIN) {
       const long long c0 = wall_clock64();
-      lds_load_block(L, H, ldh, lo, m);
+      lds_load_block<C>(L, H, ldh, lo, m);
       if ((t >> 6) == 0) {
         const int r = wave_schur(L.Hw, L.U, m, true, t & 63);
         if (t == 0) L.ctl[1] = r;
       }
       __syncthreads();
       const int r = L.ctl[1];
-      lds_store_block(L, H, ldh, lo, m);
+      lds_store_block<C>(L, H, ldh, lo, m);
       if (r != 0) {
         fail = lo + r;
         break;
       }
-      apply_window(L, N, H, ldh, Z, ldz, lo, m);
+      apply_window<C>(L, N, H, ldh, Z, ldz, lo, m);
       st[4] += wall_clock64() - c0;
       ihi = lo - 1;
       continue;
@@ -1087,7 +1237,7 @@ __global__ __launch_bounds__(QT) void hqr_kernel(int N, cd* H, long ldh, long sH
     if (use_aed) {
       const long long c0 = wall_clock64();
       const int nw = m - 1 < use_aed ? m - 1 : use_aed;
-      const int nd = aed(L, N, H, ldh, Z, ldz, lo, ihi, nw, &nsa);
+      const int nd = aed<C>(L, N, H, ldh, Z, ldz, lo, ihi, nw, &nsa);
       st[4] += wall_clock64() - c0;
       if (nd > 0) {
         ihi -= nd;
@@ -1125,8 +1275,8 @@ __global__ __launch_bounds__(QT) void hqr_kernel(int N, cd* H, long ldh, long sH
       __syncthreads();
     } else {
       const int r0 = ihi - ns + 1;
-      for (int e = t; e < QW * QW; e += QT) {
-        int i = e & 63, j = e >> 6;
+      for (int e = t; e < C::W * C::W; e += C::T) {
+        const int i = e % C::W, j = e / C::W;
         L.Hw[i][j] = (i < ns && j < ns) ? H[(r0 + i) + (long)(r0 + j) * ldh] : mk(0.0, 0.0);
       }
       __syncthreads();
@@ -1135,11 +1285,11 @@ __global__ __launch_bounds__(QT) void hqr_kernel(int N, cd* H, long ldh, long sH
       if (t < ns) L.shifts[t] = L.Hw[t][t];
       __syncthreads();
     }
-    qr_sweep(L, N, H, ldh, Z, ldz, lo, ihi, nbul, st);
+    qr_sweep<C>(L, N, H, ldh, Z, ldz, lo, ihi, nbul, st);
     ++sweeps;
   }
   __syncthreads();
-  for (int i = t; i < N; i += QT) wout[i] = H[i + (long)i * ldh];
+  for (int i = t; i < N; i += C::T) wout[i] = H[i + (long)i * ldh];
   if (t == 0) {
     info[b] = fail;
     if (stats) {
@@ -1148,6 +1298,28 @@ __global__ __launch_bounds__(QT) void hqr_kernel(int N, cd* H, long ldh, long sH
     }
   }
 }
+
+// Kernels of the three configurations (one workgroup per matrix): W = 64 on 512 threads with
+// the register (hqr_kernel) or the LDS-staged (hqr_lds_kernel) window products, and W = 48
+// on 256 threads, two workgroups per CU (at most 256 VGPRs per lane).
+#define HQR_ARGS                                                                                       \
+  int N, cd *H, long ldh, long sH, cd *Z, long ldz, long sZ, cd *wout, long sw, int *info, long long *stats, \
+      int use_aed, int max_bul, int max_sweeps
+#define HQR_PASS N, H, ldh, sH, Z, ldz, sZ, wout, sw, info, stats, use_aed, max_bul, max_sweeps
+__global__ __launch_bounds__(512) void hqr_kernel(HQR_ARGS) {
+  __shared__ QrLds<64, 512> L;
+  hqr_body<QrCfg<64, 512, true>>(L, HQR_PASS);
+}
+__global__ __launch_bounds__(512) void hqr_lds_kernel(HQR_ARGS) {
+  __shared__ QrLds<64, 512> L;
+  hqr_body<QrCfg<64, 512, false>>(L, HQR_PASS);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void hqr_w48_kernel(HQR_ARGS) {
+  __shared__ QrLds<48, 256> L;
+  hqr_body<QrCfg<48, 256, true>>(L, HQR_PASS);
+}
+#undef HQR_ARGS
+#undef HQR_PASS
 
 // ===================================================================== eigenvectors
 // X (n x n, pre-zeroed) <- upper-triangular eigenvector matrix of upper-triangular T:
@@ -1384,6 +1556,14 @@ static int g_max_sweeps = 0;  // 0: 30 * max(N, 10), LAPACK zlahqr's iteration b
 void zgeev_set_max_sweeps(int n) { g_max_sweeps = n < 0 ? 0 : n; }
 void zgeev_set_aed(int window) { g_aed_window = window < 0 ? 0 : (window > QW ? QW : window); }
 void zgeev_set_bulges(int nb) { g_bulges = nb < 1 ? 1 : (nb > NBUL ? NBUL : nb); }
+// 0: hqr_kernel (W = 64, window products from registers), 1: hqr_lds_kernel (W = 64,
+// window products staged through LDS), 2: hqr_w48_kernel (W = 48, two matrices per CU)
+static int g_qr_variant = 0;
+static int g_bulges_w48 = 8;  // bulges per sweep of the W = 48 kernel (<= 12: >= 11 chase steps per window)
+void zgeev_set_qr_variant(int v) { g_qr_variant = v < 0 ? 0 : (v > 2 ? 2 : v); }
+void zgeev_set_bulges_w48(int nb) { g_bulges_w48 = nb < 1 ? 1 : (nb > 12 ? 12 : nb); }
+int zgeev_get_qr_variant() { return g_qr_variant; }
+int zgeev_full_launch() { return g_qr_variant == 2 ? 512 : 256; }
 int zgeev_get_aed() { return g_aed_window; }
 int zgeev_get_bulges() { return g_bulges; }
 
@@ -1412,8 +1592,17 @@ void zgeev_factors(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
     const bool want = c && c->timing;
     DevBuf sb;
     if (want) sb = ws.get(sizeof(long long) * 5 * batch);
-    hqr_kernel<<<batch, QT, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, want ? sb.as<long long>() : nullptr,
-                                     g_aed_window, g_bulges, g_max_sweeps);
+    long long* stp = want ? sb.as<long long>() : nullptr;
+    if (g_qr_variant == 2) {
+      hqr_w48_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, stp,
+                                           g_aed_window < 48 ? g_aed_window : 47, g_bulges_w48, g_max_sweeps);
+    } else if (g_qr_variant == 1) {
+      hqr_lds_kernel<<<batch, QT, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, stp, g_aed_window, g_bulges,
+                                          g_max_sweeps);
+    } else {
+      hqr_kernel<<<batch, QT, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, stp, g_aed_window, g_bulges,
+                                      g_max_sweeps);
+    }
     launch_check("hqr");
     health_note_info(s, info, batch, H_EIG);
     if (want) {

@@ -1039,13 +1039,14 @@ static void batch_stack_solve(vrc_ctx& c, int nx, int ny, const std::vector<Stac
     homog = homog && I.bottom->x.size() == 2 && I.bottom->y.size() == 2 && I.top->x.size() == 2 &&
             I.top->y.size() == 2;
   if (homog && !tape) {
-    batch_stack_homog(c, nx, ny, inst, chunk > 0 ? chunk * L : 256, S_out, W0o, V0o, g0o, W1o, V1o, g1o, flags);
+    batch_stack_homog(c, nx, ny, inst, chunk > 0 ? chunk * L : zgeev_full_launch(), S_out, W0o, V0o, g0o, W1o, V1o,
+                      g1o, flags);
     return;
   }
-  // Default chunk: at most 256 dense eigensolves (L slices, + 2 end media per instance
-  // unless closed form) per launch, one matrix per CU of the QR kernel; the instances
-  // are then split evenly over the chunks so that no launch runs a short tail.
-  if (chunk <= 0) chunk = std::max(1, std::min(ninst, 256 / (L + (homog ? 0 : 2))));
+  // Default chunk: at most one full launch of dense eigensolves (L slices, + 2 end media
+  // per instance unless closed form), one matrix per QR workgroup slot of the GPU; the
+  // instances are then split evenly over the chunks so that no launch runs a short tail.
+  if (chunk <= 0) chunk = std::max(1, std::min(ninst, zgeev_full_launch() / (L + (homog ? 0 : 2))));
   const int nchunks = (ninst + chunk - 1) / chunk;
   std::vector<int> starts(nchunks + 1, 0);
   for (int ci = 0; ci < nchunks; ++ci) starts[ci + 1] = starts[ci] + ninst / nchunks + (ci < ninst % nchunks ? 1 : 0);

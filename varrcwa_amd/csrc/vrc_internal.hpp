@@ -143,6 +143,12 @@ void zgeev_set_max_sweeps(int n);  // QR sweep budget per matrix (0 = automatic)
 void zgeev_set_bulges(int nb);  // bulges per QR sweep (1..16)
 int zgeev_get_aed();
 int zgeev_get_bulges();
+void zgeev_set_qr_variant(int v);   // 0: W = 64, register products, 1: W = 64, LDS-staged, 2: W = 48
+int zgeev_get_qr_variant();
+// matrices of one eigensolve launch that fill every QR workgroup slot (256 CUs; two slots
+// per CU for variant 2)
+int zgeev_full_launch();
+void zgeev_set_bulges_w48(int nb);  // bulges per sweep of the W = 48 kernel (1..12)
 
 // ---------------------------------------------------------------- autotune.cpp
 // The kernel switches an autotuned harmonic count overrides (process-wide values).
