@@ -282,12 +282,13 @@ def test_metasurface_inverse_design_gradient(ctx):
     objective: 0th-order transmitted power (TE+TM). Gradient w.r.t. all 20480 pixels by
     the adjoint.
 
-    This objective carries a rounding noise of a few 1e-9 on any implementation:
-    LAPACK zgeev + the reference formulas give J values 3.8e-9 apart for P/Q matrices
-    that differ only by rounding, and J(eps + 1e-12 e_k) - J(eps) reaches 1.4e-10 with
-    LAPACK and 6e-9 here. A central difference at h = 1e-6 is therefore dominated by
-    noise (error ~1e-3 absolute vs. pixel gradients of 1e-4..5e-3) for the reference's
-    own numerics as for these kernels, and is only printed. Checked instead:
+    This objective carries rounding noise on any implementation: over six pixels,
+    J(eps + 1e-12 e_k) - J(eps) reaches 1.1e-10 with LAPACK zgeev + the reference formulas
+    (the oracle), 2.0e-10 with these kernels' P, Q fed to the oracle, and 6.7e-12 with these
+    kernels throughout (MI355X); P and Q agree with the oracle's to 1.4e-15. A central
+    difference at h = 1e-6 is therefore noise-dominated (relative error ~1e-3 on the 32
+    pixels with these kernels, several times more with LAPACK's noise), and is only
+    printed. Checked instead:
       - the derivative along the unit gradient direction (= |g| if g is right) by the
         fourth-order central difference at h = 3e-3: relative error <= 1e-5 (measured
         5.3e-8 on MI355X);
