@@ -1,7 +1,8 @@
 """The autotuner (vrc_autotune_stack): a bounded search on a sample of the caller's batch,
 per-N winners applied to later calls, a text cache written and read back. Results under
-the tuned switches equal the untuned ones to rounding (the switches choose tile shapes
-and the QR iteration's deflation window / bulge count, not the mathematics)."""
+the tuned switches equal the untuned ones to rounding (the switches choose tile shapes,
+the QR kernel and the QR iteration's deflation window / bulge count, not the
+mathematics)."""
 import ctypes as C
 
 import numpy as np
@@ -26,9 +27,10 @@ def test_autotune_sample_applies_and_caches(ctx, tmp_path):
         assert rc == 0, lib.vrc_last_error(ctx.h).decode()
         text = rep.value.decode()
         print(text)
-        assert text.startswith("N=50:") and "aed_window" in text
+        assert text.startswith("N=50:") and "aed_window" in text and "qr_variant" in text
         lines = [l for l in cache.read_text().splitlines() if l and not l.startswith("#")]
-        assert len(lines) == 1 and lines[0].split()[0] == "50" and len(lines[0].split()) == 7
+        assert len(lines) == 1 and lines[0].split()[0] == "50" and len(lines[0].split()) == 8
+        assert lines[0].split()[7] in ("0", "2")
         after = from_dev_colmajor(api.batch_stack_rcwa(ctx, 5, 5, insts)["S"])
         assert np.abs(after - before).max() <= 1e-10 * np.abs(before).max()
         # another N is not affected, and the cache reads back
@@ -38,6 +40,11 @@ def test_autotune_sample_applies_and_caches(ctx, tmp_path):
         bad = tmp_path / "bad.txt"
         bad.write_text("50 24 twelve\n")
         assert lib.vrc_autotune_load(str(bad).encode()) == -1
+        bad.write_text("50 24 12 128 8 8 129 1\n")  # QR kernel 1 is not a tuning candidate
+        assert lib.vrc_autotune_load(str(bad).encode()) == -1
+        old = tmp_path / "seven_fields.txt"  # a line without the QR kernel field means kernel 0
+        old.write_text("# written before the QR kernel became a switch\n50 24 12 128 8 8 129\n")
+        assert lib.vrc_autotune_load(str(old).encode()) == 1
         assert lib.vrc_autotune_stack(ctx.h, 5, 5, 0, arr, 3.0, None, None, 0) == -1
     finally:
         lib.vrc_autotune_clear()

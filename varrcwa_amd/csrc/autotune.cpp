@@ -26,6 +26,7 @@ TuneKnobs tune_get() {
   TuneKnobs k;
   k.aed = zgeev_get_aed();
   k.bulges = zgeev_get_bulges();
+  k.qr_variant = zgeev_get_qr_variant();
   zgemm_get_knobs(&k.big_n, &k.big_waves, &k.small_waves, &k.big_min_k);
   return k;
 }
@@ -33,6 +34,7 @@ TuneKnobs tune_get() {
 void tune_put(const TuneKnobs& k) {
   zgeev_set_aed(k.aed);
   zgeev_set_bulges(k.bulges);
+  zgeev_set_qr_variant(k.qr_variant);
   zgemm_set_big_n(k.big_n);
   zgemm_set_big_waves(k.big_waves);
   zgemm_set_small_waves(k.small_waves);
@@ -63,9 +65,10 @@ void tune_manual_begin() {
 }
 
 static std::string knobs_text(const TuneKnobs& k) {
-  char b[160];
-  snprintf(b, sizeof(b), "aed_window %d bulges %d big_n %d big_waves %d small_waves %d large_tile_min_k %d", k.aed,
-           k.bulges, k.big_n, k.big_waves, k.small_waves, k.big_min_k);
+  char b[192];
+  snprintf(b, sizeof(b),
+           "qr_variant %d aed_window %d bulges %d big_n %d big_waves %d small_waves %d large_tile_min_k %d",
+           k.qr_variant, k.aed, k.bulges, k.big_n, k.big_waves, k.small_waves, k.big_min_k);
   return b;
 }
 
@@ -81,14 +84,15 @@ void tune_clear() {
   g_table.clear();
 }
 
-// Cache file: one line per N, "N aed bulges big_n big_waves small_waves big_min_k";
-// lines starting with '#' are comments. Later lines for the same N win.
+// Cache file: one line per N, "N aed bulges big_n big_waves small_waves big_min_k
+// [qr_variant]" (a line without the last field means QR kernel 0); lines starting with '#'
+// are comments. Later lines for the same N win.
 void tune_save(const std::string& path, int N, const TuneKnobs& k, const std::string& note) {
   std::ofstream f(path, std::ios::app);
   if (!f) throw ArgFailure("cannot append to the tuning cache " + path);
   f << "# " << note << "\n"
     << N << " " << k.aed << " " << k.bulges << " " << k.big_n << " " << k.big_waves << " " << k.small_waves << " "
-    << k.big_min_k << "\n";
+    << k.big_min_k << " " << k.qr_variant << "\n";
 }
 
 int tune_load(const std::string& path) {
@@ -100,9 +104,11 @@ int tune_load(const std::string& path) {
     if (line.empty() || line[0] == '#') continue;
     std::istringstream is(line);
     int N;
-    TuneKnobs k;
+    TuneKnobs k{};
     if (!(is >> N >> k.aed >> k.bulges >> k.big_n >> k.big_waves >> k.small_waves >> k.big_min_k) || N < 2)
       throw ArgFailure("malformed tuning cache line: " + line);
+    if (!(is >> k.qr_variant)) k.qr_variant = 0;
+    if (k.qr_variant != 0 && k.qr_variant != 2) throw ArgFailure("malformed tuning cache line: " + line);
     tune_store(N, k);
     ++n;
   }
@@ -145,6 +151,7 @@ std::string autotune_stack(vrc_ctx& c, int nx, int ny, const std::vector<StackIn
     const double t_own = run(own);
     double t_best = t_own;
     const std::vector<std::pair<int TuneKnobs::*, std::vector<int>>> space = {
+        {&TuneKnobs::qr_variant, {0, 2}},
         {&TuneKnobs::aed, {16, 20, 24, 32, 40, 48}},  {&TuneKnobs::bulges, {6, 8, 10, 12, 14, 16}},
         {&TuneKnobs::big_n, {64, 128}},              {&TuneKnobs::big_waves, {4, 8}},
         {&TuneKnobs::small_waves, {4, 8}},           {&TuneKnobs::big_min_k, {33, 65, 129}}};

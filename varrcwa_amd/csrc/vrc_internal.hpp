@@ -154,6 +154,7 @@ void zgeev_set_bulges_w48(int nb);  // bulges per sweep of the W = 48 kernel (1.
 // The kernel switches an autotuned harmonic count overrides (process-wide values).
 struct TuneKnobs {
   int aed, bulges, big_n, big_waves, small_waves, big_min_k;
+  int qr_variant;  // 0 or 2 (zgeev_set_qr_variant)
 };
 TuneKnobs tune_get();
 void tune_put(const TuneKnobs& k);
