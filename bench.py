@@ -340,6 +340,37 @@ def deep_stack_gradient_bench(ctx, nlam: int, nh: int = 25, nlayers: int = 200, 
                           "eigensystems (recomputed per block of layers in the reverse pass)"}
 
 
+def zgemm_ceiling(ctx, N: int, batch: int = 128, reps: int = 5):
+    """Measured FP64 matrix ceiling: TFLOP/s of the library's batched ZGEMM (N x N x N, 4M
+    product, 8 N^3 real flops per matrix) on device-resident operands, wall clock around
+    `reps` synchronized launches after two warm-up launches."""
+    import ctypes as C
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(5)
+    A = torch.randn((batch, N, N), dtype=torch.complex128, device="cuda", generator=g)
+    B = torch.randn((batch, N, N), dtype=torch.complex128, device="cuda", generator=g)
+    Cm = torch.empty_like(A)
+    one = (C.c_double * 2)(1.0, 0.0)
+    zero = (C.c_double * 2)(0.0, 0.0)
+    nn = N * N
+
+    def call():
+        ctx.check(ctx.lib.vrc_zgemm(ctx.h, 0, 0, N, N, N, one, C.c_void_p(A.data_ptr()), N, nn,
+                                    C.c_void_p(B.data_ptr()), N, nn, zero, C.c_void_p(Cm.data_ptr()), N, nn, batch),
+                  "zgemm")
+
+    for _ in range(2):
+        call()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    ctx.sync()
+    t = time.perf_counter() - t0
+    del A, B, Cm
+    return 8.0 * N ** 3 * batch * reps / t / 1e12
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -547,11 +578,22 @@ def main():
         per_launch = flops_qr(N) * mats_per_launch
         ach = per_launch / (avg_ms * 1e-3) / 1e12
         tpm = hqr_traffic_per_matrix()
-        roof = {"bound": "mfma", "kernel": "hqr_kernel (multishift QR, Schur phase)", "achieved": ach,
+        qv = [int(t.split("=")[1]) for t in args.tune if t.startswith("zgeev.qr_variant=")]
+        kname = {0: "hqr_kernel", 1: "hqr_lds_kernel", 2: "hqr_w48_kernel"}[qv[-1] if qv else 0]
+        roof = {"bound": "mfma", "kernel": f"{kname} (multishift QR, Schur phase)", "achieved": ach,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                 "traffic": tpm * mats_per_launch if tpm else None,
                 "traffic_source": TRAFFIC_SUMMARY.relative_to(ROOT).as_posix() if tpm else None,
                 "avg_launch_ms": avg_ms, "flops_per_launch": per_launch, "matrices_per_launch": mats_per_launch}
+        try:
+            ceil = zgemm_ceiling(ctx, N)
+            roof["measured_zgemm_ceiling"] = ceil
+            roof["frac_of_measured_zgemm_ceiling"] = ach / ceil
+            roof["measured_zgemm_ceiling_definition"] = (
+                f"this library's batched ZGEMM, 128 x ({N} x {N} x {N}), 8 N^3 flops each, wall clock of 5 launches")
+            log(f"ZGEMM ceiling at N={N}: {ceil:.1f} TFLOP/s; QR kernel at {ach / ceil:.1%} of it")
+        except Exception as e:  # the ceiling is supplementary
+            log(f"ZGEMM ceiling not measured: {e}")
     fourier = None
     if st["fourier"][1]:
         fb = fourier_bytes(nh, cells_per_step / (len(insts) * L)) * len(insts) * L * steps
