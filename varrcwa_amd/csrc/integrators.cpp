@@ -19,10 +19,11 @@ static const cd ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
 static int g_cascade_group = 128;
 void stack_set_cascade_group(int n) { g_cascade_group = n < 1 ? 1 : n; }
 // Instances per cascade group: g_cascade_group, fewer when their slab S-matrices and end
-// systems would take more than 64 GiB (deep stacks at large N).
+// systems would take more than 96 GiB (deep stacks at large N; 128 instances of 10 slabs at
+// N = 1250 take 77 GB, so their 1280 slab eigensolves still fill five 256-matrix launches).
 static int cascade_group_cap(int N, int L) {
   const size_t per_inst = sizeof(cd) * (size_t)N * N * (2 * (size_t)L + 4);
-  const size_t cap = std::max<size_t>(1, ((size_t)64 << 30) / per_inst);
+  const size_t cap = std::max<size_t>(1, ((size_t)96 << 30) / per_inst);
   return (int)std::max<size_t>(1, std::min<size_t>((size_t)g_cascade_group, cap));
 }
 
