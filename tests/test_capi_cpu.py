@@ -67,3 +67,29 @@ def test_gds_reader_host_entry_point(tmp_path):
             assert api.gds_polygons(path, 1e-6, cell) == ref[cell]
     with pytest.raises(api.VrcError):
         api.gds_polygons(str(tmp_path / "missing.gds"))
+
+
+def test_tuning_switches_and_cache_on_the_host(tmp_path):
+    """vrc_tune_set validates keys and values, and the autotuner's cache file is parsed on the
+    host: seven-field lines (QR kernel 0) and eight-field lines load, a QR kernel that is not
+    a tuning candidate, a malformed line or a missing file is an error. No device call."""
+    lib = _lib.load()
+    try:
+        assert lib.vrc_tune_set(b"no.such.switch", 1) == -1
+        assert lib.vrc_tune_set(None, 1) == -1
+        assert lib.vrc_tune_set(b"zgeev.qr_variant", 4) == -1
+        assert lib.vrc_tune_set(b"zgeev.qr_variant", 2) == 0
+        assert lib.vrc_tune_set(b"zgeev.qr_variant", 0) == 0
+        assert lib.vrc_tune_set(b"zgeev.bulges_w48", 8) == 0
+        good = tmp_path / "tune.txt"
+        good.write_text("# a comment\n882 24 12 128 8 8 129\n242 16 10 64 8 4 65 2\n")
+        assert lib.vrc_autotune_load(str(good).encode()) == 2
+        for bad in ("882 24 12 128 8 8\n", "882 24 12 128 8 8 129 1\n", "1 24 12 128 8 8 129\n", "x y\n"):
+            f = tmp_path / "bad.txt"
+            f.write_text(bad)
+            assert lib.vrc_autotune_load(str(f).encode()) == -1, bad
+        assert lib.vrc_autotune_load(str(tmp_path / "missing.txt").encode()) == -1
+        assert lib.vrc_autotune_load(None) == -1
+    finally:
+        assert lib.vrc_autotune_clear() == 0
+        lib.vrc_tune_set(b"zgeev.qr_variant", 0)
