@@ -579,7 +579,7 @@ def main():
         ach = per_launch / (avg_ms * 1e-3) / 1e12
         tpm = hqr_traffic_per_matrix()
         qv = [int(t.split("=")[1]) for t in args.tune if t.startswith("zgeev.qr_variant=")]
-        kname = {1: "hqr_lds_kernel", 2: "hqr_w48_kernel"}.get(qv[-1] if qv else 0, "hqr_kernel")
+        kname = {1: "hqr_lds_kernel", 2: "hqr_w48_kernel", 4: "hqr_w32_kernel"}.get(qv[-1] if qv else 0, "hqr_kernel")
         roof = {"bound": "mfma", "kernel": f"{kname} (multishift QR, Schur phase)", "achieved": ach,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                 "traffic": tpm * mats_per_launch if tpm else None,

@@ -70,7 +70,10 @@ int vrc_timing_enable(vrc_ctx* ctx, int on);
  * "zgeev.qr_variant" (default 0): the QR (Schur) kernel, 0: 64-wide window on 512 threads,
  * window products formed from registers; 1: the same with the window products staged
  * through LDS; 2: 48-wide window on 256 threads, two matrices per CU, register products;
- * "zgeev.bulges_w48" (default 8, 1..12): bulges per sweep of variant 2;
+ * 4: 32-wide window on 128 threads, four matrices per CU (small N); batched calls size their
+ * eigensolver launches to 256, 512 or 1024 matrices accordingly;
+ * "zgeev.bulges_w48" (default 8, 1..12) and "zgeev.bulges_w32" (default 6, 1..7): bulges per
+ * sweep of variants 2 and 4;
  * "zgeev.max_sweeps" (default 0 = 30 * max(N, 10)): QR sweeps per matrix before the
  * eigensolver gives up and reports info > 0; "redheffer.max_depth" (default 40, 0..60):
  * trisection depth at which vrc_redheffer_compute accepts a segment regardless of pe
@@ -81,7 +84,7 @@ int vrc_timing_read(vrc_ctx* ctx, double* ms, int* counts, int nstages);
 /* Autotuner (no reference counterpart). vrc_autotune_stack times vrc_batch_stack_rcwa on
  * the given instances (a sample of the batch to be solved; a few hundred slabs make full
  * launches) under candidate kernel switch settings, one switch at a time (the QR kernel
- * (zgeev.qr_variant 0 or 2), the QR iteration's deflation window and bulge count, the
+ * (zgeev.qr_variant 0, 2 or 4), the QR iteration's deflation window and bulge count, the
  * ZGEMM tile shape, wavefronts and large-tile threshold), keeping a candidate only when it
  * is >= 1 % faster, until
  * budget_s seconds have been spent. The winners are kept for N = 2 nx ny and put in

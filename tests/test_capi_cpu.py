@@ -77,13 +77,15 @@ def test_tuning_switches_and_cache_on_the_host(tmp_path):
     try:
         assert lib.vrc_tune_set(b"no.such.switch", 1) == -1
         assert lib.vrc_tune_set(None, 1) == -1
-        assert lib.vrc_tune_set(b"zgeev.qr_variant", 4) == -1
+        assert lib.vrc_tune_set(b"zgeev.qr_variant", 3) == -1
+        assert lib.vrc_tune_set(b"zgeev.qr_variant", 5) == -1
+        assert lib.vrc_tune_set(b"zgeev.qr_variant", 4) == 0
         assert lib.vrc_tune_set(b"zgeev.qr_variant", 2) == 0
         assert lib.vrc_tune_set(b"zgeev.qr_variant", 0) == 0
         assert lib.vrc_tune_set(b"zgeev.bulges_w48", 8) == 0
         good = tmp_path / "tune.txt"
-        good.write_text("# a comment\n882 24 12 128 8 8 129\n242 16 10 64 8 4 65 2\n")
-        assert lib.vrc_autotune_load(str(good).encode()) == 2
+        good.write_text("# a comment\n882 24 12 128 8 8 129\n242 16 10 64 8 4 65 2\n98 16 6 64 8 4 65 4\n")
+        assert lib.vrc_autotune_load(str(good).encode()) == 3
         for bad in ("882 24 12 128 8 8\n", "882 24 12 128 8 8 129 1\n", "1 24 12 128 8 8 129\n", "x y\n"):
             f = tmp_path / "bad.txt"
             f.write_text(bad)

@@ -168,12 +168,13 @@ def test_zgeev_triangular_and_diag(ctx):
     _eig_check(ctx, np.stack([A, D]))
 
 
-@pytest.mark.parametrize("variant", [1, 2])
-@pytest.mark.parametrize("n", [49, 97, 200, 441])
+@pytest.mark.parametrize("variant", [1, 2, 4])
+@pytest.mark.parametrize("n", [20, 49, 97, 200, 441])
 def test_zgeev_qr_kernel_variants(ctx, variant, n):
     """The other QR kernel configurations of "zgeev.qr_variant": 1 (64-wide window, window
-    products staged through LDS), 2 (48-wide window on 256 threads, two matrices per CU)
-    against LAPACK eigenvalues and eigen-residuals, like the default kernel."""
+    products staged through LDS), 2 (48-wide window on 256 threads, two matrices per CU),
+    4 (32-wide window on 128 threads, four per CU) against LAPACK eigenvalues and
+    eigen-residuals, like the default kernel."""
     rng = np.random.default_rng(2000 + n)
     A = crand(rng, 3, n, n)
     assert ctx.lib.vrc_tune_set(b"zgeev.qr_variant", variant) == 0

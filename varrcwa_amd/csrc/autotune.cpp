@@ -108,7 +108,8 @@ int tune_load(const std::string& path) {
     if (!(is >> N >> k.aed >> k.bulges >> k.big_n >> k.big_waves >> k.small_waves >> k.big_min_k) || N < 2)
       throw ArgFailure("malformed tuning cache line: " + line);
     if (!(is >> k.qr_variant)) k.qr_variant = 0;
-    if (k.qr_variant != 0 && k.qr_variant != 2) throw ArgFailure("malformed tuning cache line: " + line);
+    if (k.qr_variant != 0 && k.qr_variant != 2 && k.qr_variant != 4)
+      throw ArgFailure("malformed tuning cache line: " + line);
     tune_store(N, k);
     ++n;
   }
@@ -151,7 +152,7 @@ std::string autotune_stack(vrc_ctx& c, int nx, int ny, const std::vector<StackIn
     const double t_own = run(own);
     double t_best = t_own;
     const std::vector<std::pair<int TuneKnobs::*, std::vector<int>>> space = {
-        {&TuneKnobs::qr_variant, {0, 2}},
+        {&TuneKnobs::qr_variant, {0, 2, 4}},
         {&TuneKnobs::aed, {16, 20, 24, 32, 40, 48}},  {&TuneKnobs::bulges, {6, 8, 10, 12, 14, 16}},
         {&TuneKnobs::big_n, {64, 128}},              {&TuneKnobs::big_waves, {4, 8}},
         {&TuneKnobs::small_waves, {4, 8}},           {&TuneKnobs::big_min_k, {33, 65, 129}}};

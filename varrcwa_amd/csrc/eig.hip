@@ -570,6 +570,7 @@ struct QrLds {
   int ctl[4];
 };
 static_assert(sizeof(QrLds<48, 256>) <= 80 * 1024, "two W = 48 QR workgroups must fit one CU's LDS");
+static_assert(sizeof(QrLds<32, 128>) <= 40 * 1024, "four W = 32 QR workgroups must fit one CU's LDS");
 
 // Window products from registers (no LDS chunk buffer, no barrier): with the 16x16x4 FP64
 // MFMA, lane l supplies A[l & 15][l >> 4] and B[l >> 4][l & 15] and holds
@@ -1196,6 +1197,12 @@ __device__ void hqr_body(QrLds<C::W, C::T>& L, int N, cd* H, long ldh, long sH, 
   Z += b * sZ;
   wout += b * sw;
   const int t = threadIdx.x;
+  // blocks this small are finished by the single-wave QR in the window; at least 8 chase
+  // steps per window
+  constexpr int NMIN = QR_NMIN < C::W ? QR_NMIN : C::W;
+  constexpr int BMAX = (C::W - 9) / 3 < NBUL ? (C::W - 9) / 3 : NBUL;
+  if (max_bul > BMAX) max_bul = BMAX;
+  if (use_aed > C::W - 1) use_aed = C::W - 1;
   int ihi = N - 1;
   int sweeps = 0, last_progress = 0, prev_lo = -1, prev_hi = N;
   const int itmax = max_sweeps > 0 ? max_sweeps : 30 * (N > 10 ? N : 10);
@@ -1206,7 +1213,7 @@ __device__ void hqr_body(QrLds<C::W, C::T>& L, int N, cd* H, long ldh, long sH, 
     if (lo != prev_lo || ihi != prev_hi) last_progress = sweeps;
     prev_lo = lo;
     prev_hi = ihi;
-    if (m <= QR_NMIN) {
+    if (m <= NMIN) {
       const long long c0 = wall_clock64();
       lds_load_block<C>(L, H, ldh, lo, m);
       if ((t >> 6) == 0) {
@@ -1244,7 +1251,7 @@ __device__ void hqr_body(QrLds<C::W, C::T>& L, int N, cd* H, long ldh, long sH, 
         last_progress = sweeps;
         prev_hi = ihi;
         // a productive window: look again before sweeping (LAPACK's nibble rule)
-        if (100 * nd > AED_NIBBLE * nw || ihi - lo + 1 <= QR_NMIN) {
+        if (100 * nd > AED_NIBBLE * nw || ihi - lo + 1 <= NMIN) {
           ++sweeps;
           continue;
         }
@@ -1317,6 +1324,11 @@ __global__ __launch_bounds__(512) void hqr_lds_kernel(HQR_ARGS) {
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void hqr_w48_kernel(HQR_ARGS) {
   __shared__ QrLds<48, 256> L;
   hqr_body<QrCfg<48, 256, true>>(L, HQR_PASS);
+}
+// W = 32 on 128 threads: four workgroups (matrices) per CU, for small N.
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void hqr_w32_kernel(HQR_ARGS) {
+  __shared__ QrLds<32, 128> L;
+  hqr_body<QrCfg<32, 128, true>>(L, HQR_PASS);
 }
 #undef HQR_ARGS
 #undef HQR_PASS
@@ -1557,13 +1569,16 @@ void zgeev_set_max_sweeps(int n) { g_max_sweeps = n < 0 ? 0 : n; }
 void zgeev_set_aed(int window) { g_aed_window = window < 0 ? 0 : (window > QW ? QW : window); }
 void zgeev_set_bulges(int nb) { g_bulges = nb < 1 ? 1 : (nb > NBUL ? NBUL : nb); }
 // 0: hqr_kernel (W = 64, window products from registers), 1: hqr_lds_kernel (W = 64,
-// window products staged through LDS), 2: hqr_w48_kernel (W = 48, two matrices per CU)
+// window products staged through LDS), 2: hqr_w48_kernel (W = 48, two matrices per CU),
+// 4: hqr_w32_kernel (W = 32, four matrices per CU)
 static int g_qr_variant = 0;
 static int g_bulges_w48 = 8;  // bulges per sweep of the W = 48 kernel (<= 12: >= 11 chase steps per window)
-void zgeev_set_qr_variant(int v) { g_qr_variant = v < 0 ? 0 : (v > 2 ? 2 : v); }
+static int g_bulges_w32 = 6;  // bulges per sweep of the W = 32 kernel (<= 7: >= 10 chase steps per window)
+void zgeev_set_qr_variant(int v) { g_qr_variant = (v < 0 || v > 4 || v == 3) ? 0 : v; }
 void zgeev_set_bulges_w48(int nb) { g_bulges_w48 = nb < 1 ? 1 : (nb > 12 ? 12 : nb); }
+void zgeev_set_bulges_w32(int nb) { g_bulges_w32 = nb < 1 ? 1 : (nb > 7 ? 7 : nb); }
 int zgeev_get_qr_variant() { return g_qr_variant; }
-int zgeev_full_launch() { return g_qr_variant == 2 ? 512 : 256; }
+int zgeev_full_launch() { return g_qr_variant == 4 ? 1024 : (g_qr_variant == 2 ? 512 : 256); }
 int zgeev_get_aed() { return g_aed_window; }
 int zgeev_get_bulges() { return g_bulges; }
 
@@ -1593,7 +1608,10 @@ void zgeev_factors(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
     DevBuf sb;
     if (want) sb = ws.get(sizeof(long long) * 5 * batch);
     long long* stp = want ? sb.as<long long>() : nullptr;
-    if (g_qr_variant == 2) {
+    if (g_qr_variant == 4) {
+      hqr_w32_kernel<<<batch, 128, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, stp, g_aed_window, g_bulges_w32,
+                                           g_max_sweeps);
+    } else if (g_qr_variant == 2) {
       hqr_w48_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, stp,
                                            g_aed_window < 48 ? g_aed_window : 47, g_bulges_w48, g_max_sweeps);
     } else if (g_qr_variant == 1) {
