@@ -577,9 +577,13 @@ def main():
         avg_ms = qr_ms / qr_cnt
         per_launch = flops_qr(N) * mats_per_launch
         ach = per_launch / (avg_ms * 1e-3) / 1e12
-        tpm = hqr_traffic_per_matrix()
+        # the committed traffic profile is of the default kernel at N = 882
         qv = [int(t.split("=")[1]) for t in args.tune if t.startswith("zgeev.qr_variant=")]
-        kname = {1: "hqr_lds_kernel", 2: "hqr_w48_kernel", 4: "hqr_w32_kernel"}.get(qv[-1] if qv else 0, "hqr_kernel")
+        qsel = qv[-1] if qv else -1
+        if qsel < 0:  # the library's automatic choice
+            qsel = 4 if (N <= 512 and mats_per_launch >= 768) else 0
+        kname = {1: "hqr_lds_kernel", 2: "hqr_w48_kernel", 4: "hqr_w32_kernel"}.get(qsel, "hqr_kernel")
+        tpm = hqr_traffic_per_matrix() if (N == 882 and qsel == 0) else None
         roof = {"bound": "mfma", "kernel": f"{kname} (multishift QR, Schur phase)", "achieved": ach,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                 "traffic": tpm * mats_per_launch if tpm else None,

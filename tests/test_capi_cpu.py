@@ -71,8 +71,8 @@ def test_gds_reader_host_entry_point(tmp_path):
 
 def test_tuning_switches_and_cache_on_the_host(tmp_path):
     """vrc_tune_set validates keys and values, and the autotuner's cache file is parsed on the
-    host: seven-field lines (QR kernel 0) and eight-field lines load, a QR kernel that is not
-    a tuning candidate, a malformed line or a missing file is an error. No device call."""
+    host: seven-field lines (automatic QR kernel) and eight-field lines load, a QR kernel that
+    is not a tuning candidate, a malformed line or a missing file is an error. No device call."""
     lib = _lib.load()
     try:
         assert lib.vrc_tune_set(b"no.such.switch", 1) == -1
@@ -80,6 +80,8 @@ def test_tuning_switches_and_cache_on_the_host(tmp_path):
         assert lib.vrc_tune_set(b"zgeev.qr_variant", 3) == -1
         assert lib.vrc_tune_set(b"zgeev.qr_variant", 5) == -1
         assert lib.vrc_tune_set(b"zgeev.qr_variant", 4) == 0
+        assert lib.vrc_tune_set(b"zgeev.qr_variant", -1) == 0
+        assert lib.vrc_tune_set(b"zgeev.qr_variant", -2) == -1
         assert lib.vrc_tune_set(b"zgeev.qr_variant", 2) == 0
         assert lib.vrc_tune_set(b"zgeev.qr_variant", 0) == 0
         assert lib.vrc_tune_set(b"zgeev.bulges_w48", 8) == 0
@@ -94,4 +96,4 @@ def test_tuning_switches_and_cache_on_the_host(tmp_path):
         assert lib.vrc_autotune_load(None) == -1
     finally:
         assert lib.vrc_autotune_clear() == 0
-        lib.vrc_tune_set(b"zgeev.qr_variant", 0)
+        lib.vrc_tune_set(b"zgeev.qr_variant", -1)

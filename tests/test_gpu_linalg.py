@@ -181,4 +181,4 @@ def test_zgeev_qr_kernel_variants(ctx, variant, n):
     try:
         _eig_check(ctx, A)
     finally:
-        assert ctx.lib.vrc_tune_set(b"zgeev.qr_variant", 0) == 0
+        assert ctx.lib.vrc_tune_set(b"zgeev.qr_variant", -1) == 0

@@ -85,8 +85,8 @@ void tune_clear() {
 }
 
 // Cache file: one line per N, "N aed bulges big_n big_waves small_waves big_min_k
-// [qr_variant]" (a line without the last field means QR kernel 0); lines starting with '#'
-// are comments. Later lines for the same N win.
+// [qr_variant]" (a line without the last field leaves the QR kernel to the automatic
+// choice, -1); lines starting with '#' are comments. Later lines for the same N win.
 void tune_save(const std::string& path, int N, const TuneKnobs& k, const std::string& note) {
   std::ofstream f(path, std::ios::app);
   if (!f) throw ArgFailure("cannot append to the tuning cache " + path);
@@ -107,8 +107,8 @@ int tune_load(const std::string& path) {
     TuneKnobs k{};
     if (!(is >> N >> k.aed >> k.bulges >> k.big_n >> k.big_waves >> k.small_waves >> k.big_min_k) || N < 2)
       throw ArgFailure("malformed tuning cache line: " + line);
-    if (!(is >> k.qr_variant)) k.qr_variant = 0;
-    if (k.qr_variant != 0 && k.qr_variant != 2 && k.qr_variant != 4)
+    if (!(is >> k.qr_variant)) k.qr_variant = -1;
+    if (k.qr_variant != -1 && k.qr_variant != 0 && k.qr_variant != 2 && k.qr_variant != 4)
       throw ArgFailure("malformed tuning cache line: " + line);
     tune_store(N, k);
     ++n;

@@ -118,7 +118,7 @@ int vrc_tune_set(const char* key, long long value) {
     return 0;
   }
   if (std::strcmp(key, "zgeev.qr_variant") == 0) {
-    if (value < 0 || value > 4 || value == 3) return -1;
+    if (value < -1 || value > 4 || value == 3) return -1;
     zgeev_set_qr_variant((int)value);
     return 0;
   }

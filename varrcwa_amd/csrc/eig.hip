@@ -1570,15 +1570,23 @@ void zgeev_set_aed(int window) { g_aed_window = window < 0 ? 0 : (window > QW ? 
 void zgeev_set_bulges(int nb) { g_bulges = nb < 1 ? 1 : (nb > NBUL ? NBUL : nb); }
 // 0: hqr_kernel (W = 64, window products from registers), 1: hqr_lds_kernel (W = 64,
 // window products staged through LDS), 2: hqr_w48_kernel (W = 48, two matrices per CU),
-// 4: hqr_w32_kernel (W = 32, four matrices per CU)
-static int g_qr_variant = 0;
+// 4: hqr_w32_kernel (W = 32, four matrices per CU); -1: automatic, 4 for N <= 512 in launches
+// of at least 768 matrices (three or more per CU), else 0
+static int g_qr_variant = -1;
+static int qr_variant_for(int n, int batch) {
+  if (g_qr_variant >= 0) return g_qr_variant;
+  return (n <= 512 && batch >= 768) ? 4 : 0;
+}
 static int g_bulges_w48 = 8;  // bulges per sweep of the W = 48 kernel (<= 12: >= 11 chase steps per window)
 static int g_bulges_w32 = 6;  // bulges per sweep of the W = 32 kernel (<= 7: >= 10 chase steps per window)
-void zgeev_set_qr_variant(int v) { g_qr_variant = (v < 0 || v > 4 || v == 3) ? 0 : v; }
+void zgeev_set_qr_variant(int v) { g_qr_variant = (v < -1 || v > 4 || v == 3) ? -1 : v; }
 void zgeev_set_bulges_w48(int nb) { g_bulges_w48 = nb < 1 ? 1 : (nb > 12 ? 12 : nb); }
 void zgeev_set_bulges_w32(int nb) { g_bulges_w32 = nb < 1 ? 1 : (nb > 7 ? 7 : nb); }
 int zgeev_get_qr_variant() { return g_qr_variant; }
-int zgeev_full_launch() { return g_qr_variant == 4 ? 1024 : (g_qr_variant == 2 ? 512 : 256); }
+int zgeev_full_launch(int n) {
+  const int v = g_qr_variant >= 0 ? g_qr_variant : (n <= 512 ? 4 : 0);
+  return v == 4 ? 1024 : (v == 2 ? 512 : 256);
+}
 int zgeev_get_aed() { return g_aed_window; }
 int zgeev_get_bulges() { return g_bulges; }
 
@@ -1608,13 +1616,14 @@ void zgeev_factors(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
     DevBuf sb;
     if (want) sb = ws.get(sizeof(long long) * 5 * batch);
     long long* stp = want ? sb.as<long long>() : nullptr;
-    if (g_qr_variant == 4) {
+    const int qv = qr_variant_for(n, batch);
+    if (qv == 4) {
       hqr_w32_kernel<<<batch, 128, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, stp, g_aed_window, g_bulges_w32,
                                            g_max_sweeps);
-    } else if (g_qr_variant == 2) {
+    } else if (qv == 2) {
       hqr_w48_kernel<<<batch, 256, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, stp,
                                            g_aed_window < 48 ? g_aed_window : 47, g_bulges_w48, g_max_sweeps);
-    } else if (g_qr_variant == 1) {
+    } else if (qv == 1) {
       hqr_lds_kernel<<<batch, QT, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, stp, g_aed_window, g_bulges,
                                           g_max_sweeps);
     } else {

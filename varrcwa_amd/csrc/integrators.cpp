@@ -16,7 +16,7 @@ static const cd ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
 
 // Host -> device copy of a small array on the context stream (kept alive until done).
 // Instances whose star products and projections run in one batch (batch_stack_rcwa).
-static int g_cascade_group = 128;
+static int g_cascade_group = 512;
 void stack_set_cascade_group(int n) { g_cascade_group = n < 1 ? 1 : n; }
 // Instances per cascade group: g_cascade_group, fewer when their slab S-matrices and end
 // systems would take more than 96 GiB (deep stacks at large N; 128 instances of 10 slabs at
@@ -1040,14 +1040,14 @@ static void batch_stack_solve(vrc_ctx& c, int nx, int ny, const std::vector<Stac
     homog = homog && I.bottom->x.size() == 2 && I.bottom->y.size() == 2 && I.top->x.size() == 2 &&
             I.top->y.size() == 2;
   if (homog && !tape) {
-    batch_stack_homog(c, nx, ny, inst, chunk > 0 ? chunk * L : zgeev_full_launch(), S_out, W0o, V0o, g0o, W1o, V1o,
+    batch_stack_homog(c, nx, ny, inst, chunk > 0 ? chunk * L : zgeev_full_launch(N), S_out, W0o, V0o, g0o, W1o, V1o,
                       g1o, flags);
     return;
   }
   // Default chunk: at most one full launch of dense eigensolves (L slices, + 2 end media
   // per instance unless closed form), one matrix per QR workgroup slot of the GPU; the
   // instances are then split evenly over the chunks so that no launch runs a short tail.
-  if (chunk <= 0) chunk = std::max(1, std::min(ninst, zgeev_full_launch() / (L + (homog ? 0 : 2))));
+  if (chunk <= 0) chunk = std::max(1, std::min(ninst, zgeev_full_launch(N) / (L + (homog ? 0 : 2))));
   const int nchunks = (ninst + chunk - 1) / chunk;
   std::vector<int> starts(nchunks + 1, 0);
   for (int ci = 0; ci < nchunks; ++ci) starts[ci + 1] = starts[ci] + ninst / nchunks + (ci < ninst % nchunks ? 1 : 0);

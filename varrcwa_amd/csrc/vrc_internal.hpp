@@ -144,11 +144,11 @@ void zgeev_set_bulges(int nb);  // bulges per QR sweep (1..16)
 int zgeev_get_aed();
 int zgeev_get_bulges();
 void zgeev_set_qr_variant(int v);   // 0: W = 64, register products, 1: W = 64, LDS-staged, 2: W = 48,
-                                    // 4: W = 32
+                                    // 4: W = 32; -1 (default): by N and batch
 int zgeev_get_qr_variant();
 // matrices of one eigensolve launch that fill every QR workgroup slot (256 CUs; two slots
 // per CU for variant 2, four for variant 4)
-int zgeev_full_launch();
+int zgeev_full_launch(int n);
 void zgeev_set_bulges_w48(int nb);  // bulges per sweep of the W = 48 kernel (1..12)
 void zgeev_set_bulges_w32(int nb);  // bulges per sweep of the W = 32 kernel (1..7)
 
@@ -156,7 +156,7 @@ void zgeev_set_bulges_w32(int nb);  // bulges per sweep of the W = 32 kernel (1.
 // The kernel switches an autotuned harmonic count overrides (process-wide values).
 struct TuneKnobs {
   int aed, bulges, big_n, big_waves, small_waves, big_min_k;
-  int qr_variant;  // 0, 2 or 4 (zgeev_set_qr_variant)
+  int qr_variant;  // -1 (automatic), 0, 2 or 4 (zgeev_set_qr_variant)
 };
 TuneKnobs tune_get();
 void tune_put(const TuneKnobs& k);
