@@ -581,7 +581,7 @@ def main():
         qv = [int(t.split("=")[1]) for t in args.tune if t.startswith("zgeev.qr_variant=")]
         qsel = qv[-1] if qv else -1
         if qsel < 0:  # the library's automatic choice
-            qsel = 4 if (N <= 512 and mats_per_launch >= 768) else 0
+            qsel = 4 if ((N <= 256 and mats_per_launch >= 384) or (N <= 512 and mats_per_launch >= 768)) else 0
         kname = {1: "hqr_lds_kernel", 2: "hqr_w48_kernel", 4: "hqr_w32_kernel"}.get(qsel, "hqr_kernel")
         tpm = hqr_traffic_per_matrix() if (N == 882 and qsel == 0) else None
         roof = {"bound": "mfma", "kernel": f"{kname} (multishift QR, Schur phase)", "achieved": ach,

@@ -1570,12 +1570,14 @@ void zgeev_set_aed(int window) { g_aed_window = window < 0 ? 0 : (window > QW ? 
 void zgeev_set_bulges(int nb) { g_bulges = nb < 1 ? 1 : (nb > NBUL ? NBUL : nb); }
 // 0: hqr_kernel (W = 64, window products from registers), 1: hqr_lds_kernel (W = 64,
 // window products staged through LDS), 2: hqr_w48_kernel (W = 48, two matrices per CU),
-// 4: hqr_w32_kernel (W = 32, four matrices per CU); -1: automatic, 4 for N <= 512 in launches
-// of at least 768 matrices (three or more per CU), else 0
+// 4: hqr_w32_kernel (W = 32, four matrices per CU); -1: automatic, 4 for N <= 256 in launches
+// of at least 384 matrices and for N <= 512 in launches of at least 768 (measured: at N = 98
+// and 242 the four-per-CU kernel wins from about 1.5 matrices per CU, at N = 450 from about 3),
+// else 0
 static int g_qr_variant = -1;
 static int qr_variant_for(int n, int batch) {
   if (g_qr_variant >= 0) return g_qr_variant;
-  return (n <= 512 && batch >= 768) ? 4 : 0;
+  return ((n <= 256 && batch >= 384) || (n <= 512 && batch >= 768)) ? 4 : 0;
 }
 static int g_bulges_w48 = 8;  // bulges per sweep of the W = 48 kernel (<= 12: >= 11 chase steps per window)
 static int g_bulges_w32 = 6;  // bulges per sweep of the W = 32 kernel (<= 7: >= 10 chase steps per window)
