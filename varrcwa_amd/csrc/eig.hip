@@ -94,11 +94,12 @@ __device__ inline cd load_stream(const cd* p) {
 }
 
 // out[i] = sum_{r >= r0} conj(Vf[r, p+i]) x[r] for i < j (x in LDS): wave w reduces the
-// columns i = w, w + PT/64, ... (the columns proceed in parallel, four loads in flight per
+// columns i = w, w + PTT/64, ... (the columns proceed in parallel, four loads in flight per
 // lane, rows summed in increasing order within a lane).
+template <int PTT>
 __device__ void panel_multi_dot(int N, int r0, int p, int j, const cd* Vf, const cd* x, cd* out) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int i = wave; i < j; i += PT / 64) {
+  for (int i = wave; i < j; i += PTT / 64) {
     const cd* vc = Vf + (long)(p + i) * N;
     double sr = 0.0, si = 0.0;
     int r = r0 + lane;
@@ -139,7 +140,11 @@ __device__ inline cd sub_row_dot(cd acc, const cd* M, long ld, int r, int j, con
   return acc;
 }
 
-__global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, cd* A, long lda, long sA, cd* Vf,
+// PTT threads: 1024 (16 waves) in general, 256 or 512 when N - 1 fits, so that several
+// matrices share a CU (at most 128 VGPRs; the panel's T factor packed upper-triangular).
+template <int PTT>
+__global__ __launch_bounds__(PTT) __attribute__((amdgpu_waves_per_eu(4))) void gehrd_panel_kernel(
+    int N, int p, int nb, cd* A, long lda, long sA, cd* Vf,
                                                           long sV, cd* Y, long sY, cd* Tp, long sT, cd* tau,
                                                           long sTau) {
   const long b = blockIdx.x;
@@ -151,38 +156,38 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
   extern __shared__ cd dyn[];
   cd* bvec = dyn;
   cd* vvec = dyn + N;
-  __shared__ cd Tl[HB][HB + 1];
+  __shared__ cd Tl[HB * (HB + 1) / 2];  // T(i, j), i <= j, at j (j + 1) / 2 + i
   __shared__ cd vec1[HB], vec2[HB];
-  __shared__ double rn[PT / 64];
-  __shared__ cd gpart[PT];  // per-thread partial sums of the column-split product
+  __shared__ double rn[PTT / 64];
+  __shared__ cd gpart[PTT];  // per-thread partial sums of the column-split product
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   // rows below the panel; when they fit in the workgroup, the columns of the trailing
   // product are split over G groups of R threads (partial sums reduced through LDS)
   const int R = N - p - 1;
-  const int G = (R > 0 && R <= PT) ? (PT / R < GV_MAXG ? PT / R : GV_MAXG) : 1;
+  const int G = (R > 0 && R <= PTT) ? (PTT / R < GV_MAXG ? PTT / R : GV_MAXG) : 1;
 
   for (int j = 0; j < nb; ++j) {
     const int c = p + j;
     if (t < j) vec1[t] = cconj(Vf[c + (long)(p + t) * N]);
     __syncthreads();
     // column c of A0*Q_j, rows below p (rows <= p are updated after the panel)
-    for (int r = p + 1 + t; r < N; r += PT) bvec[r] = sub_row_dot(A[r + (long)c * lda], Y, N, r, j, vec1);
+    for (int r = p + 1 + t; r < N; r += PTT) bvec[r] = sub_row_dot(A[r + (long)c * lda], Y, N, r, j, vec1);
     __syncthreads();
     if (j > 0) {
       // left transform by Q_j^H on rows >= p+1
-      panel_multi_dot(N, p + 1, p, j, Vf, bvec, vec2);
+      panel_multi_dot<PTT>(N, p + 1, p, j, Vf, bvec, vec2);
       if (t < j) {
         cd s = mk(0.0, 0.0);
-        for (int l = 0; l <= t; ++l) s = cadd(s, cmul(cconj(Tl[l][t]), vec2[l]));
+        for (int l = 0; l <= t; ++l) s = cadd(s, cmul(cconj(Tl[t * (t + 1) / 2 + l]), vec2[l]));
         vec1[t] = s;
       }
       __syncthreads();
-      for (int r = p + 1 + t; r < N; r += PT) bvec[r] = sub_row_dot(bvec[r], Vf + (long)p * N, N, r, j, vec1);
+      for (int r = p + 1 + t; r < N; r += PTT) bvec[r] = sub_row_dot(bvec[r], Vf + (long)p * N, N, r, j, vec1);
       __syncthreads();
     }
     // final entries of column c and the reflector from rows c+1..N-1
     double part = 0.0;
-    for (int r = p + 1 + t; r < N; r += PT) {
+    for (int r = p + 1 + t; r < N; r += PTT) {
       if (r <= c) A[r + (long)c * lda] = bvec[r];
       if (r >= c + 2) part += cnorm2(bvec[r]);
     }
@@ -190,13 +195,13 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
     if (lane == 0) rn[wave] = part;
     __syncthreads();
     double xn2 = 0.0;
-    for (int w = 0; w < PT / 64; ++w) xn2 += rn[w];
+    for (int w = 0; w < PTT / 64; ++w) xn2 += rn[w];
     const double xnorm = sqrt(xn2);
     double beta;
     cd tv, scale;
     house(bvec[c + 1], xnorm, beta, tv, scale);
     const bool ident = (tv.x == 0.0 && tv.y == 0.0);
-    for (int r = t; r < N; r += PT) {
+    for (int r = t; r < N; r += PTT) {
       cd v;
       if (r <= c) v = mk(0.0, 0.0);
       else if (r == c + 1) v = mk(1.0, 0.0);
@@ -209,10 +214,10 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
     if (t == 0) tau[c] = tv;
     __syncthreads();
     // z = V_j^H v  (rows >= c+1)
-    if (j > 0) panel_multi_dot(N, c + 1, p, j, Vf, vvec, vec2);
+    if (j > 0) panel_multi_dot<PTT>(N, c + 1, p, j, Vf, vvec, vec2);
     // y_j = tau*(A0[:, c+1:] v - Y_j z) for the rows below p (the rows above are one
     // ZGEMM after the panel). The trailing-matrix product is the memory-bound part.
-    if (R > 0 && R <= PT) {
+    if (R > 0 && R <= PTT) {
       // thread (g, i): row p+1+i, columns c+1+GV_W g+GV_W Gk .. +GV_W (GV_W loads in flight)
       const int g = t / R, i = t - g * R;
       double sr = 0.0, si = 0.0;
@@ -243,7 +248,7 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
         Y[r + (long)j * N] = cmul(tv, acc);
       }
     } else
-    for (int rb = p + 1 + t; rb < N; rb += 4 * PT) {
+    for (int rb = p + 1 + t; rb < N; rb += 4 * PTT) {
       double gr[4] = {0.0, 0.0, 0.0, 0.0}, gi[4] = {0.0, 0.0, 0.0, 0.0};
       int col = c + 1;
       for (; col + 3 < N; col += 4) {
@@ -252,7 +257,7 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
         for (int u = 0; u < 4; ++u) v[u] = vvec[col + u];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int r = rb + PT * q;
+          const int r = rb + PTT * q;
           if (r < N) {
             const cd* ar = A + r + (long)col * lda;
             cd a[4];
@@ -270,7 +275,7 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
         const cd v = vvec[col];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int r = rb + PT * q;
+          const int r = rb + PTT * q;
           if (r < N) {
             const cd a = A[r + (long)col * lda];
             gr[q] += a.x * v.x - a.y * v.y;
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int r = rb + PT * q;
+        const int r = rb + PTT * q;
         if (r < N) {
           cd acc = mk(gr[q], gi[q]);
           for (int i = 0; i < j; ++i) acc = csub(acc, cmul(Y[r + (long)i * N], vec2[i]));
@@ -290,15 +295,15 @@ __global__ __launch_bounds__(PT) void gehrd_panel_kernel(int N, int p, int nb, c
     }
     if (t < j) {
       cd s = mk(0.0, 0.0);
-      for (int l = t; l < j; ++l) s = cadd(s, cmul(Tl[t][l], vec2[l]));
-      Tl[t][j] = cneg(cmul(tv, s));
+      for (int l = t; l < j; ++l) s = cadd(s, cmul(Tl[l * (l + 1) / 2 + t], vec2[l]));
+      Tl[j * (j + 1) / 2 + t] = cneg(cmul(tv, s));
     }
-    if (t == 0) Tl[j][j] = tv;
+    if (t == 0) Tl[j * (j + 1) / 2 + j] = tv;
     __syncthreads();
   }
-  for (int e = t; e < nb * nb; e += PT) {
+  for (int e = t; e < nb * nb; e += PTT) {
     int i = e % nb, jj = e / nb;
-    Tp[i + (long)jj * HB] = (i <= jj) ? Tl[i][jj] : mk(0.0, 0.0);
+    Tp[i + (long)jj * HB] = (i <= jj) ? Tl[jj * (jj + 1) / 2 + i] : mk(0.0, 0.0);
   }
 }
 
@@ -322,8 +327,15 @@ static void gehrd(hipStream_t s, int n, cd* A, long lda, long sA, cd* Vf, cd* Ta
   for (int p = 0; p < nref; p += HB) {
     const int nb = nref - p < HB ? nref - p : HB;
     cd* Tp = Tall + (long)(p / HB) * HB * HB;
-    gehrd_panel_kernel<<<batch, PT, 2 * sizeof(cd) * n, s>>>(n, p, nb, A, lda, sA, Vf, sV, Y, sY, Tp, sT,
-                                                                taub.as<cd>(), n);
+    if (n - 1 <= 256)
+      gehrd_panel_kernel<256><<<batch, 256, 2 * sizeof(cd) * n, s>>>(n, p, nb, A, lda, sA, Vf, sV, Y, sY, Tp, sT,
+                                                                     taub.as<cd>(), n);
+    else if (n - 1 <= 512)
+      gehrd_panel_kernel<512><<<batch, 512, 2 * sizeof(cd) * n, s>>>(n, p, nb, A, lda, sA, Vf, sV, Y, sY, Tp, sT,
+                                                                     taub.as<cd>(), n);
+    else
+      gehrd_panel_kernel<PT><<<batch, PT, 2 * sizeof(cd) * n, s>>>(n, p, nb, A, lda, sA, Vf, sV, Y, sY, Tp, sT,
+                                                                   taub.as<cd>(), n);
     launch_check("gehrd_panel");
     // rows 0..p: Y = A0 V T, then the panel columns' A0 Q = A0 - Y V^H
     zgemm_strided(s, OP_N, OP_N, p + 1, nb, n - p - 1, one, A + (long)(p + 1) * lda, lda, sA,

@@ -96,8 +96,11 @@ __global__ __launch_bounds__(256) void getrf_panel_kernel(int n, int k, int kb, 
 constexpr int LUR_NB = 16;
 constexpr int LUR_ROWS = 1024;
 
-__global__ __launch_bounds__(LUR_ROWS) void getrf_panel_reg_kernel(int n, int K, int KB, int k, int kb, cd* A,
-                                                                   long lda, long sA, int* ipiv, long sP, int* info) {
+// ROWS threads (one row each; ROWS >= n - k): 256 or 512 for small n, so that four or two
+// matrices share a CU (at most 128 VGPRs).
+template <int ROWS>
+__global__ __launch_bounds__(ROWS) __attribute__((amdgpu_waves_per_eu(4))) void getrf_panel_reg_kernel(
+    int n, int K, int KB, int k, int kb, cd* A, long lda, long sA, int* ipiv, long sP, int* info) {
   const long b = blockIdx.x;
   A += b * sA;
   ipiv += b * sP;
@@ -106,7 +109,7 @@ __global__ __launch_bounds__(LUR_ROWS) void getrf_panel_reg_kernel(int n, int K,
   const bool own = r < n;
   // One barrier per column: every wave posts its best candidate and that candidate's
   // row, the owner of row c posts row c; buffers alternate between columns.
-  constexpr int NW = LUR_ROWS / 64;
+  constexpr int NW = ROWS / 64;
   __shared__ double wv[2][NW];
   __shared__ int wi[2][NW];
   __shared__ cd wrow[2][NW][LUR_NB];
@@ -372,7 +375,12 @@ void zgetrf(hipStream_t s, int n, cd* A, long lda, long sA, int* ipiv, long sP, 
     if (reg) {
       for (int k = K; k < K + KB; k += LUR_NB) {
         const int kb = K + KB - k < LUR_NB ? K + KB - k : LUR_NB;
-        getrf_panel_reg_kernel<<<batch, LUR_ROWS, 0, s>>>(n, K, KB, k, kb, A, lda, sA, ipiv, sP, info);
+        if (n <= 256)
+          getrf_panel_reg_kernel<256><<<batch, 256, 0, s>>>(n, K, KB, k, kb, A, lda, sA, ipiv, sP, info);
+        else if (n <= 512)
+          getrf_panel_reg_kernel<512><<<batch, 512, 0, s>>>(n, K, KB, k, kb, A, lda, sA, ipiv, sP, info);
+        else
+          getrf_panel_reg_kernel<LUR_ROWS><<<batch, LUR_ROWS, 0, s>>>(n, K, KB, k, kb, A, lda, sA, ipiv, sP, info);
         launch_check("getrf_panel_reg");
         const int c0 = k + kb, c1 = K + KB;
         if (c1 > c0)
