@@ -30,7 +30,7 @@ def test_autotune_sample_applies_and_caches(ctx, tmp_path):
         assert text.startswith("N=50:") and "aed_window" in text and "qr_variant" in text
         lines = [l for l in cache.read_text().splitlines() if l and not l.startswith("#")]
         assert len(lines) == 1 and lines[0].split()[0] == "50" and len(lines[0].split()) == 8
-        assert lines[0].split()[7] in ("0", "2")
+        assert lines[0].split()[7] in ("-1", "0", "2", "4")
         after = from_dev_colmajor(api.batch_stack_rcwa(ctx, 5, 5, insts)["S"])
         assert np.abs(after - before).max() <= 1e-10 * np.abs(before).max()
         # another N is not affected, and the cache reads back
