@@ -3,6 +3,8 @@
 Tolerances: relative 1e-9 on basis-invariant S transforms and on
 |S| of non-degenerate modes; Fourier/P/Q stages compared elementwise at 1e-12.
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -14,6 +16,7 @@ from varrcwa_amd._lib import from_dev_colmajor, to_dev_colmajor
 pytestmark = pytest.mark.gpu
 
 TOL_S = 1e-9
+ROOT = Path(__file__).resolve().parent.parent
 
 
 def rel(a, b):
@@ -666,3 +669,21 @@ def test_homogeneous_closed_form(ctx):
     assert np.abs(PQ @ W0 - W0 * (g0 ** 2)[None, :]).max() <= 1e-13 * np.abs(PQ).max()
     assert np.abs(V0 - (Q @ W0) / g0[None, :]).max() <= 1e-13 * np.abs(V0).max()
     assert np.all(g0.imag >= 0)
+
+
+def test_context_in_a_fresh_process():
+    """A process whose first GPU use is the library's context (torch.cuda not touched
+    before) creates it and runs a small product: the loader brings torch's HIP runtime up
+    before the library registers its kernels."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import numpy as np\n"
+            "from varrcwa_amd import api\n"
+            "ctx = api.default_context()\n"
+            "import torch\n"
+            "a = torch.eye(3, dtype=torch.complex128, device='cuda')\n"
+            "print('ok', ctx.h.value is not None)\n") % str(ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "ok True" in r.stdout

@@ -54,6 +54,14 @@ def load():
         return _lib
     if not LIB_PATH.exists():
         raise FileNotFoundError(f"{LIB_PATH} is not built; run `python -m varrcwa_amd.build`")
+    # The process's HIP runtime is the one torch brings up: initialize it before the library
+    # registers its kernels (loading the library first made the first context creation fail
+    # with a HIP error in a process that had not touched torch.cuda yet).
+    try:
+        import torch
+        torch.cuda.is_available()
+    except ImportError:  # pragma: no cover - torch is required for device memory
+        pass
     _lib = C.CDLL(str(LIB_PATH))
     vp, i, l, d = C.c_void_p, C.c_int, C.c_long, C.c_double
     _sig("vrc_version", C.c_char_p)
