@@ -69,17 +69,23 @@ def fourier_bytes(nh, cells):
     return 16.0 * cells + 48.0 * n * n
 
 
-TRAFFIC_SUMMARY = ROOT / "profiles" / "pmc_traffic__hqr_kernel_rect_stack_N882_25inst.md"
+TRAFFIC_SUMMARIES = {
+    "hqr_kernel": ROOT / "profiles" / "pmc_traffic__hqr_kernel_rect_stack_N882_25inst.md",
+    "hqr_w48_kernel": ROOT / "profiles" / "pmc_traffic__hqr_w48_kernel_rect_stack_N882_40inst.md",
+}
 
 
-def hqr_traffic_per_matrix():
-    """HBM bytes per matrix of hqr_kernel (FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc,
-    each scaled by the counter's reading for a plain copy of known size), from the
-    committed profile summary of the 25-instance launch; None when it is absent."""
+def hqr_traffic_per_matrix(kname: str = "hqr_kernel"):
+    """HBM bytes per matrix of the QR kernel `kname` at N = 882 (FETCH_SIZE + WRITE_SIZE from
+    rocprofv3 --pmc, each scaled by the counter's reading for a plain copy of known size), from
+    the committed profile summary; None when there is none for that kernel."""
     import re
+    path = TRAFFIC_SUMMARIES.get(kname)
     try:
-        text = TRAFFIC_SUMMARY.read_text()
+        text = path.read_text() if path else None
     except OSError:
+        return None
+    if text is None:
         return None
     m = re.search(r"hbm_bytes_per_matrix:\s*([0-9.eE+]+)", text)
     return float(m.group(1)) if m else None
@@ -581,13 +587,18 @@ def main():
         qv = [int(t.split("=")[1]) for t in args.tune if t.startswith("zgeev.qr_variant=")]
         qsel = qv[-1] if qv else -1
         if qsel < 0:  # the library's automatic choice
-            qsel = 4 if ((N <= 256 and mats_per_launch >= 384) or (N <= 512 and mats_per_launch >= 768)) else 0
+            if (N <= 256 and mats_per_launch >= 384) or (N <= 512 and mats_per_launch >= 768):
+                qsel = 4
+            elif 512 < N <= 1024 and mats_per_launch >= 384:
+                qsel = 2
+            else:
+                qsel = 0
         kname = {1: "hqr_lds_kernel", 2: "hqr_w48_kernel", 4: "hqr_w32_kernel"}.get(qsel, "hqr_kernel")
-        tpm = hqr_traffic_per_matrix() if (N == 882 and qsel == 0) else None
+        tpm = hqr_traffic_per_matrix(kname) if N == 882 else None
         roof = {"bound": "mfma", "kernel": f"{kname} (multishift QR, Schur phase)", "achieved": ach,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                 "traffic": tpm * mats_per_launch if tpm else None,
-                "traffic_source": TRAFFIC_SUMMARY.relative_to(ROOT).as_posix() if tpm else None,
+                "traffic_source": TRAFFIC_SUMMARIES[kname].relative_to(ROOT).as_posix() if tpm else None,
                 "avg_launch_ms": avg_ms, "flops_per_launch": per_launch, "matrices_per_launch": mats_per_launch}
         try:
             ceil = zgemm_ceiling(ctx, N)

@@ -69,8 +69,8 @@ int vrc_timing_enable(vrc_ctx* ctx, int on);
  * S-matrices would exceed 96 GiB): instances of vrc_batch_stack_rcwa whose star products
  * and projections run in one batch;
  * "zgeev.qr_variant" (default -1: automatic, variant 4 for N <= 256 in launches of at least
- * 384 matrices and for N <= 512 in launches of at least 768, else 0): the QR (Schur) kernel,
- * 0: 64-wide window on 512 threads,
+ * 384 matrices and for N <= 512 in launches of at least 768, variant 2 for N <= 1024 in
+ * launches of at least 384, else 0): the QR (Schur) kernel, 0: 64-wide window on 512 threads,
  * window products formed from registers; 1: the same with the window products staged
  * through LDS; 2: 48-wide window on 256 threads, two matrices per CU, register products;
  * 4: 32-wide window on 128 threads, four matrices per CU (small N); batched calls size their

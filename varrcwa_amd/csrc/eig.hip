@@ -1583,13 +1583,16 @@ void zgeev_set_bulges(int nb) { g_bulges = nb < 1 ? 1 : (nb > NBUL ? NBUL : nb);
 // 0: hqr_kernel (W = 64, window products from registers), 1: hqr_lds_kernel (W = 64,
 // window products staged through LDS), 2: hqr_w48_kernel (W = 48, two matrices per CU),
 // 4: hqr_w32_kernel (W = 32, four matrices per CU); -1: automatic, 4 for N <= 256 in launches
-// of at least 384 matrices and for N <= 512 in launches of at least 768 (measured: at N = 98
-// and 242 the four-per-CU kernel wins from about 1.5 matrices per CU, at N = 450 from about 3),
-// else 0
+// of at least 384 matrices and for N <= 512 in launches of at least 768, 2 for N <= 1024 in
+// launches of at least 384, else 0 (measured: at N = 98 and 242 the four-per-CU kernel wins
+// from about 1.5 matrices per CU, at N = 450 from about 3; at N = 882 the two-per-CU kernel in
+// 512-matrix launches is 2.7 % faster than the one-per-CU kernel in 256-matrix launches)
 static int g_qr_variant = -1;
 static int qr_variant_for(int n, int batch) {
   if (g_qr_variant >= 0) return g_qr_variant;
-  return ((n <= 256 && batch >= 384) || (n <= 512 && batch >= 768)) ? 4 : 0;
+  if ((n <= 256 && batch >= 384) || (n <= 512 && batch >= 768)) return 4;
+  if (n > 512 && n <= 1024 && batch >= 384) return 2;
+  return 0;
 }
 static int g_bulges_w48 = 8;  // bulges per sweep of the W = 48 kernel (<= 12: >= 11 chase steps per window)
 static int g_bulges_w32 = 6;  // bulges per sweep of the W = 32 kernel (<= 7: >= 10 chase steps per window)
@@ -1598,7 +1601,7 @@ void zgeev_set_bulges_w48(int nb) { g_bulges_w48 = nb < 1 ? 1 : (nb > 12 ? 12 : 
 void zgeev_set_bulges_w32(int nb) { g_bulges_w32 = nb < 1 ? 1 : (nb > 7 ? 7 : nb); }
 int zgeev_get_qr_variant() { return g_qr_variant; }
 int zgeev_full_launch(int n) {
-  const int v = g_qr_variant >= 0 ? g_qr_variant : (n <= 512 ? 4 : 0);
+  const int v = g_qr_variant >= 0 ? g_qr_variant : (n <= 512 ? 4 : (n <= 1024 ? 2 : 0));
   return v == 4 ? 1024 : (v == 2 ? 512 : 256);
 }
 int zgeev_get_aed() { return g_aed_window; }
