@@ -6,8 +6,8 @@
 // The array is a file-scope __device__ symbol, so each GPU the library runs on holds its
 // own copy (SCRATCH_BYTES) from the code object's load until the process ends. Host code
 // never passes the symbol's name to a kernel: its device address is looked up once per
-// device with hipGetSymbolAddress when a context is created on that device and kept in
-// g_scratch[device]. Every call on a GPU shares that GPU's array and leaves its contents
+// device with hipGetSymbolAddress, with that device current, when a context is created on
+// that device and kept in g_scratch[device]. Every call on a GPU shares that GPU's array and leaves its contents
 // behind; calls that overlap on one GPU (two streams, contexts or threads) race on it and
 // must be serialized by the caller. Calls on one context's stream are ordered and safe.
 #include <mutex>
@@ -30,14 +30,23 @@ void* g_scratch[MAX_DEVICES] = {};
 
 namespace vrc {
 
-// Looks up this device's copy of the scratch array (the device must be current).
-// Called by vrc_ctx_create; later calls for the same device return at once.
+// Looks up this device's copy of the scratch array with that device current (the caller's
+// current device is restored). Called by vrc_ctx_create; later calls for the same device
+// return at once.
 void noalloc_init_device(int device) {
   if (device < 0 || device >= MAX_DEVICES) throw ArgFailure("device index out of range for the scratch table");
   std::lock_guard<std::mutex> lk(g_mu);
   if (g_scratch[device]) return;
+  int prev = -1;
+  hipError_t e = hipGetDevice(&prev);
+  if (e != hipSuccess) throw HipFailure(std::string("hipGetDevice failed: ") + hipGetErrorString(e));
+  if (prev != device) {
+    e = hipSetDevice(device);
+    if (e != hipSuccess) throw HipFailure(std::string("hipSetDevice failed: ") + hipGetErrorString(e));
+  }
   void* p = nullptr;
-  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (prev != device) (void)hipSetDevice(prev);
   if (e != hipSuccess || !p)
     throw HipFailure(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
   g_scratch[device] = p;
