@@ -87,9 +87,10 @@ int vrc_timing_read(vrc_ctx* ctx, double* ms, int* counts, int nstages);
 /* Autotuner (no reference counterpart). vrc_autotune_stack times vrc_batch_stack_rcwa on
  * the given instances (a sample of the batch to be solved; a few hundred slabs make full
  * launches) under candidate kernel switch settings, one switch at a time (the QR kernel
- * (zgeev.qr_variant 0, 2 or 4), the QR iteration's deflation window and bulge count, the
- * ZGEMM tile shape, wavefronts and large-tile threshold), keeping a candidate only when it
- * is >= 1 % faster, until
+ * (zgeev.qr_variant 0, 2 or 4), the QR iteration's deflation window, the bulge count of
+ * each QR kernel the sample launches, the ZGEMM tile shape, wavefronts and large-tile
+ * threshold; settings that cannot change what runs are skipped), keeping a candidate only
+ * when it is >= 1 % faster, until
  * budget_s seconds have been spent. The winners are kept for N = 2 nx ny and put in
  * place before every later call that solves N x N systems (other N keep the values set
  * by vrc_tune_set); with cache_path the entry (and the report as a comment) is appended

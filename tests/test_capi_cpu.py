@@ -71,8 +71,10 @@ def test_gds_reader_host_entry_point(tmp_path):
 
 def test_tuning_switches_and_cache_on_the_host(tmp_path):
     """vrc_tune_set validates keys and values, and the autotuner's cache file is parsed on the
-    host: seven-field lines (automatic QR kernel) and eight-field lines load, a QR kernel that
-    is not a tuning candidate, a malformed line or a missing file is an error. No device call."""
+    host: seven-field lines (automatic QR kernel), eight-field lines and ten-field lines (with the
+    bulge counts of QR kernels 2 and 4) load; a QR kernel that is not a tuning candidate, a bulge
+    count out of range, a ninth field without a tenth, trailing text, a malformed line or a
+    missing file is an error. No device call."""
     lib = _lib.load()
     try:
         assert lib.vrc_tune_set(b"no.such.switch", 1) == -1
@@ -86,9 +88,13 @@ def test_tuning_switches_and_cache_on_the_host(tmp_path):
         assert lib.vrc_tune_set(b"zgeev.qr_variant", 0) == 0
         assert lib.vrc_tune_set(b"zgeev.bulges_w48", 8) == 0
         good = tmp_path / "tune.txt"
-        good.write_text("# a comment\n882 24 12 128 8 8 129\n242 16 10 64 8 4 65 2\n98 16 6 64 8 4 65 4\n")
-        assert lib.vrc_autotune_load(str(good).encode()) == 3
-        for bad in ("882 24 12 128 8 8\n", "882 24 12 128 8 8 129 1\n", "1 24 12 128 8 8 129\n", "x y\n"):
+        good.write_text("# a comment\n882 24 12 128 8 8 129\n242 16 10 64 8 4 65 2\n98 16 6 64 8 4 65 4\n"
+                        "450 24 12 128 8 8 129 2 10 5\n")
+        assert lib.vrc_autotune_load(str(good).encode()) == 4
+        for bad in ("882 24 12 128 8 8\n", "882 24 12 128 8 8 129 1\n", "1 24 12 128 8 8 129\n", "x y\n",
+                    "882 24 12 128 8 8 129 2 8\n", "882 24 12 128 8 8 129 2 13 6\n",
+                    "882 24 12 128 8 8 129 2 8 0\n", "882 24 12 128 8 8 129 2 8 6 extra\n",
+                    "882 24 12 128 8 8 129 two\n"):
             f = tmp_path / "bad.txt"
             f.write_text(bad)
             assert lib.vrc_autotune_load(str(f).encode()) == -1, bad

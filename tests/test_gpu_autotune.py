@@ -28,9 +28,11 @@ def test_autotune_sample_applies_and_caches(ctx, tmp_path):
         text = rep.value.decode()
         print(text)
         assert text.startswith("N=50:") and "aed_window" in text and "qr_variant" in text
+        assert "bulges_w48" in text and "bulges_w32" in text
         lines = [l for l in cache.read_text().splitlines() if l and not l.startswith("#")]
-        assert len(lines) == 1 and lines[0].split()[0] == "50" and len(lines[0].split()) == 8
-        assert lines[0].split()[7] in ("-1", "0", "2", "4")
+        assert len(lines) == 1 and lines[0].split()[0] == "50" and len(lines[0].split()) == 10
+        f = lines[0].split()
+        assert f[7] in ("-1", "0", "2", "4") and 1 <= int(f[8]) <= 12 and 1 <= int(f[9]) <= 7
         after = from_dev_colmajor(api.batch_stack_rcwa(ctx, 5, 5, insts)["S"])
         assert np.abs(after - before).max() <= 1e-10 * np.abs(before).max()
         # another N is not affected, and the cache reads back
@@ -42,7 +44,7 @@ def test_autotune_sample_applies_and_caches(ctx, tmp_path):
         assert lib.vrc_autotune_load(str(bad).encode()) == -1
         bad.write_text("50 24 12 128 8 8 129 1\n")  # QR kernel 1 is not a tuning candidate
         assert lib.vrc_autotune_load(str(bad).encode()) == -1
-        old = tmp_path / "seven_fields.txt"  # a line without the QR kernel field means kernel 0
+        old = tmp_path / "seven_fields.txt"  # without the QR kernel field: the automatic choice
         old.write_text("# written before the QR kernel became a switch\n50 24 12 128 8 8 129\n")
         assert lib.vrc_autotune_load(str(old).encode()) == 1
         assert lib.vrc_autotune_stack(ctx.h, 5, 5, 0, arr, 3.0, None, None, 0) == -1

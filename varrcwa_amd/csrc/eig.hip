@@ -15,6 +15,8 @@
 //     finished by single-shift QR inside LDS.
 //  4. Eigenvectors of T by a tiled back-substitution (64x64 tiles; one workgroup per
 //     block column), then VR = Z * X (ZGEMM) and LAPACK zgeev normalisation.
+#include <atomic>
+
 #include "vrc_common.hpp"
 #include "vrc_internal.hpp"
 #include "vrc_context.hpp"
@@ -1599,7 +1601,12 @@ static int g_bulges_w32 = 6;  // bulges per sweep of the W = 32 kernel (<= 7: >=
 void zgeev_set_qr_variant(int v) { g_qr_variant = (v < -1 || v > 4 || v == 3) ? -1 : v; }
 void zgeev_set_bulges_w48(int nb) { g_bulges_w48 = nb < 1 ? 1 : (nb > 12 ? 12 : nb); }
 void zgeev_set_bulges_w32(int nb) { g_bulges_w32 = nb < 1 ? 1 : (nb > 7 ? 7 : nb); }
+int zgeev_get_bulges_w48() { return g_bulges_w48; }
+int zgeev_get_bulges_w32() { return g_bulges_w32; }
 int zgeev_get_qr_variant() { return g_qr_variant; }
+// bit v set: QR kernel variant v was launched since the last zgeev_take_qr_kernels()
+static std::atomic<unsigned> g_qr_ran{0};
+unsigned zgeev_take_qr_kernels() { return g_qr_ran.exchange(0u); }
 int zgeev_full_launch(int n) {
   const int v = g_qr_variant >= 0 ? g_qr_variant : (n <= 512 ? 4 : (n <= 1024 ? 2 : 0));
   return v == 4 ? 1024 : (v == 2 ? 512 : 256);
@@ -1634,6 +1641,7 @@ void zgeev_factors(hipStream_t s, int n, cd* A, long lda, long sA, cd* w, long s
     if (want) sb = ws.get(sizeof(long long) * 5 * batch);
     long long* stp = want ? sb.as<long long>() : nullptr;
     const int qv = qr_variant_for(n, batch);
+    g_qr_ran.fetch_or(1u << qv, std::memory_order_relaxed);
     if (qv == 4) {
       hqr_w32_kernel<<<batch, 128, 0, s>>>(n, A, lda, sA, Z, n, nn, w, sw, info, stp, g_aed_window, g_bulges_w32,
                                            g_max_sweeps);

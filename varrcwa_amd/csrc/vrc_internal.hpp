@@ -151,12 +151,17 @@ int zgeev_get_qr_variant();
 int zgeev_full_launch(int n);
 void zgeev_set_bulges_w48(int nb);  // bulges per sweep of the W = 48 kernel (1..12)
 void zgeev_set_bulges_w32(int nb);  // bulges per sweep of the W = 32 kernel (1..7)
+int zgeev_get_bulges_w48();
+int zgeev_get_bulges_w32();
+// Bit v set for each QR kernel variant v launched since the previous call (then cleared).
+unsigned zgeev_take_qr_kernels();
 
 // ---------------------------------------------------------------- autotune.cpp
 // The kernel switches an autotuned harmonic count overrides (process-wide values).
 struct TuneKnobs {
   int aed, bulges, big_n, big_waves, small_waves, big_min_k;
-  int qr_variant;  // -1 (automatic), 0, 2 or 4 (zgeev_set_qr_variant)
+  int qr_variant;              // -1 (automatic), 0, 2 or 4 (zgeev_set_qr_variant)
+  int bulges_w48, bulges_w32;  // bulges per sweep of QR variants 2 and 4 (bulges: 0 and 1)
 };
 TuneKnobs tune_get();
 void tune_put(const TuneKnobs& k);
