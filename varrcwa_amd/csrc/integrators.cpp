@@ -908,8 +908,9 @@ static void cascade_group(vrc_ctx& c, int N, int L, int GB, int I0, bool homog, 
 
 // Forward path for instances whose end media are single-valued (closed-form eigen-systems):
 // the slab eigensolves of a cascade group are split evenly into launches of at most
-// `per_launch` slabs, regardless of instance boundaries (the eigen kernels run one matrix
-// per CU, so full launches matter more than whole instances).
+// `per_launch` slabs (0: the launch count of zgeev_launches), regardless of instance
+// boundaries (the QR kernels run one to four matrices per CU, so launches that fill the
+// kernels' slots matter more than whole instances).
 static void batch_stack_homog(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int per_launch,
                               cd* S_out, cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o, int* flags) {
   hipStream_t st = c.stream;
@@ -951,9 +952,10 @@ static void batch_stack_homog(vrc_ctx& c, int nx, int ny, const std::vector<Stac
                       WeG.as<cd>(), VeG.as<cd>(), geG.as<cd>());
     }
     const cd* V1 = VeG.as<cd>() + (long)GB * NN;
-    // slab launches: the group's GB * L slabs (instance-major) split evenly
+    // slab launches: the group's GB * L slabs (instance-major) split evenly (per_launch 0:
+    // the launch count that fills the QR kernels' slots)
     const int SLall = GB * L;
-    const int nch = (SLall + per_launch - 1) / per_launch;
+    const int nch = per_launch > 0 ? (SLall + per_launch - 1) / per_launch : zgeev_launches(N, SLall, 1);
     for (int ch = 0; ch < nch; ++ch) {
       const int s0 = (int)((long)SLall * ch / nch), SL = (int)((long)SLall * (ch + 1) / nch) - s0;
       std::vector<const HostLayer*> jl(SL);
@@ -1040,15 +1042,20 @@ static void batch_stack_solve(vrc_ctx& c, int nx, int ny, const std::vector<Stac
     homog = homog && I.bottom->x.size() == 2 && I.bottom->y.size() == 2 && I.top->x.size() == 2 &&
             I.top->y.size() == 2;
   if (homog && !tape) {
-    batch_stack_homog(c, nx, ny, inst, chunk > 0 ? chunk * L : zgeev_full_launch(N), S_out, W0o, V0o, g0o, W1o, V1o,
-                      g1o, flags);
+    batch_stack_homog(c, nx, ny, inst, chunk > 0 ? chunk * L : 0, S_out, W0o, V0o, g0o, W1o, V1o, g1o, flags);
     return;
   }
-  // Default chunk: at most one full launch of dense eigensolves (L slices, + 2 end media
-  // per instance unless closed form), one matrix per QR workgroup slot of the GPU; the
-  // instances are then split evenly over the chunks so that no launch runs a short tail.
-  if (chunk <= 0) chunk = std::max(1, std::min(ninst, zgeev_full_launch(N) / (L + (homog ? 0 : 2))));
-  const int nchunks = (ninst + chunk - 1) / chunk;
+  // Default chunks: at most one full launch of dense eigensolves each (L slices, + 2 end
+  // media per instance unless closed form), one matrix per QR workgroup slot of the GPU,
+  // and no more matrices than the QR kernel chosen for the chunk's size has slots; the
+  // instances are split evenly over the chunks so that no launch runs a short tail.
+  int nchunks;
+  if (chunk <= 0) {
+    nchunks = zgeev_launches(N, ninst, L + (homog ? 0 : 2));
+    chunk = (ninst + nchunks - 1) / nchunks;
+  } else {
+    nchunks = (ninst + chunk - 1) / chunk;
+  }
   std::vector<int> starts(nchunks + 1, 0);
   for (int ci = 0; ci < nchunks; ++ci) starts[ci + 1] = starts[ci] + ninst / nchunks + (ci < ninst % nchunks ? 1 : 0);
   // Cascade groups: the slab S-matrices and end systems of consecutive chunks are kept
