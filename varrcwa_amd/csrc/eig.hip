@@ -1613,23 +1613,15 @@ static int qr_slots(int v) { return v == 4 ? 1024 : (v == 2 ? 512 : 256); }
 int zgeev_full_launch(int n) {
   return qr_slots(g_qr_variant >= 0 ? g_qr_variant : (n <= 512 ? 4 : (n <= 1024 ? 2 : 0)));
 }
-int zgeev_launches(int n, int units, int per_unit) {
-  if (units <= 0) return 0;
+std::vector<int> zgeev_launch_plan(int n, int units, int per_unit) {
+  std::vector<int> plan;
+  if (units <= 0) return plan;
   if (per_unit < 1) per_unit = 1;
-  const int per = zgeev_full_launch(n) / per_unit > 1 ? zgeev_full_launch(n) / per_unit : 1;
-  int nch = (units + per - 1) / per;
-  while (nch < units) {
-    // the even split gives launches of lo or hi units; each must fit one round of the
-    // kernel chosen for its size
-    const int hi = (units + nch - 1) / nch, lo = units / nch;
-    const int vh = qr_variant_for(n, hi * per_unit), vl = qr_variant_for(n, lo * per_unit);
-    if (hi * per_unit <= qr_slots(vh) && lo * per_unit <= qr_slots(vl)) break;
-    const int cap = qr_slots(hi * per_unit > qr_slots(vh) ? vh : vl);
-    const int fit = cap / per_unit > 1 ? cap / per_unit : 1;
-    const int next = (units + fit - 1) / fit;
-    nch = next > nch ? next : nch + 1;
-  }
-  return nch;
+  const int full = zgeev_full_launch(n);
+  const int per = full / per_unit > 1 ? full / per_unit : 1;
+  plan.assign(units / per, per);
+  if (units % per) plan.push_back(units % per);
+  return plan;
 }
 int zgeev_get_aed() { return g_aed_window; }
 int zgeev_get_bulges() { return g_bulges; }

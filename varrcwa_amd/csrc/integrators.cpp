@@ -908,9 +908,9 @@ static void cascade_group(vrc_ctx& c, int N, int L, int GB, int I0, bool homog, 
 
 // Forward path for instances whose end media are single-valued (closed-form eigen-systems):
 // the slab eigensolves of a cascade group are split evenly into launches of at most
-// `per_launch` slabs (0: the launch count of zgeev_launches), regardless of instance
-// boundaries (the QR kernels run one to four matrices per CU, so launches that fill the
-// kernels' slots matter more than whole instances).
+// `per_launch` slabs (0: zgeev_launch_plan, full launches of the QR kernels and one launch of
+// the rest), regardless of instance boundaries (the QR kernels run one to four matrices per
+// CU, so launches that fill the kernels' slots matter more than whole instances).
 static void batch_stack_homog(vrc_ctx& c, int nx, int ny, const std::vector<StackInstance>& inst, int per_launch,
                               cd* S_out, cd* W0o, cd* V0o, cd* g0o, cd* W1o, cd* V1o, cd* g1o, int* flags) {
   hipStream_t st = c.stream;
@@ -952,12 +952,17 @@ static void batch_stack_homog(vrc_ctx& c, int nx, int ny, const std::vector<Stac
                       WeG.as<cd>(), VeG.as<cd>(), geG.as<cd>());
     }
     const cd* V1 = VeG.as<cd>() + (long)GB * NN;
-    // slab launches: the group's GB * L slabs (instance-major) split evenly (per_launch 0:
-    // the launch count that fills the QR kernels' slots)
+    // slab launches over the group's GB * L slabs (instance-major)
     const int SLall = GB * L;
-    const int nch = per_launch > 0 ? (SLall + per_launch - 1) / per_launch : zgeev_launches(N, SLall, 1);
-    for (int ch = 0; ch < nch; ++ch) {
-      const int s0 = (int)((long)SLall * ch / nch), SL = (int)((long)SLall * (ch + 1) / nch) - s0;
+    std::vector<int> lstart{0};
+    if (per_launch > 0) {
+      const int nch = (SLall + per_launch - 1) / per_launch;
+      for (int ch = 1; ch <= nch; ++ch) lstart.push_back((int)((long)SLall * ch / nch));
+    } else {
+      for (int sz : zgeev_launch_plan(N, SLall, 1)) lstart.push_back(lstart.back() + sz);
+    }
+    for (size_t ch = 0; ch + 1 < lstart.size(); ++ch) {
+      const int s0 = lstart[ch], SL = lstart[ch + 1] - s0;
       std::vector<const HostLayer*> jl(SL);
       std::vector<double> k2(SL), sv(SL);
       std::vector<cd> ab((size_t)SL * 2 * n);
@@ -1045,19 +1050,18 @@ static void batch_stack_solve(vrc_ctx& c, int nx, int ny, const std::vector<Stac
     batch_stack_homog(c, nx, ny, inst, chunk > 0 ? chunk * L : 0, S_out, W0o, V0o, g0o, W1o, V1o, g1o, flags);
     return;
   }
-  // Default chunks: at most one full launch of dense eigensolves each (L slices, + 2 end
-  // media per instance unless closed form), one matrix per QR workgroup slot of the GPU,
-  // and no more matrices than the QR kernel chosen for the chunk's size has slots; the
-  // instances are split evenly over the chunks so that no launch runs a short tail.
-  int nchunks;
+  // Chunks of instances, one eigensolver launch each (L slices, + 2 end media per instance
+  // unless closed form): by default full launches (one matrix per QR workgroup slot of the
+  // GPU) and one launch of the rest; a given chunk size splits the instances evenly.
+  std::vector<int> starts{0};
   if (chunk <= 0) {
-    nchunks = zgeev_launches(N, ninst, L + (homog ? 0 : 2));
-    chunk = (ninst + nchunks - 1) / nchunks;
+    for (int sz : zgeev_launch_plan(N, ninst, L + (homog ? 0 : 2))) starts.push_back(starts.back() + sz);
+    chunk = starts[1];
   } else {
-    nchunks = (ninst + chunk - 1) / chunk;
+    const int nch = (ninst + chunk - 1) / chunk;
+    for (int ci = 0; ci < nch; ++ci) starts.push_back(starts.back() + ninst / nch + (ci < ninst % nch ? 1 : 0));
   }
-  std::vector<int> starts(nchunks + 1, 0);
-  for (int ci = 0; ci < nchunks; ++ci) starts[ci + 1] = starts[ci] + ninst / nchunks + (ci < ninst % nchunks ? 1 : 0);
+  const int nchunks = (int)starts.size() - 1;
   // Cascade groups: the slab S-matrices and end systems of consecutive chunks are kept
   // and cascaded together, so that the star products and projections run batched over
   // up to g_cascade_group instances (chunks split evenly over the groups).

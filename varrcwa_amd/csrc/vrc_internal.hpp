@@ -149,11 +149,13 @@ int zgeev_get_qr_variant();
 // matrices of one eigensolve launch that fill every QR workgroup slot (256 CUs; two slots
 // per CU for variant 2, four for variant 4)
 int zgeev_full_launch(int n);
-// Launches over which `units` units of `per_unit` n x n eigensolves each are split evenly:
-// as few as fill zgeev_full_launch(n), more when the even share of a launch would run a
-// QR kernel with fewer slots than the share (at n = 882, 520 slabs go in three launches of
-// about 174 for the one-per-CU kernel, not two of 260 that would each take two rounds).
-int zgeev_launches(int n, int units, int per_unit);
+// Sizes (in units) of the launches over which `units` units of `per_unit` n x n eigensolves
+// each are solved: full launches of zgeev_full_launch(n) matrices, then one launch of the
+// rest, whose QR kernel is chosen for its size. A QR launch takes as long as its busiest
+// CU: the multi-matrix kernels only pay off in full launches (427 matrices of the two-per-CU
+// kernel take as long as 512), while the one-per-CU kernel takes one round for any launch
+// of up to 256.
+std::vector<int> zgeev_launch_plan(int n, int units, int per_unit);
 void zgeev_set_bulges_w48(int nb);  // bulges per sweep of the W = 48 kernel (1..12)
 void zgeev_set_bulges_w32(int nb);  // bulges per sweep of the W = 32 kernel (1..7)
 int zgeev_get_bulges_w48();
