@@ -201,7 +201,7 @@ int vrc_zgetrf(vrc_ctx* c, int n, void* A, long lda, long sA, int* ipiv, long sP
   return guarded(c, [&] {
     need(n >= 0 && batch >= 0 && lda >= n, "bad size");
     need(A && ipiv, "null pointer");
-    if (info) VRC_HIP(hipMemsetAsync(info, 0, sizeof(int) * batch, c->stream));
+    if (info) fill_int(c->stream, info, 0, batch);  // a fill kernel: safe under graph capture
     zgetrf(c->stream, n, (cd*)A, lda, sA, ipiv, sP, info, batch, c->ws);
   });
 }

@@ -7,9 +7,10 @@
 // own copy (SCRATCH_BYTES) from the code object's load until the process ends. Host code
 // never passes the symbol's name to a kernel: its device address is looked up once per
 // device with hipGetSymbolAddress, with that device current, when a context is created on
-// that device and kept in g_scratch[device]. Every call on a GPU shares that GPU's array and leaves its contents
-// behind; calls that overlap on one GPU (two streams, contexts or threads) race on it and
-// must be serialized by the caller. Calls on one context's stream are ordered and safe.
+// that device, and kept in g_scratch[device]. Every call on a GPU shares that GPU's array
+// and leaves its contents behind; calls that overlap on one GPU (two streams, contexts or
+// threads) race on it and must be serialized by the caller. Calls on one context's stream
+// are ordered and safe.
 #include <mutex>
 #include <stdexcept>
 
