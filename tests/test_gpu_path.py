@@ -3,6 +3,7 @@
 Tolerances: relative 1e-9 on basis-invariant S transforms and on
 |S| of non-degenerate modes; Fourier/P/Q stages compared elementwise at 1e-12.
 """
+import ctypes as C
 from pathlib import Path
 
 import numpy as np
@@ -291,7 +292,48 @@ def test_batch_stack_chunks_and_groups(ctx):
             got = from_dev_colmajor(api.batch_stack_rcwa(ctx, nx, ny, insts, chunk=2)["S"])
             assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), group
     finally:
-        ctx.lib.vrc_tune_set(b"stack.cascade_group", 128)
+        ctx.lib.vrc_tune_set(b"stack.cascade_group", 512)  # the library's default
+
+
+def test_batch_stack_full_launch_and_remainder(ctx):
+    """A batch larger than one full eigensolver launch: at 7 x 7 harmonics (N = 98) 400
+    instances of 3 slabs run one launch of 1024 slabs on the four-per-CU QR kernel and one of
+    the remaining 176 on the one-per-CU kernel. Every instance equals the run with the
+    one-per-CU kernel everywhere (to rounding), and the first and last instances (one in each
+    launch) equal the oracle."""
+    nx = ny = 7
+    insts = [_stack_case(s, nlayers=3) for s in range(400)]
+    lib = ctx.lib
+    nst = 10  # stage timers of vrc_timing_read; the QR stage is the fourth
+    ms, cnt, qs = (C.c_double * nst)(), (C.c_int * nst)(), (C.c_longlong * 6)()
+    assert lib.vrc_timing_enable(ctx.h, 1) == 0
+    try:
+        lib.vrc_timing_read(ctx.h, ms, cnt, nst)  # reset
+        lib.vrc_qr_stats(ctx.h, qs)
+        out = api.batch_stack_rcwa(ctx, nx, ny, insts, want_bases=True)
+        assert lib.vrc_timing_read(ctx.h, ms, cnt, nst) == 0 and lib.vrc_qr_stats(ctx.h, qs) == 0
+    finally:
+        lib.vrc_timing_enable(ctx.h, 0)
+    assert (cnt[3], qs[0]) == (2, 1200), (cnt[3], qs[0])  # two QR launches, 1200 dense eigensolves
+    S = from_dev_colmajor(out["S"])
+    try:
+        assert ctx.lib.vrc_tune_set(b"zgeev.qr_variant", 0) == 0
+        S0 = from_dev_colmajor(api.batch_stack_rcwa(ctx, nx, ny, insts)["S"])
+    finally:
+        ctx.lib.vrc_tune_set(b"zgeev.qr_variant", -1)
+    scale = np.abs(S0).reshape(len(insts), -1).max(axis=1)
+    dev = np.abs(S - S0).reshape(len(insts), -1).max(axis=1) / scale
+    assert dev.max() <= 1e-11, (int(dev.argmax()), float(dev.max()))
+    bases = {k: (from_dev_colmajor(out[k]) if not k.startswith("gam") else out[k].cpu().numpy())
+             for k in ["W0", "V0", "gam0", "W1", "V1", "gam1"]}
+    for i in (0, len(insts) - 1):
+        inst = insts[i]
+        ze = np.concatenate([[0.0], np.cumsum(inst["thickness"])])
+        so = O.StackLayerSampler(inst["lam"], nx, ny, inst["layers"], ze, inst["bottom"], inst["top"])
+        ref = O.rcwa_integrate(so, ze[0], ze[-1], len(inst["layers"]), z_edges=ze)
+        got = O.SResult(S[i, 0], S[i, 1], S[i, 2], S[i, 3], bases["W0"][i], bases["V0"][i], bases["gam0"][i],
+                        bases["W1"][i], bases["V1"][i], bases["gam1"][i])
+        invariants_close(got, ref)
 
 
 def test_batch_stack_patterned_ends(ctx):
