@@ -374,10 +374,11 @@ int vrc_integrator_tree(vrc_integrator* it, int cap, double* host_out);
  * half-space media bottom/top as end bases. All instances of a call share nx, ny
  * and nlayers. S_out: per instance [Tdd|Tuu|Rud|Rdu] (4 N x N, device); the
  * optional W0/V0/gam0/W1/V1/gam1 outputs may be NULL. Instances are solved in chunks
- * of at most `chunk` (<= 0: as many as keep one launch within 256 eigensolves), split
- * evenly over the chunks; with single-valued half-spaces (and no tape) the slab eigensolves
- * of each cascade group go in even launches of at most chunk * nlayers (<= 0: 256) slabs,
- * across instance boundaries. ninst = 0 does nothing (S_out may then be NULL).
+ * of at most `chunk`, split evenly over the chunks; with single-valued half-spaces (and no
+ * tape) the slab eigensolves of each cascade group go in even launches of at most
+ * chunk * nlayers slabs, across instance boundaries. chunk <= 0: full eigensolver launches
+ * (the QR kernel's slots: 1024, 512 or 256 matrices by N, see "zgeev.qr_variant") and then
+ * one launch of the rest. ninst = 0 does nothing (S_out may then be NULL).
  * info_out (host, ninst ints, or NULL) receives per instance 0, 1 (an eigensolve of the
  * instance did not converge) or 4 (its S-matrix holds NaN or infinity, e.g. a zero
  * propagation constant or a singular system); the status is then the first flagged
