@@ -230,7 +230,7 @@ std::string autotune_stack(vrc_ctx& c, int nx, int ny, const std::vector<StackIn
     tune_put(own);
     c.timing = timing;
     if (best_out) *best_out = best;
-    char b[256];
+    char b[384];
     snprintf(b, sizeof(b), "N=%d: %s; sample %zu instances %.3f s (own switches %.3f s), %d runs in %.1f s%s", N,
              knobs_text(best).c_str(), sample.size(), t_best, t_own, runs, elapsed(),
              out_of_time ? " (budget reached)" : "");
